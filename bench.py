@@ -1,0 +1,175 @@
+"""Benchmark: FSSH trajectory-steps/s on one MI355X, batched (persistent, FP32 nuclei /
+FP64 electronics) vs the plain GPU port (one launch per step, FP64), with the final-
+population error of the batched form against the FP64 path.
+
+Workload (one "step" of this benchmark = one full ensemble pass): 1e6 trajectories
+mixing the three Tully models (single avoided crossing, dual crossing, extended
+coupling), p0 on the 5..35 a.u. grid, x0 ~ N(-5, p0/(20 sqrt 2)), m = 2000, dt = 1 a.u.,
+t_max = 10000 a.u., terminated at |x| > 10 (uneven lengths).  Initial conditions are
+sampled on the device before the timed region; the timed region is the trajectory
+integration.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nqcdynamics.jl_amd"))
+
+METRIC = "FSSH trajectory-steps/s on 1 GPU, learned/batched vs plain at ≤0.01 pop. error"
+KEY = 0x4E5143444D493335
+TULLY = [(0, [0.01, 1.6, 0.005, 1.0]), (1, [0.1, 0.28, 0.015, 0.06, 0.05]), (2, [6e-4, 0.1, 0.9])]
+MASS = 2000.0
+NTRAJ = 1_000_000
+TMAX = 10000.0
+XTERM = 10.0
+GRID = [-5.0, 5.0, 31]
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
+HBM_PEAK_GBS = 8000.0
+# FP64 flops of one trajectory-step of the compact two-state formulation (counted from
+# fssh2.hip: 5 Tsit5 sub-steps x (6 RHS x 15 + stage combinations 4 x 48 + 12) = 1470,
+# plus ~80 for the Verlet step, model, eigen, NAC and hop test).
+FLOP_PER_STEP = 1550.0
+FLOP_PER_STEP_DENSE = 6.8e3  # dense complex formulation (reference data flow)
+
+
+def _ensemble(L, precision, schedule):
+    E = L.Ensemble(0)
+    E.set_problem(L.METHOD_FSSH, 2, 1, 1, [MASS], 1.0, precision=precision, schedule=schedule)
+    for slot, (kind, p) in enumerate(TULLY):
+        E.set_model(slot, kind, p)
+    E.sample_initial(L.SAMPLER_TULLY_GRID, NTRAJ, GRID, KEY, n_models=3)
+    return E
+
+
+def _final_pops(L, E):
+    fin = E.reduce(L.OUT_FINAL_ADIABATIC_POPULATION, L.RED_MEAN, 2)
+    scat = E.reduce(L.OUT_SCATTERING_1D, L.RED_MEAN, 4)
+    return np.concatenate([fin, scat])
+
+
+def cpu_baseline(L, gpu_states, nthreads):
+    """FP64 oracle (CPU restatement) on a stratified sample of the same trajectories."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    stride = 97  # every 97th trajectory id: covers every model and momentum of the grid
+    ids = np.arange(0, NTRAJ, stride)
+    p0 = GRID[1] + (ids // 3) % GRID[2]
+    r0, v0 = O.sample_gaussian(len(ids), 1, GRID[0], (p0 / (20 * np.sqrt(2)))[:, None], (p0 / MASS)[:, None], 0.0,
+                               KEY, traj_ids=ids)
+    sig0 = np.zeros((len(ids), 2, 2), complex); sig0[:, 0, 0] = 1
+    models = [O.make_model(k, 2, 1, p) for k, p in TULLY]
+    t0 = time.perf_counter()
+    out = O.run_ensemble(0, models, MASS, r0, v0, sig0, state0=np.zeros(len(ids), np.int32),
+                         model_id=(ids % 3).astype(np.int32), dt=1.0, nsteps=int(TMAX), key=KEY, term_x=XTERM,
+                         nthreads=nthreads, traj_ids=ids)
+    el = time.perf_counter() - t0
+    states = out["state"]; steps = int(out["steps"].sum())
+    agree = float(np.mean(states == gpu_states[ids]))
+    pc = np.bincount(states, minlength=2) / len(ids)
+    pg = np.bincount(gpu_states[ids], minlength=2) / len(ids)
+    return {"value": steps / el, "unit": "trajectory-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"{len(ids)} trajectories (every {stride}th id of the 1e6 ensemble, all lengths, "
+                      f"{steps} steps, {el:.1f} s) of the FP64 CPU restatement of the reference algorithm",
+            "pop_err_fp32_gpu_vs_fp64_oracle_sample": float(np.abs(pc - pg).max()),
+            "state_agreement_sample": agree}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-port comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    from nqcd_mi355x import _lib as L
+
+    # --- batched: persistent work-queue kernel, FP32 nuclei / FP64 electronics -------------------
+    E = _ensemble(L, L.PREC_FP32_NUCLEI, L.SCHED_PERSISTENT)
+    for _ in range(args.warmup):
+        E.run(0.0, TMAX, KEY, term=(0, XTERM))
+    kernel_ms, wall_ms, steps_total = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = E.run(0.0, TMAX, KEY, term=(0, XTERM))
+        kernel_ms.append(st["kernel_ms"]); wall_ms.append(st["wall_ms"]); steps_total.append(st["traj_steps"])
+    t_timed = time.perf_counter() - t0
+    assert len(set(steps_total)) == 1, "runs must be identical"
+    traj_steps = steps_total[0]
+    ms_per_step = 1e3 * t_timed / args.steps
+    value = traj_steps / (t_timed / args.steps)
+    avg_kernel_ms = float(np.mean(kernel_ms))
+    pops32 = _final_pops(L, E)
+    states32 = E.download(L.FIELD_STATE)
+    grid, block = st["grid"], st["block"]
+    E.close()
+
+    # --- FP64 persistent (the parity path) on the same ensemble ----------------------------------
+    E64 = _ensemble(L, L.PREC_FP64, L.SCHED_PERSISTENT)
+    st64 = E64.run(0.0, TMAX, KEY, term=(0, XTERM))
+    pops64 = _final_pops(L, E64)
+    fp64_rate = st64["traj_steps"] / (st64["kernel_ms"] / 1e3)
+    E64.close()
+    pop_err = float(np.abs(pops32 - pops64).max())
+    p = pops64[:2]
+    sigma = float(np.sqrt(np.max(p * (1 - p)) / NTRAJ))
+
+    # --- plain GPU port: one launch per step, HBM state, dense FP64 -------------------------------
+    plain = None
+    if not args.no_plain:
+        Ep = _ensemble(L, L.PREC_FP64, L.SCHED_STEP_PER_LAUNCH)
+        stp = Ep.run(0.0, TMAX, KEY, term=(0, XTERM))
+        popsp = _final_pops(L, Ep)
+        Ep.close()
+        plain = {"value": stp["traj_steps"] / (stp["wall_ms"] / 1e3), "kernel_ms": stp["kernel_ms"],
+                 "wall_ms": stp["wall_ms"], "launches": stp["launches"],
+                 "pop_diff_vs_fp64_persistent": float(np.abs(popsp - pops64).max())}
+
+    achieved = FLOP_PER_STEP * traj_steps / (avg_kernel_ms / 1e3) / 1e12
+    algo_bytes = NTRAJ * (8 + 8 + 32 + 4 + 4 + 4 + 8 + 8 + 32 + 12)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "trajectory-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 nuclei / f64 electronics",
+        "data": "synthetic (device-sampled Philox initial conditions)",
+        "config": {"workload": "batched FSSH, 1e6 trajectories mixing Tully 1/2/3, p0 grid 5..35 a.u., "
+                               "dt 1 a.u., t_max 10000 a.u., terminate |x|>10",
+                   "ntraj": NTRAJ, "traj_steps_per_pass": int(traj_steps), "parallelism": "1 GPU",
+                   "schedule": "persistent work-queue kernel", "grid": grid, "block": block},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "fssh2_persistent<1,float>", "kernel_avg_ms": avg_kernel_ms,
+                     "flop_per_traj_step": FLOP_PER_STEP,
+                     "achieved_dense_convention": FLOP_PER_STEP_DENSE * traj_steps / (avg_kernel_ms / 1e3) / 1e12,
+                     "hbm_algorithmic_GBs": algo_bytes / (avg_kernel_ms / 1e3) / 1e9},
+        "pop_error_vs_fp64": pop_err,
+        "pop_sigma": sigma,
+        "fp64_persistent": {"value": fp64_rate, "kernel_ms": st64["kernel_ms"]},
+        "plain_gpu": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+    }
+    if not args.no_cpu:
+        nthreads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(L, states32, nthreads)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,235 @@
+/*
+ * nqcd_mi355x.h -- C ABI of libnqcd_mi355x.so, the MI355X (gfx950) ensemble
+ * trajectory integrator that backs the EnsembleMI355X ensemble algorithm.
+ *
+ * The reference (NQCDynamics.jl v1.0.5) has no C ABI: ensemble integration is
+ * reached through run_dynamics -> SciMLBase.solve(ensemble_problem, algorithm,
+ * ensemble_algorithm; trajectories, ...) (src/Ensembles/run_dynamics.jl:91-97).
+ * Each entry point below replaces one piece of what that call does for the
+ * supported (method, algorithm, model) combinations; the citation on each
+ * declaration names the reference code it stands in for.
+ *
+ * Conventions
+ *  - Status: int32_t, 0 = OK, negative = error class (NQCD_ERR_*); the message
+ *    is available from nqcd_mi355x_last_error(handle) until the next call on
+ *    that handle.  No C++ exception crosses this boundary.
+ *  - Ownership: the caller owns every host buffer for the duration of a call;
+ *    the library copies inputs and never retains host pointers.  Device memory
+ *    belongs to the handle and is freed by nqcd_mi355x_close.
+ *  - Threading: one handle = one HIP stream.  Calls on one handle must be
+ *    serialised by the caller and return after the device work completed.
+ *  - Layout: host arrays use the reference's column-major Julia layout with the
+ *    trajectory index last: positions/velocities (ndofs, natoms[, nbeads], ntraj),
+ *    density matrices (nstates, nstates, ntraj) as separate real/imaginary parts.
+ *    States are 0-based (the Julia side subtracts 1).
+ *  - Determinism: results are a pure function of the inputs and philox_key; they
+ *    do not depend on launch configuration or trajectory scheduling.
+ */
+#ifndef NQCD_MI355X_H
+#define NQCD_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NQCD_MI355X_ABI_VERSION 1
+
+typedef struct nqcd_handle nqcd_handle_t;
+
+enum nqcd_status {
+    NQCD_OK = 0,
+    NQCD_ERR_ARG = -1,          /* invalid argument (shape, null pointer, range) */
+    NQCD_ERR_UNSUPPORTED = -2,  /* method/algorithm/model combination not supported */
+    NQCD_ERR_DEVICE = -3,       /* HIP runtime error */
+    NQCD_ERR_STATE = -4,        /* call out of order (e.g. run before load) */
+    NQCD_ERR_NOMEM = -5
+};
+
+/* DynamicsMethods (src/DynamicsMethods/): FSSH fssh.jl:23, Ehrenfest ehrenfest.jl:27,
+ * AdiabaticIESH iesh.jl:26, EhrenfestNA ehrenfest_na.jl:6, RingPolymerSimulation{Classical}
+ * classical.jl:28, RingPolymerSimulation{FSSH} rpsh.jl:8, RingPolymerSimulation{Ehrenfest}
+ * ehrenfest_rpmd.jl:3. */
+enum nqcd_method {
+    NQCD_METHOD_FSSH = 0,
+    NQCD_METHOD_EHRENFEST = 1,
+    NQCD_METHOD_IESH = 2,
+    NQCD_METHOD_EHRENFEST_NA = 3,
+    NQCD_METHOD_RPMD = 4,
+    NQCD_METHOD_RPSH = 5,
+    NQCD_METHOD_RP_EHRENFEST = 6
+};
+
+/* IntegrationAlgorithms (IntegrationAlgorithms.jl:19-56): BABwithTsit5, VerletwithElectronics,
+ * BCB, BCBwithTsit5. */
+enum nqcd_algorithm {
+    NQCD_ALG_BAB_TSIT5 = 0,
+    NQCD_ALG_VERLET_ELECTRONICS = 1,
+    NQCD_ALG_BCB = 2,
+    NQCD_ALG_BCB_TSIT5 = 3
+};
+
+/* rescaling keyword of Simulation{FSSH} (fssh.jl:41, surface_hopping.jl:219,234-245). */
+enum nqcd_rescaling { NQCD_RESCALE_STANDARD = 0, NQCD_RESCALE_VINVERSION = 1, NQCD_RESCALE_OFF = 2 };
+
+/* Nuclear precision: FP64 everywhere (parity path) or FP32 positions, velocities,
+ * forces and model evaluation with FP64 eigen/NAC/electronics. */
+enum nqcd_precision { NQCD_PREC_FP64 = 0, NQCD_PREC_FP32_NUCLEI = 1 };
+
+/* Scheduling: persistent work-queue kernel (trajectory state in registers) or one
+ * launch per nuclear step with the state in HBM (the plain GPU port). */
+enum nqcd_schedule { NQCD_SCHED_PERSISTENT = 0, NQCD_SCHED_STEP_PER_LAUNCH = 1 };
+
+/* Model kinds (NQCModels, external to the reference tree). */
+enum nqcd_model_kind {
+    NQCD_MODEL_TULLY1 = 0,       /* params: A, B, C, D                         */
+    NQCD_MODEL_TULLY2 = 1,       /* params: A, B, C, D, E0                     */
+    NQCD_MODEL_TULLY3 = 2,       /* params: A, B, C                            */
+    NQCD_MODEL_SPIN_BOSON = 3,   /* params: eps, Delta, omega[nb], c[nb]       */
+    NQCD_MODEL_DW_GAUSSIAN = 4,  /* params: a, b, offset, C, D                 */
+    NQCD_MODEL_HARMONIC = 5      /* params: m, omega, r0                       */
+};
+
+typedef struct {
+    int32_t method;              /* enum nqcd_method */
+    int32_t algorithm;           /* enum nqcd_algorithm */
+    int32_t n_states;
+    int32_t ndofs;
+    int32_t natoms;
+    int32_t nbeads;              /* 1 for Simulation */
+    int32_t n_electrons;         /* IESH / EhrenfestNA */
+    int32_t rescaling;           /* enum nqcd_rescaling */
+    int32_t precision;           /* enum nqcd_precision */
+    int32_t schedule;            /* enum nqcd_schedule */
+    int32_t electronic_substeps; /* Tsit5 sub-steps per nuclear step; 0 -> 5 (bab_electronics.jl:42) */
+    int32_t reserved0;
+    const double* masses;        /* natoms (simulations.jl:86-91) */
+    double dt;                   /* nuclear time step (run_dynamics.jl:71-73 default 1.0) */
+    double temperature;          /* ring polymers: omega_n = nbeads * temperature (ring_polymer.jl:18) */
+    int32_t reserved[8];
+} nqcd_problem_desc;
+
+/* TerminatingCallback with the predicate |x_dof| > value (DynamicsUtils/callbacks.jl:15-29). */
+enum nqcd_term_kind { NQCD_TERM_NONE = 0, NQCD_TERM_ABS_GT = 1 };
+typedef struct {
+    int32_t kind;
+    int32_t dof;                 /* flattened (ndofs, natoms) index */
+    double value;
+} nqcd_term_desc;
+
+/* Device-side initial-condition samplers (stand in for NQCDistributions' rand at
+ * src/Ensembles/selections.jl:70-99 for the recognised distributions). */
+enum nqcd_sampler_kind {
+    /* params = [mu_r, sd_r, mu_v, sd_v] per flattened dof (4*D values). */
+    NQCD_SAMPLER_GAUSSIAN = 0,
+    /* Mixed Tully ensemble: model_id = traj % n_models, p0 = p_lo + ((traj / n_models) % n_p),
+     * x ~ N(x0, p0/(20 sqrt 2)), v = p0/m.  params = [x0, p_lo, n_p]. */
+    NQCD_SAMPLER_TULLY_GRID = 1,
+    /* Harmonic Wigner bath (PositionHarmonicWigner/VelocityHarmonicWigner):
+     * params = [beta, mass]; frequencies from the spin-boson model. */
+    NQCD_SAMPLER_HARMONIC_WIGNER = 2
+};
+typedef struct {
+    int32_t kind;
+    int32_t initial_state;       /* pure electronic state, 0-based */
+    int32_t state_basis;         /* 0 = adiabatic, 1 = diabatic (NQCDistributions PureState) */
+    int32_t n_models;            /* models set with nqcd_mi355x_set_model slots 0..n_models-1 */
+    int64_t ntraj;
+    const double* params;
+    int64_t nparams;
+} nqcd_sampler_desc;
+
+/* Outputs reduced on the device (DynamicsOutputs.jl / TimeCorrelationFunctions.jl). */
+enum nqcd_output_kind {
+    NQCD_OUT_ADIABATIC_POPULATION = 0,   /* per save slot, nstates   (DynamicsOutputs.jl:198-203) */
+    NQCD_OUT_DIABATIC_POPULATION = 1,    /* per save slot, nstates   (DynamicsOutputs.jl:186-189) */
+    NQCD_OUT_SCATTERING_1D = 2,          /* final, [reflection(n), transmission(n)] (DynamicsOutputs.jl:313-338) */
+    NQCD_OUT_POPULATION_CORRELATION = 3, /* per save slot, n x n P(0)P(t)^T diabatic (TimeCorrelationFunctions.jl:61-88) */
+    NQCD_OUT_FINAL_ADIABATIC_POPULATION = 4 /* final, nstates */
+};
+/* Reductions (src/Ensembles/reductions.jl:8-52). */
+enum nqcd_reduction_kind { NQCD_RED_MEAN = 0, NQCD_RED_SUM = 1 };
+
+/* Per-trajectory fields for nqcd_mi355x_download (SortByTrajectoryReduction data). */
+enum nqcd_field {
+    NQCD_FIELD_R = 0,            /* double (ndofs, natoms[, nbeads], ntraj) final positions   */
+    NQCD_FIELD_V = 1,            /* double, final velocities                                  */
+    NQCD_FIELD_SIGMA_RE = 2,     /* double (n, n, ntraj) final density matrix, real part      */
+    NQCD_FIELD_SIGMA_IM = 3,     /* double (n, n, ntraj) imaginary part                       */
+    NQCD_FIELD_STATE = 4,        /* int32 (ntraj) final active state                          */
+    NQCD_FIELD_STEPS = 5,        /* int32 (ntraj) nuclear steps taken                         */
+    NQCD_FIELD_NHOPS = 6,        /* int32 (ntraj) accepted hops (OutputSurfaceHops)            */
+    NQCD_FIELD_SAVE_POP_AD = 7,  /* double (n, nsave, ntraj) adiabatic population per save     */
+    NQCD_FIELD_SAVE_POP_DI = 8,  /* double (n, nsave, ntraj) diabatic population per save      */
+    NQCD_FIELD_SAVE_STATE = 9    /* int32 (nsave, ntraj) active state per save                 */
+};
+
+typedef struct {
+    double kernel_ms;            /* device time of the trajectory kernels (HIP events) */
+    double wall_ms;              /* host wall time of nqcd_mi355x_run */
+    int64_t traj_steps;          /* sum over trajectories of nuclear steps taken */
+    int64_t launches;            /* kernel launches (or graph replays) issued */
+    int32_t block;               /* workgroup size used */
+    int32_t grid;                /* workgroups launched */
+} nqcd_run_stats;
+
+/* ABI version of the loaded library (compare with NQCD_MI355X_ABI_VERSION). No reference counterpart. */
+int32_t nqcd_mi355x_abi_version(void);
+
+/* Create a handle on HIP device `device` (one stream).  Replaces the ensemble_algorithm
+ * object handed to SciMLBase.solve (run_dynamics.jl:49,91-97). */
+int32_t nqcd_mi355x_open(int32_t device, nqcd_handle_t** out);
+
+/* Release every device allocation of the handle. */
+void nqcd_mi355x_close(nqcd_handle_t* h);
+
+/* Describe the simulation: Simulation/RingPolymerSimulation fields read by the integrator
+ * (simulations.jl:12-73,86-94), the method (fssh.jl:23-45 etc.) and the algorithm chosen
+ * by select_algorithm (IntegrationAlgorithms.jl:65-103). */
+int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* desc);
+
+/* Set model slot `slot` (0..3).  Replaces the NQCModels model held in sim.cache
+ * (Simulation cache, simulations.jl:28-30); several slots let one ensemble mix models,
+ * selected per trajectory by model_id. */
+int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t model_kind, const double* params, int64_t n);
+
+/* Upload initial conditions for ntraj trajectories (the DynamicsVariables built by
+ * sample_distribution/prob_func, selections.jl:38-99, fssh.jl:47-65, ehrenfest.jl:43-48).
+ * state may be NULL (sampled from diag(sigma), fssh.jl:54) and model_id may be NULL (slot 0). */
+int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const double* v,
+                                 const double* sig_re, const double* sig_im, const int32_t* state,
+                                 const int32_t* model_id);
+
+/* Sample initial conditions on the device from a recognised distribution
+ * (RandomSelection, selections.jl:70-99) using Philox stream 0 of `seed`. */
+int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* sampler, uint64_t seed);
+
+/* Integrate every trajectory over [t0, t1] with the method's callbacks (hopping after each
+ * step, surface_hopping.jl:2-16; termination, callbacks.jl:15-29) and record the save
+ * slots.  save_every > 0 saves slot k at step k*save_every (slot 0 = t0), mirroring
+ * saveat = t0:save_every*dt:t1.  Hop uniforms come from Philox stream 1 of philox_key.
+ * Replaces SciMLBase.solve(ensemble_problem, algorithm, ensemble_algorithm; ...). */
+int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_every,
+                        const nqcd_term_desc* term, uint64_t philox_key);
+
+/* Reduce an output over trajectories (MeanReduction / SumReduction, reductions.jl:8-52):
+ * out receives len doubles (n per save slot, 2n for SCATTERING_1D, n*n per save slot for
+ * POPULATION_CORRELATION, column-major). */
+int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t output_kind, int32_t reduction_kind, double* out, int64_t len);
+
+/* Copy a per-trajectory field to host memory (SortByTrajectoryReduction / EnsembleSaver data,
+ * Ensembles.jl:39-63, reductions.jl:208-209).  bytes must equal the field size. */
+int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes);
+
+/* Statistics of the last nqcd_mi355x_run (device time from HIP events on the handle's stream).
+ * Replaces the @timed wall clock around solve (run_dynamics.jl:91). */
+int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out);
+
+/* Message for the last failed call on h (thread-local when h is NULL). */
+const char* nqcd_mi355x_last_error(nqcd_handle_t* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NQCD_MI355X_H */

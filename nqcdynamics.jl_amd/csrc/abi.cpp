@@ -1,0 +1,540 @@
+// abi.cpp -- implementation of the C ABI declared in include/nqcd_mi355x.h.
+//
+// Host side of EnsembleMI355X: owns the device buffers of one ensemble, converts
+// between the reference's column-major (dofs, atoms, traj) host layout and the
+// device's structure-of-arrays layout, dispatches the trajectory kernels and
+// the deterministic reductions, and times the trajectory phase with HIP events.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nqcd_mi355x.h"
+#include "ensemble.h"
+#include "fssh2.h"
+#include "handle.h"
+#include "spin_boson.h"
+
+using namespace nqcd;
+
+namespace nqcd {
+
+thread_local std::string g_tls_error;
+
+int32_t fail(nqcd_handle_t* h, int32_t code, const std::string& msg) {
+    if (h) h->err = msg; else g_tls_error = msg;
+    return code;
+}
+
+}  // namespace nqcd
+
+namespace {
+
+bool is_two_state_1d(const nqcd_handle* h) {
+    return (h->prob.method == NQCD_METHOD_FSSH || h->prob.method == NQCD_METHOD_EHRENFEST) &&
+           h->prob.algorithm == NQCD_ALG_BAB_TSIT5 && h->prob.n_states == 2 && h->D == 1;
+}
+
+bool is_spin_boson(const nqcd_handle* h) {
+    return (h->prob.method == NQCD_METHOD_FSSH || h->prob.method == NQCD_METHOD_EHRENFEST) &&
+           h->prob.algorithm == NQCD_ALG_BAB_TSIT5 && h->prob.n_states == 2 && h->model_set[0] &&
+           h->models[0].kind == MODEL_SPIN_BOSON;
+}
+
+int nsub_of(const nqcd_handle* h) { return h->prob.electronic_substeps > 0 ? h->prob.electronic_substeps : 5; }
+
+// Queue order: stable grouping by model so a wavefront refills with one model.
+std::vector<int32_t> order_by_model(const std::vector<int32_t>& mid, int64_t ntraj) {
+    std::vector<int32_t> ord(ntraj);
+    int64_t pos = 0;
+    for (int m = 0; m < kMaxModels; ++m)
+        for (int64_t i = 0; i < ntraj; ++i)
+            if ((mid.empty() ? 0 : mid[i]) == m) ord[pos++] = (int32_t)i;
+    return ord;
+}
+
+int32_t alloc_results(nqcd_handle* h) {
+    const int64_t n = h->ntraj;
+    const int D = h->D;
+    HIPCHK(h, h->r.ensure(sizeof(double) * D * n));
+    HIPCHK(h, h->v.ensure(sizeof(double) * D * n));
+    HIPCHK(h, h->sig.ensure(sizeof(double) * 4 * n));
+    HIPCHK(h, h->state.ensure(sizeof(int32_t) * n));
+    HIPCHK(h, h->steps.ensure(sizeof(int32_t) * n));
+    HIPCHK(h, h->nhops.ensure(sizeof(int32_t) * n));
+    HIPCHK(h, h->save_ad.ensure(sizeof(double) * 2 * h->nsave * n));
+    HIPCHK(h, h->save_di.ensure(sizeof(double) * 2 * h->nsave * n));
+    HIPCHK(h, h->save_state.ensure(sizeof(int32_t) * h->nsave * n));
+    HIPCHK(h, h->queue.ensure(sizeof(unsigned long long)));
+    return NQCD_OK;
+}
+
+Fssh2Args make_args(nqcd_handle* h, double t0, int64_t nsteps, const nqcd_term_desc* term, uint64_t key) {
+    Fssh2Args A{};
+    A.ntraj = h->ntraj;
+    A.method = (h->prob.method == NQCD_METHOD_FSSH) ? METHOD_FSSH : METHOD_EHRENFEST;
+    A.rescaling = h->prob.rescaling;
+    A.nsub = nsub_of(h);
+    A.save_every = h->save_every;
+    A.nsave = h->nsave;
+    A.term_kind = term ? term->kind : 0;
+    A.term_dof = term ? term->dof : 0;
+    A.term_value = term ? term->value : 0.0;
+    A.dt = h->prob.dt;
+    A.t0 = t0;
+    A.nsteps = nsteps;
+    A.key = key;
+    for (int I = 0; I < h->D && I < kMaxDof1; ++I) A.inv_mass[I] = h->inv_mass[I];
+    int nm = 0;
+    for (int m = 0; m < kMaxModels; ++m) { A.models[m] = h->models[m]; if (h->model_set[m]) nm = m + 1; }
+    A.nmodels = nm;
+    A.r0 = h->r0.as<double>(); A.v0 = h->v0.as<double>(); A.sig0 = h->sig0.as<double>();
+    A.state0 = h->state0_given ? h->state0.as<int32_t>() : nullptr;
+    A.model_id = h->model_id_given ? h->model_id.as<int32_t>() : nullptr;
+    A.order = h->order_given ? h->order.as<int32_t>() : nullptr;
+    A.r = h->r.as<double>(); A.v = h->v.as<double>(); A.sig = h->sig.as<double>();
+    A.state = h->state.as<int32_t>(); A.steps = h->steps.as<int32_t>(); A.nhops = h->nhops.as<int32_t>();
+    A.save_pop_ad = h->save_ad.as<double>(); A.save_pop_di = h->save_di.as<double>();
+    A.save_state = h->save_state.as<int32_t>();
+    A.queue = h->queue.as<unsigned long long>();
+    return A;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nqcd_mi355x_abi_version(void) { return NQCD_MI355X_ABI_VERSION; }
+
+const char* nqcd_mi355x_last_error(nqcd_handle_t* h) { return h ? h->err.c_str() : g_tls_error.c_str(); }
+
+int32_t nqcd_mi355x_open(int32_t device, nqcd_handle_t** out) {
+    if (!out) return fail(nullptr, NQCD_ERR_ARG, "out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return fail(nullptr, NQCD_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    if (device < 0 || device >= count) return fail(nullptr, NQCD_ERR_ARG, "device index out of range");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, NQCD_ERR_DEVICE, hipGetErrorString(e));
+    nqcd_handle* h = new nqcd_handle();
+    h->device = device;
+    if (hipStreamCreate(&h->stream) != hipSuccess || hipEventCreate(&h->ev0) != hipSuccess ||
+        hipEventCreate(&h->ev1) != hipSuccess) {
+        delete h;
+        return fail(nullptr, NQCD_ERR_DEVICE, "stream/event creation failed");
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->cu_count = prop.multiProcessorCount;
+    if (h->cu_count <= 0) h->cu_count = 256;
+    *out = h;
+    return NQCD_OK;
+}
+
+void nqcd_mi355x_close(nqcd_handle_t* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    DevBuf* bufs[] = {&h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath,
+                      &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
+                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small};
+    for (DevBuf* b : bufs) b->release();
+    (void)hipEventDestroy(h->ev0);
+    (void)hipEventDestroy(h->ev1);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
+    if (!h || !d) return fail(h, NQCD_ERR_ARG, "null argument");
+    if (d->n_states < 1 || d->ndofs < 1 || d->natoms < 1 || d->nbeads < 1)
+        return fail(h, NQCD_ERR_ARG, "n_states, ndofs, natoms and nbeads must be >= 1");
+    if (!(d->dt > 0.0) || !std::isfinite(d->dt)) return fail(h, NQCD_ERR_ARG, "dt must be positive and finite");
+    if (!d->masses) return fail(h, NQCD_ERR_ARG, "masses is null");
+    if (d->rescaling < 0 || d->rescaling > 2) return fail(h, NQCD_ERR_ARG, "This mode of rescaling is not implemented");
+    if (d->precision < 0 || d->precision > 1) return fail(h, NQCD_ERR_ARG, "unknown precision");
+    if (d->schedule < 0 || d->schedule > 1) return fail(h, NQCD_ERR_ARG, "unknown schedule");
+    const bool sh_or_eh = d->method == NQCD_METHOD_FSSH || d->method == NQCD_METHOD_EHRENFEST;
+    if (!(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states == 2))
+        return fail(h, NQCD_ERR_UNSUPPORTED,
+                    "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} with two states and BABwithTsit5 in this build");
+    h->prob = *d;
+    h->masses.assign(d->masses, d->masses + d->natoms);
+    for (double m : h->masses)
+        if (!(m > 0.0)) return fail(h, NQCD_ERR_ARG, "masses must be positive");
+    h->D = d->ndofs * d->natoms;
+    h->inv_mass.resize(h->D);
+    for (int I = 0; I < h->D; ++I) h->inv_mass[I] = 1.0 / h->masses[I / d->ndofs];
+    h->prob.masses = nullptr;
+    h->have_problem = true;
+    h->have_initial = h->have_results = false;
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t kind, const double* params, int64_t n) {
+    if (!h) return fail(h, NQCD_ERR_ARG, "null handle");
+    if (slot < 0 || slot >= kMaxModels) return fail(h, NQCD_ERR_ARG, "model slot must be in 0..3");
+    if (n < 0 || (n > 0 && !params)) return fail(h, NQCD_ERR_ARG, "params");
+    ModelSlot m{};
+    m.kind = kind;
+    int need = -1;
+    switch (kind) {
+    case NQCD_MODEL_TULLY1: need = 4; break;
+    case NQCD_MODEL_TULLY2: need = 5; break;
+    case NQCD_MODEL_TULLY3: need = 3; break;
+    case NQCD_MODEL_DW_GAUSSIAN: need = 5; break;
+    case NQCD_MODEL_SPIN_BOSON: {
+        if (slot != 0) return fail(h, NQCD_ERR_UNSUPPORTED, "spin-boson model must use slot 0");
+        if (n < 4 || (n - 2) % 2 != 0) return fail(h, NQCD_ERR_ARG, "spin-boson params: eps, Delta, omega[nb], c[nb]");
+        const int64_t nb = (n - 2) / 2;
+        h->sb_eps = params[0]; h->sb_delta = params[1];
+        h->sb_omega.assign(params + 2, params + 2 + nb);
+        h->sb_c.assign(params + 2 + nb, params + 2 + 2 * nb);
+        m.p[0] = params[0]; m.p[1] = params[1]; m.p[2] = (double)nb;
+        h->models[slot] = m;
+        h->model_set[slot] = true;
+        return NQCD_OK;
+    }
+    default: return fail(h, NQCD_ERR_UNSUPPORTED, "unknown model kind");
+    }
+    if (n != need) return fail(h, NQCD_ERR_ARG, "wrong number of model parameters");
+    for (int i = 0; i < need; ++i) m.p[i] = params[i];
+    h->models[slot] = m;
+    h->model_set[slot] = true;
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const double* v,
+                                 const double* sig_re, const double* sig_im, const int32_t* state,
+                                 const int32_t* model_id) {
+    if (!h) return fail(h, NQCD_ERR_ARG, "null handle");
+    if (!h->have_problem) return fail(h, NQCD_ERR_STATE, "set_problem must be called first");
+    if (ntraj < 1 || ntraj > (int64_t)1 << 31) return fail(h, NQCD_ERR_ARG, "ntraj out of range");
+    if (!r || !v || !sig_re || !sig_im) return fail(h, NQCD_ERR_ARG, "null input array");
+    const int D = h->D, n = h->prob.n_states;
+    if (model_id)
+        for (int64_t i = 0; i < ntraj; ++i)
+            if (model_id[i] < 0 || model_id[i] >= kMaxModels || !h->model_set[model_id[i]])
+                return fail(h, NQCD_ERR_ARG, "model_id refers to an unset model slot");
+    if (!h->model_set[0] && !model_id) return fail(h, NQCD_ERR_STATE, "model slot 0 is not set");
+    if (state)
+        for (int64_t i = 0; i < ntraj; ++i)
+            if (state[i] < -1 || state[i] >= n) return fail(h, NQCD_ERR_ARG, "state out of range");
+    h->ntraj = ntraj;
+    std::vector<double> tr((size_t)D * ntraj), tv((size_t)D * ntraj), ts((size_t)4 * ntraj);
+    for (int64_t i = 0; i < ntraj; ++i)
+        for (int I = 0; I < D; ++I) {
+            tr[(size_t)I * ntraj + i] = r[(size_t)i * D + I];
+            tv[(size_t)I * ntraj + i] = v[(size_t)i * D + I];
+        }
+    for (int64_t i = 0; i < ntraj; ++i) {  // (n, n, ntraj) column-major -> compact Hermitian rows
+        const size_t o = (size_t)i * n * n;
+        ts[i] = sig_re[o + 0];
+        ts[ntraj + i] = sig_re[o + 3];
+        ts[2 * ntraj + i] = sig_re[o + 2];  // element (1,2) in 1-based = row 0, col 1
+        ts[3 * ntraj + i] = sig_im[o + 2];
+    }
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, h->r0.ensure(sizeof(double) * tr.size()));
+    HIPCHK(h, h->v0.ensure(sizeof(double) * tv.size()));
+    HIPCHK(h, h->sig0.ensure(sizeof(double) * ts.size()));
+    HIPCHK(h, hipMemcpyAsync(h->r0.p, tr.data(), sizeof(double) * tr.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->v0.p, tv.data(), sizeof(double) * tv.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->sig0.p, ts.data(), sizeof(double) * ts.size(), hipMemcpyHostToDevice, h->stream));
+    h->state0_given = state != nullptr;
+    if (state) {
+        HIPCHK(h, h->state0.ensure(sizeof(int32_t) * ntraj));
+        HIPCHK(h, hipMemcpyAsync(h->state0.p, state, sizeof(int32_t) * ntraj, hipMemcpyHostToDevice, h->stream));
+    }
+    std::vector<int32_t> mid;
+    h->model_id_given = model_id != nullptr;
+    if (model_id) {
+        mid.assign(model_id, model_id + ntraj);
+        HIPCHK(h, h->model_id.ensure(sizeof(int32_t) * ntraj));
+        HIPCHK(h, hipMemcpyAsync(h->model_id.p, model_id, sizeof(int32_t) * ntraj, hipMemcpyHostToDevice, h->stream));
+    }
+    std::vector<int32_t> ord = order_by_model(mid, ntraj);
+    HIPCHK(h, h->order.ensure(sizeof(int32_t) * ntraj));
+    HIPCHK(h, hipMemcpyAsync(h->order.p, ord.data(), sizeof(int32_t) * ntraj, hipMemcpyHostToDevice, h->stream));
+    h->order_given = true;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->have_initial = true;
+    h->sb_initial = false;
+    h->have_results = false;
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S, uint64_t seed) {
+    if (!h || !S) return fail(h, NQCD_ERR_ARG, "null argument");
+    if (!h->have_problem) return fail(h, NQCD_ERR_STATE, "set_problem must be called first");
+    if (S->ntraj < 1 || S->ntraj > (int64_t)1 << 31) return fail(h, NQCD_ERR_ARG, "ntraj out of range");
+    const int n = h->prob.n_states;
+    if (S->initial_state < 0 || S->initial_state >= n) return fail(h, NQCD_ERR_ARG, "initial_state out of range");
+    const int64_t ntraj = S->ntraj;
+    const int D = h->D;
+    HIPCHK(h, hipSetDevice(h->device));
+    if (S->kind == NQCD_SAMPLER_HARMONIC_WIGNER) {
+        if (!is_spin_boson(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "harmonic Wigner sampling needs the spin-boson model");
+        int32_t rc = sb_sample_initial(h, S, seed);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = true; h->have_results = false; }
+        return rc;
+    }
+    if (!is_two_state_1d(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "sampler needs a two-state one-dof problem");
+    h->ntraj = ntraj;
+    HIPCHK(h, h->r0.ensure(sizeof(double) * D * ntraj));
+    HIPCHK(h, h->v0.ensure(sizeof(double) * D * ntraj));
+    HIPCHK(h, h->sig0.ensure(sizeof(double) * 4 * ntraj));
+    HIPCHK(h, h->model_id.ensure(sizeof(int32_t) * ntraj));
+    HIPCHK(h, h->order.ensure(sizeof(int32_t) * ntraj));
+    HIPCHK(h, h->models_dev.ensure(sizeof(ModelSlot) * kMaxModels));
+    HIPCHK(h, hipMemcpyAsync(h->models_dev.p, h->models, sizeof(ModelSlot) * kMaxModels, hipMemcpyHostToDevice, h->stream));
+    std::vector<int32_t> ord(ntraj);
+    if (S->kind == NQCD_SAMPLER_TULLY_GRID) {
+        if (S->nparams != 3 || !S->params) return fail(h, NQCD_ERR_ARG, "TULLY_GRID params: x0, p_lo, n_p");
+        const int nm = S->n_models;
+        if (nm < 1 || nm > kMaxModels) return fail(h, NQCD_ERR_ARG, "n_models must be 1..4");
+        for (int m = 0; m < nm; ++m)
+            if (!h->model_set[m]) return fail(h, NQCD_ERR_STATE, "model slot not set");
+        const int n_p = (int)S->params[2];
+        if (n_p < 1) return fail(h, NQCD_ERR_ARG, "n_p must be >= 1");
+        HIPCHK(h, launch_sample_tully_grid(ntraj, nm, S->params[0], S->params[1], n_p, h->masses[0], seed,
+                                           h->r0.as<double>(), h->v0.as<double>(), h->model_id.as<int32_t>(), h->stream));
+        // Queue order: slowest initial momentum first (longest trajectories), grouped by model.
+        int64_t pos = 0;
+        for (int pi = 0; pi < n_p; ++pi)
+            for (int m = 0; m < nm; ++m)
+                for (int64_t i = (int64_t)m + (int64_t)nm * pi; i < ntraj; i += (int64_t)nm * n_p) ord[pos++] = (int32_t)i;
+        h->model_id_given = true;
+    } else if (S->kind == NQCD_SAMPLER_GAUSSIAN) {
+        if (S->nparams != 4 * D || !S->params) return fail(h, NQCD_ERR_ARG, "GAUSSIAN params: 4 per dof");
+        if (!h->model_set[0]) return fail(h, NQCD_ERR_STATE, "model slot 0 not set");
+        HIPCHK(h, h->small.ensure(sizeof(double) * 4 * D));
+        HIPCHK(h, hipMemcpyAsync(h->small.p, S->params, sizeof(double) * 4 * D, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, launch_sample_gaussian(ntraj, D, h->small.as<double>(), seed, h->r0.as<double>(), h->v0.as<double>(),
+                                         h->stream));
+        for (int64_t i = 0; i < ntraj; ++i) ord[i] = (int32_t)i;
+        h->model_id_given = false;
+    } else {
+        return fail(h, NQCD_ERR_UNSUPPORTED, "unknown sampler kind");
+    }
+    HIPCHK(h, launch_init_density2(ntraj, S->initial_state, S->state_basis == 1, h->models_dev.as<ModelSlot>(),
+                                   h->model_id_given ? h->model_id.as<int32_t>() : nullptr, h->r0.as<double>(),
+                                   h->sig0.as<double>(), h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->order.p, ord.data(), sizeof(int32_t) * ntraj, hipMemcpyHostToDevice, h->stream));
+    h->order_given = true;
+    if (S->state_basis == 1) {
+        h->state0_given = false;  // active state drawn from diag(sigma) (fssh.jl:54)
+    } else {
+        std::vector<int32_t> st(ntraj, S->initial_state);
+        HIPCHK(h, h->state0.ensure(sizeof(int32_t) * ntraj));
+        HIPCHK(h, hipMemcpyAsync(h->state0.p, st.data(), sizeof(int32_t) * ntraj, hipMemcpyHostToDevice, h->stream));
+        h->state0_given = true;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->have_initial = true;
+    h->sb_initial = false;
+    h->have_results = false;
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_every, const nqcd_term_desc* term,
+                        uint64_t key) {
+    if (!h) return fail(h, NQCD_ERR_ARG, "null handle");
+    if (!h->have_initial) return fail(h, NQCD_ERR_STATE, "no initial conditions loaded");
+    if (!(t1 >= t0)) return fail(h, NQCD_ERR_ARG, "t1 must be >= t0");
+    if (save_every < 0) return fail(h, NQCD_ERR_ARG, "save_every must be >= 0");
+    if (term && term->kind != NQCD_TERM_NONE) {
+        if (term->kind != NQCD_TERM_ABS_GT) return fail(h, NQCD_ERR_UNSUPPORTED, "unknown termination predicate");
+        if (term->dof < 0 || term->dof >= h->D) return fail(h, NQCD_ERR_ARG, "termination dof out of range");
+    }
+    const double nsteps_d = std::round((t1 - t0) / h->prob.dt);
+    if (nsteps_d > 2.0e9) return fail(h, NQCD_ERR_ARG, "too many steps");
+    const int64_t nsteps = (int64_t)nsteps_d;
+    h->save_every = save_every;
+    h->nsave = (save_every > 0) ? (int32_t)(nsteps / save_every + 1) : 1;
+    HIPCHK(h, hipSetDevice(h->device));
+    const auto wall0 = std::chrono::steady_clock::now();
+    h->stats = nqcd_run_stats{};
+    if (is_spin_boson(h)) {
+        if (!h->sb_initial) return fail(h, NQCD_ERR_STATE, "spin-boson runs need nqcd_mi355x_sample_initial");
+        int32_t rc = sb_run(h, t0, nsteps, term, key);
+        if (rc != NQCD_OK) return rc;
+    } else if (is_two_state_1d(h)) {
+        if (int32_t rc = alloc_results(h)) return rc;
+        Fssh2Args A = make_args(h, t0, nsteps, term, key);
+        if (h->prob.schedule == NQCD_SCHED_PERSISTENT) {
+            const int block = 256;
+            int bpc = 0;
+            HIPCHK(h, fssh2_persistent_occupancy(h->prob.precision, block, &bpc));
+            if (bpc < 1) bpc = 1;
+            int grid = bpc * h->cu_count;
+            const int64_t need = (h->ntraj + block - 1) / block;
+            if ((int64_t)grid > need) grid = (int)need;
+            HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+            HIPCHK(h, fssh2_launch_persistent(A, h->prob.precision, grid, block, h->stream));
+            HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+            h->stats.block = block; h->stats.grid = grid; h->stats.launches = 2;
+        } else {
+            if (h->prob.precision != NQCD_PREC_FP64)
+                return fail(h, NQCD_ERR_UNSUPPORTED, "the step-per-launch schedule runs in FP64 only");
+            const int64_t n = h->ntraj;
+            HIPCHK(h, h->plainS.ensure(sizeof(double) * kPlainRows * n));
+            HIPCHK(h, h->plainSi.ensure(sizeof(int32_t) * kPlainIntRows * n));
+            double* S = h->plainS.as<double>();
+            int32_t* Si = h->plainSi.as<int32_t>();
+            HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+            HIPCHK(h, fssh2_launch_plain_init(A, S, Si, h->stream));
+            // Identical step launches captured once into a graph and replayed.
+            const int64_t chunk = 64;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            if (nsteps >= chunk) {
+                HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                for (int64_t j = 0; j < chunk; ++j) (void)fssh2_launch_plain_step(A, S, Si, h->stream);
+                HIPCHK(h, hipStreamEndCapture(h->stream, &graph));
+                HIPCHK(h, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            }
+            int64_t done = 0, launches = 1;
+            for (; done + chunk <= nsteps && exec; done += chunk) {
+                HIPCHK(h, hipGraphLaunch(exec, h->stream));
+                launches += chunk;
+            }
+            for (; done < nsteps; ++done, ++launches) HIPCHK(h, fssh2_launch_plain_step(A, S, Si, h->stream));
+            HIPCHK(h, fssh2_launch_plain_finish(A, S, Si, h->stream));
+            HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            h->stats.block = 256; h->stats.grid = (int)((n + 255) / 256); h->stats.launches = launches + 1;
+        }
+    } else {
+        return fail(h, NQCD_ERR_UNSUPPORTED, "unsupported problem for run");
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    float ms = 0.0f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->stats.kernel_ms = ms;
+    std::vector<int32_t> st(h->ntraj);
+    HIPCHK(h, hipMemcpy(st.data(), h->steps.p, sizeof(int32_t) * h->ntraj, hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (int32_t s : st) tot += s;
+    h->stats.traj_steps = tot;
+    h->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    h->have_results = true;
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* out, int64_t len) {
+    if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
+    if (!h->have_results) return fail(h, NQCD_ERR_STATE, "no results: call nqcd_mi355x_run first");
+    if (red != NQCD_RED_MEAN && red != NQCD_RED_SUM) return fail(h, NQCD_ERR_ARG, "unknown reduction");
+    const int n = h->prob.n_states;
+    const int64_t nt = h->ntraj;
+    int32_t rows = 0;
+    const double* src = nullptr;
+    HIPCHK(h, hipSetDevice(h->device));
+    switch (kind) {
+    case NQCD_OUT_ADIABATIC_POPULATION: rows = n * h->nsave; src = h->save_ad.as<double>(); break;
+    case NQCD_OUT_DIABATIC_POPULATION: rows = n * h->nsave; src = h->save_di.as<double>(); break;
+    case NQCD_OUT_FINAL_ADIABATIC_POPULATION: rows = n; break;
+    case NQCD_OUT_SCATTERING_1D: rows = 2 * n; break;
+    case NQCD_OUT_POPULATION_CORRELATION: rows = n * n * h->nsave; break;
+    default: return fail(h, NQCD_ERR_UNSUPPORTED, "unknown output kind");
+    }
+    if (len != rows) return fail(h, NQCD_ERR_ARG, "len does not match the output size");
+    if (!src) {
+        HIPCHK(h, h->tmp.ensure(sizeof(double) * rows * nt));
+        HIPCHK(h, launch_fill_output2(kind, h->prob.method == NQCD_METHOD_FSSH ? 0 : 1, nt, h->nsave, 0,
+                                      h->r.as<double>(), h->sig.as<double>(), h->state.as<int32_t>(),
+                                      h->save_di.as<double>(), h->tmp.as<double>(), h->stream));
+        src = h->tmp.as<double>();
+    }
+    HIPCHK(h, h->partial.ensure(sizeof(double) * rows * reduce_nchunk(nt)));
+    HIPCHK(h, h->small.ensure(sizeof(double) * rows));
+    const double scale = (red == NQCD_RED_MEAN) ? 1.0 / (double)nt : 1.0;
+    HIPCHK(h, launch_reduce_rows(src, rows, nt, h->partial.as<double>(), scale, h->small.as<double>(), h->stream));
+    HIPCHK(h, hipMemcpyAsync(out, h->small.p, sizeof(double) * rows, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes) {
+    if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
+    if (!h->have_results) return fail(h, NQCD_ERR_STATE, "no results: call nqcd_mi355x_run first");
+    const int64_t nt = h->ntraj;
+    const int D = h->D, n = h->prob.n_states, ns = h->nsave;
+    HIPCHK(h, hipSetDevice(h->device));
+    auto fetch = [&](const DevBuf& b, size_t sz, std::vector<char>& host) -> hipError_t {
+        host.resize(sz);
+        return hipMemcpy(host.data(), b.p, sz, hipMemcpyDeviceToHost);
+    };
+    std::vector<char> raw;
+    switch (field) {
+    case NQCD_FIELD_R:
+    case NQCD_FIELD_V: {
+        if (bytes != (int64_t)sizeof(double) * D * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(field == NQCD_FIELD_R ? h->r : h->v, sizeof(double) * D * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int I = 0; I < D; ++I) o[(size_t)i * D + I] = s[(size_t)I * nt + i];
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_SIGMA_RE:
+    case NQCD_FIELD_SIGMA_IM: {
+        if (bytes != (int64_t)sizeof(double) * n * n * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(h->sig, sizeof(double) * 4 * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i) {
+            double* m = o + (size_t)i * 4;
+            if (field == NQCD_FIELD_SIGMA_RE) { m[0] = s[i]; m[3] = s[nt + i]; m[1] = m[2] = s[2 * nt + i]; }
+            else { m[0] = m[3] = 0.0; m[2] = s[3 * nt + i]; m[1] = -s[3 * nt + i]; }
+        }
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_STATE:
+    case NQCD_FIELD_STEPS:
+    case NQCD_FIELD_NHOPS: {
+        if (bytes != (int64_t)sizeof(int32_t) * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        const DevBuf& b = field == NQCD_FIELD_STATE ? h->state : (field == NQCD_FIELD_STEPS ? h->steps : h->nhops);
+        HIPCHK(h, hipMemcpy(out, b.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost));
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_SAVE_POP_AD:
+    case NQCD_FIELD_SAVE_POP_DI: {
+        if (bytes != (int64_t)sizeof(double) * n * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(field == NQCD_FIELD_SAVE_POP_AD ? h->save_ad : h->save_di, sizeof(double) * n * ns * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl)
+                for (int c = 0; c < n; ++c) o[((size_t)i * ns + sl) * n + c] = s[((size_t)sl * n + c) * nt + i];
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_SAVE_STATE: {
+        if (bytes != (int64_t)sizeof(int32_t) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(h->save_state, sizeof(int32_t) * ns * nt, raw));
+        const int32_t* s = reinterpret_cast<const int32_t*>(raw.data());
+        int32_t* o = static_cast<int32_t*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = s[(size_t)sl * nt + i];
+        return NQCD_OK;
+    }
+    default: return fail(h, NQCD_ERR_ARG, "unknown field");
+    }
+}
+
+int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out) {
+    if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
+    if (!h->have_results) return fail(h, NQCD_ERR_STATE, "no run yet");
+    *out = h->stats;
+    return NQCD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// ensemble.hip -- device-side initial-condition sampling and deterministic
+// ensemble reductions.
+//
+// Sampling stands in for RandomSelection/sample_distribution
+// (src/Ensembles/selections.jl:70-99) for the recognised distributions; the
+// reductions stand in for MeanReduction/SumReduction over the outputs of
+// DynamicsOutputs.jl / TimeCorrelationFunctions.jl (src/Ensembles/reductions.jl:8-52).
+// Reductions run in a fixed order (chunked tree sums, then an ordered sum of
+// chunk partials) so results do not depend on the device or launch shape.
+#include "ensemble.h"
+
+namespace nqcd {
+
+// x ~ N(x0, p0/(20 sqrt 2)), v = p0/m; model_id = traj % nm; p0 = p_lo + (traj/nm) % n_p.
+__global__ __launch_bounds__(256) void sample_tully_grid(int64_t ntraj, int32_t nm, double x0, double p_lo, int32_t n_p,
+                                                         double mass, uint64_t key, double* r0, double* v0,
+                                                         int32_t* model_id) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntraj) return;
+    const int32_t m = (int32_t)(i % nm);
+    const double p0 = p_lo + (double)((i / nm) % n_p);
+    double z0, z1;
+    philox_normal2((uint64_t)i, 0u, STREAM_INIT, key, z0, z1);
+    r0[i] = x0 + (p0 / (20.0 * 1.4142135623730951)) * z0;
+    v0[i] = p0 / mass;
+    model_id[i] = m;
+}
+
+// Per-dof Gaussian: r = mu_r + sd_r z0, v = mu_v + sd_v z1 with (z0, z1) from counter (traj, dof, 0).
+__global__ __launch_bounds__(256) void sample_gaussian(int64_t ntraj, int32_t D, const double* prm, uint64_t key,
+                                                       double* r0, double* v0) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntraj) return;
+    for (int I = 0; I < D; ++I) {
+        double z0, z1;
+        philox_normal2((uint64_t)i, (uint32_t)I, STREAM_INIT, key, z0, z1);
+        r0[(int64_t)I * ntraj + i] = prm[4 * I + 0] + prm[4 * I + 1] * z0;
+        v0[(int64_t)I * ntraj + i] = prm[4 * I + 2] + prm[4 * I + 3] * z1;
+    }
+}
+
+// Pure electronic state in the adiabatic or diabatic basis for two-state one-dof
+// models (density_matrix_dynamics.jl:37-75): sigma_ad = U^T rho U.
+__global__ __launch_bounds__(256) void init_density2(int64_t ntraj, int32_t state, int32_t diabatic, const ModelSlot* models,
+                                                     const int32_t* model_id, const double* r0, double* sig0) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntraj) return;
+    double a, b, x;
+    if (!diabatic) {
+        a = (state == 0) ? 1.0 : 0.0; b = 1.0 - a; x = 0.0;
+    } else {
+        const ModelSlot m = models[model_id ? model_id[i] : 0];
+        double v11, v22, v12, d11, d22, d12, E0, E1, U[4];
+        model1d_eval<double>(m, r0[i], v11, v22, v12, d11, d22, d12);
+        eigen2(v11, v12, v22, E0, E1, U, false);
+        const double u0 = U[state * 2 + 0], u1 = U[state * 2 + 1];  // row `state` of U
+        a = u0 * u0; b = u1 * u1; x = u0 * u1;
+    }
+    sig0[i] = a; sig0[ntraj + i] = b; sig0[2 * ntraj + i] = x; sig0[3 * ntraj + i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Per-trajectory output rows for the two-state kernels.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fill_output2(int32_t kind, int32_t method, int64_t ntraj, int32_t nsave,
+                                                    int32_t xdof, const double* r, const double* sig,
+                                                    const int32_t* state, const double* save_pop_di, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntraj) return;
+    if (kind == OUTK_FINAL_ADIABATIC || kind == OUTK_SCATTERING_1D) {
+        double p0, p1;
+        if (method == 0) { p0 = (state[i] == 0) ? 1.0 : 0.0; p1 = 1.0 - p0; }
+        else { p0 = sig[i]; p1 = sig[ntraj + i]; }
+        if (kind == OUTK_FINAL_ADIABATIC) { out[i] = p0; out[ntraj + i] = p1; return; }
+        // OutputStateResolvedScattering1D: rows [reflection(2), transmission(2)] (DynamicsOutputs.jl:331-336)
+        const bool trans = r[(int64_t)xdof * ntraj + i] > 0.0;
+        out[0 * ntraj + i] = trans ? 0.0 : p0;
+        out[1 * ntraj + i] = trans ? 0.0 : p1;
+        out[2 * ntraj + i] = trans ? p0 : 0.0;
+        out[3 * ntraj + i] = trans ? p1 : 0.0;
+        return;
+    }
+    if (kind == OUTK_POP_CORRELATION) {
+        // out[slot][i + 2j] = P_i(0) P_j(t) (TimeCorrelationFunctions.jl:86-88), column-major 2x2
+        const double q0 = save_pop_di[0 * ntraj + i], q1 = save_pop_di[1 * ntraj + i];
+        for (int s = 0; s < nsave; ++s) {
+            const double p0 = save_pop_di[((int64_t)s * 2 + 0) * ntraj + i];
+            const double p1 = save_pop_di[((int64_t)s * 2 + 1) * ntraj + i];
+            out[((int64_t)s * 4 + 0) * ntraj + i] = q0 * p0;
+            out[((int64_t)s * 4 + 1) * ntraj + i] = q1 * p0;
+            out[((int64_t)s * 4 + 2) * ntraj + i] = q0 * p1;
+            out[((int64_t)s * 4 + 3) * ntraj + i] = q1 * p1;
+        }
+    }
+}
+
+// Deterministic two-pass row reduction: rows x ntraj -> rows.
+__global__ __launch_bounds__(256) void reduce_rows_pass1(const double* data, int64_t ntraj, int32_t nchunk, double* partial) {
+    __shared__ double sm[256];
+    const int row = blockIdx.y, chunk = blockIdx.x;
+    const int64_t len = (ntraj + nchunk - 1) / nchunk;
+    const int64_t b = (int64_t)chunk * len;
+    const int64_t e = (b + len < ntraj) ? b + len : ntraj;
+    const double* d = data + (int64_t)row * ntraj;
+    double s = 0.0;
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) s += d[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)row * nchunk + chunk] = sm[0];
+}
+
+__global__ void reduce_rows_pass2(const double* partial, int32_t rows, int32_t nchunk, double scale, double* out) {
+    const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (row >= rows) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += partial[(int64_t)row * nchunk + c];
+    out[row] = s * scale;
+}
+
+static inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+hipError_t launch_sample_tully_grid(int64_t ntraj, int32_t nm, double x0, double p_lo, int32_t n_p, double mass,
+                                    uint64_t key, double* r0, double* v0, int32_t* model_id, hipStream_t s) {
+    hipLaunchKernelGGL(sample_tully_grid, dim3(grid_for(ntraj, 256)), dim3(256), 0, s, ntraj, nm, x0, p_lo, n_p, mass,
+                       key, r0, v0, model_id);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_gaussian(int64_t ntraj, int32_t D, const double* prm, uint64_t key, double* r0, double* v0,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(sample_gaussian, dim3(grid_for(ntraj, 256)), dim3(256), 0, s, ntraj, D, prm, key, r0, v0);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_density2(int64_t ntraj, int32_t state, int32_t diabatic, const ModelSlot* models,
+                                const int32_t* model_id, const double* r0, double* sig0, hipStream_t s) {
+    hipLaunchKernelGGL(init_density2, dim3(grid_for(ntraj, 256)), dim3(256), 0, s, ntraj, state, diabatic, models,
+                       model_id, r0, sig0);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_output2(int32_t kind, int32_t method, int64_t ntraj, int32_t nsave, int32_t xdof, const double* r,
+                               const double* sig, const int32_t* state, const double* save_pop_di, double* out,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(fill_output2, dim3(grid_for(ntraj, 256)), dim3(256), 0, s, kind, method, ntraj, nsave, xdof, r,
+                       sig, state, save_pop_di, out);
+    return hipGetLastError();
+}
+
+int reduce_nchunk(int64_t ntraj) {
+    int64_t c = (ntraj + 4095) / 4096;
+    if (c < 1) c = 1;
+    if (c > 1024) c = 1024;
+    return (int)c;
+}
+
+hipError_t launch_reduce_rows(const double* data, int32_t rows, int64_t ntraj, double* partial, double scale, double* out,
+                              hipStream_t s) {
+    const int nchunk = reduce_nchunk(ntraj);
+    hipLaunchKernelGGL(reduce_rows_pass1, dim3(nchunk, rows), dim3(256), 0, s, data, ntraj, nchunk, partial);
+    hipLaunchKernelGGL(reduce_rows_pass2, dim3(grid_for(rows, 64)), dim3(64), 0, s, partial, rows, nchunk, scale, out);
+    return hipGetLastError();
+}
+
+}  // namespace nqcd

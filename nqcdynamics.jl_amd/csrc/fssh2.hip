@@ -1,0 +1,397 @@
+// fssh2.hip -- two-state FSSH / Ehrenfest trajectory kernels (BABwithTsit5).
+//
+// Restates bab_electronics.jl:48-91 (B-A-B velocity Verlet, acceleration on the
+// active surface or mean field, Tsit5 density propagation with linearly
+// interpolated eigenvalues and v.d) followed by the HoppingCallback of
+// surface_hopping.jl:2-16 (fssh.jl:96-121 probability and selection,
+// surface_hopping.jl:64-99 rescaling).
+//
+// Two schedules:
+//  * fssh2_persistent: one trajectory per lane, all trajectory state in VGPRs,
+//    a grid sized to the device that refills lanes from a global work queue as
+//    trajectories terminate (uneven lengths), the electronic 2x2 Hermitian
+//    density carried as four reals.  Templated on the nuclear precision.
+//  * fssh2_plain_*: the straightforward port -- one launch per nuclear step, the
+//    state round-trips through HBM every step, the electronic step uses the
+//    dense complex 2x2 formulation, FP64 throughout, terminated trajectories are
+//    masked rather than replaced.
+#include "fssh2.h"
+
+namespace nqcd {
+
+// ---------------------------------------------------------------------------
+// Per-lane trajectory state for the persistent kernel.
+// ---------------------------------------------------------------------------
+template <int D, typename R>
+struct Lane2 {
+    R r[D], v[D], k[D];
+    Sig2 s;
+    double U[4];
+    double dEc, wc, tc;  // electronic buffer at the start of the step
+    double t;
+    ModelSlot m;
+    int64_t tid;
+    int32_t st, nh, step;
+};
+
+template <int D, typename R>
+__device__ __forceinline__ void eval_cache(const Fssh2Args& A, Lane2<D, R>& L, bool track, double& E0, double& E1,
+                                           double G00[D], double G11[D], double G01[D]) {
+    static_assert(D == 1, "one-dimensional models");
+    R v11, v22, v12, d11, d22, d12;
+    model1d_eval<R>(L.m, L.r[0], v11, v22, v12, d11, d22, d12);
+    eigen2((double)v11, (double)v12, (double)v22, E0, E1, L.U, track);
+    rotate_derivative(L.U, (double)d11, (double)d22, (double)d12, G00[0], G11[0], G01[0]);
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void set_accel(const Fssh2Args& A, Lane2<D, R>& L, const double G00[D],
+                                          const double G11[D], const double G01[D]) {
+#pragma unroll
+    for (int I = 0; I < D; ++I) {
+        double f;
+        if (A.method == METHOD_FSSH) f = (L.st == 0) ? G00[I] : G11[I];
+        else f = G00[I] * L.s.a + G11[I] * L.s.b + 2.0 * G01[I] * L.s.x;
+        L.k[I] = (R)(-f * A.inv_mass[I]);
+    }
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void save_slot(const Fssh2Args& A, const Lane2<D, R>& L, int slot) {
+    if (slot >= A.nsave) return;
+    const int64_t n = A.ntraj;
+    double pa0, pa1;
+    double S00, S11;
+    if (A.method == METHOD_FSSH) {
+        pa0 = (L.st == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0;
+        S00 = pa0; S11 = pa1;
+    } else {
+        pa0 = L.s.a; pa1 = L.s.b;
+        S00 = L.s.a; S11 = L.s.b;
+    }
+    // diag(U S U^T) with S01 = Re sigma_01 (fssh.jl:132-142, ehrenfest.jl:75-83)
+    const double S01 = L.s.x;
+    const double pd0 = L.U[0] * L.U[0] * S00 + L.U[1] * L.U[1] * S11 + 2.0 * L.U[0] * L.U[1] * S01;
+    const double pd1 = L.U[2] * L.U[2] * S00 + L.U[3] * L.U[3] * S11 + 2.0 * L.U[2] * L.U[3] * S01;
+    if (A.save_pop_ad) {
+        A.save_pop_ad[((int64_t)slot * 2 + 0) * n + L.tid] = pa0;
+        A.save_pop_ad[((int64_t)slot * 2 + 1) * n + L.tid] = pa1;
+    }
+    if (A.save_pop_di) {
+        A.save_pop_di[((int64_t)slot * 2 + 0) * n + L.tid] = pd0;
+        A.save_pop_di[((int64_t)slot * 2 + 1) * n + L.tid] = pd1;
+    }
+    if (A.save_state) A.save_state[(int64_t)slot * n + L.tid] = L.st;
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void load_traj(const Fssh2Args& A, Lane2<D, R>& L, int64_t tid) {
+    const int64_t n = A.ntraj;
+    L.tid = tid;
+#pragma unroll
+    for (int I = 0; I < D; ++I) {
+        L.r[I] = (R)A.r0[(int64_t)I * n + tid];
+        L.v[I] = (R)A.v0[(int64_t)I * n + tid];
+    }
+    L.s.a = A.sig0[0 * n + tid];
+    L.s.b = A.sig0[1 * n + tid];
+    L.s.x = A.sig0[2 * n + tid];
+    L.s.y = A.sig0[3 * n + tid];
+    const int mid = A.model_id ? A.model_id[tid] : 0;
+    L.m = A.models[mid];
+    int st = A.state0 ? A.state0[tid] : -1;
+    if (A.method == METHOD_FSSH && st < 0) {  // fssh.jl:54: sample(Weights(diag(sigma)))
+        const double tot = L.s.a + L.s.b;
+        const double u = philox_uniform((uint64_t)tid, 0u, STREAM_STATE, A.key) * tot;
+        st = (u < L.s.a) ? 0 : 1;
+    }
+    L.st = st;
+    double E0, E1, G00[D], G11[D], G01[D];
+    eval_cache(A, L, false, E0, E1, G00, G11, G01);  // initialize! (bab_electronics.jl:48-59)
+    set_accel(A, L, G00, G11, G01);
+    L.dEc = 0.0; L.wc = 0.0; L.tc = 0.0;  // zero buffers stamped t = 0 (electronic_dynamics.jl:118-127)
+    L.t = A.t0;
+    L.nh = 0;
+    L.step = 0;
+    save_slot(A, L, 0);
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void finish_traj(const Fssh2Args& A, const Lane2<D, R>& L) {
+    const int64_t n = A.ntraj;
+#pragma unroll
+    for (int I = 0; I < D; ++I) {
+        A.r[(int64_t)I * n + L.tid] = (double)L.r[I];
+        A.v[(int64_t)I * n + L.tid] = (double)L.v[I];
+    }
+    A.sig[0 * n + L.tid] = L.s.a;
+    A.sig[1 * n + L.tid] = L.s.b;
+    A.sig[2 * n + L.tid] = L.s.x;
+    A.sig[3 * n + L.tid] = L.s.y;
+    A.state[L.tid] = L.st;
+    A.steps[L.tid] = L.step;
+    A.nhops[L.tid] = L.nh;
+}
+
+template <int D, typename R>
+__device__ __forceinline__ bool is_done(const Fssh2Args& A, const Lane2<D, R>& L) {
+    if (L.step >= A.nsteps) return true;
+    if (A.term_kind == 1 && fabs((double)L.r[A.term_dof]) > A.term_value) return true;
+    return false;
+}
+
+// One nuclear step + hopping callback.
+template <int D, typename R>
+__device__ __forceinline__ void step_traj(const Fssh2Args& A, Lane2<D, R>& L) {
+    const R hdt = (R)(0.5 * A.dt), dtR = (R)A.dt;
+    R vt[D];
+#pragma unroll
+    for (int I = 0; I < D; ++I) vt[I] = L.v[I] + hdt * L.k[I];           // step_B!
+#pragma unroll
+    for (int I = 0; I < D; ++I) L.r[I] = L.r[I] + dtR * vt[I];           // step_A!
+    double E0, E1, G00[D], G11[D], G01[D];
+    eval_cache(A, L, true, E0, E1, G00, G11, G01);                       // update_cache!
+    set_accel(A, L, G00, G11, G01);                                      // sigma of the step start
+#pragma unroll
+    for (int I = 0; I < D; ++I) L.v[I] = vt[I] + hdt * L.k[I];           // step_B!
+    const double gap = E1 - E0;
+    double d01[D];
+    double w = 0.0;
+#pragma unroll
+    for (int I = 0; I < D; ++I) { d01[I] = G01[I] / gap; w += d01[I] * (double)L.v[I]; }
+    const double tn = L.t + A.dt;
+    Interp2 P;
+    P.dEc = L.dEc; P.dEn = E0 - E1; P.wc = L.wc; P.wn = w; P.tc = L.tc; P.inv = 1.0 / (tn - L.tc);
+    propagate2(P, L.s, L.t, A.dt, A.nsub);
+    L.dEc = E0 - E1; L.wc = w; L.tc = tn;
+    L.t = tn;
+    if (A.method == METHOD_FSSH) {
+        const int s = L.st;
+        const double ratio = L.s.x / ((s == 0) ? L.s.a : L.s.b);  // Re(sigma_ms / sigma_ss)
+        double p = 0.0;
+#pragma unroll
+        for (int I = 0; I < D; ++I) {
+            const double dsm = (s == 0) ? d01[I] : -d01[I];
+            p += 2.0 * (double)L.v[I] * ratio * dsm * A.dt;
+        }
+        p = fmin(fmax(p, 0.0), 1.0);
+        const double rnd = philox_uniform((uint64_t)L.tid, (uint32_t)L.step, STREAM_HOP, A.key);
+        if (p > rnd) {
+            double dms[D];
+#pragma unroll
+            for (int I = 0; I < D; ++I) dms[I] = (s == 0) ? -d01[I] : d01[I];  // coupling[new, old]
+            const double dE = (s == 0) ? (E1 - E0) : (E0 - E1);
+            if (rescale_velocity<D, R>(A.rescaling, L.v, A.inv_mass, dms, dE)) { L.st = 1 - s; ++L.nh; }
+        }
+    }
+    ++L.step;
+    if (A.save_every > 0 && (L.step % A.save_every) == 0) save_slot(A, L, L.step / A.save_every);
+}
+
+template <int D, typename R>
+__global__ __launch_bounds__(256) void fssh2_persistent(const Fssh2Args A) {
+    const int lane = __lane_id();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    Lane2<D, R> L;
+    bool active = false, exhausted = false;
+    for (;;) {
+        const uint64_t want = __ballot(!active && !exhausted);
+        if (want) {
+            const int cnt = __popcll(want);
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(A.queue, (unsigned long long)cnt);
+            base = __shfl(base, leader);
+            if (!active && !exhausted) {
+                const unsigned long long q = base + (unsigned long long)__popcll(want & below);
+                if (q < (unsigned long long)A.ntraj) {
+                    load_traj(A, L, A.order ? (int64_t)A.order[q] : (int64_t)q);
+                    if (is_done(A, L)) finish_traj(A, L); else active = true;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            step_traj(A, L);
+            if (is_done(A, L)) { finish_traj(A, L); active = false; }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain schedule: HBM-resident state, one launch per nuclear step.
+// State rows (double): r, v, k, sigma re[4], im[4], E0c, E1c, Wc, tc, U[4], t.
+// Int rows: state, nhops, step, done.
+// ---------------------------------------------------------------------------
+enum { PL_R = 0, PL_V = 1, PL_K = 2, PL_SRE = 3, PL_SIM = 7, PL_E0C = 11, PL_E1C = 12, PL_WC = 13, PL_TC = 14,
+       PL_U = 15, PL_T = 19, PL_NROWS = 20 };
+enum { PI_ST = 0, PI_NH = 1, PI_STEP = 2, PI_DONE = 3, PI_NROWS = 4 };
+
+__device__ __forceinline__ void plain_save(const Fssh2Args& A, int64_t tid, int slot, int st, const C2& s,
+                                           const double U[4]) {
+    if (slot >= A.nsave) return;
+    const int64_t n = A.ntraj;
+    double S00, S11, pa0, pa1;
+    if (A.method == METHOD_FSSH) { pa0 = (st == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0; S00 = pa0; S11 = pa1; }
+    else { pa0 = s.re[0]; pa1 = s.re[3]; S00 = pa0; S11 = pa1; }
+    const double S01 = s.re[1];
+    const double pd0 = U[0] * U[0] * S00 + U[1] * U[1] * S11 + 2.0 * U[0] * U[1] * S01;
+    const double pd1 = U[2] * U[2] * S00 + U[3] * U[3] * S11 + 2.0 * U[2] * U[3] * S01;
+    if (A.save_pop_ad) { A.save_pop_ad[((int64_t)slot * 2) * n + tid] = pa0; A.save_pop_ad[((int64_t)slot * 2 + 1) * n + tid] = pa1; }
+    if (A.save_pop_di) { A.save_pop_di[((int64_t)slot * 2) * n + tid] = pd0; A.save_pop_di[((int64_t)slot * 2 + 1) * n + tid] = pd1; }
+    if (A.save_state) A.save_state[(int64_t)slot * n + tid] = st;
+}
+
+__global__ __launch_bounds__(256) void fssh2_plain_init(const Fssh2Args A, double* S, int32_t* Si) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= A.ntraj) return;
+    const int64_t n = A.ntraj;
+    const double r = A.r0[tid], v = A.v0[tid];
+    C2 s;
+    s.re[0] = A.sig0[tid]; s.re[3] = A.sig0[n + tid]; s.re[1] = s.re[2] = A.sig0[2 * n + tid];
+    s.im[0] = s.im[3] = 0.0; s.im[1] = A.sig0[3 * n + tid]; s.im[2] = -A.sig0[3 * n + tid];
+    const int mid = A.model_id ? A.model_id[tid] : 0;
+    int st = A.state0 ? A.state0[tid] : -1;
+    if (A.method == METHOD_FSSH && st < 0) {
+        const double tot = s.re[0] + s.re[3];
+        st = (philox_uniform((uint64_t)tid, 0u, STREAM_STATE, A.key) * tot < s.re[0]) ? 0 : 1;
+    }
+    double v11, v22, v12, d11, d22, d12, E0, E1, U[4], G00, G11, G01;
+    model1d_eval<double>(A.models[mid], r, v11, v22, v12, d11, d22, d12);
+    eigen2(v11, v12, v22, E0, E1, U, false);
+    rotate_derivative(U, d11, d22, d12, G00, G11, G01);
+    const double f = (A.method == METHOD_FSSH) ? ((st == 0) ? G00 : G11) : (G00 * s.re[0] + G11 * s.re[3] + 2.0 * G01 * s.re[1]);
+    S[PL_R * n + tid] = r; S[PL_V * n + tid] = v; S[PL_K * n + tid] = -f * A.inv_mass[0];
+    for (int J = 0; J < 4; ++J) { S[(PL_SRE + J) * n + tid] = s.re[J]; S[(PL_SIM + J) * n + tid] = s.im[J]; }
+    S[PL_E0C * n + tid] = 0.0; S[PL_E1C * n + tid] = 0.0; S[PL_WC * n + tid] = 0.0; S[PL_TC * n + tid] = 0.0;
+    for (int J = 0; J < 4; ++J) S[(PL_U + J) * n + tid] = U[J];
+    S[PL_T * n + tid] = A.t0;
+    Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = 0; Si[PI_STEP * n + tid] = 0;
+    const bool done = (A.nsteps <= 0) || (A.term_kind == 1 && fabs(r) > A.term_value);
+    Si[PI_DONE * n + tid] = done ? 1 : 0;
+    plain_save(A, tid, 0, st, s, U);
+}
+
+__global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, double* S, int32_t* Si) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= A.ntraj) return;
+    const int64_t n = A.ntraj;
+    if (Si[PI_DONE * n + tid]) return;
+    const int mid = A.model_id ? A.model_id[tid] : 0;
+    const ModelSlot& M = A.models[mid];
+    double r = S[PL_R * n + tid], v = S[PL_V * n + tid], k = S[PL_K * n + tid];
+    C2 s;
+    for (int J = 0; J < 4; ++J) { s.re[J] = S[(PL_SRE + J) * n + tid]; s.im[J] = S[(PL_SIM + J) * n + tid]; }
+    double U[4];
+    for (int J = 0; J < 4; ++J) U[J] = S[(PL_U + J) * n + tid];
+    const double t = S[PL_T * n + tid];
+    int st = Si[PI_ST * n + tid], nh = Si[PI_NH * n + tid], step = Si[PI_STEP * n + tid];
+    const double dt = A.dt;
+    const double vt = v + 0.5 * dt * k;
+    r = r + dt * vt;
+    double v11, v22, v12, d11, d22, d12, E0, E1, G00, G11, G01;
+    model1d_eval<double>(M, r, v11, v22, v12, d11, d22, d12);
+    eigen2(v11, v12, v22, E0, E1, U, true);
+    rotate_derivative(U, d11, d22, d12, G00, G11, G01);
+    const double f = (A.method == METHOD_FSSH) ? ((st == 0) ? G00 : G11) : (G00 * s.re[0] + G11 * s.re[3] + 2.0 * G01 * s.re[1]);
+    k = -f * A.inv_mass[0];
+    v = vt + 0.5 * dt * k;
+    const double d01 = G01 / (E1 - E0);
+    const double w = d01 * v;
+    const double tn = t + dt;
+    InterpDense P;
+    P.E0c = S[PL_E0C * n + tid]; P.E1c = S[PL_E1C * n + tid]; P.Wc = S[PL_WC * n + tid]; P.tc = S[PL_TC * n + tid];
+    P.E0n = E0; P.E1n = E1; P.Wn = w; P.inv = 1.0 / (tn - P.tc);
+    propagate_dense(P, s, t, dt, A.nsub);
+    if (A.method == METHOD_FSSH) {
+        const int m = 1 - st;
+        // real(sigma[m,s] / sigma[s,s]) with full complex division
+        const double nr = s.re[m * 2 + st], ni = s.im[m * 2 + st];
+        const double dr = s.re[st * 3], di = s.im[st * 3];
+        const double ratio = (nr * dr + ni * di) / (dr * dr + di * di);
+        const double dsm = (st == 0) ? d01 : -d01;
+        double p = 2.0 * v * ratio * dsm * dt;
+        p = fmin(fmax(p, 0.0), 1.0);
+        const double rnd = philox_uniform((uint64_t)tid, (uint32_t)step, STREAM_HOP, A.key);
+        if (p > rnd) {
+            double vv[1] = {v};
+            const double dms[1] = {(st == 0) ? -d01 : d01};
+            const double dE = (st == 0) ? (E1 - E0) : (E0 - E1);
+            if (rescale_velocity<1, double>(A.rescaling, vv, A.inv_mass, dms, dE)) { st = m; ++nh; }
+            v = vv[0];
+        }
+    }
+    ++step;
+    S[PL_R * n + tid] = r; S[PL_V * n + tid] = v; S[PL_K * n + tid] = k;
+    for (int J = 0; J < 4; ++J) { S[(PL_SRE + J) * n + tid] = s.re[J]; S[(PL_SIM + J) * n + tid] = s.im[J]; }
+    S[PL_E0C * n + tid] = E0; S[PL_E1C * n + tid] = E1; S[PL_WC * n + tid] = w; S[PL_TC * n + tid] = tn;
+    for (int J = 0; J < 4; ++J) S[(PL_U + J) * n + tid] = U[J];
+    S[PL_T * n + tid] = tn;
+    Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = nh; Si[PI_STEP * n + tid] = step;
+    if (A.save_every > 0 && step % A.save_every == 0) plain_save(A, tid, step / A.save_every, st, s, U);
+    const bool done = (step >= A.nsteps) || (A.term_kind == 1 && fabs(r) > A.term_value);
+    if (done) Si[PI_DONE * n + tid] = 1;
+}
+
+__global__ __launch_bounds__(256) void fssh2_plain_finish(const Fssh2Args A, const double* S, const int32_t* Si) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= A.ntraj) return;
+    const int64_t n = A.ntraj;
+    A.r[tid] = S[PL_R * n + tid];
+    A.v[tid] = S[PL_V * n + tid];
+    A.sig[tid] = S[PL_SRE * n + tid];
+    A.sig[n + tid] = S[(PL_SRE + 3) * n + tid];
+    A.sig[2 * n + tid] = S[(PL_SRE + 1) * n + tid];
+    A.sig[3 * n + tid] = S[(PL_SIM + 1) * n + tid];
+    A.state[tid] = Si[PI_ST * n + tid];
+    A.nhops[tid] = Si[PI_NH * n + tid];
+    A.steps[tid] = Si[PI_STEP * n + tid];
+}
+
+__global__ void queue_reset(unsigned long long* q) { *q = 0ull; }
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+template <int D, typename R>
+static hipError_t launch_persistent_t(const Fssh2Args& A, int grid, int block, hipStream_t stream) {
+    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(1), 0, stream, A.queue);
+    hipLaunchKernelGGL((fssh2_persistent<D, R>), dim3(grid), dim3(block), 0, stream, A);
+    return hipGetLastError();
+}
+
+hipError_t fssh2_launch_persistent(const Fssh2Args& A, int precision, int grid, int block, hipStream_t stream) {
+    if (precision == 0) return launch_persistent_t<1, double>(A, grid, block, stream);
+    return launch_persistent_t<1, float>(A, grid, block, stream);
+}
+
+hipError_t fssh2_persistent_occupancy(int precision, int block, int* blocks_per_cu) {
+    if (precision == 0)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fssh2_persistent<1, double>, block, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fssh2_persistent<1, float>, block, 0);
+}
+
+hipError_t fssh2_launch_plain_init(const Fssh2Args& A, double* S, int32_t* Si, hipStream_t stream) {
+    const int block = 256;
+    const int grid = (int)((A.ntraj + block - 1) / block);
+    hipLaunchKernelGGL(fssh2_plain_init, dim3(grid), dim3(block), 0, stream, A, S, Si);
+    return hipGetLastError();
+}
+
+hipError_t fssh2_launch_plain_step(const Fssh2Args& A, double* S, int32_t* Si, hipStream_t stream) {
+    const int block = 256;
+    const int grid = (int)((A.ntraj + block - 1) / block);
+    hipLaunchKernelGGL(fssh2_plain_step, dim3(grid), dim3(block), 0, stream, A, S, Si);
+    return hipGetLastError();
+}
+
+hipError_t fssh2_launch_plain_finish(const Fssh2Args& A, double* S, int32_t* Si, hipStream_t stream) {
+    const int block = 256;
+    const int grid = (int)((A.ntraj + block - 1) / block);
+    hipLaunchKernelGGL(fssh2_plain_finish, dim3(grid), dim3(block), 0, stream, A, S, Si);
+    return hipGetLastError();
+}
+
+}  // namespace nqcd

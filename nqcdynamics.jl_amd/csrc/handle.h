@@ -1,0 +1,63 @@
+// handle.h -- internal definition of the EnsembleMI355X handle (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/nqcd_mi355x.h"
+#include "fssh2.h"
+
+namespace nqcd {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (n == 0) n = 8;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+int32_t fail(nqcd_handle_t* h, int32_t code, const std::string& msg);
+
+}  // namespace nqcd
+
+#define HIPCHK(h, expr)                                                                                    \
+    do {                                                                                                   \
+        hipError_t _e = (expr);                                                                            \
+        if (_e != hipSuccess)                                                                              \
+            return ::nqcd::fail((h), NQCD_ERR_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+struct nqcd_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    bool have_problem = false;
+    nqcd_problem_desc prob{};
+    std::vector<double> masses;    // natoms
+    std::vector<double> inv_mass;  // per flattened dof
+    int D = 0;
+    nqcd::ModelSlot models[nqcd::kMaxModels]{};
+    bool model_set[nqcd::kMaxModels]{};
+    // spin-boson bath (slot 0 only)
+    std::vector<double> sb_omega, sb_c;
+    double sb_eps = 0.0, sb_delta = 0.0;
+    int64_t ntraj = 0;
+    bool have_initial = false, have_results = false, sb_initial = false;
+    int32_t nsave = 1, save_every = 0;
+    nqcd::DevBuf r0, v0, sig0, state0, model_id, order, models_dev, bath;
+    nqcd::DevBuf r, v, sig, state, steps, nhops, save_ad, save_di, save_state;
+    nqcd::DevBuf queue, plainS, plainSi, tmp, partial, small;
+    bool state0_given = false, model_id_given = false, order_given = false;
+    nqcd_run_stats stats{};
+    int cu_count = 0;
+};

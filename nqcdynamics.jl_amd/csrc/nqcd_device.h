@@ -1,0 +1,385 @@
+// nqcd_device.h -- device building blocks shared by the trajectory kernels.
+//
+// Everything here runs per lane (one trajectory per lane) in registers:
+// counter-based Philox draws, the closed-form model Hamiltonians, the 2x2
+// symmetric eigen-decomposition with eigenvector tracking, the nonadiabatic
+// coupling, the Tsit5 density-matrix propagation and the fewest-switches hop.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nqcd {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Counter (traj_lo, traj_hi, step, stream),
+// key (key_lo, key_hi).  Replaces the global rand() of fssh.jl:112.
+// ---------------------------------------------------------------------------
+enum { STREAM_INIT = 0, STREAM_HOP = 1, STREAM_STATE = 2 };
+
+__device__ __forceinline__ void philox4x32_10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                                              uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+    const uint64_t x = ((((uint64_t)a) << 32) | (uint64_t)b) >> 11;
+    return (double)x * 0x1.0p-53;
+}
+
+__device__ __forceinline__ double philox_uniform(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key) {
+    uint32_t c0 = (uint32_t)traj, c1 = (uint32_t)(traj >> 32), c2 = step, c3 = stream;
+    philox4x32_10(c0, c1, c2, c3, (uint32_t)key, (uint32_t)(key >> 32));
+    return u53(c0, c1);
+}
+
+__device__ __forceinline__ void philox_normal2(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key,
+                                               double& z0, double& z1) {
+    uint32_t c0 = (uint32_t)traj, c1 = (uint32_t)(traj >> 32), c2 = step, c3 = stream;
+    philox4x32_10(c0, c1, c2, c3, (uint32_t)key, (uint32_t)(key >> 32));
+    const double u1 = u53(c0, c1), u2 = u53(c2, c3);
+    const double rad = sqrt(-2.0 * log(1.0 - u1));
+    double s, c;
+    sincos(6.283185307179586 * u2, &s, &c);
+    z0 = rad * c;
+    z1 = rad * s;
+}
+
+// ---------------------------------------------------------------------------
+// Transcendentals by precision: FP64 path uses ocml's correctly-rounded-ish
+// double routines; the FP32 path uses the hardware exp.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+__device__ __forceinline__ float fexp(float x) { return __expf(x); }
+__device__ __forceinline__ double fabs_(double x) { return fabs(x); }
+__device__ __forceinline__ float fabs_(float x) { return fabsf(x); }
+
+// ---------------------------------------------------------------------------
+// One-DOF two-state models (NQCModels Tully 1/2/3, double well + Gaussian
+// coupling).  Outputs V = [[v11, v12],[v12, v22]] and dV/dx.
+// ---------------------------------------------------------------------------
+enum { MODEL_TULLY1 = 0, MODEL_TULLY2 = 1, MODEL_TULLY3 = 2, MODEL_SPIN_BOSON = 3, MODEL_DW_GAUSSIAN = 4,
+       MODEL_HARMONIC = 5 };
+
+struct ModelSlot {
+    int32_t kind;
+    int32_t pad;
+    double p[8];
+};
+
+template <typename R>
+__device__ __forceinline__ void model1d_eval(const ModelSlot& m, R x, R& v11, R& v22, R& v12,
+                                             R& d11, R& d22, R& d12) {
+    switch (m.kind) {
+    case MODEL_TULLY1: {
+        const R A = (R)m.p[0], B = (R)m.p[1], C = (R)m.p[2], D = (R)m.p[3];
+        const R e = fexp(-B * fabs_(x));
+        const R sgn = (x > (R)0) ? (R)1 : ((x < (R)0) ? (R)-1 : (R)0);
+        v11 = A * ((R)1 - e) * sgn;
+        v22 = -v11;
+        d11 = A * B * e;
+        d22 = -d11;
+        v12 = C * fexp(-D * x * x);
+        d12 = (R)-2 * D * x * v12;
+        break;
+    }
+    case MODEL_TULLY2: {
+        const R A = (R)m.p[0], B = (R)m.p[1], C = (R)m.p[2], D = (R)m.p[3], E0 = (R)m.p[4];
+        const R g = A * fexp(-B * x * x);
+        v11 = (R)0; d11 = (R)0;
+        v22 = -g + E0;
+        d22 = (R)2 * B * x * g;
+        v12 = C * fexp(-D * x * x);
+        d12 = (R)-2 * D * x * v12;
+        break;
+    }
+    case MODEL_TULLY3: {
+        const R A = (R)m.p[0], B = (R)m.p[1], C = (R)m.p[2];
+        v11 = A; v22 = -A; d11 = (R)0; d22 = (R)0;
+        if (x < (R)0) { const R e = fexp(C * x); v12 = B * e; d12 = B * C * e; }
+        else { const R e = fexp(-C * x); v12 = B * ((R)2 - e); d12 = B * C * e; }
+        break;
+    }
+    case MODEL_DW_GAUSSIAN: {
+        const R a = (R)m.p[0], b = (R)m.p[1], off = (R)m.p[2], C = (R)m.p[3], D = (R)m.p[4];
+        const R x2 = x * x;
+        v11 = a * x2 * x2 - b * x2;
+        d11 = (R)4 * a * x2 * x - (R)2 * b * x;
+        v22 = v11 + off; d22 = d11;
+        v12 = C * fexp(-D * x2);
+        d12 = (R)-2 * D * x * v12;
+        break;
+    }
+    default:
+        v11 = v22 = v12 = d11 = d22 = d12 = (R)0;
+        break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2x2 real symmetric eigen-decomposition, eigenvalues ascending.  U is stored
+// row-major (U[0]=U00, U[1]=U01, U[2]=U10, U[3]=U11); column j is the eigenvector
+// of E_j.  A fresh decomposition makes the largest-magnitude component of each
+// column positive; with a previous U, a column whose overlap with the previous
+// one is negative is flipped (eigenvector tracking).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void eigen2(double a, double c, double d, double& E0, double& E1, double U[4],
+                                       bool track) {
+    const double mean = 0.5 * (a + d), h = 0.5 * (a - d);
+    const double rad = sqrt(h * h + c * c);
+    E0 = mean - rad;
+    E1 = mean + rad;
+    double p, q;  // eigenvector of E1
+    if (h >= 0.0) { p = h + rad; q = c; } else { p = c; q = rad - h; }
+    const double nrm2 = p * p + q * q;
+    if (nrm2 > 0.0) { const double inv = 1.0 / sqrt(nrm2); p *= inv; q *= inv; } else { p = 1.0; q = 0.0; }
+    // column 1 = (p, q), column 0 = (-q, p)
+    double u00 = -q, u10 = p, u01 = p, u11 = q;
+    if (track) {
+        if (u00 * U[0] + u10 * U[2] < 0.0) { u00 = -u00; u10 = -u10; }
+        if (u01 * U[1] + u11 * U[3] < 0.0) { u01 = -u01; u11 = -u11; }
+    } else {
+        const double m0 = (fabs(u10) > fabs(u00)) ? u10 : u00;
+        if (m0 < 0.0) { u00 = -u00; u10 = -u10; }
+        const double m1 = (fabs(u11) > fabs(u01)) ? u11 : u01;
+        if (m1 < 0.0) { u01 = -u01; u11 = -u11; }
+    }
+    U[0] = u00; U[1] = u01; U[2] = u10; U[3] = u11;
+}
+
+// G = U^T dV U for dV = [[d11, d12],[d12, d22]]; returns G00, G11, G01.
+__device__ __forceinline__ void rotate_derivative(const double U[4], double d11, double d22, double d12,
+                                                  double& G00, double& G11, double& G01) {
+    const double u00 = U[0], u01 = U[1], u10 = U[2], u11 = U[3];
+    G00 = u00 * u00 * d11 + 2.0 * u00 * u10 * d12 + u10 * u10 * d22;
+    G11 = u01 * u01 * d11 + 2.0 * u01 * u11 * d12 + u11 * u11 * d22;
+    G01 = u00 * u01 * d11 + (u00 * u11 + u10 * u01) * d12 + u10 * u11 * d22;
+}
+
+// ---------------------------------------------------------------------------
+// Tsit5 (Tsitouras 2011) coefficients.
+// ---------------------------------------------------------------------------
+namespace ts5 {
+constexpr double c1 = 0.161, c2 = 0.327, c3 = 0.9, c4 = 0.9800255409045097;
+constexpr double a21 = 0.161;
+constexpr double a31 = -0.008480655492356989, a32 = 0.335480655492357;
+constexpr double a41 = 2.897153057105493, a42 = -6.359448489975075, a43 = 4.3622954328695815;
+constexpr double a51 = 5.325864828439257, a52 = -11.748883564062828, a53 = 7.4955393428898365,
+                 a54 = -0.09249506636175525;
+constexpr double a61 = 5.86145544294642, a62 = -12.92096931784711, a63 = 8.159367898576159,
+                 a64 = -0.071584973281401, a65 = -0.028269050394068383;
+constexpr double a71 = 0.09646076681806523, a72 = 0.01, a73 = 0.4798896504144996, a74 = 1.379008574103742,
+                 a75 = -3.290069515436081, a76 = 2.324710524099774;
+}  // namespace ts5
+
+// Linear interpolation of the electronic parameters between the two ends of the
+// nuclear step (DoubleBuffer, electronic_dynamics.jl:15-79): dE = E0 - E1 and
+// w = (v.d)_{01}.
+struct Interp2 {
+    double dEc, dEn, wc, wn, tc, inv;
+};
+
+// Hermitian 2x2 density sigma = [[a, x+iy],[x-iy, b]] under V = [[E0, -iw],[iw, E1]]:
+// d sigma/dt = -i[V, sigma] written out for the four real unknowns.
+struct Sig2 {
+    double a, b, x, y;
+};
+
+__device__ __forceinline__ Sig2 rhs2(const Interp2& P, double tau, const Sig2& s) {
+    const double loc = (tau - P.tc) * P.inv;
+    const double dE = P.dEc + (P.dEn - P.dEc) * loc;
+    const double w = P.wc + (P.wn - P.wc) * loc;
+    const double wx2 = 2.0 * w * s.x;
+    Sig2 d;
+    d.a = -wx2;
+    d.b = wx2;
+    d.x = dE * s.y + w * (s.a - s.b);
+    d.y = -dE * s.x;
+    return d;
+}
+
+#define NQCD_S2_COMB(out, base, h, EXPR_A, EXPR_B, EXPR_X, EXPR_Y) \
+    out.a = base.a + (h) * (EXPR_A);                                 \
+    out.b = base.b + (h) * (EXPR_B);                                 \
+    out.x = base.x + (h) * (EXPR_X);                                 \
+    out.y = base.y + (h) * (EXPR_Y);
+
+__device__ __forceinline__ void tsit5_step2(const Interp2& P, Sig2& u, double tau, double h) {
+    using namespace ts5;
+    Sig2 t;
+    const Sig2 k1 = rhs2(P, tau, u);
+    NQCD_S2_COMB(t, u, h, a21 * k1.a, a21 * k1.b, a21 * k1.x, a21 * k1.y);
+    const Sig2 k2 = rhs2(P, tau + c1 * h, t);
+    NQCD_S2_COMB(t, u, h, a31 * k1.a + a32 * k2.a, a31 * k1.b + a32 * k2.b, a31 * k1.x + a32 * k2.x,
+                 a31 * k1.y + a32 * k2.y);
+    const Sig2 k3 = rhs2(P, tau + c2 * h, t);
+    NQCD_S2_COMB(t, u, h, a41 * k1.a + a42 * k2.a + a43 * k3.a, a41 * k1.b + a42 * k2.b + a43 * k3.b,
+                 a41 * k1.x + a42 * k2.x + a43 * k3.x, a41 * k1.y + a42 * k2.y + a43 * k3.y);
+    const Sig2 k4 = rhs2(P, tau + c3 * h, t);
+    NQCD_S2_COMB(t, u, h, a51 * k1.a + a52 * k2.a + a53 * k3.a + a54 * k4.a,
+                 a51 * k1.b + a52 * k2.b + a53 * k3.b + a54 * k4.b,
+                 a51 * k1.x + a52 * k2.x + a53 * k3.x + a54 * k4.x,
+                 a51 * k1.y + a52 * k2.y + a53 * k3.y + a54 * k4.y);
+    const Sig2 k5 = rhs2(P, tau + c4 * h, t);
+    NQCD_S2_COMB(t, u, h, a61 * k1.a + a62 * k2.a + a63 * k3.a + a64 * k4.a + a65 * k5.a,
+                 a61 * k1.b + a62 * k2.b + a63 * k3.b + a64 * k4.b + a65 * k5.b,
+                 a61 * k1.x + a62 * k2.x + a63 * k3.x + a64 * k4.x + a65 * k5.x,
+                 a61 * k1.y + a62 * k2.y + a63 * k3.y + a64 * k4.y + a65 * k5.y);
+    const Sig2 k6 = rhs2(P, tau + h, t);
+    NQCD_S2_COMB(u, u, h, a71 * k1.a + a72 * k2.a + a73 * k3.a + a74 * k4.a + a75 * k5.a + a76 * k6.a,
+                 a71 * k1.b + a72 * k2.b + a73 * k3.b + a74 * k4.b + a75 * k5.b + a76 * k6.b,
+                 a71 * k1.x + a72 * k2.x + a73 * k3.x + a74 * k4.x + a75 * k5.x + a76 * k6.x,
+                 a71 * k1.y + a72 * k2.y + a73 * k3.y + a74 * k4.y + a75 * k5.y + a76 * k6.y);
+}
+
+// step!(electronic_integrator, dt, true) from time t: nsub sub-steps of dt/nsub,
+// each clipped so the last lands on t + dt (bab_electronics.jl:40-43,87-90).
+__device__ __forceinline__ void propagate2(const Interp2& P, Sig2& u, double t, double dt, int nsub) {
+    const double h = dt / (double)nsub;
+    const double tend = t + dt;
+    double tau = t;
+    for (int s = 0; s < nsub; ++s) {
+        const double hs = fmin(h, tend - tau);
+        tsit5_step2(P, u, tau, hs);
+        tau = (s == nsub - 1) ? tend : tau + hs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense complex 2x2 formulation (the reference's literal data flow: A = -iW,
+// diag(A) = E, du = -i(A u - u A) on all eight real components).  Used by the
+// step-per-launch plain kernel.
+// ---------------------------------------------------------------------------
+struct C2 {
+    double re[4], im[4];
+};
+
+struct InterpDense {
+    double E0c, E1c, E0n, E1n, Wc, Wn, tc, inv;  // W = (v.d)_{01}; W_{10} = -W
+};
+
+__device__ __forceinline__ void rhs_dense(const InterpDense& P, double tau, const C2& u, C2& du) {
+    const double loc = (tau - P.tc) * P.inv;
+    const double E0 = P.E0c + (P.E0n - P.E0c) * loc;
+    const double E1 = P.E1c + (P.E1n - P.E1c) * loc;
+    const double w = P.Wc + (P.Wn - P.Wc) * loc;
+    // A = [[E0, -i w],[ i w, E1]] as complex entries
+    double Are[4] = {E0, 0.0, 0.0, E1};
+    double Aim[4] = {0.0, -w, w, 0.0};
+    double cre[4], cim[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                // A[i,k] * u[k,j] - u[i,k] * A[k,j]
+                sr += Are[i * 2 + k] * u.re[k * 2 + j] - Aim[i * 2 + k] * u.im[k * 2 + j];
+                si += Are[i * 2 + k] * u.im[k * 2 + j] + Aim[i * 2 + k] * u.re[k * 2 + j];
+                sr -= u.re[i * 2 + k] * Are[k * 2 + j] - u.im[i * 2 + k] * Aim[k * 2 + j];
+                si -= u.re[i * 2 + k] * Aim[k * 2 + j] + u.im[i * 2 + k] * Are[k * 2 + j];
+            }
+            cre[i * 2 + j] = sr;
+            cim[i * 2 + j] = si;
+        }
+#pragma unroll
+    for (int J = 0; J < 4; ++J) { du.re[J] = cim[J]; du.im[J] = -cre[J]; }  // * (-i)
+}
+
+__device__ __forceinline__ void tsit5_step_dense(const InterpDense& P, C2& u, double tau, double h) {
+    using namespace ts5;
+    C2 k1, k2, k3, k4, k5, k6, t;
+    rhs_dense(P, tau, u, k1);
+#pragma unroll
+    for (int J = 0; J < 4; ++J) { t.re[J] = u.re[J] + h * (a21 * k1.re[J]); t.im[J] = u.im[J] + h * (a21 * k1.im[J]); }
+    rhs_dense(P, tau + c1 * h, t, k2);
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        t.re[J] = u.re[J] + h * (a31 * k1.re[J] + a32 * k2.re[J]);
+        t.im[J] = u.im[J] + h * (a31 * k1.im[J] + a32 * k2.im[J]);
+    }
+    rhs_dense(P, tau + c2 * h, t, k3);
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        t.re[J] = u.re[J] + h * (a41 * k1.re[J] + a42 * k2.re[J] + a43 * k3.re[J]);
+        t.im[J] = u.im[J] + h * (a41 * k1.im[J] + a42 * k2.im[J] + a43 * k3.im[J]);
+    }
+    rhs_dense(P, tau + c3 * h, t, k4);
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        t.re[J] = u.re[J] + h * (a51 * k1.re[J] + a52 * k2.re[J] + a53 * k3.re[J] + a54 * k4.re[J]);
+        t.im[J] = u.im[J] + h * (a51 * k1.im[J] + a52 * k2.im[J] + a53 * k3.im[J] + a54 * k4.im[J]);
+    }
+    rhs_dense(P, tau + c4 * h, t, k5);
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        t.re[J] = u.re[J] + h * (a61 * k1.re[J] + a62 * k2.re[J] + a63 * k3.re[J] + a64 * k4.re[J] + a65 * k5.re[J]);
+        t.im[J] = u.im[J] + h * (a61 * k1.im[J] + a62 * k2.im[J] + a63 * k3.im[J] + a64 * k4.im[J] + a65 * k5.im[J]);
+    }
+    rhs_dense(P, tau + h, t, k6);
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        u.re[J] = u.re[J] + h * (a71 * k1.re[J] + a72 * k2.re[J] + a73 * k3.re[J] + a74 * k4.re[J] +
+                                 a75 * k5.re[J] + a76 * k6.re[J]);
+        u.im[J] = u.im[J] + h * (a71 * k1.im[J] + a72 * k2.im[J] + a73 * k3.im[J] + a74 * k4.im[J] +
+                                 a75 * k5.im[J] + a76 * k6.im[J]);
+    }
+}
+
+__device__ __forceinline__ void propagate_dense(const InterpDense& P, C2& u, double t, double dt, int nsub) {
+    const double h = dt / (double)nsub;
+    const double tend = t + dt;
+    double tau = t;
+    for (int s = 0; s < nsub; ++s) {
+        const double hs = fmin(h, tend - tau);
+        tsit5_step_dense(P, u, tau, hs);
+        tau = (s == nsub - 1) ? tend : tau + hs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity rescaling along d (surface_hopping.jl:64-99), D flattened DOFs.
+// Returns true when the hop is accepted.
+// ---------------------------------------------------------------------------
+template <int D, typename R>
+__device__ __forceinline__ bool rescale_velocity(int rescaling, R v[D], const double inv_mass[D],
+                                                 const double dcoup[D], double dEnergy) {
+    if (rescaling == 2) return true;  // :off
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int I = 0; I < D; ++I) { a += dcoup[I] * dcoup[I] * inv_mass[I]; b += dcoup[I] * (double)v[I]; }
+    a *= 0.5;
+    const double disc = b * b - 4.0 * a * dEnergy;
+    if (disc < 0.0) {
+        if (rescaling == 1) {  // frustrated_hop_invert_velocity! (:155-164)
+            double nrm = 0.0;
+#pragma unroll
+            for (int I = 0; I < D; ++I) nrm += dcoup[I] * dcoup[I];
+            nrm = sqrt(nrm);
+            double g = 0.0;
+#pragma unroll
+            for (int I = 0; I < D; ++I) g += (double)v[I] * (dcoup[I] / nrm);
+#pragma unroll
+            for (int I = 0; I < D; ++I) v[I] = (R)((double)v[I] - 2.0 * g * (dcoup[I] / nrm));
+        }
+        return false;
+    }
+    const double root = sqrt(disc);
+    const double gamma = (b < 0.0) ? (b + root) / (2.0 * a) : (b - root) / (2.0 * a);
+#pragma unroll
+    for (int I = 0; I < D; ++I) v[I] = (R)((double)v[I] - gamma * dcoup[I] * inv_mass[I]);
+    return true;
+}
+
+// Wave-level helpers (wave64).
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+}  // namespace nqcd

@@ -1,0 +1,219 @@
+"""ctypes binding of libnqcd_mi355x.so (the C ABI in include/nqcd_mi355x.h).
+
+This is the product path: there is no CPU fallback.  Loading fails loudly when
+the shared library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("NQCD_MI355X_LIB", os.path.join(_PKG_ROOT, "libnqcd_mi355x.so"))
+
+ABI_VERSION = 1
+
+# enum nqcd_status
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_DEVICE, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
+# enum nqcd_method
+METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST = range(7)
+# enum nqcd_algorithm
+ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5 = range(4)
+RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = range(3)
+PREC_FP64, PREC_FP32_NUCLEI = range(2)
+SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
+MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC = range(6)
+TERM_NONE, TERM_ABS_GT = range(2)
+SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER = range(3)
+OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION, OUT_FINAL_ADIABATIC_POPULATION = range(5)
+RED_MEAN, RED_SUM = range(2)
+(FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
+ FIELD_SAVE_POP_DI, FIELD_SAVE_STATE) = range(10)
+
+EXPORTED_SYMBOLS = (
+    "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
+    "nqcd_mi355x_set_model", "nqcd_mi355x_load_initial", "nqcd_mi355x_sample_initial", "nqcd_mi355x_run",
+    "nqcd_mi355x_reduce", "nqcd_mi355x_download", "nqcd_mi355x_last_run_stats", "nqcd_mi355x_last_error",
+)
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [("method", C.c_int32), ("algorithm", C.c_int32), ("n_states", C.c_int32), ("ndofs", C.c_int32),
+                ("natoms", C.c_int32), ("nbeads", C.c_int32), ("n_electrons", C.c_int32), ("rescaling", C.c_int32),
+                ("precision", C.c_int32), ("schedule", C.c_int32), ("electronic_substeps", C.c_int32),
+                ("reserved0", C.c_int32), ("masses", C.POINTER(C.c_double)), ("dt", C.c_double),
+                ("temperature", C.c_double), ("reserved", C.c_int32 * 8)]
+
+
+class TermDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dof", C.c_int32), ("value", C.c_double)]
+
+
+class SamplerDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("initial_state", C.c_int32), ("state_basis", C.c_int32), ("n_models", C.c_int32),
+                ("ntraj", C.c_int64), ("params", C.POINTER(C.c_double)), ("nparams", C.c_int64)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("traj_steps", C.c_int64), ("launches", C.c_int64),
+                ("block", C.c_int32), ("grid", C.c_int32)]
+
+
+class NQCDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[nqcd_mi355x status {code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libnqcd_mi355x.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libnqcd_mi355x.so not found at {LIB_PATH}: build it with "
+                          f"`python nqcdynamics.jl_amd/build.py` (the MI355X backend has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    H = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    L.nqcd_mi355x_abi_version.restype = C.c_int32
+    L.nqcd_mi355x_open.argtypes = [C.c_int32, C.POINTER(H)]
+    L.nqcd_mi355x_close.argtypes = [H]
+    L.nqcd_mi355x_close.restype = None
+    L.nqcd_mi355x_set_problem.argtypes = [H, C.POINTER(ProblemDesc)]
+    L.nqcd_mi355x_set_model.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
+    L.nqcd_mi355x_load_initial.argtypes = [H, C.c_int64, dp, dp, dp, dp, ip, ip]
+    L.nqcd_mi355x_sample_initial.argtypes = [H, C.POINTER(SamplerDesc), C.c_uint64]
+    L.nqcd_mi355x_run.argtypes = [H, C.c_double, C.c_double, C.c_int32, C.POINTER(TermDesc), C.c_uint64]
+    L.nqcd_mi355x_reduce.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
+    L.nqcd_mi355x_download.argtypes = [H, C.c_int32, C.c_void_p, C.c_int64]
+    L.nqcd_mi355x_last_run_stats.argtypes = [H, C.POINTER(RunStats)]
+    L.nqcd_mi355x_last_error.argtypes = [H]
+    L.nqcd_mi355x_last_error.restype = C.c_char_p
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("nqcd_mi355x_close", "nqcd_mi355x_last_error", "nqcd_mi355x_abi_version"):
+            getattr(L, name).restype = C.c_int32
+    v = L.nqcd_mi355x_abi_version()
+    if v != ABI_VERSION:
+        raise ImportError(f"libnqcd_mi355x ABI version {v} != expected {ABI_VERSION}")
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Ensemble:
+    """One device ensemble: a thin owner of an nqcd_handle_t."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self._h = C.c_void_p()
+        self._check(self._L.nqcd_mi355x_open(device, C.byref(self._h)), None)
+        self.ntraj = 0
+        self.n_states = 0
+        self.D = 0
+        self.nsave = 1
+
+    def _check(self, rc, h="self"):
+        if rc != OK:
+            handle = self._h if h == "self" else None
+            msg = self._L.nqcd_mi355x_last_error(handle)
+            raise NQCDError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.nqcd_mi355x_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_problem(self, method, n_states, ndofs, natoms, masses, dt, algorithm=ALG_BAB_TSIT5, nbeads=1,
+                    n_electrons=0, rescaling=RESCALE_STANDARD, precision=PREC_FP64, schedule=SCHED_PERSISTENT,
+                    electronic_substeps=5, temperature=0.0):
+        self._masses = np.ascontiguousarray(np.broadcast_to(np.asarray(masses, dtype=np.float64), (natoms,)))
+        d = ProblemDesc()
+        d.method, d.algorithm, d.n_states, d.ndofs, d.natoms, d.nbeads = method, algorithm, n_states, ndofs, natoms, nbeads
+        d.n_electrons, d.rescaling, d.precision, d.schedule = n_electrons, rescaling, precision, schedule
+        d.electronic_substeps, d.masses, d.dt, d.temperature = electronic_substeps, _dp(self._masses), dt, temperature
+        self._check(self._L.nqcd_mi355x_set_problem(self._h, C.byref(d)))
+        self.n_states, self.D, self.dt = n_states, ndofs * natoms, dt
+
+    def set_model(self, slot, kind, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self._check(self._L.nqcd_mi355x_set_model(self._h, slot, kind, _dp(p), len(p)))
+
+    def load_initial(self, r, v, sig, state=None, model_id=None):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        ntraj = r.shape[0]
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        sig = np.asarray(sig)
+        # Julia column-major (n, n, ntraj) == C row-major (ntraj, n, n) transposed per matrix
+        sr = np.ascontiguousarray(np.real(sig).transpose(0, 2, 1), dtype=np.float64)
+        si = np.ascontiguousarray(np.imag(sig).transpose(0, 2, 1), dtype=np.float64)
+        st = None if state is None else np.ascontiguousarray(state, dtype=np.int32)
+        mid = None if model_id is None else np.ascontiguousarray(model_id, dtype=np.int32)
+        self._check(self._L.nqcd_mi355x_load_initial(self._h, ntraj, _dp(r), _dp(v), _dp(sr), _dp(si), _ip(st), _ip(mid)))
+        self.ntraj = ntraj
+
+    def sample_initial(self, kind, ntraj, params, seed, initial_state=0, state_basis=0, n_models=1):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        s = SamplerDesc()
+        s.kind, s.initial_state, s.state_basis, s.n_models, s.ntraj = kind, initial_state, state_basis, n_models, ntraj
+        s.params, s.nparams = _dp(p), len(p)
+        self._check(self._L.nqcd_mi355x_sample_initial(self._h, C.byref(s), C.c_uint64(seed)))
+        self.ntraj = ntraj
+
+    def run(self, t0, t1, key, save_every=0, term=None):
+        td = None
+        if term is not None:
+            td = TermDesc()
+            td.kind, td.dof, td.value = TERM_ABS_GT, term[0], term[1]
+        self._check(self._L.nqcd_mi355x_run(self._h, t0, t1, save_every, C.byref(td) if td is not None else None,
+                                            C.c_uint64(key)))
+        nsteps = int(round((t1 - t0) / self.dt))
+        self.nsave = nsteps // save_every + 1 if save_every > 0 else 1
+        return self.stats()
+
+    def stats(self):
+        s = RunStats()
+        self._check(self._L.nqcd_mi355x_last_run_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in RunStats._fields_}
+
+    def reduce(self, output_kind, reduction=RED_MEAN, length=None):
+        out = np.zeros(length, dtype=np.float64)
+        self._check(self._L.nqcd_mi355x_reduce(self._h, output_kind, reduction, _dp(out), length))
+        return out
+
+    def download(self, field):
+        n, D, nt, nsave = self.n_states, self.D, self.ntraj, self.nsave
+        shapes = {FIELD_R: ((nt, D), np.float64), FIELD_V: ((nt, D), np.float64),
+                  FIELD_SIGMA_RE: ((nt, n, n), np.float64), FIELD_SIGMA_IM: ((nt, n, n), np.float64),
+                  FIELD_STATE: ((nt,), np.int32), FIELD_STEPS: ((nt,), np.int32), FIELD_NHOPS: ((nt,), np.int32),
+                  FIELD_SAVE_POP_AD: ((nt, nsave, n), np.float64), FIELD_SAVE_POP_DI: ((nt, nsave, n), np.float64),
+                  FIELD_SAVE_STATE: ((nt, nsave), np.int32)}
+        shape, dtype = shapes[field]
+        out = np.zeros(shape, dtype=dtype)
+        self._check(self._L.nqcd_mi355x_download(self._h, field, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        if field in (FIELD_SIGMA_RE, FIELD_SIGMA_IM):
+            out = out.transpose(0, 2, 1).copy()  # column-major (n, n) per trajectory -> row-major
+        return out

@@ -1,0 +1,696 @@
+// nqcd_oracle.cpp -- FP64 CPU restatement of NQCDynamics.jl ensemble trajectory
+// integration (FSSH and Ehrenfest with BABwithTsit5, hopping, rescaling, outputs).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP library
+// and the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py load it.  Nothing in the product links it.
+//
+// Every function cites the reference file:line it restates (paths relative to the
+// NQCDynamics.jl v1.0.5 source tree).  Where behaviour lives in third-party
+// Julia packages absent from the reference tree (NQCModels, NQCCalculators,
+// OrdinaryDiffEq's Tsit5, RingPolymerArrays, NQCDistributions) the published
+// algorithm is restated and the choice is named in the function comment.
+//
+// Numerics: IEEE FP64, no FMA contraction (build with -ffp-contract=off).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+using cd = std::complex<double>;
+
+#define OR_API extern "C" __attribute__((visibility("default")))
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 counter-based RNG (Salmon, Moraes, Dror, Shaw, SC'11).
+// Replaces Julia's global rand() at fssh.jl:112 and iesh.jl:393 so that the CPU
+// and GPU draw identical hop uniforms.  Counter = (traj_lo, traj_hi, step, stream),
+// key = (key_lo, key_hi).
+// ---------------------------------------------------------------------------
+static inline void philox10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline double u53(uint32_t a, uint32_t b) {
+    const uint64_t x = (((uint64_t)a << 32) | (uint64_t)b) >> 11;
+    return (double)x * 0x1.0p-53;
+}
+
+static inline void philox_draw(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key, uint32_t o[4]) {
+    const uint32_t ctr[4] = {(uint32_t)traj, (uint32_t)(traj >> 32), step, stream};
+    const uint32_t k[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    philox10(ctr, k, o);
+}
+
+// One uniform in [0,1) per (traj, step, stream): the hop draw of fssh.jl:112.
+static inline double uniform01(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key) {
+    uint32_t o[4];
+    philox_draw(traj, step, stream, key, o);
+    return u53(o[0], o[1]);
+}
+
+// Two standard normals per counter (Box-Muller on the two 53-bit uniforms).
+static inline void normal2(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key, double* z0, double* z1) {
+    uint32_t o[4];
+    philox_draw(traj, step, stream, key, o);
+    const double u1 = u53(o[0], o[1]);
+    const double u2 = u53(o[2], o[3]);
+    const double rad = std::sqrt(-2.0 * std::log(1.0 - u1));
+    const double ang = 6.283185307179586 * u2;
+    *z0 = rad * std::cos(ang);
+    *z1 = rad * std::sin(ang);
+}
+
+enum { STREAM_INIT = 0, STREAM_HOP = 1, STREAM_STATE = 2 };
+
+OR_API void oracle_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox10(ctr, key, out); }
+OR_API double oracle_uniform(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key) {
+    return uniform01(traj, step, stream, key);
+}
+OR_API void oracle_normal2(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key, double* z) {
+    normal2(traj, step, stream, key, z, z + 1);
+}
+
+// ---------------------------------------------------------------------------
+// Models (NQCModels is not in the reference tree; closed forms from the model
+// definitions: Tully, J. Chem. Phys. 93, 1061 (1990) for TULLY1/2/3;
+// docs/src/NQCModels/systembathmodels.md:21-27 for the spin-boson matrix).
+// V is the state-dependent (electronic) part, V0 the state-independent part.
+// ---------------------------------------------------------------------------
+enum {
+    MODEL_TULLY1 = 0,
+    MODEL_TULLY2 = 1,
+    MODEL_TULLY3 = 2,
+    MODEL_SPIN_BOSON = 3,
+    MODEL_DW_GAUSSIAN = 4,
+    MODEL_HARMONIC = 5,
+};
+
+struct OModelDesc {
+    int32_t kind;
+    int32_t nstates;
+    int32_t ndof;        // flattened ndofs*natoms
+    int32_t nbath;       // spin-boson bath modes (ndof == nbath)
+    double p[16];
+    const double* omega; // nbath
+    const double* cbath; // nbath
+};
+
+static void model_eval(const OModelDesc& m, const double* r, double* V, double* dV, double* V0, double* dV0) {
+    const int n = m.nstates, D = m.ndof;
+    std::fill(V, V + n * n, 0.0);
+    std::fill(dV, dV + (size_t)D * n * n, 0.0);
+    *V0 = 0.0;
+    std::fill(dV0, dV0 + D, 0.0);
+    switch (m.kind) {
+    case MODEL_TULLY1: {  // V11 = A(1-exp(-B|x|))sign(x), V22 = -V11, V12 = C exp(-D x^2)
+        const double A = m.p[0], B = m.p[1], C = m.p[2], Dd = m.p[3];
+        const double x = r[0];
+        const double e = std::exp(-B * std::fabs(x));
+        const double sgn = (x > 0) ? 1.0 : ((x < 0) ? -1.0 : 0.0);
+        const double v11 = A * (1.0 - e) * sgn;
+        const double dv11 = A * B * e;
+        const double v12 = C * std::exp(-Dd * x * x);
+        const double dv12 = -2.0 * Dd * x * v12;
+        V[0] = v11; V[3] = -v11; V[1] = V[2] = v12;
+        dV[0] = dv11; dV[3] = -dv11; dV[1] = dV[2] = dv12;
+        break;
+    }
+    case MODEL_TULLY2: {  // V11 = 0, V22 = -A exp(-B x^2) + E0, V12 = C exp(-D x^2)
+        const double A = m.p[0], B = m.p[1], C = m.p[2], Dd = m.p[3], E0 = m.p[4];
+        const double x = r[0];
+        const double g = A * std::exp(-B * x * x);
+        const double v12 = C * std::exp(-Dd * x * x);
+        V[0] = 0.0; V[3] = -g + E0; V[1] = V[2] = v12;
+        dV[0] = 0.0; dV[3] = 2.0 * B * x * g; dV[1] = dV[2] = -2.0 * Dd * x * v12;
+        break;
+    }
+    case MODEL_TULLY3: {  // V11 = A, V22 = -A, V12 = B e^{Cx} (x<0), B(2 - e^{-Cx}) (x>=0)
+        const double A = m.p[0], B = m.p[1], C = m.p[2];
+        const double x = r[0];
+        double v12, dv12;
+        if (x < 0) { const double e = std::exp(C * x); v12 = B * e; dv12 = B * C * e; }
+        else { const double e = std::exp(-C * x); v12 = B * (2.0 - e); dv12 = B * C * e; }
+        V[0] = A; V[3] = -A; V[1] = V[2] = v12;
+        dV[1] = dV[2] = dv12;
+        break;
+    }
+    case MODEL_SPIN_BOSON: {  // [[eps + c.x, Delta],[Delta, -eps - c.x]] + sum w^2 x^2 / 2
+        const double eps = m.p[0], Delta = m.p[1];
+        double cx = 0.0, v0 = 0.0;
+        for (int j = 0; j < m.nbath; ++j) {
+            cx += m.cbath[j] * r[j];
+            v0 += 0.5 * m.omega[j] * m.omega[j] * r[j] * r[j];
+            dV[(size_t)j * 4 + 0] = m.cbath[j];
+            dV[(size_t)j * 4 + 3] = -m.cbath[j];
+            dV0[j] = m.omega[j] * m.omega[j] * r[j];
+        }
+        V[0] = eps + cx; V[3] = -eps - cx; V[1] = V[2] = Delta;
+        *V0 = v0;
+        break;
+    }
+    case MODEL_DW_GAUSSIAN: {  // V11 = a x^4 - b x^2, V22 = V11 + off, V12 = C exp(-D x^2)
+        const double a = m.p[0], b = m.p[1], off = m.p[2], C = m.p[3], Dd = m.p[4];
+        const double x = r[0];
+        const double w = a * x * x * x * x - b * x * x;
+        const double dw = 4.0 * a * x * x * x - 2.0 * b * x;
+        const double v12 = C * std::exp(-Dd * x * x);
+        if (n == 1) { V[0] = w; dV[0] = dw; break; }
+        V[0] = w; V[3] = w + off; V[1] = V[2] = v12;
+        dV[0] = dw; dV[3] = dw; dV[1] = dV[2] = -2.0 * Dd * x * v12;
+        break;
+    }
+    case MODEL_HARMONIC: {  // single state: V = m w^2 (x - x0)^2 / 2
+        const double mass = m.p[0], w = m.p[1], x0 = m.p[2];
+        for (int j = 0; j < D; ++j) {
+            V[0] += 0.5 * mass * w * w * (r[j] - x0) * (r[j] - x0);
+            dV[j] = mass * w * w * (r[j] - x0);
+        }
+        break;
+    }
+    default: break;
+    }
+}
+
+OR_API void oracle_model_eval(const OModelDesc* m, const double* r, double* V, double* dV, double* V0, double* dV0) {
+    model_eval(*m, r, V, dV, V0, dV0);
+}
+
+// ---------------------------------------------------------------------------
+// Real symmetric eigen-decomposition (stands in for NQCCalculators' eigen; the
+// reference calls LAPACK through it).  n == 2: closed form via the rotation
+// angle; n > 2: cyclic Jacobi.  Eigenvalues ascending.  Sign convention of a
+// fresh decomposition: the largest-magnitude component of each column is
+// positive (ties -> first).  Subsequent decompositions flip columns whose
+// overlap with the previous eigenvector is negative (eigenvector tracking).
+// ---------------------------------------------------------------------------
+static void canonical_sign(int n, double* U) {
+    for (int j = 0; j < n; ++j) {
+        int imax = 0; double amax = -1.0;
+        for (int i = 0; i < n; ++i) { const double a = std::fabs(U[i * n + j]); if (a > amax) { amax = a; imax = i; } }
+        if (U[imax * n + j] < 0) for (int i = 0; i < n; ++i) U[i * n + j] = -U[i * n + j];
+    }
+}
+
+static void track_sign(int n, double* U, const double* Uprev) {
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += U[i * n + j] * Uprev[i * n + j];
+        if (s < 0) for (int i = 0; i < n; ++i) U[i * n + j] = -U[i * n + j];
+    }
+}
+
+static void eigen2(const double* H, double* E, double* U) {
+    const double a = H[0], b = H[1], d = H[3];
+    const double mean = 0.5 * (a + d), half = 0.5 * (a - d);
+    const double rad = std::hypot(half, b);
+    const double th = 0.5 * std::atan2(b, half);
+    const double c = std::cos(th), s = std::sin(th);
+    E[0] = mean - rad; E[1] = mean + rad;
+    U[0] = -s; U[2] = c;   // column 0: eigenvector of E0
+    U[1] = c;  U[3] = s;   // column 1: eigenvector of E1
+}
+
+static void eigen_jacobi(int n, const double* H, double* E, double* U) {
+    std::vector<double> A(H, H + n * n);
+    for (int i = 0; i < n * n; ++i) U[i] = 0.0;
+    for (int i = 0; i < n; ++i) U[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) { const double a2 = A[i * n + j] * A[i * n + j]; tot += a2; if (i != j) off += a2; }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p) for (int q = p + 1; q < n; ++q) {
+            const double apq = A[p * n + q];
+            if (apq == 0.0) continue;
+            const double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+            const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+            const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+            for (int k = 0; k < n; ++k) {  // A <- A J (columns p, q)
+                const double akp = A[k * n + p], akq = A[k * n + q];
+                A[k * n + p] = c * akp - s * akq; A[k * n + q] = s * akp + c * akq;
+            }
+            for (int k = 0; k < n; ++k) {  // A <- J^T A (rows p, q)
+                const double apk = A[p * n + k], aqk = A[q * n + k];
+                A[p * n + k] = c * apk - s * aqk; A[q * n + k] = s * apk + c * aqk;
+            }
+            for (int k = 0; k < n; ++k) {
+                const double ukp = U[k * n + p], ukq = U[k * n + q];
+                U[k * n + p] = c * ukp - s * ukq; U[k * n + q] = s * ukp + c * ukq;
+            }
+        }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A[x * n + x] < A[y * n + y]; });
+    std::vector<double> Us(U, U + n * n);
+    for (int j = 0; j < n; ++j) { E[j] = A[idx[j] * n + idx[j]]; for (int i = 0; i < n; ++i) U[i * n + j] = Us[i * n + idx[j]]; }
+}
+
+static void eigen_sym(int n, const double* H, double* E, double* U, const double* Uprev) {
+    if (n == 1) { E[0] = H[0]; U[0] = 1.0; return; }
+    if (n == 2) eigen2(H, E, U); else eigen_jacobi(n, H, E, U);
+    if (Uprev) track_sign(n, U, Uprev); else canonical_sign(n, U);
+}
+
+OR_API void oracle_eigen_sym(int32_t n, const double* H, double* E, double* U, const double* Uprev) {
+    eigen_sym(n, H, E, U, Uprev);
+}
+
+// Adiabatic derivative G_I = U^T dV_I U and nonadiabatic coupling
+// d_I[i][j] = G_I[i][j] / (E_j - E_i) (i != j), 0 on the diagonal.
+// (NQCCalculators get_adiabatic_derivative / get_nonadiabatic_coupling, called
+// at bab_electronics.jl:83-84 and fssh.jl:68.)
+static void adiabatic_quantities(int n, int D, const double* E, const double* U, const double* dV, double* G, double* d) {
+    std::vector<double> tmp(n * n);
+    for (int I = 0; I < D; ++I) {
+        const double* A = dV + (size_t)I * n * n;
+        double* g = G + (size_t)I * n * n;
+        double* dd = d + (size_t)I * n * n;
+        for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) {  // tmp = A U
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += A[i * n + k] * U[k * n + j];
+            tmp[i * n + j] = s;
+        }
+        for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) {  // g = U^T tmp
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += U[k * n + i] * tmp[k * n + j];
+            g[i * n + j] = s;
+        }
+        for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j)
+            dd[i * n + j] = (i == j) ? 0.0 : g[i * n + j] / (E[j] - E[i]);
+    }
+}
+
+OR_API void oracle_adiabatic_quantities(int32_t n, int32_t D, const double* E, const double* U, const double* dV, double* G, double* d) {
+    adiabatic_quantities(n, D, E, U, dV, G, d);
+}
+
+// ---------------------------------------------------------------------------
+// Density-matrix electronic propagation: electronic_dynamics.jl:15-79,104-140,
+// density_matrix_dynamics.jl:29-33, integrated by Tsit5 (OrdinaryDiffEq 6.103,
+// coefficients from Tsitouras, Comput. Math. Appl. 62, 770 (2011)) with 5 fixed
+// sub-steps of dt/5 (bab_electronics.jl:40-43), the last one clipped to t+dt.
+// ---------------------------------------------------------------------------
+namespace tsit5 {
+static const double c1 = 0.161, c2 = 0.327, c3 = 0.9, c4 = 0.9800255409045097;
+static const double a21 = 0.161;
+static const double a31 = -0.008480655492356989, a32 = 0.335480655492357;
+static const double a41 = 2.897153057105493, a42 = -6.359448489975075, a43 = 4.3622954328695815;
+static const double a51 = 5.325864828439257, a52 = -11.748883564062828, a53 = 7.4955393428898365, a54 = -0.09249506636175525;
+static const double a61 = 5.86145544294642, a62 = -12.92096931784711, a63 = 8.159367898576159, a64 = -0.071584973281401, a65 = -0.028269050394068383;
+static const double a71 = 0.09646076681806523, a72 = 0.01, a73 = 0.4798896504144996, a74 = 1.379008574103742, a75 = -3.290069515436081, a76 = 2.324710524099774;
+}
+
+// DoubleBuffer of electronic_dynamics.jl:15-53: "cur" holds (E, W, t) at the
+// start of the nuclear step, "next" at its end; W = sum_I d_I v_I.
+struct EBuf {
+    std::vector<double> E, W;
+    double t = 0.0;
+};
+
+struct EParams {
+    int n;
+    EBuf cur, next;
+};
+
+// update_parameters! (electronic_dynamics.jl:38-53): swap, then fill next.
+static void update_parameters(EParams& P, const double* E, const double* d, const double* v, int D, double t) {
+    std::swap(P.cur, P.next);
+    const int n = P.n;
+    for (int i = 0; i < n; ++i) P.next.E[i] = E[i];
+    std::fill(P.next.W.begin(), P.next.W.end(), 0.0);
+    for (int I = 0; I < D; ++I)
+        for (int J = 0; J < n * n; ++J) P.next.W[J] += d[(size_t)I * n * n + J] * v[I];
+    P.next.t = t;
+}
+
+// f! of electronic_dynamics.jl:106-116: A = -i W(tau); diag(A) = E(tau); du = -i(A u - u A).
+static void density_rhs(const EParams& P, const cd* u, double tau, cd* du, cd* A, cd* tmp) {
+    const int n = P.n;
+    double loc = (tau - P.cur.t) / (P.next.t - P.cur.t);
+    if (std::isnan(loc)) loc = 0.0;
+    for (int J = 0; J < n * n; ++J) {
+        const double w = P.cur.W[J] + (P.next.W[J] - P.cur.W[J]) * loc;
+        A[J] = cd(0.0, -w);  // (w + 0i) * (-1im)
+    }
+    for (int i = 0; i < n; ++i) A[i * n + i] = cd(P.cur.E[i] + (P.next.E[i] - P.cur.E[i]) * loc, 0.0);
+    for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) {  // tmp = u A
+        cd s = 0.0;
+        for (int k = 0; k < n; ++k) s += u[i * n + k] * A[k * n + j];
+        tmp[i * n + j] = s;
+    }
+    for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) {  // du = A u - tmp, then * -i
+        cd s = 0.0;
+        for (int k = 0; k < n; ++k) s += A[i * n + k] * u[k * n + j];
+        const cd c = s - tmp[i * n + j];
+        du[i * n + j] = cd(c.imag(), -c.real());
+    }
+}
+
+static void tsit5_step(const EParams& P, cd* u, double tau, double h, std::vector<cd>& ws) {
+    using namespace tsit5;
+    const int nn = P.n * P.n;
+    cd* k1 = ws.data(); cd* k2 = k1 + nn; cd* k3 = k2 + nn; cd* k4 = k3 + nn; cd* k5 = k4 + nn; cd* k6 = k5 + nn;
+    cd* tmp = k6 + nn; cd* A = tmp + nn; cd* t2 = A + nn;
+    density_rhs(P, u, tau, k1, A, t2);
+    for (int J = 0; J < nn; ++J) tmp[J] = u[J] + h * (a21 * k1[J]);
+    density_rhs(P, tmp, tau + c1 * h, k2, A, t2);
+    for (int J = 0; J < nn; ++J) tmp[J] = u[J] + h * (a31 * k1[J] + a32 * k2[J]);
+    density_rhs(P, tmp, tau + c2 * h, k3, A, t2);
+    for (int J = 0; J < nn; ++J) tmp[J] = u[J] + h * (a41 * k1[J] + a42 * k2[J] + a43 * k3[J]);
+    density_rhs(P, tmp, tau + c3 * h, k4, A, t2);
+    for (int J = 0; J < nn; ++J) tmp[J] = u[J] + h * (a51 * k1[J] + a52 * k2[J] + a53 * k3[J] + a54 * k4[J]);
+    density_rhs(P, tmp, tau + c4 * h, k5, A, t2);
+    for (int J = 0; J < nn; ++J) tmp[J] = u[J] + h * (a61 * k1[J] + a62 * k2[J] + a63 * k3[J] + a64 * k4[J] + a65 * k5[J]);
+    density_rhs(P, tmp, tau + h, k6, A, t2);
+    for (int J = 0; J < nn; ++J)
+        u[J] = u[J] + h * (a71 * k1[J] + a72 * k2[J] + a73 * k3[J] + a74 * k4[J] + a75 * k5[J] + a76 * k6[J]);
+}
+
+// step!(electronic_integrator, dt, true) after set_ut!(.., sigma, t) (bab_electronics.jl:87-90).
+static void propagate_density(const EParams& P, cd* u, double t, double dt, int nsub, std::vector<cd>& ws) {
+    const double h = dt / nsub;
+    const double tend = t + dt;
+    double tau = t;
+    for (int s = 0; s < nsub; ++s) {
+        double hs = std::min(h, tend - tau);
+        tsit5_step(P, u, tau, hs, ws);
+        tau = (s == nsub - 1) ? tend : tau + hs;
+    }
+}
+
+OR_API void oracle_propagate_density(int32_t n, double* sig_re, double* sig_im,
+                                     const double* E0, const double* W0, double t0,
+                                     const double* E1, const double* W1, double t1,
+                                     double t, double dt, int32_t nsub) {
+    EParams P; P.n = n;
+    P.cur.E.assign(E0, E0 + n); P.cur.W.assign(W0, W0 + n * n); P.cur.t = t0;
+    P.next.E.assign(E1, E1 + n); P.next.W.assign(W1, W1 + n * n); P.next.t = t1;
+    std::vector<cd> u(n * n), ws(9 * n * n);
+    for (int J = 0; J < n * n; ++J) u[J] = cd(sig_re[J], sig_im[J]);
+    propagate_density(P, u.data(), t, dt, nsub, ws);
+    for (int J = 0; J < n * n; ++J) { sig_re[J] = u[J].real(); sig_im[J] = u[J].imag(); }
+}
+
+// ---------------------------------------------------------------------------
+// Surface hopping: fssh.jl:96-121, surface_hopping.jl:64-168.
+// ---------------------------------------------------------------------------
+// fewest_switches_probability! (fssh.jl:96-108): clamp then cumsum.
+static void fssh_probability(int n, int D, const double* v, const cd* sig, int s, const double* d, double dt, double* prob) {
+    for (int m = 0; m < n; ++m) prob[m] = 0.0;
+    for (int m = 0; m < n; ++m) {
+        if (m == s) continue;
+        const double ratio = (sig[m * n + s] / sig[s * n + s]).real();
+        for (int I = 0; I < D; ++I) prob[m] += 2.0 * v[I] * ratio * d[(size_t)I * n * n + s * n + m] * dt;
+    }
+    for (int m = 0; m < n; ++m) prob[m] = std::min(std::max(prob[m], 0.0), 1.0);
+    for (int m = 1; m < n; ++m) prob[m] += prob[m - 1];
+}
+
+// select_new_state (fssh.jl:110-121): first m != s with cumulative prob > r.
+static int fssh_select(int n, const double* prob, int s, double rnd) {
+    for (int m = 0; m < n; ++m) {
+        if (m == s) continue;
+        if (prob[m] > rnd) return m;
+    }
+    return s;
+}
+
+enum { RESCALE_STANDARD = 0, RESCALE_VINVERSION = 1, RESCALE_OFF = 2 };
+
+// rescale_velocity! (surface_hopping.jl:64-99) for a flat DOF vector.
+// Returns 1 if the hop is accepted.  d = coupling[new, old] (surface_hopping.jl:106-108).
+static int rescale_velocity(int n, int D, int rescaling, double* v, const double* mass, const double* E,
+                            const double* dcoup, int new_state, int old_state) {
+    if (rescaling == RESCALE_OFF) return 1;
+    std::vector<double> d(D);
+    for (int I = 0; I < D; ++I) d[I] = dcoup[(size_t)I * n * n + new_state * n + old_state];
+    double a = 0.0, b = 0.0;
+    for (int I = 0; I < D; ++I) { a += d[I] * d[I] / mass[I]; b += d[I] * v[I]; }
+    a /= 2.0;
+    const double c = E[new_state] - E[old_state];
+    const double disc = b * b - 4.0 * a * c;
+    if (disc < 0) {
+        if (rescaling == RESCALE_VINVERSION) {  // frustrated_hop_invert_velocity! (:155-164)
+            double nrm = 0.0;
+            for (int I = 0; I < D; ++I) nrm += d[I] * d[I];
+            nrm = std::sqrt(nrm);
+            double g = 0.0;
+            for (int I = 0; I < D; ++I) g += v[I] * (d[I] / nrm);
+            for (int I = 0; I < D; ++I) v[I] -= 2.0 * g * (d[I] / nrm);
+        }
+        return 0;
+    }
+    const double root = std::sqrt(disc);
+    const double gamma = (b < 0) ? (b + root) / (2.0 * a) : (b - root) / (2.0 * a);
+    for (int I = 0; I < D; ++I) v[I] -= gamma * d[I] / mass[I];
+    return 1;
+}
+
+OR_API void oracle_fssh_probability(int32_t n, int32_t D, const double* v, const double* sig_re, const double* sig_im,
+                                    int32_t s, const double* d, double dt, double* prob) {
+    std::vector<cd> sig(n * n);
+    for (int J = 0; J < n * n; ++J) sig[J] = cd(sig_re[J], sig_im[J]);
+    fssh_probability(n, D, v, sig.data(), s, d, dt, prob);
+}
+OR_API int32_t oracle_fssh_select(int32_t n, const double* prob, int32_t s, double rnd) { return fssh_select(n, prob, s, rnd); }
+OR_API int32_t oracle_rescale_velocity(int32_t n, int32_t D, int32_t rescaling, double* v, const double* mass,
+                                       const double* E, const double* d, int32_t new_state, int32_t old_state) {
+    return rescale_velocity(n, D, rescaling, v, mass, E, d, new_state, old_state);
+}
+
+// ---------------------------------------------------------------------------
+// Per-trajectory state and the BABwithTsit5 step (bab_electronics.jl:48-91).
+// ---------------------------------------------------------------------------
+enum { METHOD_FSSH = 0, METHOD_EHRENFEST = 1 };
+
+struct ORunDesc {
+    int32_t method;
+    int32_t nstates;
+    int32_t ndof;
+    int32_t rescaling;
+    double dt;
+    double t0;
+    int64_t nsteps;       // maximum number of nuclear steps
+    double term_x;        // terminate when |r[0]| > term_x (disabled when <= 0)
+    int32_t save_every;   // save slot every this many steps (slot 0 = t0); 0 = none
+    int32_t nsave;        // number of save slots per trajectory
+    uint64_t key;
+    int64_t traj_offset;  // global id of trajectory 0 of this call
+    int32_t nsub;         // electronic sub-steps per nuclear step (5)
+    int32_t pad;
+};
+
+struct Cache {
+    int n, D;
+    std::vector<double> V, dV, dV0, E, U, Uprev, G, d;
+    double V0 = 0.0;
+    bool have_prev = false;
+    void init(int n_, int D_) {
+        n = n_; D = D_;
+        V.assign(n * n, 0); dV.assign((size_t)D * n * n, 0); dV0.assign(D, 0);
+        E.assign(n, 0); U.assign(n * n, 0); Uprev.assign(n * n, 0);
+        G.assign((size_t)D * n * n, 0); d.assign((size_t)D * n * n, 0);
+        have_prev = false;
+    }
+    // update_cache!(cache, r): model, eigen with tracking, U^T dV U, NAC.
+    void update(const OModelDesc& m, const double* r) {
+        model_eval(m, r, V.data(), dV.data(), &V0, dV0.data());
+        if (have_prev) Uprev = U;
+        eigen_sym(n, V.data(), E.data(), U.data(), have_prev ? Uprev.data() : nullptr);
+        have_prev = true;
+        adiabatic_quantities(n, D, E.data(), U.data(), dV.data(), G.data(), d.data());
+    }
+};
+
+// FSSH acceleration (fssh.jl:67-74): -dE_s/dR / m, with the state-independent
+// derivative of system-bath models added (QuantumModel convention).
+static void accel_fssh(const Cache& c, int s, const double* mass, double* k) {
+    const int n = c.n;
+    for (int I = 0; I < c.D; ++I) k[I] = -(c.G[(size_t)I * n * n + s * n + s] + c.dV0[I]) / mass[I];
+}
+
+// Ehrenfest acceleration (ehrenfest.jl:50-68): -(dV0 + sum_nm G_nm Re sigma_nm) / m.
+static void accel_ehrenfest(const Cache& c, const cd* sig, const double* mass, double* k) {
+    const int n = c.n;
+    for (int I = 0; I < c.D; ++I) {
+        double f = -c.dV0[I];
+        for (int m = 0; m < n; ++m) for (int nn = 0; nn < n; ++nn)
+            f -= c.G[(size_t)I * n * n + nn * n + m] * sig[nn * n + m].real();
+        k[I] = f / mass[I];
+    }
+}
+
+// Diabatic population estimators: FSSH (fssh.jl:132-142) and Ehrenfest (ehrenfest.jl:75-83).
+static void diabatic_population(int method, int n, const double* U, const cd* sig, int s, double* out) {
+    std::vector<double> S(n * n);
+    for (int J = 0; J < n * n; ++J) S[J] = sig[J].real();
+    if (method == METHOD_FSSH) {
+        for (int i = 0; i < n; ++i) S[i * n + i] = 0.0;
+        S[s * n + s] = 1.0;
+    }
+    for (int i = 0; i < n; ++i) {  // diag(U S U^T)
+        double p = 0.0;
+        for (int a = 0; a < n; ++a) for (int b = 0; b < n; ++b) p += U[i * n + a] * S[a * n + b] * U[i * n + b];
+        out[i] = p;
+    }
+}
+
+static void adiabatic_population(int method, int n, const cd* sig, int s, double* out) {
+    if (method == METHOD_FSSH) { for (int i = 0; i < n; ++i) out[i] = (i == s) ? 1.0 : 0.0; }
+    else { for (int i = 0; i < n; ++i) out[i] = sig[i * n + i].real(); }
+}
+
+struct TrajOut {
+    double* r; double* v; double* sig_re; double* sig_im;
+    int32_t* state; int64_t* steps; int32_t* nhops;
+    double* pop_ad; double* pop_di; int32_t* save_state;
+};
+
+static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, int64_t i, uint64_t gid,
+                    const double* r0, const double* v0, const double* sr0, const double* si0, int32_t st0,
+                    const TrajOut& O) {
+    const int n = R.nstates, D = R.ndof, nn = n * n;
+    std::vector<double> r(r0, r0 + D), v(v0, v0 + D), k(D), vtmp(D);
+    std::vector<cd> sig(nn), ws(9 * nn);
+    for (int J = 0; J < nn; ++J) sig[J] = cd(sr0[J], si0[J]);
+    Cache c; c.init(n, D);
+    c.update(m, r.data());  // initialize! (bab_electronics.jl:48-59)
+    int s = st0;
+    if (R.method == METHOD_FSSH && s < 0) {  // fssh.jl:54 sample(Weights(diag(real.(sigma))))
+        double tot = 0.0;
+        for (int a = 0; a < n; ++a) tot += sig[a * n + a].real();
+        const double u = uniform01(gid, 0, STREAM_STATE, R.key) * tot;
+        double cum = 0.0; s = n - 1;
+        for (int a = 0; a < n; ++a) { cum += sig[a * n + a].real(); if (u < cum) { s = a; break; } }
+    }
+    if (R.method == METHOD_FSSH) accel_fssh(c, s, mass, k.data());
+    else accel_ehrenfest(c, sig.data(), mass, k.data());
+    EParams P; P.n = n;
+    P.cur.E.assign(n, 0.0); P.cur.W.assign(nn, 0.0); P.cur.t = 0.0;  // Q1: zero buffers at t = 0
+    P.next = P.cur;
+    double t = R.t0;
+    int32_t nhops = 0;
+    auto save = [&](int slot) {
+        if (slot < 0 || slot >= R.nsave) return;
+        if (O.pop_ad) adiabatic_population(R.method, n, sig.data(), s, O.pop_ad + ((size_t)i * R.nsave + slot) * n);
+        if (O.pop_di) diabatic_population(R.method, n, c.U.data(), sig.data(), s, O.pop_di + ((size_t)i * R.nsave + slot) * n);
+        if (O.save_state) O.save_state[(size_t)i * R.nsave + slot] = s;
+    };
+    save(0);
+    int64_t step = 0;
+    std::vector<double> prob(n);
+    while (step < R.nsteps) {
+        if (R.term_x > 0 && std::fabs(r[0]) > R.term_x) break;
+        std::vector<cd> sigprev = sig;
+        for (int I = 0; I < D; ++I) vtmp[I] = v[I] + (R.dt / 2) * k[I];   // step_B!
+        for (int I = 0; I < D; ++I) r[I] = r[I] + R.dt * vtmp[I];         // step_A!
+        c.update(m, r.data());
+        if (R.method == METHOD_FSSH) accel_fssh(c, s, mass, k.data());
+        else accel_ehrenfest(c, sigprev.data(), mass, k.data());
+        for (int I = 0; I < D; ++I) v[I] = vtmp[I] + (R.dt / 2) * k[I];
+        update_parameters(P, c.E.data(), c.d.data(), v.data(), D, t + R.dt);
+        propagate_density(P, sig.data(), t, R.dt, R.nsub, ws);
+        t = t + R.dt;
+        if (R.method == METHOD_FSSH) {  // HoppingCallback after the step (surface_hopping.jl:2-16)
+            fssh_probability(n, D, v.data(), sig.data(), s, c.d.data(), R.dt, prob.data());
+            const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
+            const int ns = fssh_select(n, prob.data(), s, rnd);
+            if (ns != s) {
+                if (rescale_velocity(n, D, R.rescaling, v.data(), mass, c.E.data(), c.d.data(), ns, s)) { s = ns; ++nhops; }
+            }
+            // Q4: k is not recomputed after an accepted hop.
+        }
+        ++step;
+        if (R.save_every > 0 && step % R.save_every == 0) save((int)(step / R.save_every));
+    }
+    for (int I = 0; I < D; ++I) { O.r[(size_t)i * D + I] = r[I]; O.v[(size_t)i * D + I] = v[I]; }
+    for (int J = 0; J < nn; ++J) { O.sig_re[(size_t)i * nn + J] = sig[J].real(); O.sig_im[(size_t)i * nn + J] = sig[J].imag(); }
+    O.state[i] = s; O.steps[i] = step;
+    if (O.nhops) O.nhops[i] = nhops;
+}
+
+template <class F>
+static void parallel_for(int64_t n, int nthreads, F f) {
+    if (nthreads <= 1 || n < 2) { for (int64_t i = 0; i < n; ++i) f(i); return; }
+    std::atomic<int64_t> next(0);
+    std::vector<std::thread> th;
+    const int64_t chunk = std::max<int64_t>(1, n / (nthreads * 16));
+    for (int t = 0; t < nthreads; ++t) th.emplace_back([&]() {
+        for (;;) {
+            const int64_t b = next.fetch_add(chunk);
+            if (b >= n) break;
+            const int64_t e = std::min(n, b + chunk);
+            for (int64_t i = b; i < e; ++i) f(i);
+        }
+    });
+    for (auto& x : th) x.join();
+}
+
+// Ensemble driver (the EnsembleThreads analogue): each trajectory independent.  The Philox
+// counters use the global trajectory id traj_ids[i] (or traj_offset + i when traj_ids is null).
+OR_API int32_t oracle_run_ensemble(const ORunDesc* R, const OModelDesc* models, const double* mass, int64_t ntraj,
+                                   const int32_t* model_id, const double* r0, const double* v0,
+                                   const double* sr0, const double* si0, const int32_t* st0,
+                                   double* r, double* v, double* sr, double* si, int32_t* state, int64_t* steps,
+                                   int32_t* nhops, double* pop_ad, double* pop_di, int32_t* save_state, int32_t nthreads,
+                                   const int64_t* traj_ids) {
+    const int D = R->ndof, nn = R->nstates * R->nstates;
+    TrajOut O{r, v, sr, si, state, steps, nhops, pop_ad, pop_di, save_state};
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        const int mid = model_id ? model_id[i] : 0;
+        const uint64_t gid = traj_ids ? (uint64_t)traj_ids[i] : (uint64_t)(R->traj_offset + i);
+        run_one(*R, models[mid], mass, i, gid, r0 + (size_t)i * D, v0 + (size_t)i * D, sr0 + (size_t)i * nn, si0 + (size_t)i * nn,
+                st0 ? st0[i] : 0, O);
+    });
+    return 0;
+}
+
+// Initial-condition sampling used by the workloads (stream 0 normals):
+// position x ~ N(mu_x, sx) per DOF, velocity v ~ N(mu_v, sv) per DOF.
+// Draw j of trajectory g is philox counter (g, j, STREAM_INIT): DOF I uses
+// counter j = I and takes z0 for r, z1 for v.
+OR_API void oracle_sample_gaussian(int64_t ntraj, int64_t traj_offset, int32_t D, const double* mu_r, const double* sd_r,
+                                   const double* mu_v, const double* sd_v, uint64_t key, double* r, double* v,
+                                   const int64_t* traj_ids) {
+    for (int64_t i = 0; i < ntraj; ++i) {
+        const uint64_t g = traj_ids ? (uint64_t)traj_ids[i] : (uint64_t)(traj_offset + i);
+        for (int I = 0; I < D; ++I) {
+            double z0, z1;
+            normal2(g, (uint32_t)I, STREAM_INIT, key, &z0, &z1);
+            r[(size_t)i * D + I] = mu_r[(size_t)i * D + I] + sd_r[(size_t)i * D + I] * z0;
+            v[(size_t)i * D + I] = mu_v[(size_t)i * D + I] + sd_v[(size_t)i * D + I] * z1;
+        }
+    }
+}
+
+// Adiabatic initial density from a diabatic pure state (density_matrix_dynamics.jl:37-46,64-75):
+// sigma_ad = U^T rho_diab U with U from the eigen-decomposition at r0.
+OR_API void oracle_diabatic_to_adiabatic(const OModelDesc* m, int64_t ntraj, const double* r0, int32_t diabatic_state,
+                                         double* sig_re, double* sig_im) {
+    const int n = m->nstates, D = m->ndof, nn = n * n;
+    for (int64_t i = 0; i < ntraj; ++i) {
+        Cache c; c.init(n, D);
+        c.update(*m, r0 + (size_t)i * D);
+        for (int a = 0; a < n; ++a) for (int b = 0; b < n; ++b) {
+            sig_re[(size_t)i * nn + a * n + b] = c.U[diabatic_state * n + a] * c.U[diabatic_state * n + b];
+            sig_im[(size_t)i * nn + a * n + b] = 0.0;
+        }
+    }
+}

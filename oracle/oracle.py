@@ -1,0 +1,211 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/nqcd_oracle.cpp).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libnqcd_oracle.so")
+
+MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC = range(6)
+METHOD_FSSH, METHOD_EHRENFEST = 0, 1
+RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = 0, 1, 2
+
+
+class OModelDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nstates", C.c_int32), ("ndof", C.c_int32), ("nbath", C.c_int32),
+                ("p", C.c_double * 16), ("omega", C.POINTER(C.c_double)), ("cbath", C.POINTER(C.c_double))]
+
+
+class ORunDesc(C.Structure):
+    _fields_ = [("method", C.c_int32), ("nstates", C.c_int32), ("ndof", C.c_int32), ("rescaling", C.c_int32),
+                ("dt", C.c_double), ("t0", C.c_double), ("nsteps", C.c_int64), ("term_x", C.c_double),
+                ("save_every", C.c_int32), ("nsave", C.c_int32), ("key", C.c_uint64), ("traj_offset", C.c_int64),
+                ("nsub", C.c_int32), ("pad", C.c_int32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def make_model(kind, nstates, ndof, params, omega=None, cbath=None):
+    m = OModelDesc()
+    m.kind, m.nstates, m.ndof = kind, nstates, ndof
+    for i, x in enumerate(params):
+        m.p[i] = float(x)
+    keep = []
+    if omega is not None:
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        cb = np.ascontiguousarray(cbath, dtype=np.float64)
+        keep = [om, cb]
+        m.nbath = len(om)
+        m.omega, m.cbath = _p(om), _p(cb)
+    m._keep = keep  # keep the arrays alive with the descriptor
+    return m
+
+
+def philox(ctr, key):
+    out = np.zeros(4, dtype=np.uint32)
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    lib().oracle_philox4x32_10(_p(c, C.c_uint32), _p(k, C.c_uint32), _p(out, C.c_uint32))
+    return out
+
+
+def uniform(traj, step, stream, key):
+    f = lib().oracle_uniform
+    f.restype = C.c_double
+    f.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64]
+    return f(traj, step, stream, key)
+
+
+def normal2(traj, step, stream, key):
+    z = np.zeros(2)
+    f = lib().oracle_normal2
+    f.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(C.c_double)]
+    f(traj, step, stream, key, _p(z))
+    return z
+
+
+def model_eval(m, r):
+    n, D = m.nstates, m.ndof
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    V = np.zeros((n, n)); dV = np.zeros((D, n, n)); V0 = C.c_double(0); dV0 = np.zeros(D)
+    lib().oracle_model_eval(C.byref(m), _p(r), _p(V), _p(dV), C.byref(V0), _p(dV0))
+    return V, dV, V0.value, dV0
+
+
+def eigen_sym(H, Uprev=None):
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    n = H.shape[0]
+    E = np.zeros(n); U = np.zeros((n, n))
+    up = None if Uprev is None else np.ascontiguousarray(Uprev, dtype=np.float64)
+    lib().oracle_eigen_sym(C.c_int32(n), _p(H), _p(E), _p(U), _p(up))
+    return E, U
+
+
+def adiabatic_quantities(E, U, dV):
+    n = len(E); D = dV.shape[0]
+    G = np.zeros((D, n, n)); d = np.zeros((D, n, n))
+    lib().oracle_adiabatic_quantities(C.c_int32(n), C.c_int32(D), _p(np.ascontiguousarray(E)), _p(np.ascontiguousarray(U)),
+                                      _p(np.ascontiguousarray(dV)), _p(G), _p(d))
+    return G, d
+
+
+def propagate_density(sig, E0, W0, t0, E1, W1, t1, t, dt, nsub=5):
+    n = sig.shape[0]
+    sr = np.ascontiguousarray(sig.real, dtype=np.float64).copy()
+    si = np.ascontiguousarray(sig.imag, dtype=np.float64).copy()
+    f = lib().oracle_propagate_density
+    f.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                  C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_int32]
+    arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (E0, W0, E1, W1)]
+    f(n, _p(sr), _p(si), _p(arr[0]), _p(arr[1]), t0, _p(arr[2]), _p(arr[3]), t1, t, dt, nsub)
+    return sr + 1j * si
+
+
+def fssh_probability(v, sig, s, d, dt):
+    n = sig.shape[0]; D = len(v)
+    prob = np.zeros(n)
+    f = lib().oracle_fssh_probability
+    f.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                  C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
+    f(n, D, _p(np.ascontiguousarray(v, dtype=np.float64)), _p(np.ascontiguousarray(sig.real)), _p(np.ascontiguousarray(sig.imag)),
+      s, _p(np.ascontiguousarray(d, dtype=np.float64)), dt, _p(prob))
+    return prob
+
+
+def fssh_select(prob, s, rnd):
+    f = lib().oracle_fssh_select
+    f.argtypes = [C.c_int32, C.POINTER(C.c_double), C.c_int32, C.c_double]
+    f.restype = C.c_int32
+    p = np.ascontiguousarray(prob, dtype=np.float64)
+    return f(len(p), _p(p), s, rnd)
+
+
+def rescale_velocity(v, mass, E, d, new_state, old_state, rescaling=RESCALE_STANDARD):
+    v = np.ascontiguousarray(v, dtype=np.float64).copy()
+    n = len(E); D = len(v)
+    f = lib().oracle_rescale_velocity
+    f.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                  C.POINTER(C.c_double), C.c_int32, C.c_int32]
+    f.restype = C.c_int32
+    ok = f(n, D, rescaling, _p(v), _p(np.ascontiguousarray(mass, dtype=np.float64)), _p(np.ascontiguousarray(E, dtype=np.float64)),
+           _p(np.ascontiguousarray(d, dtype=np.float64)), new_state, old_state)
+    return bool(ok), v
+
+
+def sample_gaussian(ntraj, D, mu_r, sd_r, mu_v, sd_v, key, traj_offset=0, traj_ids=None):
+    shape = (ntraj, D)
+    arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shape)) for a in (mu_r, sd_r, mu_v, sd_v)]
+    r = np.zeros(shape); v = np.zeros(shape)
+    f = lib().oracle_sample_gaussian
+    f.argtypes = [C.c_int64, C.c_int64, C.c_int32] + [C.POINTER(C.c_double)] * 4 + [C.c_uint64, C.POINTER(C.c_double),
+                                                                                    C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    tid = None if traj_ids is None else np.ascontiguousarray(traj_ids, dtype=np.int64)
+    f(ntraj, traj_offset, D, *[_p(a) for a in arrs], key, _p(r), _p(v), _p(tid, C.c_int64))
+    return r, v
+
+
+def diabatic_to_adiabatic(m, r0, diabatic_state):
+    ntraj = r0.shape[0]; n = m.nstates
+    sr = np.zeros((ntraj, n, n)); si = np.zeros((ntraj, n, n))
+    f = lib().oracle_diabatic_to_adiabatic
+    f.argtypes = [C.POINTER(OModelDesc), C.c_int64, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    f(C.byref(m), ntraj, _p(np.ascontiguousarray(r0, dtype=np.float64)), diabatic_state, _p(sr), _p(si))
+    return sr + 1j * si
+
+
+def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None, dt=1.0, nsteps=100, term_x=0.0,
+                 save_every=0, nsave=1, key=0, traj_offset=0, rescaling=RESCALE_STANDARD, nthreads=None, t0=0.0, nsub=5,
+                 traj_ids=None):
+    """Run an ensemble on the CPU; returns a dict of per-trajectory results."""
+    if not isinstance(models, (list, tuple)):
+        models = [models]
+    ntraj, D = r0.shape
+    n = models[0].nstates
+    R = ORunDesc()
+    R.method, R.nstates, R.ndof, R.rescaling = method, n, D, rescaling
+    R.dt, R.t0, R.nsteps, R.term_x = dt, t0, nsteps, term_x
+    R.save_every, R.nsave, R.key, R.traj_offset, R.nsub = save_every, nsave, key, traj_offset, nsub
+    marr = (OModelDesc * len(models))(*models)
+    mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, dtype=np.float64), (D,)))
+    sr0 = np.ascontiguousarray(np.real(sig0), dtype=np.float64).reshape(ntraj, n, n)
+    si0 = np.ascontiguousarray(np.imag(sig0), dtype=np.float64).reshape(ntraj, n, n)
+    st0 = None if state0 is None else np.ascontiguousarray(state0, dtype=np.int32)
+    mid = None if model_id is None else np.ascontiguousarray(model_id, dtype=np.int32)
+    out = dict(r=np.zeros((ntraj, D)), v=np.zeros((ntraj, D)), sig_re=np.zeros((ntraj, n, n)), sig_im=np.zeros((ntraj, n, n)),
+               state=np.zeros(ntraj, dtype=np.int32), steps=np.zeros(ntraj, dtype=np.int64), nhops=np.zeros(ntraj, dtype=np.int32),
+               pop_ad=np.zeros((ntraj, nsave, n)), pop_di=np.zeros((ntraj, nsave, n)), save_state=np.zeros((ntraj, nsave), dtype=np.int32))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    tid = None if traj_ids is None else np.ascontiguousarray(traj_ids, dtype=np.int64)
+    f = lib().oracle_run_ensemble
+    f.restype = C.c_int32
+    f(C.byref(R), marr, _p(mass), C.c_int64(ntraj), _p(mid, C.c_int32),
+      _p(np.ascontiguousarray(r0, dtype=np.float64)), _p(np.ascontiguousarray(v0, dtype=np.float64)), _p(sr0), _p(si0), _p(st0, C.c_int32),
+      _p(out["r"]), _p(out["v"]), _p(out["sig_re"]), _p(out["sig_im"]), _p(out["state"], C.c_int32), _p(out["steps"], C.c_int64),
+      _p(out["nhops"], C.c_int32), _p(out["pop_ad"]), _p(out["pop_di"]), _p(out["save_state"], C.c_int32), C.c_int32(nthreads),
+      _p(tid, C.c_int64))
+    return out

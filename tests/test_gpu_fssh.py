@@ -1,0 +1,215 @@
+"""GPU parity of the two-state FSSH / Ehrenfest kernels against the FP64 oracle.
+
+Bar (north star): identical hop sequences, per-trajectory R/P/sigma within 1e-9
+relative on the FP64 path over short windows, ensemble populations within 2 sigma;
+the FP32-nuclei path within 0.01 absolute final population of the FP64 oracle.
+All calls go through libnqcd_mi355x.so's C ABI.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TULLY = {0: [0.01, 1.6, 0.005, 1.0], 1: [0.1, 0.28, 0.015, 0.06, 0.05], 2: [6e-4, 0.1, 0.9]}
+MASS = 2000.0
+
+
+def _ens():
+    from nqcd_mi355x import _lib as L
+    return L, L.Ensemble(0)
+
+
+def _window_inputs(oracle, key, ntraj, kind):
+    """Initial conditions near the coupling region so hops happen inside the window."""
+    rng = np.random.default_rng(11 + kind)
+    x0 = -6.5 if kind == 2 else -2.0  # Tully 3 couples strongly near x ~ -5.7
+    r0 = rng.normal(x0, 0.6, size=(ntraj, 1))
+    p0 = rng.uniform(6.0, 32.0, size=ntraj)
+    v0 = (p0 / MASS)[:, None]
+    sig0 = np.zeros((ntraj, 2, 2), complex)
+    start = rng.integers(0, 2, size=ntraj)
+    sig0[np.arange(ntraj), start, start] = 1.0
+    # a quarter start in a coherent superposition (exercises Ehrenfest and the off-diagonals)
+    q = ntraj // 4
+    c = np.stack([np.full(q, np.sqrt(0.7)), np.sqrt(0.3) * np.exp(1j * rng.uniform(0, 2 * np.pi, q))], 1)
+    sig0[:q] = np.einsum("ti,tj->tij", c, c.conj())
+    return r0, v0, sig0, start.astype(np.int32)
+
+
+def _run_gpu(L, E, method, kind, r0, v0, sig0, state0, nsteps, key, precision=0, schedule=0, save_every=0, dt=1.0,
+             term=None, rescaling=0):
+    E.set_problem(method, 2, 1, 1, [MASS], dt, precision=precision, schedule=schedule, rescaling=rescaling)
+    E.set_model(0, kind, TULLY[kind])
+    E.load_initial(r0, v0, sig0, state=state0)
+    st = E.run(0.0, nsteps * dt, key, save_every=save_every, term=term)
+    out = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
+           "sig": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
+           "state": E.download(L.FIELD_STATE), "steps": E.download(L.FIELD_STEPS), "nhops": E.download(L.FIELD_NHOPS)}
+    if save_every:
+        out["save_state"] = E.download(L.FIELD_SAVE_STATE)
+        out["pop_ad"] = E.download(L.FIELD_SAVE_POP_AD)
+        out["pop_di"] = E.download(L.FIELD_SAVE_POP_DI)
+    out["stats"] = st
+    return out
+
+
+def _run_cpu(oracle, method, kind, r0, v0, sig0, state0, nsteps, key, save_every=0, dt=1.0, term_x=0.0, rescaling=0):
+    m = oracle.make_model(kind, 2, 1, TULLY[kind])
+    nsave = nsteps // save_every + 1 if save_every else 1
+    return oracle.run_ensemble(method, m, MASS, r0, v0, sig0, state0=state0, dt=dt, nsteps=nsteps, key=key,
+                               save_every=save_every, nsave=nsave, term_x=term_x, rescaling=rescaling)
+
+
+def _assert_window_parity(g, c, rtol=1e-9):
+    assert np.array_equal(g["state"], c["state"])
+    assert np.array_equal(g["nhops"], c["nhops"])
+    assert np.allclose(g["r"], c["r"], rtol=rtol, atol=1e-12)
+    assert np.allclose(g["v"], c["v"], rtol=rtol, atol=1e-14)
+    sc = c["sig_re"] + 1j * c["sig_im"]
+    assert np.allclose(g["sig"], sc, rtol=rtol, atol=1e-11)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_fssh_fp64_window_parity(oracle, key, kind, schedule):
+    """1024 trajectories, 400 steps: identical hop sequences, R/P/sigma to 1e-9 relative."""
+    L, E = _ens()
+    ntraj, nsteps = 1024, 400
+    r0, v0, sig0, st0 = _window_inputs(oracle, key, ntraj, kind)
+    c = _run_cpu(oracle, 0, kind, r0, v0, sig0, st0, nsteps, key, save_every=1)
+    g = _run_gpu(L, E, 0, kind, r0, v0, sig0, st0, nsteps, key, schedule=schedule, save_every=1)
+    assert c["nhops"].sum() > 20, "window must contain hops"
+    assert np.array_equal(g["save_state"], c["save_state"])  # hop sequence, step by step
+    _assert_window_parity(g, c)
+    assert np.allclose(g["pop_di"], c["pop_di"], rtol=1e-9, atol=1e-11)
+    assert np.all(g["steps"] == nsteps)
+    E.close()
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_ehrenfest_fp64_window_parity(oracle, key, kind):
+    L, E = _ens()
+    ntraj, nsteps = 512, 400
+    r0, v0, sig0, _ = _window_inputs(oracle, key, ntraj, kind)
+    c = _run_cpu(oracle, 1, kind, r0, v0, sig0, None, nsteps, key, save_every=10)
+    g = _run_gpu(L, E, 1, kind, r0, v0, sig0, None, nsteps, key, save_every=10)
+    assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-14)
+    assert np.allclose(g["sig"], c["sig_re"] + 1j * c["sig_im"], rtol=1e-9, atol=1e-11)
+    assert np.allclose(g["pop_ad"], c["pop_ad"], rtol=1e-9, atol=1e-11)
+    assert np.allclose(g["pop_di"], c["pop_di"], rtol=1e-9, atol=1e-11)
+    E.close()
+
+
+def test_rescaling_modes_parity(oracle, key):
+    L, E = _ens()
+    ntraj, nsteps = 1024, 400
+    r0, v0, sig0, st0 = _window_inputs(oracle, key, ntraj, 1)
+    v0 = v0 * 0.35  # slow: many frustrated hops
+    for resc in (1, 2):
+        c = _run_cpu(oracle, 0, 1, r0, v0, sig0, st0, nsteps, key, rescaling=resc)
+        g = _run_gpu(L, E, 0, 1, r0, v0, sig0, st0, nsteps, key, rescaling=resc)
+        _assert_window_parity(g, c)
+    E.close()
+
+
+def test_tully1_workload_fp64_matches_oracle(oracle, key):
+    """The 1000-trajectory Tully-1 workload (p0 = 20, 3000 steps of dt = 1): same final states as
+    the oracle, hence the same populations and transmission/reflection probabilities."""
+    L, E = _ens()
+    ntraj, p0 = 1000, 20.0
+    r0, v0 = oracle.sample_gaussian(ntraj, 1, -5.0, p0 / (20 * np.sqrt(2)), p0 / MASS, 0.0, key)
+    sig0 = np.zeros((ntraj, 2, 2), complex); sig0[:, 0, 0] = 1
+    st0 = np.zeros(ntraj, np.int32)
+    c = _run_cpu(oracle, 0, 0, r0, v0, sig0, st0, 3000, key)
+    g = _run_gpu(L, E, 0, 0, r0, v0, sig0, st0, 3000, key)
+    mismatch = np.mean(g["state"] != c["state"])
+    assert mismatch <= 0.002
+    pc = np.bincount(c["state"], minlength=2) / ntraj
+    sig = np.sqrt(pc * (1 - pc) / ntraj)
+    assert np.all(np.abs(np.bincount(g["state"], minlength=2) / ntraj - pc) <= 2 * sig + 1e-12)
+    scat = E.reduce(L.OUT_SCATTERING_1D, L.RED_MEAN, 4)
+    x = c["r"][:, 0]
+    ref = [np.mean((x <= 0) & (c["state"] == 0)), np.mean((x <= 0) & (c["state"] == 1)),
+           np.mean((x > 0) & (c["state"] == 0)), np.mean((x > 0) & (c["state"] == 1))]
+    assert np.allclose(scat, ref, atol=2.5 * 0.5 / np.sqrt(ntraj))
+    fin = E.reduce(L.OUT_FINAL_ADIABATIC_POPULATION, L.RED_MEAN, 2)
+    assert np.allclose(fin, np.bincount(g["state"], minlength=2) / ntraj, atol=1e-14)
+    E.close()
+
+
+def test_fp32_nuclei_population_error(oracle, key):
+    """FP32 nuclei / FP64 electronics: final populations within 0.01 of the FP64 oracle."""
+    L, E = _ens()
+    ntraj = 4000
+    res = {}
+    for kind in (0, 1, 2):
+        p0 = 16.0
+        r0, v0 = oracle.sample_gaussian(ntraj, 1, -5.0, p0 / (20 * np.sqrt(2)), p0 / MASS, 0.0, key)
+        sig0 = np.zeros((ntraj, 2, 2), complex); sig0[:, 0, 0] = 1
+        st0 = np.zeros(ntraj, np.int32)
+        c = _run_cpu(oracle, 0, kind, r0, v0, sig0, st0, 10000, key, term_x=10.0)
+        g = _run_gpu(L, E, 0, kind, r0, v0, sig0, st0, 10000, key, precision=1, term=(0, 10.0))
+        pc = np.bincount(c["state"], minlength=2) / ntraj
+        pg = np.bincount(g["state"], minlength=2) / ntraj
+        res[kind] = (pc, pg)
+        assert np.abs(pg - pc).max() <= 0.01, (kind, pc, pg)
+    E.close()
+
+
+def test_device_sampler_matches_oracle(oracle, key):
+    """TULLY_GRID device sampling = oracle Philox normals (x0 = -5, p0 grid 5..35, three models)."""
+    L, E = _ens()
+    ntraj = 3 * 31 * 4
+    E.set_problem(0, 2, 1, 1, [MASS], 1.0)
+    for k in range(3):
+        E.set_model(k, k, TULLY[k])
+    E.sample_initial(L.SAMPLER_TULLY_GRID, ntraj, [-5.0, 5.0, 31], key, n_models=3)
+    E.run(0.0, 0.0, key)  # zero steps: outputs equal the initial conditions
+    ids = np.arange(ntraj)
+    p0 = 5.0 + (ids // 3) % 31
+    r0, v0 = oracle.sample_gaussian(ntraj, 1, -5.0, (p0 / (20 * np.sqrt(2)))[:, None], (p0 / MASS)[:, None], 0.0, key)
+    assert np.allclose(E.download(L.FIELD_R), r0, rtol=1e-13, atol=1e-14)
+    assert np.allclose(E.download(L.FIELD_V), v0, rtol=1e-15)
+    E.close()
+
+
+def test_mixed_models_uneven_lengths_parity(oracle, key):
+    """Mixed Tully 1/2/3 ensemble with |x|>10 termination: persistent refill queue gives the
+    oracle's per-trajectory results (FP64)."""
+    L, E = _ens()
+    ntraj = 3 * 31 * 8
+    E.set_problem(0, 2, 1, 1, [MASS], 1.0)
+    for k in range(3):
+        E.set_model(k, k, TULLY[k])
+    E.sample_initial(L.SAMPLER_TULLY_GRID, ntraj, [-5.0, 5.0, 31], key, n_models=3)
+    E.run(0.0, 10000.0, key, term=(0, 10.0))
+    g = {f: E.download(getattr(L, "FIELD_" + f.upper())) for f in ("r", "v", "state", "steps", "nhops")}
+    ids = np.arange(ntraj)
+    p0 = 5.0 + (ids // 3) % 31
+    r0, v0 = oracle.sample_gaussian(ntraj, 1, -5.0, (p0 / (20 * np.sqrt(2)))[:, None], (p0 / MASS)[:, None], 0.0, key)
+    sig0 = np.zeros((ntraj, 2, 2), complex); sig0[:, 0, 0] = 1
+    models = [oracle.make_model(k, 2, 1, TULLY[k]) for k in range(3)]
+    c = oracle.run_ensemble(0, models, MASS, r0, v0, sig0, state0=np.zeros(ntraj, np.int32), model_id=ids % 3,
+                            dt=1.0, nsteps=10000, key=key, term_x=10.0)
+    same = (g["state"] == c["state"]) & (g["steps"] == c["steps"])
+    assert same.mean() >= 0.995
+    assert np.allclose(g["r"][same], c["r"][same], rtol=1e-8, atol=1e-9)
+    st = E.stats()
+    assert st["traj_steps"] == g["steps"].sum()
+    assert len(np.unique(g["steps"])) > 100  # uneven lengths
+    E.close()
+
+
+def test_abi_errors_are_reported():
+    L, E = _ens()
+    with pytest.raises(L.NQCDError) as ei:
+        E.run(0.0, 1.0, 0)
+    assert ei.value.code == L.ERR_STATE
+    with pytest.raises(L.NQCDError) as ei:
+        E.set_problem(0, 2, 1, 1, [MASS], 1.0, rescaling=7)
+    assert "rescaling" in str(ei.value)
+    with pytest.raises(L.NQCDError) as ei:
+        E.set_problem(2, 60, 1, 1, [MASS], 1.0, algorithm=1)
+    assert ei.value.code == L.ERR_UNSUPPORTED
+    E.close()
