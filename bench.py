@@ -79,6 +79,10 @@ def cpu_baseline(L, gpu_states, nthreads):
             "state_agreement_sample": agree}
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,12 +100,14 @@ def main():
     # --- batched: persistent work-queue kernel, FP32 nuclei / FP64 electronics -------------------
     E = _ensemble(L, L.PREC_FP32_NUCLEI, L.SCHED_PERSISTENT)
     for _ in range(args.warmup):
-        E.run(0.0, TMAX, KEY, term=(0, XTERM))
+        st = E.run(0.0, TMAX, KEY, term=(0, XTERM))
+        _progress(f"warmup pass: {st['kernel_ms']:.1f} ms, {st['traj_steps']} trajectory-steps")
     kernel_ms, wall_ms, steps_total = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = E.run(0.0, TMAX, KEY, term=(0, XTERM))
         kernel_ms.append(st["kernel_ms"]); wall_ms.append(st["wall_ms"]); steps_total.append(st["traj_steps"])
+        _progress(f"timed pass: {st['kernel_ms']:.1f} ms kernel, {st['wall_ms']:.1f} ms wall")
     t_timed = time.perf_counter() - t0
     assert len(set(steps_total)) == 1, "runs must be identical"
     traj_steps = steps_total[0]
@@ -118,6 +124,7 @@ def main():
     st64 = E64.run(0.0, TMAX, KEY, term=(0, XTERM))
     pops64 = _final_pops(L, E64)
     fp64_rate = st64["traj_steps"] / (st64["kernel_ms"] / 1e3)
+    _progress(f"fp64 persistent pass: {st64['kernel_ms']:.1f} ms")
     E64.close()
     pop_err = float(np.abs(pops32 - pops64).max())
     p = pops64[:2]
@@ -129,6 +136,7 @@ def main():
         Ep = _ensemble(L, L.PREC_FP64, L.SCHED_STEP_PER_LAUNCH)
         stp = Ep.run(0.0, TMAX, KEY, term=(0, XTERM))
         popsp = _final_pops(L, Ep)
+        _progress(f"plain pass: {stp['wall_ms']:.1f} ms wall, {stp['launches']} launches")
         Ep.close()
         plain = {"value": stp["traj_steps"] / (stp["wall_ms"] / 1e3), "kernel_ms": stp["kernel_ms"],
                  "wall_ms": stp["wall_ms"], "launches": stp["launches"],
@@ -167,6 +175,7 @@ def main():
     }
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
+        _progress(f"cpu baseline on {nthreads} threads")
         out["cpu_baseline"] = cpu_baseline(L, states32, nthreads)
     print(json.dumps(out))
 

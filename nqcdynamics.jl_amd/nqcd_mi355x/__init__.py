@@ -1,3 +1,20 @@
-"""EnsembleMI355X host package: the reference's run_dynamics / Simulation API backed by
-libnqcd_mi355x.so (hand-written gfx950 kernels behind a C ABI)."""
+"""EnsembleMI355X host package.
+
+Mirrors the reference's user-facing API for ensemble trajectory integration --
+`run_dynamics(sim, tspan, distribution; output, reduction, ensemble_algorithm,
+trajectories, dt, saveat, callback)`, `Simulation{Method}`, the NQCModels models,
+NQCDistributions initial conditions, DynamicsOutputs and reductions -- with every
+trajectory integrated by libnqcd_mi355x.so (hand-written gfx950 kernels behind a
+C ABI).  There is no CPU fallback: the library must be built.
+"""
 from . import _lib  # noqa: F401
+from .distributions import (Adiabatic, Diabatic, DynamicalDistribution, DynamicsVariables, Normal,  # noqa: F401
+                            PositionHarmonicWigner, PureState, VelocityHarmonicWigner)
+from .ensembles import EnsembleMI355X, TerminatingCallback, run_dynamics  # noqa: F401
+from .models import (DebyeSpectralDensity, DoubleWellGaussian, OhmicSpectralDensity, SpinBoson,  # noqa: F401
+                     TullyModelOne, TullyModelThree, TullyModelTwo, nstates)
+from .outputs import (FileReduction, MeanReduction, OutputAdiabaticPopulation, OutputDiabaticPopulation,  # noqa: F401
+                      OutputDiscreteState, OutputFinal, OutputFinalTime, OutputStateResolvedScattering1D,
+                      OutputSurfaceHops, PopulationCorrelationFunction, SortByOutputReduction,
+                      SortByTrajectoryReduction, SumReduction)
+from .simulations import Atoms, Classical, Ehrenfest, FSSH, RingPolymerSimulation, Simulation, masses  # noqa: F401

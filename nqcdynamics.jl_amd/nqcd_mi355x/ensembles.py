@@ -1,0 +1,232 @@
+"""run_dynamics with the EnsembleMI355X ensemble algorithm (src/Ensembles/run_dynamics.jl:42-135).
+
+The reference forwards `ensemble_algorithm` to SciMLBase.solve (run_dynamics.jl:91-97);
+here EnsembleMI355X is that algorithm: it hands the whole ensemble to
+libnqcd_mi355x.so and returns what the given reduction would produce over the
+per-trajectory EnsembleSaver dictionaries (Ensembles.jl:39-63).
+"""
+import logging
+
+import numpy as np
+
+from . import _lib as L
+from .distributions import Adiabatic, DynamicalDistribution, ProductDistribution, PureState
+from .outputs import (MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction, SumReduction,
+                      output_name)
+from .simulations import Ehrenfest, FSSH, RingPolymerSimulation, Simulation
+
+log = logging.getLogger("nqcd_mi355x")
+
+
+class EnsembleMI355X:
+    """Ensemble algorithm running every trajectory on one MI355X (device index `device`).
+
+    schedule: "persistent" (work-queue kernel, default) or "step_per_launch" (plain port).
+    """
+
+    def __init__(self, device=0, schedule="persistent", seed=0x4E5143444D493335):
+        if schedule not in ("persistent", "step_per_launch"):
+            raise ValueError("schedule must be 'persistent' or 'step_per_launch'")
+        self.device, self.schedule, self.seed = device, schedule, seed
+
+
+class TerminatingCallback:
+    """TerminatingCallback(|x_dof| > value) (DynamicsUtils/callbacks.jl:15-29); the device
+    evaluates this predicate after every step.  `dof` is 1-based in (ndofs, natoms) order."""
+
+    def __init__(self, dof=1, value=10.0):
+        self.dof, self.value = int(dof), float(value)
+
+
+def _problem(E, sim, dt):
+    method = sim.method
+    if not isinstance(method, (FSSH, Ehrenfest)):
+        raise ValueError(f"EnsembleMI355X does not support {type(method).__name__} simulations")
+    if isinstance(sim, RingPolymerSimulation):
+        raise ValueError("EnsembleMI355X does not support ring-polymer simulations of this method yet")
+    E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                  algorithm=method.algorithm, rescaling=getattr(method, "rescaling_code", 0),
+                  precision=L.PREC_FP64 if sim.precision == "fp64" else L.PREC_FP32_NUCLEI,
+                  schedule=L.SCHED_PERSISTENT if E._schedule == "persistent" else L.SCHED_STEP_PER_LAUNCH,
+                  temperature=sim.temperature)
+    E.set_model(0, sim.model.kind, sim.model.params())
+
+
+def _electronic(dist):
+    if isinstance(dist, ProductDistribution):
+        return dist.electronic
+    if isinstance(dist, dict):
+        return dist.get("electronic", PureState(1, Adiabatic()))
+    return PureState(1, Adiabatic())
+
+
+def _initial_conditions(E, sim, dist, trajectories, selection, seed):
+    """Sample on the device where the distribution is recognised, otherwise upload."""
+    el = _electronic(dist)
+    nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    D = sim.ndofs * sim.natoms
+    n = sim.model.nstates
+    if el.state < 1 or el.state > n:
+        raise ValueError(f"electronic state {el.state} out of range 1..{n}")
+    state0 = el.state - 1
+    if isinstance(nuc, dict):  # a single DynamicsVariables: every trajectory starts there
+        r = np.tile(nuc["r"].reshape(-1, order="F"), (trajectories, 1))
+        v = np.tile(nuc["v"].reshape(-1, order="F"), (trajectories, 1))
+        return _upload(E, sim, r, v, el)
+    if isinstance(nuc, DynamicalDistribution) and nuc.is_list():
+        idx = np.arange(trajectories) if selection is None else np.asarray(selection) - 1
+        r = np.stack([np.asarray(nuc.position[i]).reshape(-1, order="F") for i in idx])
+        v = np.stack([np.asarray(nuc.velocity[i]).reshape(-1, order="F") for i in idx])
+        return _upload(E, sim, r, v, el)
+    if isinstance(nuc, DynamicalDistribution):
+        mr, sr, mv, sv = nuc.moments()
+        if sim.model.kind == L.MODEL_SPIN_BOSON:
+            prm = np.concatenate([mr, sr, mv, sv])
+            E.sample_initial(L.SAMPLER_HARMONIC_WIGNER, trajectories, prm, seed, initial_state=state0,
+                             state_basis=1 if el.diabatic else 0)
+            return
+        prm = np.stack([mr, sr, mv, sv], axis=1).reshape(-1)
+        E.sample_initial(L.SAMPLER_GAUSSIAN, trajectories, prm, seed, initial_state=state0,
+                         state_basis=1 if el.diabatic else 0)
+        return
+    raise ValueError(f"unsupported distribution {type(nuc).__name__}")
+
+
+def _upload(E, sim, r, v, el):
+    n = sim.model.nstates
+    ntraj = r.shape[0]
+    sig = np.zeros((ntraj, n, n), complex)
+    if el.diabatic:
+        raise ValueError("uploaded initial conditions need an adiabatic PureState (or use a distribution)")
+    sig[:, el.state - 1, el.state - 1] = 1.0
+    E.load_initial(r, v, sig, state=np.full(ntraj, el.state - 1, np.int32))
+
+
+def _results(E, t0, dt, save_every, nsteps):
+    res = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
+           "sigma": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
+           "state": E.download(L.FIELD_STATE), "steps": E.download(L.FIELD_STEPS),
+           "nhops": E.download(L.FIELD_NHOPS), "pop_ad": E.download(L.FIELD_SAVE_POP_AD),
+           "pop_di": E.download(L.FIELD_SAVE_POP_DI), "save_state": E.download(L.FIELD_SAVE_STATE)}
+    return res
+
+
+def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=None, ensemble_algorithm=None,
+                 trajectories=1, savetime=True, precompile_dynamics=False, dt=1.0, saveat=None, callback=None,
+                 **kwargs):
+    """Run `trajectories` trajectories over `tspan` sampling `distribution` (run_dynamics.jl:42-135)."""
+    if kwargs:
+        raise TypeError(f"unsupported keyword arguments for EnsembleMI355X: {sorted(kwargs)}")
+    if ensemble_algorithm is None:
+        ensemble_algorithm = EnsembleMI355X()
+    if not isinstance(ensemble_algorithm, EnsembleMI355X):
+        raise ValueError("this package provides only the EnsembleMI355X ensemble algorithm")
+    if reduction is None:
+        reduction = SortByTrajectoryReduction()
+    outputs = output if isinstance(output, tuple) else (output,)
+    t0, t1 = float(tspan[0]), float(tspan[1])
+    nsteps = int(round((t1 - t0) / dt))
+    if saveat is None:
+        save_every = 0
+    else:
+        step = float(saveat) if np.isscalar(saveat) else float(np.asarray(saveat)[1] - np.asarray(saveat)[0])
+        save_every = int(round(step / dt))
+        if save_every < 1 or abs(save_every * dt - step) > 1e-9 * max(1.0, step):
+            raise ValueError("saveat must be a positive multiple of dt")
+    term = None
+    if callback is not None:
+        if not isinstance(callback, TerminatingCallback):
+            raise ValueError("EnsembleMI355X evaluates TerminatingCallback(|x_dof| > value) predicates only")
+        term = (callback.dof - 1, callback.value)
+    if not isinstance(sim.method, (FSSH, Ehrenfest)) or isinstance(sim, RingPolymerSimulation):
+        raise ValueError(f"EnsembleMI355X does not support {type(sim).__name__}{{{type(sim.method).__name__}}}")
+    if int(trajectories) < 1:
+        raise ValueError("trajectories must be >= 1")
+    log.info("Performing %d trajectories on MI355X device %d.", trajectories, ensemble_algorithm.device)
+    E = L.Ensemble(ensemble_algorithm.device)
+    E._schedule = ensemble_algorithm.schedule
+    try:
+        _problem(E, sim, dt)
+        _initial_conditions(E, sim, distribution, int(trajectories), selection, ensemble_algorithm.seed)
+        stats = E.run(t0, t1, ensemble_algorithm.seed, save_every=save_every, term=term)
+        nsave = E.nsave
+        times = t0 + dt * save_every * np.arange(nsave) if save_every else np.array([t0, t1])
+        result = _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nsteps)
+    finally:
+        E.close()
+    if isinstance(reduction, SortByTrajectoryReduction) and trajectories == 1:
+        return result[0]
+    return result
+
+
+def _per_traj_results(E, sim, t0, dt, save_every, nsteps):
+    res = _results(E, t0, dt, save_every, nsteps)
+    n = sim.model.nstates
+    fin = np.zeros((E.ntraj, n))
+    if isinstance(sim.method, FSSH):
+        fin[np.arange(E.ntraj), res["state"]] = 1.0
+    else:
+        fin = np.real(np.einsum("tii->ti", res["sigma"]))
+    res["final_pop_ad"] = fin
+    out = []
+    for i in range(E.ntraj):
+        out.append({"r": res["r"][i].reshape(sim.size(), order="F"), "v": res["v"][i].reshape(sim.size(), order="F"),
+                    "sigma": res["sigma"][i], "state": int(res["state"][i]), "steps": int(res["steps"][i]),
+                    "nhops": int(res["nhops"][i]), "pop_ad": res["pop_ad"][i], "pop_di": res["pop_di"][i],
+                    "save_state": res["save_state"][i], "final_pop_ad": fin[i], "t0": t0, "dt": dt})
+    return out
+
+
+def _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nsteps):
+    n = sim.model.nstates
+    if isinstance(reduction, (MeanReduction, SumReduction)):
+        red = L.RED_MEAN if isinstance(reduction, MeanReduction) else L.RED_SUM
+        result = {"Time": times} if savetime else {}
+        host = None
+        for f in outputs:
+            kind = f.device_reduction if isinstance(f, Output) else None
+            if kind is not None:
+                if kind in (L.OUT_ADIABATIC_POPULATION, L.OUT_DIABATIC_POPULATION):
+                    v = E.reduce(kind, red, n * E.nsave).reshape(E.nsave, n)
+                    result[output_name(f)] = list(v)
+                elif kind == L.OUT_SCATTERING_1D:
+                    v = E.reduce(kind, red, 2 * n)
+                    result[output_name(f)] = {"reflection": v[:n], "transmission": v[n:]}
+                elif kind == L.OUT_POPULATION_CORRELATION:
+                    v = E.reduce(kind, red, n * n * E.nsave).reshape(E.nsave, n, n)
+                    result[output_name(f)] = [m.T for m in v]  # column-major n x n per save
+                continue
+            if host is None:
+                host = _per_traj_results(E, sim, t0, dt, save_every, nsteps)
+            vals = [f.per_trajectory(h) for h in host]
+            tot = _tree_sum(vals)
+            result[output_name(f)] = _scale(tot, 1.0 / len(vals)) if red == L.RED_MEAN else tot
+        return result
+    host = _per_traj_results(E, sim, t0, dt, save_every, nsteps)
+    per = []
+    for h in host:
+        d = {"Time": times} if savetime else {}
+        for f in outputs:
+            d[output_name(f)] = f.per_trajectory(h) if isinstance(f, Output) else f(h, None)
+        per.append(d)
+    if isinstance(reduction, SortByOutputReduction):
+        keys = per[0].keys() if per else []
+        return {k: [d[k] for d in per] for k in keys}
+    return per
+
+
+def _tree_sum(vals):
+    first = vals[0]
+    if isinstance(first, dict):
+        return {k: _tree_sum([v[k] for v in vals]) for k in first}
+    if isinstance(first, list):
+        return [_tree_sum([v[j] for v in vals]) for j in range(len(first))]
+    return np.sum(np.asarray(vals, dtype=np.float64), axis=0)
+
+
+def _scale(x, s):
+    if isinstance(x, dict):
+        return {k: _scale(v, s) for k, v in x.items()}
+    if isinstance(x, list):
+        return [_scale(v, s) for v in x]
+    return x * s

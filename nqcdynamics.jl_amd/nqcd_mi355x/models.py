@@ -1,0 +1,114 @@
+"""Model Hamiltonians understood by the device kernels (NQCModels names and defaults).
+
+Each model carries the kind code and parameter vector that nqcd_mi355x_set_model
+takes.  Parameters follow the published definitions: Tully, J. Chem. Phys. 93,
+1061 (1990) for the three Tully models; the spin-boson discretisations of
+docs/src/NQCModels/systembathmodels.md:21-27,54-66,89-96.
+"""
+import numpy as np
+
+from . import _lib as L
+
+
+class Model:
+    nstates = 2
+    ndofs = 1
+    kind = None
+
+    def params(self):
+        raise NotImplementedError
+
+    def __repr__(self):
+        return f"{type(self).__name__}({', '.join(f'{k}={v!r}' for k, v in vars(self).items())})"
+
+
+class TullyModelOne(Model):
+    """Single avoided crossing: V11 = A(1 - e^{-B|x|}) sign x, V22 = -V11, V12 = C e^{-D x^2}."""
+    kind = L.MODEL_TULLY1
+
+    def __init__(self, A=0.01, B=1.6, C=0.005, D=1.0):
+        self.A, self.B, self.C, self.D = A, B, C, D
+
+    def params(self):
+        return [self.A, self.B, self.C, self.D]
+
+
+class TullyModelTwo(Model):
+    """Dual avoided crossing: V11 = 0, V22 = -A e^{-B x^2} + E0, V12 = C e^{-D x^2}."""
+    kind = L.MODEL_TULLY2
+
+    def __init__(self, A=0.1, B=0.28, C=0.015, D=0.06, E0=0.05):
+        self.A, self.B, self.C, self.D, self.E0 = A, B, C, D, E0
+
+    def params(self):
+        return [self.A, self.B, self.C, self.D, self.E0]
+
+
+class TullyModelThree(Model):
+    """Extended coupling with reflection: V11 = A, V22 = -A, V12 = B e^{Cx} (x<0), B(2 - e^{-Cx}) (x>=0)."""
+    kind = L.MODEL_TULLY3
+
+    def __init__(self, A=6e-4, B=0.1, C=0.9):
+        self.A, self.B, self.C = A, B, C
+
+    def params(self):
+        return [self.A, self.B, self.C]
+
+
+class DoubleWellGaussian(Model):
+    """Quartic double well V11 = a x^4 - b x^2, V22 = V11 + offset, Gaussian coupling V12 = C e^{-D x^2}.
+    With nstates=1 only V11 is used (ring-polymer classical dynamics)."""
+    kind = L.MODEL_DW_GAUSSIAN
+
+    def __init__(self, a=1 / 1200, b=1 / 40, offset=0.01, C=0.005, D=1.0, nstates=2):
+        self.a, self.b, self.offset, self.C, self.D = a, b, offset, C, D
+        self.nstates = nstates
+
+    def params(self):
+        return [self.a, self.b, self.offset, self.C, self.D]
+
+
+class OhmicSpectralDensity:
+    """J(w) = (pi/2) alpha w e^{-w/wc}; w_j = -wc ln(1 - j/(N+1)), c_j = sqrt(alpha wc/(N+1)) w_j."""
+
+    def __init__(self, wc, alpha):
+        self.wc, self.alpha = wc, alpha
+
+    def discretise(self, N):
+        j = np.arange(1, N + 1)
+        w = -self.wc * np.log(1 - j / (N + 1))
+        return w, np.sqrt(self.alpha * self.wc / (N + 1)) * w
+
+
+class DebyeSpectralDensity:
+    """J(w) = 2 lambda wc w / (wc^2 + w^2); w_j = wc tan(pi/2 (1 - j/(N+1))), c_j = sqrt(2 lambda/(N+1)) w_j."""
+
+    def __init__(self, wc, lam):
+        self.wc, self.lam = wc, lam
+
+    def discretise(self, N):
+        j = np.arange(1, N + 1)
+        w = self.wc * np.tan(np.pi / 2 * (1 - j / (N + 1)))
+        return w, np.sqrt(2 * self.lam / (N + 1)) * w
+
+
+class SpinBoson(Model):
+    """[[eps + c.R, Delta], [Delta, -eps - c.R]] + R^T W^2 R / 2 with unit bath masses;
+    bath positions laid out (1, N) as in the reference."""
+    kind = L.MODEL_SPIN_BOSON
+
+    def __init__(self, density, N, eps, Delta):
+        self.density, self.N, self.eps, self.Delta = density, N, eps, Delta
+        self.omega, self.c = density.discretise(N)
+        self.ndofs = 1
+
+    @property
+    def natoms(self):
+        return self.N
+
+    def params(self):
+        return np.concatenate([[self.eps, self.Delta], self.omega, self.c])
+
+
+def nstates(model):
+    return model.nstates

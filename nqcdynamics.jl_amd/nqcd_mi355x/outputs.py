@@ -1,0 +1,102 @@
+"""Ensemble outputs (src/DynamicsOutputs.jl, src/TimeCorrelationFunctions.jl) and
+reductions (src/Ensembles/reductions.jl).
+
+Each output turns the device results of one run into the per-trajectory value the
+reference's `f(sol, i)` returns.  Mean/Sum reductions of the outputs that have a
+device reduction run on the GPU in a fixed order; the others reduce on the host.
+"""
+import numpy as np
+
+from . import _lib as L
+from .distributions import Adiabatic, Diabatic
+
+
+class Output:
+    name = None
+    device_reduction = None   # nqcd_output_kind with a device-side Mean/Sum
+
+    def per_trajectory(self, res):
+        raise NotImplementedError
+
+
+class _FnOutput(Output):
+    def __init__(self, name, fn, device_reduction=None):
+        self.name, self._fn, self.device_reduction = name, fn, device_reduction
+
+    def per_trajectory(self, res):
+        return self._fn(res)
+
+    def __repr__(self):
+        return self.name
+
+
+# OutputAdiabaticPopulation / OutputDiabaticPopulation (DynamicsOutputs.jl:186-203): one n-vector per save.
+OutputAdiabaticPopulation = _FnOutput("OutputAdiabaticPopulation", lambda r: list(r["pop_ad"]), L.OUT_ADIABATIC_POPULATION)
+OutputDiabaticPopulation = _FnOutput("OutputDiabaticPopulation", lambda r: list(r["pop_di"]), L.OUT_DIABATIC_POPULATION)
+# OutputDiscreteState (DynamicsOutputs.jl:178): active state (1-based) per save.
+OutputDiscreteState = _FnOutput("OutputDiscreteState", lambda r: list(r["save_state"] + 1))
+# OutputSurfaceHops (DynamicsOutputs.jl:235-243): accepted hops.
+OutputSurfaceHops = _FnOutput("OutputSurfaceHops", lambda r: r["nhops"])
+# OutputFinalTime (DynamicsOutputs.jl:228-229).
+OutputFinalTime = _FnOutput("OutputFinalTime", lambda r: r["t0"] + r["steps"] * r["dt"])
+# OutputFinal (DynamicsOutputs.jl:222): final dynamics variables.
+OutputFinal = _FnOutput("OutputFinal", lambda r: {"v": r["v"], "r": r["r"], "sigma": r["sigma"], "state": r["state"] + 1})
+
+
+class OutputStateResolvedScattering1D(Output):
+    """Final populations split by the sign of the first coordinate (DynamicsOutputs.jl:313-338)."""
+    name = "OutputStateResolvedScattering1D"
+
+    def __init__(self, sim, kind="adiabatic"):
+        if kind not in ("adiabatic", "diabatic"):
+            raise ValueError(f"{kind} not recognised. Only `:diabatic` or `:adiabatic` accepted.")
+        self.sim, self.kind = sim, kind
+        self.device_reduction = L.OUT_SCATTERING_1D if kind == "adiabatic" else None
+
+    def per_trajectory(self, res):
+        pops = res["final_pop_ad"] if self.kind == "adiabatic" else res["pop_di_final"]
+        x = res["r"].reshape(-1)[0]
+        z = np.zeros_like(pops)
+        return {"reflection": z if x > 0 else pops, "transmission": pops if x > 0 else z}
+
+
+class PopulationCorrelationFunction(Output):
+    """P(0) P(t)^T per save point (TimeCorrelationFunctions.jl:26-40,61-88)."""
+    name = "PopulationCorrelationFunction"
+
+    def __init__(self, sim, statetype=None):
+        self.sim = sim
+        self.statetype = statetype if statetype is not None else Diabatic()
+        diab = isinstance(self.statetype, Diabatic) or self.statetype is Diabatic
+        self.diabatic = diab
+        self.device_reduction = L.OUT_POPULATION_CORRELATION if diab else None
+
+    def per_trajectory(self, res):
+        P = res["pop_di"] if self.diabatic else res["pop_ad"]
+        return [np.outer(P[0], p) for p in P]
+
+
+# --- reductions -----------------------------------------------------------------------------
+class SortByTrajectoryReduction:
+    """One dictionary per trajectory (reductions.jl:208-209)."""
+
+
+class SortByOutputReduction:
+    """One list per output name (reductions.jl:272-297)."""
+
+
+class MeanReduction:
+    """Mean over trajectories (reductions.jl:190-206)."""
+
+
+class SumReduction:
+    """Sum over trajectories (reductions.jl:162-185)."""
+
+
+class FileReduction:
+    def __init__(self, filename):
+        raise NotImplementedError("FileReduction (HDF5) is not provided by the MI355X backend")
+
+
+def output_name(f):
+    return f.name if isinstance(f, Output) else getattr(f, "__name__", type(f).__name__)

@@ -1,0 +1,112 @@
+"""Simulation containers and dynamics methods (src/simulations.jl, src/DynamicsMethods).
+
+`Simulation[FSSH](atoms, model, rescaling="standard")` mirrors the reference's
+`Simulation{FSSH}(atoms, model; rescaling=:standard)` (fssh.jl:38-45); the
+method objects carry the options the device kernels read.
+"""
+import numpy as np
+
+from . import _lib as L
+
+_ELEMENT_MASS = {"H": 1837.4715941070515, "D": 3671.4829, "He": 7296.2993}
+
+
+class Atoms:
+    """Atom masses (NQCBase.Atoms): Atoms(2000), Atoms([1.0]*100) or Atoms("H")."""
+
+    def __init__(self, spec):
+        if isinstance(spec, str):
+            spec = [spec]
+        if np.isscalar(spec):
+            spec = [spec]
+        masses = [(_ELEMENT_MASS[s] if isinstance(s, str) else float(s)) for s in spec]
+        self.masses = np.asarray(masses, dtype=np.float64)
+
+    def __len__(self):
+        return len(self.masses)
+
+
+class Method:
+    method_code = None
+    algorithm = None
+
+
+class FSSH(Method):
+    """Fewest-switches surface hopping (fssh.jl:23-45); rescaling :standard, :vinversion or :off."""
+    method_code = L.METHOD_FSSH
+    algorithm = L.ALG_BAB_TSIT5
+
+    def __init__(self, rescaling="standard"):
+        codes = {"standard": L.RESCALE_STANDARD, "vinversion": L.RESCALE_VINVERSION, "off": L.RESCALE_OFF}
+        if rescaling not in codes:
+            raise ValueError("This mode of rescaling is not implemented")  # surface_hopping.jl:244
+        self.rescaling = rescaling
+        self.rescaling_code = codes[rescaling]
+        self.state = 1
+
+
+class Ehrenfest(Method):
+    """Mean-field Ehrenfest dynamics (ehrenfest.jl:27-41)."""
+    method_code = L.METHOD_EHRENFEST
+    algorithm = L.ALG_BAB_TSIT5
+    rescaling_code = L.RESCALE_STANDARD
+
+
+class Classical(Method):
+    """Classical molecular dynamics (classical.jl:22)."""
+    method_code = None
+
+
+class AbstractSimulation:
+    def __init__(self, method, atoms, model, temperature=0.0, precision="fp64", **method_kwargs):
+        self.method = method(**method_kwargs) if isinstance(method, type) else method
+        self.atoms = atoms
+        self.model = model
+        self.temperature = float(temperature)
+        if precision not in ("fp64", "fp32_nuclei"):
+            raise ValueError("precision must be 'fp64' or 'fp32_nuclei'")
+        self.precision = precision
+        natoms = getattr(model, "natoms", len(atoms))
+        if natoms != len(atoms):
+            raise ValueError(f"model has {natoms} atoms but Atoms has {len(atoms)}")
+
+    @property
+    def ndofs(self):
+        return self.model.ndofs
+
+    @property
+    def natoms(self):
+        return len(self.atoms)
+
+    def size(self):
+        return (self.ndofs, self.natoms)
+
+
+class _SimulationMeta(type):
+    def __getitem__(cls, method):
+        def make(atoms, model, *args, **kw):
+            return cls(method, atoms, model, *args, **kw)
+        return make
+
+
+class Simulation(AbstractSimulation, metaclass=_SimulationMeta):
+    """Simulation{Method}(atoms, model; temperature) (simulations.jl:12-44)."""
+
+    def __init__(self, method, atoms, model, **kw):
+        super().__init__(method, atoms, model, **kw)
+        self.nbeads = 1
+
+
+class RingPolymerSimulation(AbstractSimulation, metaclass=_SimulationMeta):
+    """RingPolymerSimulation{Method}(atoms, model, n_beads; temperature) (simulations.jl:46-73)."""
+
+    def __init__(self, method, atoms, model, n_beads, **kw):
+        super().__init__(method, atoms, model, **kw)
+        self.nbeads = int(n_beads)
+
+    def size(self):
+        return (self.ndofs, self.natoms, self.nbeads)
+
+
+def masses(sim):
+    return sim.atoms.masses
