@@ -56,7 +56,7 @@ def cpu_baseline(L, gpu_states, nthreads):
     """FP64 oracle (CPU restatement) on a stratified sample of the same trajectories."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    stride = 97  # every 97th trajectory id: covers every model and momentum of the grid
+    stride = 13  # every 13th trajectory id: covers every model and momentum of the grid
     ids = np.arange(0, NTRAJ, stride)
     p0 = GRID[1] + (ids // 3) % GRID[2]
     r0, v0 = O.sample_gaussian(len(ids), 1, GRID[0], (p0 / (20 * np.sqrt(2)))[:, None], (p0 / MASS)[:, None], 0.0,

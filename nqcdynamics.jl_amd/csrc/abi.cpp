@@ -71,6 +71,11 @@ int32_t alloc_results(nqcd_handle* h) {
     HIPCHK(h, h->save_di.ensure(sizeof(double) * 2 * h->nsave * n));
     HIPCHK(h, h->save_state.ensure(sizeof(int32_t) * h->nsave * n));
     HIPCHK(h, h->queue.ensure(sizeof(unsigned long long)));
+    // constant tables read by the kernels (models per slot, 1/m per flattened dof)
+    HIPCHK(h, h->models_dev.ensure(sizeof(ModelSlot) * kMaxModels));
+    HIPCHK(h, h->inv_mass_dev.ensure(sizeof(double) * D));
+    HIPCHK(h, hipMemcpyAsync(h->models_dev.p, h->models, sizeof(ModelSlot) * kMaxModels, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->inv_mass_dev.p, h->inv_mass.data(), sizeof(double) * D, hipMemcpyHostToDevice, h->stream));
     return NQCD_OK;
 }
 
@@ -89,10 +94,11 @@ Fssh2Args make_args(nqcd_handle* h, double t0, int64_t nsteps, const nqcd_term_d
     A.t0 = t0;
     A.nsteps = nsteps;
     A.key = key;
-    for (int I = 0; I < h->D && I < kMaxDof1; ++I) A.inv_mass[I] = h->inv_mass[I];
     int nm = 0;
-    for (int m = 0; m < kMaxModels; ++m) { A.models[m] = h->models[m]; if (h->model_set[m]) nm = m + 1; }
+    for (int m = 0; m < kMaxModels; ++m) if (h->model_set[m]) nm = m + 1;
     A.nmodels = nm;
+    A.inv_mass = h->inv_mass_dev.as<double>();
+    A.models = h->models_dev.as<ModelSlot>();
     A.r0 = h->r0.as<double>(); A.v0 = h->v0.as<double>(); A.sig0 = h->sig0.as<double>();
     A.state0 = h->state0_given ? h->state0.as<int32_t>() : nullptr;
     A.model_id = h->model_id_given ? h->model_id.as<int32_t>() : nullptr;
@@ -141,7 +147,7 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    DevBuf* bufs[] = {&h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath,
+    DevBuf* bufs[] = {&h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
                       &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small};
     for (DevBuf* b : bufs) b->release();
@@ -377,7 +383,8 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
             const int64_t need = (h->ntraj + block - 1) / block;
             if ((int64_t)grid > need) grid = (int)need;
             HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-            HIPCHK(h, fssh2_launch_persistent(A, h->prob.precision, grid, block, h->stream));
+            HIPCHK(h, h->args_dev.ensure(sizeof(Fssh2Args)));
+            HIPCHK(h, fssh2_launch_persistent(A, h->args_dev.as<Fssh2Args>(), h->prob.precision, grid, block, h->stream));
             HIPCHK(h, hipEventRecord(h->ev1, h->stream));
             h->stats.block = block; h->stats.grid = grid; h->stats.launches = 2;
         } else {

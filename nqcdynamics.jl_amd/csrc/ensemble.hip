@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void init_density2(int64_t ntraj, int32_t stat
     if (!diabatic) {
         a = (state == 0) ? 1.0 : 0.0; b = 1.0 - a; x = 0.0;
     } else {
-        const ModelSlot m = models[model_id ? model_id[i] : 0];
+        const ModelRegs m = model_regs(models, model_id ? model_id[i] : 0);
         double v11, v22, v12, d11, d22, d12, E0, E1, U[4];
         model1d_eval<double>(m, r0[i], v11, v22, v12, d11, d22, d12);
         eigen2(v11, v12, v22, E0, E1, U, false);

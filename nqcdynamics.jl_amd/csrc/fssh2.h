@@ -20,8 +20,8 @@ struct Fssh2Args {
     double term_value, dt, t0;
     int64_t nsteps;
     uint64_t key;
-    double inv_mass[kMaxDof1];
-    ModelSlot models[kMaxModels];
+    const double* inv_mass;     // [D] per flattened dof
+    const ModelSlot* models;    // [kMaxModels]
     // inputs
     const double* r0;      // [D][ntraj]
     const double* v0;      // [D][ntraj]
@@ -42,7 +42,9 @@ struct Fssh2Args {
     unsigned long long* queue;
 };
 
-hipError_t fssh2_launch_persistent(const Fssh2Args& A, int precision, int grid, int block, hipStream_t stream);
+// Adev: device copy of the arguments (the kernel reads them through this pointer).
+hipError_t fssh2_launch_persistent(const Fssh2Args& A, Fssh2Args* Adev, int precision, int grid, int block,
+                                   hipStream_t stream);
 hipError_t fssh2_persistent_occupancy(int precision, int block, int* blocks_per_cu);
 hipError_t fssh2_launch_plain_init(const Fssh2Args& A, double* S, int32_t* Si, hipStream_t stream);
 hipError_t fssh2_launch_plain_step(const Fssh2Args& A, double* S, int32_t* Si, hipStream_t stream);

@@ -19,24 +19,34 @@
 
 namespace nqcd {
 
+// Makes a pointer opaque to the optimiser at this point, so loads through it are
+// issued here (scalar loads of the argument block) instead of being hoisted into
+// SGPRs that stay live, and spill, across the whole persistent loop.
+template <class T>
+__device__ __forceinline__ const T* opaque(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // ---------------------------------------------------------------------------
 // Per-lane trajectory state for the persistent kernel.
 // ---------------------------------------------------------------------------
 template <int D, typename R>
 struct Lane2 {
     R r[D], v[D], k[D];
-    Sig2 s;
+    Sig2 s;             // q = sigma00 - sigma11, x + iy = sigma01
+    double tr;          // sigma00 + sigma11 (constant)
     double U[4];
-    double dEc, wc, tc;  // electronic buffer at the start of the step
+    double dEc, wc, tc; // electronic buffer at the start of the step
     double t;
-    ModelSlot m;
+    ModelRegs m;
     int64_t tid;
     int32_t st, nh, step;
 };
 
 template <int D, typename R>
-__device__ __forceinline__ void eval_cache(const Fssh2Args& A, Lane2<D, R>& L, bool track, double& E0, double& E1,
-                                           double G00[D], double G11[D], double G01[D]) {
+__device__ __forceinline__ void eval_cache(Lane2<D, R>& L, bool track, double& E0, double& E1, double G00[D],
+                                           double G11[D], double G01[D]) {
     static_assert(D == 1, "one-dimensional models");
     R v11, v22, v12, d11, d22, d12;
     model1d_eval<R>(L.m, L.r[0], v11, v22, v12, d11, d22, d12);
@@ -47,32 +57,28 @@ __device__ __forceinline__ void eval_cache(const Fssh2Args& A, Lane2<D, R>& L, b
 template <int D, typename R>
 __device__ __forceinline__ void set_accel(const Fssh2Args& A, Lane2<D, R>& L, const double G00[D],
                                           const double G11[D], const double G01[D]) {
+    const double a = 0.5 * (L.tr + L.s.q), b = 0.5 * (L.tr - L.s.q);
 #pragma unroll
     for (int I = 0; I < D; ++I) {
         double f;
         if (A.method == METHOD_FSSH) f = (L.st == 0) ? G00[I] : G11[I];
-        else f = G00[I] * L.s.a + G11[I] * L.s.b + 2.0 * G01[I] * L.s.x;
+        else f = G00[I] * a + G11[I] * b + 2.0 * G01[I] * L.s.x;
         L.k[I] = (R)(-f * A.inv_mass[I]);
     }
 }
 
 template <int D, typename R>
-__device__ __forceinline__ void save_slot(const Fssh2Args& A, const Lane2<D, R>& L, int slot) {
+__device__ __forceinline__ void save_slot(const Fssh2Args* Ap, const Lane2<D, R>& L, int slot) {
+    const Fssh2Args& A = *opaque(Ap);
     if (slot >= A.nsave) return;
     const int64_t n = A.ntraj;
     double pa0, pa1;
-    double S00, S11;
-    if (A.method == METHOD_FSSH) {
-        pa0 = (L.st == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0;
-        S00 = pa0; S11 = pa1;
-    } else {
-        pa0 = L.s.a; pa1 = L.s.b;
-        S00 = L.s.a; S11 = L.s.b;
-    }
-    // diag(U S U^T) with S01 = Re sigma_01 (fssh.jl:132-142, ehrenfest.jl:75-83)
+    if (A.method == METHOD_FSSH) { pa0 = (L.st == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0; }
+    else { pa0 = 0.5 * (L.tr + L.s.q); pa1 = 0.5 * (L.tr - L.s.q); }
+    // diag(U S U^T) with S = [[pa0, Re sigma01],[Re sigma01, pa1]] (fssh.jl:132-142, ehrenfest.jl:75-83)
     const double S01 = L.s.x;
-    const double pd0 = L.U[0] * L.U[0] * S00 + L.U[1] * L.U[1] * S11 + 2.0 * L.U[0] * L.U[1] * S01;
-    const double pd1 = L.U[2] * L.U[2] * S00 + L.U[3] * L.U[3] * S11 + 2.0 * L.U[2] * L.U[3] * S01;
+    const double pd0 = L.U[0] * L.U[0] * pa0 + L.U[1] * L.U[1] * pa1 + 2.0 * L.U[0] * L.U[1] * S01;
+    const double pd1 = L.U[2] * L.U[2] * pa0 + L.U[3] * L.U[3] * pa1 + 2.0 * L.U[2] * L.U[3] * S01;
     if (A.save_pop_ad) {
         A.save_pop_ad[((int64_t)slot * 2 + 0) * n + L.tid] = pa0;
         A.save_pop_ad[((int64_t)slot * 2 + 1) * n + L.tid] = pa1;
@@ -85,7 +91,8 @@ __device__ __forceinline__ void save_slot(const Fssh2Args& A, const Lane2<D, R>&
 }
 
 template <int D, typename R>
-__device__ __forceinline__ void load_traj(const Fssh2Args& A, Lane2<D, R>& L, int64_t tid) {
+__device__ __forceinline__ void load_traj(const Fssh2Args* Ap, Lane2<D, R>& L, int64_t tid) {
+    const Fssh2Args& A = *opaque(Ap);
     const int64_t n = A.ntraj;
     L.tid = tid;
 #pragma unroll
@@ -93,39 +100,40 @@ __device__ __forceinline__ void load_traj(const Fssh2Args& A, Lane2<D, R>& L, in
         L.r[I] = (R)A.r0[(int64_t)I * n + tid];
         L.v[I] = (R)A.v0[(int64_t)I * n + tid];
     }
-    L.s.a = A.sig0[0 * n + tid];
-    L.s.b = A.sig0[1 * n + tid];
+    const double a = A.sig0[0 * n + tid], b = A.sig0[1 * n + tid];
+    L.tr = a + b;
+    L.s.q = a - b;
     L.s.x = A.sig0[2 * n + tid];
     L.s.y = A.sig0[3 * n + tid];
     const int mid = A.model_id ? A.model_id[tid] : 0;
-    L.m = A.models[mid];
+    L.m = model_regs(A.models, mid);
     int st = A.state0 ? A.state0[tid] : -1;
     if (A.method == METHOD_FSSH && st < 0) {  // fssh.jl:54: sample(Weights(diag(sigma)))
-        const double tot = L.s.a + L.s.b;
-        const double u = philox_uniform((uint64_t)tid, 0u, STREAM_STATE, A.key) * tot;
-        st = (u < L.s.a) ? 0 : 1;
+        const double u = philox_uniform((uint64_t)tid, 0u, STREAM_STATE, A.key) * (a + b);
+        st = (u < a) ? 0 : 1;
     }
     L.st = st;
     double E0, E1, G00[D], G11[D], G01[D];
-    eval_cache(A, L, false, E0, E1, G00, G11, G01);  // initialize! (bab_electronics.jl:48-59)
+    eval_cache(L, false, E0, E1, G00, G11, G01);  // initialize! (bab_electronics.jl:48-59)
     set_accel(A, L, G00, G11, G01);
     L.dEc = 0.0; L.wc = 0.0; L.tc = 0.0;  // zero buffers stamped t = 0 (electronic_dynamics.jl:118-127)
     L.t = A.t0;
     L.nh = 0;
     L.step = 0;
-    save_slot(A, L, 0);
+    save_slot(Ap, L, 0);
 }
 
 template <int D, typename R>
-__device__ __forceinline__ void finish_traj(const Fssh2Args& A, const Lane2<D, R>& L) {
+__device__ __forceinline__ void finish_traj(const Fssh2Args* Ap, const Lane2<D, R>& L) {
+    const Fssh2Args& A = *opaque(Ap);
     const int64_t n = A.ntraj;
 #pragma unroll
     for (int I = 0; I < D; ++I) {
         A.r[(int64_t)I * n + L.tid] = (double)L.r[I];
         A.v[(int64_t)I * n + L.tid] = (double)L.v[I];
     }
-    A.sig[0 * n + L.tid] = L.s.a;
-    A.sig[1 * n + L.tid] = L.s.b;
+    A.sig[0 * n + L.tid] = 0.5 * (L.tr + L.s.q);
+    A.sig[1 * n + L.tid] = 0.5 * (L.tr - L.s.q);
     A.sig[2 * n + L.tid] = L.s.x;
     A.sig[3 * n + L.tid] = L.s.y;
     A.state[L.tid] = L.st;
@@ -136,13 +144,18 @@ __device__ __forceinline__ void finish_traj(const Fssh2Args& A, const Lane2<D, R
 template <int D, typename R>
 __device__ __forceinline__ bool is_done(const Fssh2Args& A, const Lane2<D, R>& L) {
     if (L.step >= A.nsteps) return true;
-    if (A.term_kind == 1 && fabs((double)L.r[A.term_dof]) > A.term_value) return true;
+    if (A.term_kind == 1) {
+        double x = 0.0;  // select, not a runtime index, so the lane state stays in registers
+#pragma unroll
+        for (int I = 0; I < D; ++I) x = (I == A.term_dof) ? (double)L.r[I] : x;
+        if (fabs(x) > A.term_value) return true;
+    }
     return false;
 }
 
 // One nuclear step + hopping callback.
 template <int D, typename R>
-__device__ __forceinline__ void step_traj(const Fssh2Args& A, Lane2<D, R>& L) {
+__device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& A, Lane2<D, R>& L) {
     const R hdt = (R)(0.5 * A.dt), dtR = (R)A.dt;
     R vt[D];
 #pragma unroll
@@ -150,46 +163,53 @@ __device__ __forceinline__ void step_traj(const Fssh2Args& A, Lane2<D, R>& L) {
 #pragma unroll
     for (int I = 0; I < D; ++I) L.r[I] = L.r[I] + dtR * vt[I];           // step_A!
     double E0, E1, G00[D], G11[D], G01[D];
-    eval_cache(A, L, true, E0, E1, G00, G11, G01);                       // update_cache!
+    eval_cache(L, true, E0, E1, G00, G11, G01);                          // update_cache!
     set_accel(A, L, G00, G11, G01);                                      // sigma of the step start
 #pragma unroll
     for (int I = 0; I < D; ++I) L.v[I] = vt[I] + hdt * L.k[I];           // step_B!
     const double gap = E1 - E0;
+    const double inv_gap = 1.0 / gap;
     double d01[D];
     double w = 0.0;
 #pragma unroll
-    for (int I = 0; I < D; ++I) { d01[I] = G01[I] / gap; w += d01[I] * (double)L.v[I]; }
+    for (int I = 0; I < D; ++I) { d01[I] = G01[I] * inv_gap; w += d01[I] * (double)L.v[I]; }
     const double tn = L.t + A.dt;
+    const double dEn = E0 - E1;
+    const double inv = 1.0 / (tn - L.tc);
     Interp2 P;
-    P.dEc = L.dEc; P.dEn = E0 - E1; P.wc = L.wc; P.wn = w; P.tc = L.tc; P.inv = 1.0 / (tn - L.tc);
-    propagate2(P, L.s, L.t, A.dt, A.nsub);
-    L.dEc = E0 - E1; L.wc = w; L.tc = tn;
+    P.dE0 = L.dEc; P.dE1 = (dEn - L.dEc) * inv; P.w0 = L.wc; P.w1 = (w - L.wc) * inv;
+    propagate2(P, L.s, L.t, L.tc, A.dt, A.nsub);
+    L.dEc = dEn; L.wc = w; L.tc = tn;
     L.t = tn;
     if (A.method == METHOD_FSSH) {
         const int s = L.st;
-        const double ratio = L.s.x / ((s == 0) ? L.s.a : L.s.b);  // Re(sigma_ms / sigma_ss)
+        const double sss = 0.5 * (L.tr + ((s == 0) ? L.s.q : -L.s.q));
+        const double ratio = L.s.x / sss;  // Re(sigma_ms / sigma_ss)
         double p = 0.0;
 #pragma unroll
         for (int I = 0; I < D; ++I) {
             const double dsm = (s == 0) ? d01[I] : -d01[I];
             p += 2.0 * (double)L.v[I] * ratio * dsm * A.dt;
         }
-        p = fmin(fmax(p, 0.0), 1.0);
-        const double rnd = philox_uniform((uint64_t)L.tid, (uint32_t)L.step, STREAM_HOP, A.key);
-        if (p > rnd) {
-            double dms[D];
+        if (p > 0.0) {  // a uniform in [0,1) can only fall below a positive probability
+            p = fmin(p, 1.0);
+            const double rnd = philox_uniform((uint64_t)L.tid, (uint32_t)L.step, STREAM_HOP, A.key);
+            if (p > rnd) {
+                double dms[D];
 #pragma unroll
-            for (int I = 0; I < D; ++I) dms[I] = (s == 0) ? -d01[I] : d01[I];  // coupling[new, old]
-            const double dE = (s == 0) ? (E1 - E0) : (E0 - E1);
-            if (rescale_velocity<D, R>(A.rescaling, L.v, A.inv_mass, dms, dE)) { L.st = 1 - s; ++L.nh; }
+                for (int I = 0; I < D; ++I) dms[I] = (s == 0) ? -d01[I] : d01[I];  // coupling[new, old]
+                const double dE = (s == 0) ? gap : -gap;
+                if (rescale_velocity<D, R>(A.rescaling, L.v, A.inv_mass, dms, dE)) { L.st = 1 - s; ++L.nh; }
+            }
         }
     }
     ++L.step;
-    if (A.save_every > 0 && (L.step % A.save_every) == 0) save_slot(A, L, L.step / A.save_every);
+    if (A.save_every > 0 && (L.step % A.save_every) == 0) save_slot(Ap, L, L.step / A.save_every);
 }
 
 template <int D, typename R>
-__global__ __launch_bounds__(256) void fssh2_persistent(const Fssh2Args A) {
+__device__ __forceinline__ void persistent_body(const Fssh2Args* __restrict__ Ap) {
+    const Fssh2Args& A = *Ap;
     const int lane = __lane_id();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     Lane2<D, R> L;
@@ -200,13 +220,14 @@ __global__ __launch_bounds__(256) void fssh2_persistent(const Fssh2Args A) {
             const int cnt = __popcll(want);
             const int leader = __ffsll((unsigned long long)want) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(A.queue, (unsigned long long)cnt);
+            if (lane == leader) base = atomicAdd(opaque(Ap)->queue, (unsigned long long)cnt);
             base = __shfl(base, leader);
             if (!active && !exhausted) {
                 const unsigned long long q = base + (unsigned long long)__popcll(want & below);
                 if (q < (unsigned long long)A.ntraj) {
-                    load_traj(A, L, A.order ? (int64_t)A.order[q] : (int64_t)q);
-                    if (is_done(A, L)) finish_traj(A, L); else active = true;
+                    const int32_t* order = opaque(Ap)->order;
+                    load_traj(Ap, L, order ? (int64_t)order[q] : (int64_t)q);
+                    if (is_done(A, L)) finish_traj(Ap, L); else active = true;
                 } else {
                     exhausted = true;
                 }
@@ -214,10 +235,21 @@ __global__ __launch_bounds__(256) void fssh2_persistent(const Fssh2Args A) {
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            step_traj(A, L);
-            if (is_done(A, L)) { finish_traj(A, L); active = false; }
+            step_traj(Ap, A, L);
+            if (is_done(A, L)) { finish_traj(Ap, L); active = false; }
         }
     }
+}
+
+// FP32 nuclei: fits 128 VGPRs, so four waves per SIMD.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void fssh2_persistent_f32(
+    const Fssh2Args* __restrict__ Ap) {
+    persistent_body<1, float>(Ap);
+}
+
+// FP64 (parity) path: three waves per SIMD without spilling.
+__global__ __launch_bounds__(256) void fssh2_persistent_f64(const Fssh2Args* __restrict__ Ap) {
+    persistent_body<1, double>(Ap);
 }
 
 // ---------------------------------------------------------------------------
@@ -259,7 +291,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_init(const Fssh2Args A, doubl
         st = (philox_uniform((uint64_t)tid, 0u, STREAM_STATE, A.key) * tot < s.re[0]) ? 0 : 1;
     }
     double v11, v22, v12, d11, d22, d12, E0, E1, U[4], G00, G11, G01;
-    model1d_eval<double>(A.models[mid], r, v11, v22, v12, d11, d22, d12);
+    model1d_eval<double>(model_regs(A.models, mid), r, v11, v22, v12, d11, d22, d12);
     eigen2(v11, v12, v22, E0, E1, U, false);
     rotate_derivative(U, d11, d22, d12, G00, G11, G01);
     const double f = (A.method == METHOD_FSSH) ? ((st == 0) ? G00 : G11) : (G00 * s.re[0] + G11 * s.re[3] + 2.0 * G01 * s.re[1]);
@@ -280,7 +312,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, doubl
     const int64_t n = A.ntraj;
     if (Si[PI_DONE * n + tid]) return;
     const int mid = A.model_id ? A.model_id[tid] : 0;
-    const ModelSlot& M = A.models[mid];
+    const ModelRegs M = model_regs(A.models, mid);
     double r = S[PL_R * n + tid], v = S[PL_V * n + tid], k = S[PL_K * n + tid];
     C2 s;
     for (int J = 0; J < 4; ++J) { s.re[J] = S[(PL_SRE + J) * n + tid]; s.im[J] = S[(PL_SIM + J) * n + tid]; }
@@ -355,22 +387,22 @@ __global__ void queue_reset(unsigned long long* q) { *q = 0ull; }
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-template <int D, typename R>
-static hipError_t launch_persistent_t(const Fssh2Args& A, int grid, int block, hipStream_t stream) {
+hipError_t fssh2_launch_persistent(const Fssh2Args& A, Fssh2Args* Adev, int precision, int grid, int block,
+                                   hipStream_t stream) {
+    hipError_t e = hipMemcpyAsync(Adev, &A, sizeof(Fssh2Args), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(queue_reset, dim3(1), dim3(1), 0, stream, A.queue);
-    hipLaunchKernelGGL((fssh2_persistent<D, R>), dim3(grid), dim3(block), 0, stream, A);
+    if (precision == 0)
+        hipLaunchKernelGGL(fssh2_persistent_f64, dim3(grid), dim3(block), 0, stream, (const Fssh2Args*)Adev);
+    else
+        hipLaunchKernelGGL(fssh2_persistent_f32, dim3(grid), dim3(block), 0, stream, (const Fssh2Args*)Adev);
     return hipGetLastError();
-}
-
-hipError_t fssh2_launch_persistent(const Fssh2Args& A, int precision, int grid, int block, hipStream_t stream) {
-    if (precision == 0) return launch_persistent_t<1, double>(A, grid, block, stream);
-    return launch_persistent_t<1, float>(A, grid, block, stream);
 }
 
 hipError_t fssh2_persistent_occupancy(int precision, int block, int* blocks_per_cu) {
     if (precision == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fssh2_persistent<1, double>, block, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fssh2_persistent<1, float>, block, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fssh2_persistent_f64, block, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fssh2_persistent_f32, block, 0);
 }
 
 hipError_t fssh2_launch_plain_init(const Fssh2Args& A, double* S, int32_t* Si, hipStream_t stream) {

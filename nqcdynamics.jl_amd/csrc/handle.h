@@ -51,10 +51,13 @@ struct nqcd_handle {
     // spin-boson bath (slot 0 only)
     std::vector<double> sb_omega, sb_c;
     double sb_eps = 0.0, sb_delta = 0.0;
+    int32_t sb_state = 0;
+    bool sb_diabatic = false;
+    uint64_t sb_seed = 0;
     int64_t ntraj = 0;
     bool have_initial = false, have_results = false, sb_initial = false;
     int32_t nsave = 1, save_every = 0;
-    nqcd::DevBuf r0, v0, sig0, state0, model_id, order, models_dev, bath;
+    nqcd::DevBuf r0, v0, sig0, state0, model_id, order, models_dev, bath, inv_mass_dev, args_dev;
     nqcd::DevBuf r, v, sig, state, steps, nhops, save_ad, save_di, save_state;
     nqcd::DevBuf queue, plainS, plainSi, tmp, partial, small;
     bool state0_given = false, model_id_given = false, order_given = false;

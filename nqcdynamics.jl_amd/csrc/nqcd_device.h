@@ -16,13 +16,32 @@ namespace nqcd {
 // ---------------------------------------------------------------------------
 enum { STREAM_INIT = 0, STREAM_HOP = 1, STREAM_STATE = 2 };
 
+// Round-key bump on the scalar unit.  The key is wavefront-uniform; computing the
+// schedule in place (instead of letting it be hoisted into ten SGPR pairs that are
+// later spilled to VGPR lanes) keeps the round-key work on the co-issued SALU.
+__device__ __forceinline__ uint32_t key_bump0(uint32_t k) {
+    uint32_t r;
+    k = __builtin_amdgcn_readfirstlane(k);
+    asm volatile("s_add_u32 %0, %1, 0x9e3779b9" : "=s"(r) : "s"(k) : "scc");
+    return r;
+}
+__device__ __forceinline__ uint32_t key_bump1(uint32_t k) {
+    uint32_t r;
+    k = __builtin_amdgcn_readfirstlane(k);
+    asm volatile("s_add_u32 %0, %1, 0xbb67ae85" : "=s"(r) : "s"(k) : "scc");
+    return r;
+}
+
+// k0, k1 must be wavefront-uniform (the run's Philox key).
 __device__ __forceinline__ void philox4x32_10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
                                               uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
-        if (round > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        if (round > 0) { k0 = key_bump0(k0); k1 = key_bump1(k1); }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
@@ -74,8 +93,30 @@ struct ModelSlot {
     double p[8];
 };
 
+// A model slot held in registers (one per lane): kind and the first five parameters.
+struct ModelRegs {
+    int32_t kind;
+    double p[5];
+};
+
+__device__ __forceinline__ ModelRegs model_regs(const ModelSlot* slots, int mid) {
+    ModelRegs m;
+    m.kind = slots[mid].kind;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) m.p[i] = slots[mid].p[i];
+    return m;
+}
+
+__device__ __forceinline__ ModelRegs model_regs(const ModelSlot& s) {
+    ModelRegs m;
+    m.kind = s.kind;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) m.p[i] = s.p[i];
+    return m;
+}
+
 template <typename R>
-__device__ __forceinline__ void model1d_eval(const ModelSlot& m, R x, R& v11, R& v22, R& v12,
+__device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R& v22, R& v12,
                                              R& d11, R& d22, R& d12) {
     switch (m.kind) {
     case MODEL_TULLY1: {
@@ -181,73 +222,65 @@ constexpr double a71 = 0.09646076681806523, a72 = 0.01, a73 = 0.4798896504144996
 
 // Linear interpolation of the electronic parameters between the two ends of the
 // nuclear step (DoubleBuffer, electronic_dynamics.jl:15-79): dE = E0 - E1 and
-// w = (v.d)_{01}.
+// w = (v.d)_{01}, each value(tau) = v0 + v1 * (tau - tc).
 struct Interp2 {
-    double dEc, dEn, wc, wn, tc, inv;
+    double dE0, dE1, w0, w1;
 };
 
-// Hermitian 2x2 density sigma = [[a, x+iy],[x-iy, b]] under V = [[E0, -iw],[iw, E1]]:
-// d sigma/dt = -i[V, sigma] written out for the four real unknowns.
+// Hermitian 2x2 density sigma = [[a, x+iy],[x-iy, b]] carried as q = a - b, x, y with the
+// trace a + b held fixed.  Under V = [[E0, -iw],[iw, E1]], d sigma/dt = -i[V, sigma] reads
+//   dq/dt = -4 w x,   dx/dt = dE y + w q,   dy/dt = -dE x,   dE = E0 - E1.
 struct Sig2 {
-    double a, b, x, y;
+    double q, x, y;
 };
 
-__device__ __forceinline__ Sig2 rhs2(const Interp2& P, double tau, const Sig2& s) {
-    const double loc = (tau - P.tc) * P.inv;
-    const double dE = P.dEc + (P.dEn - P.dEc) * loc;
-    const double w = P.wc + (P.wn - P.wc) * loc;
-    const double wx2 = 2.0 * w * s.x;
-    Sig2 d;
-    d.a = -wx2;
-    d.b = wx2;
-    d.x = dE * s.y + w * (s.a - s.b);
-    d.y = -dE * s.x;
-    return d;
-}
-
-#define NQCD_S2_COMB(out, base, h, EXPR_A, EXPR_B, EXPR_X, EXPR_Y) \
-    out.a = base.a + (h) * (EXPR_A);                                 \
-    out.b = base.b + (h) * (EXPR_B);                                 \
-    out.x = base.x + (h) * (EXPR_X);                                 \
-    out.y = base.y + (h) * (EXPR_Y);
-
-__device__ __forceinline__ void tsit5_step2(const Interp2& P, Sig2& u, double tau, double h) {
+// One Tsit5 step of size h from tau (s0 = tau - tc).  The stage derivatives are carried
+// pre-multiplied by h (K = h f), so a stage is u + sum_j a_ij K_j.
+__device__ __forceinline__ void tsit5_step2(const Interp2& P, Sig2& u, double s0, double h) {
     using namespace ts5;
-    Sig2 t;
-    const Sig2 k1 = rhs2(P, tau, u);
-    NQCD_S2_COMB(t, u, h, a21 * k1.a, a21 * k1.b, a21 * k1.x, a21 * k1.y);
-    const Sig2 k2 = rhs2(P, tau + c1 * h, t);
-    NQCD_S2_COMB(t, u, h, a31 * k1.a + a32 * k2.a, a31 * k1.b + a32 * k2.b, a31 * k1.x + a32 * k2.x,
-                 a31 * k1.y + a32 * k2.y);
-    const Sig2 k3 = rhs2(P, tau + c2 * h, t);
-    NQCD_S2_COMB(t, u, h, a41 * k1.a + a42 * k2.a + a43 * k3.a, a41 * k1.b + a42 * k2.b + a43 * k3.b,
-                 a41 * k1.x + a42 * k2.x + a43 * k3.x, a41 * k1.y + a42 * k2.y + a43 * k3.y);
-    const Sig2 k4 = rhs2(P, tau + c3 * h, t);
-    NQCD_S2_COMB(t, u, h, a51 * k1.a + a52 * k2.a + a53 * k3.a + a54 * k4.a,
-                 a51 * k1.b + a52 * k2.b + a53 * k3.b + a54 * k4.b,
-                 a51 * k1.x + a52 * k2.x + a53 * k3.x + a54 * k4.x,
-                 a51 * k1.y + a52 * k2.y + a53 * k3.y + a54 * k4.y);
-    const Sig2 k5 = rhs2(P, tau + c4 * h, t);
-    NQCD_S2_COMB(t, u, h, a61 * k1.a + a62 * k2.a + a63 * k3.a + a64 * k4.a + a65 * k5.a,
-                 a61 * k1.b + a62 * k2.b + a63 * k3.b + a64 * k4.b + a65 * k5.b,
-                 a61 * k1.x + a62 * k2.x + a63 * k3.x + a64 * k4.x + a65 * k5.x,
-                 a61 * k1.y + a62 * k2.y + a63 * k3.y + a64 * k4.y + a65 * k5.y);
-    const Sig2 k6 = rhs2(P, tau + h, t);
-    NQCD_S2_COMB(u, u, h, a71 * k1.a + a72 * k2.a + a73 * k3.a + a74 * k4.a + a75 * k5.a + a76 * k6.a,
-                 a71 * k1.b + a72 * k2.b + a73 * k3.b + a74 * k4.b + a75 * k5.b + a76 * k6.b,
-                 a71 * k1.x + a72 * k2.x + a73 * k3.x + a74 * k4.x + a75 * k5.x + a76 * k6.x,
-                 a71 * k1.y + a72 * k2.y + a73 * k3.y + a74 * k4.y + a75 * k5.y + a76 * k6.y);
+    const double he0 = h * (P.dE0 + P.dE1 * s0), he1 = (h * h) * P.dE1;
+    const double hg0 = h * (P.w0 + P.w1 * s0), hg1 = (h * h) * P.w1;
+#define NQCD_K(c, Q, X, Y, KQ, KX, KY)             \
+    {                                              \
+        const double e = fma(he1, (c), he0);       \
+        const double g = fma(hg1, (c), hg0);       \
+        KQ = -4.0 * (g * (X));                     \
+        KX = fma(e, (Y), g * (Q));                 \
+        KY = -(e * (X));                           \
+    }
+    double q1, x1, y1, q2, x2, y2, q3, x3, y3, q4, x4, y4, q5, x5, y5, q6, x6, y6;
+    q1 = -4.0 * (hg0 * u.x); x1 = fma(he0, u.y, hg0 * u.q); y1 = -(he0 * u.x);
+    double tq = fma(a21, q1, u.q), tx = fma(a21, x1, u.x), ty = fma(a21, y1, u.y);
+    NQCD_K(c1, tq, tx, ty, q2, x2, y2);
+    tq = fma(a32, q2, fma(a31, q1, u.q)); tx = fma(a32, x2, fma(a31, x1, u.x)); ty = fma(a32, y2, fma(a31, y1, u.y));
+    NQCD_K(c2, tq, tx, ty, q3, x3, y3);
+    tq = fma(a43, q3, fma(a42, q2, fma(a41, q1, u.q)));
+    tx = fma(a43, x3, fma(a42, x2, fma(a41, x1, u.x)));
+    ty = fma(a43, y3, fma(a42, y2, fma(a41, y1, u.y)));
+    NQCD_K(c3, tq, tx, ty, q4, x4, y4);
+    tq = fma(a54, q4, fma(a53, q3, fma(a52, q2, fma(a51, q1, u.q))));
+    tx = fma(a54, x4, fma(a53, x3, fma(a52, x2, fma(a51, x1, u.x))));
+    ty = fma(a54, y4, fma(a53, y3, fma(a52, y2, fma(a51, y1, u.y))));
+    NQCD_K(c4, tq, tx, ty, q5, x5, y5);
+    tq = fma(a65, q5, fma(a64, q4, fma(a63, q3, fma(a62, q2, fma(a61, q1, u.q)))));
+    tx = fma(a65, x5, fma(a64, x4, fma(a63, x3, fma(a62, x2, fma(a61, x1, u.x)))));
+    ty = fma(a65, y5, fma(a64, y4, fma(a63, y3, fma(a62, y2, fma(a61, y1, u.y)))));
+    NQCD_K(1.0, tq, tx, ty, q6, x6, y6);
+    u.q = fma(a76, q6, fma(a75, q5, fma(a74, q4, fma(a73, q3, fma(a72, q2, fma(a71, q1, u.q))))));
+    u.x = fma(a76, x6, fma(a75, x5, fma(a74, x4, fma(a73, x3, fma(a72, x2, fma(a71, x1, u.x))))));
+    u.y = fma(a76, y6, fma(a75, y5, fma(a74, y4, fma(a73, y3, fma(a72, y2, fma(a71, y1, u.y))))));
+#undef NQCD_K
 }
 
 // step!(electronic_integrator, dt, true) from time t: nsub sub-steps of dt/nsub,
 // each clipped so the last lands on t + dt (bab_electronics.jl:40-43,87-90).
-__device__ __forceinline__ void propagate2(const Interp2& P, Sig2& u, double t, double dt, int nsub) {
+__device__ __forceinline__ void propagate2(const Interp2& P, Sig2& u, double t, double tc, double dt, int nsub) {
     const double h = dt / (double)nsub;
     const double tend = t + dt;
     double tau = t;
     for (int s = 0; s < nsub; ++s) {
         const double hs = fmin(h, tend - tau);
-        tsit5_step2(P, u, tau, hs);
+        tsit5_step2(P, u, tau - tc, hs);
         tau = (s == nsub - 1) ? tend : tau + hs;
     }
 }
