@@ -126,9 +126,14 @@ enum nqcd_sampler_kind {
     /* Mixed Tully ensemble: model_id = traj % n_models, p0 = p_lo + ((traj / n_models) % n_p),
      * x ~ N(x0, p0/(20 sqrt 2)), v = p0/m.  params = [x0, p_lo, n_p]. */
     NQCD_SAMPLER_TULLY_GRID = 1,
-    /* Harmonic Wigner bath (PositionHarmonicWigner/VelocityHarmonicWigner):
-     * params = [beta, mass]; frequencies from the spin-boson model. */
-    NQCD_SAMPLER_HARMONIC_WIGNER = 2
+    /* Gaussian bath of a system-bath model (PositionHarmonicWigner/VelocityHarmonicWigner
+     * widths): params = [mu_r(nb), sd_r(nb), mu_v(nb), sd_v(nb)]; counters as GAUSSIAN. */
+    NQCD_SAMPLER_HARMONIC_WIGNER = 2,
+    /* Thermal ring polymers in the harmonic approximation of a well: params = [x0, w_well, beta];
+     * normal mode k ~ N(sqrt(P) x0 delta_k0, 1/(beta_P m (w_k^2 + w_well^2))), velocities
+     * ~ N(0, 1/(beta_P m)), beta_P = beta/P (stands in for the PIMC sampler of
+     * src/InitialConditions/MetropolisHastings.jl). */
+    NQCD_SAMPLER_RING_POLYMER_THERMAL = 3
 };
 typedef struct {
     int32_t kind;
@@ -146,7 +151,8 @@ enum nqcd_output_kind {
     NQCD_OUT_DIABATIC_POPULATION = 1,    /* per save slot, nstates   (DynamicsOutputs.jl:186-189) */
     NQCD_OUT_SCATTERING_1D = 2,          /* final, [reflection(n), transmission(n)] (DynamicsOutputs.jl:313-338) */
     NQCD_OUT_POPULATION_CORRELATION = 3, /* per save slot, n x n P(0)P(t)^T diabatic (TimeCorrelationFunctions.jl:61-88) */
-    NQCD_OUT_FINAL_ADIABATIC_POPULATION = 4 /* final, nstates */
+    NQCD_OUT_FINAL_ADIABATIC_POPULATION = 4, /* final, nstates */
+    NQCD_OUT_CENTROID_POSITION = 5       /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
 };
 /* Reductions (src/Ensembles/reductions.jl:8-52). */
 enum nqcd_reduction_kind { NQCD_RED_MEAN = 0, NQCD_RED_SUM = 1 };
@@ -162,7 +168,8 @@ enum nqcd_field {
     NQCD_FIELD_NHOPS = 6,        /* int32 (ntraj) accepted hops (OutputSurfaceHops)            */
     NQCD_FIELD_SAVE_POP_AD = 7,  /* double (n, nsave, ntraj) adiabatic population per save     */
     NQCD_FIELD_SAVE_POP_DI = 8,  /* double (n, nsave, ntraj) diabatic population per save      */
-    NQCD_FIELD_SAVE_STATE = 9    /* int32 (nsave, ntraj) active state per save                 */
+    NQCD_FIELD_SAVE_STATE = 9,   /* int32 (nsave, ntraj) active state per save                 */
+    NQCD_FIELD_SAVE_CENTROID = 10 /* double (nsave, ntraj) ring-polymer centroid per save       */
 };
 
 typedef struct {

@@ -19,6 +19,7 @@
 #include "fssh2.h"
 #include "handle.h"
 #include "spin_boson.h"
+#include "ring_polymer.h"
 
 using namespace nqcd;
 
@@ -45,6 +46,14 @@ bool is_spin_boson(const nqcd_handle* h) {
            h->prob.algorithm == NQCD_ALG_BAB_TSIT5 && h->prob.n_states == 2 && h->model_set[0] &&
            h->models[0].kind == MODEL_SPIN_BOSON;
 }
+
+bool is_rp(const nqcd_handle* h) {
+    return h->prob.method == NQCD_METHOD_RPMD || h->prob.method == NQCD_METHOD_RPSH ||
+           h->prob.method == NQCD_METHOD_RP_EHRENFEST;
+}
+
+// rows of the per-trajectory position / velocity arrays (dofs x atoms x beads)
+int64_t nqcd_rows(const nqcd_handle* h) { return (int64_t)h->D * (is_rp(h) ? h->prob.nbeads : 1); }
 
 int nsub_of(const nqcd_handle* h) { return h->prob.electronic_substeps > 0 ? h->prob.electronic_substeps : 5; }
 
@@ -149,7 +158,7 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
     (void)hipStreamSynchronize(h->stream);
     DevBuf* bufs[] = {&h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
-                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small};
+                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid};
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(h->ev0);
     (void)hipEventDestroy(h->ev1);
@@ -167,9 +176,11 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     if (d->precision < 0 || d->precision > 1) return fail(h, NQCD_ERR_ARG, "unknown precision");
     if (d->schedule < 0 || d->schedule > 1) return fail(h, NQCD_ERR_ARG, "unknown schedule");
     const bool sh_or_eh = d->method == NQCD_METHOD_FSSH || d->method == NQCD_METHOD_EHRENFEST;
-    if (!(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states == 2))
+    const bool rp = d->method == NQCD_METHOD_RPMD || d->method == NQCD_METHOD_RPSH || d->method == NQCD_METHOD_RP_EHRENFEST;
+    if (!rp && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states == 2))
         return fail(h, NQCD_ERR_UNSUPPORTED,
-                    "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} with two states and BABwithTsit5 in this build");
+                    "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (two states, BABwithTsit5) and "
+                    "RingPolymerSimulation{Classical|FSSH|Ehrenfest} (BCB / BCBwithTsit5)");
     h->prob = *d;
     h->masses.assign(d->masses, d->masses + d->natoms);
     for (double m : h->masses)
@@ -178,6 +189,10 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     h->inv_mass.resize(h->D);
     for (int I = 0; I < h->D; ++I) h->inv_mass[I] = 1.0 / h->masses[I / d->ndofs];
     h->prob.masses = nullptr;
+    if (rp) {
+        int32_t rc = rp_check_problem(h);
+        if (rc != NQCD_OK) return rc;
+    }
     h->have_problem = true;
     h->have_initial = h->have_results = false;
     return NQCD_OK;
@@ -195,6 +210,7 @@ int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t kind, cons
     case NQCD_MODEL_TULLY2: need = 5; break;
     case NQCD_MODEL_TULLY3: need = 3; break;
     case NQCD_MODEL_DW_GAUSSIAN: need = 5; break;
+    case NQCD_MODEL_HARMONIC: need = 3; break;
     case NQCD_MODEL_SPIN_BOSON: {
         if (slot != 0) return fail(h, NQCD_ERR_UNSUPPORTED, "spin-boson model must use slot 0");
         if (n < 4 || (n - 2) % 2 != 0) return fail(h, NQCD_ERR_ARG, "spin-boson params: eps, Delta, omega[nb], c[nb]");
@@ -222,7 +238,8 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
     if (!h) return fail(h, NQCD_ERR_ARG, "null handle");
     if (!h->have_problem) return fail(h, NQCD_ERR_STATE, "set_problem must be called first");
     if (ntraj < 1 || ntraj > (int64_t)1 << 31) return fail(h, NQCD_ERR_ARG, "ntraj out of range");
-    if (!r || !v || !sig_re || !sig_im) return fail(h, NQCD_ERR_ARG, "null input array");
+    if (!r || !v || ((!sig_re || !sig_im) && h->prob.method != NQCD_METHOD_RPMD))
+        return fail(h, NQCD_ERR_ARG, "null input array");
     const int D = h->D, n = h->prob.n_states;
     if (model_id)
         for (int64_t i = 0; i < ntraj; ++i)
@@ -232,6 +249,12 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
     if (state)
         for (int64_t i = 0; i < ntraj; ++i)
             if (state[i] < -1 || state[i] >= n) return fail(h, NQCD_ERR_ARG, "state out of range");
+    if (is_rp(h)) {
+        HIPCHK(h, hipSetDevice(h->device));
+        int32_t rc = rp_load_initial(h, ntraj, r, v, sig_re, sig_im, state);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     h->ntraj = ntraj;
     std::vector<double> tr((size_t)D * ntraj), tv((size_t)D * ntraj), ts((size_t)4 * ntraj);
     for (int64_t i = 0; i < ntraj; ++i)
@@ -285,6 +308,11 @@ int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S,
     const int64_t ntraj = S->ntraj;
     const int D = h->D;
     HIPCHK(h, hipSetDevice(h->device));
+    if (is_rp(h)) {
+        int32_t rc = rp_sample_initial(h, S, seed);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     if (S->kind == NQCD_SAMPLER_HARMONIC_WIGNER) {
         if (!is_spin_boson(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "harmonic Wigner sampling needs the spin-boson model");
         int32_t rc = sb_sample_initial(h, S, seed);
@@ -367,7 +395,10 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
     HIPCHK(h, hipSetDevice(h->device));
     const auto wall0 = std::chrono::steady_clock::now();
     h->stats = nqcd_run_stats{};
-    if (is_spin_boson(h)) {
+    if (is_rp(h)) {
+        int32_t rc = rp_run(h, t0, nsteps, term, key);
+        if (rc != NQCD_OK) return rc;
+    } else if (is_spin_boson(h)) {
         if (!h->sb_initial) return fail(h, NQCD_ERR_STATE, "spin-boson runs need nqcd_mi355x_sample_initial");
         int32_t rc = sb_run(h, t0, nsteps, term, key);
         if (rc != NQCD_OK) return rc;
@@ -451,9 +482,14 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* 
     case NQCD_OUT_FINAL_ADIABATIC_POPULATION: rows = n; break;
     case NQCD_OUT_SCATTERING_1D: rows = 2 * n; break;
     case NQCD_OUT_POPULATION_CORRELATION: rows = n * n * h->nsave; break;
+    case NQCD_OUT_CENTROID_POSITION:
+        if (!is_rp(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "centroid output needs a ring-polymer problem");
+        rows = h->nsave; src = h->save_centroid.as<double>(); break;
     default: return fail(h, NQCD_ERR_UNSUPPORTED, "unknown output kind");
     }
     if (len != rows) return fail(h, NQCD_ERR_ARG, "len does not match the output size");
+    if (h->prob.method == NQCD_METHOD_RPMD && kind != NQCD_OUT_CENTROID_POSITION)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "RPMD has no electronic outputs");
     if (!src) {
         HIPCHK(h, h->tmp.ensure(sizeof(double) * rows * nt));
         HIPCHK(h, launch_fill_output2(kind, h->prob.method == NQCD_METHOD_FSSH ? 0 : 1, nt, h->nsave, 0,
@@ -474,7 +510,7 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
     if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
     if (!h->have_results) return fail(h, NQCD_ERR_STATE, "no results: call nqcd_mi355x_run first");
     const int64_t nt = h->ntraj;
-    const int D = h->D, n = h->prob.n_states, ns = h->nsave;
+    const int D = (int)nqcd_rows(h), n = h->prob.n_states == 1 ? 2 : h->prob.n_states, ns = h->nsave;
     HIPCHK(h, hipSetDevice(h->device));
     auto fetch = [&](const DevBuf& b, size_t sz, std::vector<char>& host) -> hipError_t {
         host.resize(sz);
@@ -522,6 +558,16 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
         for (int64_t i = 0; i < nt; ++i)
             for (int sl = 0; sl < ns; ++sl)
                 for (int c = 0; c < n; ++c) o[((size_t)i * ns + sl) * n + c] = s[((size_t)sl * n + c) * nt + i];
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_SAVE_CENTROID: {
+        if (!is_rp(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "centroids exist for ring-polymer problems only");
+        if (bytes != (int64_t)sizeof(double) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(h->save_centroid, sizeof(double) * ns * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = s[(size_t)sl * nt + i];
         return NQCD_OK;
     }
     case NQCD_FIELD_SAVE_STATE: {

@@ -158,6 +158,13 @@ __device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R&
         d12 = (R)-2 * D * x * v12;
         break;
     }
+    case MODEL_HARMONIC: {  // single state: V = m w^2 (x - x0)^2 / 2
+        const R mm = (R)m.p[0], w = (R)m.p[1], x0 = (R)m.p[2];
+        const R dx = x - x0;
+        v11 = (R)0.5 * mm * w * w * dx * dx; d11 = mm * w * w * dx;
+        v22 = v11; d22 = d11; v12 = (R)0; d12 = (R)0;
+        break;
+    }
     default:
         v11 = v22 = v12 = d11 = d22 = d12 = (R)0;
         break;

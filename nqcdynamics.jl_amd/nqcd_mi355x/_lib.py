@@ -24,11 +24,12 @@ PREC_FP64, PREC_FP32_NUCLEI = range(2)
 SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
 MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC = range(6)
 TERM_NONE, TERM_ABS_GT = range(2)
-SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER = range(3)
-OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION, OUT_FINAL_ADIABATIC_POPULATION = range(5)
+SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLYMER_THERMAL = range(4)
+(OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION, OUT_FINAL_ADIABATIC_POPULATION,
+ OUT_CENTROID_POSITION) = range(6)
 RED_MEAN, RED_SUM = range(2)
 (FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
- FIELD_SAVE_POP_DI, FIELD_SAVE_STATE) = range(10)
+ FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID) = range(11)
 
 EXPORTED_SYMBOLS = (
     "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
@@ -122,6 +123,8 @@ class Ensemble:
         self.ntraj = 0
         self.n_states = 0
         self.D = 0
+        self.rows = 0
+        self.nbeads = 1
         self.nsave = 1
 
     def _check(self, rc, h="self"):
@@ -157,6 +160,8 @@ class Ensemble:
         d.electronic_substeps, d.masses, d.dt, d.temperature = electronic_substeps, _dp(self._masses), dt, temperature
         self._check(self._L.nqcd_mi355x_set_problem(self._h, C.byref(d)))
         self.n_states, self.D, self.dt = n_states, ndofs * natoms, dt
+        self.nbeads = nbeads
+        self.rows = ndofs * natoms * (nbeads if method in (METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST) else 1)
 
     def set_model(self, slot, kind, params):
         p = np.ascontiguousarray(params, dtype=np.float64)
@@ -205,8 +210,9 @@ class Ensemble:
         return out
 
     def download(self, field):
-        n, D, nt, nsave = self.n_states, self.D, self.ntraj, self.nsave
+        n, D, nt, nsave = max(self.n_states, 2), self.rows, self.ntraj, self.nsave
         shapes = {FIELD_R: ((nt, D), np.float64), FIELD_V: ((nt, D), np.float64),
+                  FIELD_SAVE_CENTROID: ((nt, nsave), np.float64),
                   FIELD_SIGMA_RE: ((nt, n, n), np.float64), FIELD_SIGMA_IM: ((nt, n, n), np.float64),
                   FIELD_STATE: ((nt,), np.int32), FIELD_STEPS: ((nt,), np.int32), FIELD_NHOPS: ((nt,), np.int32),
                   FIELD_SAVE_POP_AD: ((nt, nsave, n), np.float64), FIELD_SAVE_POP_DI: ((nt, nsave, n), np.float64),

@@ -694,3 +694,198 @@ OR_API void oracle_diabatic_to_adiabatic(const OModelDesc* m, int64_t ntraj, con
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Ring polymers: ring_polymer.jl:6-107 (frequencies, Cayley propagator), steps.jl:10-39
+// (step_C!), bcb.jl:81-116 (BCB, RPMD), bcb_electronics.jl:53-97 (BCBwithTsit5: RPSH and
+// RP-Ehrenfest with centroid electronics), rpsh.jl:30-50 (bead-broadcast rescaling).
+// One degree of freedom per bead.  RingPolymerArrays (external) supplies the orthogonal
+// normal-mode matrix; any orthonormal basis whose column k spans the spring eigenspace of
+// w_k = 2 w_n sin(k pi / P) gives the same bead-space B-C-B step, and the real DFT basis
+// is used here.
+// ---------------------------------------------------------------------------
+enum { RP_CLASSICAL = 0, RP_FSSH = 1, RP_EHRENFEST = 2 };
+
+struct ORPDesc {
+    int32_t method;     // RP_*
+    int32_t nstates;
+    int32_t nbeads;
+    int32_t rescaling;
+    double dt;
+    double t0;
+    int64_t nsteps;
+    int32_t save_every;
+    int32_t nsave;
+    uint64_t key;
+    double temperature; // w_n = nbeads * temperature
+    int32_t nsub;
+    int32_t pad;
+};
+
+// Column k of the real DFT basis (j = bead index).
+static void normal_mode_matrix(int P, double* U) {
+    for (int j = 0; j < P; ++j)
+        for (int k = 0; k < P; ++k) {
+            double u;
+            if (k == 0) u = 1.0 / std::sqrt((double)P);
+            else if (2 * k < P) u = std::sqrt(2.0 / P) * std::cos(2.0 * M_PI * j * k / P);
+            else if (2 * k == P) u = ((j % 2) ? -1.0 : 1.0) / std::sqrt((double)P);
+            else u = std::sqrt(2.0 / P) * std::sin(2.0 * M_PI * j * k / P);
+            U[j * P + k] = u;
+        }
+}
+
+static double matsubara(int P, double wn, int k) { return 2.0 * wn * std::sin(k * M_PI / P); }
+
+// cay(dt A) = (I - dt A/2)^{-1} (I + dt A/2), A = [0 1; -w^2 0] (ring_polymer.jl:71-82), half = false.
+static void cayley(double w, double dt, double C[4]) {
+    const double u = dt * dt * w * w / 4.0;
+    const double s = 1.0 / (1.0 + u);
+    C[0] = (1.0 - u) * s; C[1] = dt * s; C[2] = -dt * w * w * s; C[3] = (1.0 - u) * s;
+}
+
+OR_API void oracle_rp_normal_modes(int32_t P, double wn, double dt, double* U, double* w, double* C) {
+    normal_mode_matrix(P, U);
+    for (int k = 0; k < P; ++k) { w[k] = matsubara(P, wn, k); cayley(w[k], dt, C + 4 * k); }
+}
+
+struct RPTrajOut {
+    double* r; double* v; double* sig_re; double* sig_im; int32_t* state; int32_t* nhops;
+    double* centroid; double* pop_ad; double* pop_di; int32_t* save_state;
+};
+
+static void rp_run_one(const ORPDesc& R, const OModelDesc& m, double mass, int64_t i, uint64_t gid,
+                       const double* r0, const double* v0, const double* sr0, const double* si0, int32_t st0,
+                       const RPTrajOut& O) {
+    const int P = R.nbeads, n = R.nstates, nn = n * n;
+    const double wn = P * R.temperature;
+    std::vector<double> U(P * P), w(P), C(4 * P);
+    normal_mode_matrix(P, U.data());
+    for (int k = 0; k < P; ++k) { w[k] = matsubara(P, wn, k); cayley(w[k], R.dt, &C[4 * k]); }
+    std::vector<double> r(r0, r0 + P), v(v0, v0 + P), a(P), vtmp(P), rn(P), vn(P);
+    std::vector<cd> sig(nn), ws(9 * nn);
+    for (int J = 0; J < nn; ++J) sig[J] = sr0 ? cd(sr0[J], si0[J]) : cd(0.0, 0.0);
+    std::vector<Cache> bead(P);
+    for (int b = 0; b < P; ++b) bead[b].init(n, 1);
+    Cache cen; cen.init(n, 1);
+    int s = st0;
+    auto centroid_of = [&](const std::vector<double>& x) { double c = 0.0; for (int b = 0; b < P; ++b) c += x[b]; return c / P; };
+    // per-bead acceleration (no springs: they are inside the C step)
+    auto accel = [&](const std::vector<cd>& sg) {
+        for (int b = 0; b < P; ++b) {
+            bead[b].update(m, &r[b]);
+            double f;
+            if (R.method == RP_CLASSICAL) f = bead[b].dV[0];  // single-state model: dV/dx
+            else if (R.method == RP_FSSH) f = bead[b].G[s * n + s] + bead[b].dV0[0];
+            else {
+                f = bead[b].dV0[0];
+                for (int mm = 0; mm < n; ++mm) for (int q = 0; q < n; ++q) f += bead[b].G[q * n + mm] * sg[q * n + mm].real();
+            }
+            a[b] = -f / mass;
+        }
+    };
+    double rc = centroid_of(r);
+    if (R.method != RP_CLASSICAL) cen.update(m, &rc);
+    if (R.method == RP_FSSH && s < 0) {
+        double tot = 0.0;
+        for (int q = 0; q < n; ++q) tot += sig[q * n + q].real();
+        const double u = uniform01(gid, 0, STREAM_STATE, R.key) * tot;
+        double cum = 0.0; s = n - 1;
+        for (int q = 0; q < n; ++q) { cum += sig[q * n + q].real(); if (u < cum) { s = q; break; } }
+    }
+    accel(sig);
+    EParams Pe; Pe.n = n;
+    Pe.cur.E.assign(n, 0.0); Pe.cur.W.assign(nn, 0.0); Pe.cur.t = 0.0; Pe.next = Pe.cur;
+    double t = R.t0;
+    int32_t nh = 0;
+    std::vector<double> prob(n);
+    auto save = [&](int slot) {
+        if (slot < 0 || slot >= R.nsave) return;
+        const size_t o = (size_t)i * R.nsave + slot;
+        if (O.centroid) O.centroid[o] = centroid_of(r);
+        if (R.method == RP_CLASSICAL) return;
+        if (O.pop_ad) adiabatic_population(R.method == RP_FSSH ? METHOD_FSSH : METHOD_EHRENFEST, n, sig.data(), s, O.pop_ad + o * n);
+        if (O.pop_di) diabatic_population(R.method == RP_FSSH ? METHOD_FSSH : METHOD_EHRENFEST, n, cen.U.data(), sig.data(), s, O.pop_di + o * n);
+        if (O.save_state) O.save_state[o] = s;
+    };
+    save(0);
+    for (int64_t step = 0; step < R.nsteps; ++step) {
+        std::vector<cd> sigprev = sig;
+        for (int b = 0; b < P; ++b) vtmp[b] = v[b] + (R.dt / 2) * a[b];                  // step_B!
+        for (int k = 0; k < P; ++k) {                                                    // to normal modes
+            double x = 0.0, y = 0.0;
+            for (int b = 0; b < P; ++b) { x += U[b * P + k] * r[b]; y += U[b * P + k] * vtmp[b]; }
+            rn[k] = x; vn[k] = y;
+        }
+        for (int k = 0; k < P; ++k) {                                                    // step_C!
+            const double x = C[4 * k] * rn[k] + C[4 * k + 1] * vn[k];
+            const double y = C[4 * k + 2] * rn[k] + C[4 * k + 3] * vn[k];
+            rn[k] = x; vn[k] = y;
+        }
+        for (int b = 0; b < P; ++b) {                                                    // from normal modes
+            double x = 0.0, y = 0.0;
+            for (int k = 0; k < P; ++k) { x += U[b * P + k] * rn[k]; y += U[b * P + k] * vn[k]; }
+            r[b] = x; vtmp[b] = y;
+        }
+        accel(sigprev);
+        for (int b = 0; b < P; ++b) v[b] = vtmp[b] + (R.dt / 2) * a[b];                  // step_B!
+        if (R.method != RP_CLASSICAL) {
+            rc = centroid_of(r);
+            const double vc = centroid_of(v);
+            cen.update(m, &rc);
+            update_parameters(Pe, cen.E.data(), cen.d.data(), &vc, 1, t + R.dt);
+            propagate_density(Pe, sig.data(), t, R.dt, R.nsub, ws);
+        }
+        t = t + R.dt;
+        if (R.method == RP_FSSH) {
+            const double vc = centroid_of(v);
+            fssh_probability(n, 1, &vc, sig.data(), s, cen.d.data(), R.dt, prob.data());
+            const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
+            const int ns = fssh_select(n, prob.data(), s, rnd);
+            if (ns != s) {
+                bool ok = true;
+                if (R.rescaling != RESCALE_OFF) {
+                    const double d = cen.d[ns * n + s];                    // coupling[new, old]
+                    const double aq = d * d / mass / 2.0;
+                    const double bq = d * vc;
+                    const double cq = cen.E[ns] - cen.E[s];
+                    const double disc = bq * bq - 4.0 * aq * cq;
+                    if (disc < 0) {
+                        ok = false;
+                        if (R.rescaling == RESCALE_VINVERSION) {          // rpsh.jl:39-50
+                            const double dn = d / std::fabs(d);
+                            const double g = vc * dn;
+                            for (int b = 0; b < P; ++b) v[b] -= 2.0 * g * dn;
+                        }
+                    } else {
+                        const double root = std::sqrt(disc);
+                        const double gamma = (bq < 0) ? (bq + root) / (2.0 * aq) : (bq - root) / (2.0 * aq);
+                        for (int b = 0; b < P; ++b) v[b] -= gamma * d / mass;  // rpsh.jl:30-37
+                    }
+                }
+                if (ok) { s = ns; ++nh; }
+            }
+        }
+        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save((int)((step + 1) / R.save_every));
+    }
+    for (int b = 0; b < P; ++b) { O.r[(size_t)i * P + b] = r[b]; O.v[(size_t)i * P + b] = v[b]; }
+    if (R.method != RP_CLASSICAL) {
+        for (int J = 0; J < nn; ++J) { O.sig_re[(size_t)i * nn + J] = sig[J].real(); O.sig_im[(size_t)i * nn + J] = sig[J].imag(); }
+        O.state[i] = s;
+        O.nhops[i] = nh;
+    }
+}
+
+OR_API int32_t oracle_rp_run_ensemble(const ORPDesc* R, const OModelDesc* m, double mass, int64_t ntraj,
+                                      const double* r0, const double* v0, const double* sr0, const double* si0,
+                                      const int32_t* st0, double* r, double* v, double* sr, double* si, int32_t* state,
+                                      int32_t* nhops, double* centroid, double* pop_ad, double* pop_di,
+                                      int32_t* save_state, int32_t nthreads) {
+    const int P = R->nbeads, nn = R->nstates * R->nstates;
+    RPTrajOut O{r, v, sr, si, state, nhops, centroid, pop_ad, pop_di, save_state};
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        rp_run_one(*R, *m, mass, i, (uint64_t)i, r0 + (size_t)i * P, v0 + (size_t)i * P,
+                   sr0 ? sr0 + (size_t)i * nn : nullptr, si0 ? si0 + (size_t)i * nn : nullptr, st0 ? st0[i] : 0, O);
+    });
+    return 0;
+}

@@ -209,3 +209,51 @@ def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None,
       _p(out["nhops"], C.c_int32), _p(out["pop_ad"]), _p(out["pop_di"]), _p(out["save_state"], C.c_int32), C.c_int32(nthreads),
       _p(tid, C.c_int64))
     return out
+
+
+# --- ring polymers ---------------------------------------------------------------------------
+RP_CLASSICAL, RP_FSSH, RP_EHRENFEST = 0, 1, 2
+
+
+class ORPDesc(C.Structure):
+    _fields_ = [("method", C.c_int32), ("nstates", C.c_int32), ("nbeads", C.c_int32), ("rescaling", C.c_int32),
+                ("dt", C.c_double), ("t0", C.c_double), ("nsteps", C.c_int64), ("save_every", C.c_int32),
+                ("nsave", C.c_int32), ("key", C.c_uint64), ("temperature", C.c_double), ("nsub", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+def rp_normal_modes(P, wn, dt):
+    U = np.zeros((P, P)); w = np.zeros(P); Cm = np.zeros((P, 2, 2))
+    f = lib().oracle_rp_normal_modes
+    f.argtypes = [C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    f(P, wn, dt, _p(U), _p(w), _p(Cm))
+    return U, w, Cm
+
+
+def rp_run_ensemble(method, model, mass, r0, v0, sig0=None, state0=None, dt=1.0, nsteps=100, temperature=1.0,
+                    save_every=0, nsave=1, key=0, rescaling=RESCALE_STANDARD, nthreads=None, t0=0.0, nsub=5):
+    """r0, v0: (ntraj, nbeads).  Returns final beads, density, states and per-save centroid/populations."""
+    ntraj, P = r0.shape
+    n = model.nstates
+    R = ORPDesc()
+    R.method, R.nstates, R.nbeads, R.rescaling = method, n, P, rescaling
+    R.dt, R.t0, R.nsteps, R.save_every, R.nsave, R.key = dt, t0, nsteps, save_every, nsave, key
+    R.temperature, R.nsub = temperature, nsub
+    sr0 = si0 = None
+    if sig0 is not None:
+        sr0 = np.ascontiguousarray(np.real(sig0), dtype=np.float64)
+        si0 = np.ascontiguousarray(np.imag(sig0), dtype=np.float64)
+    st0 = None if state0 is None else np.ascontiguousarray(state0, dtype=np.int32)
+    out = dict(r=np.zeros((ntraj, P)), v=np.zeros((ntraj, P)), sig_re=np.zeros((ntraj, n, n)), sig_im=np.zeros((ntraj, n, n)),
+               state=np.zeros(ntraj, np.int32), nhops=np.zeros(ntraj, np.int32), centroid=np.zeros((ntraj, nsave)),
+               pop_ad=np.zeros((ntraj, nsave, n)), pop_di=np.zeros((ntraj, nsave, n)), save_state=np.zeros((ntraj, nsave), np.int32))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_rp_run_ensemble
+    f.restype = C.c_int32
+    f(C.byref(R), C.byref(model), C.c_double(mass), C.c_int64(ntraj),
+      _p(np.ascontiguousarray(r0, dtype=np.float64)), _p(np.ascontiguousarray(v0, dtype=np.float64)), _p(sr0), _p(si0),
+      _p(st0, C.c_int32), _p(out["r"]), _p(out["v"]), _p(out["sig_re"]), _p(out["sig_im"]), _p(out["state"], C.c_int32),
+      _p(out["nhops"], C.c_int32), _p(out["centroid"]), _p(out["pop_ad"]), _p(out["pop_di"]), _p(out["save_state"], C.c_int32),
+      C.c_int32(nthreads))
+    return out

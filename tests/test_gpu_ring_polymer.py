@@ -1,0 +1,118 @@
+"""GPU ring-polymer dynamics (RPMD with BCB; RPSH and RP-Ehrenfest with BCBwithTsit5) against
+the FP64 oracle, plus the analytic harmonic ring polymer."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+MASS = 1836.0
+BETA = 8.0
+DW1 = [1 / 1200, 1 / 40, 0.0, 0.0, 1.0]     # single-state double well
+DW2 = [1 / 1200, 1 / 40, 0.01, 0.005, 1.0]  # + second diabat offset 0.01, Gaussian coupling
+
+
+def _ens(L, method, P, nstates, dt=1.0, precision=0, rescaling=0):
+    E = L.Ensemble(0)
+    alg = L.ALG_BCB if method == L.METHOD_RPMD else L.ALG_BCB_TSIT5
+    E.set_problem(method, nstates, 1, 1, [MASS], dt, algorithm=alg, nbeads=P, precision=precision,
+                  temperature=1.0 / BETA, rescaling=rescaling)
+    E.set_model(0, L.MODEL_DW_GAUSSIAN, DW2 if nstates == 2 else DW1)
+    return E
+
+
+def _thermal_beads(L, P, ntraj, key, x0=-3.87, nstates=1):
+    """Device sampler, read back through a zero-step run."""
+    method = L.METHOD_RPMD if nstates == 1 else L.METHOD_RPSH
+    E = _ens(L, method, P, nstates)
+    E.sample_initial(L.SAMPLER_RING_POLYMER_THERMAL, ntraj, [x0, np.sqrt(0.1 / MASS), BETA], key)
+    E.run(0.0, 0.0, key)
+    r, v = E.download(L.FIELD_R), E.download(L.FIELD_V)
+    E.close()
+    return r, v
+
+
+@pytest.mark.parametrize("P", [8, 16, 32, 64])
+def test_rpmd_fp64_parity(oracle, key, P):
+    from nqcd_mi355x import _lib as L
+    ntraj, nsteps = 256, 500
+    r0, v0 = _thermal_beads(L, P, ntraj, key)
+    assert np.isfinite(r0).all() and abs(r0.mean() + 3.87) < 0.2
+    E = _ens(L, L.METHOD_RPMD, P, 1)
+    E.load_initial(r0, v0, np.zeros((ntraj, 2, 2)))
+    E.run(0.0, float(nsteps), key, save_every=50)
+    g_r, g_v, g_c = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_SAVE_CENTROID)
+    E.close()
+    m = oracle.make_model(oracle.MODEL_DW_GAUSSIAN, 1, 1, DW1)
+    c = oracle.rp_run_ensemble(oracle.RP_CLASSICAL, m, MASS, r0, v0, dt=1.0, nsteps=nsteps, temperature=1 / BETA,
+                               save_every=50, nsave=nsteps // 50 + 1, key=key)
+    assert np.allclose(g_r, c["r"], rtol=1e-9, atol=1e-10)
+    assert np.allclose(g_v, c["v"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(g_c, c["centroid"], rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("P", [8, 32])
+@pytest.mark.parametrize("method", ["rpsh", "rpeh"])
+def test_rp_electronic_fp64_parity(oracle, key, P, method):
+    from nqcd_mi355x import _lib as L
+    ntraj, nsteps = 256, 600
+    rng = np.random.default_rng(P)
+    # ring polymers crossing the coupling region at x ~ 0 with a spread of velocities
+    r0 = rng.normal(-2.0, 0.05, size=(ntraj, 1)) + rng.normal(0, 0.05, size=(ntraj, P))
+    v0 = rng.uniform(0.011, 0.018, size=(ntraj, 1)) + rng.normal(0, 1e-4, size=(ntraj, P))
+    sig0 = np.zeros((ntraj, 2, 2), complex); sig0[:, 0, 0] = 1
+    st0 = np.zeros(ntraj, np.int32)
+    lm = L.METHOD_RPSH if method == "rpsh" else L.METHOD_RP_EHRENFEST
+    E = _ens(L, lm, P, 2)
+    E.load_initial(r0, v0, sig0, state=st0)
+    E.run(0.0, float(nsteps), key, save_every=10)
+    g = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V), "state": E.download(L.FIELD_STATE),
+         "sig": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
+         "save_state": E.download(L.FIELD_SAVE_STATE), "pop_di": E.download(L.FIELD_SAVE_POP_DI)}
+    E.close()
+    m = oracle.make_model(oracle.MODEL_DW_GAUSSIAN, 2, 1, DW2)
+    om = oracle.RP_FSSH if method == "rpsh" else oracle.RP_EHRENFEST
+    c = oracle.rp_run_ensemble(om, m, MASS, r0, v0, sig0, st0, dt=1.0, nsteps=nsteps, temperature=1 / BETA,
+                               save_every=10, nsave=nsteps // 10 + 1, key=key)
+    if method == "rpsh":
+        assert c["nhops"].sum() > 10
+        assert np.array_equal(g["save_state"], c["save_state"])
+    assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-10)
+    assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(g["sig"], c["sig_re"] + 1j * c["sig_im"], rtol=1e-9, atol=1e-10)
+    assert np.allclose(g["pop_di"], c["pop_di"], rtol=1e-9, atol=1e-10)
+
+
+def test_rpmd_harmonic_matches_analytic_normal_modes(oracle, key):
+    """Harmonic ring polymer: normal mode k oscillates at sqrt(w_k^2 + w^2) (the reference's
+    BCB convergence test, test/Dynamics/algorithms/bcb.jl:18-44); dt small so the error is ~dt^2."""
+    from nqcd_mi355x import _lib as L
+    P, T, w0, dt, tend = 16, 0.3, 1.0, 0.002, 5.0
+    E = L.Ensemble(0)
+    E.set_problem(L.METHOD_RPMD, 1, 1, 1, [1.0], dt, algorithm=L.ALG_BCB, nbeads=P, temperature=T)
+    E.set_model(0, L.MODEL_HARMONIC, [1.0, w0, 0.0])
+    rng = np.random.default_rng(1)
+    r0 = rng.normal(0, 0.3, size=(8, P)); v0 = rng.normal(0, 0.5, size=(8, P))
+    E.load_initial(r0, v0, np.zeros((8, 2, 2)))
+    E.run(0.0, tend, key)
+    g = E.download(L.FIELD_R)
+    E.close()
+    U, w, _ = oracle.rp_normal_modes(P, P * T, dt)
+    Om = np.sqrt(w ** 2 + w0 ** 2)
+    for i in range(8):
+        rn0, vn0 = U.T @ r0[i], U.T @ v0[i]
+        exact = U @ (rn0 * np.cos(Om * tend) + vn0 / Om * np.sin(Om * tend))
+        assert np.abs(g[i] - exact).max() < 2e-4
+
+
+def test_rpmd_fp32_centroid_statistics(oracle, key):
+    """FP32 beads: the mean centroid trajectory of 20000 thermal ring polymers (P = 32) matches
+    the FP64 path within 1e-3 bohr."""
+    from nqcd_mi355x import _lib as L
+    ntraj, P, nsteps = 20000, 32, 2000
+    out = []
+    for prec in (0, 1):
+        E = _ens(L, L.METHOD_RPMD, P, 1, precision=prec)
+        E.sample_initial(L.SAMPLER_RING_POLYMER_THERMAL, ntraj, [-3.87, np.sqrt(0.1 / MASS), BETA], key)
+        E.run(0.0, float(nsteps), key, save_every=100)
+        out.append(E.reduce(L.OUT_CENTROID_POSITION, L.RED_MEAN, E.nsave))
+        E.close()
+    assert np.abs(out[0] - out[1]).max() < 1e-3
