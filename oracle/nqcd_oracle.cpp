@@ -889,3 +889,306 @@ OR_API int32_t oracle_rp_run_ensemble(const ORPDesc* R, const OModelDesc* m, dou
     });
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Independent-electron surface hopping (IESH) and EhrenfestNA on a Newns-Anderson
+// (impurity + discretised bath) model.
+//   Model (NQCModels AndersonHolstein with a MiaoSubotnik impurity and a TrapezoidalRule
+//   bath; docs/src/NQCModels/systembathmodels.md "Newns-Anderson model"):
+//     U0(x) = m w^2 x^2 / 2 (state independent), h(x) = m w^2 (x - g)^2 / 2 + dG - U0(x),
+//     V = [[h, V_1 .. V_M], [V_k, eps_k delta_kl]], eps_k = a + (k-1)(b-a)/(M-1),
+//     V_k = sqrt(Gamma / 2 pi) sqrt((b-a)/(M-1)).
+//   Step: VerletwithElectronics (verlet_with_electronics.jl:42-69): B, A, update_cache!,
+//   acceleration from the occupied states (iesh.jl:190-207), B, wavefunction propagation
+//   psi <- U exp(-i Lambda dt) U^H psi with H = diag(E) - i v.d at (r', v')
+//   (wavefunction_dynamics.jl:15-58).  Hop (iesh.jl:231-335,390-407): one uniform for the
+//   estimate and the selection, determinants by LU (FastDeterminant.jl:20-22).
+// ---------------------------------------------------------------------------
+struct ONADesc {
+    double mw2;      // m_model * omega^2
+    double g, dG, Gamma, a, b;
+    int32_t M;       // bath states (N = M + 1)
+    int32_t pad;
+};
+
+static void na_matrices(const ONADesc& md, double x, std::vector<double>& V, double& dh, double& U0, double& dU0) {
+    const int N = md.M + 1;
+    V.assign((size_t)N * N, 0.0);
+    const double de = (md.b - md.a) / (md.M - 1);
+    const double Vk = std::sqrt(md.Gamma / (2.0 * M_PI)) * std::sqrt(de);
+    U0 = 0.5 * md.mw2 * x * x;
+    dU0 = md.mw2 * x;
+    const double U1 = 0.5 * md.mw2 * (x - md.g) * (x - md.g) + md.dG;
+    V[0] = U1 - U0;
+    dh = md.mw2 * (x - md.g) - md.mw2 * x;
+    for (int k = 1; k < N; ++k) {
+        V[(size_t)k * N + k] = md.a + (k - 1) * de;
+        V[k] = V[(size_t)k * N] = Vk;
+    }
+}
+
+// Complex Hermitian cyclic Jacobi (stands in for LAPACK zheevr); eigenvalues ascending,
+// U columns orthonormal eigenvectors.
+static void eigen_hermitian(int n, std::vector<cd> A, std::vector<double>& E, std::vector<cd>& U) {
+    U.assign((size_t)n * n, cd(0.0, 0.0));
+    for (int i = 0; i < n; ++i) U[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) {
+            const double a2 = std::norm(A[(size_t)i * n + j]);
+            tot += a2; if (i != j) off += a2;
+        }
+        if (off <= 1e-32 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p) for (int q = p + 1; q < n; ++q) {
+            const cd apq = A[(size_t)p * n + q];
+            const double mag = std::abs(apq);
+            if (mag == 0.0) continue;
+            const cd ph = apq / mag;  // a_pq = |a_pq| e^{i phi}
+            const double app = A[(size_t)p * n + p].real(), aqq = A[(size_t)q * n + q].real();
+            const double theta = (aqq - app) / (2.0 * mag);
+            const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+            const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+            // J = [[c, s ph], [-s conj(ph), c]] acting on columns p, q: A <- J^H A J
+            for (int k = 0; k < n; ++k) {
+                const cd akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                A[(size_t)k * n + p] = c * akp - s * std::conj(ph) * akq;
+                A[(size_t)k * n + q] = s * ph * akp + c * akq;
+            }
+            for (int k = 0; k < n; ++k) {
+                const cd apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                A[(size_t)p * n + k] = c * apk - s * ph * aqk;
+                A[(size_t)q * n + k] = s * std::conj(ph) * apk + c * aqk;
+            }
+            for (int k = 0; k < n; ++k) {
+                const cd ukp = U[(size_t)k * n + p], ukq = U[(size_t)k * n + q];
+                U[(size_t)k * n + p] = c * ukp - s * std::conj(ph) * ukq;
+                U[(size_t)k * n + q] = s * ph * ukp + c * ukq;
+            }
+        }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A[(size_t)x * n + x].real() < A[(size_t)y * n + y].real(); });
+    std::vector<cd> Us(U);
+    E.assign(n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        E[j] = A[(size_t)idx[j] * n + idx[j]].real();
+        for (int i = 0; i < n; ++i) U[(size_t)i * n + j] = Us[(size_t)i * n + idx[j]];
+    }
+}
+
+OR_API void oracle_eigen_hermitian(int32_t n, const double* Hre, const double* Him, double* E, double* Ure, double* Uim) {
+    std::vector<cd> A((size_t)n * n);
+    for (int i = 0; i < n * n; ++i) A[i] = cd(Hre[i], Him[i]);
+    std::vector<double> e; std::vector<cd> U;
+    eigen_hermitian(n, A, e, U);
+    for (int i = 0; i < n; ++i) E[i] = e[i];
+    for (int i = 0; i < n * n; ++i) { Ure[i] = U[i].real(); Uim[i] = U[i].imag(); }
+}
+
+// LU determinant with partial pivoting (FastDeterminant.det!, FastDeterminant.jl:20-22).
+static cd det_lu(int n, std::vector<cd> A) {
+    cd det = 1.0;
+    for (int k = 0; k < n; ++k) {
+        int piv = k; double best = std::abs(A[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; ++i) { const double a = std::abs(A[(size_t)i * n + k]); if (a > best) { best = a; piv = i; } }
+        if (best == 0.0) return 0.0;
+        if (piv != k) { for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)piv * n + j]); det = -det; }
+        const cd d = A[(size_t)k * n + k];
+        det *= d;
+        for (int i = k + 1; i < n; ++i) {
+            const cd f = A[(size_t)i * n + k] / d;
+            for (int j = k + 1; j < n; ++j) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+        }
+    }
+    return det;
+}
+
+OR_API void oracle_det(int32_t n, const double* re, const double* im, double* out) {
+    std::vector<cd> A((size_t)n * n);
+    for (int i = 0; i < n * n; ++i) A[i] = cd(re[i], im[i]);
+    const cd d = det_lu(n, A);
+    out[0] = d.real(); out[1] = d.imag();
+}
+
+struct OIEDesc {
+    int32_t method;      // 0 = IESH, 1 = EhrenfestNA
+    int32_t nelec;
+    int32_t rescaling;
+    int32_t estimate;    // estimate_probability (iesh.jl:251-254)
+    double dt, t0, mass;
+    int64_t nsteps;
+    uint64_t key;
+    int32_t edc;         // DecoherenceCorrectionEDC applied after each step (iesh.jl:424-441)
+    int32_t pad;
+    double edc_C;
+    int32_t save_every, nsave;
+};
+
+static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t gid, double r0, double v0,
+                       const int32_t* occ0, const double* psi_re0, const double* psi_im0,
+                       double* r_out, double* v_out, double* psi_re, double* psi_im, int32_t* occ_out,
+                       int32_t* nhops_out, double* pop_imp) {
+    const int N = md.M + 1, Ne = R.nelec;
+    std::vector<double> V, E(N), U((size_t)N * N), Up((size_t)N * N), G((size_t)N * N), d((size_t)N * N);
+    double dh, U0, dU0;
+    std::vector<cd> psi((size_t)N * Ne);
+    for (int k = 0; k < N * Ne; ++k) psi[k] = cd(psi_re0[(size_t)i * N * Ne + k], psi_im0[(size_t)i * N * Ne + k]);
+    std::vector<int> occ(occ0 + (size_t)i * Ne, occ0 + (size_t)i * Ne + Ne);
+    bool have = false;
+    double x = r0, v = v0, kacc;
+    auto update = [&]() {
+        na_matrices(md, x, V, dh, U0, dU0);
+        if (have) Up = U;
+        eigen_sym(N, V.data(), E.data(), U.data(), have ? Up.data() : nullptr);
+        have = true;
+        std::vector<double> dV((size_t)N * N, 0.0);
+        dV[0] = dh;
+        adiabatic_quantities(N, 1, E.data(), U.data(), dV.data(), G.data(), d.data());
+    };
+    auto accel = [&]() {
+        double f = dU0;
+        if (R.method == 0) { for (int e = 0; e < Ne; ++e) f += G[(size_t)occ[e] * N + occ[e]]; }
+        else {  // EhrenfestNA (ehrenfest_na.jl:72-86): sum over electrons of psi^H G psi
+            for (int e = 0; e < Ne; ++e)
+                for (int a = 0; a < N; ++a) for (int b = 0; b < N; ++b)
+                    f += (std::conj(psi[(size_t)a * Ne + e]) * G[(size_t)a * N + b] * psi[(size_t)b * Ne + e]).real();
+        }
+        return -f / R.mass;
+    };
+    auto save_pop = [&](int slot) {
+        if (!pop_imp || slot >= R.nsave) return;
+        // iesh_diabatic_population (iesh.jl:351-369), impurity (diabatic state 0) component
+        double p = 0.0;
+        for (int e = 0; e < Ne; ++e) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int a = 0; a < N; ++a) { const double c = psi[(size_t)a * Ne + e].real(); s1 += U[a] * c; s2 += U[a] * U[a] * c * c; }
+            const int s = occ[e];
+            p += s1 * s1 - s2 + U[s] * U[s];
+        }
+        pop_imp[(size_t)i * R.nsave + slot] = p;
+    };
+    update();
+    kacc = accel();
+    save_pop(0);
+    int nh = 0;
+    double t = R.t0;
+    std::vector<cd> H((size_t)N * N), P((size_t)N * N), tmp((size_t)N * Ne);
+    std::vector<double> Eh; std::vector<cd> Uh;
+    std::vector<cd> S((size_t)Ne * Ne);
+    for (int64_t step = 0; step < R.nsteps; ++step) {
+        const double vt = v + (R.dt / 2) * kacc;
+        x = x + R.dt * vt;
+        update();
+        if (R.method == 1) {
+            // EhrenfestNA uses psi of the step start (verlet_with_electronics.jl:76-79)
+        }
+        kacc = accel();
+        v = vt + (R.dt / 2) * kacc;
+        // H = diag(E) - i d v; P = U e^{-i Lambda dt} U^H; psi <- P psi
+        for (int a = 0; a < N; ++a) for (int b = 0; b < N; ++b)
+            H[(size_t)a * N + b] = cd((a == b) ? E[a] : 0.0, 0.0) - cd(0.0, 1.0) * (d[(size_t)a * N + b] * v);
+        eigen_hermitian(N, H, Eh, Uh);
+        for (int a = 0; a < N; ++a) for (int b = 0; b < N; ++b) {
+            cd s = 0.0;
+            for (int k = 0; k < N; ++k) s += Uh[(size_t)a * N + k] * std::exp(cd(0.0, -Eh[k] * R.dt)) * std::conj(Uh[(size_t)b * N + k]);
+            P[(size_t)a * N + b] = s;
+        }
+        for (int a = 0; a < N; ++a) for (int e = 0; e < Ne; ++e) {
+            cd s = 0.0;
+            for (int b = 0; b < N; ++b) s += P[(size_t)a * N + b] * psi[(size_t)b * Ne + e];
+            tmp[(size_t)a * Ne + e] = s;
+        }
+        psi = tmp;
+        t += R.dt;
+        if (R.method == 0) {
+            const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
+            // overlap S[j][e] = psi[occ[j], e]
+            for (int j = 0; j < Ne; ++j) for (int e = 0; e < Ne; ++e) S[(size_t)j * Ne + e] = psi[(size_t)occ[j] * Ne + e];
+            const cd det_cur = det_lu(Ne, S);
+            const double Akk = std::norm(det_cur);
+            const double pref = 2.0 * R.dt / Akk;
+            std::vector<char> isocc(N, 0);
+            for (int j = 0; j < Ne; ++j) isocc[occ[j]] = 1;
+            std::vector<int> unocc;
+            for (int a = 0; a < N; ++a) if (!isocc[a]) unocc.push_back(a);
+            // v_dot_d[m, n] = -v d[m, occ[n]]
+            double sumvd = 0.0;
+            for (int n = 0; n < Ne; ++n) for (int m : unocc) sumvd += std::fabs(-v * d[(size_t)m * N + occ[n]]);
+            bool evaluate = true;
+            if (R.estimate) {
+                const double est = pref * sumvd * (std::fabs(det_cur.real()) + std::fabs(det_cur.imag()));
+                if (est < rnd) evaluate = false;
+            }
+            if (evaluate) {
+                double cum = 0.0;
+                int hn = -1, hm = -1;
+                for (int n = 0; n < Ne && hn < 0; ++n)
+                    for (int m : unocc) {
+                        std::vector<cd> Sn(S);
+                        for (int e = 0; e < Ne; ++e) Sn[(size_t)n * Ne + e] = psi[(size_t)m * Ne + e];
+                        const cd Akj = det_cur * std::conj(det_lu(Ne, Sn));
+                        double prob = pref * Akj.real() * (-v * d[(size_t)m * N + occ[n]]);
+                        prob = std::min(std::max(prob, 0.0), 1.0);
+                        cum += prob;
+                        if (rnd < cum) { hn = n; hm = m; break; }
+                    }
+                if (hn >= 0) {
+                    const int old = occ[hn];
+                    bool ok = true;
+                    if (R.rescaling != RESCALE_OFF) {
+                        const double dd = d[(size_t)hm * N + old];  // coupling[new, old]
+                        const double aq = dd * dd / R.mass / 2.0, bq = dd * v, cq = E[hm] - E[old];
+                        const double disc = bq * bq - 4.0 * aq * cq;
+                        if (disc < 0) {
+                            ok = false;
+                            if (R.rescaling == RESCALE_VINVERSION) v -= 2.0 * (v * (dd / std::fabs(dd))) * (dd / std::fabs(dd));
+                        } else {
+                            const double root = std::sqrt(disc);
+                            const double gam = (bq < 0) ? (bq + root) / (2.0 * aq) : (bq - root) / (2.0 * aq);
+                            v -= gam * dd / R.mass;
+                        }
+                    }
+                    if (ok) { occ[hn] = hm; ++nh; }
+                }
+            }
+            if (R.edc) {  // Granucci-Persico EDC on each electron (decoherence_corrections.jl:21-39)
+                const double Ekin = 0.5 * R.mass * v * v;
+                for (int e = 0; e < Ne; ++e) {
+                    const int s = occ[e];
+                    double nrm = 0.0;
+                    for (int a = 0; a < N; ++a) {
+                        if (a == s) continue;
+                        const double tau = (1.0 + R.edc_C / Ekin) / std::fabs(E[a] - E[s]);
+                        psi[(size_t)a * Ne + e] *= std::exp(-R.dt / tau);
+                        nrm += std::norm(psi[(size_t)a * Ne + e]);
+                    }
+                    const cd cm = psi[(size_t)s * Ne + e];
+                    psi[(size_t)s * Ne + e] = cm * std::sqrt((1.0 - nrm) / std::norm(cm));
+                }
+            }
+        }
+        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save_pop((int)((step + 1) / R.save_every));
+    }
+    r_out[i] = x; v_out[i] = v;
+    for (int k = 0; k < N * Ne; ++k) { psi_re[(size_t)i * N * Ne + k] = psi[k].real(); psi_im[(size_t)i * N * Ne + k] = psi[k].imag(); }
+    for (int e = 0; e < Ne; ++e) occ_out[(size_t)i * Ne + e] = occ[e];
+    nhops_out[i] = nh;
+}
+
+OR_API int32_t oracle_iesh_run_ensemble(const OIEDesc* R, const ONADesc* md, int64_t ntraj, const double* r0,
+                                        const double* v0, const int32_t* occ0, const double* psi_re0,
+                                        const double* psi_im0, double* r, double* v, double* psi_re, double* psi_im,
+                                        int32_t* occ, int32_t* nhops, double* pop_imp, int32_t nthreads) {
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        ie_run_one(*R, *md, i, (uint64_t)i, r0[i], v0[i], occ0, psi_re0, psi_im0, r, v, psi_re, psi_im, occ, nhops, pop_imp);
+    });
+    return 0;
+}
+
+OR_API void oracle_na_eigen(const ONADesc* md, double x, double* E, double* U, double* dh) {
+    std::vector<double> V; double U0, dU0;
+    na_matrices(*md, x, V, *dh, U0, dU0);
+    const int N = md->M + 1;
+    eigen_sym(N, V.data(), E, U, nullptr);
+}

@@ -257,3 +257,79 @@ def rp_run_ensemble(method, model, mass, r0, v0, sig0=None, state0=None, dt=1.0,
       _p(out["nhops"], C.c_int32), _p(out["centroid"]), _p(out["pop_ad"]), _p(out["pop_di"]), _p(out["save_state"], C.c_int32),
       C.c_int32(nthreads))
     return out
+
+
+# --- IESH / EhrenfestNA on the Newns-Anderson model ------------------------------------------
+class ONADesc(C.Structure):
+    _fields_ = [("mw2", C.c_double), ("g", C.c_double), ("dG", C.c_double), ("Gamma", C.c_double), ("a", C.c_double),
+                ("b", C.c_double), ("M", C.c_int32), ("pad", C.c_int32)]
+
+
+class OIEDesc(C.Structure):
+    _fields_ = [("method", C.c_int32), ("nelec", C.c_int32), ("rescaling", C.c_int32), ("estimate", C.c_int32),
+                ("dt", C.c_double), ("t0", C.c_double), ("mass", C.c_double), ("nsteps", C.c_int64), ("key", C.c_uint64),
+                ("edc", C.c_int32), ("pad", C.c_int32), ("edc_C", C.c_double), ("save_every", C.c_int32), ("nsave", C.c_int32)]
+
+
+def na_model(m=2000.0, omega=2e-4, g=20.6097, dG=-3.8e-3, Gamma=6.4e-3, M=40, W=None):
+    """MiaoSubotnik impurity with a TrapezoidalRule bath on [-W, W] (W = 6 Gamma / 2 by default)."""
+    if W is None:
+        W = 6 * Gamma / 2
+    d = ONADesc()
+    d.mw2, d.g, d.dG, d.Gamma, d.a, d.b, d.M = m * omega ** 2, g, dG, Gamma, -W, W, M
+    return d
+
+
+def eigen_hermitian(H):
+    H = np.asarray(H, dtype=complex)
+    n = H.shape[0]
+    E = np.zeros(n); Ur = np.zeros((n, n)); Ui = np.zeros((n, n))
+    f = lib().oracle_eigen_hermitian
+    f(C.c_int32(n), _p(np.ascontiguousarray(H.real)), _p(np.ascontiguousarray(H.imag)), _p(E), _p(Ur), _p(Ui))
+    return E, Ur + 1j * Ui
+
+
+def det(A):
+    A = np.asarray(A, dtype=complex)
+    out = np.zeros(2)
+    lib().oracle_det(C.c_int32(A.shape[0]), _p(np.ascontiguousarray(A.real)), _p(np.ascontiguousarray(A.imag)), _p(out))
+    return out[0] + 1j * out[1]
+
+
+def na_eigen(md, x):
+    N = md.M + 1
+    E = np.zeros(N); U = np.zeros((N, N)); dh = C.c_double(0)
+    f = lib().oracle_na_eigen
+    f.argtypes = [C.POINTER(ONADesc), C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    f(C.byref(md), x, _p(E), _p(U), C.byref(dh))
+    return E, U, dh.value
+
+
+def iesh_run_ensemble(method, md, nelec, mass, r0, v0, occ0, psi0=None, dt=1.0, nsteps=100, key=0,
+                      rescaling=RESCALE_STANDARD, estimate=True, edc=False, edc_C=0.1, save_every=0, nthreads=None):
+    ntraj = len(r0)
+    N = md.M + 1
+    occ0 = np.ascontiguousarray(occ0, dtype=np.int32).reshape(ntraj, nelec)
+    if psi0 is None:
+        psi0 = np.zeros((ntraj, N, nelec), complex)
+        for i in range(ntraj):
+            for e in range(nelec):
+                psi0[i, occ0[i, e], e] = 1.0
+    R = OIEDesc()
+    R.method, R.nelec, R.rescaling, R.estimate = method, nelec, rescaling, int(estimate)
+    R.dt, R.t0, R.mass, R.nsteps, R.key, R.edc, R.edc_C = dt, 0.0, mass, nsteps, key, int(edc), edc_C
+    nsave = nsteps // save_every + 1 if save_every else 1
+    R.save_every, R.nsave = save_every, nsave
+    out = dict(r=np.zeros(ntraj), v=np.zeros(ntraj), psi_re=np.zeros((ntraj, N, nelec)), psi_im=np.zeros((ntraj, N, nelec)),
+               occ=np.zeros((ntraj, nelec), np.int32), nhops=np.zeros(ntraj, np.int32), pop_imp=np.zeros((ntraj, nsave)))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_iesh_run_ensemble
+    f.restype = C.c_int32
+    f(C.byref(R), C.byref(md), C.c_int64(ntraj), _p(np.ascontiguousarray(r0, dtype=np.float64)),
+      _p(np.ascontiguousarray(v0, dtype=np.float64)), _p(occ0, C.c_int32),
+      _p(np.ascontiguousarray(psi0.real)), _p(np.ascontiguousarray(psi0.imag)), _p(out["r"]), _p(out["v"]),
+      _p(out["psi_re"]), _p(out["psi_im"]), _p(out["occ"], C.c_int32), _p(out["nhops"], C.c_int32), _p(out["pop_imp"]),
+      C.c_int32(nthreads))
+    out["psi"] = out["psi_re"] + 1j * out["psi_im"]
+    return out

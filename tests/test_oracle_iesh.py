@@ -1,0 +1,113 @@
+"""Pins the IESH / EhrenfestNA part of the oracle: Hermitian eigen-solver (zheevr's role),
+LU determinant (test/Core/FastDeterminant.jl:9-13), the Newns-Anderson matrices,
+unitarity of the wavefunction propagation, hop energy conservation (test/Dynamics/iesh.jl:122-151)
+and EhrenfestNA energy conservation."""
+import numpy as np
+import pytest
+
+
+def test_hermitian_eigen_against_numpy(oracle):
+    rng = np.random.default_rng(1)
+    for n in (2, 5, 17, 41):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        H = A + A.conj().T
+        E, U = oracle.eigen_hermitian(H)
+        assert np.allclose(E, np.linalg.eigvalsh(H), atol=1e-11)
+        assert np.allclose(U.conj().T @ U, np.eye(n), atol=1e-12)
+        assert np.allclose(U.conj().T @ H @ U, np.diag(E), atol=1e-10)
+
+
+def test_lu_determinant_against_numpy(oracle):
+    rng = np.random.default_rng(2)
+    for n in (1, 3, 10, 25):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        assert np.isclose(oracle.det(A), np.linalg.det(A), rtol=1e-11)
+
+
+def test_newns_anderson_matrices(oracle):
+    md = oracle.na_model(M=20)
+    for x in (-5.0, 3.0, 21.0):
+        E, U, dh = oracle.na_eigen(md, x)
+        N = 21
+        W = 6 * 6.4e-3 / 2
+        eps = -W + np.arange(20) * 2 * W / 19
+        Vk = np.sqrt(6.4e-3 / (2 * np.pi)) * np.sqrt(2 * W / 19)
+        mw2 = 2000 * 2e-4 ** 2
+        h = 0.5 * mw2 * (x - 20.6097) ** 2 - 3.8e-3 - 0.5 * mw2 * x ** 2
+        H = np.diag(np.concatenate([[h], eps])); H[0, 1:] = H[1:, 0] = Vk
+        assert np.allclose(E, np.linalg.eigvalsh(H), atol=1e-14)
+        assert np.allclose(U.T @ H @ U, np.diag(E), atol=1e-13)
+        assert np.isclose(dh, -mw2 * 20.6097)
+
+
+def _na_energy(md, mass, x, v, occ=None, psi=None):
+    oracle = __import__("oracle")
+    E, _, _ = oracle.na_eigen(md, x)
+    e = 0.5 * mass * v ** 2 + 0.5 * md.mw2 * x ** 2
+    if occ is not None:
+        return e + E[occ].sum()
+    return e + np.sum(E[:, None] * np.abs(psi) ** 2)
+
+
+def test_iesh_unitarity_and_hop_energy(oracle, key):
+    md = oracle.na_model(M=30)
+    N, Ne, mass = 31, 15, 2000.0
+    ntraj = 24
+    rng = np.random.default_rng(3)
+    # the impurity level crosses the Fermi level near x = 8
+    r0 = rng.normal(8.0, 2.0, size=ntraj); v0 = np.abs(rng.normal(0.0, 1e-2, size=ntraj))
+    occ0 = np.tile(np.arange(Ne), (ntraj, 1))
+    occ0[:, -1] = Ne  # one electron above the Fermi level
+    out = oracle.iesh_run_ensemble(0, md, Ne, mass, r0, v0, occ0, dt=2.0, nsteps=400, key=key)
+    assert out["nhops"].sum() > 5
+    # the early-exit estimate (iesh.jl:241-244) bounds the total probability, so it changes nothing
+    out2 = oracle.iesh_run_ensemble(0, md, Ne, mass, r0, v0, occ0, dt=2.0, nsteps=400, key=key, estimate=False)
+    assert np.array_equal(out["occ"], out2["occ"]) and np.array_equal(out["r"], out2["r"])
+    for i in range(ntraj):
+        psi = out["psi"][i]
+        assert np.allclose(psi.conj().T @ psi, np.eye(Ne), atol=1e-9)      # propagation is unitary
+        e0 = _na_energy(md, mass, r0[i], v0[i], occ0[i])
+        e1 = _na_energy(md, mass, out["r"][i], out["v"][i], out["occ"][i])
+        assert abs(e1 - e0) < 1e-2 * abs(e0) + 2e-4                          # rtol 1e-2 (iesh.jl test)
+
+
+def test_ehrenfest_na_energy_conservation(oracle, key):
+    md = oracle.na_model(M=20)
+    Ne, mass = 10, 2000.0
+    r0 = np.array([21.0, 15.0]); v0 = np.array([0.0, 1e-3])
+    occ0 = np.tile(np.arange(Ne), (2, 1))
+    out = oracle.iesh_run_ensemble(1, md, Ne, mass, r0, v0, occ0, dt=1.0, nsteps=2000, key=key)
+    for i in range(2):
+        psi0 = np.zeros((21, Ne)); psi0[np.arange(Ne), np.arange(Ne)] = 1
+        e0 = _na_energy(md, mass, r0[i], v0[i], psi=psi0)
+        e1 = _na_energy(md, mass, out["r"][i], out["v"][i], psi=out["psi"][i])
+        assert abs(e1 - e0) < 1e-6
+
+
+def test_hop_probability_determinant_lemma(oracle):
+    """det(S with row n replaced by psi_m) = det(S) (psi_m . S^-1[:, n]) -- the identity the device
+    kernel uses in place of one LU per candidate (iesh.jl:256-266,301-306)."""
+    rng = np.random.default_rng(4)
+    N, Ne = 12, 5
+    A = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    Q, _ = np.linalg.qr(A)
+    psi = Q[:, :Ne]
+    occ = np.array([0, 2, 3, 7, 9])
+    S = psi[occ, :]
+    Sinv = np.linalg.inv(S)
+    d0 = oracle.det(S)
+    for n in range(Ne):
+        for m in (1, 4, 11):
+            Sn = S.copy(); Sn[n, :] = psi[m, :]
+            assert np.isclose(oracle.det(Sn), d0 * (psi[m, :] @ Sinv[:, n]), rtol=1e-9)
+
+
+def test_edc_keeps_each_electron_normalised(oracle, key):
+    md = oracle.na_model(M=20)
+    Ne, mass, ntraj = 10, 2000.0, 8
+    rng = np.random.default_rng(5)
+    r0 = rng.normal(8.0, 1.0, size=ntraj); v0 = np.abs(rng.normal(0.0, 1e-2, size=ntraj))
+    occ0 = np.tile(np.arange(Ne), (ntraj, 1))
+    out = oracle.iesh_run_ensemble(0, md, Ne, mass, r0, v0, occ0, dt=2.0, nsteps=200, key=key, edc=True)
+    norms = np.sum(np.abs(out["psi"]) ** 2, axis=1)
+    assert np.allclose(norms, 1.0, atol=1e-12)
