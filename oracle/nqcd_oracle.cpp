@@ -1041,6 +1041,13 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
         na_matrices(md, x, V, dh, U0, dU0);
         if (have) Up = U;
         eigen_sym(N, V.data(), E.data(), U.data(), have ? Up.data() : nullptr);
+        if (!have) {
+            // Initial gauge: impurity component positive.  The impurity component 1/sqrt(1 + sum V^2/(E-eps_k)^2)
+            // never vanishes, so this gauge is continuous in x and the overlap tracking below keeps it.
+            // (The reference takes whatever signs zheevr/dsyevr return first, then tracks them.)
+            for (int c = 0; c < N; ++c)
+                if (U[c] < 0) for (int a = 0; a < N; ++a) U[(size_t)a * N + c] = -U[(size_t)a * N + c];
+        }
         have = true;
         std::vector<double> dV((size_t)N * N, 0.0);
         dV[0] = dh;

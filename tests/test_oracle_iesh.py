@@ -111,3 +111,25 @@ def test_edc_keeps_each_electron_normalised(oracle, key):
     out = oracle.iesh_run_ensemble(0, md, Ne, mass, r0, v0, occ0, dt=2.0, nsteps=200, key=key, edc=True)
     norms = np.sum(np.abs(out["psi"]) ** 2, axis=1)
     assert np.allclose(norms, 1.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("h_over_gamma", [-1.0, 0.0, 0.5, 2.0])
+def test_newns_anderson_impurity_occupation_wide_band(oracle, h_over_gamma):
+    """Thermal impurity occupation of the discretised Newns-Anderson model against the wide-band
+    Lorentzian n_d = int rho(w) f(w) dw, rho = (Gamma/2pi) / ((w - h)^2 + (Gamma/2)^2), Fermi level 0
+    (at T = 0 this is 1/2 - atan(2h/Gamma)/pi).  Pins the bath coupling V_k = sqrt(Gamma Delta_eps / 2pi)."""
+    from scipy.integrate import quad
+    Gamma = 6.4e-3
+    md = oracle.na_model(Gamma=Gamma, M=400, W=20 * Gamma)
+    kT = 2 * (md.b - md.a) / (md.M - 1)          # a few level spacings smooths the discretisation
+    h = h_over_gamma * Gamma
+    x = (0.5 * md.mw2 * md.g ** 2 + md.dG - h) / (md.mw2 * md.g)
+    E, U, _ = oracle.na_eigen(md, x)
+    f = lambda w: 0.5 * (1 - np.tanh(w / (2 * kT)))
+    nd = np.sum(U[0] ** 2 * f(E))
+    rho = lambda w: (Gamma / (2 * np.pi)) / ((w - h) ** 2 + (Gamma / 2) ** 2)
+    L = 400 * Gamma
+    expected = quad(lambda w: rho(w) * f(w), -L, L, points=[0.0, h], limit=400)[0] + quad(rho, -np.inf, -L)[0]
+    if h_over_gamma == 0.0:
+        assert abs(expected - 0.5) < 1e-9
+    assert abs(nd - expected) < 0.01          # finite band (W = 20 Gamma): |error| <= 0.006; Gamma off by 2x: 0.08
