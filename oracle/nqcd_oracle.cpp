@@ -1025,6 +1025,53 @@ struct OIEDesc {
     int32_t save_every, nsave;
 };
 
+// IESH hopping probabilities prob[m * Ne + n] for electron n -> unoccupied state m
+// (iesh.jl:231-283): S_ji = psi[occ_j, i], A_kk = |det S|^2, v_dot_d[m, n] = -v d[m, occ_n],
+// P = 2 dt Re(det S conj(det S_{n<-m})) v_dot_d / A_kk, each clamped to [0, 1].  Returns false when
+// the early-exit estimate (iesh.jl:241-244) is below the uniform, leaving prob all zero.
+static bool ie_probabilities(int N, int Ne, const std::vector<cd>& psi, const std::vector<int>& occ,
+                             const std::vector<double>& d, double v, double dt, bool estimate, double rnd,
+                             std::vector<double>& prob, bool clamp = true) {
+    prob.assign((size_t)N * Ne, 0.0);
+    std::vector<cd> S((size_t)Ne * Ne);
+    for (int j = 0; j < Ne; ++j) for (int e = 0; e < Ne; ++e) S[(size_t)j * Ne + e] = psi[(size_t)occ[j] * Ne + e];
+    const cd det_cur = det_lu(Ne, S);
+    const double Akk = std::norm(det_cur);
+    const double pref = 2.0 * dt / Akk;
+    std::vector<char> isocc(N, 0);
+    for (int j = 0; j < Ne; ++j) isocc[occ[j]] = 1;
+    double sumvd = 0.0;
+    for (int n = 0; n < Ne; ++n)
+        for (int m = 0; m < N; ++m)
+            if (!isocc[m]) sumvd += std::fabs(-v * d[(size_t)m * N + occ[n]]);
+    if (estimate) {
+        const double est = pref * sumvd * (std::fabs(det_cur.real()) + std::fabs(det_cur.imag()));
+        if (est < rnd) return false;
+    }
+    for (int n = 0; n < Ne; ++n)
+        for (int m = 0; m < N; ++m) {
+            if (isocc[m]) continue;
+            std::vector<cd> Sn(S);
+            for (int e = 0; e < Ne; ++e) Sn[(size_t)n * Ne + e] = psi[(size_t)m * Ne + e];
+            const cd Akj = det_cur * std::conj(det_lu(Ne, Sn));
+            double p = pref * Akj.real() * (-v * d[(size_t)m * N + occ[n]]);
+            if (clamp) p = std::min(std::max(p, 0.0), 1.0);
+            prob[(size_t)m * Ne + n] = p;
+        }
+    return true;
+}
+
+OR_API void oracle_iesh_probabilities(int32_t N, int32_t Ne, const double* psi_re, const double* psi_im,
+                                      const int32_t* occ, const double* d, double v, double dt, int32_t clamp,
+                                      double* prob) {
+    std::vector<cd> psi((size_t)N * Ne);
+    for (int k = 0; k < N * Ne; ++k) psi[k] = cd(psi_re[k], psi_im[k]);
+    std::vector<int> o(occ, occ + Ne);
+    std::vector<double> dd(d, d + (size_t)N * N), p;
+    ie_probabilities(N, Ne, psi, o, dd, v, dt, false, 0.0, p, clamp != 0);
+    std::copy(p.begin(), p.end(), prob);
+}
+
 static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t gid, double r0, double v0,
                        const int32_t* occ0, const double* psi_re0, const double* psi_im0,
                        double* r_out, double* v_out, double* psi_re, double* psi_im, int32_t* occ_out,
@@ -1067,6 +1114,15 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
         if (!pop_imp || slot >= R.nsave) return;
         // iesh_diabatic_population (iesh.jl:351-369), impurity (diabatic state 0) component
         double p = 0.0;
+        if (R.method == 1) {  // EhrenfestNA mean field: sum_e |(U psi)_{0e}|^2
+            for (int e = 0; e < Ne; ++e) {
+                cd s = 0.0;
+                for (int a = 0; a < N; ++a) s += U[a] * psi[(size_t)a * Ne + e];
+                p += std::norm(s);
+            }
+            pop_imp[(size_t)i * R.nsave + slot] = p;
+            return;
+        }
         for (int e = 0; e < Ne; ++e) {
             double s1 = 0.0, s2 = 0.0;
             for (int a = 0; a < N; ++a) { const double c = psi[(size_t)a * Ne + e].real(); s1 += U[a] * c; s2 += U[a] * U[a] * c * c; }
@@ -1110,34 +1166,18 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
         t += R.dt;
         if (R.method == 0) {
             const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
-            // overlap S[j][e] = psi[occ[j], e]
-            for (int j = 0; j < Ne; ++j) for (int e = 0; e < Ne; ++e) S[(size_t)j * Ne + e] = psi[(size_t)occ[j] * Ne + e];
-            const cd det_cur = det_lu(Ne, S);
-            const double Akk = std::norm(det_cur);
-            const double pref = 2.0 * R.dt / Akk;
-            std::vector<char> isocc(N, 0);
-            for (int j = 0; j < Ne; ++j) isocc[occ[j]] = 1;
-            std::vector<int> unocc;
-            for (int a = 0; a < N; ++a) if (!isocc[a]) unocc.push_back(a);
-            // v_dot_d[m, n] = -v d[m, occ[n]]
-            double sumvd = 0.0;
-            for (int n = 0; n < Ne; ++n) for (int m : unocc) sumvd += std::fabs(-v * d[(size_t)m * N + occ[n]]);
-            bool evaluate = true;
-            if (R.estimate) {
-                const double est = pref * sumvd * (std::fabs(det_cur.real()) + std::fabs(det_cur.imag()));
-                if (est < rnd) evaluate = false;
-            }
+            // hopping probabilities (iesh.jl:231-283), selection (iesh.jl:318-335)
+            std::vector<double> prob;
+            const bool evaluate = ie_probabilities(N, Ne, psi, occ, d, v, R.dt, R.estimate != 0, rnd, prob);
             if (evaluate) {
+                std::vector<char> isocc(N, 0);
+                for (int j = 0; j < Ne; ++j) isocc[occ[j]] = 1;
                 double cum = 0.0;
                 int hn = -1, hm = -1;
                 for (int n = 0; n < Ne && hn < 0; ++n)
-                    for (int m : unocc) {
-                        std::vector<cd> Sn(S);
-                        for (int e = 0; e < Ne; ++e) Sn[(size_t)n * Ne + e] = psi[(size_t)m * Ne + e];
-                        const cd Akj = det_cur * std::conj(det_lu(Ne, Sn));
-                        double prob = pref * Akj.real() * (-v * d[(size_t)m * N + occ[n]]);
-                        prob = std::min(std::max(prob, 0.0), 1.0);
-                        cum += prob;
+                    for (int m = 0; m < N; ++m) {
+                        if (isocc[m]) continue;
+                        cum += prob[(size_t)m * Ne + n];
                         if (rnd < cum) { hn = n; hm = m; break; }
                     }
                 if (hn >= 0) {

@@ -333,3 +333,15 @@ def iesh_run_ensemble(method, md, nelec, mass, r0, v0, occ0, psi0=None, dt=1.0, 
       C.c_int32(nthreads))
     out["psi"] = out["psi_re"] + 1j * out["psi_im"]
     return out
+
+
+def iesh_probabilities(psi, occ, d, v, dt, clamp=True):
+    """prob[m, n] for electron n hopping to unoccupied state m (no early exit)."""
+    psi = np.asarray(psi, dtype=complex)
+    N, Ne = psi.shape
+    out = np.zeros((N, Ne))
+    lib().oracle_iesh_probabilities(C.c_int32(N), C.c_int32(Ne), _p(np.ascontiguousarray(psi.real)),
+                                    _p(np.ascontiguousarray(psi.imag)), _p(np.ascontiguousarray(occ, dtype=np.int32), C.c_int32),
+                                    _p(np.ascontiguousarray(d, dtype=np.float64)), C.c_double(v), C.c_double(dt),
+                                    C.c_int32(int(clamp)), _p(out))
+    return out

@@ -133,3 +133,67 @@ def test_newns_anderson_impurity_occupation_wide_band(oracle, h_over_gamma):
     if h_over_gamma == 0.0:
         assert abs(expected - 0.5) < 1e-9
     assert abs(nd - expected) < 0.01          # finite band (W = 20 Gamma): |error| <= 0.006; Gamma off by 2x: 0.08
+
+
+def _na_nac(oracle, md, x):
+    E, U, dh = oracle.na_eigen(md, x)
+    U = U * np.sign(U[0])  # impurity-positive gauge
+    G = dh * np.outer(U[0], U[0])
+    with np.errstate(divide="ignore"):
+        d = G / (E[None, :] - E[:, None])
+    np.fill_diagonal(d, 0.0)
+    return E, U, d
+
+
+def test_iesh_probability_is_the_determinant_population_flux(oracle):
+    """Fewest switches for Slater determinants: the unclamped probabilities summed over (n, m) equal
+    -dt d|det S|^2/dt / |det S|^2 under i dpsi/dt = (diag(E) - i v d) psi.  Pins the prefactor, the
+    sign of v_dot_d and the index order d[m, occ_n] (iesh.jl:256-266,285-298)."""
+    from scipy.linalg import expm
+    md = oracle.na_model(M=12)
+    E, U, d = _na_nac(oracle, md, 7.5)
+    N, Ne, v, dt = 13, 5, 3e-3, 0.7
+    occ = np.array([0, 2, 3, 5, 6])
+    rng = np.random.default_rng(21)
+    X = np.eye(N)[:, occ] + 0.4 * (rng.normal(size=(N, Ne)) + 1j * rng.normal(size=(N, Ne)))
+    psi, _ = np.linalg.qr(X)
+    H = np.diag(E) - 1j * v * d
+    assert np.allclose(H, H.conj().T)
+    A = lambda p: abs(np.linalg.det(p[occ, :])) ** 2
+    h = 1e-2
+    dA = (A(expm(-1j * H * h) @ psi) - A(expm(1j * H * h) @ psi)) / (2 * h)
+    prob = oracle.iesh_probabilities(psi, occ, d, v, dt, clamp=False)
+    assert np.all(prob[occ, :] == 0.0)
+    assert abs(prob.sum() - (-dt * dA / A(psi))) < 1e-7 * max(1.0, abs(prob).sum())
+    assert abs(prob).sum() > 1e-4
+
+
+def test_ehrenfest_na_against_exact_equations_of_motion(oracle):
+    """EhrenfestNA with VerletwithElectronics at dt = 10 against a tight Runge-Kutta solution of
+    dx/dt = v, m dv/dt = -dU0/dx - sum_e psi_e^H (dh u u^T) psi_e, i dpsi/dt = (diag(E) - i v d) psi
+    (the reference's own check against Vern9 at rtol 1e-3, test/Dynamics/ehrenfest_na.jl:23-49)."""
+    from scipy.integrate import solve_ivp
+    md = oracle.na_model(M=30)
+    N, Ne, mass = 31, 15, 2000.0
+    x0, v0, T, dt = 21.0, 0.0, 2000.0, 10.0
+    occ0 = np.arange(Ne)[None, :]
+    out = oracle.iesh_run_ensemble(1, md, Ne, mass, np.array([x0]), np.array([v0]), occ0, dt=dt, nsteps=int(T / dt))
+
+    def rhs(t, y):
+        x, v = y[0], y[1]
+        psi = (y[2:2 + N * Ne] + 1j * y[2 + N * Ne:]).reshape(N, Ne)
+        E, U, d = _na_nac(oracle, md, x)
+        dh = md.mw2 * (x - md.g) - md.mw2 * x
+        s = U[0] @ psi
+        f = md.mw2 * x + dh * np.sum(np.abs(s) ** 2)
+        dpsi = -1j * (E[:, None] * psi) - v * (d @ psi)
+        return np.concatenate([[v, -f / mass], dpsi.real.ravel(), dpsi.imag.ravel()])
+
+    psi0 = np.zeros((N, Ne), complex); psi0[np.arange(Ne), np.arange(Ne)] = 1
+    y0 = np.concatenate([[x0, v0], psi0.real.ravel(), psi0.imag.ravel()])
+    sol = solve_ivp(rhs, (0.0, T), y0, method="DOP853", rtol=1e-11, atol=1e-12)
+    yT = sol.y[:, -1]
+    assert np.isclose(out["r"][0] - x0, yT[0] - x0, rtol=1e-3)      # displacement ~0.075
+    assert np.isclose(out["v"][0], yT[1], rtol=1e-3)
+    psiT = (yT[2:2 + N * Ne] + 1j * yT[2 + N * Ne:]).reshape(N, Ne)
+    assert np.allclose(out["psi"][0], psiT, rtol=1e-2, atol=1e-2)
