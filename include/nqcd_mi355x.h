@@ -88,8 +88,16 @@ enum nqcd_model_kind {
     NQCD_MODEL_TULLY3 = 2,       /* params: A, B, C                            */
     NQCD_MODEL_SPIN_BOSON = 3,   /* params: eps, Delta, omega[nb], c[nb]       */
     NQCD_MODEL_DW_GAUSSIAN = 4,  /* params: a, b, offset, C, D                 */
-    NQCD_MODEL_HARMONIC = 5      /* params: m, omega, r0                       */
+    NQCD_MODEL_HARMONIC = 5,     /* params: m, omega, r0                       */
+    /* Newns-Anderson impurity (MiaoSubotnik diabats) with a trapezoidal bath discretisation:
+     * params: m, omega, g, dG, Gamma, a, b, M.  U0 = m w^2 x^2/2, impurity h = m w^2 (x-g)^2/2 + dG - U0,
+     * bath eps_k = a + k (b-a)/(M-1), coupling sqrt(Gamma (b-a)/(M-1) / 2pi); n_states = M + 1. */
+    NQCD_MODEL_NEWNS_ANDERSON = 6
 };
+
+/* IESH decoherence keyword (iesh.jl:72-73): none, or Granucci-Persico EDC
+ * (decoherence_corrections.jl:10-39). */
+enum nqcd_decoherence { NQCD_DECOHERENCE_NONE = 0, NQCD_DECOHERENCE_EDC = 1 };
 
 typedef struct {
     int32_t method;              /* enum nqcd_method */
@@ -103,11 +111,13 @@ typedef struct {
     int32_t precision;           /* enum nqcd_precision */
     int32_t schedule;            /* enum nqcd_schedule */
     int32_t electronic_substeps; /* Tsit5 sub-steps per nuclear step; 0 -> 5 (bab_electronics.jl:42) */
-    int32_t reserved0;
+    int32_t decoherence;         /* IESH: enum nqcd_decoherence */
     const double* masses;        /* natoms (simulations.jl:86-91) */
     double dt;                   /* nuclear time step (run_dynamics.jl:71-73 default 1.0) */
     double temperature;          /* ring polymers: omega_n = nbeads * temperature (ring_polymer.jl:18) */
-    int32_t reserved[8];
+    double edc_C;                /* EDC constant C (Hartree); 0 -> 0.1 (decoherence_corrections.jl:14) */
+    int32_t disable_estimate;    /* IESH: 1 = estimate_probability=false (iesh.jl:73,241-244) */
+    int32_t reserved[7];
 } nqcd_problem_desc;
 
 /* TerminatingCallback with the predicate |x_dof| > value (DynamicsUtils/callbacks.jl:15-29). */
@@ -152,7 +162,10 @@ enum nqcd_output_kind {
     NQCD_OUT_SCATTERING_1D = 2,          /* final, [reflection(n), transmission(n)] (DynamicsOutputs.jl:313-338) */
     NQCD_OUT_POPULATION_CORRELATION = 3, /* per save slot, n x n P(0)P(t)^T diabatic (TimeCorrelationFunctions.jl:61-88) */
     NQCD_OUT_FINAL_ADIABATIC_POPULATION = 4, /* final, nstates */
-    NQCD_OUT_CENTROID_POSITION = 5       /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
+    NQCD_OUT_CENTROID_POSITION = 5,      /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
+    NQCD_OUT_IMPURITY_POPULATION = 6     /* per save slot, IESH / EhrenfestNA impurity population: the impurity
+                                            entry of the IESH diabatic population (iesh.jl:337-369); for
+                                            EhrenfestNA sum_e |(U psi)_{impurity, e}|^2 */
 };
 /* Reductions (src/Ensembles/reductions.jl:8-52). */
 enum nqcd_reduction_kind { NQCD_RED_MEAN = 0, NQCD_RED_SUM = 1 };
@@ -169,8 +182,11 @@ enum nqcd_field {
     NQCD_FIELD_SAVE_POP_AD = 7,  /* double (n, nsave, ntraj) adiabatic population per save     */
     NQCD_FIELD_SAVE_POP_DI = 8,  /* double (n, nsave, ntraj) diabatic population per save      */
     NQCD_FIELD_SAVE_STATE = 9,   /* int32 (nsave, ntraj) active state per save                 */
-    NQCD_FIELD_SAVE_CENTROID = 10 /* double (nsave, ntraj) ring-polymer centroid per save       */
+    NQCD_FIELD_SAVE_CENTROID = 10, /* double (nsave, ntraj) ring-polymer centroid per save      */
+    NQCD_FIELD_SAVE_IMPURITY = 11  /* double (nsave, ntraj) IESH / EhrenfestNA impurity population */
 };
+/* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
+ * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD (n, nsave, ntraj). */
 
 typedef struct {
     double kernel_ms;            /* device time of the trajectory kernels (HIP events) */
@@ -203,7 +219,10 @@ int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t model_kind
 
 /* Upload initial conditions for ntraj trajectories (the DynamicsVariables built by
  * sample_distribution/prob_func, selections.jl:38-99, fssh.jl:47-65, ehrenfest.jl:43-48).
- * state may be NULL (sampled from diag(sigma), fssh.jl:54) and model_id may be NULL (slot 0). */
+ * state may be NULL (sampled from diag(sigma), fssh.jl:54) and model_id may be NULL (slot 0).
+ * IESH / EhrenfestNA (iesh.jl:89-184, ehrenfest_na.jl:43-58): state holds the occupied adiabatic
+ * states (n_electrons, ntraj), sig_re/sig_im the wavefunction (n, n_electrons, ntraj) or both NULL
+ * for psi[state_e, e] = 1. */
 int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const double* v,
                                  const double* sig_re, const double* sig_im, const int32_t* state,
                                  const int32_t* model_id);

@@ -20,6 +20,7 @@
 #include "handle.h"
 #include "spin_boson.h"
 #include "ring_polymer.h"
+#include "iesh.h"
 
 using namespace nqcd;
 
@@ -50,6 +51,10 @@ bool is_spin_boson(const nqcd_handle* h) {
 bool is_rp(const nqcd_handle* h) {
     return h->prob.method == NQCD_METHOD_RPMD || h->prob.method == NQCD_METHOD_RPSH ||
            h->prob.method == NQCD_METHOD_RP_EHRENFEST;
+}
+
+bool is_ie(const nqcd_handle* h) {
+    return h->prob.method == NQCD_METHOD_IESH || h->prob.method == NQCD_METHOD_EHRENFEST_NA;
 }
 
 // rows of the per-trajectory position / velocity arrays (dofs x atoms x beads)
@@ -158,7 +163,8 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
     (void)hipStreamSynchronize(h->stream);
     DevBuf* bufs[] = {&h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
-                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid};
+                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid,
+                      &h->ie_scratch};
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(h->ev0);
     (void)hipEventDestroy(h->ev1);
@@ -177,9 +183,11 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     if (d->schedule < 0 || d->schedule > 1) return fail(h, NQCD_ERR_ARG, "unknown schedule");
     const bool sh_or_eh = d->method == NQCD_METHOD_FSSH || d->method == NQCD_METHOD_EHRENFEST;
     const bool rp = d->method == NQCD_METHOD_RPMD || d->method == NQCD_METHOD_RPSH || d->method == NQCD_METHOD_RP_EHRENFEST;
-    if (!rp && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states == 2))
+    const bool ie = d->method == NQCD_METHOD_IESH || d->method == NQCD_METHOD_EHRENFEST_NA;
+    if (!rp && !ie && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states == 2))
         return fail(h, NQCD_ERR_UNSUPPORTED,
-                    "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (two states, BABwithTsit5) and "
+                    "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (two states, BABwithTsit5), "
+                    "Simulation{AdiabaticIESH|EhrenfestNA} (VerletwithElectronics) and "
                     "RingPolymerSimulation{Classical|FSSH|Ehrenfest} (BCB / BCBwithTsit5)");
     h->prob = *d;
     h->masses.assign(d->masses, d->masses + d->natoms);
@@ -191,6 +199,10 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     h->prob.masses = nullptr;
     if (rp) {
         int32_t rc = rp_check_problem(h);
+        if (rc != NQCD_OK) return rc;
+    }
+    if (ie) {
+        int32_t rc = ie_check_problem(h);
         if (rc != NQCD_OK) return rc;
     }
     h->have_problem = true;
@@ -211,6 +223,7 @@ int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t kind, cons
     case NQCD_MODEL_TULLY3: need = 3; break;
     case NQCD_MODEL_DW_GAUSSIAN: need = 5; break;
     case NQCD_MODEL_HARMONIC: need = 3; break;
+    case NQCD_MODEL_NEWNS_ANDERSON: return ie_set_model(h, slot, params, n);
     case NQCD_MODEL_SPIN_BOSON: {
         if (slot != 0) return fail(h, NQCD_ERR_UNSUPPORTED, "spin-boson model must use slot 0");
         if (n < 4 || (n - 2) % 2 != 0) return fail(h, NQCD_ERR_ARG, "spin-boson params: eps, Delta, omega[nb], c[nb]");
@@ -238,6 +251,13 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
     if (!h) return fail(h, NQCD_ERR_ARG, "null handle");
     if (!h->have_problem) return fail(h, NQCD_ERR_STATE, "set_problem must be called first");
     if (ntraj < 1 || ntraj > (int64_t)1 << 31) return fail(h, NQCD_ERR_ARG, "ntraj out of range");
+    if (is_ie(h)) {
+        if (!r || !v) return fail(h, NQCD_ERR_ARG, "null input array");
+        HIPCHK(h, hipSetDevice(h->device));
+        int32_t rc = ie_load_initial(h, ntraj, r, v, sig_re, sig_im, state);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     if (!r || !v || ((!sig_re || !sig_im) && h->prob.method != NQCD_METHOD_RPMD))
         return fail(h, NQCD_ERR_ARG, "null input array");
     const int D = h->D, n = h->prob.n_states;
@@ -308,6 +328,11 @@ int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S,
     const int64_t ntraj = S->ntraj;
     const int D = h->D;
     HIPCHK(h, hipSetDevice(h->device));
+    if (is_ie(h)) {
+        int32_t rc = ie_sample_initial(h, S, seed);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     if (is_rp(h)) {
         int32_t rc = rp_sample_initial(h, S, seed);
         if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
@@ -398,6 +423,9 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
     if (is_rp(h)) {
         int32_t rc = rp_run(h, t0, nsteps, term, key);
         if (rc != NQCD_OK) return rc;
+    } else if (is_ie(h)) {
+        int32_t rc = ie_run(h, t0, nsteps, term, key);
+        if (rc != NQCD_OK) return rc;
     } else if (is_spin_boson(h)) {
         if (!h->sb_initial) return fail(h, NQCD_ERR_STATE, "spin-boson runs need nqcd_mi355x_sample_initial");
         int32_t rc = sb_run(h, t0, nsteps, term, key);
@@ -476,7 +504,14 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* 
     int32_t rows = 0;
     const double* src = nullptr;
     HIPCHK(h, hipSetDevice(h->device));
-    switch (kind) {
+    if (is_ie(h)) {
+        if (kind == NQCD_OUT_ADIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_ad.as<double>(); }
+        else if (kind == NQCD_OUT_IMPURITY_POPULATION) { rows = h->nsave; src = h->save_di.as<double>(); }
+        else return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA outputs: ADIABATIC_POPULATION, IMPURITY_POPULATION");
+    } else if (kind == NQCD_OUT_IMPURITY_POPULATION) {
+        return fail(h, NQCD_ERR_UNSUPPORTED, "impurity population needs IESH / EhrenfestNA");
+    }
+    if (!src) switch (kind) {
     case NQCD_OUT_ADIABATIC_POPULATION: rows = n * h->nsave; src = h->save_ad.as<double>(); break;
     case NQCD_OUT_DIABATIC_POPULATION: rows = n * h->nsave; src = h->save_di.as<double>(); break;
     case NQCD_OUT_FINAL_ADIABATIC_POPULATION: rows = n; break;
@@ -512,6 +547,7 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
     const int64_t nt = h->ntraj;
     const int D = (int)nqcd_rows(h), n = h->prob.n_states == 1 ? 2 : h->prob.n_states, ns = h->nsave;
     HIPCHK(h, hipSetDevice(h->device));
+    if (is_ie(h)) return ie_download(h, field, out, bytes);
     auto fetch = [&](const DevBuf& b, size_t sz, std::vector<char>& host) -> hipError_t {
         host.resize(sz);
         return hipMemcpy(host.data(), b.p, sz, hipMemcpyDeviceToHost);

@@ -1,0 +1,828 @@
+// iesh.hip -- independent-electron surface hopping (IESH) and EhrenfestNA on the Newns-Anderson
+// impurity model, one trajectory per workgroup.
+//
+// Reference path: VerletwithElectronics (src/DynamicsMethods/IntegrationAlgorithms/
+// verlet_with_electronics.jl:42-69), the wavefunction propagator (src/DynamicsUtils/
+// wavefunction_dynamics.jl:15-58), the IESH force and hop (src/DynamicsMethods/SurfaceHoppingMethods/
+// iesh.jl:190-207,231-335,390-441) and the EhrenfestNA force (src/DynamicsMethods/EhrenfestMethods/
+// ehrenfest_na.jl:72-86).
+//
+// Model: N = M + 1 states, impurity h(x) = U1 - U0 coupled with a constant V to M bath levels
+// eps_k = a + k (b - a)/(M - 1), V = sqrt(Gamma (b - a)/(M - 1) / 2 pi).  The Hamiltonian is an
+// arrowhead matrix, so per step:
+//   * eigen: root i of the secular function f(l) = l - h - V^2 sum_k 1/(l - eps_k) lies in
+//     (eps_{i-1}, eps_i); safeguarded Newton from the previous step's root, one thread per root.
+//     The impurity component of eigenvector i is u_i = f'(E_i)^(-1/2) > 0 (a continuous gauge).
+//   * dV/dx only touches the impurity entry, so U^T dV U = dh u u^T and the nonadiabatic coupling
+//     is d_ij = dh u_i u_j / (E_j - E_i).
+//   * propagator: psi <- exp(-i dt (diag(E) - i v d)) psi, applied as a Taylor series whose length
+//     follows from a Frobenius bound of dt H (remainder below 1e-17); each term is one real GEMM
+//     (-dt v dh W) [Re psi | Im psi] on FP64 MFMA 16x16x4 plus the diagonal part.  The reference
+//     diagonalises H with zheevr and forms U exp(-i Lambda dt) U^H; both give exp(-i H dt) to
+//     rounding.
+//   * IESH hop: S = psi[occ, :], det S by LU (partial pivoting) on one wavefront, the reference's
+//     early-exit estimate, and on the (rare) steps that pass it the replaced-row determinants by the
+//     determinant lemma det(S with row n <- psi_m) = det(S) (psi_m S^-1)_n, so
+//     P(n -> m) = 2 dt Re((psi_m S^-1)_n) v_dot_d[m, n]; cumulative selection electrons outer,
+//     unoccupied states inner (iesh.jl:318-335).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ensemble.h"
+#include "handle.h"
+#include "iesh.h"
+#include "nqcd_device.h"
+
+namespace nqcd {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIeMaxNe = 64;
+constexpr int kIeMaxStates = 128;
+
+struct IeshArgs {
+    int64_t ntraj;
+    int64_t nsteps;
+    int32_t N, M, Ne, cw;         // states, bath levels, electrons, padded psi columns (16 * column tiles)
+    int32_t method;               // 0 = IESH, 1 = EhrenfestNA
+    int32_t rescaling, estimate, edc;
+    int32_t save_every, nsave;
+    double dt, mass, edc_C;
+    uint64_t key;
+    double mw2, g, dG, V, a, de;  // model
+    const double* r0; const double* v0; const int32_t* occ0; const double* psi0;  // psi0 may be null
+    double* r; double* v; double* psi; int32_t* occ; int32_t* nhops; int32_t* steps;
+    double* save_imp;             // (nsave, ntraj)
+    double* save_ad;              // (nsave, N, ntraj)
+    double* scratch;              // per workgroup 2 Ne^2 doubles (inverse columns on hop steps)
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Sum over the workgroup, same value returned to every thread; fixed reduction order.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) s += red[j];
+    return s;
+}
+
+// Lane-level ordering of LDS traffic inside one wavefront (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct Scalars { double h, dh, dU0; };
+
+__device__ __forceinline__ Scalars na_scalars(const IeshArgs& A, double x) {
+    Scalars s;
+    const double U0 = 0.5 * A.mw2 * x * x;
+    const double U1 = 0.5 * A.mw2 * (x - A.g) * (x - A.g) + A.dG;
+    s.h = U1 - U0;
+    s.dh = A.mw2 * (x - A.g) - A.mw2 * x;
+    s.dU0 = A.mw2 * x;
+    return s;
+}
+
+// Root i of the secular function in (lo, hi); returns E_i and f'(E_i).
+__device__ double secular_root(const double* eps, int M, double V2, double h, double lo, double hi, double guess,
+                               double tol, double& fp) {
+    double lam = (guess > lo && guess < hi) ? guess : 0.5 * (lo + hi);
+    for (int it = 0; it < 100; ++it) {
+        double s = 0.0, s2 = 0.0;
+        for (int k = 0; k < M; ++k) {
+            const double q = 1.0 / (lam - eps[k]);
+            s += q;
+            s2 += q * q;
+        }
+        const double f = lam - h - V2 * s;
+        const double d = 1.0 + V2 * s2;
+        if (f > 0.0) hi = lam;
+        else if (f < 0.0) lo = lam;
+        else break;
+        double nl = lam - f / d;
+        if (!(nl > lo && nl < hi)) nl = 0.5 * (lo + hi);
+        const bool done = fabs(nl - lam) <= tol || (hi - lo) <= tol;
+        lam = nl;
+        if (done) break;
+    }
+    double s2 = 0.0;
+    for (int k = 0; k < M; ++k) {
+        const double q = 1.0 / (lam - eps[k]);
+        s2 += q * q;
+    }
+    fp = 1.0 + V2 * s2;
+    return lam;
+}
+
+template <int NT, int CTM>
+__global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
+    constexpr int NP = 16 * NT;  // padded state count (rows of psi / W tiles)
+    constexpr int B = 64 * NT;
+    constexpr int KF = 4 * NT;   // A-fragments per lane (K = NP in steps of 4)
+    extern __shared__ double smem[];
+    const int N = A.N, M = A.M, Ne = A.Ne, CW = A.cw, CT = A.cw / 16;
+    double* Tb = smem;                       // NP x CW: psi (Re, Im interleaved per electron) / Taylor term
+    double* Sb = Tb + NP * CW;               // Ne x Ne complex, LU in place
+    double* E = Sb + 2 * Ne * Ne;            // NP adiabatic energies
+    double* U = E + NP;                      // NP impurity components of the eigenvectors
+    double* eps = U + NP;                    // NP bath levels
+    double* red = eps + NP;                  // NT reduction slots
+    double* pcol = red + 8;                  // NP per-state / per-electron temporaries
+    double* scal = pcol + NP;                // 8 broadcast scalars
+    int* occ = reinterpret_cast<int*>(scal + 8);  // kIeMaxNe occupied states
+    int* isocc = occ + kIeMaxNe;                  // NP electron index occupying the state, -1 if empty
+    int* piv = isocc + NP;                        // kIeMaxNe LU pivots
+
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const double dt = A.dt;
+    const double V2 = A.V * A.V;
+    for (int k = tid; k < NP; k += B) eps[k] = (k < M) ? A.a + k * A.de : 0.0;
+    __syncthreads();
+    const double zn = sqrt((double)M) * A.V;
+    double* Xg = A.scratch + (size_t)blockIdx.x * 2 * Ne * Ne;
+
+    for (int64_t traj = blockIdx.x; traj < A.ntraj; traj += gridDim.x) {
+        __syncthreads();
+        // ---- load the trajectory --------------------------------------------------------------
+        for (int idx = tid; idx < NP * CW; idx += B) {
+            const int row = idx / CW, col = idx - row * CW;
+            double val = 0.0;
+            if (A.psi0 && row < N && col < 2 * Ne) val = A.psi0[(((size_t)traj * N + row) * Ne + (col >> 1)) * 2 + (col & 1)];
+            Tb[idx] = val;
+        }
+        for (int s = tid; s < NP; s += B) isocc[s] = -1;
+        if (tid < Ne) occ[tid] = A.occ0[(size_t)traj * Ne + tid];
+        __syncthreads();
+        if (tid < Ne) {
+            isocc[occ[tid]] = tid;
+            if (!A.psi0) Tb[occ[tid] * CW + 2 * tid] = 1.0;  // psi[state_e, e] = 1 (iesh.jl:174-180)
+        }
+        double x = A.r0[traj], v = A.v0[traj];
+        double Eprev = -1.0;
+        bool warm = false;
+        int nh = 0;
+
+        // eigen at x (all threads); leaves E, U in LDS and returns the model scalars
+        auto eigen = [&](double xx) -> Scalars {
+            const Scalars s = na_scalars(A, xx);
+            if (tid < N) {
+                const double lo = (tid == 0) ? fmin(s.h, eps[0]) - zn - 1e-3 * (zn + 1e-300) : eps[tid - 1];
+                const double hi = (tid == M) ? fmax(s.h, eps[M - 1]) + zn + 1e-3 * (zn + 1e-300) : eps[tid];
+                const double scale = fmax(fmax(fabs(A.a), fabs(eps[M - 1])), fabs(s.h)) + zn;
+                double fp;
+                const double lam = secular_root(eps, M, V2, s.h, lo, hi, warm ? Eprev : lo - 1.0, 4.0e-16 * scale, fp);
+                E[tid] = lam;
+                U[tid] = 1.0 / sqrt(fp);
+                Eprev = lam;
+            } else if (tid < NP) {
+                E[tid] = 0.0;
+                U[tid] = 0.0;
+            }
+            warm = true;
+            __syncthreads();
+            return s;
+        };
+        // acceleration at the current eigenpairs with the wavefunction in Tb (iesh.jl:190-207, ehrenfest_na.jl:72-86)
+        auto accel = [&](const Scalars& s) -> double {
+            if (A.method == 0) {
+                if (tid == 0) {
+                    double f = s.dU0;
+                    for (int e = 0; e < Ne; ++e) { const double u = U[occ[e]]; f += u * s.dh * u; }
+                    scal[0] = f;
+                }
+            } else {
+                if (tid < Ne) {
+                    double re = 0.0, im = 0.0;
+                    for (int a = 0; a < N; ++a) {
+                        re += U[a] * Tb[a * CW + 2 * tid];
+                        im += U[a] * Tb[a * CW + 2 * tid + 1];
+                    }
+                    pcol[tid] = s.dh * (re * re + im * im);
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double f = s.dU0;
+                    for (int e = 0; e < Ne; ++e) f += pcol[e];
+                    scal[0] = f;
+                }
+            }
+            __syncthreads();
+            const double f = scal[0];
+            __syncthreads();
+            return -f / A.mass;
+        };
+        auto save = [&](int slot) {
+            if (slot >= A.nsave) return;
+            if (tid < Ne) {
+                const int s = occ[tid];
+                double pe;
+                if (A.method == 0) {  // iesh_diabatic_population (iesh.jl:351-369), impurity row
+                    double s1 = 0.0, s2 = 0.0;
+                    for (int a = 0; a < N; ++a) {
+                        const double c = Tb[a * CW + 2 * tid];
+                        s1 += U[a] * c;
+                        s2 += U[a] * U[a] * c * c;
+                    }
+                    pe = s1 * s1 - s2 + U[s] * U[s];
+                } else {              // mean field: |(U psi)_{0e}|^2
+                    double re = 0.0, im = 0.0;
+                    for (int a = 0; a < N; ++a) {
+                        re += U[a] * Tb[a * CW + 2 * tid];
+                        im += U[a] * Tb[a * CW + 2 * tid + 1];
+                    }
+                    pe = re * re + im * im;
+                }
+                pcol[tid] = pe;
+            }
+            for (int a = tid; a < N; a += B) {
+                double pa;
+                if (A.method == 0) pa = isocc[a] >= 0 ? 1.0 : 0.0;  // one-hot (iesh.jl:371-375)
+                else {                                               // ehrenfest_na.jl:119-126
+                    pa = 0.0;
+                    for (int e = 0; e < Ne; ++e) {
+                        const double re = Tb[a * CW + 2 * e], im = Tb[a * CW + 2 * e + 1];
+                        pa += re * re + im * im;
+                    }
+                }
+                A.save_ad[((size_t)slot * N + a) * A.ntraj + traj] = pa;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double p = 0.0;
+                for (int e = 0; e < Ne; ++e) p += pcol[e];
+                A.save_imp[(size_t)slot * A.ntraj + traj] = p;
+            }
+            __syncthreads();
+        };
+
+        Scalars sc = eigen(x);
+        double kacc = accel(sc);
+        save(0);
+
+        for (int64_t step = 0; step < A.nsteps; ++step) {
+            // ---- nuclei: B A (update) B (verlet_with_electronics.jl:58-66) --------------------------
+            const double vt = v + (dt / 2) * kacc;
+            x = x + dt * vt;
+            sc = eigen(x);
+            kacc = accel(sc);  // EhrenfestNA uses psi of the step start
+            v = vt + (dt / 2) * kacc;
+
+            // ---- wavefunction: psi <- exp(-i dt H) psi, H = diag(E) - i v d ---------------------------
+            {
+                const double sW = dt * v * sc.dh;
+                const int ia = 16 * w + (l & 15);
+                double Af[KF];
+                double loc = 0.0;
+#pragma unroll
+                for (int kk = 0; kk < KF; ++kk) {
+                    const int k = 4 * kk + (l >> 4);
+                    double val = 0.0;
+                    if (ia < N && k < N && ia != k) val = -sW * (U[ia] * U[k] / (E[k] - E[ia]));
+                    Af[kk] = val;
+                    loc += val * val;
+                }
+                if (tid < N) loc += (dt * E[tid]) * (dt * E[tid]);
+                const double theta = sqrt(block_sum<NT>(loc, red));
+                // Sub-steps of norm <= 0.5, at most 1024 of them; a non-finite bound leaves psi as is.
+                const bool finite = theta <= 512.0;
+                const int ns = (finite && theta > 0.5) ? (int)ceil(theta / 0.5) : 1;
+                const double ths = finite ? theta / ns : 0.0, eth = exp(ths);
+                int K = 0;
+                double term = 1.0;
+                while (finite && K < 40 && term * ths / (K + 1) * eth > 1e-17) { ++K; term *= ths / K; }
+                const double inv_ns = 1.0 / ns;
+                if (ns > 1) {
+#pragma unroll
+                    for (int kk = 0; kk < KF; ++kk) Af[kk] *= inv_ns;
+                }
+                double dg[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int ir = 16 * w + 4 * r + (l >> 4);
+                    dg[r] = (ir < N) ? dt * E[ir] * inv_ns : 0.0;
+                }
+                d4 P[CTM];
+#pragma unroll
+                for (int ct = 0; ct < CTM; ++ct) {
+                    if (ct < CT) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) P[ct][r] = Tb[(16 * w + 4 * r + (l >> 4)) * CW + 16 * ct + (l & 15)];
+                    }
+                }
+                // Column tiles are independent in the GEMM: each tile of the next term is computed,
+                // then written back over the same tile after every wave has read it.
+                const double* bcol = Tb + (l >> 4) * CW + (l & 15);
+                for (int sub = 0; sub < ns; ++sub) {
+                    for (int k = 1; k <= K; ++k) {
+                        const double ik = 1.0 / k;
+                        const bool last = (k == K);
+#pragma unroll
+                        for (int ct = 0; ct < CTM; ++ct) {
+                            if (ct < CT) {
+                                d4 Y = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                                for (int kk = 0; kk < KF; ++kk)
+                                    Y = __builtin_amdgcn_mfma_f64_16x16x4f64(Af[kk], bcol[4 * kk * CW + 16 * ct], Y, 0, 0, 0);
+                                const int j = 16 * ct + (l & 15);
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    const double part = Tb[(16 * w + 4 * r + (l >> 4)) * CW + (j ^ 1)];
+                                    Y[r] = (Y[r] + ((j & 1) ? -dg[r] : dg[r]) * part) * ik;
+                                    P[ct][r] += Y[r];
+                                }
+                                __syncthreads();
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) Tb[(16 * w + 4 * r + (l >> 4)) * CW + j] = last ? P[ct][r] : Y[r];
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    if (K == 0) break;
+                }
+            }
+
+            // ---- IESH hop (iesh.jl:390-407) -------------------------------------------------------------
+            if (A.method == 0) {
+                const double rnd = philox_uniform((uint64_t)traj, (uint32_t)step, STREAM_HOP, A.key);
+                for (int idx = tid; idx < Ne * Ne; idx += B) {
+                    const int j = idx / Ne, e = idx - j * Ne;
+                    Sb[2 * idx] = Tb[occ[j] * CW + 2 * e];
+                    Sb[2 * idx + 1] = Tb[occ[j] * CW + 2 * e + 1];
+                }
+                __syncthreads();
+                if (w == 0) {  // LU with partial pivoting, lane = row (FastDeterminant.jl:20-22)
+                    double dre = 1.0, dim = 0.0;
+                    for (int p = 0; p < Ne; ++p) {
+                        double val = -1.0;
+                        int idx = l;
+                        if (l >= p && l < Ne) val = hypot(Sb[2 * (l * Ne + p)], Sb[2 * (l * Ne + p) + 1]);
+#pragma unroll
+                        for (int o = 32; o > 0; o >>= 1) {
+                            const double ov = __shfl_xor(val, o);
+                            const int oi = __shfl_xor(idx, o);
+                            if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
+                        }
+                        if (val == 0.0) { dre = 0.0; dim = 0.0; break; }
+                        if (l == 0) piv[p] = idx;
+                        if (idx != p) {
+                            if (l < Ne) {
+                                const double a0 = Sb[2 * (p * Ne + l)], a1 = Sb[2 * (p * Ne + l) + 1];
+                                Sb[2 * (p * Ne + l)] = Sb[2 * (idx * Ne + l)];
+                                Sb[2 * (p * Ne + l) + 1] = Sb[2 * (idx * Ne + l) + 1];
+                                Sb[2 * (idx * Ne + l)] = a0;
+                                Sb[2 * (idx * Ne + l) + 1] = a1;
+                            }
+                            dre = -dre; dim = -dim;
+                        }
+                        wave_sync();
+                        const double pr = Sb[2 * (p * Ne + p)], pi = Sb[2 * (p * Ne + p) + 1];
+                        const double nre = dre * pr - dim * pi, nim = dre * pi + dim * pr;
+                        dre = nre; dim = nim;
+                        if (l > p && l < Ne) {
+                            const double den = pr * pr + pi * pi;
+                            const double ar = Sb[2 * (l * Ne + p)], ai = Sb[2 * (l * Ne + p) + 1];
+                            const double fr = (ar * pr + ai * pi) / den, fi = (ai * pr - ar * pi) / den;
+                            Sb[2 * (l * Ne + p)] = fr;
+                            Sb[2 * (l * Ne + p) + 1] = fi;
+                            for (int c = p + 1; c < Ne; ++c) {
+                                const double br = Sb[2 * (p * Ne + c)], bi = Sb[2 * (p * Ne + c) + 1];
+                                Sb[2 * (l * Ne + c)] -= fr * br - fi * bi;
+                                Sb[2 * (l * Ne + c) + 1] -= fr * bi + fi * br;
+                            }
+                        }
+                        wave_sync();
+                    }
+                    if (l == 0) { scal[1] = dre; scal[2] = dim; }
+                }
+                // sum over unoccupied m and electrons n of |v_dot_d[m, n]| (iesh.jl:285-298)
+                double loc = 0.0;
+                for (int m = tid; m < N; m += B) {
+                    if (isocc[m] < 0) {
+                        for (int n = 0; n < Ne; ++n) {
+                            const int o = occ[n];
+                            loc += fabs(v * (sc.dh * (U[m] * U[o] / (E[o] - E[m]))));
+                        }
+                    }
+                }
+                const double sumvd = block_sum<NT>(loc, red);  // contains the barrier after the LU
+                const double dre = scal[1], dim = scal[2];
+                const double Akk = dre * dre + dim * dim;
+                const double pref = 2.0 * dt / Akk;
+                const double est = pref * sumvd * (fabs(dre) + fabs(dim));
+                const bool evaluate = Akk > 0.0 && (!A.estimate || !(est < rnd));
+                if (evaluate) {
+                    // columns of S^-1 (wavefront 0, lane = column) into this workgroup's scratch
+                    if (w == 0 && l < Ne) {
+                        for (int e = 0; e < Ne; ++e) { Xg[2 * (e * Ne + l)] = (e == l) ? 1.0 : 0.0; Xg[2 * (e * Ne + l) + 1] = 0.0; }
+                        for (int p = 0; p < Ne; ++p) {
+                            const int q = piv[p];
+                            if (q != p) {
+                                const double t0 = Xg[2 * (p * Ne + l)], t1 = Xg[2 * (p * Ne + l) + 1];
+                                Xg[2 * (p * Ne + l)] = Xg[2 * (q * Ne + l)];
+                                Xg[2 * (p * Ne + l) + 1] = Xg[2 * (q * Ne + l) + 1];
+                                Xg[2 * (q * Ne + l)] = t0;
+                                Xg[2 * (q * Ne + l) + 1] = t1;
+                            }
+                        }
+                        for (int e = 0; e < Ne; ++e) {  // L y = P e_l (unit lower)
+                            double yr = Xg[2 * (e * Ne + l)], yi = Xg[2 * (e * Ne + l) + 1];
+                            for (int j = 0; j < e; ++j) {
+                                const double lr = Sb[2 * (e * Ne + j)], li = Sb[2 * (e * Ne + j) + 1];
+                                const double xr = Xg[2 * (j * Ne + l)], xi = Xg[2 * (j * Ne + l) + 1];
+                                yr -= lr * xr - li * xi;
+                                yi -= lr * xi + li * xr;
+                            }
+                            Xg[2 * (e * Ne + l)] = yr;
+                            Xg[2 * (e * Ne + l) + 1] = yi;
+                        }
+                        for (int e = Ne - 1; e >= 0; --e) {  // U x = y
+                            double yr = Xg[2 * (e * Ne + l)], yi = Xg[2 * (e * Ne + l) + 1];
+                            for (int j = e + 1; j < Ne; ++j) {
+                                const double ur = Sb[2 * (e * Ne + j)], ui = Sb[2 * (e * Ne + j) + 1];
+                                const double xr = Xg[2 * (j * Ne + l)], xi = Xg[2 * (j * Ne + l) + 1];
+                                yr -= ur * xr - ui * xi;
+                                yi -= ur * xi + ui * xr;
+                            }
+                            const double dr = Sb[2 * (e * Ne + e)], di = Sb[2 * (e * Ne + e) + 1];
+                            const double den = dr * dr + di * di;
+                            Xg[2 * (e * Ne + l)] = (yr * dr + yi * di) / den;
+                            Xg[2 * (e * Ne + l) + 1] = (yi * dr - yr * di) / den;
+                        }
+                    }
+                    __syncthreads();
+                    double cum = 0.0;
+                    int hit_n = -1, hit_m = -1;
+                    for (int n = 0; n < Ne; ++n) {
+                        for (int m = tid; m < N; m += B) {
+                            double p = 0.0;
+                            if (isocc[m] < 0) {
+                                double zr = 0.0;
+                                for (int e = 0; e < Ne; ++e)
+                                    zr += Tb[m * CW + 2 * e] * Xg[2 * (e * Ne + n)] - Tb[m * CW + 2 * e + 1] * Xg[2 * (e * Ne + n) + 1];
+                                const int o = occ[n];
+                                const double vd = -v * (sc.dh * (U[m] * U[o] / (E[o] - E[m])));
+                                p = 2.0 * dt * zr * vd;
+                                p = fmin(fmax(p, 0.0), 1.0);
+                            }
+                            pcol[m] = p;
+                        }
+                        __syncthreads();
+                        if (tid == 0) {
+                            for (int m = 0; m < N; ++m) {
+                                if (isocc[m] >= 0) continue;
+                                cum += pcol[m];
+                                if (rnd < cum) { hit_n = n; hit_m = m; break; }
+                            }
+                            scal[3] = (double)hit_n;
+                            scal[4] = (double)hit_m;
+                        }
+                        __syncthreads();
+                        const int hn = (int)scal[3];
+                        __syncthreads();
+                        if (hn >= 0) break;
+                    }
+                    if (tid == 0) {
+                        scal[5] = v;
+                        if (hit_n >= 0) {  // rescale along d[new, old] (surface_hopping.jl:65-92)
+                            const int old = occ[hit_n];
+                            bool ok = true;
+                            double vn = v;
+                            if (A.rescaling != NQCD_RESCALE_OFF) {
+                                const double dd = sc.dh * (U[hit_m] * U[old] / (E[old] - E[hit_m]));
+                                const double aq = dd * dd / A.mass / 2.0, bq = dd * v, cq = E[hit_m] - E[old];
+                                const double disc = bq * bq - 4.0 * aq * cq;
+                                if (disc < 0.0) {
+                                    ok = false;
+                                    if (A.rescaling == NQCD_RESCALE_VINVERSION) vn = v - 2.0 * (v * (dd / fabs(dd))) * (dd / fabs(dd));
+                                } else {
+                                    const double root = sqrt(disc);
+                                    const double gam = (bq < 0.0) ? (bq + root) / (2.0 * aq) : (bq - root) / (2.0 * aq);
+                                    vn = v - gam * dd / A.mass;
+                                }
+                            }
+                            if (ok) {
+                                isocc[old] = -1;
+                                isocc[hit_m] = hit_n;
+                                occ[hit_n] = hit_m;
+                                ++nh;
+                            }
+                            scal[5] = vn;
+                        }
+                    }
+                    __syncthreads();
+                    v = scal[5];
+                    __syncthreads();
+                }
+                // ---- Granucci-Persico EDC per electron (decoherence_corrections.jl:21-39, iesh.jl:424-433)
+                if (A.edc) {
+                    const double Ekin = 0.5 * A.mass * v * v;
+                    if (tid < Ne) {
+                        const int s = occ[tid];
+                        double nrm = 0.0;
+                        for (int a = 0; a < N; ++a) {
+                            if (a == s) continue;
+                            const double tau = (1.0 + A.edc_C / Ekin) / fabs(E[a] - E[s]);
+                            const double f = exp(-dt / tau);
+                            const double re = Tb[a * CW + 2 * tid] * f, im = Tb[a * CW + 2 * tid + 1] * f;
+                            Tb[a * CW + 2 * tid] = re;
+                            Tb[a * CW + 2 * tid + 1] = im;
+                            nrm += re * re + im * im;
+                        }
+                        const double cr = Tb[s * CW + 2 * tid], ci = Tb[s * CW + 2 * tid + 1];
+                        const double fac = sqrt((1.0 - nrm) / (cr * cr + ci * ci));
+                        Tb[s * CW + 2 * tid] = cr * fac;
+                        Tb[s * CW + 2 * tid + 1] = ci * fac;
+                    }
+                    __syncthreads();
+                }
+            }
+            if (A.save_every > 0 && (step + 1) % A.save_every == 0) save((int)((step + 1) / A.save_every));
+        }
+
+        // ---- store --------------------------------------------------------------------------------
+        if (tid == 0) {
+            A.r[traj] = x;
+            A.v[traj] = v;
+            A.nhops[traj] = nh;
+            A.steps[traj] = (int32_t)A.nsteps;
+        }
+        if (tid < Ne) A.occ[(size_t)traj * Ne + tid] = occ[tid];
+        for (int idx = tid; idx < N * 2 * Ne; idx += B) {
+            const int row = idx / (2 * Ne), col = idx - row * 2 * Ne;
+            A.psi[(size_t)traj * N * 2 * Ne + idx] = Tb[row * CW + col];
+        }
+    }
+}
+
+size_t ie_lds_bytes(int NT, int cw, int Ne) {
+    const int NP = 16 * NT;
+    return sizeof(double) * ((size_t)NP * cw + 2 * (size_t)Ne * Ne + 3 * NP + 8 + NP + 8) +
+           sizeof(int) * (2 * kIeMaxNe + NP);
+}
+
+typedef void (*IeKernel)(const IeshArgs);
+
+IeKernel ie_pick(int NT, bool wide) {
+#define NQCD_IE(nt)                                                                                    \
+    case nt: return wide ? (IeKernel)iesh_kernel<nt, 8> : (IeKernel)iesh_kernel<nt, nt>;
+    switch (NT) {
+        NQCD_IE(1) NQCD_IE(2) NQCD_IE(3) NQCD_IE(4) NQCD_IE(5) NQCD_IE(6) NQCD_IE(7) NQCD_IE(8)
+    default: return nullptr;
+    }
+#undef NQCD_IE
+}
+
+// Model parameters in the handle's slot 0: m, omega, g, dG, Gamma, a, b, M.
+struct IeModel { double mw2, g, dG, V, a, de; int M; };
+
+IeModel ie_model(const nqcd_handle_t* h) {
+    const double* p = h->models[0].p;
+    IeModel m;
+    m.M = (int)p[7];
+    m.mw2 = p[0] * p[1] * p[1];
+    m.g = p[2];
+    m.dG = p[3];
+    m.a = p[5];
+    m.de = (p[6] - p[5]) / (m.M - 1);
+    m.V = std::sqrt(p[4] / (2.0 * M_PI)) * std::sqrt(m.de);
+    return m;
+}
+
+}  // namespace
+
+int32_t ie_check_problem(nqcd_handle_t* h) {
+    const nqcd_problem_desc& d = h->prob;
+    if (d.algorithm != NQCD_ALG_VERLET_ELECTRONICS)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA integrate with VerletwithElectronics");
+    if (d.ndofs * d.natoms != 1 || d.nbeads != 1)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA: one degree of freedom, no ring polymer");
+    if (d.n_states < 3 || d.n_states > kIeMaxStates)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA: n_states must be in 3..128");
+    if (d.n_electrons < 1 || d.n_electrons >= d.n_states || d.n_electrons > kIeMaxNe)
+        return fail(h, NQCD_ERR_ARG, "n_electrons must be in 1..min(64, n_states - 1)");
+    if (d.precision != NQCD_PREC_FP64) return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA run in FP64");
+    if (d.decoherence < 0 || d.decoherence > 1) return fail(h, NQCD_ERR_ARG, "unknown decoherence correction");
+    if (d.decoherence == NQCD_DECOHERENCE_EDC && d.method != NQCD_METHOD_IESH)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "the EDC decoherence correction applies to IESH");
+    return NQCD_OK;
+}
+
+int32_t ie_set_model(nqcd_handle_t* h, int32_t slot, const double* p, int64_t n) {
+    if (slot != 0) return fail(h, NQCD_ERR_UNSUPPORTED, "the Newns-Anderson model must use slot 0");
+    if (n != 8) return fail(h, NQCD_ERR_ARG, "Newns-Anderson params: m, omega, g, dG, Gamma, a, b, M");
+    const double M = p[7];
+    if (!(M >= 2 && M == std::floor(M) && M + 1 <= kIeMaxStates))
+        return fail(h, NQCD_ERR_ARG, "Newns-Anderson: M must be an integer in 2..127");
+    if (!(p[4] > 0.0)) return fail(h, NQCD_ERR_ARG, "Newns-Anderson: Gamma must be positive");
+    if (!(p[6] > p[5])) return fail(h, NQCD_ERR_ARG, "Newns-Anderson: the band needs b > a");
+    if (!(p[0] > 0.0)) return fail(h, NQCD_ERR_ARG, "Newns-Anderson: m must be positive");
+    ModelSlot m{};
+    m.kind = NQCD_MODEL_NEWNS_ANDERSON;
+    for (int i = 0; i < 8; ++i) m.p[i] = p[i];
+    h->models[0] = m;
+    h->model_set[0] = true;
+    return NQCD_OK;
+}
+
+static int32_t ie_check_occ(nqcd_handle_t* h, int64_t ntraj, const int32_t* occ) {
+    const int N = h->prob.n_states, Ne = h->prob.n_electrons;
+    std::vector<char> seen(N);
+    for (int64_t i = 0; i < ntraj; ++i) {
+        std::fill(seen.begin(), seen.end(), 0);
+        for (int e = 0; e < Ne; ++e) {
+            const int32_t s = occ[(size_t)i * Ne + e];
+            if (s < 0 || s >= N) return fail(h, NQCD_ERR_ARG, "occupied state out of range");
+            if (seen[s]) return fail(h, NQCD_ERR_ARG, "an adiabatic state is occupied twice");
+            seen[s] = 1;
+        }
+    }
+    return NQCD_OK;
+}
+
+int32_t ie_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const double* v, const double* psi_re,
+                        const double* psi_im, const int32_t* occ) {
+    const int N = h->prob.n_states, Ne = h->prob.n_electrons;
+    if (!occ) return fail(h, NQCD_ERR_ARG, "IESH / EhrenfestNA need the occupied states (n_electrons per trajectory)");
+    if ((psi_re == nullptr) != (psi_im == nullptr)) return fail(h, NQCD_ERR_ARG, "give both psi parts or neither");
+    if (int32_t rc = ie_check_occ(h, ntraj, occ)) return rc;
+    h->ntraj = ntraj;
+    HIPCHK(h, h->r0.ensure(sizeof(double) * ntraj));
+    HIPCHK(h, h->v0.ensure(sizeof(double) * ntraj));
+    HIPCHK(h, h->state0.ensure(sizeof(int32_t) * Ne * ntraj));
+    HIPCHK(h, hipMemcpy(h->r0.p, r, sizeof(double) * ntraj, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->v0.p, v, sizeof(double) * ntraj, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->state0.p, occ, sizeof(int32_t) * Ne * ntraj, hipMemcpyHostToDevice));
+    h->ie_psi_given = psi_re != nullptr;
+    if (psi_re) {  // (N, Ne, ntraj) column-major -> per trajectory rows of (Re, Im) pairs
+        std::vector<double> t((size_t)2 * N * Ne * ntraj);
+        for (int64_t i = 0; i < ntraj; ++i)
+            for (int a = 0; a < N; ++a)
+                for (int e = 0; e < Ne; ++e) {
+                    const size_t src = ((size_t)i * Ne + e) * N + a;
+                    const size_t dst = (((size_t)i * N + a) * Ne + e) * 2;
+                    t[dst] = psi_re[src];
+                    t[dst + 1] = psi_im[src];
+                }
+        HIPCHK(h, h->sig0.ensure(sizeof(double) * t.size()));
+        HIPCHK(h, hipMemcpy(h->sig0.p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    }
+    return NQCD_OK;
+}
+
+int32_t ie_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S, uint64_t seed) {
+    if (S->kind != NQCD_SAMPLER_GAUSSIAN || S->nparams != 4 || !S->params)
+        return fail(h, NQCD_ERR_ARG, "IESH / EhrenfestNA sampler: GAUSSIAN with mu_r, sd_r, mu_v, sd_v");
+    const int Ne = h->prob.n_electrons;
+    const int64_t n = S->ntraj;
+    h->ntraj = n;
+    HIPCHK(h, h->r0.ensure(sizeof(double) * n));
+    HIPCHK(h, h->v0.ensure(sizeof(double) * n));
+    HIPCHK(h, h->small.ensure(sizeof(double) * 4));
+    HIPCHK(h, hipMemcpy(h->small.p, S->params, sizeof(double) * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, launch_sample_gaussian(n, 1, h->small.as<double>(), seed, h->r0.as<double>(), h->v0.as<double>(), h->stream));
+    // lowest n_electrons adiabatic states occupied: the zero-temperature limit of the
+    // Fermi-Dirac occupation sampling (DynamicsUtils.jl:176-208)
+    std::vector<int32_t> occ((size_t)Ne * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int e = 0; e < Ne; ++e) occ[(size_t)i * Ne + e] = e;
+    HIPCHK(h, h->state0.ensure(sizeof(int32_t) * occ.size()));
+    HIPCHK(h, hipMemcpyAsync(h->state0.p, occ.data(), sizeof(int32_t) * occ.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->ie_psi_given = false;
+    return NQCD_OK;
+}
+
+int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc* term, uint64_t key) {
+    (void)t0;
+    if (term && term->kind != NQCD_TERM_NONE)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "termination predicates are not supported for IESH / EhrenfestNA");
+    if (!h->model_set[0] || h->models[0].kind != NQCD_MODEL_NEWNS_ANDERSON)
+        return fail(h, NQCD_ERR_STATE, "IESH / EhrenfestNA need the Newns-Anderson model in slot 0");
+    const IeModel md = ie_model(h);
+    const int N = h->prob.n_states, Ne = h->prob.n_electrons;
+    if (md.M + 1 != N) return fail(h, NQCD_ERR_ARG, "n_states must equal M + 1 for the Newns-Anderson model");
+    const int64_t n = h->ntraj;
+    const int NT = (N + 15) / 16;
+    const int CT = (2 * Ne + 15) / 16;
+    const int cw = 16 * CT;
+    IeKernel kern = ie_pick(NT, CT > NT);
+    if (!kern) return fail(h, NQCD_ERR_UNSUPPORTED, "n_states");
+    const size_t lds = ie_lds_bytes(NT, cw, Ne);
+    HIPCHK(h, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int block = 64 * NT;
+    int bpc = 0;
+    HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)kern, block, lds));
+    if (bpc < 1) return fail(h, NQCD_ERR_UNSUPPORTED, "IESH kernel does not fit on a compute unit");
+    int64_t grid = (int64_t)bpc * h->cu_count;
+    if (grid > n) grid = n;
+
+    HIPCHK(h, h->r.ensure(sizeof(double) * n));
+    HIPCHK(h, h->v.ensure(sizeof(double) * n));
+    HIPCHK(h, h->sig.ensure(sizeof(double) * 2 * N * Ne * n));
+    HIPCHK(h, h->state.ensure(sizeof(int32_t) * Ne * n));
+    HIPCHK(h, h->steps.ensure(sizeof(int32_t) * n));
+    HIPCHK(h, h->nhops.ensure(sizeof(int32_t) * n));
+    HIPCHK(h, h->save_ad.ensure(sizeof(double) * N * h->nsave * n));
+    HIPCHK(h, h->save_di.ensure(sizeof(double) * h->nsave * n));
+    HIPCHK(h, h->ie_scratch.ensure(sizeof(double) * 2 * Ne * Ne * grid));
+
+    IeshArgs A{};
+    A.ntraj = n;
+    A.nsteps = nsteps;
+    A.N = N; A.M = md.M; A.Ne = Ne; A.cw = cw;
+    A.method = (h->prob.method == NQCD_METHOD_IESH) ? 0 : 1;
+    A.rescaling = h->prob.rescaling;
+    A.estimate = h->prob.disable_estimate ? 0 : 1;
+    A.edc = h->prob.decoherence == NQCD_DECOHERENCE_EDC;
+    A.save_every = h->save_every;
+    A.nsave = h->nsave;
+    A.dt = h->prob.dt;
+    A.mass = h->masses[0];
+    A.edc_C = h->prob.edc_C > 0.0 ? h->prob.edc_C : 0.1;
+    A.key = key;
+    A.mw2 = md.mw2; A.g = md.g; A.dG = md.dG; A.V = md.V; A.a = md.a; A.de = md.de;
+    A.r0 = h->r0.as<double>(); A.v0 = h->v0.as<double>(); A.occ0 = h->state0.as<int32_t>();
+    A.psi0 = h->ie_psi_given ? h->sig0.as<double>() : nullptr;
+    A.r = h->r.as<double>(); A.v = h->v.as<double>(); A.psi = h->sig.as<double>(); A.occ = h->state.as<int32_t>();
+    A.nhops = h->nhops.as<int32_t>(); A.steps = h->steps.as<int32_t>();
+    A.save_imp = h->save_di.as<double>(); A.save_ad = h->save_ad.as<double>();
+    A.scratch = h->ie_scratch.as<double>();
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, h->stream, A);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    h->stats.block = block;
+    h->stats.grid = (int)grid;
+    h->stats.launches = 1;
+    return NQCD_OK;
+}
+
+int32_t ie_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes) {
+    const int64_t nt = h->ntraj;
+    const int N = h->prob.n_states, Ne = h->prob.n_electrons, ns = h->nsave;
+    switch (field) {
+    case NQCD_FIELD_R:
+    case NQCD_FIELD_V:
+        if (bytes != (int64_t)sizeof(double) * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, hipMemcpy(out, field == NQCD_FIELD_R ? h->r.p : h->v.p, bytes, hipMemcpyDeviceToHost));
+        return NQCD_OK;
+    case NQCD_FIELD_SIGMA_RE:
+    case NQCD_FIELD_SIGMA_IM: {  // (N, Ne, ntraj) column-major
+        if (bytes != (int64_t)sizeof(double) * N * Ne * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        std::vector<double> t((size_t)2 * N * Ne * nt);
+        HIPCHK(h, hipMemcpy(t.data(), h->sig.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+        double* o = static_cast<double*>(out);
+        const int c = field == NQCD_FIELD_SIGMA_RE ? 0 : 1;
+        for (int64_t i = 0; i < nt; ++i)
+            for (int a = 0; a < N; ++a)
+                for (int e = 0; e < Ne; ++e) o[((size_t)i * Ne + e) * N + a] = t[(((size_t)i * N + a) * Ne + e) * 2 + c];
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_STATE:
+        if (bytes != (int64_t)sizeof(int32_t) * Ne * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, hipMemcpy(out, h->state.p, bytes, hipMemcpyDeviceToHost));
+        return NQCD_OK;
+    case NQCD_FIELD_STEPS:
+    case NQCD_FIELD_NHOPS:
+        if (bytes != (int64_t)sizeof(int32_t) * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, hipMemcpy(out, field == NQCD_FIELD_STEPS ? h->steps.p : h->nhops.p, bytes, hipMemcpyDeviceToHost));
+        return NQCD_OK;
+    case NQCD_FIELD_SAVE_POP_AD: {  // (N, nsave, ntraj)
+        if (bytes != (int64_t)sizeof(double) * N * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        std::vector<double> t((size_t)N * ns * nt);
+        HIPCHK(h, hipMemcpy(t.data(), h->save_ad.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl)
+                for (int a = 0; a < N; ++a) o[((size_t)i * ns + sl) * N + a] = t[((size_t)sl * N + a) * nt + i];
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_SAVE_IMPURITY: {  // (nsave, ntraj)
+        if (bytes != (int64_t)sizeof(double) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        std::vector<double> t((size_t)ns * nt);
+        HIPCHK(h, hipMemcpy(t.data(), h->save_di.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = t[(size_t)sl * nt + i];
+        return NQCD_OK;
+    }
+    default: return fail(h, NQCD_ERR_UNSUPPORTED, "field not available for IESH / EhrenfestNA");
+    }
+}
+
+}  // namespace nqcd

@@ -32,7 +32,8 @@ EnsembleMI355X(; device=0, persistent=true, seed=0x4E5143444D493335) = EnsembleM
 struct ProblemDesc
     method::Int32; algorithm::Int32; n_states::Int32; ndofs::Int32; natoms::Int32; nbeads::Int32
     n_electrons::Int32; rescaling::Int32; precision::Int32; schedule::Int32; electronic_substeps::Int32
-    reserved0::Int32; masses::Ptr{Float64}; dt::Float64; temperature::Float64; reserved::NTuple{8,Int32}
+    decoherence::Int32; masses::Ptr{Float64}; dt::Float64; temperature::Float64; edc_C::Float64
+    disable_estimate::Int32; reserved::NTuple{7,Int32}
 end
 struct TermDesc
     kind::Int32; dof::Int32; value::Float64
@@ -63,7 +64,8 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
     try
         masses = collect(Float64, sim.atoms.masses)
         d = ProblemDesc(method_code(sim.method), 0, NQCModels.nstates(sim), size(sim)[1], size(sim)[2], 1, 0,
-            rescaling_code(sim.method), 0, ensemblealg.persistent ? 0 : 1, 5, 0, pointer(masses), dt, 0.0, ntuple(_ -> 0, 8))
+            rescaling_code(sim.method), 0, ensemblealg.persistent ? 0 : 1, 5, 0, pointer(masses), dt, 0.0, 0.1, 0,
+            ntuple(_ -> 0, 7))
         GC.@preserve masses check(h[], ccall((:nqcd_mi355x_set_problem, libnqcd), Int32, (Ptr{Cvoid}, Ref{ProblemDesc}), h[], d))
         kind, params = model_code(sim.cache.model)
         check(h[], ccall((:nqcd_mi355x_set_model, libnqcd), Int32, (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int64),

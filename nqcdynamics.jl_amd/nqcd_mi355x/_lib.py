@@ -22,14 +22,15 @@ ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5 = range(4)
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = range(3)
 PREC_FP64, PREC_FP32_NUCLEI = range(2)
 SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
-MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC = range(6)
+MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC, MODEL_NEWNS_ANDERSON = range(7)
+DECOHERENCE_NONE, DECOHERENCE_EDC = range(2)
 TERM_NONE, TERM_ABS_GT = range(2)
 SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLYMER_THERMAL = range(4)
 (OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION, OUT_FINAL_ADIABATIC_POPULATION,
- OUT_CENTROID_POSITION) = range(6)
+ OUT_CENTROID_POSITION, OUT_IMPURITY_POPULATION) = range(7)
 RED_MEAN, RED_SUM = range(2)
 (FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
- FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID) = range(11)
+ FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID, FIELD_SAVE_IMPURITY) = range(12)
 
 EXPORTED_SYMBOLS = (
     "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
@@ -42,8 +43,9 @@ class ProblemDesc(C.Structure):
     _fields_ = [("method", C.c_int32), ("algorithm", C.c_int32), ("n_states", C.c_int32), ("ndofs", C.c_int32),
                 ("natoms", C.c_int32), ("nbeads", C.c_int32), ("n_electrons", C.c_int32), ("rescaling", C.c_int32),
                 ("precision", C.c_int32), ("schedule", C.c_int32), ("electronic_substeps", C.c_int32),
-                ("reserved0", C.c_int32), ("masses", C.POINTER(C.c_double)), ("dt", C.c_double),
-                ("temperature", C.c_double), ("reserved", C.c_int32 * 8)]
+                ("decoherence", C.c_int32), ("masses", C.POINTER(C.c_double)), ("dt", C.c_double),
+                ("temperature", C.c_double), ("edc_C", C.c_double), ("disable_estimate", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
 
 
 class TermDesc(C.Structure):
@@ -152,13 +154,16 @@ class Ensemble:
 
     def set_problem(self, method, n_states, ndofs, natoms, masses, dt, algorithm=ALG_BAB_TSIT5, nbeads=1,
                     n_electrons=0, rescaling=RESCALE_STANDARD, precision=PREC_FP64, schedule=SCHED_PERSISTENT,
-                    electronic_substeps=5, temperature=0.0):
+                    electronic_substeps=5, temperature=0.0, decoherence=DECOHERENCE_NONE, edc_C=0.1,
+                    estimate_probability=True):
         self._masses = np.ascontiguousarray(np.broadcast_to(np.asarray(masses, dtype=np.float64), (natoms,)))
         d = ProblemDesc()
         d.method, d.algorithm, d.n_states, d.ndofs, d.natoms, d.nbeads = method, algorithm, n_states, ndofs, natoms, nbeads
         d.n_electrons, d.rescaling, d.precision, d.schedule = n_electrons, rescaling, precision, schedule
         d.electronic_substeps, d.masses, d.dt, d.temperature = electronic_substeps, _dp(self._masses), dt, temperature
+        d.decoherence, d.edc_C, d.disable_estimate = decoherence, edc_C, 0 if estimate_probability else 1
         self._check(self._L.nqcd_mi355x_set_problem(self._h, C.byref(d)))
+        self.method, self.n_electrons = method, n_electrons
         self.n_states, self.D, self.dt = n_states, ndofs * natoms, dt
         self.nbeads = nbeads
         self.rows = ndofs * natoms * (nbeads if method in (METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST) else 1)
@@ -178,6 +183,22 @@ class Ensemble:
         st = None if state is None else np.ascontiguousarray(state, dtype=np.int32)
         mid = None if model_id is None else np.ascontiguousarray(model_id, dtype=np.int32)
         self._check(self._L.nqcd_mi355x_load_initial(self._h, ntraj, _dp(r), _dp(v), _dp(sr), _dp(si), _ip(st), _ip(mid)))
+        self.ntraj = ntraj
+
+    def load_initial_iesh(self, r, v, occ, psi=None):
+        """IESH / EhrenfestNA: occ (ntraj, n_electrons) occupied adiabatic states (0-based), psi
+        (ntraj, n_states, n_electrons) complex or None for psi[occ_e, e] = 1."""
+        r = np.ascontiguousarray(np.ravel(r), dtype=np.float64)
+        v = np.ascontiguousarray(np.ravel(v), dtype=np.float64)
+        ntraj = r.shape[0]
+        occ = np.ascontiguousarray(occ, dtype=np.int32).reshape(ntraj, -1)
+        pr = pi = None
+        if psi is not None:
+            psi = np.asarray(psi)
+            # Julia column-major (n, ne, ntraj) == C row-major (ntraj, ne, n)
+            pr = np.ascontiguousarray(np.real(psi).transpose(0, 2, 1), dtype=np.float64)
+            pi = np.ascontiguousarray(np.imag(psi).transpose(0, 2, 1), dtype=np.float64)
+        self._check(self._L.nqcd_mi355x_load_initial(self._h, ntraj, _dp(r), _dp(v), _dp(pr), _dp(pi), _ip(occ), None))
         self.ntraj = ntraj
 
     def sample_initial(self, kind, ntraj, params, seed, initial_state=0, state_basis=0, n_models=1):
@@ -210,6 +231,8 @@ class Ensemble:
         return out
 
     def download(self, field):
+        if getattr(self, "method", None) in (METHOD_IESH, METHOD_EHRENFEST_NA):
+            return self._download_iesh(field)
         n, D, nt, nsave = max(self.n_states, 2), self.rows, self.ntraj, self.nsave
         shapes = {FIELD_R: ((nt, D), np.float64), FIELD_V: ((nt, D), np.float64),
                   FIELD_SAVE_CENTROID: ((nt, nsave), np.float64),
@@ -222,4 +245,17 @@ class Ensemble:
         self._check(self._L.nqcd_mi355x_download(self._h, field, out.ctypes.data_as(C.c_void_p), out.nbytes))
         if field in (FIELD_SIGMA_RE, FIELD_SIGMA_IM):
             out = out.transpose(0, 2, 1).copy()  # column-major (n, n) per trajectory -> row-major
+        return out
+
+    def _download_iesh(self, field):
+        n, ne, nt, nsave = self.n_states, self.n_electrons, self.ntraj, self.nsave
+        shapes = {FIELD_R: ((nt,), np.float64), FIELD_V: ((nt,), np.float64),
+                  FIELD_SIGMA_RE: ((nt, ne, n), np.float64), FIELD_SIGMA_IM: ((nt, ne, n), np.float64),
+                  FIELD_STATE: ((nt, ne), np.int32), FIELD_STEPS: ((nt,), np.int32), FIELD_NHOPS: ((nt,), np.int32),
+                  FIELD_SAVE_POP_AD: ((nt, nsave, n), np.float64), FIELD_SAVE_IMPURITY: ((nt, nsave), np.float64)}
+        shape, dtype = shapes[field]
+        out = np.zeros(shape, dtype=dtype)
+        self._check(self._L.nqcd_mi355x_download(self._h, field, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        if field in (FIELD_SIGMA_RE, FIELD_SIGMA_IM):
+            out = out.transpose(0, 2, 1).copy()  # (ntraj, n, ne)
         return out

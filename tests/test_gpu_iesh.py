@@ -1,0 +1,120 @@
+"""GPU IESH / EhrenfestNA (Newns-Anderson, VerletwithElectronics) against the FP64 oracle:
+identical occupations and hop counts, positions, velocities, wavefunctions and impurity
+populations within 1e-9 relative."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MASS = 2000.0
+MODEL = dict(m=2000.0, omega=2e-4, g=20.6097, dG=-3.8e-3, Gamma=6.4e-3)
+
+
+def _params(M):
+    W = 6 * MODEL["Gamma"] / 2
+    return [MODEL["m"], MODEL["omega"], MODEL["g"], MODEL["dG"], MODEL["Gamma"], -W, W, M]
+
+
+def _inputs(ntraj, Ne, seed, x0=8.0):
+    rng = np.random.default_rng(seed)
+    r0 = rng.normal(x0, 1.0, size=ntraj)
+    v0 = np.abs(rng.normal(0.0, 1e-2, size=ntraj))
+    occ0 = np.tile(np.arange(Ne), (ntraj, 1)).astype(np.int32)
+    occ0[:, -1] = Ne  # one electron above the Fermi level
+    return r0, v0, occ0
+
+
+def _gpu(L, method, M, Ne, r0, v0, occ0, dt, nsteps, key, save_every=10, **kw):
+    E = L.Ensemble(0)
+    E.set_problem(method, M + 1, 1, 1, [MASS], dt, algorithm=L.ALG_VERLET_ELECTRONICS, n_electrons=Ne, **kw)
+    E.set_model(0, L.MODEL_NEWNS_ANDERSON, _params(M))
+    E.load_initial_iesh(r0, v0, occ0)
+    stats = E.run(0.0, nsteps * dt, key, save_every=save_every)
+    out = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
+           "psi": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
+           "occ": E.download(L.FIELD_STATE), "nhops": E.download(L.FIELD_NHOPS),
+           "imp": E.download(L.FIELD_SAVE_IMPURITY), "ad": E.download(L.FIELD_SAVE_POP_AD),
+           "imp_mean": E.reduce(L.OUT_IMPURITY_POPULATION, L.RED_MEAN, E.nsave), "stats": stats}
+    E.close()
+    return out
+
+
+def _cpu(oracle, method, M, Ne, r0, v0, occ0, dt, nsteps, key, save_every=10, **kw):
+    W = 6 * MODEL["Gamma"] / 2
+    md = oracle.na_model(M=M, W=W, m=MODEL["m"], omega=MODEL["omega"], g=MODEL["g"], dG=MODEL["dG"], Gamma=MODEL["Gamma"])
+    return oracle.iesh_run_ensemble(method, md, Ne, MASS, r0, v0, occ0, dt=dt, nsteps=nsteps, key=key,
+                                    save_every=save_every, **kw)
+
+
+def _compare(g, c, hops=True):
+    if hops:
+        assert np.array_equal(g["occ"], c["occ"])
+        assert np.array_equal(g["nhops"], c["nhops"])
+    assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-11)
+    assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-13)
+    assert np.allclose(g["psi"], c["psi"], rtol=1e-9, atol=1e-10)
+    assert np.allclose(g["imp"], c["pop_imp"], rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("M,Ne,ntraj,nsteps", [(40, 20, 64, 300), (100, 50, 12, 120)])
+def test_iesh_fp64_parity(oracle, key, M, Ne, ntraj, nsteps):
+    from nqcd_mi355x import _lib as L
+    r0, v0, occ0 = _inputs(ntraj, Ne, 11)
+    g = _gpu(L, L.METHOD_IESH, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
+    c = _cpu(oracle, 0, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
+    if M == 40:
+        assert c["nhops"].sum() >= 3
+    _compare(g, c)
+    # one-hot adiabatic populations at the last save slot follow the final occupations
+    last = np.zeros((ntraj, M + 1))
+    np.put_along_axis(last, c["occ"], 1.0, axis=1)
+    assert np.array_equal(g["ad"][:, -1, :], last)
+    assert np.allclose(g["imp_mean"], c["pop_imp"].mean(axis=0), rtol=1e-9, atol=1e-12)
+
+
+def test_iesh_estimate_off_and_rescaling_modes(oracle, key):
+    from nqcd_mi355x import _lib as L
+    M, Ne, ntraj, nsteps = 40, 20, 32, 200
+    r0, v0, occ0 = _inputs(ntraj, Ne, 12)
+    for resc, ores in ((L.RESCALE_VINVERSION, 1), (L.RESCALE_OFF, 2)):
+        g = _gpu(L, L.METHOD_IESH, M, Ne, r0, v0, occ0, 2.0, nsteps, key, rescaling=resc, estimate_probability=False)
+        c = _cpu(oracle, 0, M, Ne, r0, v0, occ0, 2.0, nsteps, key, rescaling=ores, estimate=False)
+        _compare(g, c)
+
+
+def test_iesh_edc_parity(oracle, key):
+    from nqcd_mi355x import _lib as L
+    M, Ne, ntraj, nsteps = 40, 20, 32, 200
+    r0, v0, occ0 = _inputs(ntraj, Ne, 13)
+    g = _gpu(L, L.METHOD_IESH, M, Ne, r0, v0, occ0, 2.0, nsteps, key, decoherence=L.DECOHERENCE_EDC, edc_C=0.1)
+    c = _cpu(oracle, 0, M, Ne, r0, v0, occ0, 2.0, nsteps, key, edc=True, edc_C=0.1)
+    _compare(g, c)
+    assert np.allclose(np.sum(np.abs(g["psi"]) ** 2, axis=1), 1.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("M,Ne", [(40, 20), (100, 50)])
+def test_ehrenfest_na_fp64_parity(oracle, key, M, Ne):
+    from nqcd_mi355x import _lib as L
+    ntraj, nsteps = 16, 150
+    r0, v0, occ0 = _inputs(ntraj, Ne, 14)
+    g = _gpu(L, L.METHOD_EHRENFEST_NA, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
+    c = _cpu(oracle, 1, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
+    _compare(g, c, hops=False)
+    # adiabatic population sum_e |psi_ae|^2 sums to the electron count
+    assert np.allclose(g["ad"].sum(axis=2), Ne, atol=1e-10)
+
+
+def test_iesh_abi_errors():
+    from nqcd_mi355x import _lib as L
+    E = L.Ensemble(0)
+    with pytest.raises(L.NQCDError):  # wrong algorithm
+        E.set_problem(L.METHOD_IESH, 41, 1, 1, [MASS], 1.0, algorithm=L.ALG_BAB_TSIT5, n_electrons=20)
+    with pytest.raises(L.NQCDError):  # too many electrons
+        E.set_problem(L.METHOD_IESH, 41, 1, 1, [MASS], 1.0, algorithm=L.ALG_VERLET_ELECTRONICS, n_electrons=41)
+    E.set_problem(L.METHOD_IESH, 41, 1, 1, [MASS], 1.0, algorithm=L.ALG_VERLET_ELECTRONICS, n_electrons=20)
+    E.set_model(0, L.MODEL_NEWNS_ANDERSON, _params(40))
+    occ = np.tile(np.arange(20), (2, 1))
+    occ[1, 3] = 2  # state 2 occupied twice
+    with pytest.raises(L.NQCDError):
+        E.load_initial_iesh([8.0, 8.0], [0.0, 0.0], occ)
+    E.close()

@@ -97,10 +97,22 @@ def test_rpmd_harmonic_matches_analytic_normal_modes(oracle, key):
     E.close()
     U, w, _ = oracle.rp_normal_modes(P, P * T, dt)
     Om = np.sqrt(w ** 2 + w0 ** 2)
+    nsteps = int(round(tend / dt))
+    # the exact discrete map of one BCB step per normal mode: half kick by the bead potential,
+    # Cayley step cay(dt A_k) = (I - dt A_k/2)^-1 (I + dt A_k/2) of the free ring polymer
+    # (ring_polymer.jl:71-82), half kick
+    K = np.array([[1.0, 0.0], [-w0 ** 2 * dt / 2, 1.0]])
+    maps = []
+    for wk in w:
+        Ak = np.array([[0.0, 1.0], [-wk ** 2, 0.0]])
+        Ck = np.linalg.solve(np.eye(2) - dt * Ak / 2, np.eye(2) + dt * Ak / 2)
+        maps.append(np.linalg.matrix_power(K @ Ck @ K, nsteps))
     for i in range(8):
         rn0, vn0 = U.T @ r0[i], U.T @ v0[i]
-        exact = U @ (rn0 * np.cos(Om * tend) + vn0 / Om * np.sin(Om * tend))
-        assert np.abs(g[i] - exact).max() < 2e-4
+        discrete = U @ np.array([Mk[0, 0] * a + Mk[0, 1] * b for Mk, a, b in zip(maps, rn0, vn0)])
+        assert np.abs(g[i] - discrete).max() < 1e-10
+        continuum = U @ (rn0 * np.cos(Om * tend) + vn0 / Om * np.sin(Om * tend))
+        assert np.abs(g[i] - continuum).max() < 1e-3   # second-order integrator error at dt = 0.002
 
 
 def test_rpmd_fp32_centroid_statistics(oracle, key):
