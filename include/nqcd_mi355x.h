@@ -162,11 +162,10 @@ enum nqcd_output_kind {
     NQCD_OUT_SCATTERING_1D = 2,          /* final, [reflection(n), transmission(n)] (DynamicsOutputs.jl:313-338) */
     NQCD_OUT_POPULATION_CORRELATION = 3, /* per save slot, n x n P(0)P(t)^T diabatic (TimeCorrelationFunctions.jl:61-88) */
     NQCD_OUT_FINAL_ADIABATIC_POPULATION = 4, /* final, nstates */
-    NQCD_OUT_CENTROID_POSITION = 5,      /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
-    NQCD_OUT_IMPURITY_POPULATION = 6     /* per save slot, IESH / EhrenfestNA impurity population: the impurity
-                                            entry of the IESH diabatic population (iesh.jl:337-369); for
-                                            EhrenfestNA sum_e |(U psi)_{impurity, e}|^2 */
+    NQCD_OUT_CENTROID_POSITION = 5       /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
 };
+/* IESH: DIABATIC_POPULATION is iesh_diabatic_population (iesh.jl:337-369), state 0 = impurity;
+ * EhrenfestNA (which the reference gives no diabatic estimator) reports sum_e |(U psi)_{a e}|^2. */
 /* Reductions (src/Ensembles/reductions.jl:8-52). */
 enum nqcd_reduction_kind { NQCD_RED_MEAN = 0, NQCD_RED_SUM = 1 };
 
@@ -182,11 +181,10 @@ enum nqcd_field {
     NQCD_FIELD_SAVE_POP_AD = 7,  /* double (n, nsave, ntraj) adiabatic population per save     */
     NQCD_FIELD_SAVE_POP_DI = 8,  /* double (n, nsave, ntraj) diabatic population per save      */
     NQCD_FIELD_SAVE_STATE = 9,   /* int32 (nsave, ntraj) active state per save                 */
-    NQCD_FIELD_SAVE_CENTROID = 10, /* double (nsave, ntraj) ring-polymer centroid per save      */
-    NQCD_FIELD_SAVE_IMPURITY = 11  /* double (nsave, ntraj) IESH / EhrenfestNA impurity population */
+    NQCD_FIELD_SAVE_CENTROID = 10 /* double (nsave, ntraj) ring-polymer centroid per save       */
 };
 /* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
- * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD (n, nsave, ntraj). */
+ * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD / SAVE_POP_DI (n, nsave, ntraj). */
 
 typedef struct {
     double kernel_ms;            /* device time of the trajectory kernels (HIP events) */

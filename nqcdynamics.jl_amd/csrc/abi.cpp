@@ -506,10 +506,8 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* 
     HIPCHK(h, hipSetDevice(h->device));
     if (is_ie(h)) {
         if (kind == NQCD_OUT_ADIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_ad.as<double>(); }
-        else if (kind == NQCD_OUT_IMPURITY_POPULATION) { rows = h->nsave; src = h->save_di.as<double>(); }
-        else return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA outputs: ADIABATIC_POPULATION, IMPURITY_POPULATION");
-    } else if (kind == NQCD_OUT_IMPURITY_POPULATION) {
-        return fail(h, NQCD_ERR_UNSUPPORTED, "impurity population needs IESH / EhrenfestNA");
+        else if (kind == NQCD_OUT_DIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_di.as<double>(); }
+        else return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA outputs: ADIABATIC_POPULATION, DIABATIC_POPULATION");
     }
     if (!src) switch (kind) {
     case NQCD_OUT_ADIABATIC_POPULATION: rows = n * h->nsave; src = h->save_ad.as<double>(); break;

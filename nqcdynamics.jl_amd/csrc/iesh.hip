@@ -16,7 +16,7 @@
 //   * dV/dx only touches the impurity entry, so U^T dV U = dh u u^T and the nonadiabatic coupling
 //     is d_ij = dh u_i u_j / (E_j - E_i).
 //   * propagator: psi <- exp(-i dt (diag(E) - i v d)) psi, applied as a Taylor series whose length
-//     follows from a Frobenius bound of dt H (remainder below 1e-17); each term is one real GEMM
+//     follows from the bound ||dt H|| <= dt max|E| + ||dt v d||_F (remainder below 1e-17); each term is one real GEMM
 //     (-dt v dh W) [Re psi | Im psi] on FP64 MFMA 16x16x4 plus the diagonal part.  The reference
 //     diagonalises H with zheevr and forms U exp(-i Lambda dt) U^H; both give exp(-i H dt) to
 //     rounding.
@@ -46,7 +46,7 @@ constexpr int kIeMaxStates = 128;
 struct IeshArgs {
     int64_t ntraj;
     int64_t nsteps;
-    int32_t N, M, Ne, cw;         // states, bath levels, electrons, padded psi columns (16 * column tiles)
+    int32_t N, M, Ne, cw;         // states, bath levels, electrons, psi columns in use (16 * column tiles)
     int32_t method;               // 0 = IESH, 1 = EhrenfestNA
     int32_t rescaling, estimate, edc;
     int32_t save_every, nsave;
@@ -55,7 +55,7 @@ struct IeshArgs {
     double mw2, g, dG, V, a, de;  // model
     const double* r0; const double* v0; const int32_t* occ0; const double* psi0;  // psi0 may be null
     double* r; double* v; double* psi; int32_t* occ; int32_t* nhops; int32_t* steps;
-    double* save_imp;             // (nsave, ntraj)
+    double* save_di;              // (nsave, N, ntraj)
     double* save_ad;              // (nsave, N, ntraj)
     double* scratch;              // per workgroup 2 Ne^2 doubles (inverse columns on hop steps)
 };
@@ -76,6 +76,19 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     double s = 0.0;
 #pragma unroll
     for (int j = 0; j < NT; ++j) s += red[j];
+    return s;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = red[0];
+#pragma unroll
+    for (int j = 1; j < NT; ++j) s = fmax(s, red[j]);
     return s;
 }
 
@@ -135,7 +148,8 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
     constexpr int B = 64 * NT;
     constexpr int KF = 4 * NT;   // A-fragments per lane (K = NP in steps of 4)
     extern __shared__ double smem[];
-    const int N = A.N, M = A.M, Ne = A.Ne, CW = A.cw, CT = A.cw / 16;
+    constexpr int CW = 16 * CTM; // row stride of psi in LDS (compile time, so LDS offsets are immediates)
+    const int N = A.N, M = A.M, Ne = A.Ne, CT = A.cw / 16;
     double* Tb = smem;                       // NP x CW: psi (Re, Im interleaved per electron) / Taylor term
     double* Sb = Tb + NP * CW;               // Ne x Ne complex, LU in place
     double* E = Sb + 2 * Ne * Ne;            // NP adiabatic energies
@@ -226,46 +240,63 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
             __syncthreads();
             return -f / A.mass;
         };
+        // Diabatic populations: component a of eigenvector b is U_ab = u_b (a = 0, impurity) or
+        // V u_b / (E_b - eps_{a-1}) (bath level a - 1).  IESH (iesh.jl:351-369): per electron
+        // diag(U rho U^T), rho = offdiag(c c^T) + e_s e_s^T with c = Re psi_e, i.e.
+        // (sum_b U_ab c_b)^2 - sum_b U_ab^2 c_b^2 + U_as^2; EhrenfestNA: sum_e |(U psi)_{a e}|^2.
         auto save = [&](int slot) {
             if (slot >= A.nsave) return;
-            if (tid < Ne) {
-                const int s = occ[tid];
-                double pe;
-                if (A.method == 0) {  // iesh_diabatic_population (iesh.jl:351-369), impurity row
-                    double s1 = 0.0, s2 = 0.0;
-                    for (int a = 0; a < N; ++a) {
-                        const double c = Tb[a * CW + 2 * tid];
-                        s1 += U[a] * c;
-                        s2 += U[a] * U[a] * c * c;
-                    }
-                    pe = s1 * s1 - s2 + U[s] * U[s];
-                } else {              // mean field: |(U psi)_{0e}|^2
-                    double re = 0.0, im = 0.0;
-                    for (int a = 0; a < N; ++a) {
-                        re += U[a] * Tb[a * CW + 2 * tid];
-                        im += U[a] * Tb[a * CW + 2 * tid + 1];
-                    }
-                    pe = re * re + im * im;
+            if (A.method == 0) {
+                for (int b = tid; b < N; b += B) {
+                    double wb = 0.0;
+                    for (int e = 0; e < Ne; ++e) { const double c = Tb[b * CW + 2 * e]; wb += c * c; }
+                    pcol[b] = wb;
                 }
-                pcol[tid] = pe;
+                __syncthreads();
             }
             for (int a = tid; a < N; a += B) {
-                double pa;
-                if (A.method == 0) pa = isocc[a] >= 0 ? 1.0 : 0.0;  // one-hot (iesh.jl:371-375)
-                else {                                               // ehrenfest_na.jl:119-126
-                    pa = 0.0;
+                const double ea = (a == 0) ? 0.0 : eps[a - 1];
+                double pa = 0.0;
+                for (int e0 = 0; e0 < Ne; e0 += 8) {
+                    double sr[8], si[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) { sr[q] = 0.0; si[q] = 0.0; }
+                    for (int b = 0; b < N; ++b) {
+                        const double u = (a == 0) ? U[b] : A.V * U[b] / (E[b] - ea);
+                        const double* row = Tb + b * CW + 2 * e0;
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            if (e0 + q < Ne) {
+                                sr[q] += u * row[2 * q];
+                                if (A.method != 0) si[q] += u * row[2 * q + 1];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) pa += sr[q] * sr[q] + si[q] * si[q];
+                }
+                if (A.method == 0) {
+                    for (int b = 0; b < N; ++b) {
+                        const double u = (a == 0) ? U[b] : A.V * U[b] / (E[b] - ea);
+                        pa -= u * u * pcol[b];
+                    }
                     for (int e = 0; e < Ne; ++e) {
-                        const double re = Tb[a * CW + 2 * e], im = Tb[a * CW + 2 * e + 1];
-                        pa += re * re + im * im;
+                        const int sb = occ[e];
+                        const double u = (a == 0) ? U[sb] : A.V * U[sb] / (E[sb] - ea);
+                        pa += u * u;
                     }
                 }
-                A.save_ad[((size_t)slot * N + a) * A.ntraj + traj] = pa;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                double p = 0.0;
-                for (int e = 0; e < Ne; ++e) p += pcol[e];
-                A.save_imp[(size_t)slot * A.ntraj + traj] = p;
+                A.save_di[((size_t)slot * N + a) * A.ntraj + traj] = pa;
+                double pad;
+                if (A.method == 0) pad = isocc[a] >= 0 ? 1.0 : 0.0;  // one-hot (iesh.jl:371-375)
+                else {                                                // ehrenfest_na.jl:119-126
+                    pad = 0.0;
+                    for (int e = 0; e < Ne; ++e) {
+                        const double re = Tb[a * CW + 2 * e], im = Tb[a * CW + 2 * e + 1];
+                        pad += re * re + im * im;
+                    }
+                }
+                A.save_ad[((size_t)slot * N + a) * A.ntraj + traj] = pad;
             }
             __syncthreads();
         };
@@ -296,8 +327,9 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
                     Af[kk] = val;
                     loc += val * val;
                 }
-                if (tid < N) loc += (dt * E[tid]) * (dt * E[tid]);
-                const double theta = sqrt(block_sum<NT>(loc, red));
+                // ||dt H||_2 <= dt max|E_i| + ||dt v d||_F
+                const double fro = sqrt(block_sum<NT>(loc, red));
+                const double theta = dt * block_max<NT>(tid < N ? fabs(E[tid]) : 0.0, red) + fro;
                 // Sub-steps of norm <= 0.5, at most 1024 of them; a non-finite bound leaves psi as is.
                 const bool finite = theta <= 512.0;
                 const int ns = (finite && theta > 0.5) ? (int)ceil(theta / 0.5) : 1;
@@ -719,10 +751,10 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     const int64_t n = h->ntraj;
     const int NT = (N + 15) / 16;
     const int CT = (2 * Ne + 15) / 16;
-    const int cw = 16 * CT;
-    IeKernel kern = ie_pick(NT, CT > NT);
+    const bool wide = CT > NT;
+    IeKernel kern = ie_pick(NT, wide);
     if (!kern) return fail(h, NQCD_ERR_UNSUPPORTED, "n_states");
-    const size_t lds = ie_lds_bytes(NT, cw, Ne);
+    const size_t lds = ie_lds_bytes(NT, 16 * (wide ? 8 : NT), Ne);
     HIPCHK(h, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int block = 64 * NT;
     int bpc = 0;
@@ -738,13 +770,13 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     HIPCHK(h, h->steps.ensure(sizeof(int32_t) * n));
     HIPCHK(h, h->nhops.ensure(sizeof(int32_t) * n));
     HIPCHK(h, h->save_ad.ensure(sizeof(double) * N * h->nsave * n));
-    HIPCHK(h, h->save_di.ensure(sizeof(double) * h->nsave * n));
+    HIPCHK(h, h->save_di.ensure(sizeof(double) * N * h->nsave * n));
     HIPCHK(h, h->ie_scratch.ensure(sizeof(double) * 2 * Ne * Ne * grid));
 
     IeshArgs A{};
     A.ntraj = n;
     A.nsteps = nsteps;
-    A.N = N; A.M = md.M; A.Ne = Ne; A.cw = cw;
+    A.N = N; A.M = md.M; A.Ne = Ne; A.cw = 16 * CT;
     A.method = (h->prob.method == NQCD_METHOD_IESH) ? 0 : 1;
     A.rescaling = h->prob.rescaling;
     A.estimate = h->prob.disable_estimate ? 0 : 1;
@@ -760,7 +792,7 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     A.psi0 = h->ie_psi_given ? h->sig0.as<double>() : nullptr;
     A.r = h->r.as<double>(); A.v = h->v.as<double>(); A.psi = h->sig.as<double>(); A.occ = h->state.as<int32_t>();
     A.nhops = h->nhops.as<int32_t>(); A.steps = h->steps.as<int32_t>();
-    A.save_imp = h->save_di.as<double>(); A.save_ad = h->save_ad.as<double>();
+    A.save_di = h->save_di.as<double>(); A.save_ad = h->save_ad.as<double>();
     A.scratch = h->ie_scratch.as<double>();
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, h->stream, A);
@@ -802,23 +834,16 @@ int32_t ie_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes) {
         if (bytes != (int64_t)sizeof(int32_t) * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
         HIPCHK(h, hipMemcpy(out, field == NQCD_FIELD_STEPS ? h->steps.p : h->nhops.p, bytes, hipMemcpyDeviceToHost));
         return NQCD_OK;
-    case NQCD_FIELD_SAVE_POP_AD: {  // (N, nsave, ntraj)
+    case NQCD_FIELD_SAVE_POP_AD:
+    case NQCD_FIELD_SAVE_POP_DI: {  // (N, nsave, ntraj)
         if (bytes != (int64_t)sizeof(double) * N * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
         std::vector<double> t((size_t)N * ns * nt);
-        HIPCHK(h, hipMemcpy(t.data(), h->save_ad.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(t.data(), field == NQCD_FIELD_SAVE_POP_AD ? h->save_ad.p : h->save_di.p,
+                            sizeof(double) * t.size(), hipMemcpyDeviceToHost));
         double* o = static_cast<double*>(out);
         for (int64_t i = 0; i < nt; ++i)
             for (int sl = 0; sl < ns; ++sl)
                 for (int a = 0; a < N; ++a) o[((size_t)i * ns + sl) * N + a] = t[((size_t)sl * N + a) * nt + i];
-        return NQCD_OK;
-    }
-    case NQCD_FIELD_SAVE_IMPURITY: {  // (nsave, ntraj)
-        if (bytes != (int64_t)sizeof(double) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
-        std::vector<double> t((size_t)ns * nt);
-        HIPCHK(h, hipMemcpy(t.data(), h->save_di.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
-        double* o = static_cast<double*>(out);
-        for (int64_t i = 0; i < nt; ++i)
-            for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = t[(size_t)sl * nt + i];
         return NQCD_OK;
     }
     default: return fail(h, NQCD_ERR_UNSUPPORTED, "field not available for IESH / EhrenfestNA");

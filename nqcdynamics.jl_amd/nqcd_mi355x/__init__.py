@@ -11,10 +11,13 @@ from . import _lib  # noqa: F401
 from .distributions import (Adiabatic, Diabatic, DynamicalDistribution, DynamicsVariables, Normal,  # noqa: F401
                             PositionHarmonicWigner, PureState, VelocityHarmonicWigner)
 from .ensembles import EnsembleMI355X, TerminatingCallback, run_dynamics  # noqa: F401
-from .models import (DebyeSpectralDensity, DoubleWellGaussian, OhmicSpectralDensity, SpinBoson,  # noqa: F401
-                     TullyModelOne, TullyModelThree, TullyModelTwo, nstates)
-from .outputs import (FileReduction, MeanReduction, OutputAdiabaticPopulation, OutputDiabaticPopulation,  # noqa: F401
+from .models import (AndersonHolstein, DebyeSpectralDensity, DoubleWellGaussian, Harmonic, MiaoSubotnik,  # noqa: F401
+                     OhmicSpectralDensity, SpinBoson, TrapezoidalRule, TullyModelOne, TullyModelThree, TullyModelTwo,
+                     nelectrons, nstates)
+from .outputs import (FileReduction, MeanReduction, OutputAdiabaticPopulation, OutputCentroidPosition,  # noqa: F401
+                      OutputDiabaticPopulation,
                       OutputDiscreteState, OutputFinal, OutputFinalTime, OutputStateResolvedScattering1D,
                       OutputSurfaceHops, PopulationCorrelationFunction, SortByOutputReduction,
                       SortByTrajectoryReduction, SumReduction)
-from .simulations import Atoms, Classical, Ehrenfest, FSSH, RingPolymerSimulation, Simulation, masses  # noqa: F401
+from .simulations import (AdiabaticIESH, Atoms, Classical, DecoherenceCorrectionEDC, Ehrenfest,  # noqa: F401
+                          EhrenfestNA, FSSH, RingPolymerSimulation, Simulation, masses)

@@ -13,7 +13,7 @@ from . import _lib as L
 from .distributions import Adiabatic, DynamicalDistribution, ProductDistribution, PureState
 from .outputs import (MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction, SumReduction,
                       output_name)
-from .simulations import Ehrenfest, FSSH, RingPolymerSimulation, Simulation
+from .simulations import AdiabaticIESH, Classical, Ehrenfest, EhrenfestNA, FSSH, RingPolymerSimulation, Simulation
 
 log = logging.getLogger("nqcd_mi355x")
 
@@ -38,17 +38,40 @@ class TerminatingCallback:
         self.dof, self.value = int(dof), float(value)
 
 
+def _kind(sim):
+    """'sh' (FSSH/Ehrenfest), 'ie' (IESH/EhrenfestNA) or 'rp' (ring polymers); raises for the rest."""
+    m = sim.method
+    if isinstance(sim, RingPolymerSimulation):
+        if isinstance(m, (Classical, FSSH, Ehrenfest)):
+            return "rp"
+    elif isinstance(m, (FSSH, Ehrenfest)):
+        return "sh"
+    elif isinstance(m, (AdiabaticIESH, EhrenfestNA)):
+        return "ie"
+    raise ValueError(f"EnsembleMI355X does not support {type(sim).__name__}{{{type(m).__name__}}}")
+
+
 def _problem(E, sim, dt):
     method = sim.method
-    if not isinstance(method, (FSSH, Ehrenfest)):
-        raise ValueError(f"EnsembleMI355X does not support {type(method).__name__} simulations")
-    if isinstance(sim, RingPolymerSimulation):
-        raise ValueError("EnsembleMI355X does not support ring-polymer simulations of this method yet")
-    E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
-                  algorithm=method.algorithm, rescaling=getattr(method, "rescaling_code", 0),
-                  precision=L.PREC_FP64 if sim.precision == "fp64" else L.PREC_FP32_NUCLEI,
+    kind = _kind(sim)
+    common = dict(precision=L.PREC_FP64 if sim.precision == "fp64" else L.PREC_FP32_NUCLEI,
                   schedule=L.SCHED_PERSISTENT if E._schedule == "persistent" else L.SCHED_STEP_PER_LAUNCH,
-                  temperature=sim.temperature)
+                  rescaling=getattr(method, "rescaling_code", 0))
+    if kind == "sh":
+        E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=method.algorithm, temperature=sim.temperature, **common)
+    elif kind == "ie":
+        edc = isinstance(method, AdiabaticIESH) and method.decoherence is not None
+        E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=method.algorithm, n_electrons=sim.model.nelectrons,
+                      decoherence=L.DECOHERENCE_EDC if edc else L.DECOHERENCE_NONE,
+                      edc_C=method.decoherence.C if edc else 0.1,
+                      estimate_probability=getattr(method, "estimate_probability", True), **common)
+    else:
+        code = {Classical: L.METHOD_RPMD, FSSH: L.METHOD_RPSH, Ehrenfest: L.METHOD_RP_EHRENFEST}[type(method)]
+        E.set_problem(code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=L.ALG_BCB if code == L.METHOD_RPMD else L.ALG_BCB_TSIT5, nbeads=sim.nbeads,
+                      temperature=sim.temperature, **common)
     E.set_model(0, sim.model.kind, sim.model.params())
 
 
@@ -60,8 +83,62 @@ def _electronic(dist):
     return PureState(1, Adiabatic())
 
 
+def _configurations(nuc, trajectories, selection):
+    """(r, v) stacks (ntraj, flattened column-major) from one DynamicsVariables or a list distribution."""
+    if isinstance(nuc, dict):
+        r = np.tile(nuc["r"].reshape(-1, order="F"), (trajectories, 1))
+        v = np.tile(nuc["v"].reshape(-1, order="F"), (trajectories, 1))
+        return r, v
+    if isinstance(nuc, DynamicalDistribution) and nuc.is_list():
+        idx = np.arange(trajectories) if selection is None else np.asarray(selection) - 1
+        r = np.stack([np.asarray(nuc.position[i]).reshape(-1, order="F") for i in idx])
+        v = np.stack([np.asarray(nuc.velocity[i]).reshape(-1, order="F") for i in idx])
+        return r, v
+    return None
+
+
+def _initial_ie(E, sim, dist, trajectories, selection, seed):
+    """IESH / EhrenfestNA: the lowest n_electrons adiabatic orbitals occupied, psi[state_e, e] = 1
+    (DynamicsVariables(sim, v, r), iesh.jl:89-99, ehrenfest_na.jl:43-48)."""
+    nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    conf = _configurations(nuc, trajectories, selection)
+    if conf is not None:
+        r, v = conf
+        occ = np.tile(np.arange(sim.model.nelectrons, dtype=np.int32), (r.shape[0], 1))
+        E.load_initial_iesh(r[:, 0], v[:, 0], occ)
+        return
+    if isinstance(nuc, DynamicalDistribution):
+        mr, sr, mv, sv = nuc.moments()
+        E.sample_initial(L.SAMPLER_GAUSSIAN, trajectories, [mr[0], sr[0], mv[0], sv[0]], seed)
+        return
+    raise ValueError(f"unsupported distribution {type(nuc).__name__}")
+
+
+def _initial_rp(E, sim, dist, trajectories, selection, seed):
+    """Ring polymers: bead configurations (ndofs, natoms, nbeads) uploaded per trajectory."""
+    el = _electronic(dist)
+    nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    conf = _configurations(nuc, trajectories, selection)
+    if conf is None:
+        raise ValueError("ring-polymer initial conditions must be DynamicsVariables or a list of bead configurations")
+    r, v = conf
+    n = max(sim.model.nstates, 2)
+    sig = np.zeros((r.shape[0], n, n), complex)
+    if sim.model.nstates > 1:
+        if el.diabatic:
+            raise ValueError("ring-polymer initial conditions need an adiabatic PureState")
+        sig[:, el.state - 1, el.state - 1] = 1.0
+    state = None if isinstance(sim.method, Classical) else np.full(r.shape[0], el.state - 1, np.int32)
+    E.load_initial(r, v, sig, state=state)
+
+
 def _initial_conditions(E, sim, dist, trajectories, selection, seed):
     """Sample on the device where the distribution is recognised, otherwise upload."""
+    kind = _kind(sim)
+    if kind == "ie":
+        return _initial_ie(E, sim, dist, trajectories, selection, seed)
+    if kind == "rp":
+        return _initial_rp(E, sim, dist, trajectories, selection, seed)
     el = _electronic(dist)
     nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
     D = sim.ndofs * sim.natoms
@@ -138,8 +215,9 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
         if not isinstance(callback, TerminatingCallback):
             raise ValueError("EnsembleMI355X evaluates TerminatingCallback(|x_dof| > value) predicates only")
         term = (callback.dof - 1, callback.value)
-    if not isinstance(sim.method, (FSSH, Ehrenfest)) or isinstance(sim, RingPolymerSimulation):
-        raise ValueError(f"EnsembleMI355X does not support {type(sim).__name__}{{{type(sim.method).__name__}}}")
+    kind = _kind(sim)
+    if kind != "sh" and callback is not None:
+        raise ValueError("termination callbacks are supported for FSSH / Ehrenfest simulations")
     if int(trajectories) < 1:
         raise ValueError("trajectories must be >= 1")
     log.info("Performing %d trajectories on MI355X device %d.", trajectories, ensemble_algorithm.device)
@@ -160,6 +238,39 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
 
 
 def _per_traj_results(E, sim, t0, dt, save_every, nsteps):
+    kind = _kind(sim)
+    if kind == "ie":
+        res = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
+               "psi": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
+               "state": E.download(L.FIELD_STATE), "steps": E.download(L.FIELD_STEPS),
+               "nhops": E.download(L.FIELD_NHOPS), "pop_ad": E.download(L.FIELD_SAVE_POP_AD),
+               "pop_di": E.download(L.FIELD_SAVE_POP_DI)}
+        return [{"r": res["r"][i].reshape(sim.size()), "v": res["v"][i].reshape(sim.size()), "sigma": res["psi"][i],
+                 "state": res["state"][i], "steps": int(res["steps"][i]), "nhops": int(res["nhops"][i]),
+                 "pop_ad": res["pop_ad"][i], "pop_di": res["pop_di"][i], "final_pop_ad": res["pop_ad"][i][-1],
+                 "t0": t0, "dt": dt} for i in range(E.ntraj)]
+    if kind == "rp":
+        r, v = E.download(L.FIELD_R), E.download(L.FIELD_V)
+        cen = E.download(L.FIELD_SAVE_CENTROID)
+        out = []
+        elec = sim.model.nstates > 1
+        if elec:
+            sig = E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM)
+            st, nh = E.download(L.FIELD_STATE), E.download(L.FIELD_NHOPS)
+            pa, pd, ss = E.download(L.FIELD_SAVE_POP_AD), E.download(L.FIELD_SAVE_POP_DI), E.download(L.FIELD_SAVE_STATE)
+        for i in range(E.ntraj):
+            d = {"r": r[i].reshape(sim.size(), order="F"), "v": v[i].reshape(sim.size(), order="F"),
+                 "centroid": cen[i], "steps": nsteps, "t0": t0, "dt": dt}
+            if elec:
+                fin = np.zeros(2)
+                if isinstance(sim.method, FSSH):
+                    fin[st[i]] = 1.0
+                else:
+                    fin = np.real(np.diag(sig[i]))
+                d.update({"sigma": sig[i], "state": int(st[i]), "nhops": int(nh[i]), "pop_ad": pa[i], "pop_di": pd[i],
+                          "save_state": ss[i], "final_pop_ad": fin})
+            out.append(d)
+        return out
     res = _results(E, t0, dt, save_every, nsteps)
     n = sim.model.nstates
     fin = np.zeros((E.ntraj, n))
@@ -192,6 +303,8 @@ def _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nst
                 elif kind == L.OUT_SCATTERING_1D:
                     v = E.reduce(kind, red, 2 * n)
                     result[output_name(f)] = {"reflection": v[:n], "transmission": v[n:]}
+                elif kind == L.OUT_CENTROID_POSITION:
+                    result[output_name(f)] = list(E.reduce(kind, red, E.nsave))
                 elif kind == L.OUT_POPULATION_CORRELATION:
                     v = E.reduce(kind, red, n * n * E.nsave).reshape(E.nsave, n, n)
                     result[output_name(f)] = [m.T for m in v]  # column-major n x n per save

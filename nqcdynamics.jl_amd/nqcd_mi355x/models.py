@@ -112,3 +112,60 @@ class SpinBoson(Model):
 
 def nstates(model):
     return model.nstates
+
+
+class Harmonic(Model):
+    """Single-state harmonic well V = m w^2 (x - r0)^2 / 2 (ring-polymer tests)."""
+    kind = L.MODEL_HARMONIC
+    nstates = 1
+
+    def __init__(self, m=1.0, omega=1.0, r0=0.0):
+        self.m, self.omega, self.r0 = m, omega, r0
+
+    def params(self):
+        return [self.m, self.omega, self.r0]
+
+
+class MiaoSubotnik:
+    """Impurity diabats U0 = m w^2 x^2 / 2 and U1 = m w^2 (x - g)^2 / 2 + dG with hybridisation Gamma
+    (NQCModels' MiaoSubotnik defaults)."""
+
+    def __init__(self, m=2000.0, omega=2e-4, g=20.6097, Gamma=6.4e-3, dG=-3.8e-3):
+        self.m, self.omega, self.g, self.Gamma, self.dG = m, omega, g, Gamma, dG
+
+
+class TrapezoidalRule:
+    """Wide-band bath of M evenly spaced levels eps_n = a + (n-1)(b-a)/(M-1) with couplings
+    sqrt(Gamma/2pi) sqrt((b-a)/(M-1)) (docs/src/NQCModels/systembathmodels.md, TrapezoidalRule)."""
+
+    def __init__(self, M, bandmin, bandmax):
+        if M < 2:
+            raise ValueError("TrapezoidalRule needs M >= 2")
+        self.M, self.bandmin, self.bandmax = int(M), float(bandmin), float(bandmax)
+
+    def energies(self):
+        return self.bandmin + np.arange(self.M) * (self.bandmax - self.bandmin) / (self.M - 1)
+
+
+class AndersonHolstein(Model):
+    """Newns-Anderson impurity model: the impurity of `model` coupled to the discretised `bath`.
+    nstates = M + 1 and, unless given, M // 2 electrons (half filling: the count the reference's
+    IESH / EhrenfestNA tests use with every Fermi level they set, test/Dynamics/iesh.jl:18-30,
+    ehrenfest_na.jl:12-19).  A Fermi level shifts every orbital energy alike, which changes neither
+    the forces nor the populations of the dynamics."""
+    kind = L.MODEL_NEWNS_ANDERSON
+
+    def __init__(self, model, bath, fermi_level=0.0, nelectrons=None):
+        if not isinstance(model, MiaoSubotnik):
+            raise ValueError("EnsembleMI355X provides the MiaoSubotnik impurity")
+        self.model, self.bath, self.fermi_level = model, bath, float(fermi_level)
+        self.nstates = bath.M + 1
+        self.nelectrons = bath.M // 2 if nelectrons is None else int(nelectrons)
+
+    def params(self):
+        m = self.model
+        return [m.m, m.omega, m.g, m.dG, m.Gamma, self.bath.bandmin, self.bath.bandmax, self.bath.M]
+
+
+def nelectrons(model):
+    return model.nelectrons

@@ -39,6 +39,8 @@ OutputDiscreteState = _FnOutput("OutputDiscreteState", lambda r: list(r["save_st
 OutputSurfaceHops = _FnOutput("OutputSurfaceHops", lambda r: r["nhops"])
 # OutputFinalTime (DynamicsOutputs.jl:228-229).
 OutputFinalTime = _FnOutput("OutputFinalTime", lambda r: r["t0"] + r["steps"] * r["dt"])
+# OutputCentroidPosition (DynamicsOutputs.jl:73-79): ring-polymer centroid per save.
+OutputCentroidPosition = _FnOutput("OutputCentroidPosition", lambda r: list(r["centroid"]), L.OUT_CENTROID_POSITION)
 # OutputFinal (DynamicsOutputs.jl:222): final dynamics variables.
 OutputFinal = _FnOutput("OutputFinal", lambda r: {"v": r["v"], "r": r["r"], "sigma": r["sigma"], "state": r["state"] + 1})
 

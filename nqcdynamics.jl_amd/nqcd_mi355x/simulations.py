@@ -53,8 +53,42 @@ class Ehrenfest(Method):
 
 
 class Classical(Method):
-    """Classical molecular dynamics (classical.jl:22)."""
+    """Classical molecular dynamics (classical.jl:22); ring polymers integrate it with BCB."""
     method_code = None
+    rescaling_code = L.RESCALE_STANDARD
+
+
+class DecoherenceCorrectionEDC:
+    """Granucci-Persico energy decoherence correction (decoherence_corrections.jl:10-14)."""
+
+    def __init__(self, C=0.1):
+        self.C = float(C)
+
+
+class AdiabaticIESH(Method):
+    """Independent electron surface hopping (iesh.jl:26-87): rescaling, estimate_probability,
+    disable_hopping and decoherence keywords."""
+    method_code = L.METHOD_IESH
+    algorithm = L.ALG_VERLET_ELECTRONICS
+
+    def __init__(self, rescaling="standard", estimate_probability=True, disable_hopping=False, decoherence=None):
+        codes = {"standard": L.RESCALE_STANDARD, "vinversion": L.RESCALE_VINVERSION, "off": L.RESCALE_OFF}
+        if rescaling not in codes:
+            raise ValueError("This mode of rescaling is not implemented")
+        if disable_hopping:
+            raise ValueError("EnsembleMI355X does not provide disable_hopping=true")
+        if decoherence is not None and not isinstance(decoherence, DecoherenceCorrectionEDC):
+            raise ValueError("decoherence must be DecoherenceCorrectionEDC() or None")
+        self.rescaling, self.rescaling_code = rescaling, codes[rescaling]
+        self.estimate_probability = bool(estimate_probability)
+        self.decoherence = decoherence
+
+
+class EhrenfestNA(Method):
+    """Ehrenfest dynamics of the many-electron Newns-Anderson wavefunction (ehrenfest_na.jl:6-41)."""
+    method_code = L.METHOD_EHRENFEST_NA
+    algorithm = L.ALG_VERLET_ELECTRONICS
+    rescaling_code = L.RESCALE_STANDARD
 
 
 class AbstractSimulation:

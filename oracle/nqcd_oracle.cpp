@@ -1075,7 +1075,7 @@ OR_API void oracle_iesh_probabilities(int32_t N, int32_t Ne, const double* psi_r
 static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t gid, double r0, double v0,
                        const int32_t* occ0, const double* psi_re0, const double* psi_im0,
                        double* r_out, double* v_out, double* psi_re, double* psi_im, int32_t* occ_out,
-                       int32_t* nhops_out, double* pop_imp) {
+                       int32_t* nhops_out, double* pop_di) {
     const int N = md.M + 1, Ne = R.nelec;
     std::vector<double> V, E(N), U((size_t)N * N), Up((size_t)N * N), G((size_t)N * N), d((size_t)N * N);
     double dh, U0, dU0;
@@ -1111,25 +1111,30 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
         return -f / R.mass;
     };
     auto save_pop = [&](int slot) {
-        if (!pop_imp || slot >= R.nsave) return;
-        // iesh_diabatic_population (iesh.jl:351-369), impurity (diabatic state 0) component
-        double p = 0.0;
-        if (R.method == 1) {  // EhrenfestNA mean field: sum_e |(U psi)_{0e}|^2
-            for (int e = 0; e < Ne; ++e) {
-                cd s = 0.0;
-                for (int a = 0; a < N; ++a) s += U[a] * psi[(size_t)a * Ne + e];
-                p += std::norm(s);
+        if (!pop_di || slot >= R.nsave) return;
+        double* out = pop_di + ((size_t)i * R.nsave + slot) * N;
+        for (int a = 0; a < N; ++a) {  // diabatic state a; U[a * N + b] = component a of eigenvector b
+            double p = 0.0;
+            if (R.method == 1) {  // EhrenfestNA mean field: sum_e |(U psi)_{a e}|^2
+                for (int e = 0; e < Ne; ++e) {
+                    cd sum = 0.0;
+                    for (int b = 0; b < N; ++b) sum += U[(size_t)a * N + b] * psi[(size_t)b * Ne + e];
+                    p += std::norm(sum);
+                }
+            } else {  // iesh_diabatic_population (iesh.jl:351-369): diag(U rho_e U^T), rho_e = offdiag(c c^T) + e_s e_s^T
+                for (int e = 0; e < Ne; ++e) {
+                    double s1 = 0.0, s2 = 0.0;
+                    for (int b = 0; b < N; ++b) {
+                        const double c = psi[(size_t)b * Ne + e].real(), u = U[(size_t)a * N + b];
+                        s1 += u * c;
+                        s2 += u * u * c * c;
+                    }
+                    const double us = U[(size_t)a * N + occ[e]];
+                    p += s1 * s1 - s2 + us * us;
+                }
             }
-            pop_imp[(size_t)i * R.nsave + slot] = p;
-            return;
+            out[a] = p;
         }
-        for (int e = 0; e < Ne; ++e) {
-            double s1 = 0.0, s2 = 0.0;
-            for (int a = 0; a < N; ++a) { const double c = psi[(size_t)a * Ne + e].real(); s1 += U[a] * c; s2 += U[a] * U[a] * c * c; }
-            const int s = occ[e];
-            p += s1 * s1 - s2 + U[s] * U[s];
-        }
-        pop_imp[(size_t)i * R.nsave + slot] = p;
     };
     update();
     kacc = accel();
@@ -1226,9 +1231,9 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
 OR_API int32_t oracle_iesh_run_ensemble(const OIEDesc* R, const ONADesc* md, int64_t ntraj, const double* r0,
                                         const double* v0, const int32_t* occ0, const double* psi_re0,
                                         const double* psi_im0, double* r, double* v, double* psi_re, double* psi_im,
-                                        int32_t* occ, int32_t* nhops, double* pop_imp, int32_t nthreads) {
+                                        int32_t* occ, int32_t* nhops, double* pop_di, int32_t nthreads) {
     parallel_for(ntraj, nthreads, [&](int64_t i) {
-        ie_run_one(*R, *md, i, (uint64_t)i, r0[i], v0[i], occ0, psi_re0, psi_im0, r, v, psi_re, psi_im, occ, nhops, pop_imp);
+        ie_run_one(*R, *md, i, (uint64_t)i, r0[i], v0[i], occ0, psi_re0, psi_im0, r, v, psi_re, psi_im, occ, nhops, pop_di);
     });
     return 0;
 }

@@ -321,7 +321,7 @@ def iesh_run_ensemble(method, md, nelec, mass, r0, v0, occ0, psi0=None, dt=1.0, 
     nsave = nsteps // save_every + 1 if save_every else 1
     R.save_every, R.nsave = save_every, nsave
     out = dict(r=np.zeros(ntraj), v=np.zeros(ntraj), psi_re=np.zeros((ntraj, N, nelec)), psi_im=np.zeros((ntraj, N, nelec)),
-               occ=np.zeros((ntraj, nelec), np.int32), nhops=np.zeros(ntraj, np.int32), pop_imp=np.zeros((ntraj, nsave)))
+               occ=np.zeros((ntraj, nelec), np.int32), nhops=np.zeros(ntraj, np.int32), pop_di=np.zeros((ntraj, nsave, N)))
     if nthreads is None:
         nthreads = min(16, os.cpu_count() or 1)
     f = lib().oracle_iesh_run_ensemble
@@ -329,9 +329,10 @@ def iesh_run_ensemble(method, md, nelec, mass, r0, v0, occ0, psi0=None, dt=1.0, 
     f(C.byref(R), C.byref(md), C.c_int64(ntraj), _p(np.ascontiguousarray(r0, dtype=np.float64)),
       _p(np.ascontiguousarray(v0, dtype=np.float64)), _p(occ0, C.c_int32),
       _p(np.ascontiguousarray(psi0.real)), _p(np.ascontiguousarray(psi0.imag)), _p(out["r"]), _p(out["v"]),
-      _p(out["psi_re"]), _p(out["psi_im"]), _p(out["occ"], C.c_int32), _p(out["nhops"], C.c_int32), _p(out["pop_imp"]),
+      _p(out["psi_re"]), _p(out["psi_im"]), _p(out["occ"], C.c_int32), _p(out["nhops"], C.c_int32), _p(out["pop_di"]),
       C.c_int32(nthreads))
     out["psi"] = out["psi_re"] + 1j * out["psi_im"]
+    out["pop_imp"] = out["pop_di"][:, :, 0]
     return out
 
 

@@ -33,8 +33,9 @@ def _gpu(L, method, M, Ne, r0, v0, occ0, dt, nsteps, key, save_every=10, **kw):
     out = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
            "psi": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
            "occ": E.download(L.FIELD_STATE), "nhops": E.download(L.FIELD_NHOPS),
-           "imp": E.download(L.FIELD_SAVE_IMPURITY), "ad": E.download(L.FIELD_SAVE_POP_AD),
-           "imp_mean": E.reduce(L.OUT_IMPURITY_POPULATION, L.RED_MEAN, E.nsave), "stats": stats}
+           "di": E.download(L.FIELD_SAVE_POP_DI), "ad": E.download(L.FIELD_SAVE_POP_AD),
+           "di_mean": E.reduce(L.OUT_DIABATIC_POPULATION, L.RED_MEAN, E.nsave * (M + 1)).reshape(E.nsave, M + 1),
+           "stats": stats}
     E.close()
     return out
 
@@ -53,7 +54,7 @@ def _compare(g, c, hops=True):
     assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-11)
     assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-13)
     assert np.allclose(g["psi"], c["psi"], rtol=1e-9, atol=1e-10)
-    assert np.allclose(g["imp"], c["pop_imp"], rtol=1e-9, atol=1e-11)
+    assert np.allclose(g["di"], c["pop_di"], rtol=1e-9, atol=1e-11)
 
 
 @pytest.mark.parametrize("M,Ne,ntraj,nsteps", [(40, 20, 64, 300), (100, 50, 12, 120)])
@@ -69,7 +70,7 @@ def test_iesh_fp64_parity(oracle, key, M, Ne, ntraj, nsteps):
     last = np.zeros((ntraj, M + 1))
     np.put_along_axis(last, c["occ"], 1.0, axis=1)
     assert np.array_equal(g["ad"][:, -1, :], last)
-    assert np.allclose(g["imp_mean"], c["pop_imp"].mean(axis=0), rtol=1e-9, atol=1e-12)
+    assert np.allclose(g["di_mean"], c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-12)
 
 
 def test_iesh_estimate_off_and_rescaling_modes(oracle, key):

@@ -1,0 +1,55 @@
+"""run_dynamics (the mirrored reference API) for IESH on the Newns-Anderson model and for ring-polymer
+surface hopping, against the FP64 oracle on the same uploaded initial conditions."""
+import numpy as np
+import pytest
+
+import nqcd_mi355x as N
+
+pytestmark = pytest.mark.gpu
+
+
+def test_iesh_run_dynamics_mean_populations(oracle, key):
+    M, ntraj, dt, nsteps = 40, 48, 2.0, 200
+    W = 6 * 6.4e-3 / 2
+    model = N.AndersonHolstein(N.MiaoSubotnik(), N.TrapezoidalRule(M, -W, W))
+    sim = N.Simulation[N.AdiabaticIESH](N.Atoms(2000), model)
+    rng = np.random.default_rng(5)
+    r0 = rng.normal(8.0, 1.0, size=ntraj)
+    v0 = rng.normal(0.0, 1e-2, size=ntraj)
+    dist = N.DynamicalDistribution([np.array([[x]]) for x in v0], [np.array([[x]]) for x in r0], (1, 1))
+    res = N.run_dynamics(sim, (0.0, nsteps * dt), dist, trajectories=ntraj, dt=dt, saveat=10 * dt,
+                         output=(N.OutputDiabaticPopulation, N.OutputAdiabaticPopulation),
+                         reduction=N.MeanReduction(), ensemble_algorithm=N.EnsembleMI355X(seed=key))
+    md = oracle.na_model(M=M, W=W)
+    occ0 = np.tile(np.arange(M // 2), (ntraj, 1))
+    c = oracle.iesh_run_ensemble(0, md, M // 2, 2000.0, r0, v0, occ0, dt=dt, nsteps=nsteps, key=key, save_every=10)
+    di = np.array(res["OutputDiabaticPopulation"])
+    assert di.shape == (nsteps // 10 + 1, M + 1)
+    assert np.allclose(di, c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-12)
+    ad = np.array(res["OutputAdiabaticPopulation"])
+    final = np.zeros(M + 1)
+    for o in c["occ"]:
+        final[o] += 1.0 / ntraj
+    assert np.allclose(ad[-1], final, atol=1e-12)
+    assert np.allclose(res["Time"], np.arange(nsteps // 10 + 1) * 10 * dt)
+
+
+def test_rpsh_run_dynamics_centroid_and_populations(oracle, key):
+    P, ntraj, nsteps = 8, 64, 400
+    DW2 = [1 / 1200, 1 / 40, 0.01, 0.005, 1.0]
+    model = N.DoubleWellGaussian(*DW2)
+    sim = N.RingPolymerSimulation[N.FSSH](N.Atoms(1836.0), model, P, temperature=1 / 8)
+    rng = np.random.default_rng(6)
+    r0 = rng.normal(-2.0, 0.05, size=(ntraj, 1)) + rng.normal(0, 0.05, size=(ntraj, P))
+    v0 = rng.uniform(0.011, 0.018, size=(ntraj, 1)) + rng.normal(0, 1e-4, size=(ntraj, P))
+    dist = N.DynamicalDistribution([v.reshape(1, 1, P) for v in v0], [r.reshape(1, 1, P) for r in r0], (1, 1, P))
+    res = N.run_dynamics(sim, (0.0, float(nsteps)), dist * N.PureState(1, N.Adiabatic()), trajectories=ntraj, dt=1.0,
+                         saveat=10.0, output=(N.OutputCentroidPosition, N.OutputDiabaticPopulation),
+                         reduction=N.MeanReduction(), ensemble_algorithm=N.EnsembleMI355X(seed=key))
+    m = oracle.make_model(oracle.MODEL_DW_GAUSSIAN, 2, 1, DW2)
+    sig0 = np.zeros((ntraj, 2, 2), complex); sig0[:, 0, 0] = 1
+    c = oracle.rp_run_ensemble(oracle.RP_FSSH, m, 1836.0, r0, v0, sig0, np.zeros(ntraj, np.int32), dt=1.0,
+                               nsteps=nsteps, temperature=1 / 8, save_every=10, nsave=nsteps // 10 + 1, key=key)
+    assert c["nhops"].sum() > 0
+    assert np.allclose(res["OutputCentroidPosition"], c["centroid"].mean(axis=0), rtol=1e-9, atol=1e-11)
+    assert np.allclose(np.array(res["OutputDiabaticPopulation"]), c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-11)

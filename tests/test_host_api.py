@@ -58,6 +58,29 @@ def test_run_dynamics_rejects_before_touching_the_device():
         N.run_dynamics(sim, (0.0, 10.0), u, output=N.OutputSurfaceHops, dt=1.0, saveat=0.5)
     with pytest.raises(ValueError, match="TerminatingCallback"):
         N.run_dynamics(sim, (0.0, 10.0), u, output=N.OutputSurfaceHops, callback=lambda u, t, i: True)
-    rp = N.RingPolymerSimulation[N.Classical](N.Atoms(2000), N.DoubleWellGaussian(nstates=1), 8)
+    na = N.AndersonHolstein(N.MiaoSubotnik(), N.TrapezoidalRule(40, -0.0192, 0.0192))
+    rp = N.RingPolymerSimulation[N.AdiabaticIESH](N.Atoms(2000), na, 8)
     with pytest.raises(ValueError, match="does not support"):
         N.run_dynamics(rp, (0.0, 10.0), u, output=N.OutputSurfaceHops)
+    ie = N.Simulation[N.AdiabaticIESH](N.Atoms(2000), na)
+    with pytest.raises(ValueError, match="termination"):
+        N.run_dynamics(ie, (0.0, 10.0), u, output=N.OutputSurfaceHops, callback=N.TerminatingCallback(1, 10.0))
+
+
+def test_newns_anderson_model_and_iesh_options():
+    """AndersonHolstein(MiaoSubotnik(), TrapezoidalRule(M, -W, W)): M + 1 states, M/2 electrons for a
+    symmetric band (test/Dynamics/iesh.jl:18-30); method keywords of iesh.jl:72-87."""
+    W = 6 * 6.4e-3 / 2
+    bath = N.TrapezoidalRule(30, -W, W)
+    assert np.isclose(bath.energies()[0], -W) and np.isclose(bath.energies()[-1], W)
+    na = N.AndersonHolstein(N.MiaoSubotnik(), bath)
+    assert N.nstates(na) == 31 and N.nelectrons(na) == 15
+    assert N.nelectrons(N.AndersonHolstein(N.MiaoSubotnik(), bath, fermi_level=0.001)) == 15
+    assert na.params() == [2000.0, 2e-4, 20.6097, -3.8e-3, 6.4e-3, -W, W, 30]
+    sim = N.Simulation[N.AdiabaticIESH](N.Atoms(2000), na, rescaling="vinversion", estimate_probability=False,
+                                        decoherence=N.DecoherenceCorrectionEDC(0.2))
+    assert sim.method.rescaling_code == 1 and not sim.method.estimate_probability and sim.method.decoherence.C == 0.2
+    with pytest.raises(ValueError):
+        N.Simulation[N.AdiabaticIESH](N.Atoms(2000), na, disable_hopping=True)
+    with pytest.raises(ValueError):
+        N.Simulation[N.AdiabaticIESH](N.Atoms(2000), na, rescaling="other")

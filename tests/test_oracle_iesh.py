@@ -197,3 +197,18 @@ def test_ehrenfest_na_against_exact_equations_of_motion(oracle):
     assert np.isclose(out["v"][0], yT[1], rtol=1e-3)
     psiT = (yT[2:2 + N * Ne] + 1j * yT[2 + N * Ne:]).reshape(N, Ne)
     assert np.allclose(out["psi"][0], psiT, rtol=1e-2, atol=1e-2)
+
+
+def test_iesh_diabatic_population(oracle, key):
+    """iesh_diabatic_population (iesh.jl:351-369): for psi = the occupied adiabatic orbitals it is the
+    diabatic density sum_e U[a, s_e]^2 of the Slater determinant; it always sums to n_electrons."""
+    md = oracle.na_model(M=20)
+    N, Ne, mass = 21, 10, 2000.0
+    r0 = np.array([7.0, 9.0]); v0 = np.array([5e-3, -5e-3])
+    occ0 = np.tile(np.arange(Ne), (2, 1)); occ0[1, 0] = 15
+    out = oracle.iesh_run_ensemble(0, md, Ne, mass, r0, v0, occ0, dt=2.0, nsteps=40, key=key, save_every=10)
+    for i in range(2):
+        E, U, _ = oracle.na_eigen(md, r0[i])
+        assert np.allclose(out["pop_di"][i, 0], np.sum(U[:, occ0[i]] ** 2, axis=1), atol=1e-13)
+    assert np.allclose(out["pop_di"].sum(axis=2), Ne, atol=1e-10)
+    assert np.all(np.abs(out["pop_di"][:, -1] - out["pop_di"][:, 0]) > 0)
