@@ -100,6 +100,7 @@ enum {
     MODEL_SPIN_BOSON = 3,
     MODEL_DW_GAUSSIAN = 4,
     MODEL_HARMONIC = 5,
+    MODEL_DIATOM_SURFACE = 7,
 };
 
 struct OModelDesc {
@@ -181,6 +182,40 @@ static void model_eval(const OModelDesc& m, const double* r, double* V, double* 
         for (int j = 0; j < D; ++j) {
             V[0] += 0.5 * mass * w * w * (r[j] - x0) * (r[j] - x0);
             dV[j] = mass * w * w * (r[j] - x0);
+        }
+        break;
+    }
+    case MODEL_DIATOM_SURFACE: {
+        // Diatomic molecule above a surface, coordinates (x1, y1, z1, x2, y2, z2); bond length
+        // b = |R1 - R2|, height Z = mu z1 + (1 - mu) z2.  Two Morse diabats and a Gaussian coupling:
+        //   V11 = Dn (1 - e^{-an (b - bn)})^2 + Aw e^{-bw Z}
+        //   V22 = Di (1 - e^{-ai (b - bi)})^2 + Ds (1 - e^{-as (Z - Zs)})^2 + Delta
+        //   V12 = C e^{-g (Z - Zc)^2}
+        const double* q = m.p;
+        const double Dn = q[0], an = q[1], bn = q[2], Aw = q[3], bw = q[4], Di = q[5], ai = q[6], bi = q[7];
+        const double Ds = q[8], as = q[9], Zs = q[10], Delta = q[11], C = q[12], g = q[13], Zc = q[14], mu = q[15];
+        const double dx = r[0] - r[3], dy = r[1] - r[4], dz = r[2] - r[5];
+        const double b = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const double Z = mu * r[2] + (1.0 - mu) * r[5];
+        auto morse = [](double D, double a, double q0, double x, double& f, double& df) {
+            const double e = std::exp(-a * (x - q0));
+            f = D * (1.0 - e) * (1.0 - e);
+            df = 2.0 * D * a * e * (1.0 - e);
+        };
+        double mn, dmn, mi, dmi, ms, dms;
+        morse(Dn, an, bn, b, mn, dmn);
+        morse(Di, ai, bi, b, mi, dmi);
+        morse(Ds, as, Zs, Z, ms, dms);
+        const double wall = Aw * std::exp(-bw * Z);
+        const double v12 = C * std::exp(-g * (Z - Zc) * (Z - Zc));
+        V[0] = mn + wall; V[3] = mi + ms + Delta; V[1] = V[2] = v12;
+        const double db11 = dmn, dZ11 = -bw * wall, db22 = dmi, dZ22 = dms, dZ12 = -2.0 * g * (Z - Zc) * v12;
+        const double ub[6] = {dx / b, dy / b, dz / b, -dx / b, -dy / b, -dz / b};
+        const double uZ[6] = {0.0, 0.0, mu, 0.0, 0.0, 1.0 - mu};
+        for (int j = 0; j < 6; ++j) {
+            dV[(size_t)j * 4 + 0] = db11 * ub[j] + dZ11 * uZ[j];
+            dV[(size_t)j * 4 + 3] = db22 * ub[j] + dZ22 * uZ[j];
+            dV[(size_t)j * 4 + 1] = dV[(size_t)j * 4 + 2] = dZ12 * uZ[j];
         }
         break;
     }
@@ -488,13 +523,13 @@ struct ORunDesc {
     double dt;
     double t0;
     int64_t nsteps;       // maximum number of nuclear steps
-    double term_x;        // terminate when |r[0]| > term_x (disabled when <= 0)
+    double term_x;        // terminate when |r[term_dof]| > term_x (disabled when <= 0)
     int32_t save_every;   // save slot every this many steps (slot 0 = t0); 0 = none
     int32_t nsave;        // number of save slots per trajectory
     uint64_t key;
     int64_t traj_offset;  // global id of trajectory 0 of this call
     int32_t nsub;         // electronic sub-steps per nuclear step (5)
-    int32_t pad;
+    int32_t term_dof;     // flattened dof index of the termination predicate
 };
 
 struct Cache {
@@ -597,7 +632,7 @@ static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, 
     int64_t step = 0;
     std::vector<double> prob(n);
     while (step < R.nsteps) {
-        if (R.term_x > 0 && std::fabs(r[0]) > R.term_x) break;
+        if (R.term_x > 0 && std::fabs(r[R.term_dof]) > R.term_x) break;
         std::vector<cd> sigprev = sig;
         for (int I = 0; I < D; ++I) vtmp[I] = v[I] + (R.dt / 2) * k[I];   // step_B!
         for (int I = 0; I < D; ++I) r[I] = r[I] + R.dt * vtmp[I];         // step_A!

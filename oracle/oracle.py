@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "libnqcd_oracle.so")
 
 MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC = range(6)
+MODEL_DIATOM_SURFACE = 7
 METHOD_FSSH, METHOD_EHRENFEST = 0, 1
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = 0, 1, 2
 
@@ -26,7 +27,7 @@ class ORunDesc(C.Structure):
     _fields_ = [("method", C.c_int32), ("nstates", C.c_int32), ("ndof", C.c_int32), ("rescaling", C.c_int32),
                 ("dt", C.c_double), ("t0", C.c_double), ("nsteps", C.c_int64), ("term_x", C.c_double),
                 ("save_every", C.c_int32), ("nsave", C.c_int32), ("key", C.c_uint64), ("traj_offset", C.c_int64),
-                ("nsub", C.c_int32), ("pad", C.c_int32)]
+                ("nsub", C.c_int32), ("term_dof", C.c_int32)]
 
 
 def build():
@@ -179,7 +180,7 @@ def diabatic_to_adiabatic(m, r0, diabatic_state):
 
 def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None, dt=1.0, nsteps=100, term_x=0.0,
                  save_every=0, nsave=1, key=0, traj_offset=0, rescaling=RESCALE_STANDARD, nthreads=None, t0=0.0, nsub=5,
-                 traj_ids=None):
+                 traj_ids=None, term_dof=0):
     """Run an ensemble on the CPU; returns a dict of per-trajectory results."""
     if not isinstance(models, (list, tuple)):
         models = [models]
@@ -189,6 +190,7 @@ def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None,
     R.method, R.nstates, R.ndof, R.rescaling = method, n, D, rescaling
     R.dt, R.t0, R.nsteps, R.term_x = dt, t0, nsteps, term_x
     R.save_every, R.nsave, R.key, R.traj_offset, R.nsub = save_every, nsave, key, traj_offset, nsub
+    R.term_dof = term_dof
     marr = (OModelDesc * len(models))(*models)
     mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, dtype=np.float64), (D,)))
     sr0 = np.ascontiguousarray(np.real(sig0), dtype=np.float64).reshape(ntraj, n, n)
