@@ -27,6 +27,7 @@
 //     unoccupied states inner (iesh.jl:318-335).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -47,6 +48,7 @@ struct IeshArgs {
     int64_t ntraj;
     int64_t nsteps;
     int32_t N, M, Ne, cw;         // states, bath levels, electrons, psi columns in use (16 * column tiles)
+    int32_t sb_len, pad0;         // doubles of the S / probability workspace in LDS
     int32_t method;               // 0 = IESH, 1 = EhrenfestNA
     int32_t rescaling, estimate, edc;
     int32_t save_every, nsave;
@@ -57,7 +59,6 @@ struct IeshArgs {
     double* r; double* v; double* psi; int32_t* occ; int32_t* nhops; int32_t* steps;
     double* save_di;              // (nsave, N, ntraj)
     double* save_ad;              // (nsave, N, ntraj)
-    double* scratch;              // per workgroup 2 Ne^2 doubles (inverse columns on hop steps)
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -92,13 +93,6 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     return s;
 }
 
-// Lane-level ordering of LDS traffic inside one wavefront (no workgroup barrier).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 struct Scalars { double h, dh, dU0; };
 
 __device__ __forceinline__ Scalars na_scalars(const IeshArgs& A, double x) {
@@ -111,34 +105,41 @@ __device__ __forceinline__ Scalars na_scalars(const IeshArgs& A, double x) {
     return s;
 }
 
-// Root i of the secular function in (lo, hi); returns E_i and f'(E_i).
+// 1/x from the hardware estimate and two Newton-Raphson corrections (within an ulp).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Root of the secular function f(l) = l - h - V^2 sum_k 1/(l - eps_k) in (lo, hi), where f
+// increases from -inf to +inf: safeguarded Newton, stopping when f is at its rounding level or the
+// step is an ulp.  Returns E_i and f'(E_i) = 1 + V^2 sum_k 1/(E_i - eps_k)^2.
 __device__ double secular_root(const double* eps, int M, double V2, double h, double lo, double hi, double guess,
                                double tol, double& fp) {
     double lam = (guess > lo && guess < hi) ? guess : 0.5 * (lo + hi);
+    fp = 1.0;
     for (int it = 0; it < 100; ++it) {
-        double s = 0.0, s2 = 0.0;
+        double s = 0.0, s2 = 0.0, sa = 0.0;
         for (int k = 0; k < M; ++k) {
-            const double q = 1.0 / (lam - eps[k]);
+            const double q = rcp_nr(lam - eps[k]);
             s += q;
-            s2 += q * q;
+            s2 = fma(q, q, s2);
+            sa += fabs(q);
         }
         const double f = lam - h - V2 * s;
-        const double d = 1.0 + V2 * s2;
+        fp = 1.0 + V2 * s2;
+        if (fabs(f) <= 8.0 * __DBL_EPSILON__ * (fabs(lam) + fabs(h) + V2 * sa)) break;
         if (f > 0.0) hi = lam;
-        else if (f < 0.0) lo = lam;
-        else break;
-        double nl = lam - f / d;
+        else lo = lam;
+        double nl = lam - f / fp;
         if (!(nl > lo && nl < hi)) nl = 0.5 * (lo + hi);
-        const bool done = fabs(nl - lam) <= tol || (hi - lo) <= tol;
+        const bool done = fabs(nl - lam) <= 2.0 * __DBL_EPSILON__ * fabs(nl) || (hi - lo) <= tol;
         lam = nl;
         if (done) break;
     }
-    double s2 = 0.0;
-    for (int k = 0; k < M; ++k) {
-        const double q = 1.0 / (lam - eps[k]);
-        s2 += q * q;
-    }
-    fp = 1.0 + V2 * s2;
     return lam;
 }
 
@@ -151,16 +152,18 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
     constexpr int CW = 16 * CTM; // row stride of psi in LDS (compile time, so LDS offsets are immediates)
     const int N = A.N, M = A.M, Ne = A.Ne, CT = A.cw / 16;
     double* Tb = smem;                       // NP x CW: psi (Re, Im interleaved per electron) / Taylor term
-    double* Sb = Tb + NP * CW;               // Ne x Ne complex, LU in place
-    double* E = Sb + 2 * Ne * Ne;            // NP adiabatic energies
+    double* Sb = Tb + NP * CW;               // Ne x Ne complex S, inverted in place
+    double* E = Sb + A.sb_len;               // NP adiabatic energies
     double* U = E + NP;                      // NP impurity components of the eigenvectors
     double* eps = U + NP;                    // NP bath levels
     double* red = eps + NP;                  // NT reduction slots
-    double* pcol = red + 8;                  // NP per-state / per-electron temporaries
-    double* scal = pcol + NP;                // 8 broadcast scalars
+    double* pcol = red + 8;                  // max(NP, 2 kIeMaxNe) per-state / per-electron temporaries
+    double* part = pcol + (NP > 2 * kIeMaxNe ? NP : 2 * kIeMaxNe);  // 8 kIeMaxNe partial sums / probabilities
+    double* scal = part + 8 * kIeMaxNe;      // 8 broadcast scalars
     int* occ = reinterpret_cast<int*>(scal + 8);  // kIeMaxNe occupied states
     int* isocc = occ + kIeMaxNe;                  // NP electron index occupying the state, -1 if empty
-    int* piv = isocc + NP;                        // kIeMaxNe LU pivots
+    int* piv = isocc + NP;                        // kIeMaxNe pivot rows
+    int* unocc = piv + kIeMaxNe;                  // NP unoccupied states, ascending
 
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const double dt = A.dt;
@@ -168,7 +171,6 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
     for (int k = tid; k < NP; k += B) eps[k] = (k < M) ? A.a + k * A.de : 0.0;
     __syncthreads();
     const double zn = sqrt((double)M) * A.V;
-    double* Xg = A.scratch + (size_t)blockIdx.x * 2 * Ne * Ne;
 
     for (int64_t traj = blockIdx.x; traj < A.ntraj; traj += gridDim.x) {
         __syncthreads();
@@ -391,15 +393,23 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
             // ---- IESH hop (iesh.jl:390-407) -------------------------------------------------------------
             if (A.method == 0) {
                 const double rnd = philox_uniform((uint64_t)traj, (uint32_t)step, STREAM_HOP, A.key);
-                for (int idx = tid; idx < Ne * Ne; idx += B) {
+                for (int idx = tid; idx < Ne * Ne; idx += B) {  // S_je = psi[occ_j, e] (iesh.jl:309-316)
                     const int j = idx / Ne, e = idx - j * Ne;
                     Sb[2 * idx] = Tb[occ[j] * CW + 2 * e];
                     Sb[2 * idx + 1] = Tb[occ[j] * CW + 2 * e + 1];
                 }
-                __syncthreads();
-                if (w == 0) {  // LU with partial pivoting, lane = row (FastDeterminant.jl:20-22)
-                    double dre = 1.0, dim = 0.0;
-                    for (int p = 0; p < Ne; ++p) {
+                if (tid == 0) {  // unoccupied states in ascending order (DynamicsUtils.jl:162-171)
+                    int c = 0;
+                    for (int m = 0; m < N; ++m)
+                        if (isocc[m] < 0) unocc[c++] = m;
+                }
+                // Gauss-Jordan inversion of S in place with partial pivoting; det S is the signed
+                // product of the pivots, the same pivots an LU factorisation takes (FastDeterminant.jl:20-22).
+                double dre = 1.0, dim = 0.0;
+                bool singular = false;
+                for (int p = 0; p < Ne; ++p) {
+                    __syncthreads();
+                    if (w == 0) {
                         double val = -1.0;
                         int idx = l;
                         if (l >= p && l < Ne) val = hypot(Sb[2 * (l * Ne + p)], Sb[2 * (l * Ne + p) + 1]);
@@ -409,39 +419,69 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
                             const int oi = __shfl_xor(idx, o);
                             if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
                         }
-                        if (val == 0.0) { dre = 0.0; dim = 0.0; break; }
-                        if (l == 0) piv[p] = idx;
-                        if (idx != p) {
-                            if (l < Ne) {
-                                const double a0 = Sb[2 * (p * Ne + l)], a1 = Sb[2 * (p * Ne + l) + 1];
-                                Sb[2 * (p * Ne + l)] = Sb[2 * (idx * Ne + l)];
-                                Sb[2 * (p * Ne + l) + 1] = Sb[2 * (idx * Ne + l) + 1];
-                                Sb[2 * (idx * Ne + l)] = a0;
-                                Sb[2 * (idx * Ne + l) + 1] = a1;
-                            }
-                            dre = -dre; dim = -dim;
+                        if (l == 0) {
+                            piv[p] = (val > 0.0) ? idx : -1;
+                            scal[1] = Sb[2 * (idx * Ne + p)];
+                            scal[2] = Sb[2 * (idx * Ne + p) + 1];
+                            scal[3] = Sb[2 * (p * Ne + p)];  // moves to row idx with the interchange
+                            scal[4] = Sb[2 * (p * Ne + p) + 1];
                         }
-                        wave_sync();
-                        const double pr = Sb[2 * (p * Ne + p)], pi = Sb[2 * (p * Ne + p) + 1];
-                        const double nre = dre * pr - dim * pi, nim = dre * pi + dim * pr;
-                        dre = nre; dim = nim;
-                        if (l > p && l < Ne) {
-                            const double den = pr * pr + pi * pi;
-                            const double ar = Sb[2 * (l * Ne + p)], ai = Sb[2 * (l * Ne + p) + 1];
-                            const double fr = (ar * pr + ai * pi) / den, fi = (ai * pr - ar * pi) / den;
-                            Sb[2 * (l * Ne + p)] = fr;
-                            Sb[2 * (l * Ne + p) + 1] = fi;
-                            for (int c = p + 1; c < Ne; ++c) {
-                                const double br = Sb[2 * (p * Ne + c)], bi = Sb[2 * (p * Ne + c) + 1];
-                                Sb[2 * (l * Ne + c)] -= fr * br - fi * bi;
-                                Sb[2 * (l * Ne + c) + 1] -= fr * bi + fi * br;
-                            }
-                        }
-                        wave_sync();
                     }
-                    if (l == 0) { scal[1] = dre; scal[2] = dim; }
+                    __syncthreads();
+                    const int q = piv[p];
+                    if (q < 0) { singular = true; break; }
+                    const double pr = scal[1], pi = scal[2];
+                    {
+                        const double nre = dre * pr - dim * pi, nim = dre * pi + dim * pr;
+                        dre = (q != p) ? -nre : nre;
+                        dim = (q != p) ? -nim : nim;
+                    }
+                    const double den = pr * pr + pi * pi;
+                    const double ir = pr / den, ii = -pi / den;  // 1 / pivot
+                    // column p after the row swap, for the elimination
+                    for (int r = tid; r < Ne; r += B) {
+                        if (r == p) continue;
+                        pcol[2 * r] = (r == q) ? scal[3] : Sb[2 * (r * Ne + p)];
+                        pcol[2 * r + 1] = (r == q) ? scal[4] : Sb[2 * (r * Ne + p) + 1];
+                    }
+                    // swap rows p and q, then scale the pivot row (its pivot entry becomes 1 / pivot)
+                    for (int c = tid; c < Ne; c += B) {
+                        const double ar = Sb[2 * (p * Ne + c)], ai = Sb[2 * (p * Ne + c) + 1];
+                        double br = Sb[2 * (q * Ne + c)], bi = Sb[2 * (q * Ne + c) + 1];
+                        if (q != p) { Sb[2 * (q * Ne + c)] = ar; Sb[2 * (q * Ne + c) + 1] = ai; }
+                        if (c == p) { br = 1.0; bi = 0.0; }
+                        Sb[2 * (p * Ne + c)] = br * ir - bi * ii;
+                        Sb[2 * (p * Ne + c) + 1] = br * ii + bi * ir;
+                    }
+                    __syncthreads();
+                    for (int idx = tid; idx < Ne * Ne; idx += B) {
+                        const int r = idx / Ne, c = idx - r * Ne;
+                        if (r == p) continue;
+                        const double fr = pcol[2 * r], fi = pcol[2 * r + 1];
+                        const double xr = (c == p) ? 0.0 : Sb[2 * idx], xi = (c == p) ? 0.0 : Sb[2 * idx + 1];
+                        const double yr = Sb[2 * (p * Ne + c)], yi = Sb[2 * (p * Ne + c) + 1];
+                        Sb[2 * idx] = xr - (fr * yr - fi * yi);
+                        Sb[2 * idx + 1] = xi - (fr * yi + fi * yr);
+                    }
+                }
+                __syncthreads();
+                if (!singular) {  // undo the row interchanges as column interchanges: Sb = S^-1
+                    for (int p = Ne - 1; p >= 0; --p) {
+                        const int q = piv[p];
+                        if (q != p) {
+                            for (int r = tid; r < Ne; r += B) {
+                                const double ar = Sb[2 * (r * Ne + p)], ai = Sb[2 * (r * Ne + p) + 1];
+                                Sb[2 * (r * Ne + p)] = Sb[2 * (r * Ne + q)];
+                                Sb[2 * (r * Ne + p) + 1] = Sb[2 * (r * Ne + q) + 1];
+                                Sb[2 * (r * Ne + q)] = ar;
+                                Sb[2 * (r * Ne + q) + 1] = ai;
+                            }
+                        }
+                        __syncthreads();
+                    }
                 }
                 // sum over unoccupied m and electrons n of |v_dot_d[m, n]| (iesh.jl:285-298)
+                const int Nu = N - Ne;
                 double loc = 0.0;
                 for (int m = tid; m < N; m += B) {
                     if (isocc[m] < 0) {
@@ -451,92 +491,78 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
                         }
                     }
                 }
-                const double sumvd = block_sum<NT>(loc, red);  // contains the barrier after the LU
-                const double dre = scal[1], dim = scal[2];
+                const double sumvd = block_sum<NT>(loc, red);
                 const double Akk = dre * dre + dim * dim;
                 const double pref = 2.0 * dt / Akk;
                 const double est = pref * sumvd * (fabs(dre) + fabs(dim));
-                const bool evaluate = Akk > 0.0 && (!A.estimate || !(est < rnd));
+                const bool evaluate = !singular && Akk > 0.0 && (!A.estimate || !(est < rnd));
                 if (evaluate) {
-                    // columns of S^-1 (wavefront 0, lane = column) into this workgroup's scratch
-                    if (w == 0 && l < Ne) {
-                        for (int e = 0; e < Ne; ++e) { Xg[2 * (e * Ne + l)] = (e == l) ? 1.0 : 0.0; Xg[2 * (e * Ne + l) + 1] = 0.0; }
-                        for (int p = 0; p < Ne; ++p) {
-                            const int q = piv[p];
-                            if (q != p) {
-                                const double t0 = Xg[2 * (p * Ne + l)], t1 = Xg[2 * (p * Ne + l) + 1];
-                                Xg[2 * (p * Ne + l)] = Xg[2 * (q * Ne + l)];
-                                Xg[2 * (p * Ne + l) + 1] = Xg[2 * (q * Ne + l) + 1];
-                                Xg[2 * (q * Ne + l)] = t0;
-                                Xg[2 * (q * Ne + l) + 1] = t1;
-                            }
+                    // P(n -> m) = 2 dt Re((psi_m S^-1)_n) v_dot_d[m, n]: det(S with row n replaced by
+                    // psi_m) = det(S) (psi_m S^-1)_n, so Re(det S conj(det S_{n<-m})) / |det S|^2 is that real part.
+                    // pass 1: per-electron totals, partial sums over interleaved chunks of unoccupied states
+                    const int nch = max(1, min(8, B / Ne));
+                    if (tid < Ne * nch) {
+                        const int n = tid % Ne, ch = tid / Ne;
+                        const int o = occ[n];
+                        double part_sum = 0.0;
+                        for (int mi = ch; mi < Nu; mi += nch) {
+                            const int m = unocc[mi];
+                            double zr = 0.0;
+                            for (int e = 0; e < Ne; ++e)
+                                zr += Tb[m * CW + 2 * e] * Sb[2 * (e * Ne + n)] - Tb[m * CW + 2 * e + 1] * Sb[2 * (e * Ne + n) + 1];
+                            const double vd = -v * (sc.dh * (U[m] * U[o] / (E[o] - E[m])));
+                            part_sum += fmin(fmax(2.0 * dt * zr * vd, 0.0), 1.0);
                         }
-                        for (int e = 0; e < Ne; ++e) {  // L y = P e_l (unit lower)
-                            double yr = Xg[2 * (e * Ne + l)], yi = Xg[2 * (e * Ne + l) + 1];
-                            for (int j = 0; j < e; ++j) {
-                                const double lr = Sb[2 * (e * Ne + j)], li = Sb[2 * (e * Ne + j) + 1];
-                                const double xr = Xg[2 * (j * Ne + l)], xi = Xg[2 * (j * Ne + l) + 1];
-                                yr -= lr * xr - li * xi;
-                                yi -= lr * xi + li * xr;
-                            }
-                            Xg[2 * (e * Ne + l)] = yr;
-                            Xg[2 * (e * Ne + l) + 1] = yi;
+                        part[n * 8 + ch] = part_sum;
+                    }
+                    __syncthreads();
+                    for (int n = tid; n < Ne; n += B) {
+                        double sn = 0.0;
+                        for (int ch = 0; ch < nch; ++ch) sn += part[n * 8 + ch];
+                        pcol[n] = sn;
+                    }
+                    __syncthreads();
+                    if (tid == 0) {  // the electron whose cumulative total passes the uniform, if any
+                        double cum = 0.0;
+                        int hn = -1;
+                        for (int n = 0; n < Ne; ++n) {
+                            if (rnd < cum + pcol[n]) { hn = n; break; }
+                            cum += pcol[n];
                         }
-                        for (int e = Ne - 1; e >= 0; --e) {  // U x = y
-                            double yr = Xg[2 * (e * Ne + l)], yi = Xg[2 * (e * Ne + l) + 1];
-                            for (int j = e + 1; j < Ne; ++j) {
-                                const double ur = Sb[2 * (e * Ne + j)], ui = Sb[2 * (e * Ne + j) + 1];
-                                const double xr = Xg[2 * (j * Ne + l)], xi = Xg[2 * (j * Ne + l) + 1];
-                                yr -= ur * xr - ui * xi;
-                                yi -= ur * xi + ui * xr;
-                            }
-                            const double dr = Sb[2 * (e * Ne + e)], di = Sb[2 * (e * Ne + e) + 1];
-                            const double den = dr * dr + di * di;
-                            Xg[2 * (e * Ne + l)] = (yr * dr + yi * di) / den;
-                            Xg[2 * (e * Ne + l) + 1] = (yi * dr - yr * di) / den;
+                        scal[3] = (double)hn;
+                        scal[4] = cum;
+                    }
+                    __syncthreads();
+                    const int hsel = (int)scal[3];
+                    if (hsel >= 0) {  // pass 2: that electron's probabilities, in unoccupied order
+                        const int o = occ[hsel];
+                        for (int mi = tid; mi < Nu; mi += B) {
+                            const int m = unocc[mi];
+                            double zr = 0.0;
+                            for (int e = 0; e < Ne; ++e)
+                                zr += Tb[m * CW + 2 * e] * Sb[2 * (e * Ne + hsel)] - Tb[m * CW + 2 * e + 1] * Sb[2 * (e * Ne + hsel) + 1];
+                            const double vd = -v * (sc.dh * (U[m] * U[o] / (E[o] - E[m])));
+                            part[mi] = fmin(fmax(2.0 * dt * zr * vd, 0.0), 1.0);
                         }
                     }
                     __syncthreads();
-                    double cum = 0.0;
-                    int hit_n = -1, hit_m = -1;
-                    for (int n = 0; n < Ne; ++n) {
-                        for (int m = tid; m < N; m += B) {
-                            double p = 0.0;
-                            if (isocc[m] < 0) {
-                                double zr = 0.0;
-                                for (int e = 0; e < Ne; ++e)
-                                    zr += Tb[m * CW + 2 * e] * Xg[2 * (e * Ne + n)] - Tb[m * CW + 2 * e + 1] * Xg[2 * (e * Ne + n) + 1];
-                                const int o = occ[n];
-                                const double vd = -v * (sc.dh * (U[m] * U[o] / (E[o] - E[m])));
-                                p = 2.0 * dt * zr * vd;
-                                p = fmin(fmax(p, 0.0), 1.0);
+                    if (tid == 0) {  // cumulative selection, electrons outer, unoccupied inner (iesh.jl:318-335)
+                        int hn = -1, hm = -1;
+                        if (hsel >= 0) {
+                            double cum = scal[4];
+                            for (int mi = 0; mi < Nu; ++mi) {
+                                cum += part[mi];
+                                if (rnd < cum) { hn = hsel; hm = unocc[mi]; break; }
                             }
-                            pcol[m] = p;
                         }
-                        __syncthreads();
-                        if (tid == 0) {
-                            for (int m = 0; m < N; ++m) {
-                                if (isocc[m] >= 0) continue;
-                                cum += pcol[m];
-                                if (rnd < cum) { hit_n = n; hit_m = m; break; }
-                            }
-                            scal[3] = (double)hit_n;
-                            scal[4] = (double)hit_m;
-                        }
-                        __syncthreads();
-                        const int hn = (int)scal[3];
-                        __syncthreads();
-                        if (hn >= 0) break;
-                    }
-                    if (tid == 0) {
                         scal[5] = v;
-                        if (hit_n >= 0) {  // rescale along d[new, old] (surface_hopping.jl:65-92)
-                            const int old = occ[hit_n];
+                        if (hn >= 0) {  // rescale along d[new, old] (surface_hopping.jl:65-92)
+                            const int old = occ[hn];
                             bool ok = true;
                             double vn = v;
                             if (A.rescaling != NQCD_RESCALE_OFF) {
-                                const double dd = sc.dh * (U[hit_m] * U[old] / (E[old] - E[hit_m]));
-                                const double aq = dd * dd / A.mass / 2.0, bq = dd * v, cq = E[hit_m] - E[old];
+                                const double dd = sc.dh * (U[hm] * U[old] / (E[old] - E[hm]));
+                                const double aq = dd * dd / A.mass / 2.0, bq = dd * v, cq = E[hm] - E[old];
                                 const double disc = bq * bq - 4.0 * aq * cq;
                                 if (disc < 0.0) {
                                     ok = false;
@@ -549,8 +575,8 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
                             }
                             if (ok) {
                                 isocc[old] = -1;
-                                isocc[hit_m] = hit_n;
-                                occ[hit_n] = hit_m;
+                                isocc[hm] = hn;
+                                occ[hn] = hm;
                                 ++nh;
                             }
                             scal[5] = vn;
@@ -601,10 +627,12 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
     }
 }
 
-size_t ie_lds_bytes(int NT, int cw, int Ne) {
+int ie_sb_len(int N, int Ne) { (void)N; return 2 * Ne * Ne; }
+
+size_t ie_lds_bytes(int NT, int cw, int N, int Ne) {
     const int NP = 16 * NT;
-    return sizeof(double) * ((size_t)NP * cw + 2 * (size_t)Ne * Ne + 3 * NP + 8 + NP + 8) +
-           sizeof(int) * (2 * kIeMaxNe + NP);
+    return sizeof(double) * ((size_t)NP * cw + ie_sb_len(N, Ne) + 3 * NP + 8 + std::max(NP, 2 * kIeMaxNe) + 8 * kIeMaxNe + 8) +
+           sizeof(int) * (2 * kIeMaxNe + 2 * NP);
 }
 
 typedef void (*IeKernel)(const IeshArgs);
@@ -754,7 +782,7 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     const bool wide = CT > NT;
     IeKernel kern = ie_pick(NT, wide);
     if (!kern) return fail(h, NQCD_ERR_UNSUPPORTED, "n_states");
-    const size_t lds = ie_lds_bytes(NT, 16 * (wide ? 8 : NT), Ne);
+    const size_t lds = ie_lds_bytes(NT, 16 * (wide ? 8 : NT), N, Ne);
     HIPCHK(h, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int block = 64 * NT;
     int bpc = 0;
@@ -771,12 +799,12 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     HIPCHK(h, h->nhops.ensure(sizeof(int32_t) * n));
     HIPCHK(h, h->save_ad.ensure(sizeof(double) * N * h->nsave * n));
     HIPCHK(h, h->save_di.ensure(sizeof(double) * N * h->nsave * n));
-    HIPCHK(h, h->ie_scratch.ensure(sizeof(double) * 2 * Ne * Ne * grid));
 
     IeshArgs A{};
     A.ntraj = n;
     A.nsteps = nsteps;
     A.N = N; A.M = md.M; A.Ne = Ne; A.cw = 16 * CT;
+    A.sb_len = ie_sb_len(N, Ne);
     A.method = (h->prob.method == NQCD_METHOD_IESH) ? 0 : 1;
     A.rescaling = h->prob.rescaling;
     A.estimate = h->prob.disable_estimate ? 0 : 1;
@@ -793,7 +821,6 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     A.r = h->r.as<double>(); A.v = h->v.as<double>(); A.psi = h->sig.as<double>(); A.occ = h->state.as<int32_t>();
     A.nhops = h->nhops.as<int32_t>(); A.steps = h->steps.as<int32_t>();
     A.save_di = h->save_di.as<double>(); A.save_ad = h->save_ad.as<double>();
-    A.scratch = h->ie_scratch.as<double>();
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, h->stream, A);
     HIPCHK(h, hipGetLastError());
