@@ -92,7 +92,16 @@ enum nqcd_model_kind {
     /* Newns-Anderson impurity (MiaoSubotnik diabats) with a trapezoidal bath discretisation:
      * params: m, omega, g, dG, Gamma, a, b, M.  U0 = m w^2 x^2/2, impurity h = m w^2 (x-g)^2/2 + dG - U0,
      * bath eps_k = a + k (b-a)/(M-1), coupling sqrt(Gamma (b-a)/(M-1) / 2pi); n_states = M + 1. */
-    NQCD_MODEL_NEWNS_ANDERSON = 6
+    NQCD_MODEL_NEWNS_ANDERSON = 6,
+    /* Diatomic molecule above a surface, 6 coordinates (ndofs = 3, natoms = 2), two diabats:
+     * params: Dn an bn Aw bw Di ai bi Ds as Zs Delta C g Zc mu with bond b = |R1 - R2| and
+     * height Z = mu z1 + (1 - mu) z2:  V11 = Dn (1 - e^{-an(b-bn)})^2 + Aw e^{-bw Z},
+     * V22 = Di (1 - e^{-ai(b-bi)})^2 + Ds (1 - e^{-as(Z-Zs)})^2 + Delta, V12 = C e^{-g (Z-Zc)^2}. */
+    NQCD_MODEL_DIATOM_SURFACE = 7,
+    /* The MLP surrogate of the diatom-surface model (trained with nqcd_mi355x_surrogate_train):
+     * params: fallback threshold (Hartree), then the 16 diatom-surface parameters of the analytic
+     * model used wherever the two ensemble members disagree by more than the threshold. */
+    NQCD_MODEL_MLP_SURROGATE = 8
 };
 
 /* IESH decoherence keyword (iesh.jl:72-73): none, or Granucci-Persico EDC
@@ -181,7 +190,9 @@ enum nqcd_field {
     NQCD_FIELD_SAVE_POP_AD = 7,  /* double (n, nsave, ntraj) adiabatic population per save     */
     NQCD_FIELD_SAVE_POP_DI = 8,  /* double (n, nsave, ntraj) diabatic population per save      */
     NQCD_FIELD_SAVE_STATE = 9,   /* int32 (nsave, ntraj) active state per save                 */
-    NQCD_FIELD_SAVE_CENTROID = 10 /* double (nsave, ntraj) ring-polymer centroid per save       */
+    NQCD_FIELD_SAVE_CENTROID = 10, /* double (nsave, ntraj) ring-polymer centroid per save      */
+    NQCD_FIELD_FALLBACKS = 11,     /* int32 (ntraj) surrogate steps that fell back to the analytic model */
+    NQCD_FIELD_SAVE_R = 12         /* double (ndofs * natoms, nsave, ntraj) configuration per save (diatom-surface) */
 };
 /* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
  * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD / SAVE_POP_DI (n, nsave, ntraj). */
@@ -249,6 +260,32 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
 /* Statistics of the last nqcd_mi355x_run (device time from HIP events on the handle's stream).
  * Replaces the @timed wall clock around solve (run_dynamics.jl:91). */
 int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out);
+
+/* ---- MLP surrogate of the diabatic Hamiltonian (no counterpart in the reference) -------------
+ * A 6 -> 128 -> 128 -> 3 tanh network for (H11, H22, H12) of a six-coordinate two-state model,
+ * two ensemble members, trained on the GPU with BF16 MFMA forward / backward kernels and Adam. */
+typedef struct {
+    int32_t steps;               /* optimiser steps per member */
+    int32_t batch;               /* samples per step (>= 32) */
+    double lr, lr_final;         /* Adam learning rate, decayed geometrically to lr_final */
+    double loss_weights[3];      /* weights of the H11, H22, H12 squared errors */
+    uint64_t seed;               /* initial weights and minibatch order */
+} nqcd_train_desc;
+
+/* Train both members on X (n, 6) -> Y (n, 3) (row-major, Hartree / bohr).  Inputs and outputs are
+ * normalised by the data's mean and standard deviation.  loss_history (2 * steps) may be NULL. */
+int32_t nqcd_mi355x_surrogate_train(nqcd_handle_t* h, const double* X, const double* Y, int64_t n,
+                                    const nqcd_train_desc* desc, double* loss_history);
+/* H (n, 3) and dH/dX (n, 3, 6) of one member at X (n, 6); dH may be NULL. */
+int32_t nqcd_mi355x_surrogate_eval(nqcd_handle_t* h, int32_t member, const double* X, int64_t n, double* H, double* dH);
+/* Gradient of the training loss with respect to the 17795 FP32 parameters of one member on (X, Y),
+ * computed by the training kernels without an update (loss weights w[3]). */
+int32_t nqcd_mi355x_surrogate_gradient(nqcd_handle_t* h, int32_t member, const double* X, const double* Y, int64_t n,
+                                       const double* w, double* grad, double* loss);
+/* Parameters of one member: W1 (128x6), b1, W2 (128x128), b2, W3 (3x128), b3 row-major (17795
+ * floats); norm = in_shift[6], in_scale[6], out_shift[3], out_scale[3] (may be NULL on set). */
+int32_t nqcd_mi355x_surrogate_set_params(nqcd_handle_t* h, int32_t member, const float* params, const float* norm);
+int32_t nqcd_mi355x_surrogate_get_params(nqcd_handle_t* h, int32_t member, float* params, float* norm);
 
 /* Message for the last failed call on h (thread-local when h is NULL). */
 const char* nqcd_mi355x_last_error(nqcd_handle_t* h);

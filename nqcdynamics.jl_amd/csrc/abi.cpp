@@ -21,6 +21,8 @@
 #include "spin_boson.h"
 #include "ring_polymer.h"
 #include "iesh.h"
+#include "fssh_nd.h"
+#include "mlp_host.h"
 
 using namespace nqcd;
 
@@ -161,7 +163,9 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    DevBuf* bufs[] = {&h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
+    DevBuf* bufs[] = {&h->mlp_master, &h->mlp_packed, &h->mlp_adam_m, &h->mlp_adam_v, &h->mlp_x, &h->mlp_t, &h->mlp_idx,
+                      &h->mlp_partial, &h->mlp_losspart, &h->mlp_lossbuf, &h->mlp_grad, &h->fallbacks,
+                      &h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
                       &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid,
                       &h->ie_scratch};
@@ -224,6 +228,10 @@ int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t kind, cons
     case NQCD_MODEL_DW_GAUSSIAN: need = 5; break;
     case NQCD_MODEL_HARMONIC: need = 3; break;
     case NQCD_MODEL_NEWNS_ANDERSON: return ie_set_model(h, slot, params, n);
+    case NQCD_MODEL_DIATOM_SURFACE:
+    case NQCD_MODEL_MLP_SURROGATE:
+        if (slot != 0) return fail(h, NQCD_ERR_UNSUPPORTED, "the diatom-surface / surrogate model must use slot 0");
+        return nd_set_model(h, kind, params, n);
     case NQCD_MODEL_SPIN_BOSON: {
         if (slot != 0) return fail(h, NQCD_ERR_UNSUPPORTED, "spin-boson model must use slot 0");
         if (n < 4 || (n - 2) % 2 != 0) return fail(h, NQCD_ERR_ARG, "spin-boson params: eps, Delta, omega[nb], c[nb]");
@@ -344,7 +352,10 @@ int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S,
         if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = true; h->have_results = false; }
         return rc;
     }
-    if (!is_two_state_1d(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "sampler needs a two-state one-dof problem");
+    if (!is_two_state_1d(h) && !nd_is_problem(h))
+        return fail(h, NQCD_ERR_UNSUPPORTED, "sampler needs a two-state one-dof or diatom-surface problem");
+    if (nd_is_problem(h) && (S->kind != NQCD_SAMPLER_GAUSSIAN || S->state_basis != 0))
+        return fail(h, NQCD_ERR_UNSUPPORTED, "diatom-surface sampling: GAUSSIAN with an adiabatic initial state");
     h->ntraj = ntraj;
     HIPCHK(h, h->r0.ensure(sizeof(double) * D * ntraj));
     HIPCHK(h, h->v0.ensure(sizeof(double) * D * ntraj));
@@ -425,6 +436,9 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
         if (rc != NQCD_OK) return rc;
     } else if (is_ie(h)) {
         int32_t rc = ie_run(h, t0, nsteps, term, key);
+        if (rc != NQCD_OK) return rc;
+    } else if (nd_is_problem(h)) {
+        int32_t rc = nd_run(h, t0, nsteps, term, key);
         if (rc != NQCD_OK) return rc;
     } else if (is_spin_boson(h)) {
         if (!h->sb_initial) return fail(h, NQCD_ERR_STATE, "spin-boson runs need nqcd_mi355x_sample_initial");
@@ -604,6 +618,24 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
             for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = s[(size_t)sl * nt + i];
         return NQCD_OK;
     }
+    case NQCD_FIELD_SAVE_R: {
+        if (!nd_is_problem(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "configurations per save exist for the diatom-surface model");
+        if (bytes != (int64_t)sizeof(double) * D * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(h->save_centroid, sizeof(double) * D * ns * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl)
+                for (int I = 0; I < D; ++I) o[((size_t)i * ns + sl) * D + I] = s[((size_t)sl * D + I) * nt + i];
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_FALLBACKS: {
+        if (!nd_is_problem(h) || h->models[0].kind != NQCD_MODEL_MLP_SURROGATE)
+            return fail(h, NQCD_ERR_UNSUPPORTED, "fallback counts exist for surrogate runs only");
+        if (bytes != (int64_t)sizeof(int32_t) * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, hipMemcpy(out, h->fallbacks.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost));
+        return NQCD_OK;
+    }
     case NQCD_FIELD_SAVE_STATE: {
         if (bytes != (int64_t)sizeof(int32_t) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
         HIPCHK(h, fetch(h->save_state, sizeof(int32_t) * ns * nt, raw));
@@ -615,6 +647,33 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
     }
     default: return fail(h, NQCD_ERR_ARG, "unknown field");
     }
+}
+
+int32_t nqcd_mi355x_surrogate_train(nqcd_handle_t* h, const double* X, const double* Y, int64_t n,
+                                    const nqcd_train_desc* desc, double* loss_history) {
+    if (!h || !X || !Y || !desc) return fail(h, NQCD_ERR_ARG, "null argument");
+    return mlp_train(h, X, Y, n, desc, loss_history);
+}
+
+int32_t nqcd_mi355x_surrogate_eval(nqcd_handle_t* h, int32_t member, const double* X, int64_t n, double* H, double* dH) {
+    if (!h || !X || !H) return fail(h, NQCD_ERR_ARG, "null argument");
+    return mlp_eval(h, member, X, n, H, dH);
+}
+
+int32_t nqcd_mi355x_surrogate_gradient(nqcd_handle_t* h, int32_t member, const double* X, const double* Y, int64_t n,
+                                       const double* w, double* grad, double* loss) {
+    if (!h || !X || !Y || !w || !grad || !loss) return fail(h, NQCD_ERR_ARG, "null argument");
+    return mlp_gradient(h, member, X, Y, n, w, grad, loss);
+}
+
+int32_t nqcd_mi355x_surrogate_set_params(nqcd_handle_t* h, int32_t member, const float* params, const float* norm) {
+    if (!h || !params) return fail(h, NQCD_ERR_ARG, "null argument");
+    return mlp_set_params(h, member, params, norm);
+}
+
+int32_t nqcd_mi355x_surrogate_get_params(nqcd_handle_t* h, int32_t member, float* params, float* norm) {
+    if (!h || !params) return fail(h, NQCD_ERR_ARG, "null argument");
+    return mlp_get_params(h, member, params, norm);
 }
 
 int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out) {

@@ -7,8 +7,11 @@
 
 #include "../../include/nqcd_mi355x.h"
 #include "fssh2.h"
+#include "mlp.h"
 
 namespace nqcd {
+
+constexpr int kMlpMembers = 2;
 
 struct DevBuf {
     void* p = nullptr;
@@ -59,6 +62,14 @@ struct nqcd_handle {
     int32_t rp_state = 0;
     // IESH / EhrenfestNA
     bool ie_psi_given = false;
+    // diatom-surface model parameters
+    double nd_params[16]{};
+    // MLP surrogate: two members, FP32 masters, packed BF16 copies, Adam moments, training data
+    bool mlp_ready = false;
+    double mlp_threshold = 0.0;
+    nqcd::MlpNorm mlp_norm{};
+    nqcd::DevBuf mlp_master, mlp_packed, mlp_adam_m, mlp_adam_v, mlp_x, mlp_t, mlp_idx, mlp_partial, mlp_losspart,
+        mlp_lossbuf, mlp_grad, fallbacks;
     int64_t ntraj = 0;
     bool have_initial = false, have_results = false, sb_initial = false;
     int32_t nsave = 1, save_every = 0;

@@ -1,0 +1,157 @@
+// mlp.h -- the MLP surrogate of a two-state diabatic Hamiltonian: 6 -> 128 -> 128 -> 3, tanh,
+// BF16 MFMA (16x16x32, FP32 accumulation) for the 128-wide products, FP32 first layer.
+//
+// Register layout of a hidden layer inside a wavefront: the 64 lanes evaluate 16 samples at a
+// time (a "sample tile"); lane l (group g = l / 16, column j = l % 16) holds units
+// u = 16 t + 4 g + r (t = 0..7, r = 0..3) of sample j.  That is the accumulator layout of
+// v_mfma_f32_16x16x32_bf16 with units along M and samples along N, and it is also, unchanged, the
+// B operand of the next product once the K axis of each 32-unit step q is ordered as
+//   k = 8 g + e  <->  unit 32 q + 4 g + e (e < 4),  32 q + 16 + 4 g + (e - 4) (e >= 4).
+// The packed weights store W2 and W3 with their columns in that order, so hidden activations never
+// leave registers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nqcd {
+
+constexpr int kMlpH = 128;
+constexpr int kMlpIn = 6;
+constexpr int kMlpOut = 3;
+constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpOut * kMlpH + kMlpOut;  // 17795
+// FP32 master parameter vector: W1 (row-major 128 x 6), b1, W2 (128 x 128), b2, W3 (3 x 128), b3.
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpH * kMlpIn, kOffW2 = kOffB1 + kMlpH;
+constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH, kOffW3 = kOffB2 + kMlpH, kOffB3 = kOffW3 + kMlpOut * kMlpH;
+
+// Packed member (bytes), the layout staged into LDS by the kernels:
+//   [0, 4096)      float w1b[128][8]: W1[u][0..5], b1[u], 0
+//   [4096, 4608)   float b2[128]
+//   [4608, 4672)   float b3[16] (3 used)
+//   [4672, 37440)  bf16 W2p[128][128], columns in the permuted K order
+//   [37440, 41536) bf16 W3p[16][128], rows 3..15 zero, columns permuted
+constexpr int kPkW1B = 0, kPkB2 = 4096, kPkB3 = 4608, kPkW2 = 4672, kPkW3 = 37440, kPkBytes = 41536;
+
+struct MlpNorm {
+    float in_shift[kMlpIn], in_scale[kMlpIn];
+    float out_shift[kMlpOut], out_scale[kMlpOut];
+};
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// unit index of position k = 8 g + e inside the 32-unit K step q
+__host__ __device__ constexpr int mlp_perm_unit(int q, int k) {
+    return 32 * q + ((k & 7) < 4 ? 4 * (k >> 3) + (k & 7) : 16 + 4 * (k >> 3) + (k & 7) - 4);
+}
+
+// Hides a pointer's value from the optimiser at this point, so loads through it are issued here
+// and not hoisted out of the surrounding loop (which would keep every weight fragment live).
+template <class T>
+__device__ __forceinline__ const T* mlp_launder(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+__device__ __forceinline__ float mlp_tanh(float a) { return 1.0f - 2.0f / (__expf(2.0f * a) + 1.0f); }
+
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 pack8(const float* lo, const float* hi) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) { v[e] = (__bf16)lo[e]; v[4 + e] = (__bf16)hi[e]; }
+    return v;
+}
+
+// One member on one sample tile.  xs: this lane's sample (column j) normalised inputs.
+// Returns, in lanes 0..15 (sample j), o[0..2] and (kGrad) the forward-mode input gradient J[o][i];
+// the passes (forward, then one tangent per input) run one after the other so that only one pass's
+// operands are live.  Optional outputs for training: h1, s1 = 1 - h1^2, h2, s2 of this lane's units.
+template <bool kGrad>
+__device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, const float xs[kMlpIn], float o[kMlpOut],
+                                         float J[kMlpOut][kMlpIn], float* h1_out = nullptr, float* s1_out = nullptr,
+                                         float* h2_out = nullptr, float* s2_out = nullptr) {
+    const int l = __lane_id(), g = l >> 4, j = l & 15;
+    const float* w1b = reinterpret_cast<const float*>(W + kPkW1B);
+    const float* b2 = reinterpret_cast<const float*>(W + kPkB2);
+    const float* b3 = reinterpret_cast<const float*>(W + kPkB3);
+    const bf16x8* W2p = reinterpret_cast<const bf16x8*>(W + kPkW2);
+    const bf16x8* W3p = reinterpret_cast<const bf16x8*>(W + kPkW3);
+    // layer 1 in FP32
+    float h1[32], s1[32];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float4 wa = *reinterpret_cast<const float4*>(w1b + 8 * (16 * t + 4 * g + r));
+            const float4 wb = *reinterpret_cast<const float4*>(w1b + 8 * (16 * t + 4 * g + r) + 4);
+            const float a = wb.z + wa.x * xs[0] + wa.y * xs[1] + wa.z * xs[2] + wa.w * xs[3] + wb.x * xs[4] + wb.y * xs[5];
+            const float h = mlp_tanh(a);
+            h1[4 * t + r] = h;
+            s1[4 * t + r] = 1.0f - h * h;
+        }
+    }
+    float s2[32];
+    const int npass = kGrad ? 1 + kMlpIn : 1;
+#pragma unroll 1
+    for (int p = 0; p < npass; ++p) {
+        const int tin = p - 1;  // tangent input (-1: the forward pass)
+        const bf16x8* W2q = mlp_launder(W2p);
+        const bf16x8* W3q = mlp_launder(W3p);
+        bf16x8 B[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float lo[4], hi[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (tin < 0) { lo[r] = h1[4 * (2 * q) + r]; hi[r] = h1[4 * (2 * q + 1) + r]; }
+                else {
+                    lo[r] = s1[4 * (2 * q) + r] * w1b[8 * (16 * (2 * q) + 4 * g + r) + tin];
+                    hi[r] = s1[4 * (2 * q + 1) + r] * w1b[8 * (16 * (2 * q + 1) + 4 * g + r) + tin];
+                }
+            }
+            B[q] = pack8(lo, hi);
+        }
+        f32x4 acc3 = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q3 = 0; q3 < 4; ++q3) {
+            float val[2][4];
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int ot = 2 * q3 + half;
+                f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kq = 0; kq < 4; ++kq) acc = mfma_bf16(W2q[(16 * ot + j) * 16 + 4 * kq + g], B[kq], acc);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int idx = 4 * ot + r;
+                    if (tin < 0) {
+                        const float h = mlp_tanh(acc[r] + b2[16 * ot + 4 * g + r]);
+                        s2[idx] = 1.0f - h * h;
+                        val[half][r] = h;
+                        if (h2_out) h2_out[idx] = h;
+                    } else {
+                        val[half][r] = s2[idx] * acc[r];
+                    }
+                }
+            }
+            acc3 = mfma_bf16(W3q[j * 16 + 4 * q3 + g], pack8(val[0], val[1]), acc3);
+        }
+#pragma unroll
+        for (int r = 0; r < kMlpOut; ++r) {
+            if (tin < 0) o[r] = acc3[r] + b3[r];
+            else if (kGrad) {
+#pragma unroll
+                for (int i = 0; i < kMlpIn; ++i) J[r][i] = (tin == i) ? acc3[r] : J[r][i];
+            }
+        }
+    }
+    if (h1_out) {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) { h1_out[k] = h1[k]; s1_out[k] = s1[k]; s2_out[k] = s2[k]; }
+    }
+}
+
+}  // namespace nqcd

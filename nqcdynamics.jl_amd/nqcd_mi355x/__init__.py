@@ -11,9 +11,9 @@ from . import _lib  # noqa: F401
 from .distributions import (Adiabatic, Diabatic, DynamicalDistribution, DynamicsVariables, Normal,  # noqa: F401
                             PositionHarmonicWigner, PureState, VelocityHarmonicWigner)
 from .ensembles import EnsembleMI355X, TerminatingCallback, run_dynamics  # noqa: F401
-from .models import (AndersonHolstein, DebyeSpectralDensity, DoubleWellGaussian, Harmonic, MiaoSubotnik,  # noqa: F401
-                     OhmicSpectralDensity, SpinBoson, TrapezoidalRule, TullyModelOne, TullyModelThree, TullyModelTwo,
-                     nelectrons, nstates)
+from .models import (AndersonHolstein, DebyeSpectralDensity, DiatomSurface, DoubleWellGaussian, Harmonic,  # noqa: F401
+                     MiaoSubotnik, MLPSurrogate, OhmicSpectralDensity, SpinBoson, TrapezoidalRule, TullyModelOne,
+                     TullyModelThree, TullyModelTwo, nelectrons, nstates, train_surrogate)
 from .outputs import (FileReduction, MeanReduction, OutputAdiabaticPopulation, OutputCentroidPosition,  # noqa: F401
                       OutputDiabaticPopulation,
                       OutputDiscreteState, OutputFinal, OutputFinalTime, OutputStateResolvedScattering1D,

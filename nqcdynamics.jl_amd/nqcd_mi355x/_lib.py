@@ -22,7 +22,9 @@ ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5 = range(4)
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = range(3)
 PREC_FP64, PREC_FP32_NUCLEI = range(2)
 SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
-MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC, MODEL_NEWNS_ANDERSON = range(7)
+(MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC, MODEL_NEWNS_ANDERSON,
+ MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE) = range(9)
+MLP_PARAMS = 128 * 6 + 128 + 128 * 128 + 128 + 3 * 128 + 3
 DECOHERENCE_NONE, DECOHERENCE_EDC = range(2)
 TERM_NONE, TERM_ABS_GT = range(2)
 SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLYMER_THERMAL = range(4)
@@ -30,12 +32,14 @@ SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLY
  OUT_CENTROID_POSITION) = range(6)
 RED_MEAN, RED_SUM = range(2)
 (FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
- FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID) = range(11)
+ FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID, FIELD_FALLBACKS, FIELD_SAVE_R) = range(13)
 
 EXPORTED_SYMBOLS = (
     "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
     "nqcd_mi355x_set_model", "nqcd_mi355x_load_initial", "nqcd_mi355x_sample_initial", "nqcd_mi355x_run",
     "nqcd_mi355x_reduce", "nqcd_mi355x_download", "nqcd_mi355x_last_run_stats", "nqcd_mi355x_last_error",
+    "nqcd_mi355x_surrogate_train", "nqcd_mi355x_surrogate_eval", "nqcd_mi355x_surrogate_gradient",
+    "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params",
 )
 
 
@@ -55,6 +59,11 @@ class TermDesc(C.Structure):
 class SamplerDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("initial_state", C.c_int32), ("state_basis", C.c_int32), ("n_models", C.c_int32),
                 ("ntraj", C.c_int64), ("params", C.POINTER(C.c_double)), ("nparams", C.c_int64)]
+
+
+class TrainDesc(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_double), ("lr_final", C.c_double),
+                ("loss_weights", C.c_double * 3), ("seed", C.c_uint64)]
 
 
 class RunStats(C.Structure):
@@ -95,6 +104,12 @@ def lib():
     L.nqcd_mi355x_reduce.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_download.argtypes = [H, C.c_int32, C.c_void_p, C.c_int64]
     L.nqcd_mi355x_last_run_stats.argtypes = [H, C.POINTER(RunStats)]
+    fp = C.POINTER(C.c_float)
+    L.nqcd_mi355x_surrogate_train.argtypes = [H, dp, dp, C.c_int64, C.POINTER(TrainDesc), dp]
+    L.nqcd_mi355x_surrogate_eval.argtypes = [H, C.c_int32, dp, C.c_int64, dp, dp]
+    L.nqcd_mi355x_surrogate_gradient.argtypes = [H, C.c_int32, dp, dp, C.c_int64, dp, dp, dp]
+    L.nqcd_mi355x_surrogate_set_params.argtypes = [H, C.c_int32, fp, fp]
+    L.nqcd_mi355x_surrogate_get_params.argtypes = [H, C.c_int32, fp, fp]
     L.nqcd_mi355x_last_error.argtypes = [H]
     L.nqcd_mi355x_last_error.restype = C.c_char_p
     for name in EXPORTED_SYMBOLS:
@@ -239,13 +254,53 @@ class Ensemble:
                   FIELD_SIGMA_RE: ((nt, n, n), np.float64), FIELD_SIGMA_IM: ((nt, n, n), np.float64),
                   FIELD_STATE: ((nt,), np.int32), FIELD_STEPS: ((nt,), np.int32), FIELD_NHOPS: ((nt,), np.int32),
                   FIELD_SAVE_POP_AD: ((nt, nsave, n), np.float64), FIELD_SAVE_POP_DI: ((nt, nsave, n), np.float64),
-                  FIELD_SAVE_STATE: ((nt, nsave), np.int32)}
+                  FIELD_SAVE_STATE: ((nt, nsave), np.int32), FIELD_FALLBACKS: ((nt,), np.int32),
+                  FIELD_SAVE_R: ((nt, nsave, D), np.float64)}
         shape, dtype = shapes[field]
         out = np.zeros(shape, dtype=dtype)
         self._check(self._L.nqcd_mi355x_download(self._h, field, out.ctypes.data_as(C.c_void_p), out.nbytes))
         if field in (FIELD_SIGMA_RE, FIELD_SIGMA_IM):
             out = out.transpose(0, 2, 1).copy()  # column-major (n, n) per trajectory -> row-major
         return out
+
+    # --- MLP surrogate ----------------------------------------------------------------------------
+    def surrogate_train(self, X, Y, steps=2000, batch=1024, lr=3e-3, lr_final=1e-4, loss_weights=(1.0, 1.0, 1.0),
+                        seed=0x4E5143444D493335):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        d = TrainDesc()
+        d.steps, d.batch, d.lr, d.lr_final, d.seed = steps, batch, lr, lr_final, seed
+        for i in range(3):
+            d.loss_weights[i] = loss_weights[i]
+        hist = np.zeros(2 * steps)
+        self._check(self._L.nqcd_mi355x_surrogate_train(self._h, _dp(X), _dp(Y), len(X), C.byref(d), _dp(hist)))
+        return hist.reshape(2, steps)
+
+    def surrogate_eval(self, member, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Hout = np.zeros((len(X), 3)); dH = np.zeros((len(X), 3, 6))
+        self._check(self._L.nqcd_mi355x_surrogate_eval(self._h, member, _dp(X), len(X), _dp(Hout), _dp(dH)))
+        return Hout, dH
+
+    def surrogate_gradient(self, member, X, Y, w=(1.0, 1.0, 1.0)):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        wv = np.ascontiguousarray(w, dtype=np.float64)
+        g = np.zeros(MLP_PARAMS); loss = np.zeros(1)
+        self._check(self._L.nqcd_mi355x_surrogate_gradient(self._h, member, _dp(X), _dp(Y), len(X), _dp(wv), _dp(g), _dp(loss)))
+        return g, float(loss[0])
+
+    def surrogate_set_params(self, member, params, norm=None):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nv = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        fp = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+        self._check(self._L.nqcd_mi355x_surrogate_set_params(self._h, member, fp(p), fp(nv)))
+
+    def surrogate_get_params(self, member):
+        p = np.zeros(MLP_PARAMS, np.float32); nv = np.zeros(18, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self._check(self._L.nqcd_mi355x_surrogate_get_params(self._h, member, fp(p), fp(nv)))
+        return p, nv
 
     def _download_iesh(self, field):
         n, ne, nt, nsave = self.n_states, self.n_electrons, self.ntraj, self.nsave

@@ -73,6 +73,9 @@ def _problem(E, sim, dt):
                       algorithm=L.ALG_BCB if code == L.METHOD_RPMD else L.ALG_BCB_TSIT5, nbeads=sim.nbeads,
                       temperature=sim.temperature, **common)
     E.set_model(0, sim.model.kind, sim.model.params())
+    if sim.model.kind == L.MODEL_MLP_SURROGATE:
+        for k, p in enumerate(sim.model.members):
+            E.surrogate_set_params(k, p, sim.model.norm)
 
 
 def _electronic(dist):

@@ -169,3 +169,72 @@ class AndersonHolstein(Model):
 
 def nelectrons(model):
     return model.nelectrons
+
+
+class DiatomSurface(Model):
+    """Diatomic molecule above a surface, six coordinates (ndofs = 3, natoms = 2), two diabats:
+    bond b = |R1 - R2|, height Z = mu z1 + (1 - mu) z2,
+      V11 = Dn (1 - e^{-an (b - bn)})^2 + Aw e^{-bw Z}              (neutral molecule, repulsive wall)
+      V22 = Di (1 - e^{-ai (b - bi)})^2 + Ds (1 - e^{-as (Z - Zs)})^2 + Delta   (ion bound to the surface)
+      V12 = C e^{-g (Z - Zc)^2}.
+    `seed` jitters every parameter but mu by up to 5% (a reproducible synthetic model family)."""
+    kind = L.MODEL_DIATOM_SURFACE
+    ndofs = 3
+    natoms = 2
+    BASE = (0.24, 1.1, 2.2, 2.0, 1.2, 0.20, 1.0, 2.35, 0.12, 0.8, 3.0, 0.0, 0.003, 1.0, 3.75, 0.5)
+    NAMES = ("Dn", "an", "bn", "Aw", "bw", "Di", "ai", "bi", "Ds", "as", "Zs", "Delta", "C", "g", "Zc", "mu")
+
+    def __init__(self, params=None, seed=None):
+        p = np.array(self.BASE if params is None else params, dtype=np.float64)
+        if seed is not None:
+            jit = 1.0 + 0.05 * (2.0 * np.random.default_rng(seed).random(len(p)) - 1.0)
+            jit[-1] = 1.0
+            p = p * jit
+        self.p = p
+
+    def params(self):
+        return [float(x) for x in self.p]
+
+    def potential(self, X):
+        """(n, 6) coordinates -> (n, 3) diabatic elements (V11, V22, V12)."""
+        X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+        Dn, an, bn, Aw, bw, Di, ai, bi, Ds, a_s, Zs, Delta, C, g, Zc, mu = self.p
+        b = np.linalg.norm(X[:, :3] - X[:, 3:], axis=1)
+        Z = mu * X[:, 2] + (1 - mu) * X[:, 5]
+        morse = lambda D, a, q0, x: D * (1 - np.exp(-a * (x - q0))) ** 2
+        v11 = morse(Dn, an, bn, b) + Aw * np.exp(-bw * Z)
+        v22 = morse(Di, ai, bi, b) + morse(Ds, a_s, Zs, Z) + Delta
+        v12 = C * np.exp(-g * (Z - Zc) ** 2)
+        return np.stack([v11, v22, v12], axis=1)
+
+
+class MLPSurrogate(Model):
+    """The MLP surrogate (6 -> 128 -> 128 -> 3, two ensemble members) standing in for the diabatic
+    Hamiltonian of `analytic`; wherever the members disagree by more than `threshold` (Hartree) on
+    an element of H the analytic model is used for that trajectory step."""
+    kind = L.MODEL_MLP_SURROGATE
+    ndofs = 3
+    natoms = 2
+
+    def __init__(self, analytic, members, norm, threshold=2e-3):
+        self.analytic, self.members, self.norm, self.threshold = analytic, [np.asarray(m, np.float32) for m in members], \
+            np.asarray(norm, np.float32), float(threshold)
+
+    def params(self):
+        return [self.threshold] + self.analytic.params()
+
+
+def train_surrogate(analytic, X, Y=None, threshold=2e-3, device=0, **train_kw):
+    """Train the two-member surrogate of `analytic` on configurations X (n, 6) (labels Y = the
+    analytic diabatic elements unless given) with the device's BF16 training kernels."""
+    Y = analytic.potential(X) if Y is None else Y
+    E = L.Ensemble(device)
+    try:
+        hist = E.surrogate_train(X, Y, **train_kw)
+        members, norm = [], None
+        for k in range(2):
+            p, norm = E.surrogate_get_params(k)
+            members.append(p)
+    finally:
+        E.close()
+    return MLPSurrogate(analytic, members, norm, threshold), hist

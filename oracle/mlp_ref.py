@@ -1,0 +1,84 @@
+"""CPU reference of the MLP surrogate of the diabatic Hamiltonian (test infrastructure only).
+
+Network 6 -> 128 -> 128 -> 3 with tanh activations on normalised inputs:
+    xn = (x - in_shift) * in_scale
+    a1 = W1 xn + b1,  h1 = tanh(a1)
+    a2 = W2 h1 + b2,  h2 = tanh(a2)
+    o  = W3 h2 + b3
+    (H11, H22, H12) = o * out_scale + out_shift
+Precision points of the device kernels, reproduced when `bf16=True`: the first layer in FP32,
+h1 and h2 and the W2, W3 weights rounded to BF16 before their products (FP32 accumulation), the
+tangents of the forward-mode input gradient rounded to BF16 where they enter the W2 / W3 products.
+Loss: mean over samples of sum_o w_o (o - t_o)^2 on normalised targets t = (y - out_shift) / out_scale.
+"""
+import numpy as np
+
+H = 128
+
+
+def bf16(a):
+    """Round-to-nearest-even to bfloat16, returned as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    rounding = ((u >> 16) & 1) + 0x7FFF
+    u = ((u + rounding) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def init(seed):
+    rng = np.random.default_rng(seed)
+    P = {"W1": rng.normal(0, 1 / np.sqrt(6), (H, 6)), "b1": np.zeros(H),
+         "W2": rng.normal(0, 1 / np.sqrt(H), (H, H)), "b2": np.zeros(H),
+         "W3": rng.normal(0, 1 / np.sqrt(H), (3, H)), "b3": np.zeros(3)}
+    return {k: v.astype(np.float32) for k, v in P.items()}
+
+
+def forward(P, xn, bf=False):
+    """xn: (n, 6) normalised inputs -> dict with a1, h1, a2, h2, o (all (n, .))."""
+    xn = np.asarray(xn, dtype=np.float64)
+    W2 = bf16(P["W2"]).astype(np.float64) if bf else P["W2"].astype(np.float64)
+    W3 = bf16(P["W3"]).astype(np.float64) if bf else P["W3"].astype(np.float64)
+    a1 = xn @ P["W1"].astype(np.float64).T + P["b1"]
+    h1 = np.tanh(a1)
+    h1q = bf16(h1).astype(np.float64) if bf else h1
+    a2 = h1q @ W2.T + P["b2"]
+    h2 = np.tanh(a2)
+    h2q = bf16(h2).astype(np.float64) if bf else h2
+    o = h2q @ W3.T + P["b3"]
+    return {"a1": a1, "h1": h1, "h1q": h1q, "a2": a2, "h2": h2, "h2q": h2q, "o": o, "W2": W2, "W3": W3}
+
+
+def jacobian(P, xn, bf=False):
+    """d o / d xn, shape (n, 3, 6), by forward-mode tangents (the device's formulation)."""
+    f = forward(P, xn, bf)
+    s1 = 1.0 - f["h1"] ** 2
+    s2 = 1.0 - f["h2"] ** 2
+    W1 = P["W1"].astype(np.float64)
+    J = np.zeros((len(f["o"]), 3, 6))
+    for i in range(6):
+        t1 = s1 * W1[:, i]
+        if bf:
+            t1 = bf16(t1).astype(np.float64)
+        t2 = s2 * (t1 @ f["W2"].T)
+        if bf:
+            t2 = bf16(t2).astype(np.float64)
+        J[:, :, i] = t2 @ f["W3"].T
+    return J
+
+
+def loss_and_grads(P, xn, t, w=(1.0, 1.0, 1.0)):
+    """FP64 loss and exact parameter gradients (no BF16 rounding)."""
+    w = np.asarray(w, dtype=np.float64)
+    n = len(xn)
+    f = forward(P, xn)
+    r = f["o"] - t
+    loss = np.mean(np.sum(w * r * r, axis=1))
+    d3 = 2.0 * w * r / n
+    g = {"W3": d3.T @ f["h2"], "b3": d3.sum(0)}
+    d2 = (d3 @ P["W3"].astype(np.float64)) * (1 - f["h2"] ** 2)
+    g["W2"] = d2.T @ f["h1"]
+    g["b2"] = d2.sum(0)
+    d1 = (d2 @ P["W2"].astype(np.float64)) * (1 - f["h1"] ** 2)
+    g["W1"] = d1.T @ np.asarray(xn, dtype=np.float64)
+    g["b1"] = d1.sum(0)
+    return loss, g
