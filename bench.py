@@ -83,6 +83,94 @@ def _progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def iesh_extra(L):
+    """IESH on the Newns-Anderson model (MiaoSubotnik, W = 3 Gamma, half filling) at N = 41 and 101
+    states: trajectory-steps/s of the one-trajectory-per-workgroup kernel and its FP64 rate."""
+    res = {}
+    W = 6 * 6.4e-3 / 2
+    for M, ntraj, nsteps in ((40, 16384, 200), (100, 4096, 100)):
+        E = L.Ensemble(0)
+        E.set_problem(L.METHOD_IESH, M + 1, 1, 1, [2000.0], 2.0, algorithm=L.ALG_VERLET_ELECTRONICS, n_electrons=M // 2)
+        E.set_model(0, L.MODEL_NEWNS_ANDERSON, [2000.0, 2e-4, 20.6097, -3.8e-3, 6.4e-3, -W, W, M])
+        E.sample_initial(L.SAMPLER_GAUSSIAN, ntraj, [8.0, 1.0, 0.0, 1e-2], KEY)
+        E.run(0.0, 2.0 * 10, KEY)
+        st = E.run(0.0, 2.0 * nsteps, KEY, save_every=10)
+        E.close()
+        N, Ne = M + 1, M // 2
+        flop = 8 * 4 * N * N * Ne + 16 * N * M + (8 / 3) * Ne ** 3  # Taylor (K ~ 8) + eigen + Gauss-Jordan
+        rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+        res[f"N{N}"] = {"ntraj": ntraj, "nsteps": nsteps, "kernel_ms": st["kernel_ms"], "traj_steps_per_s": rate,
+                        "flop_per_traj_step": flop, "achieved_TFLOPs": flop * rate / 1e12,
+                        "frac_fp64_peak": flop * rate / 1e12 / FP64_PEAK_TFLOPS, "grid": st["grid"], "block": st["block"]}
+        _progress(f"iesh N={N}: {rate:.3e} trajectory-steps/s")
+    return res
+
+
+def surrogate_extra(L):
+    """Learned form on the six-coordinate diatom-surface model: train the two-member BF16 surrogate
+    on configurations the analytic FSSH visits, then FSSH on the surrogate vs on the analytic model
+    (persistent kernel) and the final-population error."""
+    import nqcd_mi355x as NQ
+    analytic = NQ.DiatomSurface()
+    mass = [12859.0, 12859.0]
+
+    def inputs(ntraj, seed):
+        rng = np.random.default_rng(seed)
+        u = rng.normal(size=(ntraj, 3)); u /= np.linalg.norm(u, axis=1, keepdims=True)
+        b = 2.2 + 0.1 * rng.normal(size=(ntraj, 1))
+        c = np.concatenate([rng.normal(size=(ntraj, 2)), np.full((ntraj, 1), 6.0)], axis=1)
+        r0 = np.concatenate([c + 0.5 * b * u, c - 0.5 * b * u], axis=1)
+        v0 = 2e-4 * rng.normal(size=(ntraj, 6)); v0[:, 2] -= 3e-3; v0[:, 5] -= 3e-3
+        return r0, v0
+
+    def run(model, r0, v0, save_every=0):
+        E = L.Ensemble(0)
+        E.set_problem(L.METHOD_FSSH, 2, 3, 2, mass, 2.0, precision=L.PREC_FP32_NUCLEI)
+        E.set_model(0, model.kind, model.params())
+        if model.kind == L.MODEL_MLP_SURROGATE:
+            for k, p in enumerate(model.members):
+                E.surrogate_set_params(k, p, model.norm)
+        sig = np.zeros((len(r0), 2, 2), complex); sig[:, 0, 0] = 1
+        E.load_initial(r0, v0, sig, state=np.zeros(len(r0), np.int32))
+        st = E.run(0.0, 2.0 * 5000, KEY, save_every=save_every, term=(2, 8.0))
+        out = {"stats": st, "final": E.reduce(L.OUT_FINAL_ADIABATIC_POPULATION, L.RED_MEAN, 2)}
+        if save_every:
+            out["configs"], out["steps"] = E.download(L.FIELD_SAVE_R), E.download(L.FIELD_STEPS)
+        if model.kind == L.MODEL_MLP_SURROGATE:
+            out["fallbacks"] = int(E.download(L.FIELD_FALLBACKS).sum())
+        E.close()
+        return out
+
+    r0, v0 = inputs(4000, 41)
+    vis = run(analytic, r0, v0, save_every=20)
+    keep = np.arange(vis["configs"].shape[1])[None, :] * 20 <= vis["steps"][:, None]
+    X = vis["configs"][keep]
+    t0 = time.perf_counter()
+    model, hist = NQ.train_surrogate(analytic, X, steps=3000, batch=1024, lr=3e-3, lr_final=5e-5, threshold=2e-3)
+    t_train = time.perf_counter() - t0
+    _progress(f"surrogate trained on {len(X)} configurations in {t_train:.1f} s")
+    r1, v1 = inputs(100000, 43)
+    ref = run(analytic, r1, v1)
+    sur = run(model, r1, v1)
+    rate = lambda o: o["stats"]["traj_steps"] / (o["stats"]["kernel_ms"] / 1e3)
+    p_ref, p_sur = float(ref["final"][1]), float(sur["final"][1])
+    _progress(f"surrogate FSSH {rate(sur):.3e} / analytic {rate(ref):.3e} trajectory-steps/s")
+    # BF16 MFMA work of one trajectory-step: two members x (forward + 6 tangents) x 128x128 product
+    # + the 128x(32 padded) output products
+    flop = 2 * 7 * (2 * 128 * 128 + 2 * 16 * 128)
+    return {"workload": "diatom-surface FSSH, 6 coordinates, 1e5 trajectories, dt 2, t_max 1e4, terminate z1 > 8",
+            "training_configurations": int(len(X)), "training_steps": 3000, "training_s": t_train,
+            "final_loss": [float(hist[0, -100:].mean()), float(hist[1, -100:].mean())],
+            "surrogate_traj_steps_per_s": rate(sur), "analytic_traj_steps_per_s": rate(ref),
+            "surrogate_kernel_ms": sur["stats"]["kernel_ms"], "analytic_kernel_ms": ref["stats"]["kernel_ms"],
+            "final_excited_population": {"analytic": p_ref, "surrogate": p_sur},
+            "pop_error_vs_analytic": abs(p_sur - p_ref),
+            "pop_sigma": float(np.sqrt(p_ref * (1 - p_ref) / len(r1))),
+            "fallback_fraction": sur["fallbacks"] / sur["stats"]["traj_steps"],
+            "bf16_flop_per_traj_step": flop,
+            "achieved_bf16_TFLOPs": flop * rate(sur) / 1e12, "frac_bf16_peak": flop * rate(sur) / 1e12 / 2500.0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +178,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-port comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true", help="skip the IESH and surrogate measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -173,6 +262,9 @@ def main():
         "plain_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
     }
+    if not args.no_extras:
+        out["iesh"] = iesh_extra(L)
+        out["learned"] = surrogate_extra(L)
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         _progress(f"cpu baseline on {nthreads} threads")
