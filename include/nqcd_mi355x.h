@@ -287,6 +287,13 @@ int32_t nqcd_mi355x_surrogate_gradient(nqcd_handle_t* h, int32_t member, const d
 int32_t nqcd_mi355x_surrogate_set_params(nqcd_handle_t* h, int32_t member, const float* params, const float* norm);
 int32_t nqcd_mi355x_surrogate_get_params(nqcd_handle_t* h, int32_t member, float* params, float* norm);
 
+/* Test hook (no counterpart in the reference): determinants of nmat complex n x n matrices (re, im
+ * row-major, matrix after matrix, n <= 64) by the device routine behind the IESH hopping estimate
+ * (the det S of FastDeterminant.jl:20-22 as used at iesh.jl:242-253); out[3 k .. 3 k + 2] = Re det,
+ * Im det, 1 (0 when a pivot column is zero). */
+int32_t nqcd_mi355x_test_determinant(nqcd_handle_t* h, const double* re, const double* im, int32_t n, int32_t nmat,
+                                     double* out);
+
 /* Message for the last failed call on h (thread-local when h is NULL). */
 const char* nqcd_mi355x_last_error(nqcd_handle_t* h);
 

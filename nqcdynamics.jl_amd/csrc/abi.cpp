@@ -676,6 +676,12 @@ int32_t nqcd_mi355x_surrogate_get_params(nqcd_handle_t* h, int32_t member, float
     return mlp_get_params(h, member, params, norm);
 }
 
+int32_t nqcd_mi355x_test_determinant(nqcd_handle_t* h, const double* re, const double* im, int32_t n, int32_t nmat,
+                                     double* out) {
+    if (!h || !re || !im || !out) return fail(h, NQCD_ERR_ARG, "null argument");
+    return ie_test_det(h, re, im, n, nmat, out);
+}
+
 int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out) {
     if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
     if (!h->have_results) return fail(h, NQCD_ERR_STATE, "no run yet");

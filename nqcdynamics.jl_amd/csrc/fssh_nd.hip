@@ -93,9 +93,14 @@ __device__ __forceinline__ void surrogate_eval(const FsshNdArgs& A, const unsign
                                                float& v11, float& v22, float& v12, float d11[kNdDof], float d22[kNdDof],
                                                float d12[kNdDof], bool& far) {
     const int l = __lane_id(), j = l & 15, grp = l >> 4;
-    float xn[kMlpIn];
+    // idle lanes (all coordinates zero) evaluate a harmless configuration instead
+    const float bb = (r[0] - r[3]) * (r[0] - r[3]) + (r[1] - r[4]) * (r[1] - r[4]) + (r[2] - r[5]) * (r[2] - r[5]);
+    float x[kMlpIn], xn[kMlpIn];
 #pragma unroll
-    for (int i = 0; i < kMlpIn; ++i) xn[i] = (r[i] - A.nrm.in_shift[i]) * A.nrm.in_scale[i];
+    for (int i = 0; i < kMlpIn; ++i) x[i] = (bb > 0.0f) ? r[i] : (i == 0 ? 1.0f : 0.0f);
+    mlp_descriptor(x, xn);
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) xn[i] = (xn[i] - A.nrm.in_shift[i]) * A.nrm.in_scale[i];
     float H[2][kMlpOut], J[2][kMlpOut][kMlpIn];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
@@ -130,12 +135,16 @@ __device__ __forceinline__ void surrogate_eval(const FsshNdArgs& A, const unsign
     }
     far = !(dmax <= A.threshold);
     v11 = Hm[0]; v22 = Hm[1]; v12 = Hm[2];
+    float g11[kMlpIn], g22[kMlpIn], g12[kMlpIn];
 #pragma unroll
     for (int i = 0; i < kMlpIn; ++i) {
-        d11[i] = 0.5f * (J[0][0][i] + J[1][0][i]) * A.nrm.out_scale[0] * A.nrm.in_scale[i];
-        d22[i] = 0.5f * (J[0][1][i] + J[1][1][i]) * A.nrm.out_scale[1] * A.nrm.in_scale[i];
-        d12[i] = 0.5f * (J[0][2][i] + J[1][2][i]) * A.nrm.out_scale[2] * A.nrm.in_scale[i];
+        g11[i] = 0.5f * (J[0][0][i] + J[1][0][i]) * A.nrm.out_scale[0] * A.nrm.in_scale[i];
+        g22[i] = 0.5f * (J[0][1][i] + J[1][1][i]) * A.nrm.out_scale[1] * A.nrm.in_scale[i];
+        g12[i] = 0.5f * (J[0][2][i] + J[1][2][i]) * A.nrm.out_scale[2] * A.nrm.in_scale[i];
     }
+    mlp_descriptor_pullback(x, g11, d11);
+    mlp_descriptor_pullback(x, g22, d22);
+    mlp_descriptor_pullback(x, g12, d12);
 }
 
 template <typename R>

@@ -11,5 +11,6 @@ int32_t ie_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const 
 int32_t ie_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S, uint64_t seed);
 int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc* term, uint64_t key);
 int32_t ie_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes);
+int32_t ie_test_det(nqcd_handle_t* h, const double* re, const double* im, int32_t Ne, int32_t nmat, double* out);
 
 }  // namespace nqcd

@@ -11,7 +11,8 @@
 // eps_k = a + k (b - a)/(M - 1), V = sqrt(Gamma (b - a)/(M - 1) / 2 pi).  The Hamiltonian is an
 // arrowhead matrix, so per step:
 //   * eigen: root i of the secular function f(l) = l - h - V^2 sum_k 1/(l - eps_k) lies in
-//     (eps_{i-1}, eps_i); safeguarded Newton from the previous step's root, one thread per root.
+//     (eps_{i-1}, eps_i); rational-model iteration from the previous step's root, one thread per
+//     root (3 to 5 evaluations of f per root and step).
 //     The impurity component of eigenvector i is u_i = f'(E_i)^(-1/2) > 0 (a continuous gauge).
 //   * dV/dx only touches the impurity entry, so U^T dV U = dh u u^T and the nonadiabatic coupling
 //     is d_ij = dh u_i u_j / (E_j - E_i).
@@ -114,33 +115,172 @@ __device__ __forceinline__ double rcp_nr(double x) {
     return fma(r, e, r);
 }
 
-// Root of the secular function f(l) = l - h - V^2 sum_k 1/(l - eps_k) in (lo, hi), where f
-// increases from -inf to +inf: safeguarded Newton, stopping when f is at its rounding level or the
-// step is an ulp.  Returns E_i and f'(E_i) = 1 + V^2 sum_k 1/(E_i - eps_k)^2.
-__device__ double secular_root(const double* eps, int M, double V2, double h, double lo, double hi, double guess,
-                               double tol, double& fp) {
+// Root i of the secular function f(l) = l - h - V^2 sum_k 1/(l - eps_k), which increases from
+// -inf to +inf on its interval (lo, hi) = (eps_{i-1}, eps_i) (the outer roots have one finite end).
+// Each iteration evaluates f, f' once and steps to the root of a two-pole rational model that
+// matches f and f' at the current point ("middle way", R.-C. Li, LAPACK Working Note 89):
+//   F(y) = c + bl / (y - eps_{i-1}) + bh / (y - eps_i),
+// where the poles below and above the interval carry their own parts of f' and the slope of the
+// linear term is shared between them; the model's root is a quadratic's, and a step that leaves
+// the bracket (kept from the signs of f) bisects instead.  Stops when |f| is at its rounding level,
+// when the step is a few ulps or when the bracket is narrower than tol.  Returns E_i and
+// f'(E_i) = 1 + V^2 sum_k 1/(E_i - eps_k)^2.
+__device__ double secular_root(const double* eps, int M, int i, double V2, double h, double lo, double hi,
+                               double guess, double tol, double& fp) {
+    const bool has_lo = i > 0, has_hi = i < M;
+    const double dl = has_lo ? eps[i - 1] : lo, dh = has_hi ? eps[i] : hi;
+    const double wl = has_lo ? (has_hi ? 0.5 : 1.0) : 0.0;  // share of the linear term's slope at the lower pole
     double lam = (guess > lo && guess < hi) ? guess : 0.5 * (lo + hi);
     fp = 1.0;
     for (int it = 0; it < 100; ++it) {
-        double s = 0.0, s2 = 0.0, sa = 0.0;
+        // q_k > 0 below the interval, < 0 above it
+        double sl = 0.0, sl2 = 0.0, sh = 0.0, sh2 = 0.0;
         for (int k = 0; k < M; ++k) {
             const double q = rcp_nr(lam - eps[k]);
-            s += q;
-            s2 = fma(q, q, s2);
-            sa += fabs(q);
+            const double q2 = q * q;
+            const bool below = k < i;
+            sl += below ? q : 0.0;
+            sh += below ? 0.0 : q;
+            sl2 += below ? q2 : 0.0;
+            sh2 += below ? 0.0 : q2;
         }
-        const double f = lam - h - V2 * s;
-        fp = 1.0 + V2 * s2;
-        if (fabs(f) <= 8.0 * __DBL_EPSILON__ * (fabs(lam) + fabs(h) + V2 * sa)) break;
+        const double f = lam - h - V2 * (sl + sh);
+        fp = 1.0 + V2 * (sl2 + sh2);
+        if (fabs(f) <= 8.0 * __DBL_EPSILON__ * (fabs(lam) + fabs(h) + V2 * (sl - sh))) break;
         if (f > 0.0) hi = lam;
         else lo = lam;
-        double nl = lam - f / fp;
+        const double bl = has_lo ? -(V2 * sl2 + wl) * (lam - dl) * (lam - dl) : 0.0;
+        const double bh = has_hi ? -(V2 * sh2 + (1.0 - wl)) * (lam - dh) * (lam - dh) : 0.0;
+        const double c = f - (has_lo ? bl / (lam - dl) : 0.0) - (has_hi ? bh / (lam - dh) : 0.0);
+        double nl;
+        if (has_lo && has_hi) {
+            // c t (t - D) + bl (t - D) + bh t = 0 with t = y - dl, D = dh - dl; root in (0, D)
+            const double D = dh - dl;
+            const double a1 = bl + bh - c * D, a0 = -bl * D;
+            const double sq = sqrt(fmax(a1 * a1 - 4.0 * c * a0, 0.0));
+            const double den = a1 > 0.0 ? -a1 - sq : -a1 + sq;
+            nl = dl + den / (2.0 * c);
+            if (!(nl >= lo && nl <= hi)) nl = dl + 2.0 * a0 / den;
+        } else if (has_hi) {
+            nl = dh - bh / c;
+        } else {
+            nl = dl - bl / c;
+        }
+        if (fabs(nl - lam) <= 4.0 * __DBL_EPSILON__ * fabs(lam) || (hi - lo) <= tol) break;
         if (!(nl > lo && nl < hi)) nl = 0.5 * (lo + hi);
-        const bool done = fabs(nl - lam) <= 2.0 * __DBL_EPSILON__ * fabs(nl) || (hi - lo) <= tol;
         lam = nl;
-        if (done) break;
     }
     return lam;
+}
+
+// Orders this wavefront's LDS accesses: its lanes exchange data through LDS without a workgroup
+// barrier (a wavefront's LDS instructions execute in program order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One step of a wavefront arg-max over (mag, idx) with the DPP lane pattern CTRL (larger mag
+// wins, the lower idx on ties).
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double& mag, int& idx) {
+    const long long bits = __double_as_longlong(mag);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(0, idx, CTRL, 0xF, 0xF, false);
+    const double om = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    if (om > mag || (om == mag && oi < idx)) { mag = om; idx = oi; }
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Wavefront-uniform arg-max of (mag, idx), the lowest idx among equal maxima: butterflies inside
+// each row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror), then the
+// four row results compared in lane order.
+__device__ __forceinline__ void wave_argmax(double& mag, int& idx) {
+    argmax_dpp<0xB1>(mag, idx);
+    argmax_dpp<0x4E>(mag, idx);
+    argmax_dpp<0x141>(mag, idx);
+    argmax_dpp<0x140>(mag, idx);
+    double bm = readlane_f64(mag, 0);
+    int bi = __builtin_amdgcn_readlane(idx, 0);
+#pragma unroll
+    for (int row = 1; row < 4; ++row) {
+        const double m = readlane_f64(mag, 16 * row);
+        const int i = __builtin_amdgcn_readlane(idx, 16 * row);
+        if (m > bm || (m == bm && i < bi)) { bm = m; bi = i; }
+    }
+    mag = bm;
+    idx = bi;
+}
+
+// det of the Ne x Ne complex matrix held as real / imaginary planes with row stride LD, by LU with
+// partial pivoting (largest |.| in the column, lowest row on ties) on one wavefront: lane r holds
+// row r, rows are eliminated in place and never moved, and the sign of the pivot permutation
+// comes from counting, at each step, the remaining rows above the pivot row.  Each lane keeps the
+// current column of its row in registers; the rest of the row is updated through LDS in chunks of
+// eight columns whose loads are all issued before its stores.  The matrix is overwritten.
+// Returns false for a singular matrix (a zero pivot column).  Called by all 64 lanes.
+__device__ bool wave_det(double* Sre, double* Sim, int LD, int Ne, double& dre, double& dim) {
+    constexpr int kChunk = 8;
+    const int r = __lane_id();
+    bool used = r >= Ne;
+    double cr = used ? 0.0 : Sre[r * LD], ci = used ? 0.0 : Sim[r * LD];  // S[r][p]
+    double ar = 1.0, ai = 0.0;
+    int inv = 0;
+    for (int p = 0; p < Ne; ++p) {
+        double mag = used ? -1.0 : cr * cr + ci * ci;
+        int idx = r;
+        wave_argmax(mag, idx);
+        if (!(mag > 0.0)) { dre = 0.0; dim = 0.0; return false; }
+        const int q = idx;
+        inv += __popcll(__ballot(!used && r < q));
+        const double pr = readlane_f64(cr, q), pi = readlane_f64(ci, q);
+        {
+            const double nr = ar * pr - ai * pi, ni = ar * pi + ai * pr;
+            ar = nr;
+            ai = ni;
+        }
+        if (r == q) used = true;
+        wave_lds_sync();  // the pivot row's stores of the previous step are visible
+        if (!used) {
+            const double ip = 1.0 / mag;
+            const double fr = (cr * pr + ci * pi) * ip, fi = (ci * pr - cr * pi) * ip;  // S[r][p] / pivot
+            for (int c0 = p + 1; c0 < Ne; c0 += kChunk) {
+                double yr[kChunk], yi[kChunk], xr[kChunk], xi[kChunk];
+#pragma unroll
+                for (int k = 0; k < kChunk; ++k) {
+                    const int c = c0 + k < Ne ? c0 + k : Ne - 1;
+                    yr[k] = Sre[q * LD + c];
+                    yi[k] = Sim[q * LD + c];
+                    xr[k] = Sre[r * LD + c];
+                    xi[k] = Sim[r * LD + c];
+                }
+#pragma unroll
+                for (int k = 0; k < kChunk; ++k) {
+                    xr[k] -= fr * yr[k] - fi * yi[k];
+                    xi[k] -= fr * yi[k] + fi * yr[k];
+                }
+                if (c0 == p + 1) { cr = xr[0]; ci = xi[0]; }
+#pragma unroll
+                for (int k = 0; k < kChunk; ++k) {
+                    if (c0 + k < Ne) {
+                        Sre[r * LD + c0 + k] = xr[k];
+                        Sim[r * LD + c0 + k] = xi[k];
+                    }
+                }
+            }
+        }
+        wave_lds_sync();
+    }
+    dre = (inv & 1) ? -ar : ar;
+    dim = (inv & 1) ? -ai : ai;
+    return true;
 }
 
 template <int NT, int CTM>
@@ -201,7 +341,7 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
                 const double hi = (tid == M) ? fmax(s.h, eps[M - 1]) + zn + 1e-3 * (zn + 1e-300) : eps[tid];
                 const double scale = fmax(fmax(fabs(A.a), fabs(eps[M - 1])), fabs(s.h)) + zn;
                 double fp;
-                const double lam = secular_root(eps, M, V2, s.h, lo, hi, warm ? Eprev : lo - 1.0, 4.0e-16 * scale, fp);
+                const double lam = secular_root(eps, M, tid, V2, s.h, lo, hi, warm ? Eprev : lo - 1.0, 4.0e-16 * scale, fp);
                 E[tid] = lam;
                 U[tid] = 1.0 / sqrt(fp);
                 Eprev = lam;
@@ -393,110 +533,129 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
             // ---- IESH hop (iesh.jl:390-407) -------------------------------------------------------------
             if (A.method == 0) {
                 const double rnd = philox_uniform((uint64_t)traj, (uint32_t)step, STREAM_HOP, A.key);
-                for (int idx = tid; idx < Ne * Ne; idx += B) {  // S_je = psi[occ_j, e] (iesh.jl:309-316)
+                // S_je = psi[occ_j, e] (iesh.jl:309-316) as real / imaginary planes, row stride LD (odd)
+                const int LD = Ne | 1, Nu = N - Ne;
+                double* Sre = Sb;
+                double* Sim = Sb + Ne * LD;
+                for (int idx = tid; idx < Ne * Ne; idx += B) {
                     const int j = idx / Ne, e = idx - j * Ne;
-                    Sb[2 * idx] = Tb[occ[j] * CW + 2 * e];
-                    Sb[2 * idx + 1] = Tb[occ[j] * CW + 2 * e + 1];
+                    Sre[j * LD + e] = Tb[occ[j] * CW + 2 * e];
+                    Sim[j * LD + e] = Tb[occ[j] * CW + 2 * e + 1];
                 }
                 if (tid == 0) {  // unoccupied states in ascending order (DynamicsUtils.jl:162-171)
                     int c = 0;
                     for (int m = 0; m < N; ++m)
                         if (isocc[m] < 0) unocc[c++] = m;
                 }
-                // Gauss-Jordan inversion of S in place with partial pivoting; det S is the signed
-                // product of the pivots, the same pivots an LU factorisation takes (FastDeterminant.jl:20-22).
-                double dre = 1.0, dim = 0.0;
-                bool singular = false;
-                for (int p = 0; p < Ne; ++p) {
-                    __syncthreads();
-                    if (w == 0) {
-                        double val = -1.0;
-                        int idx = l;
-                        if (l >= p && l < Ne) val = hypot(Sb[2 * (l * Ne + p)], Sb[2 * (l * Ne + p) + 1]);
-#pragma unroll
-                        for (int o = 32; o > 0; o >>= 1) {
-                            const double ov = __shfl_xor(val, o);
-                            const int oi = __shfl_xor(idx, o);
-                            if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
-                        }
-                        if (l == 0) {
-                            piv[p] = (val > 0.0) ? idx : -1;
-                            scal[1] = Sb[2 * (idx * Ne + p)];
-                            scal[2] = Sb[2 * (idx * Ne + p) + 1];
-                            scal[3] = Sb[2 * (p * Ne + p)];  // moves to row idx with the interchange
-                            scal[4] = Sb[2 * (p * Ne + p) + 1];
-                        }
-                    }
-                    __syncthreads();
-                    const int q = piv[p];
-                    if (q < 0) { singular = true; break; }
-                    const double pr = scal[1], pi = scal[2];
-                    {
-                        const double nre = dre * pr - dim * pi, nim = dre * pi + dim * pr;
-                        dre = (q != p) ? -nre : nre;
-                        dim = (q != p) ? -nim : nim;
-                    }
-                    const double den = pr * pr + pi * pi;
-                    const double ir = pr / den, ii = -pi / den;  // 1 / pivot
-                    // column p after the row swap, for the elimination
-                    for (int r = tid; r < Ne; r += B) {
-                        if (r == p) continue;
-                        pcol[2 * r] = (r == q) ? scal[3] : Sb[2 * (r * Ne + p)];
-                        pcol[2 * r + 1] = (r == q) ? scal[4] : Sb[2 * (r * Ne + p) + 1];
-                    }
-                    // swap rows p and q, then scale the pivot row (its pivot entry becomes 1 / pivot)
-                    for (int c = tid; c < Ne; c += B) {
-                        const double ar = Sb[2 * (p * Ne + c)], ai = Sb[2 * (p * Ne + c) + 1];
-                        double br = Sb[2 * (q * Ne + c)], bi = Sb[2 * (q * Ne + c) + 1];
-                        if (q != p) { Sb[2 * (q * Ne + c)] = ar; Sb[2 * (q * Ne + c) + 1] = ai; }
-                        if (c == p) { br = 1.0; bi = 0.0; }
-                        Sb[2 * (p * Ne + c)] = br * ir - bi * ii;
-                        Sb[2 * (p * Ne + c) + 1] = br * ii + bi * ir;
-                    }
-                    __syncthreads();
-                    for (int idx = tid; idx < Ne * Ne; idx += B) {
-                        const int r = idx / Ne, c = idx - r * Ne;
-                        if (r == p) continue;
-                        const double fr = pcol[2 * r], fi = pcol[2 * r + 1];
-                        const double xr = (c == p) ? 0.0 : Sb[2 * idx], xi = (c == p) ? 0.0 : Sb[2 * idx + 1];
-                        const double yr = Sb[2 * (p * Ne + c)], yi = Sb[2 * (p * Ne + c) + 1];
-                        Sb[2 * idx] = xr - (fr * yr - fi * yi);
-                        Sb[2 * idx + 1] = xi - (fr * yi + fi * yr);
+                __syncthreads();
+                // det S (FastDeterminant.jl:20-22) on wavefront 0 while the others sum |v_dot_d[m, n]|
+                // over unoccupied m and electrons n (iesh.jl:285-298)
+                if (w == 0) {
+                    double dr, di;
+                    const bool ok = wave_det(Sre, Sim, LD, Ne, dr, di);
+                    if (l == 0) {
+                        scal[1] = dr;
+                        scal[2] = di;
+                        scal[3] = ok ? 0.0 : 1.0;
                     }
                 }
-                __syncthreads();
-                if (!singular) {  // undo the row interchanges as column interchanges: Sb = S^-1
-                    for (int p = Ne - 1; p >= 0; --p) {
-                        const int q = piv[p];
-                        if (q != p) {
-                            for (int r = tid; r < Ne; r += B) {
-                                const double ar = Sb[2 * (r * Ne + p)], ai = Sb[2 * (r * Ne + p) + 1];
-                                Sb[2 * (r * Ne + p)] = Sb[2 * (r * Ne + q)];
-                                Sb[2 * (r * Ne + p) + 1] = Sb[2 * (r * Ne + q) + 1];
-                                Sb[2 * (r * Ne + q)] = ar;
-                                Sb[2 * (r * Ne + q) + 1] = ai;
+                double loc = 0.0;
+                {
+                    const int t0 = (NT > 1) ? 64 : 0;
+                    if (tid >= t0) {
+                        for (int k = tid - t0; k < N * Ne; k += B - t0) {
+                            const int m = k / Ne, n = k - m * Ne;
+                            if (isocc[m] < 0) {
+                                const int o = occ[n];
+                                loc += fabs(v * (sc.dh * (U[m] * U[o] / (E[o] - E[m]))));
                             }
                         }
-                        __syncthreads();
                     }
                 }
-                // sum over unoccupied m and electrons n of |v_dot_d[m, n]| (iesh.jl:285-298)
-                const int Nu = N - Ne;
-                double loc = 0.0;
-                for (int m = tid; m < N; m += B) {
-                    if (isocc[m] < 0) {
-                        for (int n = 0; n < Ne; ++n) {
-                            const int o = occ[n];
-                            loc += fabs(v * (sc.dh * (U[m] * U[o] / (E[o] - E[m]))));
-                        }
-                    }
-                }
-                const double sumvd = block_sum<NT>(loc, red);
+                const double sumvd = block_sum<NT>(loc, red);  // its barriers also publish scal[1..3]
+                const double dre = scal[1], dim = scal[2];
+                const bool singular = scal[3] != 0.0;
                 const double Akk = dre * dre + dim * dim;
                 const double pref = 2.0 * dt / Akk;
                 const double est = pref * sumvd * (fabs(dre) + fabs(dim));
                 const bool evaluate = !singular && Akk > 0.0 && (!A.estimate || !(est < rnd));
                 if (evaluate) {
+                    __syncthreads();  // wavefront 0's determinant work on Sb is complete
+                    for (int idx = tid; idx < Ne * Ne; idx += B) {
+                        const int j = idx / Ne, e = idx - j * Ne;
+                        Sb[2 * idx] = Tb[occ[j] * CW + 2 * e];
+                        Sb[2 * idx + 1] = Tb[occ[j] * CW + 2 * e + 1];
+                    }
+                    // S^-1: Gauss-Jordan inversion of S (interleaved complex, row-major) in place with partial pivoting
+                    bool gj_singular = false;
+                    for (int p = 0; p < Ne; ++p) {
+                        __syncthreads();
+                        if (w == 0) {
+                            double val = -1.0;
+                            int idx = l;
+                            if (l >= p && l < Ne) val = hypot(Sb[2 * (l * Ne + p)], Sb[2 * (l * Ne + p) + 1]);
+#pragma unroll
+                            for (int o = 32; o > 0; o >>= 1) {
+                                const double ov = __shfl_xor(val, o);
+                                const int oi = __shfl_xor(idx, o);
+                                if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
+                            }
+                            if (l == 0) {
+                                piv[p] = (val > 0.0) ? idx : -1;
+                                scal[1] = Sb[2 * (idx * Ne + p)];
+                                scal[2] = Sb[2 * (idx * Ne + p) + 1];
+                                scal[3] = Sb[2 * (p * Ne + p)];  // moves to row idx with the interchange
+                                scal[4] = Sb[2 * (p * Ne + p) + 1];
+                            }
+                        }
+                        __syncthreads();
+                        const int q = piv[p];
+                        if (q < 0) { gj_singular = true; break; }
+                        const double pr = scal[1], pi = scal[2];
+                        const double den = pr * pr + pi * pi;
+                        const double ir = pr / den, ii = -pi / den;  // 1 / pivot
+                        // column p after the row swap, for the elimination
+                        for (int r = tid; r < Ne; r += B) {
+                            if (r == p) continue;
+                            pcol[2 * r] = (r == q) ? scal[3] : Sb[2 * (r * Ne + p)];
+                            pcol[2 * r + 1] = (r == q) ? scal[4] : Sb[2 * (r * Ne + p) + 1];
+                        }
+                        // swap rows p and q, then scale the pivot row (its pivot entry becomes 1 / pivot)
+                        for (int c = tid; c < Ne; c += B) {
+                            const double ar = Sb[2 * (p * Ne + c)], ai = Sb[2 * (p * Ne + c) + 1];
+                            double br = Sb[2 * (q * Ne + c)], bi = Sb[2 * (q * Ne + c) + 1];
+                            if (q != p) { Sb[2 * (q * Ne + c)] = ar; Sb[2 * (q * Ne + c) + 1] = ai; }
+                            if (c == p) { br = 1.0; bi = 0.0; }
+                            Sb[2 * (p * Ne + c)] = br * ir - bi * ii;
+                            Sb[2 * (p * Ne + c) + 1] = br * ii + bi * ir;
+                        }
+                        __syncthreads();
+                        for (int idx = tid; idx < Ne * Ne; idx += B) {
+                            const int r = idx / Ne, c = idx - r * Ne;
+                            if (r == p) continue;
+                            const double fr = pcol[2 * r], fi = pcol[2 * r + 1];
+                            const double xr = (c == p) ? 0.0 : Sb[2 * idx], xi = (c == p) ? 0.0 : Sb[2 * idx + 1];
+                            const double yr = Sb[2 * (p * Ne + c)], yi = Sb[2 * (p * Ne + c) + 1];
+                            Sb[2 * idx] = xr - (fr * yr - fi * yi);
+                            Sb[2 * idx + 1] = xi - (fr * yi + fi * yr);
+                        }
+                    }
+                    __syncthreads();
+                    if (!gj_singular) {  // undo the row interchanges as column interchanges: Sb = S^-1
+                        for (int p = Ne - 1; p >= 0; --p) {
+                            const int q = piv[p];
+                            if (q != p) {
+                                for (int r = tid; r < Ne; r += B) {
+                                    const double ar = Sb[2 * (r * Ne + p)], ai = Sb[2 * (r * Ne + p) + 1];
+                                    Sb[2 * (r * Ne + p)] = Sb[2 * (r * Ne + q)];
+                                    Sb[2 * (r * Ne + p) + 1] = Sb[2 * (r * Ne + q) + 1];
+                                    Sb[2 * (r * Ne + q)] = ar;
+                                    Sb[2 * (r * Ne + q) + 1] = ai;
+                                }
+                            }
+                            __syncthreads();
+                        }
+                    }
                     // P(n -> m) = 2 dt Re((psi_m S^-1)_n) v_dot_d[m, n]: det(S with row n replaced by
                     // psi_m) = det(S) (psi_m S^-1)_n, so Re(det S conj(det S_{n<-m})) / |det S|^2 is that real part.
                     // pass 1: per-electron totals, partial sums over interleaved chunks of unoccupied states
@@ -505,7 +664,7 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
                         const int n = tid % Ne, ch = tid / Ne;
                         const int o = occ[n];
                         double part_sum = 0.0;
-                        for (int mi = ch; mi < Nu; mi += nch) {
+                        for (int mi = gj_singular ? Nu : ch; mi < Nu; mi += nch) {  // no hop from a singular S
                             const int m = unocc[mi];
                             double zr = 0.0;
                             for (int e = 0; e < Ne; ++e)
@@ -627,7 +786,30 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
     }
 }
 
-int ie_sb_len(int N, int Ne) { (void)N; return 2 * Ne * Ne; }
+// Determinants of nmat complex Ne x Ne matrices (re, im planes, row-major) by wave_det, one
+// wavefront per matrix: the test hook of the IESH estimate's determinant.
+__global__ __launch_bounds__(64) void wave_det_kernel(const double* __restrict__ re, const double* __restrict__ im,
+                                                      int Ne, double* __restrict__ out) {
+    extern __shared__ double S[];  // 2 Ne LD
+    const int LD = Ne | 1;
+    const size_t base = (size_t)blockIdx.x * Ne * Ne;
+    for (int idx = threadIdx.x; idx < Ne * Ne; idx += 64) {
+        const int j = idx / Ne, e = idx - j * Ne;
+        S[j * LD + e] = re[base + idx];
+        S[Ne * LD + j * LD + e] = im[base + idx];
+    }
+    __syncthreads();
+    double dr, di;
+    const bool ok = wave_det(S, S + Ne * LD, LD, Ne, dr, di);
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x] = dr;
+        out[3 * blockIdx.x + 1] = di;
+        out[3 * blockIdx.x + 2] = ok ? 1.0 : 0.0;
+    }
+}
+
+// S as two planes with an odd row stride (determinant), or interleaved Ne x Ne (inverse)
+int ie_sb_len(int N, int Ne) { (void)N; return 2 * Ne * (Ne | 1); }
 
 size_t ie_lds_bytes(int NT, int cw, int N, int Ne) {
     const int NP = 16 * NT;
@@ -875,6 +1057,29 @@ int32_t ie_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes) {
     }
     default: return fail(h, NQCD_ERR_UNSUPPORTED, "field not available for IESH / EhrenfestNA");
     }
+}
+
+int32_t ie_test_det(nqcd_handle_t* h, const double* re, const double* im, int32_t Ne, int32_t nmat, double* out) {
+    if (Ne < 1 || Ne > kIeMaxNe) return fail(h, NQCD_ERR_ARG, "1 <= n <= 64 required");
+    if (nmat < 1 || nmat > 65536) return fail(h, NQCD_ERR_ARG, "1 <= nmat <= 65536 required");
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = (size_t)nmat * Ne * Ne;
+    DevBuf a, b, o;
+    HIPCHK(h, a.ensure(sizeof(double) * n));
+    HIPCHK(h, b.ensure(sizeof(double) * n));
+    HIPCHK(h, o.ensure(sizeof(double) * 3 * nmat));
+    hipError_t e = hipMemcpy(a.p, re, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b.p, im, sizeof(double) * n, hipMemcpyHostToDevice);
+    const size_t lds = sizeof(double) * 2 * Ne * (Ne | 1);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)wave_det_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(wave_det_kernel, dim3(nmat), dim3(64), lds, h->stream, a.as<double>(), b.as<double>(), Ne, o.as<double>());
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, o.p, sizeof(double) * 3 * nmat, hipMemcpyDeviceToHost);
+    a.release(); b.release(); o.release();
+    if (e != hipSuccess) return fail(h, NQCD_ERR_DEVICE, std::string("determinant test: ") + hipGetErrorString(e));
+    return NQCD_OK;
 }
 
 }  // namespace nqcd

@@ -1,6 +1,12 @@
 // mlp.h -- the MLP surrogate of a two-state diabatic Hamiltonian: 6 -> 128 -> 128 -> 3, tanh,
 // BF16 MFMA (16x16x32, FP32 accumulation) for the 128-wide products, FP32 first layer.
 //
+// Inputs: the six-number descriptor of a diatom configuration x = (x1, y1, z1, x2, y2, z2),
+//   q = (b, z1, z2, cos(theta), X, Y),  b = |r1 - r2|,  cos(theta) = (z1 - z2) / b,
+// (X, Y) the bond midpoint parallel to the surface: smooth wherever b > 0, and invariant under
+// rotations of the bond about the surface normal (a flat, uncorrugated surface).  The network sees
+// q normalised by the training data's mean and standard deviation; dH/dx is the chain rule through q.
+//
 // Register layout of a hidden layer inside a wavefront: the 64 lanes evaluate 16 samples at a
 // time (a "sample tile"); lane l (group g = l / 16, column j = l % 16) holds units
 // u = 16 t + 4 g + r (t = 0..7, r = 0..3) of sample j.  That is the accumulator layout of
@@ -38,6 +44,36 @@ struct MlpNorm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__host__ __device__ inline void mlp_descriptor(const T x[kMlpIn], T q[kMlpIn]) {
+    const T dx = x[0] - x[3], dy = x[1] - x[4], dz = x[2] - x[5];
+    const T b = sqrt(dx * dx + dy * dy + dz * dz);
+    q[0] = b;
+    q[1] = x[2];
+    q[2] = x[5];
+    q[3] = dz / b;
+    q[4] = T(0.5) * (x[0] + x[3]);
+    q[5] = T(0.5) * (x[1] + x[4]);
+}
+
+// dH/dx from g = dH/dq at x (the transpose of dq/dx applied to g).
+template <class T>
+__host__ __device__ inline void mlp_descriptor_pullback(const T x[kMlpIn], const T g[kMlpIn], T out[kMlpIn]) {
+    const T d[3] = {x[0] - x[3], x[1] - x[4], x[2] - x[5]};
+    const T ib = T(1) / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const T c = d[2] * ib;
+    T w[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) w[a] = (g[0] - g[3] * c * ib) * d[a] * ib;  // db/dr1 = d / b, dc/dr1 = (e_z - c d / b) / b
+    w[2] += g[3] * ib;
+    out[0] = w[0] + T(0.5) * g[4];
+    out[1] = w[1] + T(0.5) * g[5];
+    out[2] = w[2] + g[1];
+    out[3] = -w[0] + T(0.5) * g[4];
+    out[4] = -w[1] + T(0.5) * g[5];
+    out[5] = -w[2] + g[2];
+}
 
 // unit index of position k = 8 g + e inside the 32-unit K step q
 __host__ __device__ constexpr int mlp_perm_unit(int q, int k) {

@@ -62,17 +62,24 @@ __global__ __launch_bounds__(256) void mlp_eval_kernel(const unsigned char* __re
     const int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 64;
     for (int T = 0; T < 4; ++T) {
         const int64_t s = base + 16 * T + j;
-        float xs[kMlpIn];
+        float x[kMlpIn], xs[kMlpIn];
 #pragma unroll
-        for (int i = 0; i < kMlpIn; ++i) xs[i] = (s < n) ? (X[s * kMlpIn + i] - nrm.in_shift[i]) * nrm.in_scale[i] : 0.0f;
+        for (int i = 0; i < kMlpIn; ++i) x[i] = (s < n) ? X[s * kMlpIn + i] : (i == 0 ? 1.0f : 0.0f);
+        mlp_descriptor(x, xs);
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) xs[i] = (xs[i] - nrm.in_shift[i]) * nrm.in_scale[i];
         float o[kMlpOut], J[kMlpOut][kMlpIn];
         mlp_tile<true>(lds, xs, o, J);
         if (l < 16 && s < n) {
 #pragma unroll
             for (int r = 0; r < kMlpOut; ++r) {
                 Hout[s * kMlpOut + r] = o[r] * nrm.out_scale[r] + nrm.out_shift[r];
+                float gq[kMlpIn], gx[kMlpIn];
 #pragma unroll
-                for (int i = 0; i < kMlpIn; ++i) dH[(s * kMlpOut + r) * kMlpIn + i] = J[r][i] * nrm.out_scale[r] * nrm.in_scale[i];
+                for (int i = 0; i < kMlpIn; ++i) gq[i] = J[r][i] * nrm.out_scale[r] * nrm.in_scale[i];
+                mlp_descriptor_pullback(x, gq, gx);
+#pragma unroll
+                for (int i = 0; i < kMlpIn; ++i) dH[(s * kMlpOut + r) * kMlpIn + i] = gx[i];
             }
         }
     }
@@ -271,11 +278,27 @@ static void init_member(std::vector<float>& P, uint64_t seed) {
     for (int i = 0; i < kMlpOut * kMlpH; ++i) P[kOffW3 + i] = nd(rng) / std::sqrt((float)kMlpH);
 }
 
-int32_t mlp_set_norm_from_data(nqcd_handle_t* h, const double* X, const double* Y, int64_t n) {
+// Descriptors of the configurations X (n, 6), in double precision.
+static std::vector<double> descriptors(const double* X, int64_t n) {
+    std::vector<double> q((size_t)n * kMlpIn);
+    for (int64_t k = 0; k < n; ++k) mlp_descriptor(X + k * kMlpIn, q.data() + k * kMlpIn);
+    return q;
+}
+
+static bool bonds_positive(const double* X, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) {
+        const double* x = X + k * kMlpIn;
+        const double d2 = (x[0] - x[3]) * (x[0] - x[3]) + (x[1] - x[4]) * (x[1] - x[4]) + (x[2] - x[5]) * (x[2] - x[5]);
+        if (!(d2 > 0.0)) return false;
+    }
+    return true;
+}
+
+static void set_norm_from_data(nqcd_handle_t* h, const std::vector<double>& Q, const double* Y, int64_t n) {
     MlpNorm& N = h->mlp_norm;
     for (int i = 0; i < kMlpIn; ++i) {
         double s = 0.0, s2 = 0.0;
-        for (int64_t k = 0; k < n; ++k) { const double x = X[k * kMlpIn + i]; s += x; s2 += x * x; }
+        for (int64_t k = 0; k < n; ++k) { const double x = Q[k * kMlpIn + i]; s += x; s2 += x * x; }
         const double mu = s / n, var = std::max(s2 / n - mu * mu, 1e-12);
         N.in_shift[i] = (float)mu;
         N.in_scale[i] = (float)(1.0 / std::sqrt(var));
@@ -287,15 +310,14 @@ int32_t mlp_set_norm_from_data(nqcd_handle_t* h, const double* X, const double* 
         N.out_shift[r] = (float)mu;
         N.out_scale[r] = (float)std::sqrt(var);
     }
-    return NQCD_OK;
 }
 
-// Normalised training data on the device.
-static int32_t upload_data(nqcd_handle_t* h, const double* X, const double* Y, int64_t n) {
+// Normalised descriptors and targets on the device.
+static int32_t upload_data(nqcd_handle_t* h, const std::vector<double>& Q, const double* Y, int64_t n) {
     const MlpNorm& N = h->mlp_norm;
     std::vector<float> xn((size_t)n * kMlpIn), tn((size_t)n * kMlpOut);
     for (int64_t k = 0; k < n; ++k) {
-        for (int i = 0; i < kMlpIn; ++i) xn[k * kMlpIn + i] = (float)((X[k * kMlpIn + i] - N.in_shift[i]) * N.in_scale[i]);
+        for (int i = 0; i < kMlpIn; ++i) xn[k * kMlpIn + i] = (float)((Q[k * kMlpIn + i] - N.in_shift[i]) * N.in_scale[i]);
         for (int r = 0; r < kMlpOut; ++r) tn[k * kMlpOut + r] = (float)((Y[k * kMlpOut + r] - N.out_shift[r]) / N.out_scale[r]);
     }
     HIPCHK(h, h->mlp_x.ensure(sizeof(float) * xn.size()));
@@ -339,10 +361,12 @@ int32_t mlp_train(nqcd_handle_t* h, const double* X, const double* Y, int64_t n,
     if (n > (int64_t)1 << 30) return fail(h, NQCD_ERR_ARG, "too many training samples");
     if (d->steps < 1 || d->batch < kTrainSamples || d->batch > n) return fail(h, NQCD_ERR_ARG, "steps >= 1, 32 <= batch <= n");
     if (!(d->lr > 0.0) || !(d->lr_final > 0.0)) return fail(h, NQCD_ERR_ARG, "learning rates must be positive");
+    if (!bonds_positive(X, n)) return fail(h, NQCD_ERR_ARG, "a configuration has coincident atoms");
     HIPCHK(h, hipSetDevice(h->device));
     if (int32_t rc = mlp_ensure(h)) return rc;
-    mlp_set_norm_from_data(h, X, Y, n);
-    if (int32_t rc = upload_data(h, X, Y, n)) return rc;
+    const std::vector<double> Q = descriptors(X, n);
+    set_norm_from_data(h, Q, Y, n);
+    if (int32_t rc = upload_data(h, Q, Y, n)) return rc;
     HIPCHK(h, h->mlp_adam_m.ensure(sizeof(float) * kMlpParams * kMlpMembers));
     HIPCHK(h, h->mlp_adam_v.ensure(sizeof(float) * kMlpParams * kMlpMembers));
     HIPCHK(h, hipMemsetAsync(h->mlp_adam_m.p, 0, sizeof(float) * kMlpParams * kMlpMembers, h->stream));
@@ -391,8 +415,9 @@ int32_t mlp_gradient(nqcd_handle_t* h, int32_t member, const double* X, const do
     if (!h->mlp_ready) return fail(h, NQCD_ERR_STATE, "no surrogate: train or load it first");
     if (member < 0 || member >= kMlpMembers) return fail(h, NQCD_ERR_ARG, "member out of range");
     if (n < 1 || n > (1 << 24)) return fail(h, NQCD_ERR_ARG, "n out of range");
+    if (!bonds_positive(X, n)) return fail(h, NQCD_ERR_ARG, "a configuration has coincident atoms");
     HIPCHK(h, hipSetDevice(h->device));
-    if (int32_t rc = upload_data(h, X, Y, n)) return rc;
+    if (int32_t rc = upload_data(h, descriptors(X, n), Y, n)) return rc;
     std::vector<int32_t> idx(n);
     for (int64_t i = 0; i < n; ++i) idx[i] = (int32_t)i;
     HIPCHK(h, h->mlp_idx.ensure(sizeof(int32_t) * n));
@@ -413,6 +438,7 @@ int32_t mlp_eval(nqcd_handle_t* h, int32_t member, const double* X, int64_t n, d
     if (!h->mlp_ready) return fail(h, NQCD_ERR_STATE, "no surrogate: train or load it first");
     if (member < 0 || member >= kMlpMembers) return fail(h, NQCD_ERR_ARG, "member out of range");
     if (n < 1 || n > (1 << 26)) return fail(h, NQCD_ERR_ARG, "n out of range");
+    if (!bonds_positive(X, n)) return fail(h, NQCD_ERR_ARG, "a configuration has coincident atoms");
     HIPCHK(h, hipSetDevice(h->device));
     std::vector<float> xf((size_t)n * kMlpIn);
     for (int64_t i = 0; i < n * kMlpIn; ++i) xf[i] = (float)X[i];

@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = (
     "nqcd_mi355x_set_model", "nqcd_mi355x_load_initial", "nqcd_mi355x_sample_initial", "nqcd_mi355x_run",
     "nqcd_mi355x_reduce", "nqcd_mi355x_download", "nqcd_mi355x_last_run_stats", "nqcd_mi355x_last_error",
     "nqcd_mi355x_surrogate_train", "nqcd_mi355x_surrogate_eval", "nqcd_mi355x_surrogate_gradient",
-    "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params",
+    "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params", "nqcd_mi355x_test_determinant",
 )
 
 
@@ -110,6 +110,7 @@ def lib():
     L.nqcd_mi355x_surrogate_gradient.argtypes = [H, C.c_int32, dp, dp, C.c_int64, dp, dp, dp]
     L.nqcd_mi355x_surrogate_set_params.argtypes = [H, C.c_int32, fp, fp]
     L.nqcd_mi355x_surrogate_get_params.argtypes = [H, C.c_int32, fp, fp]
+    L.nqcd_mi355x_test_determinant.argtypes = [H, dp, dp, C.c_int32, C.c_int32, dp]
     L.nqcd_mi355x_last_error.argtypes = [H]
     L.nqcd_mi355x_last_error.restype = C.c_char_p
     for name in EXPORTED_SYMBOLS:
@@ -301,6 +302,17 @@ class Ensemble:
         fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
         self._check(self._L.nqcd_mi355x_surrogate_get_params(self._h, member, fp(p), fp(nv)))
         return p, nv
+
+    def test_determinant(self, S):
+        """Determinants of complex matrices S (nmat, n, n) by the device routine of the IESH
+        hopping estimate -> (det (nmat,) complex, ok (nmat,) bool)."""
+        S = np.asarray(S, dtype=np.complex128)
+        nmat, n = S.shape[0], S.shape[1]
+        re = np.ascontiguousarray(S.real); im = np.ascontiguousarray(S.imag)
+        out = np.zeros((nmat, 3))
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        self._check(self._L.nqcd_mi355x_test_determinant(self._h, dp(re), dp(im), n, nmat, dp(out)))
+        return out[:, 0] + 1j * out[:, 1], out[:, 2] != 0
 
     def _download_iesh(self, field):
         n, ne, nt, nsave = self.n_states, self.n_electrons, self.ntraj, self.nsave

@@ -10,6 +10,9 @@ Precision points of the device kernels, reproduced when `bf16=True`: the first l
 h1 and h2 and the W2, W3 weights rounded to BF16 before their products (FP32 accumulation), the
 tangents of the forward-mode input gradient rounded to BF16 where they enter the W2 / W3 products.
 Loss: mean over samples of sum_o w_o (o - t_o)^2 on normalised targets t = (y - out_shift) / out_scale.
+The network's inputs are the descriptor q(x) of the configuration x = (x1, y1, z1, x2, y2, z2):
+    q = (b, z1, z2, (z1 - z2) / b, (x1 + x2) / 2, (y1 + y2) / 2),  b = |r1 - r2|,
+normalised as xn = (q - in_shift) * in_scale; forces follow by the chain rule through dq/dx.
 """
 import numpy as np
 
@@ -23,6 +26,34 @@ def bf16(a):
     rounding = ((u >> 16) & 1) + 0x7FFF
     u = ((u + rounding) >> 16) << 16
     return u.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def descriptor(X):
+    """(n, 6) configurations -> (n, 6) descriptors, FP64."""
+    X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+    d = X[:, :3] - X[:, 3:]
+    b = np.linalg.norm(d, axis=1)
+    return np.stack([b, X[:, 2], X[:, 5], d[:, 2] / b, 0.5 * (X[:, 0] + X[:, 3]), 0.5 * (X[:, 1] + X[:, 4])], axis=1)
+
+
+def descriptor_jacobian(X):
+    """dq/dx, shape (n, 6, 6), FP64, written out from the definition entry by entry."""
+    X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+    n = len(X)
+    d = X[:, :3] - X[:, 3:]
+    b = np.linalg.norm(d, axis=1)
+    D = np.zeros((n, 6, 6))
+    for a in range(3):
+        D[:, 0, a] = d[:, a] / b
+        D[:, 0, 3 + a] = -d[:, a] / b
+        dc = ((a == 2) - d[:, 2] * d[:, a] / b ** 2) / b
+        D[:, 3, a] = dc
+        D[:, 3, 3 + a] = -dc
+    D[:, 1, 2] = 1.0
+    D[:, 2, 5] = 1.0
+    D[:, 4, 0] = D[:, 4, 3] = 0.5
+    D[:, 5, 1] = D[:, 5, 4] = 0.5
+    return D
 
 
 def init(seed):

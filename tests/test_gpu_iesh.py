@@ -119,3 +119,40 @@ def test_iesh_abi_errors():
     with pytest.raises(L.NQCDError):
         E.load_initial_iesh([8.0, 8.0], [0.0, 0.0], occ)
     E.close()
+
+
+def test_device_determinant_matches_numpy():
+    """The single-wavefront LU determinant behind the hopping estimate: random complex matrices of
+    every size up to 64, permutation matrices (the sign of the pivot permutation), matrices with
+    tied pivot candidates, a singular matrix, and the occupied-row submatrices of unitary matrices
+    (the IESH case, |det| <= 1)."""
+    from nqcd_mi355x import _lib as L
+    rng = np.random.default_rng(3)
+    E = L.Ensemble(0)
+    try:
+        for n in list(range(1, 17)) + [20, 31, 32, 33, 50, 63, 64]:
+            S = rng.normal(size=(6, n, n)) + 1j * rng.normal(size=(6, n, n))
+            d, ok = E.test_determinant(S)
+            ref = np.linalg.det(S)
+            assert ok.all()
+            assert np.all(np.abs(d - ref) <= 1e-12 * np.abs(ref)), n
+        for n in (2, 3, 5, 8, 17, 64):
+            P = np.stack([np.eye(n)[rng.permutation(n)] for _ in range(4)]).astype(complex)
+            d, ok = E.test_determinant(P)
+            assert ok.all() and np.allclose(d, np.linalg.det(P), atol=0)
+        T = np.ones((1, 4, 4), complex) + np.diag([0.0, 1.0, 2.0, 3.0])[None]  # ties in the first column
+        d, _ = E.test_determinant(T)
+        assert abs(d[0] - np.linalg.det(T[0])) < 1e-12
+        Z = rng.normal(size=(1, 5, 5)) + 0j
+        Z[0, :, 2] = 0.0
+        d, ok = E.test_determinant(Z)
+        assert not ok[0] and d[0] == 0
+        for (N, Ne) in ((41, 20), (101, 50)):
+            Q, _ = np.linalg.qr(rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)))
+            occ = np.sort(rng.choice(N, Ne, replace=False))
+            S = Q[occ][:, :Ne][None]
+            d, ok = E.test_determinant(S)
+            ref = np.linalg.det(S[0])
+            assert ok[0] and abs(d[0] - ref) <= 1e-11 * abs(ref)
+    finally:
+        E.close()

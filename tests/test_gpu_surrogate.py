@@ -29,20 +29,40 @@ def _random_params(seed):
     return P
 
 
+def _configs(n, seed):
+    """Diatom configurations: bond 2.2 +- 0.3 bohr, random orientation, height 1.5 .. 8 bohr."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, 3)); u /= np.linalg.norm(u, axis=1, keepdims=True)
+    b = 2.2 + 0.3 * rng.normal(size=(n, 1))
+    c = np.concatenate([rng.normal(size=(n, 2)), rng.uniform(1.5, 8.0, size=(n, 1))], axis=1)
+    return np.concatenate([c + 0.5 * b * u, c - 0.5 * b * u], axis=1)
+
+
+def _norm_of(X):
+    q = mlp_ref.descriptor(X)
+    return np.concatenate([q.mean(0), 1.0 / q.std(0), np.zeros(3), np.ones(3)]).astype(np.float32)
+
+
 def test_forward_and_input_gradient_match_cpu_reference():
     from nqcd_mi355x import _lib as L
     P = _random_params(11)
-    X = np.random.default_rng(12).normal(size=(1000, 6))
+    X = _configs(1000, 12)
+    norm = _norm_of(X)
     E = L.Ensemble(0)
-    E.surrogate_set_params(0, _flat(P), IDENT)
+    E.surrogate_set_params(0, _flat(P), norm)
     Hg, Jg = E.surrogate_eval(0, X)
     E.close()
-    ref = mlp_ref.forward(P, X, bf=True)["o"]
-    Jref = mlp_ref.jacobian(P, X, bf=True)
-    exact, Jexact = mlp_ref.forward(P, X)["o"], mlp_ref.jacobian(P, X)
+    # the device rounds X to FP32 before the descriptor
+    Xf = X.astype(np.float32).astype(np.float64)
+    xn = (mlp_ref.descriptor(Xf) - norm[:6]) * norm[6:12]
+    Dq = mlp_ref.descriptor_jacobian(Xf) * norm[6:12][None, :, None]   # d xn / d x
+    ref = mlp_ref.forward(P, xn, bf=True)["o"]
+    Jref = mlp_ref.jacobian(P, xn, bf=True) @ Dq
+    exact, Jexact = mlp_ref.forward(P, xn)["o"], mlp_ref.jacobian(P, xn) @ Dq
     # same rounding points: differences are FP32 tanh / accumulation order flipping a BF16 rounding
     assert np.abs(Hg - ref).max() < 5e-3 and np.sqrt(np.mean((Hg - ref) ** 2)) < 5e-4
-    assert np.abs(Jg - Jref).max() < 1e-2 and np.sqrt(np.mean((Jg - Jref) ** 2)) < 1e-3
+    assert np.abs(Jg - Jref).max() < 2e-2 * np.abs(Jref).max()
+    assert np.sqrt(np.mean((Jg - Jref) ** 2)) < 2e-3 * np.sqrt(np.mean(Jref ** 2))
     # and the BF16 network stays close to the exact one
     assert np.sqrt(np.mean((Hg - exact) ** 2)) < 1e-2 * np.sqrt(np.mean(exact ** 2))
     assert np.sqrt(np.mean((Jg - Jexact) ** 2)) < 2e-2 * np.sqrt(np.mean(Jexact ** 2))
@@ -52,14 +72,16 @@ def test_training_gradient_matches_cpu_reference():
     from nqcd_mi355x import _lib as L
     P = _random_params(21)
     rng = np.random.default_rng(22)
-    X = rng.normal(size=(512, 6)); Y = rng.normal(size=(512, 3))
+    X = _configs(512, 23); Y = rng.normal(size=(512, 3))
+    norm = _norm_of(X)
     w = (1.0, 0.5, 2.0)
     E = L.Ensemble(0)
-    E.surrogate_set_params(1, _flat(P), IDENT)
+    E.surrogate_set_params(1, _flat(P), norm)
     g, loss = E.surrogate_gradient(1, X, Y, w)
     E.close()
     Pd = {k: v.astype(np.float64) for k, v in P.items()}
-    lref, gref = mlp_ref.loss_and_grads(Pd, X, Y, w)
+    xn = (mlp_ref.descriptor(X) - norm[:6]) * norm[6:12]
+    lref, gref = mlp_ref.loss_and_grads(Pd, xn, Y, w)
     gref = _flat({k: v for k, v in gref.items()}).astype(np.float64)
     assert abs(loss - lref) < 2e-2 * lref
     rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
@@ -67,6 +89,15 @@ def test_training_gradient_matches_cpu_reference():
     for a, b in ((0, 768), (768, 896), (896, 17280), (17280, 17408), (17408, 17792), (17792, 17795)):
         cos = g[a:b] @ gref[a:b] / (np.linalg.norm(g[a:b]) * np.linalg.norm(gref[a:b]))
         assert cos > 0.995, (a, b, cos)
+
+
+def test_coincident_atoms_rejected():
+    from nqcd_mi355x import _lib as L
+    E = L.Ensemble(0)
+    E.surrogate_set_params(0, _flat(_random_params(3)), IDENT)
+    with pytest.raises(RuntimeError, match="coincident"):
+        E.surrogate_eval(0, np.zeros((4, 6)))
+    E.close()
 
 
 def _inputs(ntraj, seed, Z=6.0, vz=3e-3):
@@ -114,8 +145,9 @@ def test_surrogate_trained_on_fssh_configurations_reproduces_populations(key):
     keep = np.arange(nsave)[None, :] * 20 <= vis["steps"][:, None]
     X = vis["configs"][keep]
     assert len(X) > 200000
-    model, hist = N.train_surrogate(analytic, X, steps=3000, batch=1024, lr=3e-3, lr_final=5e-5, threshold=2e-3)
-    assert hist[:, -100:].mean() < 0.02 * hist[:, :10].mean()
+    model, hist = N.train_surrogate(analytic, X, steps=20000, batch=1024, lr=3e-3, lr_final=1e-5, threshold=1e-3)
+    print(f"training loss: first {hist[:, :10].mean():.4g}  last {hist[:, -100:].mean():.4g}")
+    assert hist[:, -100:].mean() < 2e-3 * hist[:, :10].mean()
     # held-out accuracy of the ensemble mean
     Xt = X[np.random.default_rng(1).choice(len(X), 5000, replace=False)] + 0.01 * np.random.default_rng(2).normal(size=(5000, 6))
     E = L.Ensemble(0)
@@ -124,7 +156,8 @@ def test_surrogate_trained_on_fssh_configurations_reproduces_populations(key):
     Hm = 0.5 * (E.surrogate_eval(0, Xt)[0] + E.surrogate_eval(1, Xt)[0])
     E.close()
     err = np.sqrt(np.mean((Hm - analytic.potential(Xt)) ** 2, axis=0))
-    assert np.all(err < 3e-3), err
+    print("held-out RMS error of (H11, H22, H12):", err)
+    assert np.all(err < 2e-3), err
     # FSSH on new initial conditions
     r1, v1 = _inputs(20000, 43)
     ref = _fssh(L, analytic, r1, v1, key)

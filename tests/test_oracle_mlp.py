@@ -47,3 +47,23 @@ def test_forward_mode_input_gradient_matches_finite_differences():
         Xp, Xm = X.copy(), X.copy(); Xp[:, i] += h; Xm[:, i] -= h
         fd = (mlp_ref.forward(P, Xp)["o"] - mlp_ref.forward(P, Xm)["o"]) / (2 * h)
         assert np.allclose(J[:, :, i], fd, atol=1e-7)
+
+
+def test_descriptor_jacobian_matches_finite_differences_and_invariances():
+    rng = np.random.default_rng(7)
+    X = rng.normal(size=(50, 6)) + np.array([0, 0, 5.0, 0, 0, 3.0])
+    D = mlp_ref.descriptor_jacobian(X)
+    h = 1e-6
+    for i in range(6):
+        e = np.zeros(6); e[i] = h
+        fd = (mlp_ref.descriptor(X + e) - mlp_ref.descriptor(X - e)) / (2 * h)
+        assert np.abs(fd - D[:, :, i]).max() < 1e-8
+    # rotation about the surface normal and translation parallel to it leave (b, z1, z2, cos) unchanged
+    c, s = np.cos(0.7), np.sin(0.7)
+    R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
+    Xr = np.concatenate([X[:, :3] @ R.T, X[:, 3:] @ R.T], axis=1) + np.array([1.5, -2.0, 0, 1.5, -2.0, 0])
+    q, qr = mlp_ref.descriptor(X), mlp_ref.descriptor(Xr)
+    assert np.allclose(q[:, :4], qr[:, :4], atol=1e-12)
+    # known values
+    q1 = mlp_ref.descriptor([[0, 0, 3.0, 0, 4.0, 0]])[0]
+    assert np.allclose(q1, [5.0, 3.0, 0.0, 0.6, 0.0, 2.0])
