@@ -106,6 +106,9 @@ def iesh_extra(L):
     return res
 
 
+TRAIN_STEPS = 20000
+
+
 def surrogate_extra(L):
     """Learned form on the six-coordinate diatom-surface model: train the two-member BF16 surrogate
     on configurations the analytic FSSH visits, then FSSH on the surrogate vs on the analytic model
@@ -146,7 +149,7 @@ def surrogate_extra(L):
     keep = np.arange(vis["configs"].shape[1])[None, :] * 20 <= vis["steps"][:, None]
     X = vis["configs"][keep]
     t0 = time.perf_counter()
-    model, hist = NQ.train_surrogate(analytic, X, steps=3000, batch=1024, lr=3e-3, lr_final=5e-5, threshold=2e-3)
+    model, hist = NQ.train_surrogate(analytic, X, steps=TRAIN_STEPS, batch=1024, lr=3e-3, lr_final=1e-5, threshold=1e-3)
     t_train = time.perf_counter() - t0
     _progress(f"surrogate trained on {len(X)} configurations in {t_train:.1f} s")
     r1, v1 = inputs(100000, 43)
@@ -159,7 +162,7 @@ def surrogate_extra(L):
     # + the 128x(32 padded) output products
     flop = 2 * 7 * (2 * 128 * 128 + 2 * 16 * 128)
     return {"workload": "diatom-surface FSSH, 6 coordinates, 1e5 trajectories, dt 2, t_max 1e4, terminate z1 > 8",
-            "training_configurations": int(len(X)), "training_steps": 3000, "training_s": t_train,
+            "training_configurations": int(len(X)), "training_steps": TRAIN_STEPS, "training_s": t_train,
             "final_loss": [float(hist[0, -100:].mean()), float(hist[1, -100:].mean())],
             "surrogate_traj_steps_per_s": rate(sur), "analytic_traj_steps_per_s": rate(ref),
             "surrogate_kernel_ms": sur["stats"]["kernel_ms"], "analytic_kernel_ms": ref["stats"]["kernel_ms"],

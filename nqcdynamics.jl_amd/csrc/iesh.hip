@@ -336,18 +336,19 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
         // eigen at x (all threads); leaves E, U in LDS and returns the model scalars
         auto eigen = [&](double xx) -> Scalars {
             const Scalars s = na_scalars(A, xx);
-            if (tid < N) {
-                const double lo = (tid == 0) ? fmin(s.h, eps[0]) - zn - 1e-3 * (zn + 1e-300) : eps[tid - 1];
-                const double hi = (tid == M) ? fmax(s.h, eps[M - 1]) + zn + 1e-3 * (zn + 1e-300) : eps[tid];
+            const int ir = l * NT + w;  // roots dealt round-robin over the wavefronts
+            if (ir < N) {
+                const double lo = (ir == 0) ? fmin(s.h, eps[0]) - zn - 1e-3 * (zn + 1e-300) : eps[ir - 1];
+                const double hi = (ir == M) ? fmax(s.h, eps[M - 1]) + zn + 1e-3 * (zn + 1e-300) : eps[ir];
                 const double scale = fmax(fmax(fabs(A.a), fabs(eps[M - 1])), fabs(s.h)) + zn;
                 double fp;
-                const double lam = secular_root(eps, M, tid, V2, s.h, lo, hi, warm ? Eprev : lo - 1.0, 4.0e-16 * scale, fp);
-                E[tid] = lam;
-                U[tid] = 1.0 / sqrt(fp);
+                const double lam = secular_root(eps, M, ir, V2, s.h, lo, hi, warm ? Eprev : lo - 1.0, 4.0e-16 * scale, fp);
+                E[ir] = lam;
+                U[ir] = 1.0 / sqrt(fp);
                 Eprev = lam;
-            } else if (tid < NP) {
-                E[tid] = 0.0;
-                U[tid] = 0.0;
+            } else if (ir < NP) {
+                E[ir] = 0.0;
+                U[ir] = 0.0;
             }
             warm = true;
             __syncthreads();
