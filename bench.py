@@ -174,6 +174,22 @@ def surrogate_extra(L):
             "achieved_bf16_TFLOPs": flop * rate(sur) / 1e12, "frac_bf16_peak": flop * rate(sur) / 1e12 / 2500.0}
 
 
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fssh2_persistent_c2_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel from the committed counter summary (the line
+    `traffic_bytes_per_launch: N`), or None when absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,7 +198,18 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-port comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the IESH and surrogate measurements")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB three times (the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 1 << 30, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -254,7 +281,7 @@ def main():
                    "ntraj": NTRAJ, "traj_steps_per_pass": int(traj_steps), "parallelism": "1 GPU",
                    "schedule": "persistent work-queue kernel", "grid": grid, "block": block},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "fssh2_persistent<1,float>", "kernel_avg_ms": avg_kernel_ms,
                      "flop_per_traj_step": FLOP_PER_STEP,
                      "achieved_dense_convention": FLOP_PER_STEP_DENSE * traj_steps / (avg_kernel_ms / 1e3) / 1e12,
