@@ -152,16 +152,16 @@ def surrogate_extra(L):
     model, hist = NQ.train_surrogate(analytic, X, steps=TRAIN_STEPS, batch=1024, lr=3e-3, lr_final=1e-5, threshold=1e-3)
     t_train = time.perf_counter() - t0
     _progress(f"surrogate trained on {len(X)} configurations in {t_train:.1f} s")
-    r1, v1 = inputs(100000, 43)
+    r1, v1 = inputs(1000000, 43)
     ref = run(analytic, r1, v1)
     sur = run(model, r1, v1)
     rate = lambda o: o["stats"]["traj_steps"] / (o["stats"]["kernel_ms"] / 1e3)
     p_ref, p_sur = float(ref["final"][1]), float(sur["final"][1])
     _progress(f"surrogate FSSH {rate(sur):.3e} / analytic {rate(ref):.3e} trajectory-steps/s")
-    # BF16 MFMA work of one trajectory-step: two members x (forward + 6 tangents) x 128x128 product
-    # + the 128x(32 padded) output products
-    flop = 2 * 7 * (2 * 128 * 128 + 2 * 16 * 128)
-    return {"workload": "diatom-surface FSSH, 6 coordinates, 1e5 trajectories, dt 2, t_max 1e4, terminate z1 > 8",
+    # BF16 MFMA work of one trajectory-step: two members x (forward 128x128 and 16(3)x128 products +
+    # three reverse-mode 128x128 products for the input gradient)
+    flop = 2 * (4 * 2 * 128 * 128 + 2 * 16 * 128)
+    return {"workload": "diatom-surface FSSH, 6 coordinates, 1e6 trajectories, dt 2, t_max 1e4, terminate z1 > 8",
             "training_configurations": int(len(X)), "training_steps": TRAIN_STEPS, "training_s": t_train,
             "final_loss": [float(hist[0, -100:].mean()), float(hist[1, -100:].mean())],
             "surrogate_traj_steps_per_s": rate(sur), "analytic_traj_steps_per_s": rate(ref),

@@ -87,12 +87,26 @@ __device__ __forceinline__ void diatom_eval(const double* P, const R r[kNdDof], 
     }
 }
 
-// Both surrogate members at every lane's coordinates; the ensemble mean is returned, and `far`
-// says the members disagree by more than the threshold on some element of H.
+// Lane (0-based) of the k-th set bit of a 64-bit mask (k < popcount).
+__device__ __forceinline__ int select_bit(uint64_t m, int k) {
+    int base = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (k >= c) { k -= c; m >>= w; base += w; }
+    }
+    return base;
+}
+
+// Both surrogate members at the coordinates of the wavefront's active lanes; the ensemble mean is
+// returned in each active lane, and `far` says the members disagree by more than the threshold on
+// some element of H.  Active lanes are packed into the first ceil(n_active / 16) sample tiles (slot
+// s takes the lane of the s-th set bit of the active mask), so idle lanes cost no network work.
 __device__ __forceinline__ void surrogate_eval(const FsshNdArgs& A, const unsigned char* lds, const float r[kNdDof],
-                                               float& v11, float& v22, float& v12, float d11[kNdDof], float d22[kNdDof],
-                                               float d12[kNdDof], bool& far) {
-    const int l = __lane_id(), j = l & 15, grp = l >> 4;
+                                               bool active, float& v11, float& v22, float& v12, float d11[kNdDof],
+                                               float d22[kNdDof], float d12[kNdDof], bool& far) {
+    const int l = __lane_id(), j = l & 15;
     // idle lanes (all coordinates zero) evaluate a harmless configuration instead
     const float bb = (r[0] - r[3]) * (r[0] - r[3]) + (r[1] - r[4]) * (r[1] - r[4]) + (r[2] - r[5]) * (r[2] - r[5]);
     float x[kMlpIn], xn[kMlpIn];
@@ -101,27 +115,42 @@ __device__ __forceinline__ void surrogate_eval(const FsshNdArgs& A, const unsign
     mlp_descriptor(x, xn);
 #pragma unroll
     for (int i = 0; i < kMlpIn; ++i) xn[i] = (xn[i] - A.nrm.in_shift[i]) * A.nrm.in_scale[i];
-    float H[2][kMlpOut], J[2][kMlpOut][kMlpIn];
+    const uint64_t mask = __ballot(active);
+    const int nact = __popcll(mask);
+    const int ntile = (nact + 15) >> 4;
+    const int slot = __popcll(mask & ((1ull << l) - 1ull));               // this lane's slot, if active
+    const int src = (l < nact) ? select_bit(mask, l) : 0;               // lane feeding slot l
+    const int mt = slot >> 4, mj = slot & 15;
+    // members and sample tiles in one rolled loop: member it / ntile, tile it % ntile
+    float H0[kMlpOut], H1[kMlpOut], Js[kMlpOut][kMlpIn];
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
+    for (int r3 = 0; r3 < kMlpOut; ++r3) {
+        H0[r3] = 0.0f;
+        H1[r3] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) Js[r3][i] = 0.0f;
+    }
 #pragma unroll 1
-        for (int T = 0; T < 4; ++T) {
-            float xs[kMlpIn];
+    for (int it = 0; it < 2 * ntile; ++it) {
+        const int m = it >= ntile ? 1 : 0, T = it - m * ntile;
+        const int sl = __shfl(src, 16 * T + j);  // lane whose configuration is sample j of tile T
+        float xs[kMlpIn];
 #pragma unroll
-            for (int i = 0; i < kMlpIn; ++i) xs[i] = __shfl(xn[i], 16 * T + j);
-            float o[kMlpOut], Jt[kMlpOut][kMlpIn];
-            mlp_tile<true>(lds + m * kPkBytes, xs, o, Jt);
-            // the tile's results sit in lanes 0..15; lane 16 T + j takes those of sample 16 T + j
-            const bool mine = (grp == T);
+        for (int i = 0; i < kMlpIn; ++i) xs[i] = __shfl(xn[i], sl);
+        float o[kMlpOut], Jt[kMlpOut][kMlpIn];
+        mlp_tile<true>(lds + m * kPkBytes, xs, o, Jt);
+        // outputs sit in lanes 0..15 and the gradient in every lane of the sample: the owner of
+        // slot 16 T + mj takes those of sample mj
+        const bool mine = active && (mt == T);
 #pragma unroll
-            for (int r3 = 0; r3 < kMlpOut; ++r3) {
-                const float ov = __shfl(o[r3], j);
-                if (mine) H[m][r3] = ov;
+        for (int r3 = 0; r3 < kMlpOut; ++r3) {
+            const float ov = __shfl(o[r3], mj);
+            if (mine && m == 0) H0[r3] = ov;
+            if (mine && m == 1) H1[r3] = ov;
 #pragma unroll
-                for (int i = 0; i < kMlpIn; ++i) {
-                    const float jv = __shfl(Jt[r3][i], j);
-                    if (mine) J[m][r3][i] = jv;
-                }
+            for (int i = 0; i < kMlpIn; ++i) {
+                const float gv = __shfl(Jt[r3][i], mj);
+                if (mine) Js[r3][i] += gv;
             }
         }
     }
@@ -129,7 +158,7 @@ __device__ __forceinline__ void surrogate_eval(const FsshNdArgs& A, const unsign
     float Hm[kMlpOut];
 #pragma unroll
     for (int r3 = 0; r3 < kMlpOut; ++r3) {
-        const float a = H[0][r3] * A.nrm.out_scale[r3], b = H[1][r3] * A.nrm.out_scale[r3];
+        const float a = H0[r3] * A.nrm.out_scale[r3], b = H1[r3] * A.nrm.out_scale[r3];
         dmax = fmaxf(dmax, fabsf(a - b));
         Hm[r3] = 0.5f * (a + b) + A.nrm.out_shift[r3];
     }
@@ -138,9 +167,9 @@ __device__ __forceinline__ void surrogate_eval(const FsshNdArgs& A, const unsign
     float g11[kMlpIn], g22[kMlpIn], g12[kMlpIn];
 #pragma unroll
     for (int i = 0; i < kMlpIn; ++i) {
-        g11[i] = 0.5f * (J[0][0][i] + J[1][0][i]) * A.nrm.out_scale[0] * A.nrm.in_scale[i];
-        g22[i] = 0.5f * (J[0][1][i] + J[1][1][i]) * A.nrm.out_scale[1] * A.nrm.in_scale[i];
-        g12[i] = 0.5f * (J[0][2][i] + J[1][2][i]) * A.nrm.out_scale[2] * A.nrm.in_scale[i];
+        g11[i] = 0.5f * Js[0][i] * A.nrm.out_scale[0] * A.nrm.in_scale[i];
+        g22[i] = 0.5f * Js[1][i] * A.nrm.out_scale[1] * A.nrm.in_scale[i];
+        g12[i] = 0.5f * Js[2][i] * A.nrm.out_scale[2] * A.nrm.in_scale[i];
     }
     mlp_descriptor_pullback(x, g11, d11);
     mlp_descriptor_pullback(x, g22, d22);
@@ -342,11 +371,11 @@ __device__ __forceinline__ void nd_body(const FsshNdArgs* __restrict__ Ap, const
                 L.r[I] = L.r[I] + dtR * L.vt[I];
             }
         }
-        // (3) model at the current coordinates
+        // (3) model at the current coordinates (the surrogate packs the wavefront's active lanes)
         R v11, v22, v12, d11[kNdDof], d22[kNdDof], d12[kNdDof];
         if constexpr (kMlp) {
             bool far;
-            surrogate_eval(A, lds, L.r, v11, v22, v12, d11, d22, d12, far);
+            surrogate_eval(A, lds, L.r, active, v11, v22, v12, d11, d22, d12, far);
             if (active && far) {  // members disagree: the analytic model for this lane and step
                 diatom_eval<R>(nd_opaque(Ap)->p, L.r, v11, v22, v12, d11, d22, d12);
                 ++L.nfb;

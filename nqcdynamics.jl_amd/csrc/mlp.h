@@ -35,7 +35,8 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH, kOffW3 = kOffB2 + kMlpH, kOffB3 =
 //   [4608, 4672)   float b3[16] (3 used)
 //   [4672, 37440)  bf16 W2p[128][128], columns in the permuted K order
 //   [37440, 41536) bf16 W3p[16][128], rows 3..15 zero, columns permuted
-constexpr int kPkW1B = 0, kPkB2 = 4096, kPkB3 = 4608, kPkW2 = 4672, kPkW3 = 37440, kPkBytes = 41536;
+//   [41536, 74304) bf16 W2Tp[128][128] = W2^T, columns in the permuted K order (input gradients)
+constexpr int kPkW1B = 0, kPkB2 = 4096, kPkB3 = 4608, kPkW2 = 4672, kPkW3 = 37440, kPkW2T = 41536, kPkBytes = 74304;
 
 struct MlpNorm {
     float in_shift[kMlpIn], in_scale[kMlpIn];
@@ -102,12 +103,14 @@ __device__ __forceinline__ bf16x8 pack8(const float* lo, const float* hi) {
 }
 
 // One member on one sample tile.  xs: this lane's sample (column j) normalised inputs.
-// Returns, in lanes 0..15 (sample j), o[0..2] and (kGrad) the forward-mode input gradient J[o][i];
-// the passes (forward, then one tangent per input) run one after the other so that only one pass's
-// operands are live.  Optional outputs for training: h1, s1 = 1 - h1^2, h2, s2 of this lane's units.
+// Returns, in lanes 0..15 (sample j), o[0..2]; with kGrad also G[o][i] = d o / d xs[i] in every
+// lane of sample j (lanes j, j + 16, j + 32, j + 48).  The gradient is reverse mode: per output o,
+// delta2 = W3[o] * s2 (BF16), delta1 = (W2^T delta2) * s1 on MFMA (s = 1 - h^2), G[o] = W1^T delta1 in FP32;
+// the three outputs share every W2^T fragment read.  Optional outputs for training: h1, s1 = 1 - h1^2,
+// h2, s2 of this lane's units.
 template <bool kGrad>
 __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, const float xs[kMlpIn], float o[kMlpOut],
-                                         float J[kMlpOut][kMlpIn], float* h1_out = nullptr, float* s1_out = nullptr,
+                                         float G[kMlpOut][kMlpIn], float* h1_out = nullptr, float* s1_out = nullptr,
                                          float* h2_out = nullptr, float* s2_out = nullptr) {
     const int l = __lane_id(), g = l >> 4, j = l & 15;
     const float* w1b = reinterpret_cast<const float*>(W + kPkW1B);
@@ -116,7 +119,7 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
     const bf16x8* W2p = reinterpret_cast<const bf16x8*>(W + kPkW2);
     const bf16x8* W3p = reinterpret_cast<const bf16x8*>(W + kPkW3);
     // layer 1 in FP32
-    float h1[32], s1[32];
+    float h1[32];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
 #pragma unroll
@@ -124,69 +127,109 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
             const float4 wa = *reinterpret_cast<const float4*>(w1b + 8 * (16 * t + 4 * g + r));
             const float4 wb = *reinterpret_cast<const float4*>(w1b + 8 * (16 * t + 4 * g + r) + 4);
             const float a = wb.z + wa.x * xs[0] + wa.y * xs[1] + wa.z * xs[2] + wa.w * xs[3] + wb.x * xs[4] + wb.y * xs[5];
-            const float h = mlp_tanh(a);
-            h1[4 * t + r] = h;
-            s1[4 * t + r] = 1.0f - h * h;
+            h1[4 * t + r] = mlp_tanh(a);
         }
     }
-    float s2[32];
-    const int npass = kGrad ? 1 + kMlpIn : 1;
-#pragma unroll 1
-    for (int p = 0; p < npass; ++p) {
-        const int tin = p - 1;  // tangent input (-1: the forward pass)
-        const bf16x8* W2q = mlp_launder(W2p);
-        const bf16x8* W3q = mlp_launder(W3p);
+    // layer 2 and the output layer; with kGrad, delta2 of each output in the B layout, built as
+    // each K step's units come out: lane (g, j), K step q, e < 4: unit 32 q + 4 g + e, e >= 4:
+    // unit 32 q + 16 + 4 g + e - 4, the W3p row's permuted columns 32 q + 8 g + e
+    bf16x8 D[kMlpOut][4];
+    {
         bf16x8 B[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float lo[4], hi[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (tin < 0) { lo[r] = h1[4 * (2 * q) + r]; hi[r] = h1[4 * (2 * q + 1) + r]; }
-                else {
-                    lo[r] = s1[4 * (2 * q) + r] * w1b[8 * (16 * (2 * q) + 4 * g + r) + tin];
-                    hi[r] = s1[4 * (2 * q + 1) + r] * w1b[8 * (16 * (2 * q + 1) + 4 * g + r) + tin];
-                }
-            }
-            B[q] = pack8(lo, hi);
-        }
+        for (int q = 0; q < 4; ++q) B[q] = pack8(&h1[4 * (2 * q)], &h1[4 * (2 * q + 1)]);
         f32x4 acc3 = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int q3 = 0; q3 < 4; ++q3) {
-            float val[2][4];
+            float val[2][4], sv[2][4];
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
                 const int ot = 2 * q3 + half;
                 f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int kq = 0; kq < 4; ++kq) acc = mfma_bf16(W2q[(16 * ot + j) * 16 + 4 * kq + g], B[kq], acc);
+                for (int kq = 0; kq < 4; ++kq) acc = mfma_bf16(W2p[(16 * ot + j) * 16 + 4 * kq + g], B[kq], acc);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int idx = 4 * ot + r;
-                    if (tin < 0) {
-                        const float h = mlp_tanh(acc[r] + b2[16 * ot + 4 * g + r]);
-                        s2[idx] = 1.0f - h * h;
-                        val[half][r] = h;
-                        if (h2_out) h2_out[idx] = h;
-                    } else {
-                        val[half][r] = s2[idx] * acc[r];
-                    }
+                    const float h = mlp_tanh(acc[r] + b2[16 * ot + 4 * g + r]);
+                    sv[half][r] = 1.0f - h * h;
+                    val[half][r] = h;
+                    if (h2_out) { h2_out[idx] = h; s2_out[idx] = sv[half][r]; }
                 }
             }
-            acc3 = mfma_bf16(W3q[j * 16 + 4 * q3 + g], pack8(val[0], val[1]), acc3);
-        }
+            const bf16x8 w3o = W3p[j * 16 + 4 * q3 + g];
+            acc3 = mfma_bf16(w3o, pack8(val[0], val[1]), acc3);
+            if constexpr (kGrad) {
 #pragma unroll
-        for (int r = 0; r < kMlpOut; ++r) {
-            if (tin < 0) o[r] = acc3[r] + b3[r];
-            else if (kGrad) {
+                for (int oo = 0; oo < kMlpOut; ++oo) {
+                    const bf16x8 w3 = W3p[oo * 16 + 4 * q3 + g];
+                    float lo[4], hi[4];
 #pragma unroll
-                for (int i = 0; i < kMlpIn; ++i) J[r][i] = (tin == i) ? acc3[r] : J[r][i];
+                    for (int e = 0; e < 4; ++e) {
+                        lo[e] = (float)w3[e] * sv[0][e];
+                        hi[e] = (float)w3[4 + e] * sv[1][e];
+                    }
+                    D[oo][q3] = pack8(lo, hi);
+                }
             }
         }
+#pragma unroll
+        for (int r = 0; r < kMlpOut; ++r) o[r] = acc3[r] + b3[r];
     }
     if (h1_out) {
 #pragma unroll
-        for (int k = 0; k < 32; ++k) { h1_out[k] = h1[k]; s1_out[k] = s1[k]; s2_out[k] = s2[k]; }
+        for (int k = 0; k < 32; ++k) { h1_out[k] = h1[k]; s1_out[k] = 1.0f - h1[k] * h1[k]; }
+    }
+    if constexpr (kGrad) {
+        const bf16x8* W2Tp = reinterpret_cast<const bf16x8*>(W + kPkW2T);
+        float gs[kMlpOut][kMlpIn];
+#pragma unroll
+        for (int oo = 0; oo < kMlpOut; ++oo) {
+#pragma unroll
+            for (int i = 0; i < kMlpIn; ++i) gs[oo][i] = 0.0f;
+        }
+        // one 16-unit tile of W2^T delta2 per iteration, not unrolled (operands stay few); h1 of the
+        // tile's units is kept in h1[0..3] by rotating the array after each tile
+#pragma unroll 1
+        for (int ut = 0; ut < 8; ++ut) {
+            const bf16x8* W2Tq = mlp_launder(W2Tp) + (16 * ut + j) * 16 + g;
+            const float* w1q = mlp_launder(w1b) + 8 * (16 * ut + 4 * g);
+            f32x4 acc[kMlpOut];
+#pragma unroll
+            for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const bf16x8 A = W2Tq[4 * kq];
+#pragma unroll
+                for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = mfma_bf16(A, D[oo][kq], acc[oo]);
+            }
+            // acc[oo][r] = (W2^T delta2)[u] for u = 16 ut + 4 g + r, sample j
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float4 wa = *reinterpret_cast<const float4*>(w1q + 8 * r);
+                const float4 wb = *reinterpret_cast<const float4*>(w1q + 8 * r + 4);
+                const float w1[kMlpIn] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y};
+#pragma unroll
+                for (int oo = 0; oo < kMlpOut; ++oo) {
+                    const float d1 = acc[oo][r] * (1.0f - h1[r] * h1[r]);
+#pragma unroll
+                    for (int i = 0; i < kMlpIn; ++i) gs[oo][i] += w1[i] * d1;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 28; ++k) h1[k] = h1[k + 4];
+        }
+        // sum over the four lane groups holding the other units of sample j
+#pragma unroll
+        for (int oo = 0; oo < kMlpOut; ++oo) {
+#pragma unroll
+            for (int i = 0; i < kMlpIn; ++i) {
+                float v = gs[oo][i];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                G[oo][i] = v;
+            }
+        }
     }
 }
 

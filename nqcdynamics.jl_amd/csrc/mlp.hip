@@ -21,6 +21,7 @@ namespace nqcd {
 namespace {
 
 constexpr int kTrainSamples = 32;  // samples per training wavefront (two sample tiles)
+constexpr int kPackBlocks = 64;
 
 __global__ __launch_bounds__(256) void mlp_pack(const float* __restrict__ P, unsigned char* __restrict__ out) {
     float* w1b = reinterpret_cast<float*>(out + kPkW1B);
@@ -28,7 +29,7 @@ __global__ __launch_bounds__(256) void mlp_pack(const float* __restrict__ P, uns
     float* b3 = reinterpret_cast<float*>(out + kPkB3);
     __bf16* W2p = reinterpret_cast<__bf16*>(out + kPkW2);
     __bf16* W3p = reinterpret_cast<__bf16*>(out + kPkW3);
-    const int tid = threadIdx.x, B = blockDim.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, B = gridDim.x * blockDim.x;
     for (int i = tid; i < kMlpH * 8; i += B) {
         const int u = i >> 3, c = i & 7;
         w1b[i] = (c < kMlpIn) ? P[kOffW1 + u * kMlpIn + c] : (c == kMlpIn ? P[kOffB1 + u] : 0.0f);
@@ -42,6 +43,11 @@ __global__ __launch_bounds__(256) void mlp_pack(const float* __restrict__ P, uns
     for (int i = tid; i < 16 * kMlpH; i += B) {
         const int row = i >> 7, c = i & 127;
         W3p[i] = (row < kMlpOut) ? (__bf16)P[kOffW3 + row * kMlpH + mlp_perm_unit(c >> 5, c & 31)] : (__bf16)0.0f;
+    }
+    __bf16* W2Tp = reinterpret_cast<__bf16*>(out + kPkW2T);
+    for (int i = tid; i < kMlpH * kMlpH; i += B) {
+        const int u = i >> 7, c = i & 127;
+        W2Tp[i] = (__bf16)P[kOffW2 + mlp_perm_unit(c >> 5, c & 31) * kMlpH + u];
     }
 }
 
@@ -88,8 +94,7 @@ __global__ __launch_bounds__(256) void mlp_eval_kernel(const unsigned char* __re
 // One wavefront per 32 samples of the minibatch: partial gradient (kMlpParams floats, master
 // order) and partial loss of those samples.
 struct TrainLds {
-    unsigned char member[kPkBytes];
-    __bf16 W2Tp[kMlpH * kMlpH];            // W2^T with columns in the permuted K order
+    unsigned char member[kPkBytes];        // the packed member, W2^T included
     float W3[kMlpOut * kMlpH];
     __bf16 h1q[kMlpH * kTrainSamples];     // [unit][sample]
     float d2[kMlpH * kTrainSamples];       // [unit][sample]
@@ -108,10 +113,6 @@ __global__ __launch_bounds__(64) void mlp_train_grad(const unsigned char* __rest
     TrainLds& S = *reinterpret_cast<TrainLds*>(lds_raw);
     const int l = threadIdx.x, g = l >> 4, j = l & 15;
     stage_member(packed, S.member);
-    for (int i = l; i < kMlpH * kMlpH; i += 64) {
-        const int u = i >> 7, c = i & 127;
-        S.W2Tp[i] = (__bf16)P[kOffW2 + mlp_perm_unit(c >> 5, c & 31) * kMlpH + u];
-    }
     for (int i = l; i < kMlpOut * kMlpH; i += 64) S.W3[i] = P[kOffW3 + i];
     __syncthreads();
     const float wl[3] = {w0, w1, w2};
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(64) void mlp_train_grad(const unsigned char* __rest
         bf16x8 Bd[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) Bd[q] = pack8(&d2[4 * (2 * q)], &d2[4 * (2 * q + 1)]);
-        const bf16x8* W2Tp = reinterpret_cast<const bf16x8*>(S.W2Tp);
+        const bf16x8* W2Tp = reinterpret_cast<const bf16x8*>(S.member + kPkW2T);
 #pragma unroll
         for (int ut = 0; ut < 8; ++ut) {
             f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -262,7 +263,7 @@ int32_t mlp_ensure(nqcd_handle_t* h) {
 
 int32_t mlp_pack_all(nqcd_handle_t* h) {
     for (int k = 0; k < kMlpMembers; ++k) {
-        hipLaunchKernelGGL(mlp_pack, dim3(1), dim3(256), 0, h->stream, h->mlp_master.as<float>() + (size_t)k * kMlpParams,
+        hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks), dim3(256), 0, h->stream, h->mlp_master.as<float>() + (size_t)k * kMlpParams,
                            h->mlp_packed.as<unsigned char>() + (size_t)k * kPkBytes);
         HIPCHK(h, hipGetLastError());
     }
@@ -349,7 +350,7 @@ static int32_t train_step(nqcd_handle_t* h, int k, const int32_t* idx_dev, int32
                        h->mlp_losspart.as<float>(), loss_dev);
     HIPCHK(h, hipGetLastError());
     if (apply) {
-        hipLaunchKernelGGL(mlp_pack, dim3(1), dim3(256), 0, h->stream, P, h->mlp_packed.as<unsigned char>() + (size_t)k * kPkBytes);
+        hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks), dim3(256), 0, h->stream, P, h->mlp_packed.as<unsigned char>() + (size_t)k * kPkBytes);
         HIPCHK(h, hipGetLastError());
     }
     return NQCD_OK;
@@ -473,7 +474,7 @@ int32_t mlp_set_params(nqcd_handle_t* h, int32_t member, const float* P, const f
     HIPCHK(h, hipMemcpy(h->mlp_master.as<float>() + (size_t)member * kMlpParams, P, sizeof(float) * kMlpParams,
                         hipMemcpyHostToDevice));
     if (norm) std::memcpy(&h->mlp_norm, norm, sizeof(MlpNorm));
-    hipLaunchKernelGGL(mlp_pack, dim3(1), dim3(256), 0, h->stream, h->mlp_master.as<float>() + (size_t)member * kMlpParams,
+    hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks), dim3(256), 0, h->stream, h->mlp_master.as<float>() + (size_t)member * kMlpParams,
                        h->mlp_packed.as<unsigned char>() + (size_t)member * kPkBytes);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -80,20 +80,20 @@ def forward(P, xn, bf=False):
 
 
 def jacobian(P, xn, bf=False):
-    """d o / d xn, shape (n, 3, 6), by forward-mode tangents (the device's formulation)."""
+    """d o / d xn, shape (n, 3, 6), in reverse mode (the device's formulation): per output o,
+    delta2 = W3[o] * (1 - h2^2), delta1 = (W2^T delta2) * (1 - h1^2), d o / d xn = W1^T delta1.
+    With bf16=True, delta2 is rounded to BF16 before the W2^T product and W3, W2 are the BF16 weights."""
     f = forward(P, xn, bf)
     s1 = 1.0 - f["h1"] ** 2
     s2 = 1.0 - f["h2"] ** 2
     W1 = P["W1"].astype(np.float64)
     J = np.zeros((len(f["o"]), 3, 6))
-    for i in range(6):
-        t1 = s1 * W1[:, i]
+    for o in range(3):
+        d2 = s2 * f["W3"][o]
         if bf:
-            t1 = bf16(t1).astype(np.float64)
-        t2 = s2 * (t1 @ f["W2"].T)
-        if bf:
-            t2 = bf16(t2).astype(np.float64)
-        J[:, :, i] = t2 @ f["W3"].T
+            d2 = bf16(d2).astype(np.float64)
+        d1 = (d2 @ f["W2"]) * s1
+        J[:, o, :] = d1 @ W1
     return J
 
 

@@ -37,7 +37,7 @@ def test_parameter_gradients_match_finite_differences():
             assert abs(fd - g[name][ix]) < 1e-6 * max(1.0, abs(fd)), (name, ix, fd, g[name][ix])
 
 
-def test_forward_mode_input_gradient_matches_finite_differences():
+def test_reverse_mode_input_gradient_matches_finite_differences():
     rng = np.random.default_rng(2)
     P = {k: v.astype(np.float64) for k, v in mlp_ref.init(3).items()}
     X = rng.normal(size=(5, 6))
@@ -47,6 +47,9 @@ def test_forward_mode_input_gradient_matches_finite_differences():
         Xp, Xm = X.copy(), X.copy(); Xp[:, i] += h; Xm[:, i] -= h
         fd = (mlp_ref.forward(P, Xp)["o"] - mlp_ref.forward(P, Xm)["o"]) / (2 * h)
         assert np.allclose(J[:, :, i], fd, atol=1e-7)
+    # the BF16 rounding points move the gradient by the BF16 resolution only
+    Jb = mlp_ref.jacobian({k: v.astype(np.float32) for k, v in P.items()}, X, bf=True)
+    assert np.abs(Jb - J).max() < 3e-2 * np.abs(J).max()
 
 
 def test_descriptor_jacobian_matches_finite_differences_and_invariances():
