@@ -167,8 +167,7 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
                       &h->mlp_partial, &h->mlp_losspart, &h->mlp_lossbuf, &h->mlp_grad, &h->fallbacks,
                       &h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
-                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid,
-                      &h->ie_scratch};
+                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid};
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(h->ev0);
     (void)hipEventDestroy(h->ev1);
