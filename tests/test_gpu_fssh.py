@@ -209,7 +209,10 @@ def test_abi_errors_are_reported():
     with pytest.raises(L.NQCDError) as ei:
         E.set_problem(0, 2, 1, 1, [MASS], 1.0, rescaling=7)
     assert "rescaling" in str(ei.value)
-    with pytest.raises(L.NQCDError) as ei:
-        E.set_problem(2, 60, 1, 1, [MASS], 1.0, algorithm=1)
+    with pytest.raises(L.NQCDError) as ei:  # IESH integrates with VerletwithElectronics only
+        E.set_problem(L.METHOD_IESH, 60, 1, 1, [MASS], 1.0, algorithm=L.ALG_BAB_TSIT5, n_electrons=30)
     assert ei.value.code == L.ERR_UNSUPPORTED
+    with pytest.raises(L.NQCDError) as ei:
+        E.set_problem(L.METHOD_IESH, 60, 1, 1, [MASS], 1.0, algorithm=L.ALG_VERLET_ELECTRONICS)
+    assert ei.value.code == L.ERR_ARG and "n_electrons" in str(ei.value)
     E.close()
