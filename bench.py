@@ -30,11 +30,11 @@ XTERM = 10.0
 GRID = [-5.0, 5.0, 31]
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 HBM_PEAK_GBS = 8000.0
-# FP64 flops of one trajectory-step of the compact two-state formulation (counted from
-# fssh2.hip: 5 Tsit5 sub-steps x (6 RHS x 15 + stage combinations 4 x 48 + 12) = 1470,
-# plus ~80 for the Verlet step, model, eigen, NAC and hop test).
-FLOP_PER_STEP = 1550.0
-FLOP_PER_STEP_DENSE = 6.8e3  # dense complex formulation (reference data flow)
+# FP64 flops (an FMA counts 2) of one trajectory-step of the trace-fixed two-state formulation,
+# counted from nqcd_device.h / fssh2.hip and checked against the compiled sub-step loop (81 FMA,
+# 29 MUL, 3 ADD): 5 Tsit5 sub-steps x 194 = 970, plus ~75 for the eigen, the coupling rotation,
+# the interpolation set-up and the hop test.  The FP32 model and Verlet work are not counted.
+FLOP_PER_STEP = 1045.0
 
 
 def _ensemble(L, precision, schedule):
@@ -284,7 +284,6 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "fssh2_persistent<1,float>", "kernel_avg_ms": avg_kernel_ms,
                      "flop_per_traj_step": FLOP_PER_STEP,
-                     "achieved_dense_convention": FLOP_PER_STEP_DENSE * traj_steps / (avg_kernel_ms / 1e3) / 1e12,
                      "hbm_algorithmic_GBs": algo_bytes / (avg_kernel_ms / 1e3) / 1e9},
         "pop_error_vs_fp64": pop_err,
         "pop_sigma": sigma,
