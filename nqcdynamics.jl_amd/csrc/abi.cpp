@@ -87,6 +87,9 @@ int32_t alloc_results(nqcd_handle* h) {
     HIPCHK(h, h->save_di.ensure(sizeof(double) * 2 * h->nsave * n));
     HIPCHK(h, h->save_state.ensure(sizeof(int32_t) * h->nsave * n));
     HIPCHK(h, h->queue.ensure(sizeof(unsigned long long)));
+    HIPCHK(h, launch_zero(h->save_ad.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_di.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_state.p, sizeof(int32_t) * h->nsave * n, h->stream));
     // constant tables read by the kernels (models per slot, 1/m per flattened dof)
     HIPCHK(h, h->models_dev.ensure(sizeof(ModelSlot) * kMaxModels));
     HIPCHK(h, h->inv_mass_dev.ensure(sizeof(double) * D));

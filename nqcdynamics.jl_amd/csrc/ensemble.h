@@ -18,6 +18,9 @@ hipError_t launch_init_density2(int64_t ntraj, int32_t state, int32_t diabatic, 
 hipError_t launch_fill_output2(int32_t kind, int32_t method, int64_t ntraj, int32_t nsave, int32_t xdof, const double* r,
                                const double* sig, const int32_t* state, const double* save_pop_di, double* out,
                                hipStream_t s);
+// Clears `bytes` (a multiple of 4) at p with a fill kernel: save slots a trajectory never reaches
+// (it terminated first) read as zero, whatever the buffer held before.
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 int reduce_nchunk(int64_t ntraj);
 hipError_t launch_reduce_rows(const double* data, int32_t rows, int64_t ntraj, double* partial, double scale, double* out,
                               hipStream_t s);

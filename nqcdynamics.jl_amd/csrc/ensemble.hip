@@ -7,6 +7,8 @@
 // DynamicsOutputs.jl / TimeCorrelationFunctions.jl (src/Ensembles/reductions.jl:8-52).
 // Reductions run in a fixed order (chunked tree sums, then an ordered sum of
 // chunk partials) so results do not depend on the device or launch shape.
+#include <algorithm>
+
 #include "ensemble.h"
 
 namespace nqcd {
@@ -163,6 +165,18 @@ hipError_t launch_reduce_rows(const double* data, int32_t rows, int64_t ntraj, d
     const int nchunk = reduce_nchunk(ntraj);
     hipLaunchKernelGGL(reduce_rows_pass1, dim3(nchunk, rows), dim3(256), 0, s, data, ntraj, nchunk, partial);
     hipLaunchKernelGGL(reduce_rows_pass2, dim3(grid_for(rows, 64)), dim3(64), 0, s, partial, rows, nchunk, scale, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void zero_words(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 4;
+    if (!p || n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(zero_words, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p), n);
     return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 //   V11 = Dn (1 - e^{-an (b - bn)})^2 + Aw e^{-bw Z}
 //   V22 = Di (1 - e^{-ai (b - bi)})^2 + Ds (1 - e^{-as (Z - Zs)})^2 + Delta
 //   V12 = C e^{-g (Z - Zc)^2}
+#include "ensemble.h"
 #include "fssh2.h"
 #include "fssh_nd.h"
 #include "mlp.h"
@@ -458,6 +459,10 @@ int32_t nd_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     HIPCHK(h, h->save_state.ensure(sizeof(int32_t) * h->nsave * n));
     HIPCHK(h, h->queue.ensure(sizeof(unsigned long long)));
     HIPCHK(h, h->save_centroid.ensure(sizeof(double) * kNdDof * h->nsave * n));  // configuration per save slot
+    HIPCHK(h, launch_zero(h->save_ad.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_di.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_state.p, sizeof(int32_t) * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_centroid.p, sizeof(double) * kNdDof * h->nsave * n, h->stream));
     FsshNdArgs A{};
     A.ntraj = n;
     A.method = (h->prob.method == NQCD_METHOD_FSSH) ? METHOD_FSSH : METHOD_EHRENFEST;

@@ -982,6 +982,8 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     HIPCHK(h, h->nhops.ensure(sizeof(int32_t) * n));
     HIPCHK(h, h->save_ad.ensure(sizeof(double) * N * h->nsave * n));
     HIPCHK(h, h->save_di.ensure(sizeof(double) * N * h->nsave * n));
+    HIPCHK(h, launch_zero(h->save_ad.p, sizeof(double) * N * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_di.p, sizeof(double) * N * h->nsave * n, h->stream));
 
     IeshArgs A{};
     A.ntraj = n;

@@ -12,6 +12,7 @@
 // modes) are P-term dot products against values broadcast through LDS.  The centroid
 // electronics (eigen, coupling, Tsit5 density, hop) are evaluated identically by every lane
 // of the ring polymer.
+#include "ensemble.h"
 #include "ring_polymer.h"
 
 #include <cmath>
@@ -422,6 +423,10 @@ int32_t rp_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     HIPCHK(h, h->save_di.ensure(sizeof(double) * 2 * h->nsave * n));
     HIPCHK(h, h->save_state.ensure(sizeof(int32_t) * h->nsave * n));
     HIPCHK(h, h->save_centroid.ensure(sizeof(double) * h->nsave * n));
+    HIPCHK(h, launch_zero(h->save_ad.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_di.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_state.p, sizeof(int32_t) * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_centroid.p, sizeof(double) * h->nsave * n, h->stream));
     RpArgs A{};
     A.ntraj = n;
     A.method = (h->prob.method == NQCD_METHOD_RPMD) ? 0 : ((h->prob.method == NQCD_METHOD_RPSH) ? 1 : 2);

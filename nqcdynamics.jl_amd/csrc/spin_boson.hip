@@ -276,6 +276,9 @@ int32_t sb_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     HIPCHK(h, h->save_ad.ensure(sizeof(double) * 2 * h->nsave * n));
     HIPCHK(h, h->save_di.ensure(sizeof(double) * 2 * h->nsave * n));
     HIPCHK(h, h->save_state.ensure(sizeof(int32_t) * h->nsave * n));
+    HIPCHK(h, launch_zero(h->save_ad.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_di.p, sizeof(double) * 2 * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_state.p, sizeof(int32_t) * h->nsave * n, h->stream));
     SbArgs A{};
     A.ntraj = n;
     A.method = (h->prob.method == NQCD_METHOD_FSSH) ? 0 : 1;
