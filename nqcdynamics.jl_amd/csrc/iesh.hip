@@ -44,6 +44,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIeMaxNe = 64;
 constexpr int kIeMaxStates = 128;
+constexpr size_t kIeLdsBytes = 160 * 1024;  // LDS of one gfx950 compute unit
 
 struct IeshArgs {
     int64_t ntraj;
@@ -966,6 +967,11 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     IeKernel kern = ie_pick(NT, wide);
     if (!kern) return fail(h, NQCD_ERR_UNSUPPORTED, "n_states");
     const size_t lds = ie_lds_bytes(NT, 16 * (wide ? 8 : NT), N, Ne);
+    if (lds > kIeLdsBytes)
+        return fail(h, NQCD_ERR_UNSUPPORTED,
+                    "IESH / EhrenfestNA with " + std::to_string(N) + " states and " + std::to_string(Ne) +
+                        " electrons needs " + std::to_string(lds) + " bytes of LDS per workgroup (" +
+                        std::to_string(kIeLdsBytes) + " available)");
     HIPCHK(h, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int block = 64 * NT;
     int bpc = 0;

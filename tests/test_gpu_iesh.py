@@ -73,6 +73,28 @@ def test_iesh_fp64_parity(oracle, key, M, Ne, ntraj, nsteps):
     assert np.allclose(g["di_mean"], c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("M,Ne,ntraj,nsteps", [(2, 1, 32, 200), (16, 8, 32, 200), (40, 30, 16, 120), (63, 40, 8, 80),
+                                              (127, 20, 8, 60)])
+def test_iesh_fp64_parity_sizes(oracle, key, M, Ne, ntraj, nsteps):
+    """The smallest problem (3 states, one electron, one wavefront), a two-wavefront block, the wide
+    column-tile variant (2 n_e above the padded state count: N = 41 with 30 electrons, N = 64 with
+    40) and the largest state count (N = 128)."""
+    from nqcd_mi355x import _lib as L
+    r0, v0, occ0 = _inputs(ntraj, Ne, 21)
+    g = _gpu(L, L.METHOD_IESH, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
+    c = _cpu(oracle, 0, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
+    _compare(g, c)
+
+
+def test_iesh_too_large_for_lds_is_rejected(key):
+    """128 states with 64 electrons would need more than a compute unit's 160 KB of LDS: the run
+    fails with a message naming the sizes instead of launching."""
+    from nqcd_mi355x import _lib as L
+    r0, v0, occ0 = _inputs(4, 64, 22)
+    with pytest.raises(L.NQCDError, match="LDS"):
+        _gpu(L, L.METHOD_IESH, 127, 64, r0, v0, occ0, 2.0, 10, key)
+
+
 def test_iesh_estimate_off_and_rescaling_modes(oracle, key):
     from nqcd_mi355x import _lib as L
     M, Ne, ntraj, nsteps = 40, 20, 32, 200
