@@ -30,7 +30,7 @@ def _thermal_beads(L, P, ntraj, key, x0=-3.87, nstates=1):
     return r, v
 
 
-@pytest.mark.parametrize("P", [8, 16, 32, 64])
+@pytest.mark.parametrize("P", [1, 2, 4, 8, 16, 32, 64])
 def test_rpmd_fp64_parity(oracle, key, P):
     from nqcd_mi355x import _lib as L
     ntraj, nsteps = 256, 500
@@ -49,7 +49,7 @@ def test_rpmd_fp64_parity(oracle, key, P):
     assert np.allclose(g_c, c["centroid"], rtol=1e-9, atol=1e-10)
 
 
-@pytest.mark.parametrize("P", [8, 32])
+@pytest.mark.parametrize("P", [1, 4, 8, 32, 64])
 @pytest.mark.parametrize("method", ["rpsh", "rpeh"])
 def test_rp_electronic_fp64_parity(oracle, key, P, method):
     from nqcd_mi355x import _lib as L
@@ -76,7 +76,9 @@ def test_rp_electronic_fp64_parity(oracle, key, P, method):
         assert c["nhops"].sum() > 10
         assert np.array_equal(g["save_state"], c["save_state"])
     assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-10)
-    assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-12)
+    # bead velocities reach |v| ~ 1 through the stiff internal modes at large P, while some elements
+    # cross zero: 1e-9 relative to the largest velocity (the observed differences are 5e-12 at P = 64)
+    assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-9 * np.abs(c["v"]).max())
     assert np.allclose(g["sig"], c["sig_re"] + 1j * c["sig_im"], rtol=1e-9, atol=1e-10)
     assert np.allclose(g["pop_di"], c["pop_di"], rtol=1e-9, atol=1e-10)
 
