@@ -36,6 +36,9 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH, kOffW3 = kOffB2 + kMlpH, kOffB3 =
 //   [4672, 37440)  bf16 W2p[128][128], columns in the permuted K order
 //   [37440, 41536) bf16 W3p[16][128], rows 3..15 zero, columns permuted
 //   [41536, 74304) bf16 W2Tp[128][128] = W2^T, columns in the permuted K order (input gradients)
+// In the three BF16 matrices the 16-byte chunks of each row are XOR-swizzled by the row index
+// (mlp_swz): a fragment read has the 16 lanes of a lane group on rows 16 t + j, one chunk each, and
+// unswizzled rows 256 bytes apart would put all 16 in the same LDS banks.
 constexpr int kPkW1B = 0, kPkB2 = 4096, kPkB3 = 4608, kPkW2 = 4672, kPkW3 = 37440, kPkW2T = 41536, kPkBytes = 74304;
 
 struct MlpNorm {
@@ -75,6 +78,11 @@ __host__ __device__ inline void mlp_descriptor_pullback(const T x[kMlpIn], const
     out[4] = -w[1] + T(0.5) * g[5];
     out[5] = -w[2] + g[2];
 }
+
+// Element (row, permuted column c) of a packed 128-column BF16 matrix, and the bf16x8 fragment
+// holding chunk ch (8 columns) of a row.
+__host__ __device__ constexpr int mlp_swz(int row, int c) { return row * 128 + ((((c >> 3) ^ (row & 15))) << 3) + (c & 7); }
+__host__ __device__ constexpr int mlp_frag(int row, int ch) { return row * 16 + (ch ^ (row & 15)); }
 
 // unit index of position k = 8 g + e inside the 32-unit K step q
 __host__ __device__ constexpr int mlp_perm_unit(int q, int k) {
@@ -147,7 +155,7 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
                 const int ot = 2 * q3 + half;
                 f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int kq = 0; kq < 4; ++kq) acc = mfma_bf16(W2p[(16 * ot + j) * 16 + 4 * kq + g], B[kq], acc);
+                for (int kq = 0; kq < 4; ++kq) acc = mfma_bf16(W2p[mlp_frag(16 * ot + j, 4 * kq + g)], B[kq], acc);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int idx = 4 * ot + r;
@@ -157,12 +165,12 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
                     if (h2_out) { h2_out[idx] = h; s2_out[idx] = sv[half][r]; }
                 }
             }
-            const bf16x8 w3o = W3p[j * 16 + 4 * q3 + g];
+            const bf16x8 w3o = W3p[mlp_frag(j, 4 * q3 + g)];
             acc3 = mfma_bf16(w3o, pack8(val[0], val[1]), acc3);
             if constexpr (kGrad) {
 #pragma unroll
                 for (int oo = 0; oo < kMlpOut; ++oo) {
-                    const bf16x8 w3 = W3p[oo * 16 + 4 * q3 + g];
+                    const bf16x8 w3 = W3p[mlp_frag(oo, 4 * q3 + g)];
                     float lo[4], hi[4];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
@@ -192,14 +200,14 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
         // tile's units is kept in h1[0..3] by rotating the array after each tile
 #pragma unroll 1
         for (int ut = 0; ut < 8; ++ut) {
-            const bf16x8* W2Tq = mlp_launder(W2Tp) + (16 * ut + j) * 16 + g;
+            const bf16x8* W2Tq = mlp_launder(W2Tp) + (16 * ut + j) * 16;  // row 16 ut + j: chunk ch at ch ^ j
             const float* w1q = mlp_launder(w1b) + 8 * (16 * ut + 4 * g);
             f32x4 acc[kMlpOut];
 #pragma unroll
             for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kq = 0; kq < 4; ++kq) {
-                const bf16x8 A = W2Tq[4 * kq];
+                const bf16x8 A = W2Tq[(4 * kq + g) ^ j];
 #pragma unroll
                 for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = mfma_bf16(A, D[oo][kq], acc[oo]);
             }

@@ -38,16 +38,16 @@ __global__ __launch_bounds__(256) void mlp_pack(const float* __restrict__ P, uns
     for (int o = tid; o < 16; o += B) b3[o] = (o < kMlpOut) ? P[kOffB3 + o] : 0.0f;
     for (int i = tid; i < kMlpH * kMlpH; i += B) {
         const int row = i >> 7, c = i & 127;
-        W2p[i] = (__bf16)P[kOffW2 + row * kMlpH + mlp_perm_unit(c >> 5, c & 31)];
+        W2p[mlp_swz(row, c)] = (__bf16)P[kOffW2 + row * kMlpH + mlp_perm_unit(c >> 5, c & 31)];
     }
     for (int i = tid; i < 16 * kMlpH; i += B) {
         const int row = i >> 7, c = i & 127;
-        W3p[i] = (row < kMlpOut) ? (__bf16)P[kOffW3 + row * kMlpH + mlp_perm_unit(c >> 5, c & 31)] : (__bf16)0.0f;
+        W3p[mlp_swz(row, c)] = (row < kMlpOut) ? (__bf16)P[kOffW3 + row * kMlpH + mlp_perm_unit(c >> 5, c & 31)] : (__bf16)0.0f;
     }
     __bf16* W2Tp = reinterpret_cast<__bf16*>(out + kPkW2T);
     for (int i = tid; i < kMlpH * kMlpH; i += B) {
         const int u = i >> 7, c = i & 127;
-        W2Tp[i] = (__bf16)P[kOffW2 + mlp_perm_unit(c >> 5, c & 31) * kMlpH + u];
+        W2Tp[mlp_swz(u, c)] = (__bf16)P[kOffW2 + mlp_perm_unit(c >> 5, c & 31) * kMlpH + u];
     }
 }
 
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(64) void mlp_train_grad(const unsigned char* __rest
         for (int ut = 0; ut < 8; ++ut) {
             f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc = mfma_bf16(W2Tp[(16 * ut + j) * 16 + 4 * q + g], Bd[q], acc);
+            for (int q = 0; q < 4; ++q) acc = mfma_bf16(W2Tp[mlp_frag(16 * ut + j, 4 * q + g)], Bd[q], acc);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int u = 16 * ut + 4 * g + r;
