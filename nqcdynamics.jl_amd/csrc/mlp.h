@@ -36,10 +36,12 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH, kOffW3 = kOffB2 + kMlpH, kOffB3 =
 //   [4672, 37440)  bf16 W2p[128][128], columns in the permuted K order
 //   [37440, 41536) bf16 W3p[16][128], rows 3..15 zero, columns permuted
 //   [41536, 74304) bf16 W2Tp[128][128] = W2^T, columns in the permuted K order (input gradients)
-// In the three BF16 matrices the 16-byte chunks of each row are XOR-swizzled by the row index
+//   [74304, 78400) bf16 W1Tp[16][128] = W1^T, rows 6..15 zero, columns permuted (input gradients)
+// In the four BF16 matrices the 16-byte chunks of each row are XOR-swizzled by the row index
 // (mlp_swz): a fragment read has the 16 lanes of a lane group on rows 16 t + j, one chunk each, and
 // unswizzled rows 256 bytes apart would put all 16 in the same LDS banks.
-constexpr int kPkW1B = 0, kPkB2 = 4096, kPkB3 = 4608, kPkW2 = 4672, kPkW3 = 37440, kPkW2T = 41536, kPkBytes = 74304;
+constexpr int kPkW1B = 0, kPkB2 = 4096, kPkB3 = 4608, kPkW2 = 4672, kPkW3 = 37440, kPkW2T = 41536, kPkW1T = 74304,
+              kPkBytes = 78400;
 
 struct MlpNorm {
     float in_shift[kMlpIn], in_scale[kMlpIn];
@@ -97,7 +99,9 @@ __device__ __forceinline__ const T* mlp_launder(const T* p) {
     return p;
 }
 
-__device__ __forceinline__ float mlp_tanh(float a) { return 1.0f - 2.0f / (__expf(2.0f * a) + 1.0f); }
+// tanh(a) = 1 - 2 / (e^{2a} + 1) with the hardware exponential and reciprocal (about 1 ulp each,
+// far below the BF16 rounding the activations get next); saturates to +-1 for large |a|.
+__device__ __forceinline__ float mlp_tanh(float a) { return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * a) + 1.0f); }
 
 __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -113,8 +117,9 @@ __device__ __forceinline__ bf16x8 pack8(const float* lo, const float* hi) {
 // One member on one sample tile.  xs: this lane's sample (column j) normalised inputs.
 // Returns, in lanes 0..15 (sample j), o[0..2]; with kGrad also G[o][i] = d o / d xs[i] in every
 // lane of sample j (lanes j, j + 16, j + 32, j + 48).  The gradient is reverse mode: per output o,
-// delta2 = W3[o] * s2 (BF16), delta1 = (W2^T delta2) * s1 on MFMA (s = 1 - h^2), G[o] = W1^T delta1 in FP32;
-// the three outputs share every W2^T fragment read.  Optional outputs for training: h1, s1 = 1 - h1^2,
+// delta2 = W3[o] * s2, delta1 = (W2^T delta2) * s1 (s = 1 - h^2), G[o] = W1^T delta1, both products on
+// BF16 MFMA with delta2, delta1 and W1 rounded to BF16; the three outputs share every W2^T
+// fragment read.  Optional outputs for training: h1, s1 = 1 - h1^2,
 // h2, s2 of this lane's units.
 template <bool kGrad>
 __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, const float xs[kMlpIn], float o[kMlpOut],
@@ -190,53 +195,48 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
     }
     if constexpr (kGrad) {
         const bf16x8* W2Tp = reinterpret_cast<const bf16x8*>(W + kPkW2T);
-        float gs[kMlpOut][kMlpIn];
+        const bf16x8* W1Tp = reinterpret_cast<const bf16x8*>(W + kPkW1T);
+        // K step q covers units 32 q .. 32 q + 31 = the two 16-unit tiles 2 q, 2 q + 1: W2^T delta2 on
+        // those tiles (the three outputs share each W2^T fragment), times s1, packed to BF16 is the
+        // B operand of G += W1^T delta1 for that K step.  The q loop is unrolled with the weight
+        // pointers laundered per step, so the loads stay inside their step.
+        f32x4 gacc[kMlpOut];
 #pragma unroll
-        for (int oo = 0; oo < kMlpOut; ++oo) {
+        for (int oo = 0; oo < kMlpOut; ++oo) gacc[oo] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int i = 0; i < kMlpIn; ++i) gs[oo][i] = 0.0f;
-        }
-        // one 16-unit tile of W2^T delta2 per iteration, not unrolled (operands stay few); h1 of the
-        // tile's units is kept in h1[0..3] by rotating the array after each tile
-#pragma unroll 1
-        for (int ut = 0; ut < 8; ++ut) {
-            const bf16x8* W2Tq = mlp_launder(W2Tp) + (16 * ut + j) * 16;  // row 16 ut + j: chunk ch at ch ^ j
-            const float* w1q = mlp_launder(w1b) + 8 * (16 * ut + 4 * g);
-            f32x4 acc[kMlpOut];
+        for (int q = 0; q < 4; ++q) {
+            const bf16x8* W2Tq = mlp_launder(W2Tp);
+            float v[kMlpOut][2][4];
 #pragma unroll
-            for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int half = 0; half < 2; ++half) {
+                const int ut = 2 * q + half;
+                f32x4 acc[kMlpOut];
 #pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const bf16x8 A = W2Tq[(4 * kq + g) ^ j];
+                for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = mfma_bf16(A, D[oo][kq], acc[oo]);
-            }
-            // acc[oo][r] = (W2^T delta2)[u] for u = 16 ut + 4 g + r, sample j
+                for (int kq = 0; kq < 4; ++kq) {
+                    const bf16x8 A = W2Tq[mlp_frag(16 * ut + j, 4 * kq + g)];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float4 wa = *reinterpret_cast<const float4*>(w1q + 8 * r);
-                const float4 wb = *reinterpret_cast<const float4*>(w1q + 8 * r + 4);
-                const float w1[kMlpIn] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y};
+                    for (int oo = 0; oo < kMlpOut; ++oo) acc[oo] = mfma_bf16(A, D[oo][kq], acc[oo]);
+                }
+                // acc[oo][r] = (W2^T delta2)[u] for u = 16 ut + 4 g + r, sample j
 #pragma unroll
-                for (int oo = 0; oo < kMlpOut; ++oo) {
-                    const float d1 = acc[oo][r] * (1.0f - h1[r] * h1[r]);
+                for (int r = 0; r < 4; ++r) {
+                    const float s1 = 1.0f - h1[4 * ut + r] * h1[4 * ut + r];
 #pragma unroll
-                    for (int i = 0; i < kMlpIn; ++i) gs[oo][i] += w1[i] * d1;
+                    for (int oo = 0; oo < kMlpOut; ++oo) v[oo][half][r] = acc[oo][r] * s1;
                 }
             }
+            const bf16x8 A1 = mlp_launder(W1Tp)[mlp_frag(j, 4 * q + g)];
 #pragma unroll
-            for (int k = 0; k < 28; ++k) h1[k] = h1[k + 4];
+            for (int oo = 0; oo < kMlpOut; ++oo) gacc[oo] = mfma_bf16(A1, pack8(v[oo][0], v[oo][1]), gacc[oo]);
         }
-        // sum over the four lane groups holding the other units of sample j
+        // gacc[oo][r] in lane (g, j) is d o / d xs[4 g + r] of sample j: gather inputs 0..5 into
+        // every lane of sample j
 #pragma unroll
         for (int oo = 0; oo < kMlpOut; ++oo) {
 #pragma unroll
-            for (int i = 0; i < kMlpIn; ++i) {
-                float v = gs[oo][i];
-                v += __shfl_xor(v, 16);
-                v += __shfl_xor(v, 32);
-                G[oo][i] = v;
-            }
+            for (int i = 0; i < kMlpIn; ++i) G[oo][i] = __shfl(gacc[oo][i & 3], j + 16 * (i >> 2));
         }
     }
 }

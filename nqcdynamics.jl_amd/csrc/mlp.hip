@@ -49,6 +49,11 @@ __global__ __launch_bounds__(256) void mlp_pack(const float* __restrict__ P, uns
         const int u = i >> 7, c = i & 127;
         W2Tp[mlp_swz(u, c)] = (__bf16)P[kOffW2 + mlp_perm_unit(c >> 5, c & 31) * kMlpH + u];
     }
+    __bf16* W1Tp = reinterpret_cast<__bf16*>(out + kPkW1T);
+    for (int i = tid; i < 16 * kMlpH; i += B) {
+        const int in = i >> 7, c = i & 127;
+        W1Tp[mlp_swz(in, c)] = (in < kMlpIn) ? (__bf16)P[kOffW1 + mlp_perm_unit(c >> 5, c & 31) * kMlpIn + in] : (__bf16)0.0f;
+    }
 }
 
 __device__ __forceinline__ void stage_member(const unsigned char* __restrict__ src, unsigned char* dst) {

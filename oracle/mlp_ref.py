@@ -82,7 +82,8 @@ def forward(P, xn, bf=False):
 def jacobian(P, xn, bf=False):
     """d o / d xn, shape (n, 3, 6), in reverse mode (the device's formulation): per output o,
     delta2 = W3[o] * (1 - h2^2), delta1 = (W2^T delta2) * (1 - h1^2), d o / d xn = W1^T delta1.
-    With bf16=True, delta2 is rounded to BF16 before the W2^T product and W3, W2 are the BF16 weights."""
+    With bf16=True, delta2 and delta1 are rounded to BF16 before their products and W3, W2, W1 are the
+    BF16 weights (the device runs both products on BF16 MFMA)."""
     f = forward(P, xn, bf)
     s1 = 1.0 - f["h1"] ** 2
     s2 = 1.0 - f["h2"] ** 2
@@ -93,7 +94,9 @@ def jacobian(P, xn, bf=False):
         if bf:
             d2 = bf16(d2).astype(np.float64)
         d1 = (d2 @ f["W2"]) * s1
-        J[:, o, :] = d1 @ W1
+        if bf:
+            d1 = bf16(d1).astype(np.float64)
+        J[:, o, :] = d1 @ (bf16(P["W1"]).astype(np.float64) if bf else W1)
     return J
 
 
