@@ -1,5 +1,6 @@
 // mlp.h -- the MLP surrogate of a two-state diabatic Hamiltonian: 6 -> 128 -> 128 -> 3, tanh,
-// BF16 MFMA (16x16x32, FP32 accumulation) for the 128-wide products, FP32 first layer.
+// BF16 MFMA (16x16x32, FP32 accumulation) for the 128-wide products, FP32 MFMA (16x16x4) for the
+// first layer.
 //
 // Inputs: the six-number descriptor of a diatom configuration x = (x1, y1, z1, x2, y2, z2),
 //   q = (b, z1, z2, cos(theta), X, Y),  b = |r1 - r2|,  cos(theta) = (z1 - z2) / b,
@@ -37,11 +38,12 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH, kOffW3 = kOffB2 + kMlpH, kOffB3 =
 //   [37440, 41536) bf16 W3p[16][128], rows 3..15 zero, columns permuted
 //   [41536, 74304) bf16 W2Tp[128][128] = W2^T, columns in the permuted K order (input gradients)
 //   [74304, 78400) bf16 W1Tp[16][128] = W1^T, rows 6..15 zero, columns permuted (input gradients)
+//   [78400, 79936) float W3f[3][128], columns permuted (delta2 of the input gradient)
 // In the four BF16 matrices the 16-byte chunks of each row are XOR-swizzled by the row index
 // (mlp_swz): a fragment read has the 16 lanes of a lane group on rows 16 t + j, one chunk each, and
 // unswizzled rows 256 bytes apart would put all 16 in the same LDS banks.
 constexpr int kPkW1B = 0, kPkB2 = 4096, kPkB3 = 4608, kPkW2 = 4672, kPkW3 = 37440, kPkW2T = 41536, kPkW1T = 74304,
-              kPkBytes = 78400;
+              kPkW3F = 78400, kPkBytes = 79936;
 
 struct MlpNorm {
     float in_shift[kMlpIn], in_scale[kMlpIn];
@@ -91,12 +93,13 @@ __host__ __device__ constexpr int mlp_perm_unit(int q, int k) {
     return 32 * q + ((k & 7) < 4 ? 4 * (k >> 3) + (k & 7) : 16 + 4 * (k >> 3) + (k & 7) - 4);
 }
 
-// Hides a pointer's value from the optimiser at this point, so loads through it are issued here
-// and not hoisted out of the surrounding loop (which would keep every weight fragment live).
-template <class T>
-__device__ __forceinline__ const T* mlp_launder(const T* p) {
-    asm volatile("" : "+s"(p));
-    return p;
+// A zero the optimiser cannot see through: offsetting a pointer by it at a point makes the loads
+// through that pointer issue there rather than be hoisted out of the surrounding loop (which would
+// keep every weight fragment live), while the pointer keeps its LDS address space.
+__device__ __forceinline__ int mlp_opaque_zero() {
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    return z;
 }
 
 // tanh(a) = 1 - 2 / (e^{2a} + 1) with the hardware exponential and reciprocal (about 1 ulp each,
@@ -131,17 +134,19 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
     const float* b3 = reinterpret_cast<const float*>(W + kPkB3);
     const bf16x8* W2p = reinterpret_cast<const bf16x8*>(W + kPkW2);
     const bf16x8* W3p = reinterpret_cast<const bf16x8*>(W + kPkW3);
-    // layer 1 in FP32
+    // layer 1 on FP32 MFMA (16x16x4): units along M, the augmented input (x0..x5, 1, 0) along K in
+    // two steps of 4, samples along N; lane (g, j) supplies W1aug[16 t + j][4 ks + g] and input
+    // 4 ks + g of sample j, and receives units 16 t + 4 g + r -- the layout of h1 below
+    const float xb0 = (g == 0) ? xs[0] : (g == 1) ? xs[1] : (g == 2) ? xs[2] : xs[3];
+    const float xb1 = (g == 0) ? xs[4] : (g == 1) ? xs[5] : (g == 2) ? 1.0f : 0.0f;
     float h1[32];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
+        f32x4 a = f32x4{0.f, 0.f, 0.f, 0.f};
+        a = __builtin_amdgcn_mfma_f32_16x16x4f32(w1b[8 * (16 * t + j) + g], xb0, a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x4f32(w1b[8 * (16 * t + j) + 4 + g], xb1, a, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const float4 wa = *reinterpret_cast<const float4*>(w1b + 8 * (16 * t + 4 * g + r));
-            const float4 wb = *reinterpret_cast<const float4*>(w1b + 8 * (16 * t + 4 * g + r) + 4);
-            const float a = wb.z + wa.x * xs[0] + wa.y * xs[1] + wa.z * xs[2] + wa.w * xs[3] + wb.x * xs[4] + wb.y * xs[5];
-            h1[4 * t + r] = mlp_tanh(a);
-        }
+        for (int r = 0; r < 4; ++r) h1[4 * t + r] = mlp_tanh(a[r]);
     }
     // layer 2 and the output layer; with kGrad, delta2 of each output in the B layout, built as
     // each K step's units come out: lane (g, j), K step q, e < 4: unit 32 q + 4 g + e, e >= 4:
@@ -175,13 +180,11 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
             if constexpr (kGrad) {
 #pragma unroll
                 for (int oo = 0; oo < kMlpOut; ++oo) {
-                    const bf16x8 w3 = W3p[mlp_frag(oo, 4 * q3 + g)];
-                    float lo[4], hi[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        lo[e] = (float)w3[e] * sv[0][e];
-                        hi[e] = (float)w3[4 + e] * sv[1][e];
-                    }
+                    const float* w3f = reinterpret_cast<const float*>(W + kPkW3F) + oo * kMlpH + 32 * q3 + 8 * g;
+                    const float4 wa = *reinterpret_cast<const float4*>(w3f);
+                    const float4 wb = *reinterpret_cast<const float4*>(w3f + 4);
+                    const float lo[4] = {wa.x * sv[0][0], wa.y * sv[0][1], wa.z * sv[0][2], wa.w * sv[0][3]};
+                    const float hi[4] = {wb.x * sv[1][0], wb.y * sv[1][1], wb.z * sv[1][2], wb.w * sv[1][3]};
                     D[oo][q3] = pack8(lo, hi);
                 }
             }
@@ -205,7 +208,7 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
         for (int oo = 0; oo < kMlpOut; ++oo) gacc[oo] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const bf16x8* W2Tq = mlp_launder(W2Tp);
+            const bf16x8* W2Tq = W2Tp + mlp_opaque_zero();
             float v[kMlpOut][2][4];
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
@@ -227,7 +230,7 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* __restrict__ W, co
                     for (int oo = 0; oo < kMlpOut; ++oo) v[oo][half][r] = acc[oo][r] * s1;
                 }
             }
-            const bf16x8 A1 = mlp_launder(W1Tp)[mlp_frag(j, 4 * q + g)];
+            const bf16x8 A1 = (W1Tp + mlp_opaque_zero())[mlp_frag(j, 4 * q + g)];
 #pragma unroll
             for (int oo = 0; oo < kMlpOut; ++oo) gacc[oo] = mfma_bf16(A1, pack8(v[oo][0], v[oo][1]), gacc[oo]);
         }

@@ -49,6 +49,11 @@ __global__ __launch_bounds__(256) void mlp_pack(const float* __restrict__ P, uns
         const int u = i >> 7, c = i & 127;
         W2Tp[mlp_swz(u, c)] = (__bf16)P[kOffW2 + mlp_perm_unit(c >> 5, c & 31) * kMlpH + u];
     }
+    float* W3f = reinterpret_cast<float*>(out + kPkW3F);
+    for (int i = tid; i < kMlpOut * kMlpH; i += B) {
+        const int o = i >> 7, c = i & 127;
+        W3f[i] = (float)(__bf16)P[kOffW3 + o * kMlpH + mlp_perm_unit(c >> 5, c & 31)];  // the BF16 value, held as FP32
+    }
     __bf16* W1Tp = reinterpret_cast<__bf16*>(out + kPkW1T);
     for (int i = tid; i < 16 * kMlpH; i += B) {
         const int in = i >> 7, c = i & 127;
@@ -455,6 +460,7 @@ int32_t mlp_eval(nqcd_handle_t* h, int32_t member, const double* X, int64_t n, d
     hipError_t e = hipMemcpy(xb.p, xf.data(), sizeof(float) * xf.size(), hipMemcpyHostToDevice);
     const int waves = 4;
     const int64_t grid = (n + 64 * waves - 1) / (64 * waves);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mlp_eval_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kPkBytes);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(mlp_eval_kernel, dim3((unsigned)grid), dim3(64 * waves), kPkBytes, h->stream,
                            h->mlp_packed.as<unsigned char>() + (size_t)member * kPkBytes, h->mlp_norm, n, xb.as<float>(),
