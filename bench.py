@@ -106,7 +106,8 @@ def iesh_extra(L):
     return res
 
 
-TRAIN_STEPS = 20000
+TRAIN_STEPS = 40000
+TRAIN_BATCH = 2048
 
 
 def surrogate_extra(L):
@@ -149,7 +150,7 @@ def surrogate_extra(L):
     keep = np.arange(vis["configs"].shape[1])[None, :] * 20 <= vis["steps"][:, None]
     X = vis["configs"][keep]
     t0 = time.perf_counter()
-    model, hist = NQ.train_surrogate(analytic, X, steps=TRAIN_STEPS, batch=1024, lr=3e-3, lr_final=1e-5, threshold=1e-3)
+    model, hist = NQ.train_surrogate(analytic, X, steps=TRAIN_STEPS, batch=TRAIN_BATCH, lr=3e-3, lr_final=1e-5, threshold=1e-3)
     t_train = time.perf_counter() - t0
     _progress(f"surrogate trained on {len(X)} configurations in {t_train:.1f} s")
     r1, v1 = inputs(1000000, 43)
@@ -162,7 +163,7 @@ def surrogate_extra(L):
     # three reverse-mode 128x128 products for the input gradient)
     flop = 2 * (4 * 2 * 128 * 128 + 2 * 16 * 128)
     return {"workload": "diatom-surface FSSH, 6 coordinates, 1e6 trajectories, dt 2, t_max 1e4, terminate z1 > 8",
-            "training_configurations": int(len(X)), "training_steps": TRAIN_STEPS, "training_s": t_train,
+            "training_configurations": int(len(X)), "training_steps": TRAIN_STEPS, "training_batch": TRAIN_BATCH, "training_s": t_train,
             "final_loss": [float(hist[0, -100:].mean()), float(hist[1, -100:].mean())],
             "surrogate_traj_steps_per_s": rate(sur), "analytic_traj_steps_per_s": rate(ref),
             "surrogate_kernel_ms": sur["stats"]["kernel_ms"], "analytic_kernel_ms": ref["stats"]["kernel_ms"],

@@ -145,7 +145,7 @@ def test_surrogate_trained_on_fssh_configurations_reproduces_populations(key):
     keep = np.arange(nsave)[None, :] * 20 <= vis["steps"][:, None]
     X = vis["configs"][keep]
     assert len(X) > 200000
-    model, hist = N.train_surrogate(analytic, X, steps=20000, batch=1024, lr=3e-3, lr_final=1e-5, threshold=1e-3)
+    model, hist = N.train_surrogate(analytic, X, steps=40000, batch=2048, lr=3e-3, lr_final=1e-5, threshold=1e-3)
     print(f"training loss: first {hist[:, :10].mean():.4g}  last {hist[:, -100:].mean():.4g}")
     assert hist[:, -100:].mean() < 2e-3 * hist[:, :10].mean()
     # held-out accuracy of the ensemble mean
@@ -157,7 +157,7 @@ def test_surrogate_trained_on_fssh_configurations_reproduces_populations(key):
     E.close()
     err = np.sqrt(np.mean((Hm - analytic.potential(Xt)) ** 2, axis=0))
     print("held-out RMS error of (H11, H22, H12):", err)
-    assert np.all(err < 2e-3), err
+    assert np.all(err < 1e-3), err
     # FSSH on new initial conditions
     r1, v1 = _inputs(20000, 43)
     ref = _fssh(L, analytic, r1, v1, key)
