@@ -1,32 +1,57 @@
 # EnsembleMI355X.jl -- the Julia side of the MI355X ensemble backend.
 #
 # An ensemble algorithm for SciMLBase.solve(::EnsembleProblem, alg, ::EnsembleMI355X; ...)
-# (the call at src/Ensembles/run_dynamics.jl:91-97) that runs every trajectory of a
-# supported Simulation on one MI355X through libnqcd_mi355x.so (include/nqcd_mi355x.h).
+# (the call at src/Ensembles/run_dynamics.jl:91-97) that runs every trajectory of a supported
+# simulation on one MI355X through libnqcd_mi355x.so (include/nqcd_mi355x.h):
 #
-# Not executed in this repository (no Julia toolchain in the build image); the same
-# C ABI is exercised by the Python mirror and tests.  Load it next to NQCDynamics:
+#   Simulation{FSSH} / Simulation{Ehrenfest}                      BABwithTsit5
+#   Simulation{AdiabaticIESH} / Simulation{EhrenfestNA}           VerletwithElectronics
+#   RingPolymerSimulation{Classical}                              BCB
+#   RingPolymerSimulation{FSSH} / RingPolymerSimulation{Ehrenfest} BCBwithTsit5
+#
+# Not executed in this repository (no Julia toolchain in the build image); the same C ABI is
+# exercised by the Python mirror (nqcd_mi355x/) and its tests.  Load it next to NQCDynamics:
 #
 #     include("EnsembleMI355X.jl"); using .EnsembleMI355XBackend
 #     run_dynamics(sim, tspan, distribution; output, reduction=MeanReduction(),
 #                  ensemble_algorithm=EnsembleMI355X(), trajectories=10^6, dt=1.0)
+#
+# The Tully models map to device models automatically.  For the other device models (spin-boson,
+# double well with Gaussian coupling, harmonic, Newns-Anderson, diatom-surface, MLP surrogate)
+# pass the parameter vector of include/nqcd_mi355x.h explicitly:
+#     EnsembleMI355X(device_model=(MODEL_NEWNS_ANDERSON, [m, omega, g, dG, Gamma, a, b, M]))
 module EnsembleMI355XBackend
 
 using SciMLBase
 using NQCDynamics
-using NQCDynamics: DynamicsMethods, Ensembles
+using NQCDynamics: DynamicsMethods, DynamicsUtils, Ensembles, RingPolymers
+using NQCDynamics.DynamicsMethods: SurfaceHoppingMethods, EhrenfestMethods, ClassicalMethods, IntegrationAlgorithms
 using NQCModels
 
 export EnsembleMI355X
 
 const libnqcd = get(ENV, "NQCD_MI355X_LIB", joinpath(@__DIR__, "..", "libnqcd_mi355x.so"))
 
+# --- codes of include/nqcd_mi355x.h ------------------------------------------------------------
+const METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST = Int32.(0:6)
+const ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5 = Int32.(0:3)
+const MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC,
+      MODEL_NEWNS_ANDERSON, MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE = Int32.(0:8)
+const FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS,
+      FIELD_SAVE_POP_AD, FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID = Int32.(0:10)
+const OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION,
+      OUT_FINAL_ADIABATIC_POPULATION, OUT_CENTROID_POSITION = Int32.(0:5)
+const RED_MEAN, RED_SUM = Int32(0), Int32(1)
+
 struct EnsembleMI355X <: SciMLBase.EnsembleAlgorithm
     device::Int32
-    persistent::Bool
-    seed::UInt64
+    persistent::Bool        # persistent work-queue kernel (false: one launch per step)
+    fp32_nuclei::Bool       # FP32 nuclei / FP64 electronics (false: FP64 throughout)
+    seed::UInt64            # Philox key of the hop and sampling streams
+    device_model::Any       # nothing, or (kind::Integer, params::Vector{Float64})
 end
-EnsembleMI355X(; device=0, persistent=true, seed=0x4E5143444D493335) = EnsembleMI355X(device, persistent, seed)
+EnsembleMI355X(; device=0, persistent=true, fp32_nuclei=false, seed=0x4E5143444D493335, device_model=nothing) =
+    EnsembleMI355X(device, persistent, fp32_nuclei, seed, device_model)
 
 # --- C structs (layout of include/nqcd_mi355x.h) -------------------------------------------
 struct ProblemDesc
@@ -38,63 +63,182 @@ end
 struct TermDesc
     kind::Int32; dof::Int32; value::Float64
 end
-struct RunStats
-    kernel_ms::Float64; wall_ms::Float64; traj_steps::Int64; launches::Int64; block::Int32; grid::Int32
-end
 
 check(h, rc) = rc == 0 || error("nqcd_mi355x: ", unsafe_string(ccall((:nqcd_mi355x_last_error, libnqcd), Cstring, (Ptr{Cvoid},), h)))
 
-method_code(::NQCDynamics.SurfaceHoppingMethods.FSSH) = Int32(0)
-method_code(::NQCDynamics.EhrenfestMethods.Ehrenfest) = Int32(1)
-method_code(m) = throw(ArgumentError("EnsembleMI355X does not support $(typeof(m))"))
+# --- simulation -> problem ----------------------------------------------------------------
+unsupported(x) = throw(ArgumentError("EnsembleMI355X does not support $(typeof(x))"))
+
+method_code(sim::NQCDynamics.Simulation) = method_code(sim.method)
+method_code(::SurfaceHoppingMethods.FSSH) = METHOD_FSSH
+method_code(::EhrenfestMethods.Ehrenfest) = METHOD_EHRENFEST
+method_code(::SurfaceHoppingMethods.AdiabaticIESH) = METHOD_IESH
+method_code(::EhrenfestMethods.EhrenfestNA) = METHOD_EHRENFEST_NA
+method_code(m) = unsupported(m)
+function method_code(sim::NQCDynamics.RingPolymerSimulation)
+    m = sim.method
+    m isa ClassicalMethods.Classical && return METHOD_RPMD
+    m isa SurfaceHoppingMethods.FSSH && return METHOD_RPSH
+    m isa EhrenfestMethods.Ehrenfest && return METHOD_RP_EHRENFEST
+    unsupported(m)
+end
+
+algorithm_code(::IntegrationAlgorithms.BABwithTsit5) = ALG_BAB_TSIT5
+algorithm_code(::IntegrationAlgorithms.VerletwithElectronics) = ALG_VERLET_ELECTRONICS
+algorithm_code(::IntegrationAlgorithms.BCB) = ALG_BCB
+algorithm_code(::IntegrationAlgorithms.BCBwithTsit5) = ALG_BCB_TSIT5
+algorithm_code(a) = unsupported(a)
+
+# surface_hopping.jl:65,80-91
 rescaling_code(m) = hasproperty(m, :rescaling) ? Int32(findfirst(==(m.rescaling), (:standard, :vinversion, :off)) - 1) : Int32(0)
 
-model_code(m::NQCModels.TullyModelOne) = (Int32(0), Float64[m.A, m.B, m.C, m.D])
-model_code(m::NQCModels.TullyModelTwo) = (Int32(1), Float64[m.A, m.B, m.C, m.D, m.E0])
-model_code(m::NQCModels.TullyModelThree) = (Int32(2), Float64[m.A, m.B, m.C])
-model_code(m) = throw(ArgumentError("EnsembleMI355X has no device model for $(typeof(m))"))
+model_code(m::NQCModels.TullyModelOne) = (MODEL_TULLY1, Float64[m.A, m.B, m.C, m.D])
+model_code(m::NQCModels.TullyModelTwo) = (MODEL_TULLY2, Float64[m.A, m.B, m.C, m.D, m.E0])
+model_code(m::NQCModels.TullyModelThree) = (MODEL_TULLY3, Float64[m.A, m.B, m.C])
+model_code(m) = throw(ArgumentError("no automatic device model for $(typeof(m)): pass " *
+                                    "EnsembleMI355X(device_model=(kind, params)) (include/nqcd_mi355x.h)"))
 
+function problem_desc(sim, alg, dt, ensemblealg, masses)
+    method = method_code(sim)
+    ie = method in (METHOD_IESH, METHOD_EHRENFEST_NA)
+    rp = sim isa NQCDynamics.RingPolymerSimulation
+    dec, C = Int32(0), 0.1
+    if method == METHOD_IESH && sim.method.decoherence isa SurfaceHoppingMethods.DecoherenceCorrectionEDC
+        dec, C = Int32(1), Float64(sim.method.decoherence.C)   # decoherence_corrections.jl:10-19
+    end
+    est = method == METHOD_IESH && !sim.method.estimate_probability ? Int32(1) : Int32(0)
+    ProblemDesc(method, algorithm_code(alg), NQCModels.nstates(sim), size(sim)[1], size(sim)[2],
+        rp ? RingPolymers.nbeads(sim) : 1, ie ? NQCModels.nelectrons(sim) : 0, rescaling_code(sim.method),
+        ensemblealg.fp32_nuclei ? 1 : 0, ensemblealg.persistent ? 0 : 1, 5, dec, pointer(masses), dt,
+        Float64(NQCDynamics.get_temperature(sim)), C, est, ntuple(_ -> Int32(0), 7))
+end
+
+# --- initial conditions from the Selection closure (selections.jl:38-42,70-73) --------------
+function load_initial!(h, prob, sim, trajectories)
+    us = [prob.prob_func(prob.prob, i, false).u0 for i in 1:trajectories]
+    r = reduce(hcat, [vec(DynamicsUtils.get_positions(u)) for u in us])
+    v = reduce(hcat, [vec(DynamicsUtils.get_velocities(u)) for u in us])
+    if method_code(sim) == METHOD_RPMD
+        check(h, ccall((:nqcd_mi355x_load_initial, libnqcd), Int32,
+            (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
+            h, trajectories, r, v, C_NULL, C_NULL, C_NULL, C_NULL))
+        return
+    end
+    # density matrix (n, n) for FSSH / Ehrenfest, wavefunction psi (n, n_electrons) for IESH /
+    # EhrenfestNA; the discrete state is the active state or the occupied orbitals (1-based)
+    q = [DynamicsUtils.get_quantum_subsystem(u) for u in us]
+    σr = cat([real.(Matrix(x)) for x in q]...; dims=3); σi = cat([imag.(Matrix(x)) for x in q]...; dims=3)
+    st = reduce(hcat, [Int32.(round.(Int, vec(u.state))) .- Int32(1) for u in us])
+    check(h, ccall((:nqcd_mi355x_load_initial, libnqcd), Int32,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
+        h, trajectories, r, v, σr, σi, st, C_NULL))
+end
+
+download(h, field, T, dims...) = (a = Array{T}(undef, dims...);
+    check(h, ccall((:nqcd_mi355x_download, libnqcd), Int32, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64), h, field, a, sizeof(a))); a)
+device_reduce(h, kind, red, len) = (a = Vector{Float64}(undef, len);
+    check(h, ccall((:nqcd_mi355x_reduce, libnqcd), Int32, (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int64), h, kind, red, a, len)); a)
+
+# --- outputs (DynamicsOutputs.jl, TimeCorrelationFunctions.jl) from the device fields ---------
+output_name(f) = f isa Function ? nameof(f) : nameof(typeof(f))
+device_output_kind(f) = Dict(:OutputAdiabaticPopulation => OUT_ADIABATIC_POPULATION,
+    :OutputDiabaticPopulation => OUT_DIABATIC_POPULATION, :OutputCentroidPosition => OUT_CENTROID_POSITION,
+    :OutputStateResolvedScattering1D => OUT_SCATTERING_1D, :PopulationCorrelationFunction => OUT_POPULATION_CORRELATION)[output_name(f)]
+
+struct DeviceResults
+    n::Int; nsave::Int; sz::Tuple
+    r::Array{Float64}; v::Array{Float64}; state::Array{Int32}; steps::Vector{Int32}; nhops::Vector{Int32}
+    pop_ad::Array{Float64,3}; pop_di::Array{Float64,3}; save_state::Matrix{Int32}
+end
+
+function per_trajectory(f, d::DeviceResults, i, sim, t0, dt)
+    name = output_name(f)
+    name === :OutputAdiabaticPopulation && return [d.pop_ad[:, s, i] for s in 1:d.nsave]
+    name === :OutputDiabaticPopulation && return [d.pop_di[:, s, i] for s in 1:d.nsave]
+    name === :OutputDiscreteState && return [Int(d.save_state[s, i]) + 1 for s in 1:d.nsave]
+    name === :OutputSurfaceHops && return Int(d.nhops[i])
+    name === :OutputFinalTime && return t0 + d.steps[i] * dt
+    name === :PopulationCorrelationFunction && return [d.pop_di[:, 1, i] * d.pop_di[:, s, i]' for s in 1:d.nsave]
+    if name === :OutputStateResolvedScattering1D            # DynamicsOutputs.jl:313-338
+        p = d.pop_ad[:, d.nsave, i]; z = zero(p)
+        return first(reshape(d.r[:, i], d.sz)) > 0 ? (reflection=z, transmission=p) : (reflection=p, transmission=z)
+    end
+    throw(ArgumentError("EnsembleMI355X has no device value for output $name"))
+end
+
+# --- the ensemble algorithm ----------------------------------------------------------------
 function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::EnsembleMI355X;
         trajectories, dt=1.0, saveat=nothing, callback=nothing, kwargs...)
     sim = prob.prob.p
-    sim isa NQCDynamics.Simulation || throw(ArgumentError("EnsembleMI355X needs a Simulation"))
+    sim isa NQCDynamics.AbstractSimulation || throw(ArgumentError("EnsembleMI355X needs a Simulation"))
+    isempty(kwargs) || throw(ArgumentError("EnsembleMI355X does not take $(keys(kwargs))"))
+    outputs = prob.output_func.functions       # EnsembleSaver (Ensembles.jl:39-63)
+    savetime = prob.output_func.savetime
     tspan = prob.prob.tspan
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(C_NULL, ccall((:nqcd_mi355x_open, libnqcd), Int32, (Int32, Ref{Ptr{Cvoid}}), ensemblealg.device, h))
     try
         masses = collect(Float64, sim.atoms.masses)
-        d = ProblemDesc(method_code(sim.method), 0, NQCModels.nstates(sim), size(sim)[1], size(sim)[2], 1, 0,
-            rescaling_code(sim.method), 0, ensemblealg.persistent ? 0 : 1, 5, 0, pointer(masses), dt, 0.0, 0.1, 0,
-            ntuple(_ -> 0, 7))
+        d = problem_desc(sim, alg, dt, ensemblealg, masses)
         GC.@preserve masses check(h[], ccall((:nqcd_mi355x_set_problem, libnqcd), Int32, (Ptr{Cvoid}, Ref{ProblemDesc}), h[], d))
-        kind, params = model_code(sim.cache.model)
+        kind, params = ensemblealg.device_model === nothing ? model_code(sim.cache.model) : ensemblealg.device_model
         check(h[], ccall((:nqcd_mi355x_set_model, libnqcd), Int32, (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int64),
-            h[], 0, kind, params, length(params)))
-        # initial conditions from the Selection closure (selections.jl:38-42,70-73)
-        us = [prob.prob_func(prob.prob, i, 1).u0 for i in 1:trajectories]
-        r = reduce(hcat, [vec(u.r) for u in us]); v = reduce(hcat, [vec(u.v) for u in us])
-        σr = cat([Matrix(u.σreal) for u in us]...; dims=3); σi = cat([Matrix(u.σimag) for u in us]...; dims=3)
-        st = Int32[round(Int32, first(u.state)) - 1 for u in us]
-        check(h[], ccall((:nqcd_mi355x_load_initial, libnqcd), Int32,
-            (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
-            h[], trajectories, r, v, σr, σi, st, C_NULL))
-        save_every = saveat === nothing ? 0 : round(Int32, step(saveat) / dt)
+            h[], 0, kind, collect(Float64, params), length(params)))
+        load_initial!(h[], prob, sim, trajectories)
+        save_every = saveat === nothing ? Int32(0) : round(Int32, step(saveat) / dt)
+        # TerminatingCallback with the |x_dof| > value predicate
         term = callback === nothing ? C_NULL : Ref(TermDesc(1, callback.dof - 1, callback.value))
         # long device call: do not hold the Julia GC
         rc = @threadcall((:nqcd_mi355x_run, libnqcd), Int32, (Ptr{Cvoid}, Float64, Float64, Int32, Ptr{TermDesc}, UInt64),
             h[], tspan[1], tspan[2], save_every, term, ensemblealg.seed)
         check(h[], rc)
-        n = NQCModels.nstates(sim); nsave = save_every == 0 ? 1 : div(round(Int, (tspan[2] - tspan[1]) / dt), save_every) + 1
-        popad = Array{Float64}(undef, n, nsave, trajectories)
-        check(h[], ccall((:nqcd_mi355x_download, libnqcd), Int32, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64),
-            h[], 7, popad, sizeof(popad)))
-        rf = similar(r); check(h[], ccall((:nqcd_mi355x_download, libnqcd), Int32, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64), h[], 0, rf, sizeof(rf)))
-        stf = Vector{Int32}(undef, trajectories)
-        check(h[], ccall((:nqcd_mi355x_download, libnqcd), Int32, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64), h[], 4, stf, sizeof(stf)))
-        # per-trajectory dictionaries reduced with the problem's reduction (Ensembles.jl:39-63)
-        batch = [Ensembles.Dictionary{Symbol,Any}([:Time, :OutputFinalState, :OutputFinalPosition],
-                    [collect(tspan), stf[i] + 1, rf[:, i]]) for i in 1:trajectories]
-        u, _ = prob.reduction(prob.u_init === nothing ? [] : prob.u_init, batch, 1:trajectories)
+        n = NQCModels.nstates(sim)
+        nsave = save_every == 0 ? 1 : div(round(Int, (tspan[2] - tspan[1]) / dt), save_every) + 1
+        times = save_every == 0 ? collect(tspan) : collect(tspan[1] .+ dt * save_every .* (0:nsave - 1))
+        red = prob.reduction
+        if (red isa Ensembles.MeanReduction || red isa Ensembles.SumReduction) &&
+                all(f -> haskey(Dict(:OutputAdiabaticPopulation => 1, :OutputDiabaticPopulation => 1,
+                    :OutputCentroidPosition => 1, :OutputStateResolvedScattering1D => 1, :PopulationCorrelationFunction => 1),
+                    output_name(f)), outputs)
+            # reduced on the device in a fixed order (reductions.jl:8-52)
+            rk = red isa Ensembles.MeanReduction ? RED_MEAN : RED_SUM
+            out = Ensembles.Dictionary{Symbol,Any}()
+            savetime && insert!(out, :Time, times)
+            for f in outputs
+                k = device_output_kind(f)
+                if k == OUT_SCATTERING_1D
+                    a = device_reduce(h[], k, rk, 2n); insert!(out, output_name(f), (reflection=a[1:n], transmission=a[n+1:2n]))
+                elseif k == OUT_CENTROID_POSITION
+                    insert!(out, output_name(f), device_reduce(h[], k, rk, nsave))
+                elseif k == OUT_POPULATION_CORRELATION
+                    a = reshape(device_reduce(h[], k, rk, n * n * nsave), n, n, nsave)
+                    insert!(out, output_name(f), [a[:, :, s] for s in 1:nsave])
+                else
+                    a = reshape(device_reduce(h[], k, rk, n * nsave), n, nsave)
+                    insert!(out, output_name(f), [a[:, s] for s in 1:nsave])
+                end
+            end
+            return SciMLBase.EnsembleSolution([out], 0.0, true)
+        end
+        # every other reduction: the per-trajectory dictionaries EnsembleSaver would have made,
+        # handed to the problem's own reduction functor (Sort*, FileReduction, ...)
+        ie = d.method in (METHOD_IESH, METHOD_EHRENFEST_NA)
+        nst = ie ? d.n_electrons : 1
+        res = DeviceResults(n, nsave, size(sim),
+            download(h[], FIELD_R, Float64, prod(size(sim)), trajectories), download(h[], FIELD_V, Float64, prod(size(sim)), trajectories),
+            download(h[], FIELD_STATE, Int32, nst, trajectories), download(h[], FIELD_STEPS, Int32, trajectories),
+            download(h[], FIELD_NHOPS, Int32, trajectories), download(h[], FIELD_SAVE_POP_AD, Float64, n, nsave, trajectories),
+            download(h[], FIELD_SAVE_POP_DI, Float64, n, nsave, trajectories),
+            ie ? zeros(Int32, nsave, trajectories) : download(h[], FIELD_SAVE_STATE, Int32, nsave, trajectories))
+        batch = map(1:trajectories) do i
+            out = Ensembles.Dictionary{Symbol,Any}()
+            savetime && insert!(out, :Time, times)
+            for f in outputs
+                insert!(out, output_name(f), per_trajectory(f, res, i, sim, tspan[1], dt))
+            end
+            out
+        end
+        u, _ = red(prob.u_init === nothing ? [] : prob.u_init, batch, 1:trajectories)
         return SciMLBase.EnsembleSolution(u, 0.0, true)
     finally
         ccall((:nqcd_mi355x_close, libnqcd), Cvoid, (Ptr{Cvoid},), h[])
