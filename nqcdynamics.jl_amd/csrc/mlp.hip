@@ -20,10 +20,15 @@
 namespace nqcd {
 namespace {
 
-constexpr int kTrainSamples = 32;  // samples per training wavefront (two sample tiles)
+constexpr int kTrainSamples = 32;  // samples per training workgroup (two sample tiles)
+constexpr int kTrainThreads = 128; // one wavefront per sample tile
+static_assert(kTrainThreads / 64 == kTrainSamples / 16, "one wavefront per 16-sample tile");
 constexpr int kPackBlocks = 64;
 
+// Packs member blockIdx.y: P and out advance by one member per blockIdx.y.
 __global__ __launch_bounds__(256) void mlp_pack(const float* __restrict__ P, unsigned char* __restrict__ out) {
+    P += (size_t)blockIdx.y * kMlpParams;
+    out += (size_t)blockIdx.y * kPkBytes;
     float* w1b = reinterpret_cast<float*>(out + kPkW1B);
     float* b2 = reinterpret_cast<float*>(out + kPkB2);
     float* b3 = reinterpret_cast<float*>(out + kPkB3);
@@ -101,8 +106,9 @@ __global__ __launch_bounds__(256) void mlp_eval_kernel(const unsigned char* __re
     }
 }
 
-// One wavefront per 32 samples of the minibatch: partial gradient (kMlpParams floats, master
-// order) and partial loss of those samples.
+// One workgroup of two wavefronts per 32 samples of the minibatch: partial gradient (kMlpParams
+// floats, master order) and partial loss of those samples.  Each wavefront takes one 16-sample
+// tile through the network, then the two split the parameter sums over the 32 samples.
 struct TrainLds {
     unsigned char member[kPkBytes];        // the packed member, W2^T included
     float W3[kMlpOut * kMlpH];
@@ -115,19 +121,29 @@ struct TrainLds {
     float loss[kTrainSamples];
 };
 
-__global__ __launch_bounds__(64) void mlp_train_grad(const unsigned char* __restrict__ packed, const float* __restrict__ P,
+// Member blockIdx.y (the members train side by side in one launch): its packed copy, master
+// parameters, minibatch indices (idx + member * idx_stride), partial gradients and losses.
+__global__ __launch_bounds__(128) void mlp_train_grad(const unsigned char* __restrict__ packed, const float* __restrict__ P,
                                                      const float* __restrict__ Xn, const float* __restrict__ Tn,
-                                                     const int32_t* __restrict__ idx, int32_t nb, float w0, float w1,
-                                                     float w2, float* __restrict__ partial, float* __restrict__ loss_part) {
+                                                     const int32_t* __restrict__ idx, int64_t idx_stride, int32_t nb,
+                                                     float w0, float w1, float w2, float* __restrict__ partial,
+                                                     float* __restrict__ loss_part) {
+    const int mem = blockIdx.y;
+    packed += (size_t)mem * kPkBytes;
+    P += (size_t)mem * kMlpParams;
+    idx += mem * idx_stride;
+    partial += (size_t)mem * gridDim.x * kMlpParams;
+    loss_part += (size_t)mem * gridDim.x;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TrainLds& S = *reinterpret_cast<TrainLds*>(lds_raw);
-    const int l = threadIdx.x, g = l >> 4, j = l & 15;
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, g = l >> 4, j = l & 15;
     stage_member(packed, S.member);
-    for (int i = l; i < kMlpOut * kMlpH; i += 64) S.W3[i] = P[kOffW3 + i];
+    for (int i = tid; i < kMlpOut * kMlpH; i += kTrainThreads) S.W3[i] = P[kOffW3 + i];
     __syncthreads();
     const float wl[3] = {w0, w1, w2};
     const int g0 = blockIdx.x * kTrainSamples;
-    for (int T = 0; T < kTrainSamples / 16; ++T) {
+    {
+        const int T = wv;  // this wavefront's sample tile
         const int ls = 16 * T + j;                  // local sample of this lane's column
         const bool valid = g0 + ls < nb;
         const int s = valid ? idx[g0 + ls] : 0;
@@ -192,7 +208,7 @@ __global__ __launch_bounds__(64) void mlp_train_grad(const unsigned char* __rest
     float* out = partial + (int64_t)blockIdx.x * kMlpParams;
     // dW2 = delta2 h1q^T on MFMA, K = the 32 samples: A = delta2 [v][s], B = h1q [u][s]
 #pragma unroll 1
-    for (int vt = 0; vt < 8; ++vt) {
+    for (int vt = wv; vt < 8; vt += kTrainThreads / 64) {
         float av[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) av[e] = S.d2[(16 * vt + j) * kTrainSamples + 8 * g + e];
@@ -206,49 +222,70 @@ __global__ __launch_bounds__(64) void mlp_train_grad(const unsigned char* __rest
         }
     }
     // FP32 sums over the samples for the small parameter blocks
-    for (int p = l; p < kMlpOut * kMlpH; p += 64) {
+    for (int p = tid; p < kMlpOut * kMlpH; p += kTrainThreads) {
         const int o3 = p / kMlpH, u = p % kMlpH;
         float a = 0.0f;
         for (int s = 0; s < kTrainSamples; ++s) a += S.d3[s * kMlpOut + o3] * S.h2q[u * kTrainSamples + s];
         out[kOffW3 + p] = a;
     }
-    for (int p = l; p < kMlpH * kMlpIn; p += 64) {
+    for (int p = tid; p < kMlpH * kMlpIn; p += kTrainThreads) {
         const int u = p / kMlpIn, i = p % kMlpIn;
         float a = 0.0f;
         for (int s = 0; s < kTrainSamples; ++s) a += S.d1[u * kTrainSamples + s] * S.x[s * kMlpIn + i];
         out[kOffW1 + p] = a;
     }
-    for (int u = l; u < kMlpH; u += 64) {
+    for (int u = tid; u < kMlpH; u += kTrainThreads) {
         float a1 = 0.0f, a2 = 0.0f;
         for (int s = 0; s < kTrainSamples; ++s) { a1 += S.d1[u * kTrainSamples + s]; a2 += S.d2[u * kTrainSamples + s]; }
         out[kOffB1 + u] = a1;
         out[kOffB2 + u] = a2;
     }
-    if (l < kMlpOut) {
+    if (tid < kMlpOut) {
         float a = 0.0f;
-        for (int s = 0; s < kTrainSamples; ++s) a += S.d3[s * kMlpOut + l];
-        out[kOffB3 + l] = a;
+        for (int s = 0; s < kTrainSamples; ++s) a += S.d3[s * kMlpOut + tid];
+        out[kOffB3 + tid] = a;
     }
-    if (l == 0) {
+    if (tid == 0) {
         float a = 0.0f;
         for (int s = 0; s < kTrainSamples; ++s) a += S.loss[s];
         loss_part[blockIdx.x] = a;
     }
 }
 
+// x[0] + x[stride] + ... + x[(n - 1) stride], added in that order; the loads of 16 terms are
+// issued together, so a sum costs n / 16 memory latencies instead of n.
+__device__ __forceinline__ float ordered_sum(const float* __restrict__ x, int64_t stride, int n) {
+    float a = 0.0f;
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        float t[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t[q] = x[(k + q) * stride];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a += t[q];
+    }
+    for (; k < n; ++k) a += x[k * stride];
+    return a;
+}
+
+// Member blockIdx.y: fixed-order sum of its ngroups partial gradients, then Adam.
 __global__ __launch_bounds__(256) void mlp_reduce_adam(const float* __restrict__ partial, int32_t ngroups, float* P,
                                                        float* m, float* v, float lr, float b1, float b2, float eps,
                                                        float b1t, float b2t, int32_t apply, float* grad_out,
-                                                       const float* __restrict__ loss_part, float* loss_out) {
+                                                       const float* __restrict__ loss_part, float* loss_out,
+                                                       int64_t loss_stride) {
+    const int mem = blockIdx.y;
+    partial += (size_t)mem * ngroups * kMlpParams;
+    P += (size_t)mem * kMlpParams;
+    m += (size_t)mem * kMlpParams;
+    v += (size_t)mem * kMlpParams;
+    if (grad_out) grad_out += (size_t)mem * kMlpParams;
+    loss_part += (size_t)mem * ngroups;
+    loss_out += mem * loss_stride;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
-        float a = 0.0f;
-        for (int k = 0; k < ngroups; ++k) a += loss_part[k];
-        *loss_out = a;
-    }
+    if (p == 0) *loss_out = ordered_sum(loss_part, 1, ngroups);
     if (p >= kMlpParams) return;
-    float gsum = 0.0f;
-    for (int k = 0; k < ngroups; ++k) gsum += partial[(int64_t)k * kMlpParams + p];
+    const float gsum = ordered_sum(partial + p, kMlpParams, ngroups);
     if (grad_out) grad_out[p] = gsum;
     if (!apply) return;
     const float mm = b1 * m[p] + (1.0f - b1) * gsum;
@@ -272,11 +309,9 @@ int32_t mlp_ensure(nqcd_handle_t* h) {
 }
 
 int32_t mlp_pack_all(nqcd_handle_t* h) {
-    for (int k = 0; k < kMlpMembers; ++k) {
-        hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks), dim3(256), 0, h->stream, h->mlp_master.as<float>() + (size_t)k * kMlpParams,
-                           h->mlp_packed.as<unsigned char>() + (size_t)k * kPkBytes);
-        HIPCHK(h, hipGetLastError());
-    }
+    hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks, kMlpMembers), dim3(256), 0, h->stream, h->mlp_master.as<float>(),
+                       h->mlp_packed.as<unsigned char>());
+    HIPCHK(h, hipGetLastError());
     return NQCD_OK;
 }
 
@@ -338,29 +373,32 @@ static int32_t upload_data(nqcd_handle_t* h, const std::vector<double>& Q, const
     return NQCD_OK;
 }
 
-// One gradient evaluation (and, when apply, one Adam step) of member k on the samples idx[0..nb).
-static int32_t train_step(nqcd_handle_t* h, int k, const int32_t* idx_dev, int32_t nb, const float w[3], bool apply,
-                          int64_t t, float lr, float* grad_dev, float* loss_dev) {
+// One gradient evaluation (and, when apply, one Adam step) of members k0 .. k0 + nmem - 1 side by
+// side: member k0 + i takes its samples at idx_dev + i * idx_stride and writes its loss to
+// loss_dev + i * loss_stride.
+static int32_t train_step(nqcd_handle_t* h, int k0, int nmem, const int32_t* idx_dev, int64_t idx_stride, int32_t nb,
+                          const float w[3], bool apply, int64_t t, float lr, float* grad_dev, float* loss_dev,
+                          int64_t loss_stride) {
     const int ngroups = (nb + kTrainSamples - 1) / kTrainSamples;
-    HIPCHK(h, h->mlp_partial.ensure(sizeof(float) * (size_t)ngroups * kMlpParams));
-    HIPCHK(h, h->mlp_losspart.ensure(sizeof(float) * ngroups));
-    float* P = h->mlp_master.as<float>() + (size_t)k * kMlpParams;
+    HIPCHK(h, h->mlp_partial.ensure(sizeof(float) * (size_t)nmem * ngroups * kMlpParams));
+    HIPCHK(h, h->mlp_losspart.ensure(sizeof(float) * (size_t)nmem * ngroups));
+    float* P = h->mlp_master.as<float>() + (size_t)k0 * kMlpParams;
+    unsigned char* packed = h->mlp_packed.as<unsigned char>() + (size_t)k0 * kPkBytes;
     const size_t lds = train_lds_bytes();
     HIPCHK(h, hipFuncSetAttribute((const void*)mlp_train_grad, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(mlp_train_grad, dim3(ngroups), dim3(64), lds, h->stream,
-                       h->mlp_packed.as<unsigned char>() + (size_t)k * kPkBytes, P, h->mlp_x.as<float>(),
-                       h->mlp_t.as<float>(), idx_dev, nb, w[0], w[1], w[2], h->mlp_partial.as<float>(),
+    hipLaunchKernelGGL(mlp_train_grad, dim3(ngroups, nmem), dim3(kTrainThreads), lds, h->stream, packed, P, h->mlp_x.as<float>(),
+                       h->mlp_t.as<float>(), idx_dev, idx_stride, nb, w[0], w[1], w[2], h->mlp_partial.as<float>(),
                        h->mlp_losspart.as<float>());
     HIPCHK(h, hipGetLastError());
     const float b1 = 0.9f, b2 = 0.999f;
-    hipLaunchKernelGGL(mlp_reduce_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, h->stream, h->mlp_partial.as<float>(),
-                       ngroups, P, h->mlp_adam_m.as<float>() + (size_t)k * kMlpParams,
-                       h->mlp_adam_v.as<float>() + (size_t)k * kMlpParams, lr, b1, b2, 1e-8f,
+    hipLaunchKernelGGL(mlp_reduce_adam, dim3((kMlpParams + 255) / 256, nmem), dim3(256), 0, h->stream,
+                       h->mlp_partial.as<float>(), ngroups, P, h->mlp_adam_m.as<float>() + (size_t)k0 * kMlpParams,
+                       h->mlp_adam_v.as<float>() + (size_t)k0 * kMlpParams, lr, b1, b2, 1e-8f,
                        (float)std::pow(b1, (double)t), (float)std::pow(b2, (double)t), apply ? 1 : 0, grad_dev,
-                       h->mlp_losspart.as<float>(), loss_dev);
+                       h->mlp_losspart.as<float>(), loss_dev, loss_stride);
     HIPCHK(h, hipGetLastError());
     if (apply) {
-        hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks), dim3(256), 0, h->stream, P, h->mlp_packed.as<unsigned char>() + (size_t)k * kPkBytes);
+        hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks, nmem), dim3(256), 0, h->stream, P, packed);
         HIPCHK(h, hipGetLastError());
     }
     return NQCD_OK;
@@ -389,29 +427,31 @@ int32_t mlp_train(nqcd_handle_t* h, const double* X, const double* Y, int64_t n,
                             hipMemcpyHostToDevice));
     }
     if (int32_t rc = mlp_pack_all(h)) return rc;
-    // one shuffled pass over the data per epoch, per member
-    HIPCHK(h, h->mlp_idx.ensure(sizeof(int32_t) * n));
+    // one shuffled pass over the data per epoch, per member (its own order); the members step side by side
+    HIPCHK(h, h->mlp_idx.ensure(sizeof(int32_t) * n * kMlpMembers));
     HIPCHK(h, h->mlp_lossbuf.ensure(sizeof(float) * d->steps * kMlpMembers));
     const float w[3] = {(float)d->loss_weights[0], (float)d->loss_weights[1], (float)d->loss_weights[2]};
-    std::vector<int32_t> perm(n);
-    for (int k = 0; k < kMlpMembers; ++k) {
-        std::mt19937_64 rng(d->seed ^ (0xA5A5A5A5ull + (uint64_t)k));
-        int64_t pos = n;
-        for (int64_t t = 1; t <= d->steps; ++t) {
-            if (pos + d->batch > n) {
-                for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
-                std::shuffle(perm.begin(), perm.end(), rng);
-                HIPCHK(h, hipMemcpyAsync(h->mlp_idx.p, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, h->stream));
-                HIPCHK(h, hipStreamSynchronize(h->stream));
-                pos = 0;
+    std::vector<std::mt19937_64> rngs;
+    for (int k = 0; k < kMlpMembers; ++k) rngs.emplace_back(d->seed ^ (0xA5A5A5A5ull + (uint64_t)k));
+    std::vector<int32_t> perm((size_t)n * kMlpMembers);
+    int64_t pos = n;
+    for (int64_t t = 1; t <= d->steps; ++t) {
+        if (pos + d->batch > n) {
+            for (int k = 0; k < kMlpMembers; ++k) {
+                int32_t* pk = perm.data() + (size_t)k * n;
+                for (int64_t i = 0; i < n; ++i) pk[i] = (int32_t)i;
+                std::shuffle(pk, pk + n, rngs[k]);
             }
-            const double frac = (double)(t - 1) / std::max<int64_t>(1, d->steps - 1);
-            const float lr = (float)(d->lr * std::pow(d->lr_final / d->lr, frac));
-            if (int32_t rc = train_step(h, k, h->mlp_idx.as<int32_t>() + pos, d->batch, w, true, t, lr, nullptr,
-                                        h->mlp_lossbuf.as<float>() + (size_t)k * d->steps + (t - 1)))
-                return rc;
-            pos += d->batch;
+            HIPCHK(h, hipMemcpyAsync(h->mlp_idx.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            pos = 0;
         }
+        const double frac = (double)(t - 1) / std::max<int64_t>(1, d->steps - 1);
+        const float lr = (float)(d->lr * std::pow(d->lr_final / d->lr, frac));
+        if (int32_t rc = train_step(h, 0, kMlpMembers, h->mlp_idx.as<int32_t>() + pos, n, d->batch, w, true, t, lr, nullptr,
+                                    h->mlp_lossbuf.as<float>() + (t - 1), d->steps))
+            return rc;
+        pos += d->batch;
     }
     std::vector<float> lh((size_t)d->steps * kMlpMembers);
     HIPCHK(h, hipMemcpy(lh.data(), h->mlp_lossbuf.p, sizeof(float) * lh.size(), hipMemcpyDeviceToHost));
@@ -435,8 +475,8 @@ int32_t mlp_gradient(nqcd_handle_t* h, int32_t member, const double* X, const do
     HIPCHK(h, hipMemcpy(h->mlp_idx.p, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     HIPCHK(h, h->mlp_grad.ensure(sizeof(float) * (kMlpParams + 1)));
     const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
-    if (int32_t rc = train_step(h, member, h->mlp_idx.as<int32_t>(), (int32_t)n, wf, false, 1, 0.0f, h->mlp_grad.as<float>(),
-                                h->mlp_grad.as<float>() + kMlpParams))
+    if (int32_t rc = train_step(h, member, 1, h->mlp_idx.as<int32_t>(), 0, (int32_t)n, wf, false, 1, 0.0f,
+                                h->mlp_grad.as<float>(), h->mlp_grad.as<float>() + kMlpParams, 0))
         return rc;
     std::vector<float> gv(kMlpParams + 1);
     HIPCHK(h, hipMemcpy(gv.data(), h->mlp_grad.p, sizeof(float) * gv.size(), hipMemcpyDeviceToHost));
@@ -485,7 +525,7 @@ int32_t mlp_set_params(nqcd_handle_t* h, int32_t member, const float* P, const f
     HIPCHK(h, hipMemcpy(h->mlp_master.as<float>() + (size_t)member * kMlpParams, P, sizeof(float) * kMlpParams,
                         hipMemcpyHostToDevice));
     if (norm) std::memcpy(&h->mlp_norm, norm, sizeof(MlpNorm));
-    hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks), dim3(256), 0, h->stream, h->mlp_master.as<float>() + (size_t)member * kMlpParams,
+    hipLaunchKernelGGL(mlp_pack, dim3(kPackBlocks, 1), dim3(256), 0, h->stream, h->mlp_master.as<float>() + (size_t)member * kMlpParams,
                        h->mlp_packed.as<unsigned char>() + (size_t)member * kPkBytes);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
