@@ -240,6 +240,17 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
  * (RandomSelection, selections.jl:70-99) using Philox stream 0 of `seed`. */
 int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* sampler, uint64_t seed);
 
+/* IESH / EhrenfestNA: replace the occupied states of the loaded trajectories by a Fermi-Dirac sample
+ * at each trajectory's initial position (DynamicsVariables(sim, v, r, FermiDiracState{Adiabatic}),
+ * iesh.jl:99-128, ehrenfest_na.jl:43-70): the Boltzmann-factor Metropolis chain of
+ * sample_fermi_dirac_distribution (DynamicsUtils.jl:191-205) over the adiabatic energies, n_available
+ * * n_electrons moves from the lowest available states, draws from Philox stream 3 of `seed`;
+ * psi[state_e, e] = 1.  beta = 1/kT (INFINITY for zero temperature); available_states = NULL makes
+ * every state available (get_available_states(:, n), DynamicsUtils.jl:210-214), otherwise it lists
+ * 0-based states, taken as a set.  Call after load_initial or sample_initial. */
+int32_t nqcd_mi355x_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* available_states,
+                                       int32_t n_available, uint64_t seed);
+
 /* Integrate every trajectory over [t0, t1] with the method's callbacks (hopping after each
  * step, surface_hopping.jl:2-16; termination, callbacks.jl:15-29) and record the save
  * slots.  save_every > 0 saves slot k at step k*save_every (slot 0 = t0), mirroring

@@ -183,6 +183,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // One step of a wavefront arg-max over (mag, idx) with the DPP lane pattern CTRL (larger mag
 // wins, the lower idx on ties).
+// Eigenvalue ir of the Newns-Anderson matrix at impurity energy h: the secular root on its
+// bracket (the outer brackets reach past the band and h by the coupling norm zn = sqrt(M) V);
+// warm starts from `guess`, a cold start from below the bracket.  Also returns f'(E_ir).
+__device__ __forceinline__ double na_root(const double* eps, int M, double a, double V2, double zn, double h, int ir,
+                                          bool warm, double guess, double& fp) {
+    const double lo = (ir == 0) ? fmin(h, eps[0]) - zn - 1e-3 * (zn + 1e-300) : eps[ir - 1];
+    const double hi = (ir == M) ? fmax(h, eps[M - 1]) + zn + 1e-3 * (zn + 1e-300) : eps[ir];
+    const double scale = fmax(fmax(fabs(a), fabs(eps[M - 1])), fabs(h)) + zn;
+    return secular_root(eps, M, ir, V2, h, lo, hi, warm ? guess : lo - 1.0, 4.0e-16 * scale, fp);
+}
+
 template <int CTRL>
 __device__ __forceinline__ void argmax_dpp(double& mag, int& idx) {
     const long long bits = __double_as_longlong(mag);
@@ -339,11 +350,8 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
             const Scalars s = na_scalars(A, xx);
             const int ir = l * NT + w;  // roots dealt round-robin over the wavefronts
             if (ir < N) {
-                const double lo = (ir == 0) ? fmin(s.h, eps[0]) - zn - 1e-3 * (zn + 1e-300) : eps[ir - 1];
-                const double hi = (ir == M) ? fmax(s.h, eps[M - 1]) + zn + 1e-3 * (zn + 1e-300) : eps[ir];
-                const double scale = fmax(fmax(fabs(A.a), fabs(eps[M - 1])), fabs(s.h)) + zn;
                 double fp;
-                const double lam = secular_root(eps, M, ir, V2, s.h, lo, hi, warm ? Eprev : lo - 1.0, 4.0e-16 * scale, fp);
+                const double lam = na_root(eps, M, A.a, V2, zn, s.h, ir, warm, Eprev, fp);
                 E[ir] = lam;
                 U[ir] = 1.0 / sqrt(fp);
                 Eprev = lam;
@@ -810,6 +818,81 @@ __global__ __launch_bounds__(64) void wave_det_kernel(const double* __restrict__
     }
 }
 
+// Adiabatic energies at each trajectory's initial position (the eigenvalues the Fermi-Dirac
+// occupation sampler weighs, iesh.jl:113-114): one wavefront per trajectory, root ir on lane ir % 64.
+__global__ __launch_bounds__(256) void ie_initial_energies(const IeshArgs A, double* __restrict__ E0) {
+    __shared__ double eps[kIeMaxStates];
+    for (int k = threadIdx.x; k < A.M; k += blockDim.x) eps[k] = A.a + k * A.de;
+    __syncthreads();
+    const int64_t traj = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (traj >= A.ntraj) return;
+    const Scalars s = na_scalars(A, A.r0[traj]);
+    const double zn = sqrt((double)A.M) * A.V;
+    for (int ir = threadIdx.x & 63; ir < A.N; ir += 64) {
+        double fp;
+        E0[traj * A.N + ir] = na_root(eps, A.M, A.a, A.V * A.V, zn, s.h, ir, false, 0.0, fp);
+    }
+}
+
+// Position of the k-th set bit (k < popcount) of a 64-bit word.
+__device__ __forceinline__ int select_bit64(uint64_t w, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int sh = 32; sh > 0; sh >>= 1) {
+        const int c = __popcll(w & ((1ull << sh) - 1));
+        if (k >= c) { k -= c; w >>= sh; pos += sh; }
+    }
+    return pos;
+}
+
+// Fermi-Dirac occupations by the Boltzmann-factor Metropolis chain of
+// sample_fermi_dirac_distribution (DynamicsUtils.jl:191-205), one lane per trajectory: start from
+// the n_e lowest available states, then n_avail * n_e moves; move `it` draws Philox counter (traj,
+// it, STREAM_OCCUPATION) = (o0, o1, o2, o3), takes the occupied slot (o0 n_e) >> 32 of the state
+// array and the unoccupied available state number (o1 n_free) >> 32 in ascending order, and
+// accepts when exp(-beta (E_j - E_i)) > u53(o2, o3) (beta = inf: when E_j < E_i).  The slot array
+// lives in LDS ([slot][lane]); the occupied set is a 128-bit mask, so the sorted result is its set
+// bits in order.
+__global__ __launch_bounds__(256) void ie_fermi_dirac_chain(int64_t ntraj, int N, int Ne, uint64_t avail_lo,
+                                                            uint64_t avail_hi, double beta, uint64_t key,
+                                                            const double* __restrict__ E0, int32_t* __restrict__ occ0) {
+    extern __shared__ int slots[];  // Ne x blockDim.x
+    const int tid = threadIdx.x, B = blockDim.x;
+    const int64_t traj = (int64_t)blockIdx.x * B + tid;
+    if (traj >= ntraj) return;
+    const int navail = __popcll(avail_lo) + __popcll(avail_hi), nfree = navail - Ne;
+    const double* E = E0 + traj * N;
+    uint64_t occ_lo = 0, occ_hi = 0;
+    for (int e = 0; e < Ne; ++e) {  // the first Ne available states
+        const uint64_t flo = avail_lo & ~occ_lo, fhi = avail_hi & ~occ_hi;
+        const int st = flo ? __ffsll((unsigned long long)flo) - 1 : 64 + __ffsll((unsigned long long)fhi) - 1;
+        slots[e * B + tid] = st;
+        if (st < 64) occ_lo |= 1ull << st; else occ_hi |= 1ull << (st - 64);
+    }
+    const bool cold = isinf(beta);
+    const int64_t nmoves = (int64_t)navail * Ne;
+    for (int64_t it = 0; it < nmoves; ++it) {
+        uint32_t c0 = (uint32_t)traj, c1 = (uint32_t)(traj >> 32), c2 = (uint32_t)it, c3 = STREAM_OCCUPATION;
+        philox4x32_10(c0, c1, c2, c3, (uint32_t)key, (uint32_t)(key >> 32));
+        const int ci = (int)(((uint64_t)c0 * (uint64_t)Ne) >> 32);
+        const int kk = (int)(((uint64_t)c1 * (uint64_t)nfree) >> 32);
+        const uint64_t flo = avail_lo & ~occ_lo, fhi = avail_hi & ~occ_hi;
+        const int nlo = __popcll(flo);
+        const int j = kk < nlo ? select_bit64(flo, kk) : 64 + select_bit64(fhi, kk - nlo);
+        const int i = slots[ci * B + tid];
+        const double dE = E[j] - E[i];
+        const bool accept = cold ? (dE < 0.0) : (exp(-beta * dE) > u53(c2, c3));
+        if (accept) {
+            slots[ci * B + tid] = j;
+            if (i < 64) occ_lo &= ~(1ull << i); else occ_hi &= ~(1ull << (i - 64));
+            if (j < 64) occ_lo |= 1ull << j; else occ_hi |= 1ull << (j - 64);
+        }
+    }
+    int e = 0;
+    for (uint64_t w = occ_lo; w; w &= w - 1) occ0[traj * Ne + e++] = __ffsll((unsigned long long)w) - 1;
+    for (uint64_t w = occ_hi; w; w &= w - 1) occ0[traj * Ne + e++] = 64 + __ffsll((unsigned long long)w) - 1;
+}
+
 // S as two planes with an odd row stride (determinant), or interleaved Ne x Ne (inverse)
 int ie_sb_len(int N, int Ne) { (void)N; return 2 * Ne * (Ne | 1); }
 
@@ -946,6 +1029,49 @@ int32_t ie_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S, uint64_t
         for (int e = 0; e < Ne; ++e) occ[(size_t)i * Ne + e] = e;
     HIPCHK(h, h->state0.ensure(sizeof(int32_t) * occ.size()));
     HIPCHK(h, hipMemcpyAsync(h->state0.p, occ.data(), sizeof(int32_t) * occ.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->ie_psi_given = false;
+    return NQCD_OK;
+}
+
+int32_t ie_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* avail, int32_t navail, uint64_t seed) {
+    if (!h->have_initial || h->ntraj < 1)
+        return fail(h, NQCD_ERR_STATE, "load or sample the nuclear initial conditions first");
+    if (!h->model_set[0] || h->models[0].kind != NQCD_MODEL_NEWNS_ANDERSON)
+        return fail(h, NQCD_ERR_STATE, "Fermi-Dirac occupations need the Newns-Anderson model in slot 0");
+    if (!(beta >= 0.0)) return fail(h, NQCD_ERR_ARG, "beta must be >= 0 (inf for zero temperature)");
+    const IeModel md = ie_model(h);
+    const int N = h->prob.n_states, Ne = h->prob.n_electrons;
+    if (md.M + 1 != N) return fail(h, NQCD_ERR_ARG, "n_states must equal M + 1 for the Newns-Anderson model");
+    if (N > kIeMaxStates) return fail(h, NQCD_ERR_UNSUPPORTED, "Fermi-Dirac occupations support up to 128 states");
+    uint64_t am[2] = {0, 0};
+    if (!avail) {
+        for (int s = 0; s < N; ++s) am[s >> 6] |= 1ull << (s & 63);
+    } else {
+        for (int32_t k = 0; k < navail; ++k) {
+            const int s = avail[k];
+            if (s < 0 || s >= N) return fail(h, NQCD_ERR_ARG, "available state out of range");
+            if (am[s >> 6] >> (s & 63) & 1) return fail(h, NQCD_ERR_ARG, "available states repeat");
+            am[s >> 6] |= 1ull << (s & 63);
+        }
+    }
+    const int na = __builtin_popcountll(am[0]) + __builtin_popcountll(am[1]);
+    if (na <= Ne) return fail(h, NQCD_ERR_ARG, "need more available states than electrons");
+    const int64_t n = h->ntraj;
+    IeshArgs A{};
+    A.ntraj = n;
+    A.N = N; A.M = md.M; A.Ne = Ne;
+    A.mw2 = md.mw2; A.g = md.g; A.dG = md.dG; A.V = md.V; A.a = md.a; A.de = md.de;
+    A.r0 = h->r0.as<double>();
+    HIPCHK(h, h->tmp.ensure(sizeof(double) * (size_t)N * n));
+    hipLaunchKernelGGL(ie_initial_energies, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, A, h->tmp.as<double>());
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, h->state0.ensure(sizeof(int32_t) * (size_t)Ne * n));
+    const int block = 256;
+    hipLaunchKernelGGL(ie_fermi_dirac_chain, dim3((unsigned)((n + block - 1) / block)), dim3(block),
+                       sizeof(int) * (size_t)Ne * block, h->stream, n, N, Ne, am[0], am[1], beta, seed,
+                       h->tmp.as<double>(), h->state0.as<int32_t>());
+    HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->ie_psi_given = false;
     return NQCD_OK;

@@ -14,7 +14,7 @@ namespace nqcd {
 // Philox4x32-10 (Salmon et al., SC'11).  Counter (traj_lo, traj_hi, step, stream),
 // key (key_lo, key_hi).  Replaces the global rand() of fssh.jl:112.
 // ---------------------------------------------------------------------------
-enum { STREAM_INIT = 0, STREAM_HOP = 1, STREAM_STATE = 2 };
+enum { STREAM_INIT = 0, STREAM_HOP = 1, STREAM_STATE = 2, STREAM_OCCUPATION = 3 };
 
 // Round-key bump on the scalar unit.  The key is wavefront-uniform; computing the
 // schedule in place (instead of letting it be hoisted into ten SGPR pairs that are

@@ -27,6 +27,7 @@ using NQCDynamics
 using NQCDynamics: DynamicsMethods, DynamicsUtils, Ensembles, RingPolymers
 using NQCDynamics.DynamicsMethods: SurfaceHoppingMethods, EhrenfestMethods, ClassicalMethods, IntegrationAlgorithms
 using NQCModels
+using NQCDistributions: NQCDistributions
 
 export EnsembleMI355X
 
@@ -114,7 +115,42 @@ function problem_desc(sim, alg, dt, ensemblealg, masses)
 end
 
 # --- initial conditions from the Selection closure (selections.jl:38-42,70-73) --------------
-function load_initial!(h, prob, sim, trajectories)
+# A ProductDistribution whose electronic part is FermiDiracState{Adiabatic}: the nuclear part is
+# sampled as the Selection samples it, and the occupations are drawn on the device
+# (nqcd_mi355x_sample_fermi_dirac) instead of one sample_fermi_dirac_distribution per trajectory
+# on the host (iesh.jl:99-128, ehrenfest_na.jl:43-70).
+function fermi_dirac_selection(prob, sim)
+    method_code(sim) in (METHOD_IESH, METHOD_EHRENFEST_NA) || return nothing
+    sel = prob.prob_func
+    (sel isa Ensembles.RandomSelection || sel isa Ensembles.OrderedSelection) || return nothing
+    d = sel.distribution
+    (d isa NQCDistributions.ProductDistribution &&
+        d.electronic isa NQCDistributions.FermiDiracState{NQCDistributions.Adiabatic}) || return nothing
+    NQCModels.fermilevel(sim) ≈ d.electronic.fermi_level ||
+        error("Fermi level of model and distribution do not match")      # iesh.jl:103-112
+    return sel
+end
+
+function load_fermi_dirac!(h, sel, sim, trajectories, seed)
+    d = sel.distribution
+    nuc = sel isa Ensembles.OrderedSelection ? [d.nuclear[sel.indices[i]] for i in 1:trajectories] :
+          [rand(d.nuclear) for _ in 1:trajectories]
+    r = reduce(hcat, [vec(u.r) for u in nuc]); v = reduce(hcat, [vec(u.v) for u in nuc])
+    ne = NQCModels.nelectrons(sim)
+    st = repeat(Int32.(0:ne - 1), 1, trajectories)
+    check(h, ccall((:nqcd_mi355x_load_initial, libnqcd), Int32,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
+        h, trajectories, r, v, C_NULL, C_NULL, st, C_NULL))
+    av = d.electronic.available_states
+    avail = av isa Colon ? Int32[] : Int32.(collect(DynamicsUtils.get_available_states(av, NQCModels.nstates(sim)))) .- Int32(1)
+    check(h, ccall((:nqcd_mi355x_sample_fermi_dirac, libnqcd), Int32,
+        (Ptr{Cvoid}, Float64, Ptr{Int32}, Int32, UInt64),
+        h, Float64(d.electronic.β), isempty(avail) ? C_NULL : avail, length(avail), seed))
+end
+
+function load_initial!(h, prob, sim, trajectories, seed)
+    sel = fermi_dirac_selection(prob, sim)
+    sel === nothing || return load_fermi_dirac!(h, sel, sim, trajectories, seed)
     us = [prob.prob_func(prob.prob, i, false).u0 for i in 1:trajectories]
     r = reduce(hcat, [vec(DynamicsUtils.get_positions(u)) for u in us])
     v = reduce(hcat, [vec(DynamicsUtils.get_velocities(u)) for u in us])
@@ -184,7 +220,7 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         kind, params = ensemblealg.device_model === nothing ? model_code(sim.cache.model) : ensemblealg.device_model
         check(h[], ccall((:nqcd_mi355x_set_model, libnqcd), Int32, (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int64),
             h[], 0, kind, collect(Float64, params), length(params)))
-        load_initial!(h[], prob, sim, trajectories)
+        load_initial!(h[], prob, sim, trajectories, ensemblealg.seed)
         save_every = saveat === nothing ? Int32(0) : round(Int32, step(saveat) / dt)
         # TerminatingCallback with the |x_dof| > value predicate
         term = callback === nothing ? C_NULL : Ref(TermDesc(1, callback.dof - 1, callback.value))

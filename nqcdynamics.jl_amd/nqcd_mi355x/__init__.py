@@ -8,7 +8,7 @@ trajectory integrated by libnqcd_mi355x.so (hand-written gfx950 kernels behind a
 C ABI).  There is no CPU fallback: the library must be built.
 """
 from . import _lib  # noqa: F401
-from .distributions import (Adiabatic, Diabatic, DynamicalDistribution, DynamicsVariables, Normal,  # noqa: F401
+from .distributions import (Adiabatic, Diabatic, DynamicalDistribution, DynamicsVariables, FermiDiracState, Normal,  # noqa: F401
                             PositionHarmonicWigner, PureState, VelocityHarmonicWigner)
 from .ensembles import EnsembleMI355X, TerminatingCallback, run_dynamics  # noqa: F401
 from .models import (AndersonHolstein, DebyeSpectralDensity, DiatomSurface, DoubleWellGaussian, Harmonic,  # noqa: F401

@@ -40,6 +40,7 @@ EXPORTED_SYMBOLS = (
     "nqcd_mi355x_reduce", "nqcd_mi355x_download", "nqcd_mi355x_last_run_stats", "nqcd_mi355x_last_error",
     "nqcd_mi355x_surrogate_train", "nqcd_mi355x_surrogate_eval", "nqcd_mi355x_surrogate_gradient",
     "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params", "nqcd_mi355x_test_determinant",
+    "nqcd_mi355x_sample_fermi_dirac",
 )
 
 
@@ -100,6 +101,7 @@ def lib():
     L.nqcd_mi355x_set_model.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_load_initial.argtypes = [H, C.c_int64, dp, dp, dp, dp, ip, ip]
     L.nqcd_mi355x_sample_initial.argtypes = [H, C.POINTER(SamplerDesc), C.c_uint64]
+    L.nqcd_mi355x_sample_fermi_dirac.argtypes = [H, C.c_double, ip, C.c_int32, C.c_uint64]
     L.nqcd_mi355x_run.argtypes = [H, C.c_double, C.c_double, C.c_int32, C.POINTER(TermDesc), C.c_uint64]
     L.nqcd_mi355x_reduce.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_download.argtypes = [H, C.c_int32, C.c_void_p, C.c_int64]
@@ -224,6 +226,13 @@ class Ensemble:
         s.params, s.nparams = _dp(p), len(p)
         self._check(self._L.nqcd_mi355x_sample_initial(self._h, C.byref(s), C.c_uint64(seed)))
         self.ntraj = ntraj
+
+    def sample_fermi_dirac(self, beta, seed, available_states=None):
+        """IESH / EhrenfestNA: occupations of the loaded trajectories from the Fermi-Dirac chain at
+        inverse temperature beta (numpy.inf for 0 K); available_states are 0-based (None = all)."""
+        av = None if available_states is None else np.ascontiguousarray(available_states, dtype=np.int32)
+        self._check(self._L.nqcd_mi355x_sample_fermi_dirac(self._h, float(beta), _ip(av), 0 if av is None else len(av),
+                                                           C.c_uint64(seed)))
 
     def run(self, t0, t1, key, save_every=0, term=None):
         td = None

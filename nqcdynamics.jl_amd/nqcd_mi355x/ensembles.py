@@ -10,7 +10,7 @@ import logging
 import numpy as np
 
 from . import _lib as L
-from .distributions import Adiabatic, DynamicalDistribution, ProductDistribution, PureState
+from .distributions import Adiabatic, DynamicalDistribution, FermiDiracState, ProductDistribution, PureState
 from .outputs import (MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction, SumReduction,
                       output_name)
 from .simulations import AdiabaticIESH, Classical, Ehrenfest, EhrenfestNA, FSSH, RingPolymerSimulation, Simulation
@@ -100,21 +100,46 @@ def _configurations(nuc, trajectories, selection):
     return None
 
 
+def _fermi_dirac(sim, dist):
+    """The distribution's FermiDiracState (None without one), checked as DynamicsVariables checks it:
+    IESH / EhrenfestNA only, Adiabatic basis, the model's Fermi level (iesh.jl:99-112)."""
+    el = _electronic(dist) if isinstance(dist, (ProductDistribution, dict)) else None
+    if not isinstance(el, FermiDiracState):
+        return None
+    if _kind(sim) != "ie":
+        raise ValueError("FermiDiracState initial conditions are for IESH / EhrenfestNA simulations")
+    if not (isinstance(el.basis, Adiabatic) or el.basis is Adiabatic):
+        raise ValueError("EnsembleMI355X provides FermiDiracState in the Adiabatic basis")
+    ef = sim.model.fermi_level
+    if abs(ef - el.fermi_level) > np.sqrt(np.finfo(float).eps) * max(abs(ef), abs(el.fermi_level)):
+        raise ValueError(f"Fermi level of model and distribution do not match: distribution {el.fermi_level}, "
+                         f"model {ef}")
+    if el.available_states is not None:
+        n = sim.model.nstates
+        if max(el.available_states) > n or min(el.available_states) < 1:
+            raise ValueError(f"There are only {n} states in the system.")  # get_available_states, DynamicsUtils.jl:211-214
+    return el
+
+
 def _initial_ie(E, sim, dist, trajectories, selection, seed):
     """IESH / EhrenfestNA: the lowest n_electrons adiabatic orbitals occupied, psi[state_e, e] = 1
-    (DynamicsVariables(sim, v, r), iesh.jl:89-99, ehrenfest_na.jl:43-48)."""
+    (DynamicsVariables(sim, v, r), iesh.jl:89-99, ehrenfest_na.jl:43-48), or with a FermiDiracState
+    the thermal occupations drawn on the device (iesh.jl:99-128, ehrenfest_na.jl:43-70)."""
     nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    fd = _fermi_dirac(sim, dist)
     conf = _configurations(nuc, trajectories, selection)
     if conf is not None:
         r, v = conf
         occ = np.tile(np.arange(sim.model.nelectrons, dtype=np.int32), (r.shape[0], 1))
         E.load_initial_iesh(r[:, 0], v[:, 0], occ)
-        return
-    if isinstance(nuc, DynamicalDistribution):
+    elif isinstance(nuc, DynamicalDistribution):
         mr, sr, mv, sv = nuc.moments()
         E.sample_initial(L.SAMPLER_GAUSSIAN, trajectories, [mr[0], sr[0], mv[0], sv[0]], seed)
-        return
-    raise ValueError(f"unsupported distribution {type(nuc).__name__}")
+    else:
+        raise ValueError(f"unsupported distribution {type(nuc).__name__}")
+    if fd is not None:
+        avail = None if fd.available_states is None else np.asarray(fd.available_states) - 1
+        E.sample_fermi_dirac(fd.beta, seed, avail)
 
 
 def _initial_rp(E, sim, dist, trajectories, selection, seed):
@@ -223,6 +248,7 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
         raise ValueError("termination callbacks are supported for FSSH / Ehrenfest simulations")
     if int(trajectories) < 1:
         raise ValueError("trajectories must be >= 1")
+    _fermi_dirac(sim, distribution)
     log.info("Performing %d trajectories on MI355X device %d.", trajectories, ensemble_algorithm.device)
     E = L.Ensemble(ensemble_algorithm.device)
     E._schedule = ensemble_algorithm.schedule

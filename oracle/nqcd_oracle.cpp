@@ -77,7 +77,7 @@ static inline void normal2(uint64_t traj, uint32_t step, uint32_t stream, uint64
     *z1 = rad * std::sin(ang);
 }
 
-enum { STREAM_INIT = 0, STREAM_HOP = 1, STREAM_STATE = 2 };
+enum { STREAM_INIT = 0, STREAM_HOP = 1, STREAM_STATE = 2, STREAM_OCCUPATION = 3 };
 
 OR_API void oracle_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox10(ctr, key, out); }
 OR_API double oracle_uniform(uint64_t traj, uint32_t step, uint32_t stream, uint64_t key) {
@@ -1270,6 +1270,36 @@ OR_API int32_t oracle_iesh_run_ensemble(const OIEDesc* R, const ONADesc* md, int
     parallel_for(ntraj, nthreads, [&](int64_t i) {
         ie_run_one(*R, *md, i, (uint64_t)i, r0[i], v0[i], occ0, psi_re0, psi_im0, r, v, psi_re, psi_im, occ, nhops, pop_di);
     });
+    return 0;
+}
+
+// sample_fermi_dirac_distribution with the Boltzmann factor (DynamicsUtils.jl:191-205), literally:
+// state = the first n_e entries of `avail`; n_avail * n_e times pick a random slot of `state`
+// (rand(eachindex(state))), a random element of setdiff(avail, state) in the order of `avail`, and
+// replace the slot's state when exp(-beta (E_j - E_i)) > rand(); finally sort.  Julia's rand calls
+// are replaced by Philox counter (traj, move, STREAM_OCCUPATION) = (o0, o1, o2, o3): slot
+// (o0 n_e) >> 32, element (o1 n_free) >> 32, uniform u53(o2, o3).  With beta = inf the factor is
+// IEEE's: 0 above, inf below, NaN (never accepted) for equal energies.  Returns 0, or -1 when
+// fewer than n_e + 1 states are available (setdiff would be empty, as rand() of it throws).
+OR_API int32_t oracle_fermi_dirac_occupations(const double* E, int32_t Ne, const int32_t* avail, int32_t navail,
+                                              double beta, uint64_t traj, uint64_t key, int32_t* occ) {
+    if (navail <= Ne) return -1;
+    std::vector<int32_t> state(avail, avail + Ne), rest;
+    const int64_t nmoves = (int64_t)navail * Ne;
+    for (int64_t it = 0; it < nmoves; ++it) {
+        uint32_t o[4];
+        philox_draw(traj, (uint32_t)it, STREAM_OCCUPATION, key, o);
+        const int ci = (int)(((uint64_t)o[0] * (uint64_t)Ne) >> 32);
+        rest.clear();
+        for (int32_t k = 0; k < navail; ++k)
+            if (std::find(state.begin(), state.end(), avail[k]) == state.end()) rest.push_back(avail[k]);
+        const int kk = (int)(((uint64_t)o[1] * (uint64_t)rest.size()) >> 32);
+        const int i = state[ci], j = rest[kk];
+        const double prob = std::exp(-beta * (E[j] - E[i]));
+        if (prob > u53(o[2], o[3])) state[ci] = j;
+    }
+    std::sort(state.begin(), state.end());
+    std::copy(state.begin(), state.end(), occ);
     return 0;
 }
 

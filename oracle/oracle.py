@@ -307,6 +307,22 @@ def na_eigen(md, x):
     return E, U, dh.value
 
 
+def fermi_dirac_occupations(E, nelec, beta, traj, key, available=None):
+    """Sorted 0-based occupied states from the Boltzmann-factor chain (oracle_fermi_dirac_occupations)."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    av = np.arange(len(E), dtype=np.int32) if available is None else np.ascontiguousarray(available, dtype=np.int32)
+    occ = np.zeros(nelec, np.int32)
+    f = lib().oracle_fermi_dirac_occupations
+    f.argtypes = [C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_double, C.c_uint64,
+                  C.c_uint64, C.POINTER(C.c_int32)]
+    f.restype = C.c_int32
+    rc = f(_p(E), nelec, av.ctypes.data_as(C.POINTER(C.c_int32)), len(av), float(beta), C.c_uint64(traj),
+           C.c_uint64(key), occ.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != 0:
+        raise ValueError("need more available states than electrons")
+    return occ
+
+
 def iesh_run_ensemble(method, md, nelec, mass, r0, v0, occ0, psi0=None, dt=1.0, nsteps=100, key=0,
                       rescaling=RESCALE_STANDARD, estimate=True, edc=False, edc_C=0.1, save_every=0, nthreads=None):
     ntraj = len(r0)

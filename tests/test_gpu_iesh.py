@@ -178,3 +178,83 @@ def test_device_determinant_matches_numpy():
             assert ok[0] and abs(d[0] - ref) <= 1e-11 * abs(ref)
     finally:
         E.close()
+
+
+def _fd_occupations(L, method, M, Ne, r0, beta, seed, avail=None, nsteps=0, key=0):
+    """Device Fermi-Dirac occupations at r0 (and, with nsteps > 0, the run that follows them)."""
+    E = L.Ensemble(0)
+    try:
+        E.set_problem(method, M + 1, 1, 1, [MASS], 2.0, algorithm=L.ALG_VERLET_ELECTRONICS, n_electrons=Ne)
+        E.set_model(0, L.MODEL_NEWNS_ANDERSON, _params(M))
+        v0 = np.full(len(r0), 5e-3)
+        E.load_initial_iesh(r0, v0, np.tile(np.arange(Ne), (len(r0), 1)))
+        E.sample_fermi_dirac(beta, seed, avail)
+        E.run(0.0, nsteps * 2.0, key, save_every=10 if nsteps else 0)
+        out = {"occ": E.download(L.FIELD_STATE), "r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
+               "psi": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
+               "nhops": E.download(L.FIELD_NHOPS), "di": E.download(L.FIELD_SAVE_POP_DI)}
+    finally:
+        E.close()
+    return out, v0
+
+
+def _fd_oracle(oracle, M, Ne, r0, beta, seed, avail=None):
+    W = 6 * MODEL["Gamma"] / 2
+    md = oracle.na_model(M=M, W=W, m=MODEL["m"], omega=MODEL["omega"], g=MODEL["g"], dG=MODEL["dG"], Gamma=MODEL["Gamma"])
+    av = None if avail is None else sorted(avail)
+    return np.stack([oracle.fermi_dirac_occupations(oracle.na_eigen(md, x)[0], Ne, beta, t, seed, av)
+                     for t, x in enumerate(r0)])
+
+
+KT300 = 3.166811563e-6 * 300.0
+
+
+@pytest.mark.parametrize("M,Ne,beta,avail", [(40, 20, 1 / KT300, None), (100, 50, 1 / KT300, None),
+                                             (4, 2, 0.0, None), (30, 15, 1 / (4 * KT300), [0, 2, 5] + list(range(9, 31))),
+                                             (127, 20, 0.0, None), (127, 20, 1 / KT300, None)])
+def test_fermi_dirac_occupations_match_oracle(oracle, M, Ne, beta, avail):
+    """Device chain (one lane per trajectory, the adiabatic energies from the secular roots) against the
+    oracle's literal restatement of sample_fermi_dirac_distribution on the same Philox streams:
+    identical occupations at 300 K, at infinite temperature, with a subset of available states and at
+    the largest state count (two 64-bit occupancy words; at infinite temperature electrons reach the
+    upper word)."""
+    from nqcd_mi355x import _lib as L
+    rng = np.random.default_rng(M)
+    r0 = rng.normal(3.0, 2.0, size=96)
+    g, _ = _fd_occupations(L, L.METHOD_IESH, M, Ne, r0, beta, 0x5EED, avail)
+    c = _fd_oracle(oracle, M, Ne, r0, beta, 0x5EED, avail)
+    assert np.array_equal(g["occ"], c)
+    if beta > 0:
+        assert not np.all(c == np.arange(Ne))  # thermal: some trajectories leave the ground state
+
+
+def test_fermi_dirac_zero_temperature_and_errors():
+    """0 K gives the lowest n_e states (test/Dynamics/iesh.jl:37-41); too few available states and an
+    FSSH problem are rejected."""
+    from nqcd_mi355x import _lib as L
+    g, _ = _fd_occupations(L, L.METHOD_IESH, 40, 20, np.linspace(-5, 25, 64), np.inf, 1)
+    assert np.array_equal(g["occ"], np.tile(np.arange(20), (64, 1)))
+    with pytest.raises(L.NQCDError, match="available"):
+        _fd_occupations(L, L.METHOD_IESH, 40, 20, np.zeros(4), 1.0, 1, avail=list(range(20)))
+    E = L.Ensemble(0)
+    try:
+        E.set_problem(L.METHOD_FSSH, 2, 1, 1, [MASS], 1.0)
+        with pytest.raises(L.NQCDError, match="IESH"):
+            E.sample_fermi_dirac(1.0, 1)
+    finally:
+        E.close()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_fermi_dirac_then_dynamics_parity(oracle, key, method):
+    """The whole DynamicsVariables(sim, v, r, FermiDiracState) -> run path: device-sampled thermal
+    occupations, then IESH / EhrenfestNA from them, against the oracle run from the oracle's
+    occupations (1e-9 relative, identical hops)."""
+    from nqcd_mi355x import _lib as L
+    M, Ne, nsteps = 40, 20, 150
+    r0 = np.random.default_rng(5).normal(6.0, 1.0, size=32)
+    meth = L.METHOD_IESH if method == 0 else L.METHOD_EHRENFEST_NA
+    g, v0 = _fd_occupations(L, meth, M, Ne, r0, 1 / KT300, 77, nsteps=nsteps, key=key)
+    occ0 = _fd_oracle(oracle, M, Ne, r0, 1 / KT300, 77)
+    c = _cpu(oracle, method, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
+    _compare(g, c, hops=method == 0)

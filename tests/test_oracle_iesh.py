@@ -212,3 +212,86 @@ def test_iesh_diabatic_population(oracle, key):
         assert np.allclose(out["pop_di"][i, 0], np.sum(U[:, occ0[i]] ** 2, axis=1), atol=1e-13)
     assert np.allclose(out["pop_di"].sum(axis=2), Ne, atol=1e-10)
     assert np.all(np.abs(out["pop_di"][:, -1] - out["pop_di"][:, 0]) > 0)
+
+
+def _exact_chain(E, Ne, avail, beta, nmoves):
+    """Distribution of the sorted occupied set after `nmoves` moves of the Boltzmann-factor chain
+    (DynamicsUtils.jl:191-205), by enumerating its transitions: a uniform slot, a uniform free
+    available state, acceptance min(1, exp(-beta dE))."""
+    from collections import defaultdict
+    dist = {tuple(avail[:Ne]): 1.0}
+    for _ in range(nmoves):
+        new = defaultdict(float)
+        for st, p in dist.items():
+            free = [a for a in avail if a not in st]
+            for ci in range(Ne):
+                for j in free:
+                    x = -beta * (E[j] - E[st[ci]])
+                    acc = 1.0 if x >= 0 else float(np.exp(x))
+                    w = p / (Ne * len(free))
+                    moved = list(st)
+                    moved[ci] = j
+                    new[tuple(moved)] += w * acc
+                    new[st] += w * (1.0 - acc)
+        dist = new
+    out = defaultdict(float)
+    for st, p in dist.items():
+        out[tuple(sorted(st))] += p
+    return out
+
+
+@pytest.mark.parametrize("avail", [[0, 1, 2, 3, 4], [0, 2, 3, 4]])
+def test_fermi_dirac_chain_distribution_is_exact(oracle, avail):
+    """Empirical distribution of the oracle's occupations over 40000 Philox streams against the exact
+    distribution of the reference's chain after n_avail * n_e moves (every transition enumerated):
+    pins the initial state (first n_e available), the uniform choices, the acceptance rule and the
+    restriction to the available states."""
+    from collections import Counter
+    E = np.array([-0.3, -0.1, 0.0, 0.15, 0.4])
+    Ne, beta, n = 2, 4.0, 40000
+    exact = _exact_chain(E, Ne, avail, beta, len(avail) * Ne)
+    emp = Counter(tuple(oracle.fermi_dirac_occupations(E, Ne, beta, t, 12345, avail)) for t in range(n))
+    assert set(emp) <= set(exact)
+    for st, p in exact.items():
+        assert abs(emp[st] / n - p) <= 5 * np.sqrt(p * (1 - p) / n) + 1e-4, st
+
+
+def test_fermi_dirac_chain_length(oracle):
+    """beta = 0 accepts every move: one electron on three states then walks the triangle, and after
+    K moves sits on its start with probability (1 + 2 (-1/2)^K) / 3 -- 1/4 for the chain's
+    n_avail * n_e = 3 moves (1/2 for 2, 3/8 for 4)."""
+    E = np.zeros(3)
+    n = 20000
+    at_start = np.mean([oracle.fermi_dirac_occupations(E, 1, 0.0, t, 7)[0] == 0 for t in range(n)])
+    assert abs(at_start - 0.25) < 5 * np.sqrt(0.25 * 0.75 / n)
+
+
+def test_fermi_dirac_zero_temperature(oracle):
+    """0 K leaves the lowest n_e states occupied (test/Dynamics/iesh.jl:37-41); equal energies give a
+    NaN Boltzmann factor at beta = inf, which is never accepted (IEEE, as in the reference)."""
+    md = oracle.na_model(M=30)
+    E, _, _ = oracle.na_eigen(md, 1.5)
+    for t in range(50):
+        assert np.array_equal(oracle.fermi_dirac_occupations(E, 15, np.inf, t, 3), np.arange(15))
+    avail = [1, 3, 4, 6]
+    assert np.array_equal(oracle.fermi_dirac_occupations(np.zeros(8), 2, np.inf, 0, 3, avail), [1, 3])
+    with pytest.raises(ValueError):
+        oracle.fermi_dirac_occupations(E, 15, 1.0, 0, 3, np.arange(15))
+
+
+def test_fermi_dirac_thermal_occupations(oracle):
+    """Mean occupations of 5000 samples at 300 K against the Fermi function at the chemical potential
+    that holds n_e electrons, normwise within the reference's rtol = 0.2 (test/Dynamics/iesh.jl:46-63;
+    the fixed-n_e chain is canonical, so it is sharper than the grand-canonical function near mu)."""
+    from scipy.optimize import brentq
+    md = oracle.na_model(M=30)
+    E, _, _ = oracle.na_eigen(md, 1.5)
+    beta = 1.0 / (3.166811563e-6 * 300.0)
+    n = 5000
+    avg = np.zeros(31)
+    for t in range(n):
+        avg[oracle.fermi_dirac_occupations(E, 15, beta, t, 99)] += 1
+    avg /= n
+    fermi = lambda mu: 1.0 / (1.0 + np.exp(np.clip(beta * (E - mu), -700, 700)))
+    mu = brentq(lambda m: fermi(m).sum() - 15, E.min() - 1, E.max() + 1)
+    assert np.linalg.norm(avg - fermi(mu)) <= 0.2 * np.linalg.norm(fermi(mu))
