@@ -340,6 +340,26 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
             isocc[occ[tid]] = tid;
             if (!A.psi0) Tb[occ[tid] * CW + 2 * tid] = 1.0;  // psi[state_e, e] = 1 (iesh.jl:174-180)
         }
+        // The determinant bound of the hop estimate (below) needs orthonormal electron orbitals:
+        // true for psi[state_e, e] = 1, checked for a given psi; propagation keeps it, EDC does not.
+        bool det_bound = A.method == 0 && A.estimate && !A.edc;
+        if (det_bound && A.psi0) {
+            __syncthreads();
+            double dev = 0.0;
+            for (int k = tid; k < Ne * Ne; k += B) {
+                const int e1 = k / Ne, e2 = k - e1 * Ne;
+                double gr = (e1 == e2) ? -1.0 : 0.0, gi = 0.0;
+                for (int a = 0; a < N; ++a) {
+                    const double xr = Tb[a * CW + 2 * e1], xi = Tb[a * CW + 2 * e1 + 1];
+                    const double yr = Tb[a * CW + 2 * e2], yi = Tb[a * CW + 2 * e2 + 1];
+                    gr += xr * yr + xi * yi;
+                    gi += xr * yi - xi * yr;
+                }
+                dev = fmax(dev, fmax(fabs(gr), fabs(gi)));
+            }
+            det_bound = block_max<NT>(dev, red) <= 1e-10;
+            __syncthreads();
+        }
         double x = A.r0[traj], v = A.v0[traj];
         double Eprev = -1.0;
         bool warm = false;
@@ -543,52 +563,66 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
             // ---- IESH hop (iesh.jl:390-407) -------------------------------------------------------------
             if (A.method == 0) {
                 const double rnd = philox_uniform((uint64_t)traj, (uint32_t)step, STREAM_HOP, A.key);
-                // S_je = psi[occ_j, e] (iesh.jl:309-316) as real / imaginary planes, row stride LD (odd)
                 const int LD = Ne | 1, Nu = N - Ne;
                 double* Sre = Sb;
                 double* Sim = Sb + Ne * LD;
-                for (int idx = tid; idx < Ne * Ne; idx += B) {
-                    const int j = idx / Ne, e = idx - j * Ne;
-                    Sre[j * LD + e] = Tb[occ[j] * CW + 2 * e];
-                    Sim[j * LD + e] = Tb[occ[j] * CW + 2 * e + 1];
-                }
-                if (tid == 0) {  // unoccupied states in ascending order (DynamicsUtils.jl:162-171)
-                    int c = 0;
-                    for (int m = 0; m < N; ++m)
-                        if (isocc[m] < 0) unocc[c++] = m;
-                }
-                __syncthreads();
-                // det S (FastDeterminant.jl:20-22) on wavefront 0 while the others sum |v_dot_d[m, n]|
-                // over unoccupied m and electrons n (iesh.jl:285-298)
-                if (w == 0) {
-                    double dr, di;
-                    const bool ok = wave_det(Sre, Sim, LD, Ne, dr, di);
-                    if (l == 0) {
-                        scal[1] = dr;
-                        scal[2] = di;
-                        scal[3] = ok ? 0.0 : 1.0;
+                // sum |v_dot_d[m, n]| over unoccupied m and electrons n (iesh.jl:285-298), and
+                // t2 = ||psi[unoccupied, :]||_F^2, the weight of the orbitals outside the occupied rows
+                double lvd = 0.0, lt2 = 0.0;
+                for (int k = tid; k < N * Ne; k += B) {
+                    const int m = k / Ne, n = k - m * Ne;
+                    if (isocc[m] < 0) {
+                        const int o = occ[n];
+                        lvd += fabs(v * (sc.dh * (U[m] * U[o] / (E[o] - E[m]))));
+                        const double pr = Tb[m * CW + 2 * n], pi = Tb[m * CW + 2 * n + 1];
+                        lt2 += pr * pr + pi * pi;
                     }
                 }
-                double loc = 0.0;
-                {
-                    const int t0 = (NT > 1) ? 64 : 0;
-                    if (tid >= t0) {
-                        for (int k = tid - t0; k < N * Ne; k += B - t0) {
-                            const int m = k / Ne, n = k - m * Ne;
-                            if (isocc[m] < 0) {
-                                const int o = occ[n];
-                                loc += fabs(v * (sc.dh * (U[m] * U[o] / (E[o] - E[m]))));
-                            }
+                const double sumvd = block_sum<NT>(lvd, red);
+                const double t2 = block_sum<NT>(lt2, red);
+                // The estimate 2 dt sumvd (|Re det S| + |Im det S|) / |det S|^2 (iesh.jl:250-253) is at
+                // most 2 dt sumvd sqrt(2) / |det S|, and for orthonormal orbitals (S^H S + T^H T = I,
+                // T = psi[unoccupied, :]) |det S|^2 = prod(1 - s_i^2(T)) >= 1 - t2.  Where that bound
+                // is already below the uniform, the reference's estimate is too: no hop, and the LU is
+                // skipped (1e-6 absorbs the orbitals' rounding-level loss of orthonormality; 1e-8 and
+                // the 1e-4 floor, which bounds cond(S) by 100, absorb the rounding of the estimate).
+                bool skip = false;
+                if (det_bound) {
+                    const double L2 = 1.0 - t2 - 1e-6;
+                    skip = L2 > 1e-4 && 2.0 * dt * sumvd * 1.4142135623730951 * (1.0 + 1e-8) < rnd * sqrt(L2);
+                }
+                bool evaluate = false;
+                if (!skip) {
+                    // S_je = psi[occ_j, e] (iesh.jl:309-316) as real / imaginary planes, row stride LD (odd)
+                    for (int idx = tid; idx < Ne * Ne; idx += B) {
+                        const int j = idx / Ne, e = idx - j * Ne;
+                        Sre[j * LD + e] = Tb[occ[j] * CW + 2 * e];
+                        Sim[j * LD + e] = Tb[occ[j] * CW + 2 * e + 1];
+                    }
+                    if (tid == 0) {  // unoccupied states in ascending order (DynamicsUtils.jl:162-171)
+                        int c = 0;
+                        for (int m = 0; m < N; ++m)
+                            if (isocc[m] < 0) unocc[c++] = m;
+                    }
+                    __syncthreads();
+                    // det S (FastDeterminant.jl:20-22) on wavefront 0
+                    if (w == 0) {
+                        double dr, di;
+                        const bool ok = wave_det(Sre, Sim, LD, Ne, dr, di);
+                        if (l == 0) {
+                            scal[1] = dr;
+                            scal[2] = di;
+                            scal[3] = ok ? 0.0 : 1.0;
                         }
                     }
+                    __syncthreads();
+                    const double dre = scal[1], dim = scal[2];
+                    const bool singular = scal[3] != 0.0;
+                    const double Akk = dre * dre + dim * dim;
+                    const double pref = 2.0 * dt / Akk;
+                    const double est = pref * sumvd * (fabs(dre) + fabs(dim));
+                    evaluate = !singular && Akk > 0.0 && (!A.estimate || !(est < rnd));
                 }
-                const double sumvd = block_sum<NT>(loc, red);  // its barriers also publish scal[1..3]
-                const double dre = scal[1], dim = scal[2];
-                const bool singular = scal[3] != 0.0;
-                const double Akk = dre * dre + dim * dim;
-                const double pref = 2.0 * dt / Akk;
-                const double est = pref * sumvd * (fabs(dre) + fabs(dim));
-                const bool evaluate = !singular && Akk > 0.0 && (!A.estimate || !(est < rnd));
                 if (evaluate) {
                     __syncthreads();  // wavefront 0's determinant work on Sb is complete
                     for (int idx = tid; idx < Ne * Ne; idx += B) {

@@ -24,11 +24,11 @@ def _inputs(ntraj, Ne, seed, x0=8.0):
     return r0, v0, occ0
 
 
-def _gpu(L, method, M, Ne, r0, v0, occ0, dt, nsteps, key, save_every=10, **kw):
+def _gpu(L, method, M, Ne, r0, v0, occ0, dt, nsteps, key, save_every=10, psi=None, **kw):
     E = L.Ensemble(0)
     E.set_problem(method, M + 1, 1, 1, [MASS], dt, algorithm=L.ALG_VERLET_ELECTRONICS, n_electrons=Ne, **kw)
     E.set_model(0, L.MODEL_NEWNS_ANDERSON, _params(M))
-    E.load_initial_iesh(r0, v0, occ0)
+    E.load_initial_iesh(r0, v0, occ0, psi)
     stats = E.run(0.0, nsteps * dt, key, save_every=save_every)
     out = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
            "psi": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
@@ -258,3 +258,27 @@ def test_fermi_dirac_then_dynamics_parity(oracle, key, method):
     occ0 = _fd_oracle(oracle, M, Ne, r0, 1 / KT300, 77)
     c = _cpu(oracle, method, M, Ne, r0, v0, occ0, 2.0, nsteps, key)
     _compare(g, c, hops=method == 0)
+
+
+@pytest.mark.parametrize("orthonormal", [True, False])
+def test_iesh_given_wavefunction_parity(oracle, key, orthonormal):
+    """A caller-given psi (load_initial with sig_re / sig_im): orthonormal orbitals mixed away from the
+    occupied states, where the device may decide the hop estimate from the bound |det S|^2 >= 1 - t2
+    without the LU, and orbitals 5% off normalisation, where it must not; both against the oracle
+    (which always takes the LU), identical hops."""
+    from nqcd_mi355x import _lib as L
+    M, Ne, ntraj, nsteps = 40, 20, 32, 200
+    r0, v0, occ0 = _inputs(ntraj, Ne, 15)
+    rng = np.random.default_rng(16)
+    psi = np.zeros((ntraj, M + 1, Ne), complex)
+    for i in range(ntraj):
+        X = np.zeros((M + 1, Ne), complex)
+        X[occ0[i], np.arange(Ne)] = 1.0
+        X += 0.02 * (rng.normal(size=X.shape) + 1j * rng.normal(size=X.shape))  # t2 ~ 0.3
+        psi[i] = np.linalg.qr(X)[0]
+        if not orthonormal:
+            psi[i] *= 1.05
+    g = _gpu(L, L.METHOD_IESH, M, Ne, r0, v0, occ0, 2.0, nsteps, key, psi=psi)
+    c = _cpu(oracle, 0, M, Ne, r0, v0, occ0, 2.0, nsteps, key, psi0=psi)
+    assert c["nhops"].sum() >= 1
+    _compare(g, c)

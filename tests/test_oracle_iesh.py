@@ -295,3 +295,27 @@ def test_fermi_dirac_thermal_occupations(oracle):
     fermi = lambda mu: 1.0 / (1.0 + np.exp(np.clip(beta * (E - mu), -700, 700)))
     mu = brentq(lambda m: fermi(m).sum() - 15, E.min() - 1, E.max() + 1)
     assert np.linalg.norm(avg - fermi(mu)) <= 0.2 * np.linalg.norm(fermi(mu))
+
+
+def test_determinant_lower_bound_of_the_occupied_block():
+    """The bound the device uses to skip the LU of the hop estimate: for orthonormal orbitals psi
+    (N x n_e) and any n_e occupied rows, S = psi[occ, :] and T = psi[unoccupied, :] satisfy
+    S^H S + T^H T = I, so |det S|^2 = prod(1 - s_i(T)^2) >= 1 - ||T||_F^2; hence the estimate
+    2 dt sumvd (|Re det| + |Im det|) / |det|^2 <= 2 dt sumvd sqrt(2) / sqrt(1 - ||T||_F^2).  Checked on
+    random isometries near and far from the occupied rows, including T nearly of norm 1."""
+    rng = np.random.default_rng(17)
+    for N, Ne in ((5, 2), (41, 20), (101, 50)):
+        for scale in (1e-3, 0.1, 0.3, 1.0):
+            X = np.eye(N, Ne) + scale * (rng.normal(size=(N, Ne)) + 1j * rng.normal(size=(N, Ne)))
+            psi, _ = np.linalg.qr(X)
+            occ = np.sort(rng.choice(N, Ne, replace=False)) if scale == 1.0 else np.arange(Ne)
+            mask = np.zeros(N, bool); mask[occ] = True
+            S, T = psi[mask], psi[~mask]
+            d = np.linalg.det(S)
+            t2 = np.sum(np.abs(T) ** 2)
+            assert np.allclose(S.conj().T @ S + T.conj().T @ T, np.eye(Ne), atol=1e-12)
+            assert abs(d) ** 2 >= 1 - t2 - 1e-12
+            sumvd, dt = 0.37, 2.0
+            if 1 - t2 > 0:
+                est = 2 * dt / abs(d) ** 2 * sumvd * (abs(d.real) + abs(d.imag))
+                assert est <= 2 * dt * sumvd * np.sqrt(2) / np.sqrt(1 - t2) * (1 + 1e-12)
