@@ -126,8 +126,27 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // the bracket (kept from the signs of f) bisects instead.  Stops when |f| is at its rounding level,
 // when the step is a few ulps or when the bracket is narrower than tol.  Returns E_i and
 // f'(E_i) = 1 + V^2 sum_k 1/(E_i - eps_k)^2.
+// G lanes (1, 2 or 4 consecutive lanes, sub = lane mod G) share a root: lane sub sums the poles
+// k = sub, sub + G, ..., and the partial sums are combined by DPP exchanges whose results have
+// the same bits in every lane of the group, so the group iterates in lockstep.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+    if (G >= 2) v += dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
+    if (G >= 4) v += dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+    return v;
+}
+
+template <int G = 1>
 __device__ double secular_root(const double* eps, int M, int i, double V2, double h, double lo, double hi,
-                               double guess, double tol, double& fp) {
+                               double guess, double tol, double& fp, int sub = 0) {
     const bool has_lo = i > 0, has_hi = i < M;
     const double dl = has_lo ? eps[i - 1] : lo, dh = has_hi ? eps[i] : hi;
     const double wl = has_lo ? (has_hi ? 0.5 : 1.0) : 0.0;  // share of the linear term's slope at the lower pole
@@ -136,7 +155,7 @@ __device__ double secular_root(const double* eps, int M, int i, double V2, doubl
     for (int it = 0; it < 100; ++it) {
         // q_k > 0 below the interval, < 0 above it
         double sl = 0.0, sl2 = 0.0, sh = 0.0, sh2 = 0.0;
-        for (int k = 0; k < M; ++k) {
+        for (int k = sub; k < M; k += G) {
             const double q = rcp_nr(lam - eps[k]);
             const double q2 = q * q;
             const bool below = k < i;
@@ -144,6 +163,12 @@ __device__ double secular_root(const double* eps, int M, int i, double V2, doubl
             sh += below ? 0.0 : q;
             sl2 += below ? q2 : 0.0;
             sh2 += below ? 0.0 : q2;
+        }
+        if (G > 1) {
+            sl = group_sum<G>(sl);
+            sh = group_sum<G>(sh);
+            sl2 = group_sum<G>(sl2);
+            sh2 = group_sum<G>(sh2);
         }
         const double f = lam - h - V2 * (sl + sh);
         fp = 1.0 + V2 * (sl2 + sh2);
@@ -186,12 +211,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Eigenvalue ir of the Newns-Anderson matrix at impurity energy h: the secular root on its
 // bracket (the outer brackets reach past the band and h by the coupling norm zn = sqrt(M) V);
 // warm starts from `guess`, a cold start from below the bracket.  Also returns f'(E_ir).
+template <int G = 1>
 __device__ __forceinline__ double na_root(const double* eps, int M, double a, double V2, double zn, double h, int ir,
-                                          bool warm, double guess, double& fp) {
+                                          bool warm, double guess, double& fp, int sub = 0) {
     const double lo = (ir == 0) ? fmin(h, eps[0]) - zn - 1e-3 * (zn + 1e-300) : eps[ir - 1];
     const double hi = (ir == M) ? fmax(h, eps[M - 1]) + zn + 1e-3 * (zn + 1e-300) : eps[ir];
     const double scale = fmax(fmax(fabs(a), fabs(eps[M - 1])), fabs(h)) + zn;
-    return secular_root(eps, M, ir, V2, h, lo, hi, warm ? guess : lo - 1.0, 4.0e-16 * scale, fp);
+    return secular_root<G>(eps, M, ir, V2, h, lo, hi, warm ? guess : lo - 1.0, 4.0e-16 * scale, fp, sub);
 }
 
 template <int CTRL>
@@ -296,7 +322,7 @@ __device__ bool wave_det(double* Sre, double* Sim, int LD, int Ne, double& dre, 
 }
 
 template <int NT, int CTM>
-__global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
+__global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) void iesh_kernel(const IeshArgs A) {
     constexpr int NP = 16 * NT;  // padded state count (rows of psi / W tiles)
     constexpr int B = 64 * NT;
     constexpr int KF = 4 * NT;   // A-fragments per lane (K = NP in steps of 4)
@@ -368,14 +394,17 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
         // eigen at x (all threads); leaves E, U in LDS and returns the model scalars
         auto eigen = [&](double xx) -> Scalars {
             const Scalars s = na_scalars(A, xx);
-            const int ir = l * NT + w;  // roots dealt round-robin over the wavefronts
+            // four lanes per root, roots dealt round-robin over the wavefronts (16 NT >= N slots)
+            const int ir = (l >> 2) * NT + w, sub = l & 3;
             if (ir < N) {
                 double fp;
-                const double lam = na_root(eps, M, A.a, V2, zn, s.h, ir, warm, Eprev, fp);
-                E[ir] = lam;
-                U[ir] = 1.0 / sqrt(fp);
+                const double lam = na_root<4>(eps, M, A.a, V2, zn, s.h, ir, warm, Eprev, fp, sub);
+                if (sub == 0) {
+                    E[ir] = lam;
+                    U[ir] = 1.0 / sqrt(fp);
+                }
                 Eprev = lam;
-            } else if (ir < NP) {
+            } else if (ir < NP && sub == 0) {
                 E[ir] = 0.0;
                 U[ir] = 0.0;
             }
@@ -426,49 +455,58 @@ __global__ __launch_bounds__(64 * NT) void iesh_kernel(const IeshArgs A) {
                 }
                 __syncthreads();
             }
-            for (int a = tid; a < N; a += B) {
-                const double ea = (a == 0) ? 0.0 : eps[a - 1];
+            // four lanes per diabatic state a (4 N <= 64 NT lanes): lane sub takes the 4-electron
+            // chunks sub, sub + 4, ... and every fourth term of the IESH corrections; the quad's
+            // partial sums are combined by DPP (same bits in all four lanes)
+            {
+                const int a = tid >> 2, sub = tid & 3;
                 double pa = 0.0;
-                for (int e0 = 0; e0 < Ne; e0 += 8) {
-                    double sr[8], si[8];
+                if (a < N) {
+                    const double ea = (a == 0) ? 0.0 : eps[a - 1];
+                    for (int e0 = 4 * sub; e0 < Ne; e0 += 16) {
+                        double sr[4], si[4];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) { sr[q] = 0.0; si[q] = 0.0; }
-                    for (int b = 0; b < N; ++b) {
-                        const double u = (a == 0) ? U[b] : A.V * U[b] / (E[b] - ea);
-                        const double* row = Tb + b * CW + 2 * e0;
+                        for (int q = 0; q < 4; ++q) { sr[q] = 0.0; si[q] = 0.0; }
+                        for (int b = 0; b < N; ++b) {
+                            const double u = (a == 0) ? U[b] : A.V * U[b] / (E[b] - ea);
+                            const double* row = Tb + b * CW + 2 * e0;
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            if (e0 + q < Ne) {
-                                sr[q] += u * row[2 * q];
-                                if (A.method != 0) si[q] += u * row[2 * q + 1];
+                            for (int q = 0; q < 4; ++q) {
+                                if (e0 + q < Ne) {
+                                    sr[q] += u * row[2 * q];
+                                    if (A.method != 0) si[q] += u * row[2 * q + 1];
+                                }
                             }
                         }
-                    }
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) pa += sr[q] * sr[q] + si[q] * si[q];
-                }
-                if (A.method == 0) {
-                    for (int b = 0; b < N; ++b) {
-                        const double u = (a == 0) ? U[b] : A.V * U[b] / (E[b] - ea);
-                        pa -= u * u * pcol[b];
+                        for (int q = 0; q < 4; ++q) pa += sr[q] * sr[q] + si[q] * si[q];
                     }
-                    for (int e = 0; e < Ne; ++e) {
-                        const int sb = occ[e];
-                        const double u = (a == 0) ? U[sb] : A.V * U[sb] / (E[sb] - ea);
-                        pa += u * u;
-                    }
-                }
-                A.save_di[((size_t)slot * N + a) * A.ntraj + traj] = pa;
-                double pad;
-                if (A.method == 0) pad = isocc[a] >= 0 ? 1.0 : 0.0;  // one-hot (iesh.jl:371-375)
-                else {                                                // ehrenfest_na.jl:119-126
-                    pad = 0.0;
-                    for (int e = 0; e < Ne; ++e) {
-                        const double re = Tb[a * CW + 2 * e], im = Tb[a * CW + 2 * e + 1];
-                        pad += re * re + im * im;
+                    if (A.method == 0) {
+                        for (int b = sub; b < N; b += 4) {
+                            const double u = (a == 0) ? U[b] : A.V * U[b] / (E[b] - ea);
+                            pa -= u * u * pcol[b];
+                        }
+                        for (int e = sub; e < Ne; e += 4) {
+                            const int sb = occ[e];
+                            const double u = (a == 0) ? U[sb] : A.V * U[sb] / (E[sb] - ea);
+                            pa += u * u;
+                        }
                     }
                 }
-                A.save_ad[((size_t)slot * N + a) * A.ntraj + traj] = pad;
+                pa = group_sum<4>(pa);
+                if (a < N && sub == 0) {
+                    A.save_di[((size_t)slot * N + a) * A.ntraj + traj] = pa;
+                    double pad;
+                    if (A.method == 0) pad = isocc[a] >= 0 ? 1.0 : 0.0;  // one-hot (iesh.jl:371-375)
+                    else {                                                // ehrenfest_na.jl:119-126
+                        pad = 0.0;
+                        for (int e = 0; e < Ne; ++e) {
+                            const double re = Tb[a * CW + 2 * e], im = Tb[a * CW + 2 * e + 1];
+                            pad += re * re + im * im;
+                        }
+                    }
+                    A.save_ad[((size_t)slot * N + a) * A.ntraj + traj] = pad;
+                }
             }
             __syncthreads();
         };
