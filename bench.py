@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # counted from nqcd_device.h / fssh2.hip and checked against the compiled sub-step loop (81 FMA,
 # 29 MUL, 3 ADD): 5 Tsit5 sub-steps x 194 = 970, plus ~75 for the eigen, the coupling rotation,
 # the interpolation set-up and the hop test.  The FP32 model and Verlet work are not counted.
-FLOP_PER_STEP = 1045.0
+FLOP_PER_STEP = 988.0
 
 
 def _ensemble(L, precision, schedule):

@@ -241,22 +241,24 @@ struct Sig2 {
     double q, x, y;
 };
 
-// One Tsit5 step of size h from tau (s0 = tau - tc).  The stage derivatives are carried
-// pre-multiplied by h (K = h f), so a stage is u + sum_j a_ij K_j.
-__device__ __forceinline__ void tsit5_step2(const Interp2& P, Sig2& u, double s0, double h) {
+// One Tsit5 step of the density carried as (p, x, y) with p = q / 2, so that
+//   dp/dt = -G x,   dx/dt = dE y + G p,   dy/dt = -dE x,   G = 2 w
+// (one product per right-hand side fewer than the q form; every quantity is the q form's scaled
+// by a power of two, so the rounding is the same).  The stage derivatives are carried
+// pre-multiplied by the step h (K = h f): at stage time tau0 + c h the coefficients are
+// h dE = he0 + he1 c and h G = hg0 + hg1 c.
+__device__ __forceinline__ void tsit5_step2(double he0, double he1, double hg0, double hg1, Sig2& u) {
     using namespace ts5;
-    const double he0 = h * (P.dE0 + P.dE1 * s0), he1 = (h * h) * P.dE1;
-    const double hg0 = h * (P.w0 + P.w1 * s0), hg1 = (h * h) * P.w1;
 #define NQCD_K(c, Q, X, Y, KQ, KX, KY)             \
     {                                              \
         const double e = fma(he1, (c), he0);       \
         const double g = fma(hg1, (c), hg0);       \
-        KQ = -4.0 * (g * (X));                     \
+        KQ = -(g * (X));                           \
         KX = fma(e, (Y), g * (Q));                 \
         KY = -(e * (X));                           \
     }
     double q1, x1, y1, q2, x2, y2, q3, x3, y3, q4, x4, y4, q5, x5, y5, q6, x6, y6;
-    q1 = -4.0 * (hg0 * u.x); x1 = fma(he0, u.y, hg0 * u.q); y1 = -(he0 * u.x);
+    q1 = -(hg0 * u.x); x1 = fma(he0, u.y, hg0 * u.q); y1 = -(he0 * u.x);
     double tq = fma(a21, q1, u.q), tx = fma(a21, x1, u.x), ty = fma(a21, y1, u.y);
     NQCD_K(c1, tq, tx, ty, q2, x2, y2);
     tq = fma(a32, q2, fma(a31, q1, u.q)); tx = fma(a32, x2, fma(a31, x1, u.x)); ty = fma(a32, y2, fma(a31, y1, u.y));
@@ -280,16 +282,26 @@ __device__ __forceinline__ void tsit5_step2(const Interp2& P, Sig2& u, double s0
 }
 
 // step!(electronic_integrator, dt, true) from time t: nsub sub-steps of dt/nsub,
-// each clipped so the last lands on t + dt (bab_electronics.jl:40-43,87-90).
+// each clipped so the last lands on t + dt (bab_electronics.jl:40-43,87-90).  Sub-steps before the
+// last are never clipped (t + dt - tau >= 2 h there), so the coefficients come from products with
+// h formed once; a clipped last sub-step (hs < h, by rounding) scales them by r = hs / h.
 __device__ __forceinline__ void propagate2(const Interp2& P, Sig2& u, double t, double tc, double dt, int nsub) {
     const double h = dt / (double)nsub;
-    const double tend = t + dt;
+    const double hdE0 = h * P.dE0, hdE1 = h * P.dE1, hG0 = h * (2.0 * P.w0), hG1 = h * (2.0 * P.w1);
+    const double he1 = h * hdE1, hg1 = h * hG1;
+    Sig2 v = {0.5 * u.q, u.x, u.y};
     double tau = t;
-    for (int s = 0; s < nsub; ++s) {
-        const double hs = fmin(h, tend - tau);
-        tsit5_step2(P, u, tau - tc, hs);
-        tau = (s == nsub - 1) ? tend : tau + hs;
+    for (int s = 0; s < nsub - 1; ++s) {
+        const double s0 = tau - tc;
+        tsit5_step2(fma(hdE1, s0, hdE0), he1, fma(hG1, s0, hG0), hg1, v);
+        tau += h;
     }
+    const double hs = fmin(h, (t + dt) - tau), s0 = tau - tc;
+    const double r = (hs == h) ? 1.0 : hs / h;
+    tsit5_step2(r * fma(hdE1, s0, hdE0), (r * r) * he1, r * fma(hG1, s0, hG0), (r * r) * hg1, v);
+    u.q = 2.0 * v.q;
+    u.x = v.x;
+    u.y = v.y;
 }
 
 // ---------------------------------------------------------------------------
