@@ -170,7 +170,7 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
                       &h->mlp_partial, &h->mlp_losspart, &h->mlp_lossbuf, &h->mlp_grad, &h->fallbacks,
                       &h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
-                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid};
+                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid, &h->sum64};
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(h->ev0);
     (void)hipEventDestroy(h->ev1);
@@ -513,10 +513,10 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
     float ms = 0.0f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.kernel_ms = ms;
-    std::vector<int32_t> st(h->ntraj);
-    HIPCHK(h, hipMemcpy(st.data(), h->steps.p, sizeof(int32_t) * h->ntraj, hipMemcpyDeviceToHost));
-    int64_t tot = 0;
-    for (int32_t s : st) tot += s;
+    HIPCHK(h, h->sum64.ensure(sizeof(long long)));
+    HIPCHK(h, launch_sum_i32(h->steps.as<int32_t>(), h->ntraj, h->sum64.as<long long>(), h->stream));
+    long long tot = 0;
+    HIPCHK(h, hipMemcpy(&tot, h->sum64.p, sizeof(long long), hipMemcpyDeviceToHost));
     h->stats.traj_steps = tot;
     h->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     h->have_results = true;

@@ -21,6 +21,8 @@ hipError_t launch_fill_output2(int32_t kind, int32_t method, int64_t ntraj, int3
 // Clears `bytes` (a multiple of 4) at p with a fill kernel: save slots a trajectory never reaches
 // (it terminated first) read as zero, whatever the buffer held before.
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+// out[0] = x[0] + ... + x[n - 1] (int64)
+hipError_t launch_sum_i32(const int32_t* x, int64_t n, long long* out, hipStream_t s);
 int reduce_nchunk(int64_t ntraj);
 hipError_t launch_reduce_rows(const double* data, int32_t rows, int64_t ntraj, double* partial, double scale, double* out,
                               hipStream_t s);

@@ -172,6 +172,26 @@ __global__ __launch_bounds__(256) void zero_words(uint32_t* __restrict__ p, size
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
+// Sum of n int32 values into out[0] (one workgroup: a few microseconds for 1e6 values, and the
+// host copies 8 bytes instead of the whole array).
+__global__ __launch_bounds__(1024) void sum_i32(const int32_t* __restrict__ x, int64_t n, long long* __restrict__ out) {
+    __shared__ long long part[1024];
+    long long a = 0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) a += x[i];
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0];
+}
+
+hipError_t launch_sum_i32(const int32_t* x, int64_t n, long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_i32, dim3(1), dim3(1024), 0, s, x, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 4;
     if (!p || n == 0) return hipSuccess;
