@@ -241,15 +241,18 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
 int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* sampler, uint64_t seed);
 
 /* IESH / EhrenfestNA: replace the occupied states of the loaded trajectories by a Fermi-Dirac sample
- * at each trajectory's initial position (DynamicsVariables(sim, v, r, FermiDiracState{Adiabatic}),
- * iesh.jl:99-128, ehrenfest_na.jl:43-70): the Boltzmann-factor Metropolis chain of
- * sample_fermi_dirac_distribution (DynamicsUtils.jl:191-205) over the adiabatic energies, n_available
- * * n_electrons moves from the lowest available states, draws from Philox stream 3 of `seed`;
- * psi[state_e, e] = 1.  beta = 1/kT (INFINITY for zero temperature); available_states = NULL makes
- * every state available (get_available_states(:, n), DynamicsUtils.jl:210-214), otherwise it lists
- * 0-based states, taken as a set.  Call after load_initial or sample_initial. */
+ * at each trajectory's initial position (DynamicsVariables(sim, v, r, FermiDiracState), iesh.jl:99-183,
+ * ehrenfest_na.jl:43-70): the Boltzmann-factor Metropolis chain of sample_fermi_dirac_distribution
+ * (DynamicsUtils.jl:191-205), n_available * n_electrons moves from the lowest available states, draws
+ * from Philox stream 3 of `seed`.  basis 0 (Adiabatic): the chain weighs the adiabatic energies and
+ * psi[state_e, e] = 1.  basis 1 (Diabatic, IESH only, iesh.jl:138-183): the chain weighs the diagonal
+ * of the model matrix, each sampled diabatic state d then draws an adiabatic state with weights
+ * U_db^2 until a free one comes up, and the electron's orbital is row d of U (sign of U_d0).
+ * beta = 1/kT (INFINITY for zero temperature); available_states = NULL makes every state available
+ * (get_available_states(:, n), DynamicsUtils.jl:210-214), otherwise it lists 0-based states, taken as
+ * a set.  Call after load_initial or sample_initial. */
 int32_t nqcd_mi355x_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* available_states,
-                                       int32_t n_available, uint64_t seed);
+                                       int32_t n_available, int32_t basis, uint64_t seed);
 
 /* Integrate every trajectory over [t0, t1] with the method's callbacks (hopping after each
  * step, surface_hopping.jl:2-16; termination, callbacks.jl:15-29) and record the save

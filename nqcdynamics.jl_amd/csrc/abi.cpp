@@ -330,13 +330,13 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
 }
 
 int32_t nqcd_mi355x_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* available_states,
-                                       int32_t n_available, uint64_t seed) {
+                                       int32_t n_available, int32_t basis, uint64_t seed) {
     if (!h) return fail(h, NQCD_ERR_ARG, "null argument");
     if (!h->have_problem) return fail(h, NQCD_ERR_STATE, "set_problem must be called first");
     if (!is_ie(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "Fermi-Dirac occupations are for IESH / EhrenfestNA");
     if (available_states && n_available < 1) return fail(h, NQCD_ERR_ARG, "n_available must be positive");
     HIPCHK(h, hipSetDevice(h->device));
-    const int32_t rc = ie_sample_fermi_dirac(h, beta, available_states, n_available, seed);
+    const int32_t rc = ie_sample_fermi_dirac(h, beta, available_states, n_available, basis, seed);
     if (rc == NQCD_OK) h->have_results = false;
     return rc;
 }

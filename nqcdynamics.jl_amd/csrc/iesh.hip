@@ -890,9 +890,12 @@ __global__ __launch_bounds__(64) void wave_det_kernel(const double* __restrict__
     }
 }
 
-// Adiabatic energies at each trajectory's initial position (the eigenvalues the Fermi-Dirac
-// occupation sampler weighs, iesh.jl:113-114): one wavefront per trajectory, root ir on lane ir % 64.
-__global__ __launch_bounds__(256) void ie_initial_energies(const IeshArgs A, double* __restrict__ E0) {
+// The energies the Fermi-Dirac occupation chain weighs, at each trajectory's initial position: the
+// adiabatic energies (iesh.jl:113-114), or for the diabatic basis the diagonal of the model matrix,
+// [h(x0), eps_0, ..., eps_{M-1}] (iesh.jl:150-153), followed then by the adiabatic energies E_b and
+// the impurity components u_b of the eigenvectors (row stride 3N; N for the adiabatic basis).  One
+// wavefront per trajectory, root ir on lane ir % 64.
+__global__ __launch_bounds__(256) void ie_initial_energies(const IeshArgs A, int diabatic, double* __restrict__ W) {
     __shared__ double eps[kIeMaxStates];
     for (int k = threadIdx.x; k < A.M; k += blockDim.x) eps[k] = A.a + k * A.de;
     __syncthreads();
@@ -900,9 +903,17 @@ __global__ __launch_bounds__(256) void ie_initial_energies(const IeshArgs A, dou
     if (traj >= A.ntraj) return;
     const Scalars s = na_scalars(A, A.r0[traj]);
     const double zn = sqrt((double)A.M) * A.V;
+    double* row = W + traj * (diabatic ? 3 * A.N : A.N);
     for (int ir = threadIdx.x & 63; ir < A.N; ir += 64) {
         double fp;
-        E0[traj * A.N + ir] = na_root(eps, A.M, A.a, A.V * A.V, zn, s.h, ir, false, 0.0, fp);
+        const double lam = na_root(eps, A.M, A.a, A.V * A.V, zn, s.h, ir, false, 0.0, fp);
+        if (diabatic) {
+            row[ir] = (ir == 0) ? s.h : eps[ir - 1];
+            row[A.N + ir] = lam;
+            row[2 * A.N + ir] = 1.0 / sqrt(fp);
+        } else {
+            row[ir] = lam;
+        }
     }
 }
 
@@ -927,13 +938,14 @@ __device__ __forceinline__ int select_bit64(uint64_t w, int k) {
 // bits in order.
 __global__ __launch_bounds__(256) void ie_fermi_dirac_chain(int64_t ntraj, int N, int Ne, uint64_t avail_lo,
                                                             uint64_t avail_hi, double beta, uint64_t key,
-                                                            const double* __restrict__ E0, int32_t* __restrict__ occ0) {
+                                                            const double* __restrict__ E0, int stride,
+                                                            int32_t* __restrict__ occ0) {
     extern __shared__ int slots[];  // Ne x blockDim.x
     const int tid = threadIdx.x, B = blockDim.x;
     const int64_t traj = (int64_t)blockIdx.x * B + tid;
     if (traj >= ntraj) return;
     const int navail = __popcll(avail_lo) + __popcll(avail_hi), nfree = navail - Ne;
-    const double* E = E0 + traj * N;
+    const double* E = E0 + traj * stride;
     uint64_t occ_lo = 0, occ_hi = 0;
     for (int e = 0; e < Ne; ++e) {  // the first Ne available states
         const uint64_t flo = avail_lo & ~occ_lo, fhi = avail_hi & ~occ_hi;
@@ -963,6 +975,65 @@ __global__ __launch_bounds__(256) void ie_fermi_dirac_chain(int64_t ntraj, int N
     int e = 0;
     for (uint64_t w = occ_lo; w; w &= w - 1) occ0[traj * Ne + e++] = __ffsll((unsigned long long)w) - 1;
     for (uint64_t w = occ_hi; w; w &= w - 1) occ0[traj * Ne + e++] = 64 + __ffsll((unsigned long long)w) - 1;
+}
+
+// Diabatic Fermi-Dirac occupations to IESH orbitals (iesh.jl:159-182), one lane per trajectory:
+// for each sampled diabatic state d (ascending), an adiabatic state is drawn with weights U_db^2
+// (U_db = component d of eigenvector b: u_b for the impurity, V u_b / (E_b - eps_{d-1}) for a bath
+// level) until one not yet taken comes up -- StatsBase's sample(Weights(w)): t = rand() * sum(w), the
+// first b whose running sum reaches t -- and the electron's orbital is row d of U times the sign of
+// U_d0, psi_b = U_db U_d0 / sqrt(U_d0^2) (real).  Draw k of a trajectory is Philox counter
+// (traj, 2^31 + k, STREAM_OCCUPATION), u = u53(o0, o1).  The orbitals stay in sampling order and
+// the occupations are sorted, as the reference leaves them.  A trajectory that needs more than
+// 4096 draws for one electron (free states of vanishing weight) gets occupation -1.
+__global__ __launch_bounds__(256) void ie_diabatic_orbitals(int64_t ntraj, int N, int Ne, double V, double a, double de,
+                                                            uint64_t key, const double* __restrict__ W,
+                                                            const int32_t* __restrict__ dstate,
+                                                            int32_t* __restrict__ occ0, double* __restrict__ psi0) {
+    const int64_t traj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (traj >= ntraj) return;
+    const double* E = W + traj * 3 * N + N;
+    const double* u = E + N;
+    uint64_t tk_lo = 0, tk_hi = 0;
+    uint32_t draw = 0;
+    bool failed = false;
+    for (int i = 0; i < Ne && !failed; ++i) {
+        const int d = dstate[traj * Ne + i];
+        const double ed = (d == 0) ? 0.0 : a + (d - 1) * de;
+        auto Ud = [&](int b) { return (d == 0) ? u[b] : V * u[b] / (E[b] - ed); };
+        double sw = 0.0;
+        for (int b = 0; b < N; ++b) { const double x = Ud(b); sw += x * x; }
+        int pick = -1;
+        for (int tries = 0; tries < 4096 && pick < 0; ++tries) {
+            uint32_t c0 = (uint32_t)traj, c1 = (uint32_t)(traj >> 32), c2 = 0x80000000u + draw++, c3 = STREAM_OCCUPATION;
+            philox4x32_10(c0, c1, c2, c3, (uint32_t)key, (uint32_t)(key >> 32));
+            const double t = u53(c0, c1) * sw;
+            int k = 0;
+            double x = Ud(0), cw = x * x;
+            while (cw < t && k < N - 1) {
+                ++k;
+                x = Ud(k);
+                cw += x * x;
+            }
+            const bool taken = (k < 64) ? (tk_lo >> k) & 1 : (tk_hi >> (k - 64)) & 1;
+            if (!taken) pick = k;
+        }
+        if (pick < 0) { failed = true; break; }
+        if (pick < 64) tk_lo |= 1ull << pick; else tk_hi |= 1ull << (pick - 64);
+        const double u0 = Ud(0), nrm = sqrt(u0 * u0);
+        for (int b = 0; b < N; ++b) {
+            double* p = psi0 + (((size_t)traj * N + b) * Ne + i) * 2;
+            p[0] = (Ud(b) * u0) / nrm;
+            p[1] = 0.0;
+        }
+    }
+    if (failed) {
+        for (int e = 0; e < Ne; ++e) occ0[traj * Ne + e] = -1;
+        return;
+    }
+    int e = 0;
+    for (uint64_t w = tk_lo; w; w &= w - 1) occ0[traj * Ne + e++] = __ffsll((unsigned long long)w) - 1;
+    for (uint64_t w = tk_hi; w; w &= w - 1) occ0[traj * Ne + e++] = 64 + __ffsll((unsigned long long)w) - 1;
 }
 
 // S as two planes with an odd row stride (determinant), or interleaved Ne x Ne (inverse)
@@ -1106,12 +1177,16 @@ int32_t ie_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S, uint64_t
     return NQCD_OK;
 }
 
-int32_t ie_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* avail, int32_t navail, uint64_t seed) {
+int32_t ie_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* avail, int32_t navail, int32_t basis,
+                              uint64_t seed) {
     if (!h->have_initial || h->ntraj < 1)
         return fail(h, NQCD_ERR_STATE, "load or sample the nuclear initial conditions first");
     if (!h->model_set[0] || h->models[0].kind != NQCD_MODEL_NEWNS_ANDERSON)
         return fail(h, NQCD_ERR_STATE, "Fermi-Dirac occupations need the Newns-Anderson model in slot 0");
     if (!(beta >= 0.0)) return fail(h, NQCD_ERR_ARG, "beta must be >= 0 (inf for zero temperature)");
+    if (basis != 0 && basis != 1) return fail(h, NQCD_ERR_ARG, "basis must be 0 (adiabatic) or 1 (diabatic)");
+    if (basis == 1 && h->prob.method != NQCD_METHOD_IESH)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "diabatic Fermi-Dirac occupations are defined for IESH (iesh.jl:138)");
     const IeModel md = ie_model(h);
     const int N = h->prob.n_states, Ne = h->prob.n_electrons;
     if (md.M + 1 != N) return fail(h, NQCD_ERR_ARG, "n_states must equal M + 1 for the Newns-Anderson model");
@@ -1135,15 +1210,33 @@ int32_t ie_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* avai
     A.N = N; A.M = md.M; A.Ne = Ne;
     A.mw2 = md.mw2; A.g = md.g; A.dG = md.dG; A.V = md.V; A.a = md.a; A.de = md.de;
     A.r0 = h->r0.as<double>();
-    HIPCHK(h, h->tmp.ensure(sizeof(double) * (size_t)N * n));
-    hipLaunchKernelGGL(ie_initial_energies, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, A, h->tmp.as<double>());
+    const int stride = basis ? 3 * N : N;
+    HIPCHK(h, h->tmp.ensure(sizeof(double) * (size_t)stride * n));
+    hipLaunchKernelGGL(ie_initial_energies, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, A, basis,
+                       h->tmp.as<double>());
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, h->state0.ensure(sizeof(int32_t) * (size_t)Ne * n));
+    HIPCHK(h, h->state.ensure(sizeof(int32_t) * (size_t)Ne * n));
+    int32_t* chain_out = basis ? h->state.as<int32_t>() : h->state0.as<int32_t>();  // diabatic states: scratch
     const int block = 256;
     hipLaunchKernelGGL(ie_fermi_dirac_chain, dim3((unsigned)((n + block - 1) / block)), dim3(block),
                        sizeof(int) * (size_t)Ne * block, h->stream, n, N, Ne, am[0], am[1], beta, seed,
-                       h->tmp.as<double>(), h->state0.as<int32_t>());
+                       h->tmp.as<double>(), stride, chain_out);
     HIPCHK(h, hipGetLastError());
+    if (basis) {
+        HIPCHK(h, h->sig0.ensure(sizeof(double) * 2 * (size_t)N * Ne * n));
+        hipLaunchKernelGGL(ie_diabatic_orbitals, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, h->stream,
+                           n, N, Ne, md.V, md.a, md.de, seed, h->tmp.as<double>(), h->state.as<int32_t>(),
+                           h->state0.as<int32_t>(), h->sig0.as<double>());
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        std::vector<int32_t> occ((size_t)Ne * n);
+        HIPCHK(h, hipMemcpy(occ.data(), h->state0.p, sizeof(int32_t) * occ.size(), hipMemcpyDeviceToHost));
+        for (int32_t o : occ)
+            if (o < 0) return fail(h, NQCD_ERR_ARG, "diabatic Fermi-Dirac occupations: no free adiabatic state of nonzero weight");
+        h->ie_psi_given = true;
+        return NQCD_OK;
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->ie_psi_given = false;
     return NQCD_OK;

@@ -115,17 +115,22 @@ function problem_desc(sim, alg, dt, ensemblealg, masses)
 end
 
 # --- initial conditions from the Selection closure (selections.jl:38-42,70-73) --------------
-# A ProductDistribution whose electronic part is FermiDiracState{Adiabatic}: the nuclear part is
-# sampled as the Selection samples it, and the occupations are drawn on the device
-# (nqcd_mi355x_sample_fermi_dirac) instead of one sample_fermi_dirac_distribution per trajectory
-# on the host (iesh.jl:99-128, ehrenfest_na.jl:43-70).
+# A ProductDistribution whose electronic part is a FermiDiracState (Adiabatic, or Diabatic for
+# IESH): the nuclear part is sampled as the Selection samples it, and the occupations (and, for the
+# Diabatic form, the orbitals) are drawn on the device (nqcd_mi355x_sample_fermi_dirac) instead of
+# one sample_fermi_dirac_distribution per trajectory on the host (iesh.jl:99-183,
+# ehrenfest_na.jl:43-70).
 function fermi_dirac_selection(prob, sim)
     method_code(sim) in (METHOD_IESH, METHOD_EHRENFEST_NA) || return nothing
     sel = prob.prob_func
     (sel isa Ensembles.RandomSelection || sel isa Ensembles.OrderedSelection) || return nothing
     d = sel.distribution
-    (d isa NQCDistributions.ProductDistribution &&
-        d.electronic isa NQCDistributions.FermiDiracState{NQCDistributions.Adiabatic}) || return nothing
+    d isa NQCDistributions.ProductDistribution || return nothing
+    if d.electronic isa NQCDistributions.FermiDiracState{NQCDistributions.Diabatic}
+        method_code(sim) == METHOD_IESH || return nothing
+    else
+        d.electronic isa NQCDistributions.FermiDiracState{NQCDistributions.Adiabatic} || return nothing
+    end
     NQCModels.fermilevel(sim) ≈ d.electronic.fermi_level ||
         error("Fermi level of model and distribution do not match")      # iesh.jl:103-112
     return sel
@@ -143,9 +148,10 @@ function load_fermi_dirac!(h, sel, sim, trajectories, seed)
         h, trajectories, r, v, C_NULL, C_NULL, st, C_NULL))
     av = d.electronic.available_states
     avail = av isa Colon ? Int32[] : Int32.(collect(DynamicsUtils.get_available_states(av, NQCModels.nstates(sim)))) .- Int32(1)
+    basis = d.electronic isa NQCDistributions.FermiDiracState{NQCDistributions.Diabatic} ? Int32(1) : Int32(0)
     check(h, ccall((:nqcd_mi355x_sample_fermi_dirac, libnqcd), Int32,
-        (Ptr{Cvoid}, Float64, Ptr{Int32}, Int32, UInt64),
-        h, Float64(d.electronic.β), isempty(avail) ? C_NULL : avail, length(avail), seed))
+        (Ptr{Cvoid}, Float64, Ptr{Int32}, Int32, Int32, UInt64),
+        h, Float64(d.electronic.β), isempty(avail) ? C_NULL : avail, length(avail), basis, seed))
 end
 
 function load_initial!(h, prob, sim, trajectories, seed)

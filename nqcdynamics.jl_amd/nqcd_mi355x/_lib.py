@@ -101,7 +101,7 @@ def lib():
     L.nqcd_mi355x_set_model.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_load_initial.argtypes = [H, C.c_int64, dp, dp, dp, dp, ip, ip]
     L.nqcd_mi355x_sample_initial.argtypes = [H, C.POINTER(SamplerDesc), C.c_uint64]
-    L.nqcd_mi355x_sample_fermi_dirac.argtypes = [H, C.c_double, ip, C.c_int32, C.c_uint64]
+    L.nqcd_mi355x_sample_fermi_dirac.argtypes = [H, C.c_double, ip, C.c_int32, C.c_int32, C.c_uint64]
     L.nqcd_mi355x_run.argtypes = [H, C.c_double, C.c_double, C.c_int32, C.POINTER(TermDesc), C.c_uint64]
     L.nqcd_mi355x_reduce.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_download.argtypes = [H, C.c_int32, C.c_void_p, C.c_int64]
@@ -227,12 +227,13 @@ class Ensemble:
         self._check(self._L.nqcd_mi355x_sample_initial(self._h, C.byref(s), C.c_uint64(seed)))
         self.ntraj = ntraj
 
-    def sample_fermi_dirac(self, beta, seed, available_states=None):
+    def sample_fermi_dirac(self, beta, seed, available_states=None, diabatic=False):
         """IESH / EhrenfestNA: occupations of the loaded trajectories from the Fermi-Dirac chain at
-        inverse temperature beta (numpy.inf for 0 K); available_states are 0-based (None = all)."""
+        inverse temperature beta (numpy.inf for 0 K); available_states are 0-based (None = all);
+        diabatic=True samples diabatic states and builds the orbitals from them (IESH)."""
         av = None if available_states is None else np.ascontiguousarray(available_states, dtype=np.int32)
         self._check(self._L.nqcd_mi355x_sample_fermi_dirac(self._h, float(beta), _ip(av), 0 if av is None else len(av),
-                                                           C.c_uint64(seed)))
+                                                           1 if diabatic else 0, C.c_uint64(seed)))
 
     def run(self, t0, t1, key, save_every=0, term=None):
         td = None

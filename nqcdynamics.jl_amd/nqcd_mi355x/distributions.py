@@ -34,8 +34,9 @@ class FermiDiracState:
     EhrenfestNA: each trajectory's occupied adiabatic orbitals are drawn at its initial position by
     the Boltzmann-factor chain of sample_fermi_dirac_distribution (DynamicsUtils.jl:191-205), on the
     device (iesh.jl:99-128, ehrenfest_na.jl:43-70).  temperature is kT in Hartree (0 leaves the
-    lowest orbitals occupied); available_states are 1-based like the reference's (None = all).  Only
-    the Adiabatic basis is provided."""
+    lowest orbitals occupied); available_states are 1-based like the reference's (None = all).  In
+    the Diabatic basis (IESH, iesh.jl:138-183) the chain samples diabatic states and each electron's
+    orbital is the corresponding diabatic state in the adiabatic basis."""
 
     def __init__(self, fermi_level, temperature, available_states=None, basis=None):
         self.fermi_level, self.temperature = float(fermi_level), float(temperature)
@@ -43,6 +44,10 @@ class FermiDiracState:
             raise ValueError("temperature must be >= 0")
         self.available_states = None if available_states is None else [int(s) for s in available_states]
         self.basis = basis if basis is not None else Adiabatic()
+
+    @property
+    def diabatic(self):
+        return isinstance(self.basis, Diabatic) or self.basis is Diabatic
 
     @property
     def beta(self):

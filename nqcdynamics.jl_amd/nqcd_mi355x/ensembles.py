@@ -108,8 +108,8 @@ def _fermi_dirac(sim, dist):
         return None
     if _kind(sim) != "ie":
         raise ValueError("FermiDiracState initial conditions are for IESH / EhrenfestNA simulations")
-    if not (isinstance(el.basis, Adiabatic) or el.basis is Adiabatic):
-        raise ValueError("EnsembleMI355X provides FermiDiracState in the Adiabatic basis")
+    if el.diabatic and not isinstance(sim.method, AdiabaticIESH):
+        raise ValueError("FermiDiracState in the Diabatic basis is defined for AdiabaticIESH (iesh.jl:138)")
     ef = sim.model.fermi_level
     if abs(ef - el.fermi_level) > np.sqrt(np.finfo(float).eps) * max(abs(ef), abs(el.fermi_level)):
         raise ValueError(f"Fermi level of model and distribution do not match: distribution {el.fermi_level}, "
@@ -139,7 +139,7 @@ def _initial_ie(E, sim, dist, trajectories, selection, seed):
         raise ValueError(f"unsupported distribution {type(nuc).__name__}")
     if fd is not None:
         avail = None if fd.available_states is None else np.asarray(fd.available_states) - 1
-        E.sample_fermi_dirac(fd.beta, seed, avail)
+        E.sample_fermi_dirac(fd.beta, seed, avail, diabatic=fd.diabatic)
 
 
 def _initial_rp(E, sim, dist, trajectories, selection, seed):

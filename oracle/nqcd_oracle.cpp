@@ -1303,6 +1303,59 @@ OR_API int32_t oracle_fermi_dirac_occupations(const double* E, int32_t Ne, const
     return 0;
 }
 
+OR_API void oracle_na_matrix(const ONADesc* md, double x, double* Vout) {
+    std::vector<double> V; double dh, U0, dU0;
+    na_matrices(*md, x, V, dh, U0, dU0);
+    std::copy(V.begin(), V.end(), Vout);
+}
+
+// DynamicsVariables(sim, v, r, FermiDiracState{Diabatic}) for IESH (iesh.jl:138-183), literally:
+// diabatic_state = the chain above over the diagonal of the model matrix (Ediab); then for each
+// electron i, adiabatic_density = U^T diagm(e_d) U (U[a * N + b] = component a of eigenvector b),
+// adiabatic_population = its diagonal, s = sample(Weights(adiabatic_population)) until s is not
+// yet in adiabatic_state, psi[:, i] = adiabatic_density[:, 1] ./ sqrt(adiabatic_population[1]);
+// finally sort!(adiabatic_state) (psi's columns keep their order).  StatsBase's weighted sample
+// draws t = rand() * sum(w) and returns the first index whose running sum reaches t (the last at
+// most); rand() is Philox counter (traj, 2^31 + k, STREAM_OCCUPATION) for the k-th draw of the
+// trajectory, u53(o0, o1).  psi is N x Ne row-major (real).  Returns 0, -1 as the chain, or -2
+// after 2^20 draws for one electron.
+OR_API int32_t oracle_fermi_dirac_diabatic(const double* Ediab, const double* U, int32_t N, int32_t Ne,
+                                           const int32_t* avail, int32_t navail, double beta, uint64_t traj,
+                                           uint64_t key, int32_t* occ, double* psi) {
+    std::vector<int32_t> dstate(Ne);
+    if (oracle_fermi_dirac_occupations(Ediab, Ne, avail, navail, beta, traj, key, dstate.data()) != 0) return -1;
+    std::vector<int32_t> ad(Ne, -1);
+    std::vector<double> dens((size_t)N * N), pop(N);
+    uint32_t draw = 0;
+    for (int i = 0; i < Ne; ++i) {
+        const int d = dstate[i];
+        for (int b = 0; b < N; ++b)
+            for (int c = 0; c < N; ++c) {
+                double acc = 0.0;
+                for (int a = 0; a < N; ++a) acc += U[(size_t)a * N + b] * (a == d ? 1.0 : 0.0) * U[(size_t)a * N + c];
+                dens[(size_t)b * N + c] = acc;
+            }
+        double sw = 0.0;
+        for (int b = 0; b < N; ++b) { pop[b] = dens[(size_t)b * N + b]; sw += pop[b]; }
+        int pick = -1;
+        for (int tries = 0; tries < (1 << 20) && pick < 0; ++tries) {
+            uint32_t o[4];
+            philox_draw(traj, 0x80000000u + draw++, STREAM_OCCUPATION, key, o);
+            const double t = u53(o[0], o[1]) * sw;
+            int k = 0;
+            double cw = pop[0];
+            while (cw < t && k < N - 1) { ++k; cw += pop[k]; }
+            if (std::find(ad.begin(), ad.end(), k) == ad.end()) pick = k;
+        }
+        if (pick < 0) return -2;
+        ad[i] = pick;
+        for (int b = 0; b < N; ++b) psi[(size_t)b * Ne + i] = dens[(size_t)b * N + 0] / std::sqrt(pop[0]);
+    }
+    std::sort(ad.begin(), ad.end());
+    std::copy(ad.begin(), ad.end(), occ);
+    return 0;
+}
+
 OR_API void oracle_na_eigen(const ONADesc* md, double x, double* E, double* U, double* dh) {
     std::vector<double> V; double U0, dU0;
     na_matrices(*md, x, V, *dh, U0, dU0);

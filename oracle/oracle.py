@@ -323,6 +323,35 @@ def fermi_dirac_occupations(E, nelec, beta, traj, key, available=None):
     return occ
 
 
+def na_matrix(md, x):
+    """The Newns-Anderson model matrix at x (impurity first, then the bath levels)."""
+    N = md.M + 1
+    V = np.zeros((N, N))
+    f = lib().oracle_na_matrix
+    f.argtypes = [C.POINTER(ONADesc), C.c_double, C.POINTER(C.c_double)]
+    f(C.byref(md), x, _p(V))
+    return V
+
+
+def fermi_dirac_diabatic(Ediab, U, nelec, beta, traj, key, available=None):
+    """(sorted occupied adiabatic states, psi (N, n_e) real) of oracle_fermi_dirac_diabatic."""
+    Ediab = np.ascontiguousarray(Ediab, dtype=np.float64)
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    N = len(Ediab)
+    av = np.arange(N, dtype=np.int32) if available is None else np.ascontiguousarray(available, dtype=np.int32)
+    occ = np.zeros(nelec, np.int32)
+    psi = np.zeros((N, nelec))
+    f = lib().oracle_fermi_dirac_diabatic
+    f.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                  C.c_double, C.c_uint64, C.c_uint64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    f.restype = C.c_int32
+    rc = f(_p(Ediab), _p(U), N, nelec, av.ctypes.data_as(C.POINTER(C.c_int32)), len(av), float(beta),
+           C.c_uint64(traj), C.c_uint64(key), occ.ctypes.data_as(C.POINTER(C.c_int32)), _p(psi))
+    if rc != 0:
+        raise ValueError(f"oracle_fermi_dirac_diabatic failed ({rc})")
+    return occ, psi
+
+
 def iesh_run_ensemble(method, md, nelec, mass, r0, v0, occ0, psi0=None, dt=1.0, nsteps=100, key=0,
                       rescaling=RESCALE_STANDARD, estimate=True, edc=False, edc_C=0.1, save_every=0, nthreads=None):
     ntraj = len(r0)

@@ -180,7 +180,7 @@ def test_device_determinant_matches_numpy():
         E.close()
 
 
-def _fd_occupations(L, method, M, Ne, r0, beta, seed, avail=None, nsteps=0, key=0):
+def _fd_occupations(L, method, M, Ne, r0, beta, seed, avail=None, nsteps=0, key=0, diabatic=False):
     """Device Fermi-Dirac occupations at r0 (and, with nsteps > 0, the run that follows them)."""
     E = L.Ensemble(0)
     try:
@@ -188,7 +188,7 @@ def _fd_occupations(L, method, M, Ne, r0, beta, seed, avail=None, nsteps=0, key=
         E.set_model(0, L.MODEL_NEWNS_ANDERSON, _params(M))
         v0 = np.full(len(r0), 5e-3)
         E.load_initial_iesh(r0, v0, np.tile(np.arange(Ne), (len(r0), 1)))
-        E.sample_fermi_dirac(beta, seed, avail)
+        E.sample_fermi_dirac(beta, seed, avail, diabatic=diabatic)
         E.run(0.0, nsteps * 2.0, key, save_every=10 if nsteps else 0)
         out = {"occ": E.download(L.FIELD_STATE), "r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
                "psi": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
@@ -282,3 +282,49 @@ def test_iesh_given_wavefunction_parity(oracle, key, orthonormal):
     c = _cpu(oracle, 0, M, Ne, r0, v0, occ0, 2.0, nsteps, key, psi0=psi)
     assert c["nhops"].sum() >= 1
     _compare(g, c)
+
+
+def _fd_diabatic_oracle(oracle, M, Ne, r0, beta, seed, avail=None):
+    W = 6 * MODEL["Gamma"] / 2
+    md = oracle.na_model(M=M, W=W, m=MODEL["m"], omega=MODEL["omega"], g=MODEL["g"], dG=MODEL["dG"], Gamma=MODEL["Gamma"])
+    occ, psi = [], []
+    for t, x in enumerate(r0):
+        E, U, _ = oracle.na_eigen(md, x)
+        U = U * np.sign(U[0])  # impurity component positive, as on the device
+        o, p = oracle.fermi_dirac_diabatic(np.diag(oracle.na_matrix(md, x)), U, Ne, beta, t, seed, avail)
+        occ.append(o); psi.append(p)
+    return np.stack(occ), np.stack(psi)
+
+
+@pytest.mark.parametrize("M,Ne,beta,avail", [(40, 20, 1 / KT300, None), (100, 50, 1 / KT300, None),
+                                             (30, 15, 1 / (4 * KT300), [0, 2, 5] + list(range(9, 31))),
+                                             (127, 20, 1 / KT300, None)])
+def test_fermi_dirac_diabatic_matches_oracle(oracle, M, Ne, beta, avail):
+    """FermiDiracState{Diabatic} (iesh.jl:138-183) on the device -- the chain over the model matrix's
+    diagonal, the weighted draws of adiabatic states, orbitals from rows of the closed-form
+    eigenvectors -- against the oracle's literal restatement (Jacobi eigenvectors in the same gauge):
+    identical occupations, orbitals within 1e-12."""
+    from nqcd_mi355x import _lib as L
+    r0 = np.random.default_rng(M + 1).normal(3.0, 2.0, size=64)
+    g, _ = _fd_occupations(L, L.METHOD_IESH, M, Ne, r0, beta, 0xD1AB, avail, diabatic=True)
+    occ, psi = _fd_diabatic_oracle(oracle, M, Ne, r0, beta, 0xD1AB, avail)
+    assert np.array_equal(g["occ"], occ)
+    assert np.allclose(g["psi"], psi, atol=1e-12)
+
+
+def test_fermi_dirac_diabatic_then_iesh_parity(oracle, key):
+    """Diabatic Fermi-Dirac initial orbitals (device) followed by IESH, against the oracle run from the
+    oracle's occupations and orbitals: identical hops, 1e-9 relative."""
+    from nqcd_mi355x import _lib as L
+    M, Ne, nsteps = 40, 20, 150
+    r0 = np.random.default_rng(6).normal(6.0, 1.0, size=32)
+    g, v0 = _fd_occupations(L, L.METHOD_IESH, M, Ne, r0, 1 / KT300, 78, nsteps=nsteps, key=key, diabatic=True)
+    occ, psi = _fd_diabatic_oracle(oracle, M, Ne, r0, 1 / KT300, 78)
+    c = _cpu(oracle, 0, M, Ne, r0, v0, occ, 2.0, nsteps, key, psi0=psi.astype(complex))
+    _compare(g, c)
+
+
+def test_fermi_dirac_diabatic_is_iesh_only():
+    from nqcd_mi355x import _lib as L
+    with pytest.raises(L.NQCDError, match="IESH"):
+        _fd_occupations(L, L.METHOD_EHRENFEST_NA, 40, 20, np.zeros(4), 1.0, 1, diabatic=True)

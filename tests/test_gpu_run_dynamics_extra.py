@@ -34,11 +34,13 @@ def test_iesh_run_dynamics_mean_populations(oracle, key):
     assert np.allclose(res["Time"], np.arange(nsteps // 10 + 1) * 10 * dt)
 
 
-@pytest.mark.parametrize("method", ["AdiabaticIESH", "EhrenfestNA"])
-def test_run_dynamics_fermi_dirac_initial_state(oracle, key, method):
-    """DynamicalDistribution * FermiDiracState(fermi_level, kT): thermal occupations drawn on the device
-    at each trajectory's position (iesh.jl:99-128, ehrenfest_na.jl:43-70), then the dynamics; mean
-    diabatic populations against the oracle run from the oracle's occupations of the same streams."""
+@pytest.mark.parametrize("method,basis", [("AdiabaticIESH", "Adiabatic"), ("EhrenfestNA", "Adiabatic"),
+                                          ("AdiabaticIESH", "Diabatic")])
+def test_run_dynamics_fermi_dirac_initial_state(oracle, key, method, basis):
+    """DynamicalDistribution * FermiDiracState(fermi_level, kT[, Diabatic()]): thermal occupations (and
+    for the Diabatic form the orbitals) drawn on the device at each trajectory's position
+    (iesh.jl:99-183, ehrenfest_na.jl:43-70), then the dynamics; mean diabatic populations against the
+    oracle run from the oracle's occupations of the same streams."""
     M, ntraj, dt, nsteps = 40, 48, 2.0, 100
     W = 6 * 6.4e-3 / 2
     kT = 3.166811563e-6 * 300.0
@@ -48,16 +50,25 @@ def test_run_dynamics_fermi_dirac_initial_state(oracle, key, method):
     r0 = rng.normal(6.0, 1.0, size=ntraj)
     v0 = rng.normal(0.0, 1e-2, size=ntraj)
     dist = N.DynamicalDistribution([np.array([[x]]) for x in v0], [np.array([[x]]) for x in r0], (1, 1)) * \
-        N.FermiDiracState(0.0, kT)
+        N.FermiDiracState(0.0, kT, basis=getattr(N, basis)())
     res = N.run_dynamics(sim, (0.0, nsteps * dt), dist, trajectories=ntraj, dt=dt, saveat=10 * dt,
                          output=N.OutputDiabaticPopulation, reduction=N.MeanReduction(),
                          ensemble_algorithm=N.EnsembleMI355X(seed=key))
     md = oracle.na_model(M=M, W=W)
-    occ0 = np.stack([oracle.fermi_dirac_occupations(oracle.na_eigen(md, x)[0], M // 2, 1 / kT, t, key)
-                     for t, x in enumerate(r0)])
-    assert not np.all(occ0 == np.arange(M // 2))
-    c = oracle.iesh_run_ensemble(0 if method == "AdiabaticIESH" else 1, md, M // 2, 2000.0, r0, v0, occ0, dt=dt,
-                                 nsteps=nsteps, key=key, save_every=10)
+    psi0 = None
+    if basis == "Adiabatic":
+        occ0 = np.stack([oracle.fermi_dirac_occupations(oracle.na_eigen(md, x)[0], M // 2, 1 / kT, t, key)
+                         for t, x in enumerate(r0)])
+        assert not np.all(occ0 == np.arange(M // 2))
+    else:
+        occ0, psi0 = [], []
+        for t, x in enumerate(r0):
+            U = oracle.na_eigen(md, x)[1]
+            o, p = oracle.fermi_dirac_diabatic(np.diag(oracle.na_matrix(md, x)), U * np.sign(U[0]), M // 2, 1 / kT, t, key)
+            occ0.append(o); psi0.append(p.astype(complex))
+        occ0, psi0 = np.stack(occ0), np.stack(psi0)
+    c = oracle.iesh_run_ensemble(0 if method == "AdiabaticIESH" else 1, md, M // 2, 2000.0, r0, v0, occ0, psi0=psi0,
+                                 dt=dt, nsteps=nsteps, key=key, save_every=10)
     assert np.allclose(np.array(res["OutputDiabaticPopulation"]), c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-12)
 
 

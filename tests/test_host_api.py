@@ -88,7 +88,7 @@ def test_newns_anderson_model_and_iesh_options():
 
 def test_fermi_dirac_state_checks():
     """FermiDiracState initial conditions: beta = 1/kT (inf at 0 K); the model's and the distribution's
-    Fermi levels must agree (test/Dynamics/iesh.jl:43-46); IESH / EhrenfestNA only; Adiabatic basis;
+    Fermi levels must agree (test/Dynamics/iesh.jl:43-46); IESH / EhrenfestNA only; the Diabatic basis for IESH only;
     available states within the system (DynamicsUtils.jl:211-214).  All raise before the device opens."""
     assert N.FermiDiracState(0.0, 0.0).beta == np.inf
     assert np.isclose(N.FermiDiracState(0.0, 1e-3).beta, 1e3)
@@ -100,8 +100,9 @@ def test_fermi_dirac_state_checks():
     nuc = N.DynamicalDistribution(N.Normal(0.0, 1e-2), N.Normal(8.0, 1.0), (1, 1))
     with pytest.raises(ValueError, match="Fermi level"):
         N.run_dynamics(ie, (0.0, 10.0), nuc * N.FermiDiracState(0.0, 9.5e-4), output=N.OutputDiabaticPopulation)
-    with pytest.raises(ValueError, match="Adiabatic"):
-        N.run_dynamics(ie, (0.0, 10.0), nuc * N.FermiDiracState(na.fermi_level, 9.5e-4, basis=N.Diabatic()),
+    ena = N.Simulation[N.EhrenfestNA](N.Atoms(2000), na)
+    with pytest.raises(ValueError, match="Diabatic"):
+        N.run_dynamics(ena, (0.0, 10.0), nuc * N.FermiDiracState(na.fermi_level, 9.5e-4, basis=N.Diabatic()),
                        output=N.OutputDiabaticPopulation)
     with pytest.raises(ValueError, match="only 31 states"):
         N.run_dynamics(ie, (0.0, 10.0), nuc * N.FermiDiracState(na.fermi_level, 9.5e-4, available_states=range(1, 33)),

@@ -319,3 +319,50 @@ def test_determinant_lower_bound_of_the_occupied_block():
             if 1 - t2 > 0:
                 est = 2 * dt / abs(d) ** 2 * sumvd * (abs(d.real) + abs(d.imag))
                 assert est <= 2 * dt * sumvd * np.sqrt(2) / np.sqrt(1 - t2) * (1 + 1e-12)
+
+
+def _na_gauge_fixed(oracle, md, x):
+    """Diagonal of the model matrix and the eigenpairs with the impurity component of every
+    eigenvector positive (the device's gauge)."""
+    E, U, _ = oracle.na_eigen(md, x)
+    U = U * np.sign(U[0])
+    return np.diag(oracle.na_matrix(md, x)).copy(), E, U
+
+
+def test_fermi_dirac_diabatic_orbitals(oracle):
+    """FermiDiracState{Diabatic} (iesh.jl:138-183): every orbital is a row of the eigenvector matrix
+    (the diabatic state in the adiabatic basis) with the sign that makes its first adiabatic
+    component positive; the orbitals are orthonormal and the adiabatic occupations distinct and
+    sorted."""
+    md = oracle.na_model(M=20)
+    Ed, E, U = _na_gauge_fixed(oracle, md, 3.0)
+    assert np.isclose(Ed[0], 0.5 * 2000 * 2e-4 ** 2 * ((3.0 - 20.6097) ** 2 - 9.0) - 3.8e-3)
+    for t in range(20):
+        occ, psi = oracle.fermi_dirac_diabatic(Ed, U, 10, 1.0 / (3.166811563e-6 * 300), t, 5)
+        assert len(set(occ)) == 10 and np.all(np.diff(occ) > 0)
+        assert np.allclose(psi.T @ psi, np.eye(10), atol=1e-12)
+        for i in range(10):
+            d = int(np.argmax(np.abs(U @ psi[:, i])))  # U U^T e_d = e_d
+            assert np.allclose(psi[:, i], U[d] * np.sign(U[d, 0]), atol=1e-14)
+            assert psi[0, i] > 0
+
+
+def test_fermi_dirac_diabatic_adiabatic_state_distribution(oracle):
+    """The adiabatic states drawn for fixed diabatic states (0 K and one higher available level, so the
+    chain keeps its start): one electron on bath level 5 lands on adiabatic state s with probability
+    U[5, s]^2; two electrons on levels 3 and 9 give the sorted pair {a, b} with probability
+    w3_a w9_b / (1 - w9_a) + w3_b w9_a / (1 - w9_b) (the second draw repeats until it misses the first)."""
+    from collections import Counter
+    md = oracle.na_model(M=20)
+    Ed, E, U = _na_gauge_fixed(oracle, md, 3.0)
+    n = 20000
+    w = U[6] ** 2  # bath level 5 is state 6
+    cnt = Counter(int(oracle.fermi_dirac_diabatic(Ed, U, 1, np.inf, t, 9, [6, 16])[0][0]) for t in range(n))
+    for s in range(21):
+        assert abs(cnt[s] / n - w[s]) <= 5 * np.sqrt(w[s] * (1 - w[s]) / n) + 1e-3
+    w1, w2 = U[4] ** 2, U[10] ** 2
+    cnt = Counter(tuple(oracle.fermi_dirac_diabatic(Ed, U, 2, np.inf, t, 9, [4, 10, 18])[0]) for t in range(n))
+    for a in range(21):
+        for b in range(a + 1, 21):
+            p = w1[a] * w2[b] / (1 - w2[a]) + w1[b] * w2[a] / (1 - w2[b])
+            assert abs(cnt[(a, b)] / n - p) <= 5 * np.sqrt(p * (1 - p) / n) + 1e-3, (a, b)
