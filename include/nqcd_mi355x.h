@@ -129,8 +129,9 @@ typedef struct {
     int32_t reserved[7];
 } nqcd_problem_desc;
 
-/* TerminatingCallback with the predicate |x_dof| > value (DynamicsUtils/callbacks.jl:15-29). */
-enum nqcd_term_kind { NQCD_TERM_NONE = 0, NQCD_TERM_ABS_GT = 1 };
+/* TerminatingCallback (DynamicsUtils/callbacks.jl:15-29) with a predicate the device evaluates after
+ * every step (and at t0): |x_dof| > value, x_dof > value or x_dof < value.  FSSH / Ehrenfest only. */
+enum nqcd_term_kind { NQCD_TERM_NONE = 0, NQCD_TERM_ABS_GT = 1, NQCD_TERM_GT = 2, NQCD_TERM_LT = 3 };
 typedef struct {
     int32_t kind;
     int32_t dof;                 /* flattened (ndofs, natoms) index */

@@ -434,7 +434,8 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
     if (!(t1 >= t0)) return fail(h, NQCD_ERR_ARG, "t1 must be >= t0");
     if (save_every < 0) return fail(h, NQCD_ERR_ARG, "save_every must be >= 0");
     if (term && term->kind != NQCD_TERM_NONE) {
-        if (term->kind != NQCD_TERM_ABS_GT) return fail(h, NQCD_ERR_UNSUPPORTED, "unknown termination predicate");
+        if (term->kind < NQCD_TERM_ABS_GT || term->kind > NQCD_TERM_LT)
+            return fail(h, NQCD_ERR_UNSUPPORTED, "unknown termination predicate");
         if (term->dof < 0 || term->dof >= h->D) return fail(h, NQCD_ERR_ARG, "termination dof out of range");
     }
     const double nsteps_d = std::round((t1 - t0) / h->prob.dt);

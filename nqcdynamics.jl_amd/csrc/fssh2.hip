@@ -144,11 +144,11 @@ __device__ __forceinline__ void finish_traj(const Fssh2Args* Ap, const Lane2<D, 
 template <int D, typename R>
 __device__ __forceinline__ bool is_done(const Fssh2Args& A, const Lane2<D, R>& L) {
     if (L.step >= A.nsteps) return true;
-    if (A.term_kind == 1) {
+    if (A.term_kind != 0) {
         double x = 0.0;  // select, not a runtime index, so the lane state stays in registers
 #pragma unroll
         for (int I = 0; I < D; ++I) x = (I == A.term_dof) ? (double)L.r[I] : x;
-        if (fabs(x) > A.term_value) return true;
+        if (term_hit(A.term_kind, x, A.term_value)) return true;
     }
     return false;
 }
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_init(const Fssh2Args A, doubl
     for (int J = 0; J < 4; ++J) S[(PL_U + J) * n + tid] = U[J];
     S[PL_T * n + tid] = A.t0;
     Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = 0; Si[PI_STEP * n + tid] = 0;
-    const bool done = (A.nsteps <= 0) || (A.term_kind == 1 && fabs(r) > A.term_value);
+    const bool done = (A.nsteps <= 0) || term_hit(A.term_kind, r, A.term_value);
     Si[PI_DONE * n + tid] = done ? 1 : 0;
     plain_save(A, tid, 0, st, s, U);
 }
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, doubl
     S[PL_T * n + tid] = tn;
     Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = nh; Si[PI_STEP * n + tid] = step;
     if (A.save_every > 0 && step % A.save_every == 0) plain_save(A, tid, step / A.save_every, st, s, U);
-    const bool done = (step >= A.nsteps) || (A.term_kind == 1 && fabs(r) > A.term_value);
+    const bool done = (step >= A.nsteps) || term_hit(A.term_kind, r, A.term_value);
     if (done) Si[PI_DONE * n + tid] = 1;
 }
 

@@ -255,11 +255,11 @@ __device__ __forceinline__ void nd_finish(const FsshNdArgs* Ap, const LaneN<R>& 
 template <typename R>
 __device__ __forceinline__ bool nd_done(const FsshNdArgs& A, const LaneN<R>& L) {
     if (L.step >= A.nsteps) return true;
-    if (A.term_kind == 1) {
+    if (A.term_kind != 0) {
         double x = 0.0;
 #pragma unroll
         for (int I = 0; I < kNdDof; ++I) x = (I == A.term_dof) ? (double)L.r[I] : x;
-        if (fabs(x) > A.term_value) return true;
+        if (term_hit(A.term_kind, x, A.term_value)) return true;
     }
     return false;
 }

@@ -75,6 +75,12 @@ __device__ __forceinline__ void philox_normal2(uint64_t traj, uint32_t step, uin
 // Transcendentals by precision: FP64 path uses ocml's correctly-rounded-ish
 // double routines; the FP32 path uses the hardware exp.
 // ---------------------------------------------------------------------------
+// TerminatingCallback predicates (DynamicsUtils/callbacks.jl:29) of nqcd_term_kind: 1 |x| > value,
+// 2 x > value, 3 x < value, 0 none.
+__device__ __forceinline__ bool term_hit(int kind, double x, double value) {
+    return (kind == 1) ? fabs(x) > value : (kind == 2) ? x > value : (kind == 3) ? x < value : false;
+}
+
 __device__ __forceinline__ double fexp(double x) { return exp(x); }
 __device__ __forceinline__ float fexp(float x) { return __expf(x); }
 __device__ __forceinline__ double fabs_(double x) { return fabs(x); }

@@ -29,7 +29,7 @@ using NQCDynamics.DynamicsMethods: SurfaceHoppingMethods, EhrenfestMethods, Clas
 using NQCModels
 using NQCDistributions: NQCDistributions
 
-export EnsembleMI355X
+export EnsembleMI355X, DeviceTermination
 
 const libnqcd = get(ENV, "NQCD_MI355X_LIB", joinpath(@__DIR__, "..", "libnqcd_mi355x.so"))
 
@@ -53,6 +53,20 @@ struct EnsembleMI355X <: SciMLBase.EnsembleAlgorithm
 end
 EnsembleMI355X(; device=0, persistent=true, fp32_nuclei=false, seed=0x4E5143444D493335, device_model=nothing) =
     EnsembleMI355X(device, persistent, fp32_nuclei, seed, device_model)
+
+"""
+    DeviceTermination(predicate, dof, value)
+
+A `TerminatingCallback` whose predicate the device evaluates after every step
+(DynamicsUtils/callbacks.jl:29): `predicate` is `:abs_gt` (|x_dof| > value), `:gt` (x_dof > value) or
+`:lt` (x_dof < value), `dof` 1-based in (ndofs, natoms) order.  Pass it as `callback=` to
+`run_dynamics` in place of `TerminatingCallback(func)`, whose Julia closure the device cannot run.
+"""
+struct DeviceTermination
+    predicate::Symbol
+    dof::Int
+    value::Float64
+end
 
 # --- C structs (layout of include/nqcd_mi355x.h) -------------------------------------------
 struct ProblemDesc
@@ -228,8 +242,12 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
             h[], 0, kind, collect(Float64, params), length(params)))
         load_initial!(h[], prob, sim, trajectories, ensemblealg.seed)
         save_every = saveat === nothing ? Int32(0) : round(Int32, step(saveat) / dt)
-        # TerminatingCallback with the |x_dof| > value predicate
-        term = callback === nothing ? C_NULL : Ref(TermDesc(1, callback.dof - 1, callback.value))
+        # TerminatingCallback with a predicate the device evaluates (DeviceTermination)
+        callback === nothing || callback isa DeviceTermination ||
+            throw(ArgumentError("EnsembleMI355X evaluates DeviceTermination callbacks only"))
+        kinds = Dict(:abs_gt => Int32(1), :gt => Int32(2), :lt => Int32(3))
+        term = callback === nothing ? C_NULL :
+            Ref(TermDesc(kinds[callback.predicate], Int32(callback.dof - 1), callback.value))
         # long device call: do not hold the Julia GC
         rc = @threadcall((:nqcd_mi355x_run, libnqcd), Int32, (Ptr{Cvoid}, Float64, Float64, Int32, Ptr{TermDesc}, UInt64),
             h[], tspan[1], tspan[2], save_every, term, ensemblealg.seed)

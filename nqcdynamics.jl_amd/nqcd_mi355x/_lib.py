@@ -26,7 +26,7 @@ SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
  MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE) = range(9)
 MLP_PARAMS = 128 * 6 + 128 + 128 * 128 + 128 + 3 * 128 + 3
 DECOHERENCE_NONE, DECOHERENCE_EDC = range(2)
-TERM_NONE, TERM_ABS_GT = range(2)
+TERM_NONE, TERM_ABS_GT, TERM_GT, TERM_LT = range(4)
 SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLYMER_THERMAL = range(4)
 (OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION, OUT_FINAL_ADIABATIC_POPULATION,
  OUT_CENTROID_POSITION) = range(6)
@@ -239,7 +239,7 @@ class Ensemble:
         td = None
         if term is not None:
             td = TermDesc()
-            td.kind, td.dof, td.value = TERM_ABS_GT, term[0], term[1]
+            td.kind, td.dof, td.value = (term[2] if len(term) > 2 else TERM_ABS_GT), term[0], term[1]
         self._check(self._L.nqcd_mi355x_run(self._h, t0, t1, save_every, C.byref(td) if td is not None else None,
                                             C.c_uint64(key)))
         nsteps = int(round((t1 - t0) / self.dt))

@@ -31,11 +31,16 @@ class EnsembleMI355X:
 
 
 class TerminatingCallback:
-    """TerminatingCallback(|x_dof| > value) (DynamicsUtils/callbacks.jl:15-29); the device
-    evaluates this predicate after every step.  `dof` is 1-based in (ndofs, natoms) order."""
+    """TerminatingCallback (DynamicsUtils/callbacks.jl:15-29) with a predicate on one coordinate that
+    the device evaluates after every step: "abs_gt" |x_dof| > value (default), "gt" x_dof > value,
+    "lt" x_dof < value.  `dof` is 1-based in (ndofs, natoms) order."""
 
-    def __init__(self, dof=1, value=10.0):
-        self.dof, self.value = int(dof), float(value)
+    PREDICATES = {"abs_gt": L.TERM_ABS_GT, "gt": L.TERM_GT, "lt": L.TERM_LT}
+
+    def __init__(self, dof=1, value=10.0, predicate="abs_gt"):
+        if predicate not in self.PREDICATES:
+            raise ValueError(f"predicate must be one of {sorted(self.PREDICATES)}")
+        self.dof, self.value, self.predicate = int(dof), float(value), predicate
 
 
 def _kind(sim):
@@ -242,7 +247,7 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
     if callback is not None:
         if not isinstance(callback, TerminatingCallback):
             raise ValueError("EnsembleMI355X evaluates TerminatingCallback(|x_dof| > value) predicates only")
-        term = (callback.dof - 1, callback.value)
+        term = (callback.dof - 1, callback.value, TerminatingCallback.PREDICATES[callback.predicate])
     kind = _kind(sim)
     if kind != "sh" and callback is not None:
         raise ValueError("termination callbacks are supported for FSSH / Ehrenfest simulations")

@@ -530,7 +530,16 @@ struct ORunDesc {
     int64_t traj_offset;  // global id of trajectory 0 of this call
     int32_t nsub;         // electronic sub-steps per nuclear step (5)
     int32_t term_dof;     // flattened dof index of the termination predicate
+    int32_t term_kind;    // 0: |x| > term_x when term_x > 0; 2: x > term_x; 3: x < term_x
+    int32_t pad;
 };
+
+// TerminatingCallback(func) (DynamicsUtils/callbacks.jl:29) for the predicates on one coordinate.
+static bool term_hit(const ORunDesc& R, double x) {
+    if (R.term_kind == 2) return x > R.term_x;
+    if (R.term_kind == 3) return x < R.term_x;
+    return R.term_x > 0 && std::fabs(x) > R.term_x;
+}
 
 struct Cache {
     int n, D;
@@ -632,7 +641,7 @@ static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, 
     int64_t step = 0;
     std::vector<double> prob(n);
     while (step < R.nsteps) {
-        if (R.term_x > 0 && std::fabs(r[R.term_dof]) > R.term_x) break;
+        if (term_hit(R, r[R.term_dof])) break;
         std::vector<cd> sigprev = sig;
         for (int I = 0; I < D; ++I) vtmp[I] = v[I] + (R.dt / 2) * k[I];   // step_B!
         for (int I = 0; I < D; ++I) r[I] = r[I] + R.dt * vtmp[I];         // step_A!

@@ -27,7 +27,7 @@ class ORunDesc(C.Structure):
     _fields_ = [("method", C.c_int32), ("nstates", C.c_int32), ("ndof", C.c_int32), ("rescaling", C.c_int32),
                 ("dt", C.c_double), ("t0", C.c_double), ("nsteps", C.c_int64), ("term_x", C.c_double),
                 ("save_every", C.c_int32), ("nsave", C.c_int32), ("key", C.c_uint64), ("traj_offset", C.c_int64),
-                ("nsub", C.c_int32), ("term_dof", C.c_int32)]
+                ("nsub", C.c_int32), ("term_dof", C.c_int32), ("term_kind", C.c_int32), ("pad", C.c_int32)]
 
 
 def build():
@@ -180,7 +180,7 @@ def diabatic_to_adiabatic(m, r0, diabatic_state):
 
 def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None, dt=1.0, nsteps=100, term_x=0.0,
                  save_every=0, nsave=1, key=0, traj_offset=0, rescaling=RESCALE_STANDARD, nthreads=None, t0=0.0, nsub=5,
-                 traj_ids=None, term_dof=0):
+                 traj_ids=None, term_dof=0, term_kind=0):
     """Run an ensemble on the CPU; returns a dict of per-trajectory results."""
     if not isinstance(models, (list, tuple)):
         models = [models]
@@ -191,6 +191,7 @@ def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None,
     R.dt, R.t0, R.nsteps, R.term_x = dt, t0, nsteps, term_x
     R.save_every, R.nsave, R.key, R.traj_offset, R.nsub = save_every, nsave, key, traj_offset, nsub
     R.term_dof = term_dof
+    R.term_kind = term_kind
     marr = (OModelDesc * len(models))(*models)
     mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, dtype=np.float64), (D,)))
     sr0 = np.ascontiguousarray(np.real(sig0), dtype=np.float64).reshape(ntraj, n, n)

@@ -216,3 +216,26 @@ def test_abi_errors_are_reported():
         E.set_problem(L.METHOD_IESH, 60, 1, 1, [MASS], 1.0, algorithm=L.ALG_VERLET_ELECTRONICS)
     assert ei.value.code == L.ERR_ARG and "n_electrons" in str(ei.value)
     E.close()
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("predicate,value", [(2, 2.0), (3, -7.0), (1, 6.0)])
+def test_termination_predicates_parity(oracle, key, schedule, predicate, value):
+    """TerminatingCallback predicates evaluated on the device (x > 2, x < -7, |x| > 6) in the persistent
+    and the step-per-launch kernels against the oracle: identical step counts, states and hops, final
+    positions within 1e-9 (trajectories moving both ways, so both one-sided predicates fire)."""
+    L, E = _ens()
+    rng = np.random.default_rng(40 + predicate)
+    ntraj = 256
+    r0 = rng.normal(-5.0, 0.5, (ntraj, 1))
+    v0 = rng.choice([-1.0, 1.0], (ntraj, 1)) * rng.uniform(10.0, 30.0, (ntraj, 1)) / MASS
+    sig0 = np.zeros((ntraj, 2, 2), complex); sig0[:, 0, 0] = 1
+    st0 = np.zeros(ntraj, np.int32)
+    g = _run_gpu(L, E, 0, 0, r0, v0, sig0, st0, 3000, key, schedule=schedule, term=(0, value, predicate))
+    m = oracle.make_model(0, 2, 1, TULLY[0])
+    c = oracle.run_ensemble(0, m, MASS, r0, v0, sig0, state0=st0, dt=1.0, nsteps=3000, key=key, term_x=value,
+                            term_kind=0 if predicate == 1 else predicate)
+    assert np.array_equal(g["steps"], c["steps"])
+    assert (g["steps"] < 3000).sum() > 20  # the predicate ended a good part of the ensemble
+    _assert_window_parity(g, c)
+    E.close()

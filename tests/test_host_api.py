@@ -110,3 +110,13 @@ def test_fermi_dirac_state_checks():
     sh = N.Simulation[N.FSSH](N.Atoms(2000), N.TullyModelOne())
     with pytest.raises(ValueError, match="IESH"):
         N.run_dynamics(sh, (0.0, 10.0), nuc * N.FermiDiracState(0.0, 9.5e-4), output=N.OutputDiabaticPopulation)
+
+
+def test_terminating_callback_predicates():
+    """The device-evaluated TerminatingCallback predicates map onto the C ABI's kinds."""
+    from nqcd_mi355x import _lib as L
+    assert N.TerminatingCallback(1, 10.0).predicate == "abs_gt"
+    assert N.TerminatingCallback.PREDICATES == {"abs_gt": L.TERM_ABS_GT, "gt": L.TERM_GT, "lt": L.TERM_LT}
+    assert (L.TERM_ABS_GT, L.TERM_GT, L.TERM_LT) == (1, 2, 3)
+    with pytest.raises(ValueError, match="predicate"):
+        N.TerminatingCallback(1, 10.0, predicate="ge")
