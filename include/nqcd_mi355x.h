@@ -67,7 +67,8 @@ enum nqcd_algorithm {
     NQCD_ALG_BAB_TSIT5 = 0,
     NQCD_ALG_VERLET_ELECTRONICS = 1,
     NQCD_ALG_BCB = 2,
-    NQCD_ALG_BCB_TSIT5 = 3
+    NQCD_ALG_BCB_TSIT5 = 3,
+    NQCD_ALG_BCB_WAVEFUNCTION = 4    /* ring-polymer IESH / EhrenfestNA (bcb_wavefunction.jl) */
 };
 
 /* rescaling keyword of Simulation{FSSH} (fssh.jl:41, surface_hopping.jl:219,234-245). */

@@ -54,7 +54,9 @@ struct IeshArgs {
     int32_t method;               // 0 = IESH, 1 = EhrenfestNA
     int32_t rescaling, estimate, edc;
     int32_t save_every, nsave;
+    int32_t P, pad1;              // ring-polymer beads (1: no ring polymer)
     double dt, mass, edc_C;
+    double wn;                    // ring-polymer frequency P kT
     uint64_t key;
     double mw2, g, dG, V, a, de;  // model
     const double* r0; const double* v0; const int32_t* occ0; const double* psi0;  // psi0 may be null
@@ -321,7 +323,8 @@ __device__ bool wave_det(double* Sre, double* Sim, int LD, int Ne, double& dre, 
     return true;
 }
 
-template <int NT, int CTM>
+// RPB: ring polymer (BCBWavefunction), a separate instantiation so the one-bead kernels carry none of it
+template <int NT, int CTM, bool RPB>
 __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) void iesh_kernel(const IeshArgs A) {
     constexpr int NP = 16 * NT;  // padded state count (rows of psi / W tiles)
     constexpr int B = 64 * NT;
@@ -342,6 +345,17 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
     int* isocc = occ + kIeMaxNe;                  // NP electron index occupying the state, -1 if empty
     int* piv = isocc + NP;                        // kIeMaxNe pivot rows
     int* unocc = piv + kIeMaxNe;                  // NP unoccupied states, ascending
+    // ring polymers (P > 1): bead positions, velocities, accelerations, normal-mode pair, and the
+    // eigenpairs of one bead (these arrays are allocated only when P > 1)
+    const int P = RPB ? A.P : 1;
+    constexpr bool rp = RPB;
+    double* brl = reinterpret_cast<double*>(unocc + NP);
+    double* bvl = brl + 64;
+    double* bal = bvl + 64;
+    double* bq = bal + 64;
+    double* bp = bq + 64;
+    double* Eb = bp + 64;
+    double* Ub = Eb + NP;
 
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const double dt = A.dt;
@@ -386,7 +400,24 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
             det_bound = block_max<NT>(dev, red) <= 1e-10;
             __syncthreads();
         }
-        double x = A.r0[traj], v = A.v0[traj];
+        auto bead_mean = [&](const double* a) {
+            double m = 0.0;
+            for (int b = 0; b < P; ++b) m += a[b];
+            return m / P;
+        };
+        double x, v;
+        if (rp) {
+            if (tid < P) {
+                brl[tid] = A.r0[(size_t)traj * P + tid];
+                bvl[tid] = A.v0[(size_t)traj * P + tid];
+            }
+            __syncthreads();
+            x = bead_mean(brl);
+            v = bead_mean(bvl);
+        } else {
+            x = A.r0[traj];
+            v = A.v0[traj];
+        }
         double Eprev = -1.0;
         bool warm = false;
         int nh = 0;
@@ -412,20 +443,36 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
             __syncthreads();
             return s;
         };
-        // acceleration at the current eigenpairs with the wavefunction in Tb (iesh.jl:190-207, ehrenfest_na.jl:72-86)
-        auto accel = [&](const Scalars& s) -> double {
+        // eigenpairs at a bead position into Eb, Ub (cold start: no roots are kept per bead)
+        auto bead_eigen = [&](double xx) -> Scalars {
+            const Scalars s = na_scalars(A, xx);
+            const int ir = (l >> 2) * NT + w, sub = l & 3;
+            if (ir < N) {
+                double fp;
+                const double lam = na_root<4>(eps, M, A.a, V2, zn, s.h, ir, false, 0.0, fp, sub);
+                if (sub == 0) {
+                    Eb[ir] = lam;
+                    Ub[ir] = 1.0 / sqrt(fp);
+                }
+            }
+            __syncthreads();
+            return s;
+        };
+        // acceleration at the eigenpairs whose impurity components are Uv, with the wavefunction in Tb
+        // (iesh.jl:190-207, ehrenfest_na.jl:72-86; per bead rpiesh.jl:20-38, rpehrenfest_na.jl:14-40)
+        auto accel = [&](const Scalars& s, const double* Uv) -> double {
             if (A.method == 0) {
                 if (tid == 0) {
                     double f = s.dU0;
-                    for (int e = 0; e < Ne; ++e) { const double u = U[occ[e]]; f += u * s.dh * u; }
+                    for (int e = 0; e < Ne; ++e) { const double u = Uv[occ[e]]; f += u * s.dh * u; }
                     scal[0] = f;
                 }
             } else {
                 if (tid < Ne) {
                     double re = 0.0, im = 0.0;
                     for (int a = 0; a < N; ++a) {
-                        re += U[a] * Tb[a * CW + 2 * tid];
-                        im += U[a] * Tb[a * CW + 2 * tid + 1];
+                        re += Uv[a] * Tb[a * CW + 2 * tid];
+                        im += Uv[a] * Tb[a * CW + 2 * tid + 1];
                     }
                     pcol[tid] = s.dh * (re * re + im * im);
                 }
@@ -511,17 +558,69 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
             __syncthreads();
         };
 
+        // bead accelerations into bal (one bead at a time over the whole workgroup)
+        auto bead_accels = [&]() {
+            for (int b = 0; b < P; ++b) {
+                const Scalars s = bead_eigen(brl[b]);
+                const double a = accel(s, Ub);
+                if (tid == 0) bal[b] = a;
+            }
+            __syncthreads();
+        };
+        // y = H x for the two bead vectors (H: the discrete Hartley transform, orthogonal and its own
+        // inverse; modes l and P - l share the frequency, so the Cayley step per Hartley mode is the
+        // free ring polymer's), one bead per lane of wavefront 0
+        auto hartley2 = [&](const double* x1, const double* x2, double* y1, double* y2) {
+            if (tid < P) {
+                double a1 = 0.0, a2 = 0.0;
+                for (int k = 0; k < P; ++k) {
+                    double sn, cs;
+                    sincospi(2.0 * (double)((tid * k) % P) / (double)P, &sn, &cs);
+                    a1 += (cs + sn) * x1[k];
+                    a2 += (cs + sn) * x2[k];
+                }
+                const double sP = 1.0 / sqrt((double)P);
+                y1[tid] = a1 * sP;
+                y2[tid] = a2 * sP;
+            }
+            __syncthreads();
+        };
+
         Scalars sc = eigen(x);
-        double kacc = accel(sc);
+        double kacc = 0.0;
+        if (rp) bead_accels();
+        else kacc = accel(sc, U);
         save(0);
 
         for (int64_t step = 0; step < A.nsteps; ++step) {
-            // ---- nuclei: B A (update) B (verlet_with_electronics.jl:58-66) --------------------------
-            const double vt = v + (dt / 2) * kacc;
-            x = x + dt * vt;
-            sc = eigen(x);
-            kacc = accel(sc);  // EhrenfestNA uses psi of the step start
-            v = vt + (dt / 2) * kacc;
+            if (!rp) {
+                // ---- nuclei: B A (update) B (verlet_with_electronics.jl:58-66) ----------------------
+                const double vt = v + (dt / 2) * kacc;
+                x = x + dt * vt;
+                sc = eigen(x);
+                kacc = accel(sc, U);  // EhrenfestNA uses psi of the step start
+                v = vt + (dt / 2) * kacc;
+            } else {
+                // ---- ring polymer: B, C in normal modes, B (bcb_wavefunction.jl:46-66); the bead
+                // forces see psi of the step start, and the wavefunction step below the centroid
+                // eigenpairs and velocity of the step start (E, U, sc, v are not updated here) ---
+                if (tid < P) bvl[tid] += (dt / 2) * bal[tid];
+                __syncthreads();
+                hartley2(brl, bvl, bq, bp);
+                if (tid < P) {  // Cayley map of mode tid, half = false (ring_polymer.jl:71-82)
+                    const double wk = 2.0 * A.wn * sinpi((double)tid / (double)P);
+                    const double uu = dt * dt * wk * wk / 4.0, sc1 = 1.0 / (1.0 + uu);
+                    const double c11 = (1.0 - uu) * sc1, c12 = dt * sc1, c21 = -dt * wk * wk * sc1;
+                    const double qn = c11 * bq[tid] + c12 * bp[tid], pn = c21 * bq[tid] + c11 * bp[tid];
+                    bq[tid] = qn;
+                    bp[tid] = pn;
+                }
+                __syncthreads();
+                hartley2(bq, bp, brl, bvl);
+                bead_accels();
+                if (tid < P) bvl[tid] += (dt / 2) * bal[tid];
+                __syncthreads();
+            }
 
             // ---- wavefunction: psi <- exp(-i dt H) psi, H = diag(E) - i v d ---------------------------
             {
@@ -596,6 +695,12 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
                     }
                     if (K == 0) break;
                 }
+            }
+
+            if (rp) {  // centroid at the step end: eigenpairs and velocity for the hop and the next step
+                x = bead_mean(brl);
+                v = bead_mean(bvl);
+                sc = eigen(x);
             }
 
             // ---- IESH hop (iesh.jl:390-407) -------------------------------------------------------------
@@ -797,6 +902,7 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
                             }
                         }
                         scal[5] = v;
+                        scal[6] = 0.0;
                         if (hn >= 0) {  // rescale along d[new, old] (surface_hopping.jl:65-92)
                             const int old = occ[hn];
                             bool ok = true;
@@ -807,11 +913,16 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
                                 const double disc = bq * bq - 4.0 * aq * cq;
                                 if (disc < 0.0) {
                                     ok = false;
-                                    if (A.rescaling == NQCD_RESCALE_VINVERSION) vn = v - 2.0 * (v * (dd / fabs(dd))) * (dd / fabs(dd));
+                                    if (A.rescaling == NQCD_RESCALE_VINVERSION) {
+                                        const double inc = 2.0 * (v * (dd / fabs(dd))) * (dd / fabs(dd));
+                                        vn = v - inc;
+                                        scal[6] = -inc;
+                                    }
                                 } else {
                                     const double root = sqrt(disc);
                                     const double gam = (bq < 0.0) ? (bq + root) / (2.0 * aq) : (bq - root) / (2.0 * aq);
                                     vn = v - gam * dd / A.mass;
+                                    scal[6] = -(gam * dd / A.mass);
                                 }
                             }
                             if (ok) {
@@ -825,6 +936,13 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
                     }
                     __syncthreads();
                     v = scal[5];
+                    if (rp) {  // every bead takes the centroid's change (rpsh.jl:24-44)
+                        const double inc = scal[6];
+                        __syncthreads();
+                        if (tid < P) bvl[tid] = bvl[tid] - (-inc);
+                        __syncthreads();
+                        v = bead_mean(bvl);
+                    }
                     __syncthreads();
                 }
                 // ---- Granucci-Persico EDC per electron (decoherence_corrections.jl:21-39, iesh.jl:424-433)
@@ -854,9 +972,16 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
         }
 
         // ---- store --------------------------------------------------------------------------------
-        if (tid == 0) {
+        if (rp) {
+            if (tid < P) {
+                A.r[(size_t)traj * P + tid] = brl[tid];
+                A.v[(size_t)traj * P + tid] = bvl[tid];
+            }
+        } else if (tid == 0) {
             A.r[traj] = x;
             A.v[traj] = v;
+        }
+        if (tid == 0) {
             A.nhops[traj] = nh;
             A.steps[traj] = (int32_t)A.nsteps;
         }
@@ -901,7 +1026,13 @@ __global__ __launch_bounds__(256) void ie_initial_energies(const IeshArgs A, int
     __syncthreads();
     const int64_t traj = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (traj >= A.ntraj) return;
-    const Scalars s = na_scalars(A, A.r0[traj]);
+    double x0 = A.r0[traj];
+    if (A.P > 1) {  // the centroid (the hopping eigenvalues of a ring polymer, SurfaceHoppingMethods.jl:85-87)
+        x0 = 0.0;
+        for (int b = 0; b < A.P; ++b) x0 += A.r0[(size_t)traj * A.P + b];
+        x0 /= A.P;
+    }
+    const Scalars s = na_scalars(A, x0);
     const double zn = sqrt((double)A.M) * A.V;
     double* row = W + traj * (diabatic ? 3 * A.N : A.N);
     for (int ir = threadIdx.x & 63; ir < A.N; ir += 64) {
@@ -1039,17 +1170,19 @@ __global__ __launch_bounds__(256) void ie_diabatic_orbitals(int64_t ntraj, int N
 // S as two planes with an odd row stride (determinant), or interleaved Ne x Ne (inverse)
 int ie_sb_len(int N, int Ne) { (void)N; return 2 * Ne * (Ne | 1); }
 
-size_t ie_lds_bytes(int NT, int cw, int N, int Ne) {
+size_t ie_lds_bytes(int NT, int cw, int N, int Ne, int P) {
     const int NP = 16 * NT;
     return sizeof(double) * ((size_t)NP * cw + ie_sb_len(N, Ne) + 3 * NP + 8 + std::max(NP, 2 * kIeMaxNe) + 8 * kIeMaxNe + 8) +
-           sizeof(int) * (2 * kIeMaxNe + 2 * NP);
+           sizeof(int) * (2 * kIeMaxNe + 2 * NP) + (P > 1 ? sizeof(double) * (5 * 64 + 2 * (size_t)NP) : 0);
 }
 
 typedef void (*IeKernel)(const IeshArgs);
 
-IeKernel ie_pick(int NT, bool wide) {
+IeKernel ie_pick(int NT, bool wide, bool rp) {
 #define NQCD_IE(nt)                                                                                    \
-    case nt: return wide ? (IeKernel)iesh_kernel<nt, 8> : (IeKernel)iesh_kernel<nt, nt>;
+    case nt:                                                                                           \
+        if (rp) return wide ? (IeKernel)iesh_kernel<nt, 8, true> : (IeKernel)iesh_kernel<nt, nt, true>; \
+        return wide ? (IeKernel)iesh_kernel<nt, 8, false> : (IeKernel)iesh_kernel<nt, nt, false>;
     switch (NT) {
         NQCD_IE(1) NQCD_IE(2) NQCD_IE(3) NQCD_IE(4) NQCD_IE(5) NQCD_IE(6) NQCD_IE(7) NQCD_IE(8)
     default: return nullptr;
@@ -1077,10 +1210,17 @@ IeModel ie_model(const nqcd_handle_t* h) {
 
 int32_t ie_check_problem(nqcd_handle_t* h) {
     const nqcd_problem_desc& d = h->prob;
-    if (d.algorithm != NQCD_ALG_VERLET_ELECTRONICS)
+    if (d.nbeads <= 1 && d.algorithm != NQCD_ALG_VERLET_ELECTRONICS)
         return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA integrate with VerletwithElectronics");
-    if (d.ndofs * d.natoms != 1 || d.nbeads != 1)
-        return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA: one degree of freedom, no ring polymer");
+    if (d.nbeads > 1 && d.algorithm != NQCD_ALG_BCB_WAVEFUNCTION)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "ring-polymer IESH / EhrenfestNA integrate with BCBWavefunction");
+    if (d.ndofs * d.natoms != 1)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA: one degree of freedom");
+    if (d.nbeads > 64) return fail(h, NQCD_ERR_UNSUPPORTED, "ring-polymer IESH / EhrenfestNA: at most 64 beads");
+    if (d.nbeads > 1 && !(d.temperature > 0.0))
+        return fail(h, NQCD_ERR_ARG, "ring polymers need a positive temperature");
+    if (d.nbeads > 1 && d.decoherence != NQCD_DECOHERENCE_NONE)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "the EDC decoherence correction is not defined for ring polymers");
     if (d.n_states < 3 || d.n_states > kIeMaxStates)
         return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA: n_states must be in 3..128");
     if (d.n_electrons < 1 || d.n_electrons >= d.n_states || d.n_electrons > kIeMaxNe)
@@ -1131,11 +1271,12 @@ int32_t ie_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const 
     if ((psi_re == nullptr) != (psi_im == nullptr)) return fail(h, NQCD_ERR_ARG, "give both psi parts or neither");
     if (int32_t rc = ie_check_occ(h, ntraj, occ)) return rc;
     h->ntraj = ntraj;
-    HIPCHK(h, h->r0.ensure(sizeof(double) * ntraj));
-    HIPCHK(h, h->v0.ensure(sizeof(double) * ntraj));
+    const int P = h->prob.nbeads > 1 ? h->prob.nbeads : 1;  // (nbeads, ntraj) column-major for ring polymers
+    HIPCHK(h, h->r0.ensure(sizeof(double) * P * ntraj));
+    HIPCHK(h, h->v0.ensure(sizeof(double) * P * ntraj));
     HIPCHK(h, h->state0.ensure(sizeof(int32_t) * Ne * ntraj));
-    HIPCHK(h, hipMemcpy(h->r0.p, r, sizeof(double) * ntraj, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->v0.p, v, sizeof(double) * ntraj, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->r0.p, r, sizeof(double) * P * ntraj, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->v0.p, v, sizeof(double) * P * ntraj, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->state0.p, occ, sizeof(int32_t) * Ne * ntraj, hipMemcpyHostToDevice));
     h->ie_psi_given = psi_re != nullptr;
     if (psi_re) {  // (N, Ne, ntraj) column-major -> per trajectory rows of (Re, Im) pairs
@@ -1155,6 +1296,8 @@ int32_t ie_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const 
 }
 
 int32_t ie_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S, uint64_t seed) {
+    if (h->prob.nbeads > 1)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "ring-polymer IESH / EhrenfestNA take their beads from nqcd_mi355x_load_initial");
     if (S->kind != NQCD_SAMPLER_GAUSSIAN || S->nparams != 4 || !S->params)
         return fail(h, NQCD_ERR_ARG, "IESH / EhrenfestNA sampler: GAUSSIAN with mu_r, sd_r, mu_v, sd_v");
     const int Ne = h->prob.n_electrons;
@@ -1210,6 +1353,7 @@ int32_t ie_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* avai
     A.N = N; A.M = md.M; A.Ne = Ne;
     A.mw2 = md.mw2; A.g = md.g; A.dG = md.dG; A.V = md.V; A.a = md.a; A.de = md.de;
     A.r0 = h->r0.as<double>();
+    A.P = h->prob.nbeads > 1 ? h->prob.nbeads : 1;
     const int stride = basis ? 3 * N : N;
     HIPCHK(h, h->tmp.ensure(sizeof(double) * (size_t)stride * n));
     hipLaunchKernelGGL(ie_initial_energies, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, A, basis,
@@ -1255,9 +1399,10 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     const int NT = (N + 15) / 16;
     const int CT = (2 * Ne + 15) / 16;
     const bool wide = CT > NT;
-    IeKernel kern = ie_pick(NT, wide);
+    IeKernel kern = ie_pick(NT, wide, h->prob.nbeads > 1);
     if (!kern) return fail(h, NQCD_ERR_UNSUPPORTED, "n_states");
-    const size_t lds = ie_lds_bytes(NT, 16 * (wide ? 8 : NT), N, Ne);
+    const int P = h->prob.nbeads > 1 ? h->prob.nbeads : 1;
+    const size_t lds = ie_lds_bytes(NT, 16 * (wide ? 8 : NT), N, Ne, P);
     if (lds > kIeLdsBytes)
         return fail(h, NQCD_ERR_UNSUPPORTED,
                     "IESH / EhrenfestNA with " + std::to_string(N) + " states and " + std::to_string(Ne) +
@@ -1271,8 +1416,8 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     int64_t grid = (int64_t)bpc * h->cu_count;
     if (grid > n) grid = n;
 
-    HIPCHK(h, h->r.ensure(sizeof(double) * n));
-    HIPCHK(h, h->v.ensure(sizeof(double) * n));
+    HIPCHK(h, h->r.ensure(sizeof(double) * P * n));
+    HIPCHK(h, h->v.ensure(sizeof(double) * P * n));
     HIPCHK(h, h->sig.ensure(sizeof(double) * 2 * N * Ne * n));
     HIPCHK(h, h->state.ensure(sizeof(int32_t) * Ne * n));
     HIPCHK(h, h->steps.ensure(sizeof(int32_t) * n));
@@ -1294,6 +1439,8 @@ int32_t ie_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     A.save_every = h->save_every;
     A.nsave = h->nsave;
     A.dt = h->prob.dt;
+    A.P = P;
+    A.wn = P * h->prob.temperature;  // omega_n = P kT (ring_polymer.jl:6-39)
     A.mass = h->masses[0];
     A.edc_C = h->prob.edc_C > 0.0 ? h->prob.edc_C : 0.1;
     A.key = key;
@@ -1318,8 +1465,9 @@ int32_t ie_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes) {
     const int N = h->prob.n_states, Ne = h->prob.n_electrons, ns = h->nsave;
     switch (field) {
     case NQCD_FIELD_R:
-    case NQCD_FIELD_V:
-        if (bytes != (int64_t)sizeof(double) * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+    case NQCD_FIELD_V:  // (nbeads, ntraj) for ring polymers
+        if (bytes != (int64_t)sizeof(double) * nt * (h->prob.nbeads > 1 ? h->prob.nbeads : 1))
+            return fail(h, NQCD_ERR_ARG, "bytes mismatch");
         HIPCHK(h, hipMemcpy(out, field == NQCD_FIELD_R ? h->r.p : h->v.p, bytes, hipMemcpyDeviceToHost));
         return NQCD_OK;
     case NQCD_FIELD_SIGMA_RE:

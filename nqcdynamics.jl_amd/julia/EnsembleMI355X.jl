@@ -8,6 +8,7 @@
 #   Simulation{AdiabaticIESH} / Simulation{EhrenfestNA}           VerletwithElectronics
 #   RingPolymerSimulation{Classical}                              BCB
 #   RingPolymerSimulation{FSSH} / RingPolymerSimulation{Ehrenfest} BCBwithTsit5
+#   RingPolymerSimulation{AdiabaticIESH} / {EhrenfestNA}          BCBWavefunction
 #
 # Not executed in this repository (no Julia toolchain in the build image); the same C ABI is
 # exercised by the Python mirror (nqcd_mi355x/) and its tests.  Load it next to NQCDynamics:
@@ -35,7 +36,7 @@ const libnqcd = get(ENV, "NQCD_MI355X_LIB", joinpath(@__DIR__, "..", "libnqcd_mi
 
 # --- codes of include/nqcd_mi355x.h ------------------------------------------------------------
 const METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST = Int32.(0:6)
-const ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5 = Int32.(0:3)
+const ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION = Int32.(0:4)
 const MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC,
       MODEL_NEWNS_ANDERSON, MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE = Int32.(0:8)
 const FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS,
@@ -95,6 +96,8 @@ function method_code(sim::NQCDynamics.RingPolymerSimulation)
     m isa ClassicalMethods.Classical && return METHOD_RPMD
     m isa SurfaceHoppingMethods.FSSH && return METHOD_RPSH
     m isa EhrenfestMethods.Ehrenfest && return METHOD_RP_EHRENFEST
+    m isa SurfaceHoppingMethods.AdiabaticIESH && return METHOD_IESH          # with BCBWavefunction
+    m isa EhrenfestMethods.EhrenfestNA && return METHOD_EHRENFEST_NA
     unsupported(m)
 end
 
@@ -102,6 +105,7 @@ algorithm_code(::IntegrationAlgorithms.BABwithTsit5) = ALG_BAB_TSIT5
 algorithm_code(::IntegrationAlgorithms.VerletwithElectronics) = ALG_VERLET_ELECTRONICS
 algorithm_code(::IntegrationAlgorithms.BCB) = ALG_BCB
 algorithm_code(::IntegrationAlgorithms.BCBwithTsit5) = ALG_BCB_TSIT5
+algorithm_code(::IntegrationAlgorithms.BCBWavefunction) = ALG_BCB_WAVEFUNCTION
 algorithm_code(a) = unsupported(a)
 
 # surface_hopping.jl:65,80-91

@@ -18,7 +18,7 @@ OK, ERR_ARG, ERR_UNSUPPORTED, ERR_DEVICE, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, 
 # enum nqcd_method
 METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST = range(7)
 # enum nqcd_algorithm
-ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5 = range(4)
+ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION = range(5)
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = range(3)
 PREC_FP64, PREC_FP32_NUCLEI = range(2)
 SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
@@ -204,11 +204,13 @@ class Ensemble:
         self.ntraj = ntraj
 
     def load_initial_iesh(self, r, v, occ, psi=None):
-        """IESH / EhrenfestNA: occ (ntraj, n_electrons) occupied adiabatic states (0-based), psi
-        (ntraj, n_states, n_electrons) complex or None for psi[occ_e, e] = 1."""
-        r = np.ascontiguousarray(np.ravel(r), dtype=np.float64)
-        v = np.ascontiguousarray(np.ravel(v), dtype=np.float64)
+        """IESH / EhrenfestNA: r, v (ntraj,) or, for ring polymers, (ntraj, nbeads); occ (ntraj,
+        n_electrons) occupied adiabatic states (0-based), psi (ntraj, n_states, n_electrons) complex or
+        None for psi[occ_e, e] = 1."""
+        r = np.asarray(r, dtype=np.float64)
         ntraj = r.shape[0]
+        r = np.ascontiguousarray(np.ravel(r))
+        v = np.ascontiguousarray(np.ravel(np.asarray(v, dtype=np.float64)))
         occ = np.ascontiguousarray(occ, dtype=np.int32).reshape(ntraj, -1)
         pr = pi = None
         if psi is not None:
@@ -326,7 +328,8 @@ class Ensemble:
 
     def _download_iesh(self, field):
         n, ne, nt, nsave = self.n_states, self.n_electrons, self.ntraj, self.nsave
-        shapes = {FIELD_R: ((nt,), np.float64), FIELD_V: ((nt,), np.float64),
+        rv = (nt,) if self.nbeads == 1 else (nt, self.nbeads)
+        shapes = {FIELD_R: (rv, np.float64), FIELD_V: (rv, np.float64),
                   FIELD_SIGMA_RE: ((nt, ne, n), np.float64), FIELD_SIGMA_IM: ((nt, ne, n), np.float64),
                   FIELD_STATE: ((nt, ne), np.int32), FIELD_STEPS: ((nt,), np.int32), FIELD_NHOPS: ((nt,), np.int32),
                   FIELD_SAVE_POP_AD: ((nt, nsave, n), np.float64), FIELD_SAVE_POP_DI: ((nt, nsave, n), np.float64)}

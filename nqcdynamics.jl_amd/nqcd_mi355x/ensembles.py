@@ -44,11 +44,14 @@ class TerminatingCallback:
 
 
 def _kind(sim):
-    """'sh' (FSSH/Ehrenfest), 'ie' (IESH/EhrenfestNA) or 'rp' (ring polymers); raises for the rest."""
+    """'sh' (FSSH/Ehrenfest), 'ie' (IESH/EhrenfestNA, also as ring polymers with BCBWavefunction) or
+    'rp' (ring polymers); raises for the rest."""
     m = sim.method
     if isinstance(sim, RingPolymerSimulation):
         if isinstance(m, (Classical, FSSH, Ehrenfest)):
             return "rp"
+        if isinstance(m, (AdiabaticIESH, EhrenfestNA)):
+            return "ie"
     elif isinstance(m, (FSSH, Ehrenfest)):
         return "sh"
     elif isinstance(m, (AdiabaticIESH, EhrenfestNA)):
@@ -67,8 +70,10 @@ def _problem(E, sim, dt):
                       algorithm=method.algorithm, temperature=sim.temperature, **common)
     elif kind == "ie":
         edc = isinstance(method, AdiabaticIESH) and method.decoherence is not None
+        rp = isinstance(sim, RingPolymerSimulation)
         E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
-                      algorithm=method.algorithm, n_electrons=sim.model.nelectrons,
+                      algorithm=L.ALG_BCB_WAVEFUNCTION if rp else method.algorithm, n_electrons=sim.model.nelectrons,
+                      nbeads=sim.nbeads, temperature=sim.temperature,
                       decoherence=L.DECOHERENCE_EDC if edc else L.DECOHERENCE_NONE,
                       edc_C=method.decoherence.C if edc else 0.1,
                       estimate_probability=getattr(method, "estimate_probability", True), **common)
@@ -136,7 +141,12 @@ def _initial_ie(E, sim, dist, trajectories, selection, seed):
     if conf is not None:
         r, v = conf
         occ = np.tile(np.arange(sim.model.nelectrons, dtype=np.int32), (r.shape[0], 1))
-        E.load_initial_iesh(r[:, 0], v[:, 0], occ)
+        if sim.nbeads > 1:  # (1, 1, nbeads) per trajectory
+            E.load_initial_iesh(r, v, occ)
+        else:
+            E.load_initial_iesh(r[:, 0], v[:, 0], occ)
+    elif sim.nbeads > 1:
+        raise ValueError("ring-polymer IESH / EhrenfestNA take bead configurations (DynamicsVariables or a list)")
     elif isinstance(nuc, DynamicalDistribution):
         mr, sr, mv, sv = nuc.moments()
         E.sample_initial(L.SAMPLER_GAUSSIAN, trajectories, [mr[0], sr[0], mv[0], sv[0]], seed)
@@ -254,6 +264,10 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
     if int(trajectories) < 1:
         raise ValueError("trajectories must be >= 1")
     _fermi_dirac(sim, distribution)
+    if kind == "ie" and sim.nbeads > 1:
+        nuc = distribution.nuclear if isinstance(distribution, ProductDistribution) else distribution
+        if not (isinstance(nuc, dict) or (isinstance(nuc, DynamicalDistribution) and nuc.is_list())):
+            raise ValueError("ring-polymer IESH / EhrenfestNA take bead configurations (DynamicsVariables or a list)")
     log.info("Performing %d trajectories on MI355X device %d.", trajectories, ensemble_algorithm.device)
     E = L.Ensemble(ensemble_algorithm.device)
     E._schedule = ensemble_algorithm.schedule

@@ -1272,6 +1272,190 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
     nhops_out[i] = nh;
 }
 
+// Ring-polymer IESH / EhrenfestNA with BCBWavefunction (bcb_wavefunction.jl:37-69), literally:
+// initialize! takes the bead accelerations at r (rpiesh.jl:20-38 with the occupations,
+// rpehrenfest_na.jl:14-40 with psi); perform_step!: vtmp = v + dt/2 k; r, vtmp to normal modes,
+// step_C! (Cayley, half = false), back; k = accelerations at the new beads (EhrenfestNA with psi of
+// the step start); v = vtmp + dt/2 k; then psi <- exp(-i dt H) psi with H = diag(E) - i d v at the
+// centroid of the step-start positions and velocities (get_quantum_propagator(sim, vprev, rprev),
+// wavefunction_dynamics.jl:27-58 with the centroid quantities of SurfaceHoppingMethods.jl:85-103).
+// The IESH hop (iesh.jl:390-407) uses the centroid eigenpairs, coupling and velocity at the step end;
+// an accepted hop rescales every bead by gamma d / m and a frustrated one with vinversion reverses the
+// centroid velocity along d (rpsh.jl:24-44).  Eigenvectors in the impurity-positive gauge at every
+// bead and at the centroid.  r0, v0, r_out, v_out: (ntraj, P); pop_di from the centroid eigenvectors.
+static void ie_rp_run_one(const OIEDesc& R, const ONADesc& md, int P, double temperature, int64_t i, uint64_t gid,
+                          const double* r0, const double* v0, const int32_t* occ0, const double* psi_re0,
+                          const double* psi_im0, double* r_out, double* v_out, double* psi_re, double* psi_im,
+                          int32_t* occ_out, int32_t* nhops_out, double* pop_di) {
+    const int N = md.M + 1, Ne = R.nelec;
+    std::vector<double> Tm((size_t)P * P), wk(P), Cy((size_t)4 * P);
+    normal_mode_matrix(P, Tm.data());
+    for (int k = 0; k < P; ++k) { wk[k] = matsubara(P, P * temperature, k); cayley(wk[k], R.dt, Cy.data() + 4 * k); }
+    std::vector<double> r(r0 + (size_t)i * P, r0 + (size_t)i * P + P), v(v0 + (size_t)i * P, v0 + (size_t)i * P + P);
+    std::vector<cd> psi((size_t)N * Ne);
+    for (int k = 0; k < N * Ne; ++k) psi[k] = cd(psi_re0[(size_t)i * N * Ne + k], psi_im0[(size_t)i * N * Ne + k]);
+    std::vector<int> occ(occ0 + (size_t)i * Ne, occ0 + (size_t)i * Ne + Ne);
+    // eigenpairs, U^T dV U and the couplings at position x, impurity components positive
+    auto eig = [&](double x, std::vector<double>& E, std::vector<double>& U, std::vector<double>& G, std::vector<double>& d,
+                   double& dU0) {
+        std::vector<double> V; double dh, U0;
+        na_matrices(md, x, V, dh, U0, dU0);
+        E.assign(N, 0.0); U.assign((size_t)N * N, 0.0); G.assign((size_t)N * N, 0.0); d.assign((size_t)N * N, 0.0);
+        eigen_sym(N, V.data(), E.data(), U.data(), nullptr);
+        for (int c = 0; c < N; ++c)
+            if (U[c] < 0) for (int a = 0; a < N; ++a) U[(size_t)a * N + c] = -U[(size_t)a * N + c];
+        std::vector<double> dV((size_t)N * N, 0.0);
+        dV[0] = dh;
+        adiabatic_quantities(N, 1, E.data(), U.data(), dV.data(), G.data(), d.data());
+    };
+    auto mean = [&](const std::vector<double>& a) { double s = 0.0; for (int b = 0; b < P; ++b) s += a[b]; return s / P; };
+    std::vector<double> Eb, Ub, Gb, db, k(P);
+    auto accels = [&](const std::vector<double>& rr, const std::vector<cd>& ps) {
+        for (int b = 0; b < P; ++b) {
+            double dU0;
+            eig(rr[b], Eb, Ub, Gb, db, dU0);
+            double f = dU0;
+            if (R.method == 0) { for (int e = 0; e < Ne; ++e) f += Gb[(size_t)occ[e] * N + occ[e]]; }
+            else {
+                for (int e = 0; e < Ne; ++e)
+                    for (int a = 0; a < N; ++a) for (int c = 0; c < N; ++c)
+                        f += (std::conj(ps[(size_t)a * Ne + e]) * Gb[(size_t)a * N + c] * ps[(size_t)c * Ne + e]).real();
+            }
+            k[b] = -f / R.mass;
+        }
+    };
+    std::vector<double> E, U, G, d;
+    double dU0c;
+    eig(mean(r), E, U, G, d, dU0c);
+    auto save_pop = [&](int slot) {
+        if (!pop_di || slot >= R.nsave) return;
+        double* out = pop_di + ((size_t)i * R.nsave + slot) * N;
+        for (int a = 0; a < N; ++a) {
+            double pp = 0.0;
+            if (R.method == 1) {
+                for (int e = 0; e < Ne; ++e) {
+                    cd sum = 0.0;
+                    for (int b = 0; b < N; ++b) sum += U[(size_t)a * N + b] * psi[(size_t)b * Ne + e];
+                    pp += std::norm(sum);
+                }
+            } else {
+                for (int e = 0; e < Ne; ++e) {
+                    double s1 = 0.0, s2 = 0.0;
+                    for (int b = 0; b < N; ++b) {
+                        const double c = psi[(size_t)b * Ne + e].real(), u = U[(size_t)a * N + b];
+                        s1 += u * c;
+                        s2 += u * u * c * c;
+                    }
+                    const double us = U[(size_t)a * N + occ[e]];
+                    pp += s1 * s1 - s2 + us * us;
+                }
+            }
+            out[a] = pp;
+        }
+    };
+    accels(r, psi);
+    save_pop(0);
+    int nh = 0;
+    std::vector<cd> H((size_t)N * N), Pm((size_t)N * N), tmp((size_t)N * Ne);
+    std::vector<double> Eh, q(P), pmode(P), vt(P);
+    std::vector<cd> Uh;
+    for (int64_t step = 0; step < R.nsteps; ++step) {
+        const std::vector<double> E0 = E, d0 = d;  // centroid quantities of the step start
+        const double vc0 = mean(v);
+        const std::vector<cd> psi0 = psi;
+        for (int b = 0; b < P; ++b) vt[b] = v[b] + (R.dt / 2) * k[b];                       // step_B!
+        for (int m = 0; m < P; ++m) {                                                         // to normal modes
+            double a1 = 0.0, a2 = 0.0;
+            for (int b = 0; b < P; ++b) { a1 += Tm[(size_t)b * P + m] * r[b]; a2 += Tm[(size_t)b * P + m] * vt[b]; }
+            q[m] = a1; pmode[m] = a2;
+        }
+        for (int m = 0; m < P; ++m) {                                                         // step_C!
+            const double* c = Cy.data() + 4 * m;
+            const double qn = c[0] * q[m] + c[1] * pmode[m], pn = c[2] * q[m] + c[3] * pmode[m];
+            q[m] = qn; pmode[m] = pn;
+        }
+        for (int b = 0; b < P; ++b) {                                                         // back
+            double a1 = 0.0, a2 = 0.0;
+            for (int m = 0; m < P; ++m) { a1 += Tm[(size_t)b * P + m] * q[m]; a2 += Tm[(size_t)b * P + m] * pmode[m]; }
+            r[b] = a1; vt[b] = a2;
+        }
+        accels(r, psi0);
+        for (int b = 0; b < P; ++b) v[b] = vt[b] + (R.dt / 2) * k[b];                        // step_B!
+        // psi <- exp(-i dt H(rprev, vprev)) psi
+        for (int a = 0; a < N; ++a) for (int b = 0; b < N; ++b)
+            H[(size_t)a * N + b] = cd((a == b) ? E0[a] : 0.0, 0.0) - cd(0.0, 1.0) * (d0[(size_t)a * N + b] * vc0);
+        eigen_hermitian(N, H, Eh, Uh);
+        for (int a = 0; a < N; ++a) for (int b = 0; b < N; ++b) {
+            cd sm = 0.0;
+            for (int kk = 0; kk < N; ++kk) sm += Uh[(size_t)a * N + kk] * std::exp(cd(0.0, -Eh[kk] * R.dt)) * std::conj(Uh[(size_t)b * N + kk]);
+            Pm[(size_t)a * N + b] = sm;
+        }
+        for (int a = 0; a < N; ++a) for (int e = 0; e < Ne; ++e) {
+            cd sm = 0.0;
+            for (int b = 0; b < N; ++b) sm += Pm[(size_t)a * N + b] * psi0[(size_t)b * Ne + e];
+            tmp[(size_t)a * Ne + e] = sm;
+        }
+        psi = tmp;
+        eig(mean(r), E, U, G, d, dU0c);                                                       // centroid at the step end
+        if (R.method == 0) {
+            const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
+            const double vc = mean(v);
+            std::vector<double> prob;
+            const bool evaluate = ie_probabilities(N, Ne, psi, occ, d, vc, R.dt, R.estimate != 0, rnd, prob);
+            if (evaluate) {
+                std::vector<char> isocc(N, 0);
+                for (int j = 0; j < Ne; ++j) isocc[occ[j]] = 1;
+                double cum = 0.0;
+                int hn = -1, hm = -1;
+                for (int n = 0; n < Ne && hn < 0; ++n)
+                    for (int m = 0; m < N; ++m) {
+                        if (isocc[m]) continue;
+                        cum += prob[(size_t)m * Ne + n];
+                        if (rnd < cum) { hn = n; hm = m; break; }
+                    }
+                if (hn >= 0) {
+                    const int old = occ[hn];
+                    bool ok = true;
+                    if (R.rescaling != RESCALE_OFF) {
+                        const double dd = d[(size_t)hm * N + old];
+                        const double aq = dd * dd / R.mass / 2.0, bq = dd * vc, cq = E[hm] - E[old];
+                        const double disc = bq * bq - 4.0 * aq * cq;
+                        if (disc < 0) {
+                            ok = false;
+                            if (R.rescaling == RESCALE_VINVERSION) {
+                                const double dn = dd / std::fabs(dd), g2 = vc * dn;
+                                for (int b = 0; b < P; ++b) v[b] -= 2.0 * g2 * dn;
+                            }
+                        } else {
+                            const double root = std::sqrt(disc);
+                            const double gam = (bq < 0) ? (bq + root) / (2.0 * aq) : (bq - root) / (2.0 * aq);
+                            for (int b = 0; b < P; ++b) v[b] -= gam * dd / R.mass;
+                        }
+                    }
+                    if (ok) { occ[hn] = hm; ++nh; }
+                }
+            }
+        }
+        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save_pop((int)((step + 1) / R.save_every));
+    }
+    for (int b = 0; b < P; ++b) { r_out[(size_t)i * P + b] = r[b]; v_out[(size_t)i * P + b] = v[b]; }
+    for (int kk = 0; kk < N * Ne; ++kk) { psi_re[(size_t)i * N * Ne + kk] = psi[kk].real(); psi_im[(size_t)i * N * Ne + kk] = psi[kk].imag(); }
+    for (int e = 0; e < Ne; ++e) occ_out[(size_t)i * Ne + e] = occ[e];
+    nhops_out[i] = nh;
+}
+
+OR_API int32_t oracle_rp_iesh_run_ensemble(const OIEDesc* R, const ONADesc* md, int32_t P, double temperature,
+                                           int64_t ntraj, const double* r0, const double* v0, const int32_t* occ0,
+                                           const double* psi_re0, const double* psi_im0, double* r, double* v,
+                                           double* psi_re, double* psi_im, int32_t* occ, int32_t* nhops, double* pop_di,
+                                           int32_t nthreads) {
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        ie_rp_run_one(*R, *md, P, temperature, i, (uint64_t)i, r0, v0, occ0, psi_re0, psi_im0, r, v, psi_re, psi_im,
+                      occ, nhops, pop_di);
+    });
+    return 0;
+}
+
 OR_API int32_t oracle_iesh_run_ensemble(const OIEDesc* R, const ONADesc* md, int64_t ntraj, const double* r0,
                                         const double* v0, const int32_t* occ0, const double* psi_re0,
                                         const double* psi_im0, double* r, double* v, double* psi_re, double* psi_im,

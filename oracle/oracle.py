@@ -384,6 +384,39 @@ def iesh_run_ensemble(method, md, nelec, mass, r0, v0, occ0, psi0=None, dt=1.0, 
     return out
 
 
+def rp_iesh_run_ensemble(method, md, nelec, mass, P, temperature, r0, v0, occ0, psi0=None, dt=1.0, nsteps=100, key=0,
+                         rescaling=RESCALE_STANDARD, estimate=True, save_every=0, nthreads=None):
+    """Ring-polymer IESH (method 0) / EhrenfestNA (1) with BCBWavefunction; r0, v0 (ntraj, P)."""
+    r0 = np.ascontiguousarray(r0, dtype=np.float64).reshape(-1, P)
+    v0 = np.ascontiguousarray(v0, dtype=np.float64).reshape(-1, P)
+    ntraj = r0.shape[0]
+    N = md.M + 1
+    occ0 = np.ascontiguousarray(occ0, dtype=np.int32).reshape(ntraj, nelec)
+    if psi0 is None:
+        psi0 = np.zeros((ntraj, N, nelec), complex)
+        for i in range(ntraj):
+            for e in range(nelec):
+                psi0[i, occ0[i, e], e] = 1.0
+    R = OIEDesc()
+    R.method, R.nelec, R.rescaling, R.estimate = method, nelec, rescaling, int(estimate)
+    R.dt, R.t0, R.mass, R.nsteps, R.key, R.edc, R.edc_C = dt, 0.0, mass, nsteps, key, 0, 0.1
+    nsave = nsteps // save_every + 1 if save_every else 1
+    R.save_every, R.nsave = save_every, nsave
+    out = dict(r=np.zeros((ntraj, P)), v=np.zeros((ntraj, P)), psi_re=np.zeros((ntraj, N, nelec)),
+               psi_im=np.zeros((ntraj, N, nelec)), occ=np.zeros((ntraj, nelec), np.int32), nhops=np.zeros(ntraj, np.int32),
+               pop_di=np.zeros((ntraj, nsave, N)))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_rp_iesh_run_ensemble
+    f.restype = C.c_int32
+    f(C.byref(R), C.byref(md), C.c_int32(P), C.c_double(temperature), C.c_int64(ntraj), _p(r0), _p(v0),
+      _p(occ0, C.c_int32), _p(np.ascontiguousarray(psi0.real)), _p(np.ascontiguousarray(psi0.imag)), _p(out["r"]),
+      _p(out["v"]), _p(out["psi_re"]), _p(out["psi_im"]), _p(out["occ"], C.c_int32), _p(out["nhops"], C.c_int32),
+      _p(out["pop_di"]), C.c_int32(nthreads))
+    out["psi"] = out["psi_re"] + 1j * out["psi_im"]
+    return out
+
+
 def iesh_probabilities(psi, occ, d, v, dt, clamp=True):
     """prob[m, n] for electron n hopping to unoccupied state m (no early exit)."""
     psi = np.asarray(psi, dtype=complex)

@@ -328,3 +328,61 @@ def test_fermi_dirac_diabatic_is_iesh_only():
     from nqcd_mi355x import _lib as L
     with pytest.raises(L.NQCDError, match="IESH"):
         _fd_occupations(L, L.METHOD_EHRENFEST_NA, 40, 20, np.zeros(4), 1.0, 1, diabatic=True)
+
+
+def _rp_inputs(ntraj, Ne, P, seed):
+    r0, v0, occ0 = _inputs(ntraj, Ne, seed)
+    rng = np.random.default_rng(seed + 1)
+    return r0[:, None] + rng.normal(0, 0.05, (ntraj, P)), v0[:, None] + rng.normal(0, 1e-3, (ntraj, P)), occ0
+
+
+@pytest.mark.parametrize("method,M,Ne,P,ntraj,nsteps,resc", [(0, 40, 20, 4, 32, 200, 0), (0, 20, 10, 8, 32, 200, 1),
+                                                             (1, 40, 20, 4, 16, 150, 0), (0, 63, 30, 2, 6, 80, 0)])
+def test_ring_polymer_iesh_parity(oracle, key, method, M, Ne, P, ntraj, nsteps, resc):
+    """RingPolymerSimulation{AdiabaticIESH / EhrenfestNA} with BCBWavefunction (bcb_wavefunction.jl:37-69):
+    bead forces from per-bead eigenpairs, Cayley free ring-polymer step in normal modes, the
+    wavefunction step at the centroid of the step start, hops on the centroid with every bead
+    rescaled; against the oracle: identical hops, beads, psi and populations within 1e-9."""
+    from nqcd_mi355x import _lib as L
+    temp = 3.166811563e-6 * 300.0
+    rb, vb, occ0 = _rp_inputs(ntraj, Ne, P, 31)
+    E = L.Ensemble(0)
+    try:
+        E.set_problem(L.METHOD_IESH if method == 0 else L.METHOD_EHRENFEST_NA, M + 1, 1, 1, [MASS], 2.0,
+                      algorithm=L.ALG_BCB_WAVEFUNCTION, nbeads=P, temperature=temp, n_electrons=Ne, rescaling=resc)
+        E.set_model(0, L.MODEL_NEWNS_ANDERSON, _params(M))
+        E.load_initial_iesh(rb, vb, occ0)
+        E.run(0.0, nsteps * 2.0, key, save_every=10)
+        g = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
+             "psi": E.download(L.FIELD_SIGMA_RE) + 1j * E.download(L.FIELD_SIGMA_IM),
+             "occ": E.download(L.FIELD_STATE), "nhops": E.download(L.FIELD_NHOPS), "di": E.download(L.FIELD_SAVE_POP_DI)}
+    finally:
+        E.close()
+    W = 6 * MODEL["Gamma"] / 2
+    md = oracle.na_model(M=M, W=W, m=MODEL["m"], omega=MODEL["omega"], g=MODEL["g"], dG=MODEL["dG"], Gamma=MODEL["Gamma"])
+    c = oracle.rp_iesh_run_ensemble(method, md, Ne, MASS, P, temp, rb, vb, occ0, dt=2.0, nsteps=nsteps, key=key,
+                                    rescaling=resc, save_every=10)
+    if method == 0:
+        assert np.array_equal(g["occ"], c["occ"])
+        assert np.array_equal(g["nhops"], c["nhops"])
+        if M == 40:
+            assert c["nhops"].sum() >= 3
+    assert g["r"].shape == (ntraj, P)
+    assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-11)
+    assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-13)
+    assert np.allclose(g["psi"], c["psi"], rtol=1e-9, atol=1e-10)
+    assert np.allclose(g["di"], c["pop_di"], rtol=1e-9, atol=1e-11)
+
+
+def test_ring_polymer_iesh_rejects_edc_and_verlet():
+    from nqcd_mi355x import _lib as L
+    E = L.Ensemble(0)
+    try:
+        with pytest.raises(L.NQCDError, match="BCBWavefunction"):
+            E.set_problem(L.METHOD_IESH, 41, 1, 1, [MASS], 2.0, algorithm=L.ALG_VERLET_ELECTRONICS, nbeads=4,
+                          temperature=1e-3, n_electrons=20)
+        with pytest.raises(L.NQCDError, match="EDC"):
+            E.set_problem(L.METHOD_IESH, 41, 1, 1, [MASS], 2.0, algorithm=L.ALG_BCB_WAVEFUNCTION, nbeads=4,
+                          temperature=1e-3, n_electrons=20, decoherence=L.DECOHERENCE_EDC)
+    finally:
+        E.close()

@@ -59,9 +59,13 @@ def test_run_dynamics_rejects_before_touching_the_device():
     with pytest.raises(ValueError, match="TerminatingCallback"):
         N.run_dynamics(sim, (0.0, 10.0), u, output=N.OutputSurfaceHops, callback=lambda u, t, i: True)
     na = N.AndersonHolstein(N.MiaoSubotnik(), N.TrapezoidalRule(40, -0.0192, 0.0192))
-    rp = N.RingPolymerSimulation[N.AdiabaticIESH](N.Atoms(2000), na, 8)
+    cl = N.Simulation[N.Classical](N.Atoms(2000), N.TullyModelOne())
     with pytest.raises(ValueError, match="does not support"):
-        N.run_dynamics(rp, (0.0, 10.0), u, output=N.OutputSurfaceHops)
+        N.run_dynamics(cl, (0.0, 10.0), u, output=N.OutputSurfaceHops)
+    rp = N.RingPolymerSimulation[N.AdiabaticIESH](N.Atoms(2000), na, 8, temperature=9.5e-4)
+    with pytest.raises(ValueError, match="bead configurations"):
+        N.run_dynamics(rp, (0.0, 10.0), N.DynamicalDistribution(N.Normal(0.0, 1e-2), N.Normal(8.0, 1.0), (1, 1, 8)),
+                       output=N.OutputDiabaticPopulation)
     ie = N.Simulation[N.AdiabaticIESH](N.Atoms(2000), na)
     with pytest.raises(ValueError, match="termination"):
         N.run_dynamics(ie, (0.0, 10.0), u, output=N.OutputSurfaceHops, callback=N.TerminatingCallback(1, 10.0))

@@ -366,3 +366,34 @@ def test_fermi_dirac_diabatic_adiabatic_state_distribution(oracle):
         for b in range(a + 1, 21):
             p = w1[a] * w2[b] / (1 - w2[a]) + w1[b] * w2[a] / (1 - w2[b])
             assert abs(cnt[(a, b)] / n - p) <= 5 * np.sqrt(p * (1 - p) / n) + 1e-3, (a, b)
+
+
+def test_ring_polymer_iesh_conserves_ring_polymer_energy(oracle):
+    """BCBWavefunction for RingPolymerSimulation{AdiabaticIESH} (bcb_wavefunction.jl, rpiesh.jl:20-38)
+    without hops: the beads move on U0 + sum_occ E_k with the ring-polymer springs, and the ring-polymer
+    energy sum_b [m v_b^2 / 2 + U0(x_b) + sum_occ E_k(x_b)] + m w_n^2 / 2 sum_b (x_b - x_b+1)^2 is kept
+    to 1e-9 with a second-order error (a wrong force sign, mode frequency or Cayley entry breaks it);
+    EhrenfestNA keeps every electron normalised."""
+    M, Ne, P, mass = 20, 10, 4, 2000.0
+    md = oracle.na_model(M=M)
+    mw2, temp = 2000 * 2e-4 ** 2, 3.166811563e-6 * 300.0
+    wn = P * temp
+    rng = np.random.default_rng(3)
+    r0 = rng.normal(-3.0, 0.3, (8, P))
+    v0 = rng.normal(0.0, 2e-3, (8, P))
+    occ0 = np.tile(np.arange(Ne), (8, 1))
+
+    def energy(r, v, occ):
+        e = 0.5 * mass * np.sum(v ** 2) + 0.5 * mass * wn ** 2 * np.sum((r - np.roll(r, -1)) ** 2)
+        for b in range(P):
+            e += 0.5 * mw2 * r[b] ** 2 + oracle.na_eigen(md, r[b])[0][occ].sum()
+        return e
+
+    errs = []
+    for dt in (1.0, 0.5):
+        c = oracle.rp_iesh_run_ensemble(0, md, Ne, mass, P, temp, r0, v0, occ0, dt=dt, nsteps=int(1000 / dt), key=5)
+        assert c["nhops"].sum() == 0
+        errs.append(max(abs(energy(c["r"][i], c["v"][i], c["occ"][i]) - energy(r0[i], v0[i], occ0[i])) for i in range(8)))
+    assert errs[0] < 1e-9 and 3.0 < errs[0] / errs[1] < 5.0
+    c = oracle.rp_iesh_run_ensemble(1, md, Ne, mass, P, temp, r0, v0, occ0, dt=1.0, nsteps=300, key=5)
+    assert np.allclose(np.sum(np.abs(c["psi"]) ** 2, axis=1), 1.0, atol=1e-11)
