@@ -103,6 +103,23 @@ def iesh_extra(L):
                         "flop_per_traj_step": flop, "achieved_TFLOPs": flop * rate / 1e12,
                         "frac_fp64_peak": flop * rate / 1e12 / FP64_PEAK_TFLOPS, "grid": st["grid"], "block": st["block"]}
         _progress(f"iesh N={N}: {rate:.3e} trajectory-steps/s")
+    # ring-polymer IESH (BCBWavefunction), 8 beads at 300 K: trajectory-steps/s (a ring polymer = one trajectory)
+    M, P, ntraj, nsteps = 40, 8, 4096, 100
+    rng = np.random.default_rng(1)
+    rb = rng.normal(8.0, 1.0, (ntraj, 1)) + rng.normal(0.0, 0.05, (ntraj, P))
+    vb = rng.normal(0.0, 1e-2, (ntraj, 1)) + rng.normal(0.0, 1e-3, (ntraj, P))
+    E = L.Ensemble(0)
+    E.set_problem(L.METHOD_IESH, M + 1, 1, 1, [2000.0], 2.0, algorithm=L.ALG_BCB_WAVEFUNCTION, nbeads=P,
+                  temperature=3.166811563e-6 * 300.0, n_electrons=M // 2)
+    E.set_model(0, L.MODEL_NEWNS_ANDERSON, [2000.0, 2e-4, 20.6097, -3.8e-3, 6.4e-3, -W, W, M])
+    E.load_initial_iesh(rb, vb, np.tile(np.arange(M // 2, dtype=np.int32), (ntraj, 1)))
+    E.run(0.0, 2.0 * 10, KEY)
+    st = E.run(0.0, 2.0 * nsteps, KEY, save_every=10)
+    E.close()
+    rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+    res[f"N{M + 1}_P{P}"] = {"ntraj": ntraj, "nsteps": nsteps, "nbeads": P, "kernel_ms": st["kernel_ms"],
+                              "traj_steps_per_s": rate, "grid": st["grid"], "block": st["block"]}
+    _progress(f"ring-polymer iesh N={M + 1} P={P}: {rate:.3e} trajectory-steps/s")
     return res
 
 

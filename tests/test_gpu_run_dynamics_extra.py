@@ -91,3 +91,25 @@ def test_rpsh_run_dynamics_centroid_and_populations(oracle, key):
     assert c["nhops"].sum() > 0
     assert np.allclose(res["OutputCentroidPosition"], c["centroid"].mean(axis=0), rtol=1e-9, atol=1e-11)
     assert np.allclose(np.array(res["OutputDiabaticPopulation"]), c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-11)
+
+
+def test_ring_polymer_iesh_run_dynamics(oracle, key):
+    """RingPolymerSimulation{AdiabaticIESH}(atoms, model, n_beads; temperature) through run_dynamics
+    (bead configurations from a list distribution, BCBWavefunction selected as select_algorithm does,
+    IntegrationAlgorithms.jl:99) against the oracle: mean populations and final beads."""
+    M, P, ntraj, dt, nsteps = 40, 4, 24, 2.0, 150
+    W = 6 * 6.4e-3 / 2
+    temp = 3.166811563e-6 * 300.0
+    model = N.AndersonHolstein(N.MiaoSubotnik(), N.TrapezoidalRule(M, -W, W))
+    sim = N.RingPolymerSimulation[N.AdiabaticIESH](N.Atoms(2000), model, P, temperature=temp)
+    rng = np.random.default_rng(9)
+    rb = rng.normal(8.0, 1.0, (ntraj, 1)) + rng.normal(0, 0.05, (ntraj, P))
+    vb = np.abs(rng.normal(0.0, 1e-2, (ntraj, 1))) + rng.normal(0, 1e-3, (ntraj, P))
+    dist = N.DynamicalDistribution([v.reshape(1, 1, P) for v in vb], [r.reshape(1, 1, P) for r in rb], (1, 1, P))
+    res = N.run_dynamics(sim, (0.0, nsteps * dt), dist, trajectories=ntraj, dt=dt, saveat=10 * dt,
+                         output=N.OutputDiabaticPopulation, reduction=N.MeanReduction(),
+                         ensemble_algorithm=N.EnsembleMI355X(seed=key))
+    md = oracle.na_model(M=M, W=W)
+    c = oracle.rp_iesh_run_ensemble(0, md, M // 2, 2000.0, P, temp, rb, vb, np.tile(np.arange(M // 2), (ntraj, 1)),
+                                    dt=dt, nsteps=nsteps, key=key, save_every=10)
+    assert np.allclose(np.array(res["OutputDiabaticPopulation"]), c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-12)
