@@ -173,7 +173,10 @@ enum nqcd_output_kind {
     NQCD_OUT_SCATTERING_1D = 2,          /* final, [reflection(n), transmission(n)] (DynamicsOutputs.jl:313-338) */
     NQCD_OUT_POPULATION_CORRELATION = 3, /* per save slot, n x n P(0)P(t)^T diabatic (TimeCorrelationFunctions.jl:61-88) */
     NQCD_OUT_FINAL_ADIABATIC_POPULATION = 4, /* final, nstates */
-    NQCD_OUT_CENTROID_POSITION = 5       /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
+    NQCD_OUT_CENTROID_POSITION = 5,      /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
+    NQCD_OUT_KINETIC_ENERGY = 6,         /* per save slot (OutputKineticEnergy, DynamicsOutputs.jl:97) */
+    NQCD_OUT_TOTAL_ENERGY = 7            /* per save slot, kinetic + classical_potential_energy (OutputTotalEnergy,
+                                            DynamicsOutputs.jl:90; fssh.jl:150-154, ehrenfest.jl:85-95) */
 };
 /* IESH: DIABATIC_POPULATION is iesh_diabatic_population (iesh.jl:337-369), state 0 = impurity;
  * EhrenfestNA (which the reference gives no diabatic estimator) reports sum_e |(U psi)_{a e}|^2. */
@@ -194,7 +197,9 @@ enum nqcd_field {
     NQCD_FIELD_SAVE_STATE = 9,   /* int32 (nsave, ntraj) active state per save                 */
     NQCD_FIELD_SAVE_CENTROID = 10, /* double (nsave, ntraj) ring-polymer centroid per save      */
     NQCD_FIELD_FALLBACKS = 11,     /* int32 (ntraj) surrogate steps that fell back to the analytic model */
-    NQCD_FIELD_SAVE_R = 12         /* double (ndofs * natoms, nsave, ntraj) configuration per save (diatom-surface) */
+    NQCD_FIELD_SAVE_R = 12,        /* double (ndofs * natoms, nsave, ntraj) configuration per save (diatom-surface) */
+    NQCD_FIELD_SAVE_EKIN = 13,     /* double (nsave, ntraj) kinetic energy per save (two-state one-dof FSSH / Ehrenfest) */
+    NQCD_FIELD_SAVE_EPOT = 14      /* double (nsave, ntraj) classical potential energy per save (same problems) */
 };
 /* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
  * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD / SAVE_POP_DI (n, nsave, ntraj). */

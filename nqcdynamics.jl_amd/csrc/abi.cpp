@@ -90,6 +90,10 @@ int32_t alloc_results(nqcd_handle* h) {
     HIPCHK(h, launch_zero(h->save_ad.p, sizeof(double) * 2 * h->nsave * n, h->stream));
     HIPCHK(h, launch_zero(h->save_di.p, sizeof(double) * 2 * h->nsave * n, h->stream));
     HIPCHK(h, launch_zero(h->save_state.p, sizeof(int32_t) * h->nsave * n, h->stream));
+    HIPCHK(h, h->save_ekin.ensure(sizeof(double) * h->nsave * n));
+    HIPCHK(h, h->save_epot.ensure(sizeof(double) * h->nsave * n));
+    HIPCHK(h, launch_zero(h->save_ekin.p, sizeof(double) * h->nsave * n, h->stream));
+    HIPCHK(h, launch_zero(h->save_epot.p, sizeof(double) * h->nsave * n, h->stream));
     // constant tables read by the kernels (models per slot, 1/m per flattened dof)
     HIPCHK(h, h->models_dev.ensure(sizeof(ModelSlot) * kMaxModels));
     HIPCHK(h, h->inv_mass_dev.ensure(sizeof(double) * D));
@@ -126,6 +130,7 @@ Fssh2Args make_args(nqcd_handle* h, double t0, int64_t nsteps, const nqcd_term_d
     A.state = h->state.as<int32_t>(); A.steps = h->steps.as<int32_t>(); A.nhops = h->nhops.as<int32_t>();
     A.save_pop_ad = h->save_ad.as<double>(); A.save_pop_di = h->save_di.as<double>();
     A.save_state = h->save_state.as<int32_t>();
+    A.save_ekin = h->save_ekin.as<double>(); A.save_epot = h->save_epot.as<double>();
     A.queue = h->queue.as<unsigned long long>();
     return A;
 }
@@ -170,7 +175,8 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
                       &h->mlp_partial, &h->mlp_losspart, &h->mlp_lossbuf, &h->mlp_grad, &h->fallbacks,
                       &h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
-                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid, &h->sum64};
+                      &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid, &h->sum64,
+                      &h->save_ekin, &h->save_epot};
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(h->ev0);
     (void)hipEventDestroy(h->ev1);
@@ -538,7 +544,12 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* 
         else if (kind == NQCD_OUT_DIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_di.as<double>(); }
         else return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA outputs: ADIABATIC_POPULATION, DIABATIC_POPULATION");
     }
+    const bool energy = kind == NQCD_OUT_KINETIC_ENERGY || kind == NQCD_OUT_TOTAL_ENERGY;
+    if (energy && (is_rp(h) || nd_is_problem(h) || is_spin_boson(h) || !is_two_state_1d(h)))
+        return fail(h, NQCD_ERR_UNSUPPORTED, "energy outputs exist for two-state one-dof FSSH / Ehrenfest");
     if (!src) switch (kind) {
+    case NQCD_OUT_KINETIC_ENERGY:
+    case NQCD_OUT_TOTAL_ENERGY: rows = h->nsave; src = h->save_ekin.as<double>(); break;
     case NQCD_OUT_ADIABATIC_POPULATION: rows = n * h->nsave; src = h->save_ad.as<double>(); break;
     case NQCD_OUT_DIABATIC_POPULATION: rows = n * h->nsave; src = h->save_di.as<double>(); break;
     case NQCD_OUT_FINAL_ADIABATIC_POPULATION: rows = n; break;
@@ -565,6 +576,14 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* 
     HIPCHK(h, launch_reduce_rows(src, rows, nt, h->partial.as<double>(), scale, h->small.as<double>(), h->stream));
     HIPCHK(h, hipMemcpyAsync(out, h->small.p, sizeof(double) * rows, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (kind == NQCD_OUT_TOTAL_ENERGY) {  // reduce the potential energy too; the reduction is linear
+        std::vector<double> pot(rows);
+        HIPCHK(h, launch_reduce_rows(h->save_epot.as<double>(), rows, nt, h->partial.as<double>(), scale,
+                                     h->small.as<double>(), h->stream));
+        HIPCHK(h, hipMemcpyAsync(pot.data(), h->small.p, sizeof(double) * rows, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int32_t k = 0; k < rows; ++k) out[k] += pot[k];
+    }
     return NQCD_OK;
 }
 
@@ -649,6 +668,18 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
             return fail(h, NQCD_ERR_UNSUPPORTED, "fallback counts exist for surrogate runs only");
         if (bytes != (int64_t)sizeof(int32_t) * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
         HIPCHK(h, hipMemcpy(out, h->fallbacks.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost));
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_SAVE_EKIN:
+    case NQCD_FIELD_SAVE_EPOT: {
+        if (is_rp(h) || nd_is_problem(h) || is_spin_boson(h) || !is_two_state_1d(h))
+            return fail(h, NQCD_ERR_UNSUPPORTED, "energies per save exist for two-state one-dof FSSH / Ehrenfest");
+        if (bytes != (int64_t)sizeof(double) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(field == NQCD_FIELD_SAVE_EKIN ? h->save_ekin : h->save_epot, sizeof(double) * ns * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = s[(size_t)sl * nt + i];
         return NQCD_OK;
     }
     case NQCD_FIELD_SAVE_STATE: {

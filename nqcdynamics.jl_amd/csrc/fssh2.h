@@ -39,6 +39,8 @@ struct Fssh2Args {
     double* save_pop_ad;   // [nsave][2][ntraj] or null
     double* save_pop_di;   // [nsave][2][ntraj] or null
     int32_t* save_state;   // [nsave][ntraj] or null
+    double* save_ekin;     // [nsave][ntraj] classical kinetic energy, or null
+    double* save_epot;     // [nsave][ntraj] classical potential energy (fssh.jl:150-154, ehrenfest.jl:85-95), or null
     unsigned long long* queue;
 };
 

@@ -88,6 +88,17 @@ __device__ __forceinline__ void save_slot(const Fssh2Args* Ap, const Lane2<D, R>
         A.save_pop_di[((int64_t)slot * 2 + 1) * n + L.tid] = pd1;
     }
     if (A.save_state) A.save_state[(int64_t)slot * n + L.tid] = L.st;
+    if (A.save_ekin) {  // classical_kinetic_energy and classical_potential_energy at the slot
+        R v11, v22, v12, d11, d22, d12;
+        model1d_eval<R>(L.m, L.r[0], v11, v22, v12, d11, d22, d12);
+        double E0, E1, Ut[4] = {1.0, 0.0, 0.0, 1.0};
+        eigen2((double)v11, (double)v12, (double)v22, E0, E1, Ut, false);
+        double ek = 0.0;
+#pragma unroll
+        for (int I = 0; I < D; ++I) ek += 0.5 * (double)L.v[I] * (double)L.v[I] / A.inv_mass[I];
+        A.save_ekin[(int64_t)slot * n + L.tid] = ek;
+        A.save_epot[(int64_t)slot * n + L.tid] = (A.method == METHOD_FSSH) ? ((L.st == 0) ? E0 : E1) : pa0 * E0 + pa1 * E1;
+    }
 }
 
 template <int D, typename R>
@@ -262,7 +273,7 @@ enum { PL_R = 0, PL_V = 1, PL_K = 2, PL_SRE = 3, PL_SIM = 7, PL_E0C = 11, PL_E1C
 enum { PI_ST = 0, PI_NH = 1, PI_STEP = 2, PI_DONE = 3, PI_NROWS = 4 };
 
 __device__ __forceinline__ void plain_save(const Fssh2Args& A, int64_t tid, int slot, int st, const C2& s,
-                                           const double U[4]) {
+                                           const double U[4], double v, double E0, double E1) {
     if (slot >= A.nsave) return;
     const int64_t n = A.ntraj;
     double S00, S11, pa0, pa1;
@@ -274,6 +285,10 @@ __device__ __forceinline__ void plain_save(const Fssh2Args& A, int64_t tid, int 
     if (A.save_pop_ad) { A.save_pop_ad[((int64_t)slot * 2) * n + tid] = pa0; A.save_pop_ad[((int64_t)slot * 2 + 1) * n + tid] = pa1; }
     if (A.save_pop_di) { A.save_pop_di[((int64_t)slot * 2) * n + tid] = pd0; A.save_pop_di[((int64_t)slot * 2 + 1) * n + tid] = pd1; }
     if (A.save_state) A.save_state[(int64_t)slot * n + tid] = st;
+    if (A.save_ekin) {  // classical_kinetic_energy and classical_potential_energy at the slot
+        A.save_ekin[(int64_t)slot * n + tid] = 0.5 * v * v / A.inv_mass[0];
+        A.save_epot[(int64_t)slot * n + tid] = (A.method == METHOD_FSSH) ? ((st == 0) ? E0 : E1) : pa0 * E0 + pa1 * E1;
+    }
 }
 
 __global__ __launch_bounds__(256) void fssh2_plain_init(const Fssh2Args A, double* S, int32_t* Si) {
@@ -303,7 +318,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_init(const Fssh2Args A, doubl
     Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = 0; Si[PI_STEP * n + tid] = 0;
     const bool done = (A.nsteps <= 0) || term_hit(A.term_kind, r, A.term_value);
     Si[PI_DONE * n + tid] = done ? 1 : 0;
-    plain_save(A, tid, 0, st, s, U);
+    plain_save(A, tid, 0, st, s, U, v, E0, E1);
 }
 
 __global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, double* S, int32_t* Si) {
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, doubl
     for (int J = 0; J < 4; ++J) S[(PL_U + J) * n + tid] = U[J];
     S[PL_T * n + tid] = tn;
     Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = nh; Si[PI_STEP * n + tid] = step;
-    if (A.save_every > 0 && step % A.save_every == 0) plain_save(A, tid, step / A.save_every, st, s, U);
+    if (A.save_every > 0 && step % A.save_every == 0) plain_save(A, tid, step / A.save_every, st, s, U, v, E0, E1);
     const bool done = (step >= A.nsteps) || term_hit(A.term_kind, r, A.term_value);
     if (done) Si[PI_DONE * n + tid] = 1;
 }

@@ -75,7 +75,7 @@ struct nqcd_handle {
     int32_t nsave = 1, save_every = 0;
     nqcd::DevBuf r0, v0, sig0, state0, model_id, order, models_dev, bath, inv_mass_dev, args_dev;
     nqcd::DevBuf r, v, sig, state, steps, nhops, save_ad, save_di, save_state;
-    nqcd::DevBuf queue, plainS, plainSi, tmp, partial, small, save_centroid, sum64;
+    nqcd::DevBuf queue, plainS, plainSi, tmp, partial, small, save_centroid, sum64, save_ekin, save_epot;
     bool state0_given = false, model_id_given = false, order_given = false;
     nqcd_run_stats stats{};
     int cu_count = 0;

@@ -41,8 +41,9 @@ const MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSS
       MODEL_NEWNS_ANDERSON, MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE = Int32.(0:8)
 const FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS,
       FIELD_SAVE_POP_AD, FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID = Int32.(0:10)
+const FIELD_SAVE_EKIN, FIELD_SAVE_EPOT = Int32(13), Int32(14)
 const OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION,
-      OUT_FINAL_ADIABATIC_POPULATION, OUT_CENTROID_POSITION = Int32.(0:5)
+      OUT_FINAL_ADIABATIC_POPULATION, OUT_CENTROID_POSITION, OUT_KINETIC_ENERGY, OUT_TOTAL_ENERGY = Int32.(0:7)
 const RED_MEAN, RED_SUM = Int32(0), Int32(1)
 
 struct EnsembleMI355X <: SciMLBase.EnsembleAlgorithm
@@ -203,12 +204,14 @@ device_reduce(h, kind, red, len) = (a = Vector{Float64}(undef, len);
 output_name(f) = f isa Function ? nameof(f) : nameof(typeof(f))
 device_output_kind(f) = Dict(:OutputAdiabaticPopulation => OUT_ADIABATIC_POPULATION,
     :OutputDiabaticPopulation => OUT_DIABATIC_POPULATION, :OutputCentroidPosition => OUT_CENTROID_POSITION,
-    :OutputStateResolvedScattering1D => OUT_SCATTERING_1D, :PopulationCorrelationFunction => OUT_POPULATION_CORRELATION)[output_name(f)]
+    :OutputStateResolvedScattering1D => OUT_SCATTERING_1D, :PopulationCorrelationFunction => OUT_POPULATION_CORRELATION,
+    :OutputKineticEnergy => OUT_KINETIC_ENERGY, :OutputTotalEnergy => OUT_TOTAL_ENERGY)[output_name(f)]
 
 struct DeviceResults
     n::Int; nsave::Int; sz::Tuple
     r::Array{Float64}; v::Array{Float64}; state::Array{Int32}; steps::Vector{Int32}; nhops::Vector{Int32}
     pop_ad::Array{Float64,3}; pop_di::Array{Float64,3}; save_state::Matrix{Int32}
+    ekin::Matrix{Float64}; epot::Matrix{Float64}   # empty unless the outputs ask for energies
 end
 
 function per_trajectory(f, d::DeviceResults, i, sim, t0, dt)
@@ -217,6 +220,8 @@ function per_trajectory(f, d::DeviceResults, i, sim, t0, dt)
     name === :OutputDiabaticPopulation && return [d.pop_di[:, s, i] for s in 1:d.nsave]
     name === :OutputDiscreteState && return [Int(d.save_state[s, i]) + 1 for s in 1:d.nsave]
     name === :OutputSurfaceHops && return Int(d.nhops[i])
+    name === :OutputKineticEnergy && return d.ekin[:, i]                     # DynamicsOutputs.jl:97
+    name === :OutputTotalEnergy && return d.ekin[:, i] .+ d.epot[:, i]       # DynamicsOutputs.jl:90
     name === :OutputFinalTime && return t0 + d.steps[i] * dt
     name === :PopulationCorrelationFunction && return [d.pop_di[:, 1, i] * d.pop_di[:, s, i]' for s in 1:d.nsave]
     if name === :OutputStateResolvedScattering1D            # DynamicsOutputs.jl:313-338
@@ -262,7 +267,8 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         red = prob.reduction
         if (red isa Ensembles.MeanReduction || red isa Ensembles.SumReduction) &&
                 all(f -> haskey(Dict(:OutputAdiabaticPopulation => 1, :OutputDiabaticPopulation => 1,
-                    :OutputCentroidPosition => 1, :OutputStateResolvedScattering1D => 1, :PopulationCorrelationFunction => 1),
+                    :OutputCentroidPosition => 1, :OutputStateResolvedScattering1D => 1, :PopulationCorrelationFunction => 1,
+                    :OutputKineticEnergy => 1, :OutputTotalEnergy => 1),
                     output_name(f)), outputs)
             # reduced on the device in a fixed order (reductions.jl:8-52)
             rk = red isa Ensembles.MeanReduction ? RED_MEAN : RED_SUM
@@ -272,7 +278,7 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
                 k = device_output_kind(f)
                 if k == OUT_SCATTERING_1D
                     a = device_reduce(h[], k, rk, 2n); insert!(out, output_name(f), (reflection=a[1:n], transmission=a[n+1:2n]))
-                elseif k == OUT_CENTROID_POSITION
+                elseif k in (OUT_CENTROID_POSITION, OUT_KINETIC_ENERGY, OUT_TOTAL_ENERGY)
                     insert!(out, output_name(f), device_reduce(h[], k, rk, nsave))
                 elseif k == OUT_POPULATION_CORRELATION
                     a = reshape(device_reduce(h[], k, rk, n * n * nsave), n, n, nsave)
@@ -293,7 +299,11 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
             download(h[], FIELD_STATE, Int32, nst, trajectories), download(h[], FIELD_STEPS, Int32, trajectories),
             download(h[], FIELD_NHOPS, Int32, trajectories), download(h[], FIELD_SAVE_POP_AD, Float64, n, nsave, trajectories),
             download(h[], FIELD_SAVE_POP_DI, Float64, n, nsave, trajectories),
-            ie ? zeros(Int32, nsave, trajectories) : download(h[], FIELD_SAVE_STATE, Int32, nsave, trajectories))
+            ie ? zeros(Int32, nsave, trajectories) : download(h[], FIELD_SAVE_STATE, Int32, nsave, trajectories),
+            any(f -> output_name(f) in (:OutputKineticEnergy, :OutputTotalEnergy), outputs) ?
+                download(h[], FIELD_SAVE_EKIN, Float64, nsave, trajectories) : zeros(0, 0),
+            any(f -> output_name(f) in (:OutputKineticEnergy, :OutputTotalEnergy), outputs) ?
+                download(h[], FIELD_SAVE_EPOT, Float64, nsave, trajectories) : zeros(0, 0))
         batch = map(1:trajectories) do i
             out = Ensembles.Dictionary{Symbol,Any}()
             savetime && insert!(out, :Time, times)

@@ -16,7 +16,7 @@ from .models import (AndersonHolstein, DebyeSpectralDensity, DiatomSurface, Doub
                      TullyModelThree, TullyModelTwo, nelectrons, nstates, train_surrogate)
 from .outputs import (FileReduction, MeanReduction, OutputAdiabaticPopulation, OutputCentroidPosition,  # noqa: F401
                       OutputDiabaticPopulation,
-                      OutputDiscreteState, OutputFinal, OutputFinalTime, OutputStateResolvedScattering1D,
+                      OutputDiscreteState, OutputFinal, OutputFinalTime, OutputKineticEnergy, OutputTotalEnergy, OutputStateResolvedScattering1D,
                       OutputSurfaceHops, PopulationCorrelationFunction, SortByOutputReduction,
                       SortByTrajectoryReduction, SumReduction)
 from .simulations import (AdiabaticIESH, Atoms, Classical, DecoherenceCorrectionEDC, Ehrenfest,  # noqa: F401

@@ -29,10 +29,11 @@ DECOHERENCE_NONE, DECOHERENCE_EDC = range(2)
 TERM_NONE, TERM_ABS_GT, TERM_GT, TERM_LT = range(4)
 SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLYMER_THERMAL = range(4)
 (OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION, OUT_FINAL_ADIABATIC_POPULATION,
- OUT_CENTROID_POSITION) = range(6)
+ OUT_CENTROID_POSITION, OUT_KINETIC_ENERGY, OUT_TOTAL_ENERGY) = range(8)
 RED_MEAN, RED_SUM = range(2)
 (FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
- FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID, FIELD_FALLBACKS, FIELD_SAVE_R) = range(13)
+ FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID, FIELD_FALLBACKS, FIELD_SAVE_R, FIELD_SAVE_EKIN,
+ FIELD_SAVE_EPOT) = range(15)
 
 EXPORTED_SYMBOLS = (
     "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
@@ -268,7 +269,8 @@ class Ensemble:
                   FIELD_STATE: ((nt,), np.int32), FIELD_STEPS: ((nt,), np.int32), FIELD_NHOPS: ((nt,), np.int32),
                   FIELD_SAVE_POP_AD: ((nt, nsave, n), np.float64), FIELD_SAVE_POP_DI: ((nt, nsave, n), np.float64),
                   FIELD_SAVE_STATE: ((nt, nsave), np.int32), FIELD_FALLBACKS: ((nt,), np.int32),
-                  FIELD_SAVE_R: ((nt, nsave, D), np.float64)}
+                  FIELD_SAVE_R: ((nt, nsave, D), np.float64), FIELD_SAVE_EKIN: ((nt, nsave), np.float64),
+                  FIELD_SAVE_EPOT: ((nt, nsave), np.float64)}
         shape, dtype = shapes[field]
         out = np.zeros(shape, dtype=dtype)
         self._check(self._L.nqcd_mi355x_download(self._h, field, out.ctypes.data_as(C.c_void_p), out.nbytes))

@@ -228,6 +228,10 @@ def _results(E, t0, dt, save_every, nsteps):
            "state": E.download(L.FIELD_STATE), "steps": E.download(L.FIELD_STEPS),
            "nhops": E.download(L.FIELD_NHOPS), "pop_ad": E.download(L.FIELD_SAVE_POP_AD),
            "pop_di": E.download(L.FIELD_SAVE_POP_DI), "save_state": E.download(L.FIELD_SAVE_STATE)}
+    try:  # energies per save exist for the two-state one-dof problems
+        res["ekin"], res["epot"] = E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
+    except L.NQCDError:
+        pass
     return res
 
 
@@ -333,6 +337,8 @@ def _per_traj_results(E, sim, t0, dt, save_every, nsteps):
                     "sigma": res["sigma"][i], "state": int(res["state"][i]), "steps": int(res["steps"][i]),
                     "nhops": int(res["nhops"][i]), "pop_ad": res["pop_ad"][i], "pop_di": res["pop_di"][i],
                     "save_state": res["save_state"][i], "final_pop_ad": fin[i], "t0": t0, "dt": dt})
+        if "ekin" in res:
+            out[-1].update({"ekin": res["ekin"][i], "epot": res["epot"][i]})
     return out
 
 
@@ -351,7 +357,7 @@ def _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nst
                 elif kind == L.OUT_SCATTERING_1D:
                     v = E.reduce(kind, red, 2 * n)
                     result[output_name(f)] = {"reflection": v[:n], "transmission": v[n:]}
-                elif kind == L.OUT_CENTROID_POSITION:
+                elif kind in (L.OUT_CENTROID_POSITION, L.OUT_KINETIC_ENERGY, L.OUT_TOTAL_ENERGY):
                     result[output_name(f)] = list(E.reduce(kind, red, E.nsave))
                 elif kind == L.OUT_POPULATION_CORRELATION:
                     v = E.reduce(kind, red, n * n * E.nsave).reshape(E.nsave, n, n)

@@ -41,6 +41,10 @@ OutputSurfaceHops = _FnOutput("OutputSurfaceHops", lambda r: r["nhops"])
 OutputFinalTime = _FnOutput("OutputFinalTime", lambda r: r["t0"] + r["steps"] * r["dt"])
 # OutputCentroidPosition (DynamicsOutputs.jl:73-79): ring-polymer centroid per save.
 OutputCentroidPosition = _FnOutput("OutputCentroidPosition", lambda r: list(r["centroid"]), L.OUT_CENTROID_POSITION)
+# OutputKineticEnergy / OutputTotalEnergy (DynamicsOutputs.jl:90-98): per save, two-state one-dof
+# FSSH / Ehrenfest (potential: fssh.jl:150-154, ehrenfest.jl:85-95).
+OutputKineticEnergy = _FnOutput("OutputKineticEnergy", lambda r: list(r["ekin"]), L.OUT_KINETIC_ENERGY)
+OutputTotalEnergy = _FnOutput("OutputTotalEnergy", lambda r: list(r["ekin"] + r["epot"]), L.OUT_TOTAL_ENERGY)
 # OutputFinal (DynamicsOutputs.jl:222): final dynamics variables.
 OutputFinal = _FnOutput("OutputFinal", lambda r: {"v": r["v"], "r": r["r"], "sigma": r["sigma"], "state": r["state"] + 1})
 

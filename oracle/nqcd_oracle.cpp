@@ -605,6 +605,7 @@ struct TrajOut {
     double* r; double* v; double* sig_re; double* sig_im;
     int32_t* state; int64_t* steps; int32_t* nhops;
     double* pop_ad; double* pop_di; int32_t* save_state;
+    double* ekin = nullptr; double* epot = nullptr;  // per save slot, or null
 };
 
 static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, int64_t i, uint64_t gid,
@@ -636,6 +637,18 @@ static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, 
         if (O.pop_ad) adiabatic_population(R.method, n, sig.data(), s, O.pop_ad + ((size_t)i * R.nsave + slot) * n);
         if (O.pop_di) diabatic_population(R.method, n, c.U.data(), sig.data(), s, O.pop_di + ((size_t)i * R.nsave + slot) * n);
         if (O.save_state) O.save_state[(size_t)i * R.nsave + slot] = s;
+        if (O.ekin) {  // classical_kinetic_energy; classical_potential_energy (fssh.jl:150-154, ehrenfest.jl:85-95)
+            double ek = 0.0;
+            for (int I = 0; I < D; ++I) ek += 0.5 * mass[I] * v[I] * v[I];
+            double ep;
+            if (R.method == METHOD_FSSH) ep = c.E[s];
+            else {
+                ep = c.V0;
+                for (int a = 0; a < n; ++a) ep += sig[(size_t)a * n + a].real() * c.E[a];
+            }
+            O.ekin[(size_t)i * R.nsave + slot] = ek;
+            O.epot[(size_t)i * R.nsave + slot] = ep;
+        }
     };
     save(0);
     int64_t step = 0;
@@ -702,6 +715,23 @@ OR_API int32_t oracle_run_ensemble(const ORunDesc* R, const OModelDesc* models, 
         const uint64_t gid = traj_ids ? (uint64_t)traj_ids[i] : (uint64_t)(R->traj_offset + i);
         run_one(*R, models[mid], mass, i, gid, r0 + (size_t)i * D, v0 + (size_t)i * D, sr0 + (size_t)i * nn, si0 + (size_t)i * nn,
                 st0 ? st0[i] : 0, O);
+    });
+    return 0;
+}
+
+// oracle_run_ensemble with the kinetic and classical potential energy of every save slot.
+OR_API int32_t oracle_run_ensemble_energies(const ORunDesc* R, const OModelDesc* models, const double* mass, int64_t ntraj,
+                                            const int32_t* model_id, const double* r0, const double* v0,
+                                            const double* sr0, const double* si0, const int32_t* st0,
+                                            double* r, double* v, double* sr, double* si, int32_t* state, int64_t* steps,
+                                            int32_t* nhops, double* pop_ad, double* pop_di, int32_t* save_state,
+                                            double* ekin, double* epot, int32_t nthreads) {
+    const int D = R->ndof, nn = R->nstates * R->nstates;
+    TrajOut O{r, v, sr, si, state, steps, nhops, pop_ad, pop_di, save_state, ekin, epot};
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        const int mid = model_id ? model_id[i] : 0;
+        run_one(*R, models[mid], mass, i, (uint64_t)(R->traj_offset + i), r0 + (size_t)i * D, v0 + (size_t)i * D,
+                sr0 + (size_t)i * nn, si0 + (size_t)i * nn, st0 ? st0[i] : 0, O);
     });
     return 0;
 }

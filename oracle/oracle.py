@@ -180,8 +180,9 @@ def diabatic_to_adiabatic(m, r0, diabatic_state):
 
 def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None, dt=1.0, nsteps=100, term_x=0.0,
                  save_every=0, nsave=1, key=0, traj_offset=0, rescaling=RESCALE_STANDARD, nthreads=None, t0=0.0, nsub=5,
-                 traj_ids=None, term_dof=0, term_kind=0):
-    """Run an ensemble on the CPU; returns a dict of per-trajectory results."""
+                 traj_ids=None, term_dof=0, term_kind=0, energies=False):
+    """Run an ensemble on the CPU; returns a dict of per-trajectory results (with energies=True also
+    the kinetic and potential energy of every save slot, ekin / epot (ntraj, nsave))."""
     if not isinstance(models, (list, tuple)):
         models = [models]
     ntraj, D = r0.shape
@@ -204,6 +205,18 @@ def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None,
     if nthreads is None:
         nthreads = min(16, os.cpu_count() or 1)
     tid = None if traj_ids is None else np.ascontiguousarray(traj_ids, dtype=np.int64)
+    if energies:
+        if traj_ids is not None:
+            raise ValueError("energies with traj_ids are not provided")
+        out["ekin"], out["epot"] = np.zeros((ntraj, nsave)), np.zeros((ntraj, nsave))
+        f = lib().oracle_run_ensemble_energies
+        f.restype = C.c_int32
+        f(C.byref(R), marr, _p(mass), C.c_int64(ntraj), _p(mid, C.c_int32),
+          _p(np.ascontiguousarray(r0, dtype=np.float64)), _p(np.ascontiguousarray(v0, dtype=np.float64)), _p(sr0), _p(si0),
+          _p(st0, C.c_int32), _p(out["r"]), _p(out["v"]), _p(out["sig_re"]), _p(out["sig_im"]), _p(out["state"], C.c_int32),
+          _p(out["steps"], C.c_int64), _p(out["nhops"], C.c_int32), _p(out["pop_ad"]), _p(out["pop_di"]),
+          _p(out["save_state"], C.c_int32), _p(out["ekin"]), _p(out["epot"]), C.c_int32(nthreads))
+        return out
     f = lib().oracle_run_ensemble
     f.restype = C.c_int32
     f(C.byref(R), marr, _p(mass), C.c_int64(ntraj), _p(mid, C.c_int32),

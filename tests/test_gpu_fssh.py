@@ -239,3 +239,29 @@ def test_termination_predicates_parity(oracle, key, schedule, predicate, value):
     assert (g["steps"] < 3000).sum() > 20  # the predicate ended a good part of the ensemble
     _assert_window_parity(g, c)
     E.close()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_energy_outputs_parity(oracle, key, method, schedule):
+    """OutputKineticEnergy / OutputTotalEnergy (DynamicsOutputs.jl:90-98) per save slot from the device
+    (persistent and step-per-launch kernels) against the oracle: per trajectory within 1e-9, and the
+    device Mean reduction of the total against the oracle's mean."""
+    L, E = _ens()
+    ntraj = 128
+    r0, v0, sig0, st0 = _window_inputs(oracle, key, ntraj, 0)
+    if method == 0:
+        sig0 = np.zeros((ntraj, 2, 2), complex); sig0[np.arange(ntraj), st0, st0] = 1
+    g = _run_gpu(L, E, method, 0, r0, v0, sig0, st0, 400, key, schedule=schedule, save_every=40)
+    ekin, epot = E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
+    tot_mean = E.reduce(L.OUT_TOTAL_ENERGY, L.RED_MEAN, 11)
+    kin_sum = E.reduce(L.OUT_KINETIC_ENERGY, L.RED_SUM, 11)
+    E.close()
+    m = oracle.make_model(0, 2, 1, TULLY[0])
+    c = oracle.run_ensemble(method, m, MASS, r0, v0, sig0, state0=st0, dt=1.0, nsteps=400, key=key, save_every=40,
+                            nsave=11, energies=True)
+    assert np.array_equal(g["state"], c["state"])
+    assert np.allclose(ekin, c["ekin"], rtol=1e-9, atol=1e-14)
+    assert np.allclose(epot, c["epot"], rtol=1e-9, atol=1e-14)
+    assert np.allclose(tot_mean, (c["ekin"] + c["epot"]).mean(axis=0), rtol=1e-9, atol=1e-14)
+    assert np.allclose(kin_sum, c["ekin"].sum(axis=0), rtol=1e-9)

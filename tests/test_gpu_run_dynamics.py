@@ -29,7 +29,7 @@ def _case(oracle, key, method_cls, kind, outputs, reduction, seed=3):
     sig0 = np.zeros((NTRAJ, 2, 2), complex); sig0[:, 0, 0] = 1.0
     method = 0 if method_cls is N.FSSH else 1
     c = oracle.run_ensemble(method, m, MASS, r0, v0, sig0, state0=np.zeros(NTRAJ, np.int32), dt=1.0, nsteps=NSTEPS,
-                            key=key, save_every=EVERY, nsave=NSTEPS // EVERY + 1, term_x=10.0)
+                            key=key, save_every=EVERY, nsave=NSTEPS // EVERY + 1, term_x=10.0, energies=True)
     return sim, res, c
 
 
@@ -95,6 +95,23 @@ def test_ehrenfest_populations_mean_reduction(oracle, key):
     assert alive[-1] < 1.0
     assert np.allclose(pa.sum(axis=1), alive, atol=1e-12)
     assert np.any((pa[5] > 1e-3) & (pa[5] < 1 - 1e-3))
+
+
+@pytest.mark.parametrize("method_cls,kind", [(N.FSSH, 0), (N.Ehrenfest, 1)])
+def test_energy_outputs_mean_and_per_trajectory(oracle, key, method_cls, kind):
+    """OutputKineticEnergy and OutputTotalEnergy (DynamicsOutputs.jl:90-98) at every save: the device
+    Mean reduction and the per-trajectory lists against the oracle's saved energies."""
+    outs = (N.OutputKineticEnergy, N.OutputTotalEnergy)
+    _, res, c = _case(oracle, key, method_cls, kind, outs, N.MeanReduction(), seed=6)
+    assert np.allclose(np.array(res["OutputKineticEnergy"]), c["ekin"].mean(axis=0), rtol=1e-9, atol=1e-14)
+    assert np.allclose(np.array(res["OutputTotalEnergy"]), (c["ekin"] + c["epot"]).mean(axis=0), rtol=1e-9,
+                       atol=1e-14)
+    _, per, _ = _case(oracle, key, method_cls, kind, outs, N.SortByTrajectoryReduction(), seed=6)
+    for i in (0, NTRAJ // 3, NTRAJ - 1):
+        n = c["steps"][i] // EVERY + 1  # save slots the trajectory reached before it terminated
+        assert np.allclose(np.array(per[i]["OutputKineticEnergy"])[:n], c["ekin"][i, :n], rtol=1e-9, atol=1e-14)
+        assert np.allclose(np.array(per[i]["OutputTotalEnergy"])[:n], (c["ekin"] + c["epot"])[i, :n], rtol=1e-9,
+                           atol=1e-14)
 
 
 def test_single_trajectory_returns_its_dictionary(oracle, key):

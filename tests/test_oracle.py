@@ -328,3 +328,31 @@ def test_spin_boson_ehrenfest_gao_saller(oracle, key):
     ref[lo] = yg[0] + (times[lo] - tg[0]) * (yg[1] - yg[0]) / (tg[1] - tg[0])
     ref[hi] = yg[-1] + (times[hi] - tg[-1]) * (yg[-1] - yg[-2]) / (tg[-1] - tg[-2])
     assert np.all(np.abs(corr - ref) < 0.2), np.abs(corr - ref).max()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_saved_energies(oracle, key, method):
+    """OutputKineticEnergy / OutputTotalEnergy (DynamicsOutputs.jl:90-98): the first and last save slots
+    equal m v^2 / 2 and classical_potential_energy computed independently from the initial and final
+    positions (numpy eigenvalues; FSSH: the active state's, fssh.jl:150-154; Ehrenfest: sum_i
+    sigma_ii E_i, ehrenfest.jl:85-95), and the total is conserved over the slots within the reference's
+    rtol of fssh.jl:131 (dt = 1 through the crossing; a hop's half-kick reuses the old force)."""
+    m = oracle.make_model(0, 2, 1, TULLY1)
+    ntraj = 64
+    r0, v0 = oracle.sample_gaussian(ntraj, 1, -4.0, 0.5, 15 / 2000, 0.0, key)
+    sig0 = np.zeros((ntraj, 2, 2), complex); sig0[:, 0, 0] = 1
+    out = oracle.run_ensemble(method, m, 2000.0, r0, v0, sig0, state0=np.zeros(ntraj, np.int32), dt=1.0, nsteps=600,
+                              key=key, save_every=50, nsave=13, energies=True)
+    for i in range(ntraj):
+        E0 = np.linalg.eigvalsh(oracle.model_eval(m, r0[i])[0])
+        E1 = np.linalg.eigvalsh(oracle.model_eval(m, out["r"][i])[0])
+        assert np.isclose(out["ekin"][i, 0], 0.5 * 2000 * v0[i, 0] ** 2, rtol=1e-14)
+        assert np.isclose(out["epot"][i, 0], E0[0], rtol=1e-12, atol=1e-15)
+        assert np.isclose(out["ekin"][i, -1], 0.5 * 2000 * out["v"][i, 0] ** 2, rtol=1e-14)
+        p1 = E1[out["state"][i]] if method == 0 else out["sig_re"][i, 0, 0] * E1[0] + out["sig_re"][i, 1, 1] * E1[1]
+        assert np.isclose(out["epot"][i, -1], p1, rtol=1e-12, atol=1e-15)
+    tot = out["ekin"] + out["epot"]
+    rel = np.abs(tot - tot[:, :1]) / np.abs(tot[:, :1])
+    assert rel.max() < 1e-2
+    if method == 0:
+        assert out["nhops"].sum() > 0
