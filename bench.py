@@ -236,6 +236,10 @@ def main():
 
     # --- batched: persistent work-queue kernel, FP32 nuclei / FP64 electronics -------------------
     E = _ensemble(L, L.PREC_FP32_NUCLEI, L.SCHED_PERSISTENT)
+    # launch shape of the persistent kernel chosen by the library's autotuner on 200 steps (untimed)
+    tuning = E.autotune(0.0, 200.0, KEY)
+    _progress(f"autotune: block {tuning['block']}, workgroups/CU {tuning['workgroups_per_cu'] or 'max'} "
+              f"({tuning['best_ms']:.2f} ms vs default {tuning['default_ms']:.2f} ms on the sample)")
     for _ in range(args.warmup):
         st = E.run(0.0, TMAX, KEY, term=(0, XTERM))
         _progress(f"warmup pass: {st['kernel_ms']:.1f} ms, {st['traj_steps']} trajectory-steps")
@@ -303,6 +307,7 @@ def main():
                      "kernel": "fssh2_persistent<1,float>", "kernel_avg_ms": avg_kernel_ms,
                      "flop_per_traj_step": FLOP_PER_STEP,
                      "hbm_algorithmic_GBs": algo_bytes / (avg_kernel_ms / 1e3) / 1e9},
+        "autotune": tuning,
         "pop_error_vs_fp64": pop_err,
         "pop_sigma": sigma,
         "fp64_persistent": {"value": fp64_rate, "kernel_ms": st64["kernel_ms"]},

@@ -282,6 +282,27 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
  * Replaces the @timed wall clock around solve (run_dynamics.jl:91). */
 int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out);
 
+typedef struct {
+    int32_t block;               /* chosen threads per workgroup */
+    int32_t workgroups_per_cu;   /* chosen resident workgroups per compute unit (0 = occupancy limit) */
+    int32_t n_candidates;        /* launch shapes timed */
+    int32_t family;              /* 1 two-state one-dof kernel, 2 diatom-surface kernel */
+    double best_ms;              /* kernel time of the chosen shape on the sample run */
+    double default_ms;           /* kernel time of the default shape (256 threads, occupancy limit) */
+} nqcd_tune_result;
+
+/* Autotuner of the persistent work-queue kernels (no counterpart in the reference; the natural
+ * caller is the one-step precompile_dynamics pre-run of run_dynamics.jl:100-120).  Runs the
+ * loaded problem over [t0, t1] (no saves, no termination) once per candidate launch shape --
+ * 256, 128 and 64 threads per workgroup, each at the occupancy limit and at 3/4 and 1/2 of it --
+ * keeps the fastest (the default unless a candidate is faster by more than 2%) in a process-wide
+ * table keyed by kernel family, state count, precision and compute-unit count, and later runs
+ * of that key use it.  With NQCD_MI355X_TUNE_CACHE set the table persists in that file.  Results
+ * of a run do not depend on the shape.  The sample runs overwrite the handle's results; call
+ * nqcd_mi355x_run afterwards.  Two-state one-dof (persistent schedule) and analytic
+ * diatom-surface problems only; NQCD_ERR_UNSUPPORTED otherwise. */
+int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, uint64_t philox_key, nqcd_tune_result* out);
+
 /* ---- MLP surrogate of the diabatic Hamiltonian (no counterpart in the reference) -------------
  * A 6 -> 128 -> 128 -> 3 tanh network for (H11, H22, H12) of a six-coordinate two-state model,
  * two ensemble members, trained on the GPU with BF16 MFMA forward / backward kernels and Adam. */

@@ -23,6 +23,7 @@
 #include "iesh.h"
 #include "fssh_nd.h"
 #include "mlp_host.h"
+#include "autotune.h"
 
 using namespace nqcd;
 
@@ -469,13 +470,11 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
         if (int32_t rc = alloc_results(h)) return rc;
         Fssh2Args A = make_args(h, t0, nsteps, term, key);
         if (h->prob.schedule == NQCD_SCHED_PERSISTENT) {
-            const int block = 256;
+            const TuneShape sh = launch_shape(h, TUNE_FSSH2);
+            const int block = sh.block;
             int bpc = 0;
             HIPCHK(h, fssh2_persistent_occupancy(h->prob.precision, block, &bpc));
-            if (bpc < 1) bpc = 1;
-            int grid = bpc * h->cu_count;
-            const int64_t need = (h->ntraj + block - 1) / block;
-            if ((int64_t)grid > need) grid = (int)need;
+            const int grid = (int)persistent_grid(bpc, sh, h->cu_count, h->ntraj);
             HIPCHK(h, hipEventRecord(h->ev0, h->stream));
             HIPCHK(h, h->args_dev.ensure(sizeof(Fssh2Args)));
             HIPCHK(h, fssh2_launch_persistent(A, h->args_dev.as<Fssh2Args>(), h->prob.precision, grid, block, h->stream));
@@ -732,6 +731,63 @@ int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out) {
     if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
     if (!h->have_results) return fail(h, NQCD_ERR_STATE, "no run yet");
     *out = h->stats;
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, uint64_t philox_key, nqcd_tune_result* out) {
+    if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
+    if (!h->have_problem || !h->have_initial) return fail(h, NQCD_ERR_STATE, "autotune needs a problem and initial conditions");
+    int32_t family = 0;
+    if (is_rp(h) || is_ie(h) || is_spin_boson(h)) family = 0;
+    else if (nd_is_problem(h)) family = h->models[0].kind == NQCD_MODEL_DIATOM_SURFACE ? TUNE_ND : 0;
+    else if (is_two_state_1d(h) && h->prob.schedule == NQCD_SCHED_PERSISTENT) family = TUNE_FSSH2;
+    if (!family)
+        return fail(h, NQCD_ERR_UNSUPPORTED,
+                    "autotune covers the persistent two-state one-dof and diatom-surface (analytic model) kernels");
+    HIPCHK(h, hipSetDevice(h->device));
+    std::vector<TuneShape> cand;
+    for (int block : {256, 128, 64}) {
+        int occ = 0;
+        HIPCHK(h, family == TUNE_FSSH2 ? fssh2_persistent_occupancy(h->prob.precision, block, &occ)
+                                        : nd_persistent_occupancy(h->prob.precision, block, &occ));
+        if (occ < 1) continue;
+        for (int w : {occ, occ * 3 / 4, occ / 2}) {
+            if (w < 1) continue;
+            const TuneShape s{block, w == occ ? 0 : w};
+            bool dup = false;
+            for (const TuneShape& c : cand) dup = dup || (c.block == s.block && c.wpc == s.wpc);
+            if (!dup) cand.push_back(s);
+        }
+    }
+    if (cand.empty() || cand[0].block != 256 || cand[0].wpc != 0)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "the persistent kernel does not fit on a compute unit");
+    double best = 0.0, def_ms = 0.0;
+    TuneShape best_s = cand[0];
+    for (size_t i = 0; i < cand.size(); ++i) {
+        h->force_block = cand[i].block;
+        h->force_wpc = cand[i].wpc;
+        double ms = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {
+            const int32_t rc = nqcd_mi355x_run(h, t0, t1, 0, nullptr, philox_key);
+            if (rc != NQCD_OK) {
+                h->force_block = h->force_wpc = 0;
+                return rc;
+            }
+            ms = rep == 0 ? h->stats.kernel_ms : std::min(ms, h->stats.kernel_ms);
+        }
+        if (i == 0) def_ms = best = ms;
+        else if (ms < best) { best = ms; best_s = cand[i]; }
+    }
+    h->force_block = h->force_wpc = 0;
+    if (best > 0.98 * def_ms) { best_s = cand[0]; best = def_ms; }
+    tune_store(family, h->prob.n_states, h->prob.precision, h->cu_count, best_s);
+    h->have_results = false;
+    out->block = best_s.block;
+    out->workgroups_per_cu = best_s.wpc;
+    out->n_candidates = (int32_t)cand.size();
+    out->family = family;
+    out->best_ms = best;
+    out->default_ms = def_ms;
     return NQCD_OK;
 }
 

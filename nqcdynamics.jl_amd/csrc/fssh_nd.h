@@ -10,6 +10,8 @@ constexpr int kDiatomParams = 16;
 
 int32_t nd_set_model(nqcd_handle_t* h, int32_t kind, const double* params, int64_t n);
 bool nd_is_problem(const nqcd_handle_t* h);
+// Resident workgroups per compute unit of the analytic-model persistent kernel at `block` threads.
+hipError_t nd_persistent_occupancy(int precision, int block, int* blocks_per_cu);
 int32_t nd_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc* term, uint64_t key);
 
 }  // namespace nqcd

@@ -17,6 +17,7 @@
 #include "fssh2.h"
 #include "fssh_nd.h"
 #include "mlp.h"
+#include "autotune.h"
 
 namespace nqcd {
 
@@ -420,6 +421,11 @@ __global__ void nd_queue_reset(unsigned long long* q) { *q = 0ull; }
 
 }  // namespace
 
+hipError_t nd_persistent_occupancy(int precision, int block, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, precision == NQCD_PREC_FP64 ? (const void*)fssh_nd_f64 : (const void*)fssh_nd_f32, block, 0);
+}
+
 bool nd_is_problem(const nqcd_handle_t* h) {
     return (h->prob.method == NQCD_METHOD_FSSH || h->prob.method == NQCD_METHOD_EHRENFEST) &&
            h->prob.algorithm == NQCD_ALG_BAB_TSIT5 && h->prob.n_states == 2 && h->model_set[0] &&
@@ -501,15 +507,16 @@ int32_t nd_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     }
     const bool f64 = h->prob.precision == NQCD_PREC_FP64;
     const void* kern = mlp ? (const void*)fssh_nd_mlp_f32 : (f64 ? (const void*)fssh_nd_f64 : (const void*)fssh_nd_f32);
-    const int block = mlp ? kMlpBlock : 256;
+    // The surrogate kernel's workgroup is fixed (its eight waves share the packed members in LDS);
+    // the analytic-model kernels take the tuned shape.
+    TuneShape sh = launch_shape(h, TUNE_ND);
+    if (mlp) sh = TuneShape{kMlpBlock, 0};
+    const int block = sh.block;
     const size_t lds = mlp ? 2 * (size_t)kPkBytes : 0;
     if (mlp) HIPCHK(h, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int bpc = 0;
     HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, block, lds));
-    if (bpc < 1) bpc = 1;
-    int64_t grid = (int64_t)bpc * h->cu_count;
-    const int64_t need = (n + block - 1) / block;
-    if (grid > need) grid = need;
+    const int64_t grid = persistent_grid(bpc, sh, h->cu_count, n);
     HIPCHK(h, h->args_dev.ensure(sizeof(FsshNdArgs)));
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->args_dev.p, &A, sizeof(FsshNdArgs), hipMemcpyHostToDevice, h->stream));

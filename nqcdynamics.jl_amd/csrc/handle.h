@@ -79,4 +79,6 @@ struct nqcd_handle {
     bool state0_given = false, model_id_given = false, order_given = false;
     nqcd_run_stats stats{};
     int cu_count = 0;
+    // Launch shape forced on the persistent kernels while the autotuner times candidates (0 = none).
+    int32_t force_block = 0, force_wpc = 0;
 };

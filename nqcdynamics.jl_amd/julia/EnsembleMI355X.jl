@@ -52,9 +52,11 @@ struct EnsembleMI355X <: SciMLBase.EnsembleAlgorithm
     fp32_nuclei::Bool       # FP32 nuclei / FP64 electronics (false: FP64 throughout)
     seed::UInt64            # Philox key of the hop and sampling streams
     device_model::Any       # nothing, or (kind::Integer, params::Vector{Float64})
+    autotune::Int           # > 0: time the persistent kernel's launch shapes on this many steps first
 end
-EnsembleMI355X(; device=0, persistent=true, fp32_nuclei=false, seed=0x4E5143444D493335, device_model=nothing) =
-    EnsembleMI355X(device, persistent, fp32_nuclei, seed, device_model)
+EnsembleMI355X(; device=0, persistent=true, fp32_nuclei=false, seed=0x4E5143444D493335, device_model=nothing,
+               autotune=0) =
+    EnsembleMI355X(device, persistent, fp32_nuclei, seed, device_model, autotune)
 
 """
     DeviceTermination(predicate, dof, value)
@@ -79,6 +81,10 @@ struct ProblemDesc
 end
 struct TermDesc
     kind::Int32; dof::Int32; value::Float64
+end
+
+struct TuneResult
+    block::Int32; workgroups_per_cu::Int32; n_candidates::Int32; family::Int32; best_ms::Float64; default_ms::Float64
 end
 
 check(h, rc) = rc == 0 || error("nqcd_mi355x: ", unsafe_string(ccall((:nqcd_mi355x_last_error, libnqcd), Cstring, (Ptr{Cvoid},), h)))
@@ -257,6 +263,15 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         kinds = Dict(:abs_gt => Int32(1), :gt => Int32(2), :lt => Int32(3))
         term = callback === nothing ? C_NULL :
             Ref(TermDesc(kinds[callback.predicate], Int32(callback.dof - 1), callback.value))
+        # launch shape of the persistent kernel (a no-op for the problems it does not cover; the
+        # choice stays in the library's table for later runs of the same state count and precision)
+        if ensemblealg.autotune > 0 && ensemblealg.persistent
+            tr = Ref{TuneResult}()
+            t1 = tspan[1] + dt * min(ensemblealg.autotune, round(Int, (tspan[2] - tspan[1]) / dt))
+            rc = @threadcall((:nqcd_mi355x_autotune, libnqcd), Int32, (Ptr{Cvoid}, Float64, Float64, UInt64, Ptr{TuneResult}),
+                h[], tspan[1], t1, ensemblealg.seed, tr)
+            rc == -2 || check(h[], rc)   # NQCD_ERR_UNSUPPORTED: keep the default shape
+        end
         # long device call: do not hold the Julia GC
         rc = @threadcall((:nqcd_mi355x_run, libnqcd), Int32, (Ptr{Cvoid}, Float64, Float64, Int32, Ptr{TermDesc}, UInt64),
             h[], tspan[1], tspan[2], save_every, term, ensemblealg.seed)

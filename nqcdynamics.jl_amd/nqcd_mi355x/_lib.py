@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "nqcd_mi355x_reduce", "nqcd_mi355x_download", "nqcd_mi355x_last_run_stats", "nqcd_mi355x_last_error",
     "nqcd_mi355x_surrogate_train", "nqcd_mi355x_surrogate_eval", "nqcd_mi355x_surrogate_gradient",
     "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params", "nqcd_mi355x_test_determinant",
-    "nqcd_mi355x_sample_fermi_dirac",
+    "nqcd_mi355x_sample_fermi_dirac", "nqcd_mi355x_autotune",
 )
 
 
@@ -71,6 +71,11 @@ class TrainDesc(C.Structure):
 class RunStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("traj_steps", C.c_int64), ("launches", C.c_int64),
                 ("block", C.c_int32), ("grid", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("block", C.c_int32), ("workgroups_per_cu", C.c_int32), ("n_candidates", C.c_int32),
+                ("family", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
 class NQCDError(RuntimeError):
@@ -107,6 +112,7 @@ def lib():
     L.nqcd_mi355x_reduce.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_download.argtypes = [H, C.c_int32, C.c_void_p, C.c_int64]
     L.nqcd_mi355x_last_run_stats.argtypes = [H, C.POINTER(RunStats)]
+    L.nqcd_mi355x_autotune.argtypes = [H, C.c_double, C.c_double, C.c_uint64, C.POINTER(TuneResult)]
     fp = C.POINTER(C.c_float)
     L.nqcd_mi355x_surrogate_train.argtypes = [H, dp, dp, C.c_int64, C.POINTER(TrainDesc), dp]
     L.nqcd_mi355x_surrogate_eval.argtypes = [H, C.c_int32, dp, C.c_int64, dp, dp]
@@ -248,6 +254,13 @@ class Ensemble:
         nsteps = int(round((t1 - t0) / self.dt))
         self.nsave = nsteps // save_every + 1 if save_every > 0 else 1
         return self.stats()
+
+    def autotune(self, t0, t1, key):
+        """Time the persistent kernel's candidate launch shapes on [t0, t1] of the loaded problem and keep
+        the fastest for later runs (nqcd_mi355x_autotune); the handle's results are overwritten."""
+        r = TuneResult()
+        self._check(self._L.nqcd_mi355x_autotune(self._h, t0, t1, C.c_uint64(key), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
 
     def stats(self):
         s = RunStats()

@@ -22,12 +22,17 @@ class EnsembleMI355X:
     """Ensemble algorithm running every trajectory on one MI355X (device index `device`).
 
     schedule: "persistent" (work-queue kernel, default) or "step_per_launch" (plain port).
+    autotune: time the persistent kernel's launch shapes on the first `autotune_steps` steps of the
+    ensemble before the run (FSSH / Ehrenfest on the two-state one-dof and diatom-surface models; a
+    no-op elsewhere); the choice is kept for later runs of the same state count and precision.
     """
 
-    def __init__(self, device=0, schedule="persistent", seed=0x4E5143444D493335):
+    def __init__(self, device=0, schedule="persistent", seed=0x4E5143444D493335, autotune=False, autotune_steps=200):
         if schedule not in ("persistent", "step_per_launch"):
             raise ValueError("schedule must be 'persistent' or 'step_per_launch'")
         self.device, self.schedule, self.seed = device, schedule, seed
+        self.autotune, self.autotune_steps = bool(autotune), int(autotune_steps)
+        self.tuning = None
 
 
 class TerminatingCallback:
@@ -278,6 +283,13 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
     try:
         _problem(E, sim, dt)
         _initial_conditions(E, sim, distribution, int(trajectories), selection, ensemble_algorithm.seed)
+        if getattr(ensemble_algorithm, "autotune", False) and kind == "sh":
+            try:
+                ensemble_algorithm.tuning = E.autotune(t0, t0 + dt * min(nsteps, ensemble_algorithm.autotune_steps),
+                                                       ensemble_algorithm.seed)
+            except L.NQCDError as e:
+                if e.code != L.ERR_UNSUPPORTED:
+                    raise
         stats = E.run(t0, t1, ensemble_algorithm.seed, save_every=save_every, term=term)
         nsave = E.nsave
         times = t0 + dt * save_every * np.arange(nsave) if save_every else np.array([t0, t1])

@@ -34,3 +34,13 @@ def test_library_is_gfx950():
     from nqcd_mi355x import _lib
     data = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
+
+
+def test_autotune_entry_point_rejects_null_handle():
+    """nqcd_mi355x_autotune checks its arguments before any device work; its result struct matches
+    the header's layout (four int32 then two doubles)."""
+    from nqcd_mi355x import _lib
+    L = _lib.lib()
+    r = _lib.TuneResult()
+    assert ctypes.sizeof(_lib.TuneResult) == 32
+    assert L.nqcd_mi355x_autotune(None, 0.0, 1.0, ctypes.c_uint64(1), ctypes.byref(r)) == _lib.ERR_ARG

@@ -75,3 +75,23 @@ def test_diatom_fp32_population_error(key):
     sigma = np.sqrt(2 * p64 * (1 - p64) / len(r0))
     assert abs(p64 - p32) < 0.01 and abs(p64 - p32) < 3 * sigma + 1e-12, (p64, p32, sigma)
     assert 0.05 < p64 < 0.95
+
+
+def test_diatom_autotune_keeps_trajectories(key):
+    """The autotuner on the diatom-surface kernel (family 2): the tuned run reproduces the default
+    run's trajectories bit for bit."""
+    from nqcd_mi355x import _lib as L
+    r0, v0 = _inputs(2000, 33)
+    ref = _gpu(L, 0, r0, v0, 400, key, precision=1)
+    E = L.Ensemble(0)
+    E.set_problem(0, 2, 3, 2, [MASS, MASS], 2.0, precision=1)
+    E.set_model(0, L.MODEL_DIATOM_SURFACE, BASE)
+    sig = np.zeros((len(r0), 2, 2), complex); sig[:, 0, 0] = 1
+    E.load_initial(r0, v0, sig, state=np.zeros(len(r0), np.int32))
+    res = E.autotune(0.0, 100.0, key)
+    E.close()
+    assert res["family"] == 2 and res["n_candidates"] >= 6 and res["block"] in (64, 128, 256)
+    tuned = _gpu(L, 0, r0, v0, 400, key, precision=1)
+    assert tuned["stats"]["block"] == res["block"]
+    for k in ("r", "v", "state", "steps", "nhops", "save_state"):
+        assert np.array_equal(tuned[k], ref[k]), k

@@ -15,7 +15,7 @@ MASS = 2000.0
 NTRAJ, NSTEPS, EVERY = 97, 3000, 100
 
 
-def _case(oracle, key, method_cls, kind, outputs, reduction, seed=3):
+def _case(oracle, key, method_cls, kind, outputs, reduction, seed=3, ens=None):
     rng = np.random.default_rng(seed)
     p0 = rng.uniform(8.0, 30.0, size=NTRAJ)
     r0 = np.full((NTRAJ, 1), -8.0)
@@ -24,7 +24,8 @@ def _case(oracle, key, method_cls, kind, outputs, reduction, seed=3):
     dist = N.DynamicalDistribution([np.array([[v]]) for v in v0[:, 0]], [np.array([[-8.0]])] * NTRAJ, (1, 1))
     res = N.run_dynamics(sim, (0.0, float(NSTEPS)), dist * N.PureState(1, N.Adiabatic()), trajectories=NTRAJ,
                          dt=1.0, saveat=float(EVERY), output=outputs, reduction=reduction,
-                         callback=N.TerminatingCallback(1, 10.0), ensemble_algorithm=N.EnsembleMI355X(seed=key))
+                         callback=N.TerminatingCallback(1, 10.0),
+                         ensemble_algorithm=ens or N.EnsembleMI355X(seed=key))
     m = oracle.make_model(kind, 2, 1, TULLY[kind])
     sig0 = np.zeros((NTRAJ, 2, 2), complex); sig0[:, 0, 0] = 1.0
     method = 0 if method_cls is N.FSSH else 1
@@ -112,6 +113,17 @@ def test_energy_outputs_mean_and_per_trajectory(oracle, key, method_cls, kind):
         assert np.allclose(np.array(per[i]["OutputKineticEnergy"])[:n], c["ekin"][i, :n], rtol=1e-9, atol=1e-14)
         assert np.allclose(np.array(per[i]["OutputTotalEnergy"])[:n], (c["ekin"] + c["epot"])[i, :n], rtol=1e-9,
                            atol=1e-14)
+
+
+def test_autotuned_run_dynamics_matches_oracle(oracle, key):
+    """EnsembleMI355X(autotune=True) tunes the persistent kernel's launch shape on the first steps and
+    then runs; the outputs are those of the oracle as for the default shape."""
+    ens = N.EnsembleMI355X(seed=key, autotune=True, autotune_steps=100)
+    outs = (N.OutputAdiabaticPopulation, N.OutputSurfaceHops)
+    _, res, c = _case(oracle, key, N.FSSH, 1, outs, N.MeanReduction(), seed=7, ens=ens)
+    assert ens.tuning is not None and ens.tuning["family"] == 1 and ens.tuning["n_candidates"] >= 6
+    assert np.allclose(np.array(res["OutputAdiabaticPopulation"]), c["pop_ad"].mean(axis=0), rtol=1e-9, atol=1e-12)
+    assert np.isclose(res["OutputSurfaceHops"], c["nhops"].mean(), rtol=1e-12)
 
 
 def test_single_trajectory_returns_its_dictionary(oracle, key):
