@@ -54,6 +54,7 @@ bool tune_lookup(int32_t family, int32_t n_states, int32_t precision, int32_t cu
     std::lock_guard<std::mutex> lock(g_mu);
     load_locked();
     auto it = g_table.find(Key{family, n_states, precision, cu_count});
+    if (it == g_table.end()) it = g_table.find(Key{family, n_states, precision, 0});  // any device size
     if (it == g_table.end()) return false;
     *out = it->second;
     return true;

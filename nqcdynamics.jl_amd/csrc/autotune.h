@@ -30,7 +30,8 @@ TuneShape launch_shape(const nqcd_handle* h, int32_t family);
 // Process-wide table, guarded by a mutex.  When NQCD_MI355X_TUNE_CACHE names a file, the
 // table is read from it on first use (and again when the variable names another file) and
 // every new entry is appended to it (one line per entry: family n_states precision cu_count
-// block wpc).
+// block wpc).  An entry with cu_count 0 applies to devices of any size that have no entry of
+// their own.
 bool tune_lookup(int32_t family, int32_t n_states, int32_t precision, int32_t cu_count, TuneShape* out);
 void tune_store(int32_t family, int32_t n_states, int32_t precision, int32_t cu_count, TuneShape s);
 
