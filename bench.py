@@ -123,6 +123,61 @@ def iesh_extra(L):
     return res
 
 
+def spin_boson_extra(L):
+    """Spin-boson Ehrenfest and FSSH: eps = Delta = 1, 100 modes of a Debye bath (eta = 0.25, wc = 5),
+    Wigner-sampled at beta = 5, 1e5 trajectories to t = 20 / Delta with the population-correlation
+    output every 50 steps; FP32 bath, FP64 electronics.  The step is 0.002 / Delta: the highest
+    discretised mode (wc tan(pi/2 * 100/101) = 321) puts velocity Verlet beyond its stability limit
+    at 0.01."""
+    nm, wc, lam, beta = 100, 5.0, 0.625, 5.0
+    j = np.arange(1, nm + 1)
+    w = wc * np.tan(np.pi / 2 * (1 - j / (nm + 1)))
+    c = np.sqrt(2 * lam / (nm + 1)) * w
+    t = np.tanh(beta * w / 2)
+    prm = np.concatenate([np.zeros(nm), np.sqrt(1 / (2 * w * t)), np.zeros(nm), np.sqrt(w / (2 * t))])
+    ntraj, dt, nsteps = 100000, 0.002, 10000
+    res = {}
+    for name, method in (("ehrenfest", L.METHOD_EHRENFEST), ("fssh", L.METHOD_FSSH)):
+        E = L.Ensemble(0)
+        E.set_problem(method, 2, 1, nm, np.ones(nm), dt, precision=L.PREC_FP32_NUCLEI)
+        E.set_model(0, L.MODEL_SPIN_BOSON, np.concatenate([[1.0, 1.0], w, c]))
+        E.sample_initial(L.SAMPLER_HARMONIC_WIGNER, ntraj, prm, KEY, initial_state=0, state_basis=1)
+        st = E.run(0.0, nsteps * dt, KEY, save_every=50)
+        E.reduce(L.OUT_POPULATION_CORRELATION, L.RED_MEAN, 4 * E.nsave)
+        E.close()
+        rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+        res[name] = {"ntraj": ntraj, "nsteps": nsteps, "dt": dt, "kernel_ms": st["kernel_ms"],
+                     "traj_steps_per_s": rate, "grid": st["grid"], "block": st["block"]}
+        _progress(f"spin-boson {name}: {rate:.3e} trajectory-steps/s")
+    return res
+
+
+def ring_polymer_extra(L):
+    """RPMD (one state) and ring-polymer surface hopping (two states) on the double well
+    V = x^4/1200 - x^2/40 with a Gaussian-coupled second diabat, beta = 8, thermal ring polymers
+    sampled on the device, 1e5 ring polymers for 1000 steps of dt = 1 at 8, 16, 32 and 64 beads;
+    FP32 beads, FP64 electronics.  Rates are ring-polymer steps/s (a ring polymer = one trajectory)."""
+    mass, beta, ntraj, nsteps = 1836.0, 8.0, 100000, 1000
+    res = {}
+    for name, method, nstates, prm in (("rpmd", L.METHOD_RPMD, 1, [1 / 1200, 1 / 40, 0.0, 0.0, 1.0]),
+                                       ("rpsh", L.METHOD_RPSH, 2, [1 / 1200, 1 / 40, 0.01, 0.005, 1.0])):
+        for P in (8, 16, 32, 64):
+            E = L.Ensemble(0)
+            alg = L.ALG_BCB if method == L.METHOD_RPMD else L.ALG_BCB_TSIT5
+            E.set_problem(method, nstates, 1, 1, [mass], 1.0, algorithm=alg, nbeads=P, precision=L.PREC_FP32_NUCLEI,
+                          temperature=1.0 / beta)
+            E.set_model(0, L.MODEL_DW_GAUSSIAN, prm)
+            E.sample_initial(L.SAMPLER_RING_POLYMER_THERMAL, ntraj, [-3.87, np.sqrt(0.1 / mass), beta], KEY)
+            st = E.run(0.0, float(nsteps), KEY, save_every=100)
+            E.close()
+            rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+            res[f"{name}_P{P}"] = {"ntraj": ntraj, "nsteps": nsteps, "nbeads": P, "kernel_ms": st["kernel_ms"],
+                                   "traj_steps_per_s": rate, "bead_steps_per_s": rate * P,
+                                   "grid": st["grid"], "block": st["block"]}
+            _progress(f"{name} P={P}: {rate:.3e} ring-polymer steps/s")
+    return res
+
+
 TRAIN_STEPS = 40000
 TRAIN_BATCH = 2048
 
@@ -316,6 +371,8 @@ def main():
     }
     if not args.no_extras:
         out["iesh"] = iesh_extra(L)
+        out["spin_boson"] = spin_boson_extra(L)
+        out["ring_polymer"] = ring_polymer_extra(L)
         out["learned"] = surrogate_extra(L)
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
