@@ -172,23 +172,24 @@ __global__ __launch_bounds__(256) void zero_words(uint32_t* __restrict__ p, size
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-// Sum of n int32 values into out[0] (one workgroup: a few microseconds for 1e6 values, and the
-// host copies 8 bytes instead of the whole array).
-__global__ __launch_bounds__(1024) void sum_i32(const int32_t* __restrict__ x, int64_t n, long long* __restrict__ out) {
-    __shared__ long long part[1024];
+// Sum of n int32 values into out[0]: each wave sums a strided share, reduces it with shuffles and
+// adds it to out[0] with one 64-bit atomic (integer addition, so the result does not depend on the
+// order); the host then copies 8 bytes instead of the whole array.
+__global__ void sum_i32_clear(long long* out) { out[0] = 0; }
+
+__global__ __launch_bounds__(256) void sum_i32(const int32_t* __restrict__ x, int64_t n, long long* __restrict__ out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     long long a = 0;
-    for (int64_t i = threadIdx.x; i < n; i += 1024) a += x[i];
-    part[threadIdx.x] = a;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = part[0];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a += x[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+    if ((threadIdx.x & 63) == 0 && a != 0) atomicAdd(reinterpret_cast<unsigned long long*>(out), (unsigned long long)a);
 }
 
 hipError_t launch_sum_i32(const int32_t* x, int64_t n, long long* out, hipStream_t s) {
-    hipLaunchKernelGGL(sum_i32, dim3(1), dim3(1024), 0, s, x, n, out);
+    hipLaunchKernelGGL(sum_i32_clear, dim3(1), dim3(1), 0, s, out);
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 1024);
+    hipLaunchKernelGGL(sum_i32, dim3((unsigned)blocks), dim3(256), 0, s, x, n, out);
     return hipGetLastError();
 }
 

@@ -238,7 +238,9 @@ __device__ __forceinline__ void persistent_body(const Fssh2Args* __restrict__ Ap
                 if (q < (unsigned long long)A.ntraj) {
                     const int32_t* order = opaque(Ap)->order;
                     load_traj(Ap, L, order ? (int64_t)order[q] : (int64_t)q);
-                    if (is_done(A, L)) finish_traj(Ap, L); else active = true;
+                    // the termination predicate is a DiscreteCallback, evaluated after each step and
+                    // not at t0 (callbacks.jl:15-29): a fresh trajectory is done only for zero steps
+                    if (A.nsteps <= 0) finish_traj(Ap, L); else active = true;
                 } else {
                     exhausted = true;
                 }
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_init(const Fssh2Args A, doubl
     for (int J = 0; J < 4; ++J) S[(PL_U + J) * n + tid] = U[J];
     S[PL_T * n + tid] = A.t0;
     Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = 0; Si[PI_STEP * n + tid] = 0;
-    const bool done = (A.nsteps <= 0) || term_hit(A.term_kind, r, A.term_value);
+    const bool done = A.nsteps <= 0;  // the termination predicate is first evaluated after step 1
     Si[PI_DONE * n + tid] = done ? 1 : 0;
     plain_save(A, tid, 0, st, s, U, v, E0, E1);
 }

@@ -396,10 +396,13 @@ __device__ __forceinline__ void nd_body(const FsshNdArgs* __restrict__ Ap, const
                 L.step = 0;
                 nd_save(Ap, L, 0);
                 fresh = false;
+                // the termination predicate is a DiscreteCallback, evaluated after each step and not
+                // at t0 (callbacks.jl:15-29): a fresh trajectory is done only for zero steps
+                if (A.nsteps <= 0) { nd_finish(Ap, L); active = false; }
             } else {
                 nd_finish_step(Ap, A, L, v11, v22, v12, d11, d22, d12);
+                if (nd_done(A, L)) { nd_finish(Ap, L); active = false; }
             }
-            if (nd_done(A, L)) { nd_finish(Ap, L); active = false; }
         }
     }
 }

@@ -654,7 +654,10 @@ static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, 
     int64_t step = 0;
     std::vector<double> prob(n);
     while (step < R.nsteps) {
-        if (term_hit(R, r[R.term_dof])) break;
+        // TerminatingCallback is a DiscreteCallback: its condition is evaluated after each step and
+        // not at t0 (DynamicsUtils/callbacks.jl:15-29), so a trajectory that starts past the
+        // boundary takes one step
+        if (step > 0 && term_hit(R, r[R.term_dof])) break;
         std::vector<cd> sigprev = sig;
         for (int I = 0; I < D; ++I) vtmp[I] = v[I] + (R.dt / 2) * k[I];   // step_B!
         for (int I = 0; I < D; ++I) r[I] = r[I] + R.dt * vtmp[I];         // step_A!

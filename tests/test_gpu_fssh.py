@@ -297,3 +297,23 @@ def test_autotune_launch_shapes_leave_results_unchanged(oracle, key, tmp_path, m
         assert np.array_equal(tuned[k], ref[k]), k
         assert np.array_equal(forced[k], ref[k]), k
     assert ref["nhops"].sum() > 0
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_trajectories_starting_past_the_boundary_take_one_step(oracle, key, schedule):
+    """A third of the ensemble starts beyond |x| > 10: TerminatingCallback is evaluated after each step,
+    never at t0, so those take exactly one step; the whole ensemble matches the oracle (hop sequences,
+    lengths, FP64 state) on both schedules."""
+    L, E = _ens()
+    ntraj = 130
+    r0, v0, sig0, st0 = _window_inputs(oracle, key, ntraj, 0)
+    past = np.arange(ntraj) % 3 == 0
+    r0[past, 0] = np.where(np.arange(past.sum()) % 2 == 0, 10.5, -11.0)
+    g = _run_gpu(L, E, 0, 0, r0, v0, sig0, st0, 600, key, schedule=schedule, save_every=20, term=(0, 10.0))
+    E.close()
+    c = _run_cpu(oracle, 0, 0, r0, v0, sig0, st0, 600, key, save_every=20, term_x=10.0)
+    assert np.all(g["steps"][past] == 1)
+    assert np.array_equal(g["steps"], c["steps"]) and np.array_equal(g["state"], c["state"])
+    assert np.array_equal(g["save_state"], c["save_state"])
+    assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-14)

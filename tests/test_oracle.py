@@ -356,3 +356,18 @@ def test_saved_energies(oracle, key, method):
     assert rel.max() < 1e-2
     if method == 0:
         assert out["nhops"].sum() > 0
+
+
+def test_termination_is_evaluated_after_each_step(oracle, key):
+    """TerminatingCallback is a DiscreteCallback (DynamicsUtils/callbacks.jl:15-29): its condition is
+    checked after every step and not at t0, so a trajectory that starts past |x| > 10 takes exactly
+    one step, and one that reaches the boundary stops at the first step after crossing it."""
+    m = oracle.make_model(0, 2, 1, TULLY1)
+    r0 = np.array([[11.0], [-12.0], [9.985]])
+    v0 = np.array([[0.01], [-0.01], [0.01]])
+    sig0 = np.zeros((3, 2, 2), complex); sig0[:, 0, 0] = 1
+    out = oracle.run_ensemble(0, m, 2000.0, r0, v0, sig0, state0=np.zeros(3, np.int32), dt=1.0, nsteps=100, key=key,
+                              term_x=10.0)
+    assert list(out["steps"][:2]) == [1, 1]
+    assert np.isclose(out["r"][0, 0], 11.0 + 0.01, atol=1e-6) and np.isclose(out["r"][1, 0], -12.0 - 0.01, atol=1e-6)
+    assert out["steps"][2] == 2 and out["r"][2, 0] > 10.0  # 9.985 -> 9.995 -> 10.005
