@@ -49,6 +49,131 @@ def test_model_derivative_finite_difference(oracle, name, kind, params):
         assert np.allclose(dV, (Vp - Vm) / (2 * h), atol=1e-8, rtol=1e-6), (name, x)
 
 
+# --- Spin-boson model (docs/src/NQCModels/systembathmodels.md:21-27) at eps != 0 ----------------
+# V(R) = [[eps + c.R, Delta], [Delta, -eps - c.R]] + (1/2) sum_j w_j^2 R_j^2 (mass 1), so
+# V(0) = [[eps, Delta], [Delta, -eps]] and the adiabatic energies are
+# +-sqrt((eps + c.R)^2 + Delta^2) + (1/2) sum_j w_j^2 R_j^2.
+SB_EPS, SB_DELTA = 1.0, 1.0
+
+
+def _spin_boson(oracle, eps=SB_EPS, delta=SB_DELTA, nb=6, seed=11, c_scale=1.0):
+    rng = np.random.default_rng(seed)
+    omega = rng.uniform(0.2, 3.0, nb)
+    c = c_scale * rng.uniform(0.1, 0.8, nb) * omega
+    return oracle.make_model(3, 2, nb, [eps, delta], omega=omega, cbath=c), omega, c
+
+
+def _spin_boson_fd_ok(model_eval, m, nb, rng):
+    """Central differences of V and V0 against dV and dV0 at random bath coordinates."""
+    for _ in range(8):
+        R = rng.normal(0, 0.7, nb)
+        V, dV, V0, dV0 = model_eval(m, R)
+        for j in range(nb):
+            h = 1e-6
+            Rp, Rm = R.copy(), R.copy()
+            Rp[j] += h; Rm[j] -= h
+            Vp, _, V0p, _ = model_eval(m, Rp)
+            Vm, _, V0m, _ = model_eval(m, Rm)
+            if not np.allclose(dV[j], (Vp - Vm) / (2 * h), atol=1e-8, rtol=1e-6):
+                return False
+            if not np.isclose(dV0[j], (V0p - V0m) / (2 * h), atol=1e-8, rtol=1e-6):
+                return False
+    return True
+
+
+def _spin_boson_closed_form_ok(model_eval, m, omega, c, eps, delta, rng):
+    V, dV, V0, dV0 = model_eval(m, np.zeros(len(omega)))
+    if not (np.allclose(V, [[eps, delta], [delta, -eps]], atol=1e-15) and V0 == 0.0 and np.all(dV0 == 0.0)):
+        return False
+    for _ in range(8):
+        R = rng.normal(0, 0.7, len(omega))
+        V, _, V0, _ = model_eval(m, R)
+        bath = 0.5 * np.sum(omega ** 2 * R ** 2)
+        gap = np.sqrt((eps + c @ R) ** 2 + delta ** 2)
+        if not np.allclose(np.linalg.eigvalsh(V + V0 * np.eye(2)), [bath - gap, bath + gap], rtol=1e-13, atol=1e-14):
+            return False
+    return True
+
+
+def test_spin_boson_derivative_finite_difference(oracle):
+    """dV (state-dependent, per bath mode) and dV0 (the bath's harmonic force) by central
+    differences at eps = 1 with nonzero couplings (as test/Dynamics/utils.jl:39-65)."""
+    m, omega, c = _spin_boson(oracle)
+    assert _spin_boson_fd_ok(oracle.model_eval, m, len(omega), np.random.default_rng(1))
+
+    def planted_coupling_sign(mm, R):  # a sign error on c in the derivative only
+        V, dV, V0, dV0 = oracle.model_eval(mm, R)
+        return V, -dV, V0, dV0
+    assert not _spin_boson_fd_ok(planted_coupling_sign, m, len(omega), np.random.default_rng(1))
+
+
+def test_spin_boson_closed_form_values(oracle):
+    """V(0) = [[eps, Delta], [Delta, -eps]] and eigenvalues +-sqrt((eps + c.R)^2 + Delta^2) + sum w^2 R^2 / 2;
+    a model built with the sign of eps or of c flipped (what a sign error in either term of the
+    oracle produces) fails the same check."""
+    m, omega, c = _spin_boson(oracle)
+    rng = lambda: np.random.default_rng(2)
+    assert _spin_boson_closed_form_ok(oracle.model_eval, m, omega, c, SB_EPS, SB_DELTA, rng())
+    m_eps, _, _ = _spin_boson(oracle, eps=-SB_EPS)
+    assert not _spin_boson_closed_form_ok(oracle.model_eval, m_eps, omega, c, SB_EPS, SB_DELTA, rng())
+    m_c, _, _ = _spin_boson(oracle, c_scale=-1.0)
+    assert not _spin_boson_closed_form_ok(oracle.model_eval, m_c, omega, c, SB_EPS, SB_DELTA, rng())
+
+
+def _rabi_run(oracle, key, eps, delta, dt, nsteps):
+    """Ehrenfest on the spin-boson model with the bath uncoupled (c = 0), started in diabatic state 0:
+    the diabatic population of state 0 at each step."""
+    nb = 3
+    omega = np.array([0.5, 1.0, 1.5])
+    m = oracle.make_model(3, 2, nb, [eps, delta], omega=omega, cbath=np.zeros(nb))
+    ntraj = 4
+    r0, v0 = oracle.sample_gaussian(ntraj, nb, 0.0, 0.5, 0.0, 0.5, key)
+    sig0 = oracle.diabatic_to_adiabatic(m, r0, 0)
+    out = oracle.run_ensemble(1, m, 1.0, r0, v0, sig0, dt=dt, nsteps=nsteps, save_every=1, nsave=nsteps + 1, key=key)
+    return out["pop_di"][:, :, 0]
+
+
+def test_spin_boson_uncoupled_ehrenfest_rabi(oracle, key):
+    """With c = 0 the two-level system is free: P0(t) = 1 - Delta^2/(eps^2 + Delta^2) sin^2(sqrt(eps^2 + Delta^2) tau).
+    The electronic integrator's buffers start from a zero Hamiltonian at t = 0 (bab_electronics.jl:35-38,
+    electronic_dynamics.jl:118-127), so the first step accumulates half a step of phase: tau = t - dt/2
+    for t >= dt.  Pins Delta, |eps|, the adiabatic initial density and the diabatic population estimator."""
+    dt, nsteps = 0.02, 500
+    for eps, delta in ((SB_EPS, SB_DELTA), (0.5, 1.3)):
+        P0 = _rabi_run(oracle, key, eps, delta, dt, nsteps)
+        t = np.arange(nsteps + 1) * dt
+        tau = np.where(t > 0, t - dt / 2, 0.0)
+        W = np.sqrt(eps ** 2 + delta ** 2)
+        exact = 1 - delta ** 2 / W ** 2 * np.sin(W * tau) ** 2
+        assert np.abs(P0 - exact[None, :]).max() < 1e-8, np.abs(P0 - exact[None, :]).max()
+        # the same law without the half-step of the zero start is far outside that bound
+        assert np.abs(P0 - (1 - delta ** 2 / W ** 2 * np.sin(W * t) ** 2)[None, :]).max() > 1e-3
+
+
+def test_diabatic_to_adiabatic_initial_density(oracle, key):
+    """initialise_adiabatic_density_matrix (density_matrix_dynamics.jl:37-62,64-75): sigma_ad = U^T rho U for the
+    diabatic pure state rho = e_d e_d^T, with U the eigenvectors at r0.  Checked against numpy's eigh brought to
+    the sign rule pinned above (largest-magnitude component of each column positive): the diagonal U_da^2 (gauge
+    free), the whole matrix, and the transform back, U sigma_ad U^T = rho."""
+    m, omega, c = _spin_boson(oracle)
+    ntraj, nb = 16, len(omega)
+    r0, _ = oracle.sample_gaussian(ntraj, nb, 0.0, 0.8, 0.0, 0.1, key)
+    rho = [np.diag([1.0, 0.0]), np.diag([0.0, 1.0])]
+    for d in (0, 1):
+        sig = oracle.diabatic_to_adiabatic(m, r0, d)
+        for i in range(ntraj):
+            _, Un = np.linalg.eigh(oracle.model_eval(m, r0[i])[0])
+            Un = Un * np.sign(Un[np.argmax(np.abs(Un), axis=0), [0, 1]])[None, :]
+            assert np.allclose(np.diag(sig[i]).real, Un[d, :] ** 2, atol=1e-14)
+            assert np.allclose(sig[i], np.outer(Un[d], Un[d]), atol=1e-14)
+            assert np.allclose(Un @ sig[i] @ Un.T, rho[d], atol=1e-14)
+            assert np.allclose(sig[i].imag, 0) and np.isclose(np.trace(sig[i]).real, 1.0, atol=1e-15)
+    # the other diabatic state's density (an index mistake) fails the back-transform (for two states U is a
+    # symmetric reflection, so rows and columns coincide and a transposed U is no mistake here)
+    _, Un = np.linalg.eigh(oracle.model_eval(m, r0[0])[0])
+    assert not np.allclose(Un @ oracle.diabatic_to_adiabatic(m, r0[:1], 1)[0] @ Un.T, rho[0], atol=1e-6)
+
+
 def test_model_known_values(oracle):
     t1 = oracle.make_model(0, 2, 1, TULLY1)
     V = oracle.model_eval(t1, [0.0])[0]
