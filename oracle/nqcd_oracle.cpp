@@ -968,6 +968,139 @@ OR_API int32_t oracle_rp_run_ensemble(const ORPDesc* R, const OModelDesc* m, dou
 }
 
 // ---------------------------------------------------------------------------
+// Path-integral Monte Carlo for one-dof ring polymers (MetropolisHastings.jl:58-82,131-148,170-258):
+// one Metropolis-Hastings chain per trajectory, for one quantum atom with no fixed atoms, so every
+// step is (run_main_loop!, :170-195):
+//   centroid move: the centroid normal mode shifted by (u - 1/2) Delta / sqrt(ndofs) (:208-229), i.e.
+//     every bead by that over sqrt(P), then assess_proposal!;
+//   pass_length = P / segment_length times: segment_length draws of a mode k in 2:P (1-based), each
+//     resampled from Normal(0, sigma_k) (sample_mode!, :240-248), then assess_proposal!;
+//   assess_proposal! once more on the last proposal (:194).
+// assess_proposal! accepts when min(1, exp(-(E' - E)/w_n)) > u (:260-285) with E the classical
+// potential summed over beads (DynamicsUtils.jl:143-147: element [1] of the model's potential, the
+// first diabat of a two-state model), w_n = P T.  sigma_k = sqrt(w_n/m)/(m w_k^2/2) as the reference
+// writes it (normal_mode_springs = w_k^2/2, ring_polymer.jl:59), or with width = 1 the free ring
+// polymer's sqrt(w_n/m)/w_k.  The normal modes are the Hartley basis of the device kernels (column k
+// spans the w_k eigenspace, as RingPolymerArrays' modes do; the chain's stationary law only sees the
+// eigenspaces).  Julia's rand calls become Philox stream 4: counter (traj, step, 4 + 16 sub) with
+// sub 0 the centroid move (u53(o0,o1) the shift, u53(o2,o3) the acceptance), sub 1 + j (L + 1) + s
+// segment draw s of mode pass j (mode 1 + (o0 (P-1)) >> 32, z = sqrt(-2 log(1 - u53(o1,o2)))
+// cos(2 pi o3 2^-32)), sub 1 + j (L + 1) + L its acceptance u53(o0,o1), sub 1 + J (L + 1) the last
+// re-assessment.  Velocities: v_b ~ Normal(0, sqrt(w_n/m)) from counter (traj, bead, stream 0), z1.
+// ---------------------------------------------------------------------------
+enum { STREAM_PIMC = 4 };
+
+struct OPimcDesc {
+    int32_t nbeads;
+    int32_t segment_length;   // max(1, round(segment * nbeads)), <= nbeads
+    int64_t steps;            // n_atoms * passes
+    double mass;
+    double temperature;       // w_n = nbeads * temperature
+    double delta;             // centroid move width Delta
+    int32_t width;            // 0: the reference's sigma_k, 1: the free ring polymer's
+    int32_t reassess;         // 1: the reference's extra assess_proposal! of the last proposal (:194)
+    uint64_t key;
+};
+
+static void hartley_matrix(int P, double* H) {
+    const double s = 1.0 / std::sqrt((double)P);
+    for (int j = 0; j < P; ++j)
+        for (int k = 0; k < P; ++k) {
+            const double a = 2.0 * M_PI * (double)((j * k) % P) / (double)P;
+            H[j * P + k] = (std::cos(a) + std::sin(a)) * s;
+        }
+}
+
+static inline void pimc_draw(uint64_t traj, int64_t step, uint32_t sub, uint64_t key, uint32_t o[4]) {
+    philox_draw(traj, (uint32_t)step, STREAM_PIMC + 16u * sub, key, o);
+}
+
+static double pimc_energy(const OModelDesc& m, const double* x, int P) {
+    double E = 0.0;
+    std::vector<double> V(m.nstates * m.nstates), dV(m.nstates * m.nstates * m.ndof), dV0(m.ndof);
+    double V0;
+    for (int b = 0; b < P; ++b) {
+        model_eval(m, x + b, V.data(), dV.data(), &V0, dV0.data());
+        E += V[0] + V0;
+    }
+    return E;
+}
+
+static void pimc_one(const OPimcDesc& D, const OModelDesc& m, uint64_t traj, const double* H, double* x, double* v,
+                     int64_t* accepted) {
+    const int P = D.nbeads, L = D.segment_length, J = P / L;
+    const double wn = P * D.temperature, m_ = D.mass;
+    std::vector<double> xp(P), y(P), sig(P, 0.0);
+    for (int k = 1; k < P; ++k) {
+        const double wk = matsubara(P, wn, k);
+        sig[k] = D.width == 0 ? std::sqrt(wn / m_) / (0.5 * wk * wk * m_) : std::sqrt(wn / m_) / wk;
+    }
+    double Ei = pimc_energy(m, x, P), Ep = Ei;
+    int64_t acc = 0;
+    auto assess = [&](double u) {
+        Ep = pimc_energy(m, xp.data(), P);
+        if (std::min(1.0, std::exp(-(Ep - Ei) / wn)) > u) {
+            Ei = Ep;
+            std::copy(xp.begin(), xp.end(), x);
+            ++acc;
+        }
+    };
+    uint32_t o[4];
+    for (int64_t it = 0; it < D.steps; ++it) {
+        pimc_draw(traj, it, 0, D.key, o);
+        const double shift = (u53(o[0], o[1]) - 0.5) * D.delta / std::sqrt(1.0);
+        for (int b = 0; b < P; ++b) xp[b] = x[b] + shift * H[b * P + 0];
+        assess(u53(o[2], o[3]));
+        if (P > 1) {
+            for (int j = 0; j < J; ++j) {
+                for (int k = 0; k < P; ++k) {  // to normal modes
+                    double a = 0.0;
+                    for (int b = 0; b < P; ++b) a += H[k * P + b] * x[b];
+                    y[k] = a;
+                }
+                for (int s = 0; s < L; ++s) {
+                    pimc_draw(traj, it, 1u + (uint32_t)(j * (L + 1) + s), D.key, o);
+                    const int k = 1 + (int)(((uint64_t)o[0] * (uint64_t)(P - 1)) >> 32);
+                    const double z = std::sqrt(-2.0 * std::log(1.0 - u53(o[1], o[2]))) *
+                                     std::cos(6.283185307179586 * ((double)o[3] * 0x1.0p-32));
+                    y[k] = sig[k] * z;
+                }
+                for (int b = 0; b < P; ++b) {  // back to beads
+                    double a = 0.0;
+                    for (int k = 0; k < P; ++k) a += H[b * P + k] * y[k];
+                    xp[b] = a;
+                }
+                pimc_draw(traj, it, 1u + (uint32_t)(j * (L + 1) + L), D.key, o);
+                assess(u53(o[0], o[1]));
+            }
+        }
+        if (D.reassess) {
+            pimc_draw(traj, it, 1u + (uint32_t)(J * (L + 1)), D.key, o);
+            assess(u53(o[0], o[1]));
+        }
+    }
+    for (int b = 0; b < P; ++b) {
+        double z0, z1;
+        normal2(traj, (uint32_t)b, STREAM_INIT, D.key, &z0, &z1);
+        v[b] = std::sqrt(wn / m_) * z1;
+    }
+    *accepted = acc;
+}
+
+// x (ntraj, P): starting beads in, final beads out; v (ntraj, P) out; accepted (ntraj) out.
+OR_API int32_t oracle_pimc(const OPimcDesc* D, const OModelDesc* m, int64_t ntraj, double* x, double* v,
+                           int64_t* accepted, int32_t nthreads) {
+    const int P = D->nbeads;
+    if (P < 1 || D->segment_length < 1 || D->segment_length > P) return -1;
+    std::vector<double> H((size_t)P * P);
+    hartley_matrix(P, H.data());
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        pimc_one(*D, *m, (uint64_t)i, H.data(), x + (size_t)i * P, v + (size_t)i * P, accepted + i);
+    });
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Independent-electron surface hopping (IESH) and EhrenfestNA on a Newns-Anderson
 // (impurity + discretised bath) model.
 //   Model (NQCModels AndersonHolstein with a MiaoSubotnik impurity and a TrapezoidalRule

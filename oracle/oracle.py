@@ -275,6 +275,33 @@ def rp_run_ensemble(method, model, mass, r0, v0, sig0=None, state0=None, dt=1.0,
     return out
 
 
+class OPimcDesc(C.Structure):
+    _fields_ = [("nbeads", C.c_int32), ("segment_length", C.c_int32), ("steps", C.c_int64), ("mass", C.c_double),
+                ("temperature", C.c_double), ("delta", C.c_double), ("width", C.c_int32), ("reassess", C.c_int32),
+                ("key", C.c_uint64)]
+
+
+def pimc(model, x0, mass, temperature, delta, passes, segment=1.0, width=0, key=0, nthreads=None, reassess=1):
+    """Path-integral Monte Carlo chains (MetropolisHastings.jl:131-148,170-258), one per row of x0 (ntraj, P):
+    returns the final beads, bead velocities ~ N(0, sqrt(P T / m)) and accepted moves per chain."""
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    ntraj, P = x.shape
+    D = OPimcDesc()
+    D.nbeads, D.segment_length, D.steps = P, max(1, int(round(segment * P))), int(passes)
+    D.mass, D.temperature, D.delta, D.width, D.key = mass, temperature, delta, width, key
+    D.reassess = reassess
+    v = np.zeros_like(x)
+    acc = np.zeros(ntraj, np.int64)
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_pimc
+    f.restype = C.c_int32
+    rc = f(C.byref(D), C.byref(model), C.c_int64(ntraj), _p(x), _p(v), _p(acc, C.c_int64), C.c_int32(nthreads))
+    if rc != 0:
+        raise ValueError("pimc: segment_length must be in 1..nbeads")
+    return x, v, acc
+
+
 # --- IESH / EhrenfestNA on the Newns-Anderson model ------------------------------------------
 class ONADesc(C.Structure):
     _fields_ = [("mw2", C.c_double), ("g", C.c_double), ("dG", C.c_double), ("Gamma", C.c_double), ("a", C.c_double),
