@@ -274,6 +274,17 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
  * POPULATION_CORRELATION, column-major). */
 int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t output_kind, int32_t reduction_kind, double* out, int64_t len);
 
+/* The same reduction with no device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The partial sums and the result go through one 2 MiB scratch array per
+ * device (262144 doubles) that the library holds until the process ends, resolved when the first
+ * handle is opened on the device; outputs computed from the final state are evaluated on the fly
+ * instead of being materialised.  A reduction needing rows x ceil(ntraj / 4096, at most 1024 chunks)
+ * (twice that for TOTAL_ENERGY) + rows doubles beyond that limit fails with NQCD_ERR_ARG.  Results are
+ * bit-identical to nqcd_mi355x_reduce.  Every call on a device shares that device's array: calls that
+ * overlap on one device (from different handles or threads) race. */
+int32_t nqcd_mi355x_reduce_noalloc(nqcd_handle_t* h, int32_t output_kind, int32_t reduction_kind, double* out, int64_t len);
+
 /* Copy a per-trajectory field to host memory (SortByTrajectoryReduction / EnsembleSaver data,
  * Ensembles.jl:39-63, reductions.jl:208-209).  bytes must equal the field size. */
 int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes);

@@ -154,6 +154,10 @@ int32_t nqcd_mi355x_open(int32_t device, nqcd_handle_t** out) {
     if (device < 0 || device >= count) return fail(nullptr, NQCD_ERR_ARG, "device index out of range");
     e = hipSetDevice(device);
     if (e != hipSuccess) return fail(nullptr, NQCD_ERR_DEVICE, hipGetErrorString(e));
+    // address of this device's scratch array for nqcd_mi355x_reduce_noalloc (resolved once per device)
+    e = reduce_scratch_init(device);
+    if (e != hipSuccess)
+        return fail(nullptr, NQCD_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
     nqcd_handle* h = new nqcd_handle();
     h->device = device;
     if (hipStreamCreate(&h->stream) != hipSuccess || hipEventCreate(&h->ev0) != hipSuccess ||
@@ -529,7 +533,23 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
     return NQCD_OK;
 }
 
-int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* out, int64_t len) {
+}  // extern "C"
+
+namespace {
+
+// What a Mean/Sum reduction of an output sums: materialised rows (src, and for the total energy the
+// potential-energy rows src2, summed separately and added after scaling), or rows computed per
+// trajectory from the final state / the diabatic save slots (fill).
+struct ReducePlan {
+    int32_t rows = 0;
+    const double* src = nullptr;
+    const double* src2 = nullptr;
+    bool fill = false;
+    OutputSrc S{};
+    double scale = 1.0;
+};
+
+int32_t reduce_plan(nqcd_handle_t* h, int32_t kind, int32_t red, const double* out, int64_t len, ReducePlan& P) {
     if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
     if (!h->have_results) return fail(h, NQCD_ERR_STATE, "no results: call nqcd_mi355x_run first");
     if (red != NQCD_RED_MEAN && red != NQCD_RED_SUM) return fail(h, NQCD_ERR_ARG, "unknown reduction");
@@ -537,7 +557,6 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* 
     const int64_t nt = h->ntraj;
     int32_t rows = 0;
     const double* src = nullptr;
-    HIPCHK(h, hipSetDevice(h->device));
     if (is_ie(h)) {
         if (kind == NQCD_OUT_ADIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_ad.as<double>(); }
         else if (kind == NQCD_OUT_DIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_di.as<double>(); }
@@ -562,27 +581,71 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* 
     if (len != rows) return fail(h, NQCD_ERR_ARG, "len does not match the output size");
     if (h->prob.method == NQCD_METHOD_RPMD && kind != NQCD_OUT_CENTROID_POSITION)
         return fail(h, NQCD_ERR_UNSUPPORTED, "RPMD has no electronic outputs");
+    P = ReducePlan{};
+    P.rows = rows;
+    P.src = src;
+    P.src2 = kind == NQCD_OUT_TOTAL_ENERGY ? h->save_epot.as<double>() : nullptr;
     if (!src) {
+        P.fill = true;
+        P.S.kind = kind;
+        P.S.method = h->prob.method == NQCD_METHOD_FSSH ? 0 : 1;
+        P.S.nsave = h->nsave;
+        P.S.xdof = 0;
+        P.S.r = h->r.as<double>();
+        P.S.sig = h->sig.as<double>();
+        P.S.state = h->state.as<int32_t>();
+        P.S.save_pop_di = h->save_di.as<double>();
+    }
+    P.scale = (red == NQCD_RED_MEAN) ? 1.0 / (double)nt : 1.0;
+    return NQCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t kind, int32_t red, double* out, int64_t len) {
+    ReducePlan P;
+    if (int32_t rc = reduce_plan(h, kind, red, out, len, P)) return rc;
+    const int64_t nt = h->ntraj;
+    const int32_t rows = P.rows;
+    HIPCHK(h, hipSetDevice(h->device));
+    const double* src = P.src;
+    if (P.fill) {
         HIPCHK(h, h->tmp.ensure(sizeof(double) * rows * nt));
-        HIPCHK(h, launch_fill_output2(kind, h->prob.method == NQCD_METHOD_FSSH ? 0 : 1, nt, h->nsave, 0,
-                                      h->r.as<double>(), h->sig.as<double>(), h->state.as<int32_t>(),
-                                      h->save_di.as<double>(), h->tmp.as<double>(), h->stream));
+        HIPCHK(h, launch_fill_output2(P.S, nt, h->tmp.as<double>(), h->stream));
         src = h->tmp.as<double>();
     }
     HIPCHK(h, h->partial.ensure(sizeof(double) * rows * reduce_nchunk(nt)));
     HIPCHK(h, h->small.ensure(sizeof(double) * rows));
-    const double scale = (red == NQCD_RED_MEAN) ? 1.0 / (double)nt : 1.0;
-    HIPCHK(h, launch_reduce_rows(src, rows, nt, h->partial.as<double>(), scale, h->small.as<double>(), h->stream));
+    HIPCHK(h, launch_reduce_rows(src, rows, nt, h->partial.as<double>(), P.scale, h->small.as<double>(), h->stream));
     HIPCHK(h, hipMemcpyAsync(out, h->small.p, sizeof(double) * rows, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (kind == NQCD_OUT_TOTAL_ENERGY) {  // reduce the potential energy too; the reduction is linear
+    if (P.src2) {  // total energy: reduce the potential energy too; the reduction is linear
         std::vector<double> pot(rows);
-        HIPCHK(h, launch_reduce_rows(h->save_epot.as<double>(), rows, nt, h->partial.as<double>(), scale,
-                                     h->small.as<double>(), h->stream));
+        HIPCHK(h, launch_reduce_rows(P.src2, rows, nt, h->partial.as<double>(), P.scale, h->small.as<double>(), h->stream));
         HIPCHK(h, hipMemcpyAsync(pot.data(), h->small.p, sizeof(double) * rows, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         for (int32_t k = 0; k < rows; ++k) out[k] += pot[k];
     }
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_reduce_noalloc(nqcd_handle_t* h, int32_t kind, int32_t red, double* out, int64_t len) {
+    ReducePlan P;
+    if (int32_t rc = reduce_plan(h, kind, red, out, len, P)) return rc;
+    const int64_t nt = h->ntraj;
+    const size_t need = reduce_noalloc_doubles(P.rows, nt, P.src2 != nullptr);
+    if (need > kReduceScratchDoubles)
+        return fail(h, NQCD_ERR_ARG, "reduction needs " + std::to_string(need) + " doubles of scratch (rows x chunks of " +
+                                         "4096 trajectories + rows), more than the " + std::to_string(kReduceScratchDoubles) +
+                                         " the allocation-free entry point holds; use nqcd_mi355x_reduce");
+    HIPCHK(h, hipSetDevice(h->device));
+    double* scr = reduce_scratch(h->device);
+    if (!scr) return fail(h, NQCD_ERR_STATE, "scratch address of this device not resolved (open a handle on it first)");
+    HIPCHK(h, launch_reduce_rows_noalloc(P.fill ? nullptr : P.src, P.src2, P.S, P.rows, nt, P.scale, scr, h->stream));
+    HIPCHK(h, hipMemcpyAsync(out, scr + (need - (size_t)P.rows), sizeof(double) * P.rows, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return NQCD_OK;
 }
 

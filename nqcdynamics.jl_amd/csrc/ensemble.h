@@ -15,9 +15,20 @@ hipError_t launch_sample_gaussian(int64_t ntraj, int32_t D, const double* prm, u
                                   hipStream_t s);
 hipError_t launch_init_density2(int64_t ntraj, int32_t state, int32_t diabatic, const ModelSlot* models,
                                 const int32_t* model_id, const double* r0, double* sig0, hipStream_t s);
-hipError_t launch_fill_output2(int32_t kind, int32_t method, int64_t ntraj, int32_t nsave, int32_t xdof, const double* r,
-                               const double* sig, const int32_t* state, const double* save_pop_di, double* out,
-                               hipStream_t s);
+// Source of an output computed per trajectory from the final state (final adiabatic populations,
+// state-resolved scattering) or from the diabatic save slots (population correlation).
+struct OutputSrc {
+    int32_t kind = 0;             // OUTK_*
+    int32_t method = 0;           // 0 FSSH (one-hot active state), 1 Ehrenfest (diagonal of sigma)
+    int32_t nsave = 0;
+    int32_t xdof = 0;             // row of r classified by sign for scattering
+    const double* r = nullptr;    // [D][ntraj]
+    const double* sig = nullptr;  // [4][ntraj]
+    const int32_t* state = nullptr;
+    const double* save_pop_di = nullptr;  // [nsave][2][ntraj]
+};
+hipError_t launch_fill_output2(const OutputSrc& S, int64_t ntraj, double* out, hipStream_t s);
+int32_t fill_output2_rows(int32_t kind, int32_t nsave);
 // Clears `bytes` (a multiple of 4) at p with a fill kernel: save slots a trajectory never reaches
 // (it terminated first) read as zero, whatever the buffer held before.
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
@@ -26,5 +37,19 @@ hipError_t launch_sum_i32(const int32_t* x, int64_t n, long long* out, hipStream
 int reduce_nchunk(int64_t ntraj);
 hipError_t launch_reduce_rows(const double* data, int32_t rows, int64_t ntraj, double* partial, double scale, double* out,
                               hipStream_t s);
+
+// Allocation-free reduction: a per-device scratch array of kReduceScratchDoubles doubles (2 MiB).
+constexpr size_t kReduceScratchDoubles = (size_t)1 << 18;
+// Resolves the scratch array's address on `device` (which must be current) once per process.
+hipError_t reduce_scratch_init(int device);
+// Address of the scratch array on `device`, or null before reduce_scratch_init(device).
+double* reduce_scratch(int device);
+// Doubles of scratch a reduction of `rows` rows over ntraj trajectories uses.
+size_t reduce_noalloc_doubles(int32_t rows, int64_t ntraj, bool two_sources);
+// Sums `rows` rows (data, or S computed on the fly when data is null; plus data2 when given, summed
+// separately and added after scaling) into scratch_ptr[reduce_noalloc_doubles - rows ..] (the last
+// `rows` doubles of the region used).
+hipError_t launch_reduce_rows_noalloc(const double* data, const double* data2, const OutputSrc& S, int32_t rows,
+                                      int64_t ntraj, double scale, double* scratch_ptr, hipStream_t s);
 
 }  // namespace nqcd

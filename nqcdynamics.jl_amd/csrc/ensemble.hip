@@ -8,6 +8,7 @@
 // Reductions run in a fixed order (chunked tree sums, then an ordered sum of
 // chunk partials) so results do not depend on the device or launch shape.
 #include <algorithm>
+#include <mutex>
 
 #include "ensemble.h"
 
@@ -64,48 +65,55 @@ __global__ __launch_bounds__(256) void init_density2(int64_t ntraj, int32_t stat
 // ---------------------------------------------------------------------------
 // Per-trajectory output rows for the two-state kernels.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void fill_output2(int32_t kind, int32_t method, int64_t ntraj, int32_t nsave,
-                                                    int32_t xdof, const double* r, const double* sig,
-                                                    const int32_t* state, const double* save_pop_di, double* out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntraj) return;
-    if (kind == OUTK_FINAL_ADIABATIC || kind == OUTK_SCATTERING_1D) {
-        double p0, p1;
-        if (method == 0) { p0 = (state[i] == 0) ? 1.0 : 0.0; p1 = 1.0 - p0; }
-        else { p0 = sig[i]; p1 = sig[ntraj + i]; }
-        if (kind == OUTK_FINAL_ADIABATIC) { out[i] = p0; out[ntraj + i] = p1; return; }
-        // OutputStateResolvedScattering1D: rows [reflection(2), transmission(2)] (DynamicsOutputs.jl:331-336)
-        const bool trans = r[(int64_t)xdof * ntraj + i] > 0.0;
-        out[0 * ntraj + i] = trans ? 0.0 : p0;
-        out[1 * ntraj + i] = trans ? 0.0 : p1;
-        out[2 * ntraj + i] = trans ? p0 : 0.0;
-        out[3 * ntraj + i] = trans ? p1 : 0.0;
-        return;
-    }
-    if (kind == OUTK_POP_CORRELATION) {
-        // out[slot][i + 2j] = P_i(0) P_j(t) (TimeCorrelationFunctions.jl:86-88), column-major 2x2
-        const double q0 = save_pop_di[0 * ntraj + i], q1 = save_pop_di[1 * ntraj + i];
-        for (int s = 0; s < nsave; ++s) {
-            const double p0 = save_pop_di[((int64_t)s * 2 + 0) * ntraj + i];
-            const double p1 = save_pop_di[((int64_t)s * 2 + 1) * ntraj + i];
-            out[((int64_t)s * 4 + 0) * ntraj + i] = q0 * p0;
-            out[((int64_t)s * 4 + 1) * ntraj + i] = q1 * p0;
-            out[((int64_t)s * 4 + 2) * ntraj + i] = q0 * p1;
-            out[((int64_t)s * 4 + 3) * ntraj + i] = q1 * p1;
-        }
-    }
+// Number of rows of an output computed from the final state or the save slots.
+__host__ __device__ inline int32_t output2_rows(int32_t kind, int32_t nsave) {
+    return kind == OUTK_FINAL_ADIABATIC ? 2 : kind == OUTK_SCATTERING_1D ? 4 : 4 * nsave;
 }
 
-// Deterministic two-pass row reduction: rows x ntraj -> rows.
-__global__ __launch_bounds__(256) void reduce_rows_pass1(const double* data, int64_t ntraj, int32_t nchunk, double* partial) {
+// Row `row` of trajectory i (one expression shared by the materialising and the fused reductions,
+// so both sum the same values in the same order).
+__device__ __forceinline__ double output2_value(const OutputSrc& S, int64_t ntraj, int32_t row, int64_t i) {
+    if (S.kind == OUTK_FINAL_ADIABATIC || S.kind == OUTK_SCATTERING_1D) {
+        double p0, p1;
+        if (S.method == 0) { p0 = (S.state[i] == 0) ? 1.0 : 0.0; p1 = 1.0 - p0; }
+        else { p0 = S.sig[i]; p1 = S.sig[ntraj + i]; }
+        if (S.kind == OUTK_FINAL_ADIABATIC) return row == 0 ? p0 : p1;
+        // OutputStateResolvedScattering1D: rows [reflection(2), transmission(2)] (DynamicsOutputs.jl:331-336)
+        const bool trans = S.r[(int64_t)S.xdof * ntraj + i] > 0.0;
+        const double p = (row & 1) ? p1 : p0;
+        return (row < 2) ? (trans ? 0.0 : p) : (trans ? p : 0.0);
+    }
+    // OUTK_POP_CORRELATION: row = 4 slot + (a + 2 b) holds P_a(0) P_b(t_slot) (TimeCorrelationFunctions.jl:86-88),
+    // column-major 2x2 per slot
+    const int32_t slot = row >> 2, a = row & 1, b = (row >> 1) & 1;
+    return S.save_pop_di[(int64_t)a * ntraj + i] * S.save_pop_di[((int64_t)slot * 2 + b) * ntraj + i];
+}
+
+__global__ __launch_bounds__(256) void fill_output2(OutputSrc S, int64_t ntraj, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntraj) return;
+    const int32_t rows = output2_rows(S.kind, S.nsave);
+    for (int32_t row = 0; row < rows; ++row) out[(int64_t)row * ntraj + i] = output2_value(S, ntraj, row, i);
+}
+
+// Deterministic two-pass row reduction: rows x ntraj -> rows.  Pass 1 sums chunk `blockIdx.x` of row
+// row0 + blockIdx.y with a tree in LDS; pass 2 sums a row's chunk partials in order.  The source is a
+// materialised array (data, row-major [row][traj]) or, with data null, the output rows of S computed
+// on the fly.
+__global__ __launch_bounds__(256) void reduce_rows_pass1(const double* data, OutputSrc S, int64_t ntraj, int32_t nchunk,
+                                                         int32_t row0, double* partial) {
     __shared__ double sm[256];
-    const int row = blockIdx.y, chunk = blockIdx.x;
+    const int row = row0 + (int)blockIdx.y, chunk = blockIdx.x;
     const int64_t len = (ntraj + nchunk - 1) / nchunk;
     const int64_t b = (int64_t)chunk * len;
     const int64_t e = (b + len < ntraj) ? b + len : ntraj;
-    const double* d = data + (int64_t)row * ntraj;
     double s = 0.0;
-    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) s += d[i];
+    if (data) {
+        const double* d = data + (int64_t)row * ntraj;
+        for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) s += d[i];
+    } else {
+        for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) s += output2_value(S, ntraj, row, i);
+    }
     sm[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -115,12 +123,23 @@ __global__ __launch_bounds__(256) void reduce_rows_pass1(const double* data, int
     if (threadIdx.x == 0) partial[(int64_t)row * nchunk + chunk] = sm[0];
 }
 
-__global__ void reduce_rows_pass2(const double* partial, int32_t rows, int32_t nchunk, double scale, double* out) {
+// out[row] = scale * (sum of row's partials) [+ scale * (sum of row's partials in partial2)]
+__global__ void reduce_rows_pass2(const double* partial, const double* partial2, int32_t rows, int32_t nchunk, double scale,
+                                  double* out) {
     const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (row >= rows) return;
     double s = 0.0;
     for (int c = 0; c < nchunk; ++c) s += partial[(int64_t)row * nchunk + c];
-    out[row] = s * scale;
+    double v = s * scale;
+    if (partial2) {
+#pragma clang fp contract(off)
+        // a rounded product, then the sum, as the materialising path adds two rounded rows on the host
+        double s2 = 0.0;
+        for (int c = 0; c < nchunk; ++c) s2 += partial2[(int64_t)row * nchunk + c];
+        const double w = s2 * scale;
+        v = v + w;
+    }
+    out[row] = v;
 }
 
 static inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
@@ -145,13 +164,12 @@ hipError_t launch_init_density2(int64_t ntraj, int32_t state, int32_t diabatic, 
     return hipGetLastError();
 }
 
-hipError_t launch_fill_output2(int32_t kind, int32_t method, int64_t ntraj, int32_t nsave, int32_t xdof, const double* r,
-                               const double* sig, const int32_t* state, const double* save_pop_di, double* out,
-                               hipStream_t s) {
-    hipLaunchKernelGGL(fill_output2, dim3(grid_for(ntraj, 256)), dim3(256), 0, s, kind, method, ntraj, nsave, xdof, r,
-                       sig, state, save_pop_di, out);
+hipError_t launch_fill_output2(const OutputSrc& S, int64_t ntraj, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(fill_output2, dim3(grid_for(ntraj, 256)), dim3(256), 0, s, S, ntraj, out);
     return hipGetLastError();
 }
+
+int32_t fill_output2_rows(int32_t kind, int32_t nsave) { return output2_rows(kind, nsave); }
 
 int reduce_nchunk(int64_t ntraj) {
     int64_t c = (ntraj + 4095) / 4096;
@@ -160,12 +178,83 @@ int reduce_nchunk(int64_t ntraj) {
     return (int)c;
 }
 
+// Pass 1 over row blocks of at most 65535 (the grid's y limit), then pass 2.
+static hipError_t reduce_rows_launch(const double* data, const OutputSrc& S, int32_t rows, int64_t ntraj, double* partial,
+                                     hipStream_t s) {
+    const int nchunk = reduce_nchunk(ntraj);
+    for (int32_t row0 = 0; row0 < rows; row0 += 65535) {
+        const int32_t nr = std::min<int32_t>(65535, rows - row0);
+        hipLaunchKernelGGL(reduce_rows_pass1, dim3(nchunk, nr), dim3(256), 0, s, data, S, ntraj, nchunk, row0, partial);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce_rows(const double* data, int32_t rows, int64_t ntraj, double* partial, double scale, double* out,
                               hipStream_t s) {
-    const int nchunk = reduce_nchunk(ntraj);
-    hipLaunchKernelGGL(reduce_rows_pass1, dim3(nchunk, rows), dim3(256), 0, s, data, ntraj, nchunk, partial);
-    hipLaunchKernelGGL(reduce_rows_pass2, dim3(grid_for(rows, 64)), dim3(64), 0, s, partial, rows, nchunk, scale, out);
+    hipError_t e = reduce_rows_launch(data, OutputSrc{}, rows, ntraj, partial, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_rows_pass2, dim3(grid_for(rows, 64)), dim3(64), 0, s, partial, (const double*)nullptr, rows,
+                       reduce_nchunk(ntraj), scale, out);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free reduction.  Its partial sums and result live in `scratch`, one array per device
+// that the library holds until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (the array's name is a host-side stand-in).
+// Every call on a device shares that device's array, so calls that overlap on one device race.
+// Pass 1 writes every partial it reads and pass 2 writes every result it returns, so no part of the
+// array has to start at zero.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kReduceScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t reduce_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current first
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+    return hipSuccess;
+}
+
+double* reduce_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+hipError_t launch_reduce_rows_noalloc(const double* data, const double* data2, const OutputSrc& S, int32_t rows,
+                                      int64_t ntraj, double scale, double* scratch_ptr, hipStream_t s) {
+    const int nchunk = reduce_nchunk(ntraj);
+    double* partial = scratch_ptr;
+    double* partial2 = data2 ? scratch_ptr + (size_t)rows * nchunk : nullptr;
+    double* out = scratch_ptr + (size_t)rows * nchunk * (data2 ? 2 : 1);
+    hipError_t e = reduce_rows_launch(data, S, rows, ntraj, partial, s);
+    if (e != hipSuccess) return e;
+    if (data2) {
+        e = reduce_rows_launch(data2, S, rows, ntraj, partial2, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(reduce_rows_pass2, dim3(grid_for(rows, 64)), dim3(64), 0, s, (const double*)partial,
+                       (const double*)partial2, rows, nchunk, scale, out);
+    return hipGetLastError();
+}
+
+size_t reduce_noalloc_doubles(int32_t rows, int64_t ntraj, bool two_sources) {
+    return (size_t)rows * reduce_nchunk(ntraj) * (two_sources ? 2 : 1) + (size_t)rows;
 }
 
 __global__ __launch_bounds__(256) void zero_words(uint32_t* __restrict__ p, size_t n) {

@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "nqcd_mi355x_reduce", "nqcd_mi355x_download", "nqcd_mi355x_last_run_stats", "nqcd_mi355x_last_error",
     "nqcd_mi355x_surrogate_train", "nqcd_mi355x_surrogate_eval", "nqcd_mi355x_surrogate_gradient",
     "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params", "nqcd_mi355x_test_determinant",
-    "nqcd_mi355x_sample_fermi_dirac", "nqcd_mi355x_autotune",
+    "nqcd_mi355x_sample_fermi_dirac", "nqcd_mi355x_autotune", "nqcd_mi355x_reduce_noalloc",
 )
 
 
@@ -110,6 +110,7 @@ def lib():
     L.nqcd_mi355x_sample_fermi_dirac.argtypes = [H, C.c_double, ip, C.c_int32, C.c_int32, C.c_uint64]
     L.nqcd_mi355x_run.argtypes = [H, C.c_double, C.c_double, C.c_int32, C.POINTER(TermDesc), C.c_uint64]
     L.nqcd_mi355x_reduce.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
+    L.nqcd_mi355x_reduce_noalloc.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_download.argtypes = [H, C.c_int32, C.c_void_p, C.c_int64]
     L.nqcd_mi355x_last_run_stats.argtypes = [H, C.POINTER(RunStats)]
     L.nqcd_mi355x_autotune.argtypes = [H, C.c_double, C.c_double, C.c_uint64, C.POINTER(TuneResult)]
@@ -270,6 +271,12 @@ class Ensemble:
     def reduce(self, output_kind, reduction=RED_MEAN, length=None):
         out = np.zeros(length, dtype=np.float64)
         self._check(self._L.nqcd_mi355x_reduce(self._h, output_kind, reduction, _dp(out), length))
+        return out
+
+    def reduce_noalloc(self, output_kind, reduction=RED_MEAN, length=None):
+        """The same reduction through the entry point that allocates no device memory."""
+        out = np.zeros(length, dtype=np.float64)
+        self._check(self._L.nqcd_mi355x_reduce_noalloc(self._h, output_kind, reduction, _dp(out), length))
         return out
 
     def download(self, field):
