@@ -31,10 +31,14 @@ GRID = [-5.0, 5.0, 31]
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 HBM_PEAK_GBS = 8000.0
 # FP64 flops (an FMA counts 2) of one trajectory-step of the trace-fixed two-state formulation,
-# counted from nqcd_device.h / fssh2.hip and checked against the compiled sub-step loop (81 FMA,
-# 29 MUL, 3 ADD): 5 Tsit5 sub-steps x 194 = 970, plus ~75 for the eigen, the coupling rotation,
-# the interpolation set-up and the hop test.  The FP32 model and Verlet work are not counted.
+# counted from nqcd_device.h / fssh2.hip against the compiled loop: 4 Tsit5 sub-steps x 180 (63
+# stage-combination FMA, 6 right-hand sides, coefficient set-up) + 185 for the last, clipped
+# sub-step + 8 for the products with h formed once per step = 913, plus ~75 for the eigen, the
+# coupling rotation, the interpolation set-up and the hop test.  The FP32 model and Verlet work are
+# not counted.
 FLOP_PER_STEP = 988.0
+# Threads of the CPU baseline: the host share of one GPU on the measurement machine (16 cores)
+CPU_THREADS = 16
 
 
 def _ensemble(L, precision, schedule):
@@ -52,8 +56,21 @@ def _final_pops(L, E):
     return np.concatenate([fin, scat])
 
 
-def cpu_baseline(L, gpu_states, nthreads):
-    """FP64 oracle (CPU restatement) on a stratified sample of the same trajectories."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(L, gpu_states, nthreads, samples):
+    """FP64 CPU restatement of the reference algorithm (the oracle) on a stratified sample of the headline
+    trajectories, on `nthreads` host threads and on one core, and the error of every other reported GPU
+    result against the oracle on a sample of that workload (`samples`: per-workload GPU outputs and inputs)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     stride = 13  # every 13th trajectory id: covers every model and momentum of the grid
@@ -63,20 +80,106 @@ def cpu_baseline(L, gpu_states, nthreads):
                                KEY, traj_ids=ids)
     sig0 = np.zeros((len(ids), 2, 2), complex); sig0[:, 0, 0] = 1
     models = [O.make_model(k, 2, 1, p) for k, p in TULLY]
-    t0 = time.perf_counter()
-    out = O.run_ensemble(0, models, MASS, r0, v0, sig0, state0=np.zeros(len(ids), np.int32),
-                         model_id=(ids % 3).astype(np.int32), dt=1.0, nsteps=int(TMAX), key=KEY, term_x=XTERM,
-                         nthreads=nthreads, traj_ids=ids)
-    el = time.perf_counter() - t0
+
+    def run(sel, threads):
+        t0 = time.perf_counter()
+        out = O.run_ensemble(0, models, MASS, r0[sel], v0[sel], sig0[sel], state0=np.zeros(len(sel), np.int32),
+                             model_id=(ids[sel] % 3).astype(np.int32), dt=1.0, nsteps=int(TMAX), key=KEY, term_x=XTERM,
+                             nthreads=threads, traj_ids=ids[sel])
+        return out, time.perf_counter() - t0
+
+    out, el = run(np.arange(len(ids)), nthreads)
     states = out["state"]; steps = int(out["steps"].sum())
+    one = np.arange(0, len(ids), 16)  # a sixteenth of the sample on one core
+    out1, el1 = run(one, 1)
+    steps1 = int(out1["steps"].sum())
     agree = float(np.mean(states == gpu_states[ids]))
     pc = np.bincount(states, minlength=2) / len(ids)
     pg = np.bincount(gpu_states[ids], minlength=2) / len(ids)
-    return {"value": steps / el, "unit": "trajectory-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"{len(ids)} trajectories (every {stride}th id of the 1e6 ensemble, all lengths, "
-                      f"{steps} steps, {el:.1f} s) of the FP64 CPU restatement of the reference algorithm",
-            "pop_err_fp32_gpu_vs_fp64_oracle_sample": float(np.abs(pc - pg).max()),
-            "state_agreement_sample": agree}
+    res = {"value": steps / el, "unit": "trajectory-steps/s", "cores": nthreads, "kind": "port",
+           "sample": f"{len(ids)} trajectories (every {stride}th id of the 1e6 ensemble, all lengths, "
+                     f"{steps} steps, {el:.1f} s) of the FP64 CPU restatement of the reference algorithm",
+           "cpu_model": _cpu_model(), "host_cpus_visible": os.cpu_count(),
+           "one_core": {"value": steps1 / el1, "cores": 1,
+                        "sample": f"{len(one)} of those trajectories, {steps1} steps, {el1:.1f} s"},
+           "pop_err_fp32_gpu_vs_fp64_oracle_sample": float(np.abs(pc - pg).max()),
+           "pop_sigma_sample": float(np.sqrt(np.max(pc * (1 - pc)) / len(ids))),
+           "state_agreement_sample": agree}
+    checks = {}
+    for name, fn in (("spin_boson", _check_spin_boson), ("ring_polymer", _check_ring_polymer), ("iesh", _check_iesh),
+                     ("learned", _check_learned)):
+        if name in samples:
+            t0 = time.perf_counter()
+            checks[name] = fn(O, samples[name], nthreads)
+            checks[name]["oracle_s"] = time.perf_counter() - t0
+            _progress(f"oracle check {name}: {checks[name]}")
+    res["errors_vs_fp64_oracle"] = checks
+    return res
+
+
+def _check_spin_boson(O, S, nthreads):
+    """Diabatic population difference <sigma_z(t)> of the GPU run (FP32 bath) on its first trajectories
+    against the oracle on the same draws; max over the save slots, with the sample's statistical sigma."""
+    out = {}
+    m = O.make_model(O.MODEL_SPIN_BOSON, 2, len(S["w"]), [1.0, 1.0], omega=S["w"], cbath=S["c"])
+    n = S["n"]
+    r0, v0 = O.sample_gaussian(n, len(S["w"]), 0.0, S["sdr"][None, :], 0.0, S["sdv"][None, :], KEY)
+    sig0 = O.diabatic_to_adiabatic(m, r0, 0)
+    for name, method in (("ehrenfest", 1), ("fssh", 0)):
+        c = O.run_ensemble(method, m, 1.0, r0, v0, sig0, state0=None if method == 1 else np.full(n, -1, np.int32),
+                           dt=S["dt"], nsteps=S["nsteps"], save_every=S["save_every"], nsave=S["nsave"], key=KEY,
+                           nthreads=nthreads)
+        szc = c["pop_di"][:, :, 0] - c["pop_di"][:, :, 1]
+        szg = S[name][:, :, 0] - S[name][:, :, 1]
+        out[name] = {"trajectories": n, "max_abs_err_sigma_z": float(np.abs(szg.mean(0) - szc.mean(0)).max()),
+                     "sigma": float(szc.std(0).max() / np.sqrt(n))}
+    return out
+
+
+def _check_ring_polymer(O, S, nthreads):
+    """RPSH at P = 8 (FP32 beads, FP64 electronics) on its first ring polymers against the FP64 oracle from
+    the same PIMC initial beads: final adiabatic population and centroid error."""
+    m = O.make_model(O.MODEL_DW_GAUSSIAN, 2, 1, S["prm"])
+    n = len(S["r0"])
+    sig0 = np.zeros((n, 2, 2), complex); sig0[:, 0, 0] = 1
+    c = O.rp_run_ensemble(O.RP_FSSH, m, S["mass"], S["r0"], S["v0"], sig0, np.zeros(n, np.int32), dt=1.0,
+                          nsteps=S["nsteps"], temperature=S["T"], save_every=S["save_every"], nsave=S["nsave"], key=KEY,
+                          nthreads=nthreads)
+    pc = np.mean(c["state"] == 1)
+    pg = np.mean(S["state"] == 1)
+    return {"workload": "rpsh_P8", "ring_polymers": n, "excited_population": {"gpu": float(pg), "oracle": float(pc)},
+            "pop_error": float(abs(pg - pc)), "pop_sigma": float(np.sqrt(max(pc * (1 - pc), 1 / n) / n)),
+            "state_agreement": float(np.mean(c["state"] == S["state"])),
+            "max_centroid_err": float(np.abs(c["centroid"] - S["centroid"]).max())}
+
+
+def _check_iesh(O, S, nthreads):
+    """IESH at N = 41 (FP64 throughout) on its first trajectories against the oracle: impurity population
+    per save slot."""
+    md = O.na_model(M=S["M"])
+    n = S["n"]
+    r0, v0 = O.sample_gaussian(n, 1, 8.0, 1.0, 0.0, 1e-2, KEY)
+    occ0 = np.tile(np.arange(S["ne"], dtype=np.int32), (n, 1))
+    c = O.iesh_run_ensemble(0, md, S["ne"], 2000.0, r0[:, 0], v0[:, 0], occ0, dt=2.0, nsteps=S["nsteps"], key=KEY,
+                            save_every=S["save_every"], nthreads=nthreads)
+    return {"workload": f"iesh_N{S['M'] + 1}", "trajectories": n,
+            "max_abs_err_impurity_population": float(np.abs(c["pop_di"][:, :, 0] - S["pop_imp"]).max()),
+            "hop_sequences_equal": bool(np.array_equal(c["nhops"], S["nhops"]))}
+
+
+def _check_learned(O, S, nthreads):
+    """FSSH on the trained surrogate (and on the analytic model, FP32 nuclei) for the first trajectories of
+    the 1e6-trajectory diatom workload against the FP64 oracle on the analytic model."""
+    n = len(S["r0"])
+    m = O.make_model(O.MODEL_DIATOM_SURFACE, 2, 6, S["params"])
+    sig0 = np.zeros((n, 2, 2), complex); sig0[:, 0, 0] = 1
+    c = O.run_ensemble(0, m, np.repeat(S["mass"], 3), S["r0"], S["v0"], sig0, state0=np.zeros(n, np.int32), dt=2.0,
+                       nsteps=S["nsteps"], key=KEY, term_x=S["term"], term_dof=2, nthreads=nthreads)
+    pc = float(np.mean(c["state"] == 1))
+    ps, pa = float(np.mean(S["surrogate_state"] == 1)), float(np.mean(S["analytic_state"] == 1))
+    return {"trajectories": n, "excited_population": {"oracle": pc, "surrogate_gpu": ps, "analytic_gpu": pa},
+            "pop_error_surrogate": abs(ps - pc), "pop_error_analytic_fp32": abs(pa - pc),
+            "pop_sigma": float(np.sqrt(max(pc * (1 - pc), 1 / n) / n))}
 
 
 def _progress(msg):
@@ -88,6 +191,7 @@ def iesh_extra(L):
     states: trajectory-steps/s of the one-trajectory-per-workgroup kernel and its FP64 rate."""
     res = {}
     W = 6 * 6.4e-3 / 2
+    sample = None
     for M, ntraj, nsteps in ((40, 16384, 200), (100, 4096, 100)):
         E = L.Ensemble(0)
         E.set_problem(L.METHOD_IESH, M + 1, 1, 1, [2000.0], 2.0, algorithm=L.ALG_VERLET_ELECTRONICS, n_electrons=M // 2)
@@ -95,6 +199,10 @@ def iesh_extra(L):
         E.sample_initial(L.SAMPLER_GAUSSIAN, ntraj, [8.0, 1.0, 0.0, 1e-2], KEY)
         E.run(0.0, 2.0 * 10, KEY)
         st = E.run(0.0, 2.0 * nsteps, KEY, save_every=10)
+        if M == 40:  # the first 64 trajectories for the oracle check in the CPU-baseline leg
+            n = 64
+            sample = {"M": M, "ne": M // 2, "n": n, "nsteps": nsteps, "save_every": 10,
+                      "pop_imp": E.download(L.FIELD_SAVE_POP_DI)[:n, :, 0].copy(), "nhops": E.download(L.FIELD_NHOPS)[:n]}
         E.close()
         N, Ne = M + 1, M // 2
         flop = 8 * 4 * N * N * Ne + 16 * N * M + (8 / 3) * Ne ** 3  # Taylor (K ~ 8) + eigen + Gauss-Jordan
@@ -120,7 +228,7 @@ def iesh_extra(L):
     res[f"N{M + 1}_P{P}"] = {"ntraj": ntraj, "nsteps": nsteps, "nbeads": P, "kernel_ms": st["kernel_ms"],
                               "traj_steps_per_s": rate, "grid": st["grid"], "block": st["block"]}
     _progress(f"ring-polymer iesh N={M + 1} P={P}: {rate:.3e} trajectory-steps/s")
-    return res
+    return res, sample
 
 
 def spin_boson_extra(L):
@@ -137,6 +245,9 @@ def spin_boson_extra(L):
     prm = np.concatenate([np.zeros(nm), np.sqrt(1 / (2 * w * t)), np.zeros(nm), np.sqrt(w / (2 * t))])
     ntraj, dt, nsteps = 100000, 0.002, 10000
     res = {}
+    nsamp = 512  # first trajectories handed to the oracle check of the CPU-baseline leg
+    sample = {"w": w, "c": c, "sdr": np.sqrt(1 / (2 * w * t)), "sdv": np.sqrt(w / (2 * t)), "n": nsamp, "dt": dt,
+              "nsteps": nsteps, "save_every": 50, "nsave": nsteps // 50 + 1}
     for name, method in (("ehrenfest", L.METHOD_EHRENFEST), ("fssh", L.METHOD_FSSH)):
         E = L.Ensemble(0)
         E.set_problem(method, 2, 1, nm, np.ones(nm), dt, precision=L.PREC_FP32_NUCLEI)
@@ -144,21 +255,27 @@ def spin_boson_extra(L):
         E.sample_initial(L.SAMPLER_HARMONIC_WIGNER, ntraj, prm, KEY, initial_state=0, state_basis=1)
         st = E.run(0.0, nsteps * dt, KEY, save_every=50)
         E.reduce(L.OUT_POPULATION_CORRELATION, L.RED_MEAN, 4 * E.nsave)
+        sample[name] = E.download(L.FIELD_SAVE_POP_DI)[:nsamp].copy()
         E.close()
         rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
         res[name] = {"ntraj": ntraj, "nsteps": nsteps, "dt": dt, "kernel_ms": st["kernel_ms"],
                      "traj_steps_per_s": rate, "grid": st["grid"], "block": st["block"]}
         _progress(f"spin-boson {name}: {rate:.3e} trajectory-steps/s")
-    return res
+    return res, sample
 
 
 def ring_polymer_extra(L):
     """RPMD (one state) and ring-polymer surface hopping (two states) on the double well
-    V = x^4/1200 - x^2/40 with a Gaussian-coupled second diabat, beta = 8, thermal ring polymers
-    sampled on the device, 1e5 ring polymers for 1000 steps of dt = 1 at 8, 16, 32 and 64 beads;
-    FP32 beads, FP64 electronics.  Rates are ring-polymer steps/s (a ring polymer = one trajectory)."""
+    V = x^4/1200 - x^2/40 with a Gaussian-coupled second diabat, beta = 8, 1e5 ring polymers for 1000
+    steps of dt = 1 at 8, 16, 32 and 64 beads; FP32 beads, FP64 electronics.  Initial conditions from the
+    device path-integral Monte Carlo sampler (200 chain steps per ring polymer from x0 = -3.87, centroid
+    moves of 3 bohr, the free ring polymer's mode widths without the second test of the last proposal:
+    the discretised path-integral distribution).  Rates are ring-polymer steps/s (a ring polymer = one
+    trajectory)."""
     mass, beta, ntraj, nsteps = 1836.0, 8.0, 100000, 1000
-    res = {}
+    pimc = [-3.87, 3.0, 200, 1.0, 1, 0]
+    res = {"initial_conditions": "PIMC " + str(pimc)}
+    sample = None
     for name, method, nstates, prm in (("rpmd", L.METHOD_RPMD, 1, [1 / 1200, 1 / 40, 0.0, 0.0, 1.0]),
                                        ("rpsh", L.METHOD_RPSH, 2, [1 / 1200, 1 / 40, 0.01, 0.005, 1.0])):
         for P in (8, 16, 32, 64):
@@ -167,15 +284,27 @@ def ring_polymer_extra(L):
             E.set_problem(method, nstates, 1, 1, [mass], 1.0, algorithm=alg, nbeads=P, precision=L.PREC_FP32_NUCLEI,
                           temperature=1.0 / beta)
             E.set_model(0, L.MODEL_DW_GAUSSIAN, prm)
-            E.sample_initial(L.SAMPLER_RING_POLYMER_THERMAL, ntraj, [-3.87, np.sqrt(0.1 / mass), beta], KEY)
+            t0 = time.perf_counter()
+            E.sample_initial(L.SAMPLER_RING_POLYMER_PIMC, ntraj, pimc, KEY)
+            t_pimc = time.perf_counter() - t0
+            keep = name == "rpsh" and P == 8
+            if keep:  # the first ring polymers' initial beads for the oracle check of the CPU-baseline leg
+                E.run(0.0, 0.0, KEY)
+                n = 256
+                sample = {"prm": prm, "mass": mass, "T": 1.0 / beta, "nsteps": nsteps, "save_every": 100,
+                          "nsave": nsteps // 100 + 1, "r0": E.download(L.FIELD_R)[:n].copy(),
+                          "v0": E.download(L.FIELD_V)[:n].copy()}
             st = E.run(0.0, float(nsteps), KEY, save_every=100)
+            if keep:
+                sample["state"] = E.download(L.FIELD_STATE)[:n].copy()
+                sample["centroid"] = E.download(L.FIELD_SAVE_CENTROID)[:n].copy()
             E.close()
             rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
             res[f"{name}_P{P}"] = {"ntraj": ntraj, "nsteps": nsteps, "nbeads": P, "kernel_ms": st["kernel_ms"],
-                                   "traj_steps_per_s": rate, "bead_steps_per_s": rate * P,
+                                   "traj_steps_per_s": rate, "bead_steps_per_s": rate * P, "pimc_wall_s": t_pimc,
                                    "grid": st["grid"], "block": st["block"]}
             _progress(f"{name} P={P}: {rate:.3e} ring-polymer steps/s")
-    return res
+    return res, sample
 
 
 TRAIN_STEPS = 40000
@@ -209,7 +338,8 @@ def surrogate_extra(L):
         sig = np.zeros((len(r0), 2, 2), complex); sig[:, 0, 0] = 1
         E.load_initial(r0, v0, sig, state=np.zeros(len(r0), np.int32))
         st = E.run(0.0, 2.0 * 5000, KEY, save_every=save_every, term=(2, 8.0))
-        out = {"stats": st, "final": E.reduce(L.OUT_FINAL_ADIABATIC_POPULATION, L.RED_MEAN, 2)}
+        out = {"stats": st, "final": E.reduce(L.OUT_FINAL_ADIABATIC_POPULATION, L.RED_MEAN, 2),
+               "state": E.download(L.FIELD_STATE)}
         if save_every:
             out["configs"], out["steps"] = E.download(L.FIELD_SAVE_R), E.download(L.FIELD_STEPS)
         if model.kind == L.MODEL_MLP_SURROGATE:
@@ -228,6 +358,9 @@ def surrogate_extra(L):
     r1, v1 = inputs(1000000, 43)
     ref = run(analytic, r1, v1)
     sur = run(model, r1, v1)
+    n = 2000  # the first trajectories for the oracle check of the CPU-baseline leg
+    sample = {"r0": r1[:n].copy(), "v0": v1[:n].copy(), "params": analytic.params(), "mass": np.array(mass),
+              "nsteps": 5000, "term": 8.0, "surrogate_state": sur["state"][:n], "analytic_state": ref["state"][:n]}
     rate = lambda o: o["stats"]["traj_steps"] / (o["stats"]["kernel_ms"] / 1e3)
     p_ref, p_sur = float(ref["final"][1]), float(sur["final"][1])
     _progress(f"surrogate FSSH {rate(sur):.3e} / analytic {rate(ref):.3e} trajectory-steps/s")
@@ -244,7 +377,7 @@ def surrogate_extra(L):
             "pop_sigma": float(np.sqrt(p_ref * (1 - p_ref) / len(r1))),
             "fallback_fraction": sur["fallbacks"] / sur["stats"]["traj_steps"],
             "bf16_flop_per_traj_step": flop,
-            "achieved_bf16_TFLOPs": flop * rate(sur) / 1e12, "frac_bf16_peak": flop * rate(sur) / 1e12 / 2500.0}
+            "achieved_bf16_TFLOPs": flop * rate(sur) / 1e12, "frac_bf16_peak": flop * rate(sur) / 1e12 / 2500.0}, sample
 
 
 TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fssh2_persistent_c2_traffic.md")
@@ -291,8 +424,9 @@ def main():
 
     # --- batched: persistent work-queue kernel, FP32 nuclei / FP64 electronics -------------------
     E = _ensemble(L, L.PREC_FP32_NUCLEI, L.SCHED_PERSISTENT)
-    # launch shape of the persistent kernel chosen by the library's autotuner on 200 steps (untimed)
-    tuning = E.autotune(0.0, 200.0, KEY)
+    # launch shape of the persistent kernel chosen by the library's autotuner on whole passes with the
+    # termination predicate (untimed), so the choice includes the tail of uneven trajectory lengths
+    tuning = E.autotune(0.0, TMAX, KEY, term=(0, XTERM))
     _progress(f"autotune: block {tuning['block']}, workgroups/CU {tuning['workgroups_per_cu'] or 'max'} "
               f"({tuning['best_ms']:.2f} ms vs default {tuning['default_ms']:.2f} ms on the sample)")
     for _ in range(args.warmup):
@@ -359,6 +493,8 @@ def main():
                    "schedule": "persistent work-queue kernel", "grid": grid, "block": block},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload, scaled by a 1 GiB copy's "
+                                       "reading (profiles/fssh2_persistent_c2_traffic.md), not measured in this run",
                      "kernel": "fssh2_persistent<1,float>", "kernel_avg_ms": avg_kernel_ms,
                      "flop_per_traj_step": FLOP_PER_STEP,
                      "hbm_algorithmic_GBs": algo_bytes / (avg_kernel_ms / 1e3) / 1e9},
@@ -369,15 +505,16 @@ def main():
         "plain_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
     }
+    samples = {}
     if not args.no_extras:
-        out["iesh"] = iesh_extra(L)
-        out["spin_boson"] = spin_boson_extra(L)
-        out["ring_polymer"] = ring_polymer_extra(L)
-        out["learned"] = surrogate_extra(L)
+        out["iesh"], samples["iesh"] = iesh_extra(L)
+        out["spin_boson"], samples["spin_boson"] = spin_boson_extra(L)
+        out["ring_polymer"], samples["ring_polymer"] = ring_polymer_extra(L)
+        out["learned"], samples["learned"] = surrogate_extra(L)
     if not args.no_cpu:
-        nthreads = min(16, os.cpu_count() or 1)
-        _progress(f"cpu baseline on {nthreads} threads")
-        out["cpu_baseline"] = cpu_baseline(L, states32, nthreads)
+        nthreads = min(CPU_THREADS, os.cpu_count() or 1)
+        _progress(f"cpu baseline on {nthreads} threads and on one core")
+        out["cpu_baseline"] = cpu_baseline(L, states32, nthreads, samples)
     print(json.dumps(out))
 
 

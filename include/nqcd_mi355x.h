@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define NQCD_MI355X_ABI_VERSION 1
+#define NQCD_MI355X_ABI_VERSION 2
 
 typedef struct nqcd_handle nqcd_handle_t;
 
@@ -58,7 +58,10 @@ enum nqcd_method {
     NQCD_METHOD_EHRENFEST_NA = 3,
     NQCD_METHOD_RPMD = 4,
     NQCD_METHOD_RPSH = 5,
-    NQCD_METHOD_RP_EHRENFEST = 6
+    NQCD_METHOD_RP_EHRENFEST = 6,
+    NQCD_METHOD_CME = 7,          /* classical master equation, cme.jl:63-105 */
+    NQCD_METHOD_BCME = 8,         /* broadened CME, cme.jl:115-182 (bandwidth in nqcd_problem_desc) */
+    NQCD_METHOD_CLASSICAL = 9     /* Simulation{Classical}, classical.jl (single-state model, one coordinate) */
 };
 
 /* IntegrationAlgorithms (IntegrationAlgorithms.jl:19-56): BABwithTsit5, VerletwithElectronics,
@@ -68,7 +71,9 @@ enum nqcd_algorithm {
     NQCD_ALG_VERLET_ELECTRONICS = 1,
     NQCD_ALG_BCB = 2,
     NQCD_ALG_BCB_TSIT5 = 3,
-    NQCD_ALG_BCB_WAVEFUNCTION = 4    /* ring-polymer IESH / EhrenfestNA (bcb_wavefunction.jl) */
+    NQCD_ALG_BCB_WAVEFUNCTION = 4,   /* ring-polymer IESH / EhrenfestNA (bcb_wavefunction.jl) */
+    NQCD_ALG_TSIT5 = 5,              /* the nuclear ODE by Tsit5 at a fixed step (CME / BCME, IntegrationAlgorithms.jl:68) */
+    NQCD_ALG_VELOCITY_VERLET = 6     /* OrdinaryDiffEq's VelocityVerlet (Simulation{Classical}, IntegrationAlgorithms.jl:61) */
 };
 
 /* rescaling keyword of Simulation{FSSH} (fssh.jl:41, surface_hopping.jl:219,234-245). */
@@ -102,7 +107,18 @@ enum nqcd_model_kind {
     /* The MLP surrogate of the diatom-surface model (trained with nqcd_mi355x_surrogate_train):
      * params: fallback threshold (Hartree), then the 16 diatom-surface parameters of the analytic
      * model used wherever the two ensemble members disagree by more than the threshold. */
-    NQCD_MODEL_MLP_SURROGATE = 8
+    NQCD_MODEL_MLP_SURROGATE = 8,
+    /* n-state Morse model of one coordinate (Coker and Xiao, J. Chem. Phys. 102, 496 (1995); NQCModels'
+     * ThreeStateMorse for n = 3): params: n (2..8), then [D_i, beta_i, r_i, c_i] per state, then
+     * [A_ij, a_ij, r_ij] per pair i < j in row-major order:
+     * V_ii = D_i (1 - e^{-beta_i (x - r_i)})^2 + c_i,  V_ij = A_ij e^{-a_ij (x - r_ij)^2}.
+     * FSSH / Ehrenfest with n_states >= the slot's n; slots of different n (and two-state Tully / double-well
+     * slots) mix in one ensemble. */
+    NQCD_MODEL_MORSE_N = 9,
+    /* One-mode Anderson-Holstein impurity of the CME / BCME tests (test/Dynamics/cme.jl:27-55; Dou, Nitzan,
+     * Subotnik, J. Chem. Phys. 142, 084110 (2015)): params: k, Ed, c, V12 with V11 = k x^2/2,
+     * V22 = V11 + Ed + c x, V12 constant (hybridisation Gamma = 2 pi V12^2). */
+    NQCD_MODEL_ANDERSON_HOLSTEIN_1 = 10
 };
 
 /* IESH decoherence keyword (iesh.jl:72-73): none, or Granucci-Persico EDC
@@ -127,12 +143,23 @@ typedef struct {
     double temperature;          /* ring polymers: omega_n = nbeads * temperature (ring_polymer.jl:18) */
     double edc_C;                /* EDC constant C (Hartree); 0 -> 0.1 (decoherence_corrections.jl:14) */
     int32_t disable_estimate;    /* IESH: 1 = estimate_probability=false (iesh.jl:73,241-244) */
-    int32_t reserved[7];
+    int32_t reserved0;
+    double bandwidth;            /* BCME: bandwidth of the broadening integrals (cme.jl:130-133) */
+    double fermi_level;          /* CME / BCME: Fermi level mu of the metal (NQCModels.fermilevel) */
+    int32_t reserved[2];
 } nqcd_problem_desc;
 
 /* TerminatingCallback (DynamicsUtils/callbacks.jl:15-29) with a predicate the device evaluates after
- * every step (and at t0): |x_dof| > value, x_dof > value or x_dof < value.  FSSH / Ehrenfest only. */
+ * each step and not at t0, as a DiscreteCallback's condition (a trajectory that starts past the boundary
+ * takes one step): |x_dof| > value, x_dof > value or x_dof < value.  FSSH / Ehrenfest only. */
 enum nqcd_term_kind { NQCD_TERM_NONE = 0, NQCD_TERM_ABS_GT = 1, NQCD_TERM_GT = 2, NQCD_TERM_LT = 3 };
+
+/* Cell callbacks (DynamicsUtils/callbacks.jl:4-22) on the simulation cell of nqcd_mi355x_set_cell:
+ * CellBoundaryCallback wraps the atoms into the cell after a step in which one left it
+ * (enforce_periodicity!), TerminateCellCallback ends the trajectory after such a step.  Both follow the
+ * hop callback, as the problem's callbacks precede the user's; a TerminatingCallback predicate is
+ * evaluated after the wrap. */
+enum nqcd_cell_mode { NQCD_CELL_NONE = 0, NQCD_CELL_BOUNDARY = 1, NQCD_CELL_TERMINATE = 2 };
 typedef struct {
     int32_t kind;
     int32_t dof;                 /* flattened (ndofs, natoms) index */
@@ -154,7 +181,18 @@ enum nqcd_sampler_kind {
      * normal mode k ~ N(sqrt(P) x0 delta_k0, 1/(beta_P m (w_k^2 + w_well^2))), velocities
      * ~ N(0, 1/(beta_P m)), beta_P = beta/P (stands in for the PIMC sampler of
      * src/InitialConditions/MetropolisHastings.jl). */
-    NQCD_SAMPLER_RING_POLYMER_THERMAL = 3
+    NQCD_SAMPLER_RING_POLYMER_THERMAL = 3,
+    /* Path-integral Monte Carlo (run_monte_carlo_sampling for a RingPolymerSimulation,
+     * MetropolisHastings.jl:131-148,170-258): one Metropolis-Hastings chain per ring polymer started with
+     * every bead at x0, `passes` steps of a centroid move (beads shifted by (u - 1/2) Delta / sqrt(P)),
+     * P / L normal-mode moves (L = max(1, round(segment P)) modes in 1..P-1 resampled from N(0, sigma_k)),
+     * each tested against min(1, exp(-(E' - E)/(P T))) with E the bead-summed potential (element [1] of the
+     * model matrix), then the reference's second test of the last proposal; velocities ~ N(0, sqrt(P T/m)).
+     * params = [x0, Delta, passes, segment(, width(, reassess))]: width 0 (default) takes the reference's
+     * sigma_k = sqrt(w_n/m)/(m w_k^2/2), 1 the free ring polymer's sqrt(w_n/m)/w_k; reassess 1 (default)
+     * keeps the second test, 0 drops it (the chain then samples the path-integral distribution exactly
+     * with width 1).  Random numbers: Philox stream 4. */
+    NQCD_SAMPLER_RING_POLYMER_PIMC = 4
 };
 typedef struct {
     int32_t kind;
@@ -175,8 +213,13 @@ enum nqcd_output_kind {
     NQCD_OUT_FINAL_ADIABATIC_POPULATION = 4, /* final, nstates */
     NQCD_OUT_CENTROID_POSITION = 5,      /* per save slot, ring-polymer centroid (OutputCentroidPosition, DynamicsOutputs.jl:73-79) */
     NQCD_OUT_KINETIC_ENERGY = 6,         /* per save slot (OutputKineticEnergy, DynamicsOutputs.jl:97) */
-    NQCD_OUT_TOTAL_ENERGY = 7            /* per save slot, kinetic + classical_potential_energy (OutputTotalEnergy,
+    NQCD_OUT_TOTAL_ENERGY = 7,           /* per save slot, kinetic + classical_potential_energy (OutputTotalEnergy,
                                             DynamicsOutputs.jl:90; fssh.jl:150-154, ehrenfest.jl:85-95) */
+    NQCD_OUT_SCATTERING_1D_DIABATIC = 8, /* final, [reflection(n), transmission(n)] of the diabatic populations
+                                            (OutputStateResolvedScattering1D(sim, :diabatic)); two-state one-dof */
+    NQCD_OUT_POSITION = 9,               /* per save slot, the coordinate (OutputPosition, DynamicsOutputs.jl:62-66);
+                                            CME / BCME / classical */
+    NQCD_OUT_VELOCITY = 10               /* per save slot, the velocity (OutputVelocity, DynamicsOutputs.jl:35-39); classical */
 };
 /* IESH: DIABATIC_POPULATION is iesh_diabatic_population (iesh.jl:337-369), state 0 = impurity;
  * EhrenfestNA (which the reference gives no diabatic estimator) reports sum_e |(U psi)_{a e}|^2. */
@@ -199,7 +242,10 @@ enum nqcd_field {
     NQCD_FIELD_FALLBACKS = 11,     /* int32 (ntraj) surrogate steps that fell back to the analytic model */
     NQCD_FIELD_SAVE_R = 12,        /* double (ndofs * natoms, nsave, ntraj) configuration per save (diatom-surface) */
     NQCD_FIELD_SAVE_EKIN = 13,     /* double (nsave, ntraj) kinetic energy per save (two-state one-dof FSSH / Ehrenfest) */
-    NQCD_FIELD_SAVE_EPOT = 14      /* double (nsave, ntraj) classical potential energy per save (same problems) */
+    NQCD_FIELD_SAVE_EPOT = 14,     /* double (nsave, ntraj) classical potential energy per save (same problems) */
+    NQCD_FIELD_FINAL_POP_DI = 15,  /* double (n, ntraj) diabatic populations of the final state (fssh.jl:132-142,
+                                      ehrenfest.jl:75-83); two-state one-dof FSSH / Ehrenfest */
+    NQCD_FIELD_SAVE_V = 16         /* double (nsave, ntraj) velocity per save (classical dynamics) */
 };
 /* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
  * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD / SAVE_POP_DI (n, nsave, ntraj). */
@@ -227,6 +273,17 @@ void nqcd_mi355x_close(nqcd_handle_t* h);
  * (simulations.jl:12-73,86-94), the method (fssh.jl:23-45 etc.) and the algorithm chosen
  * by select_algorithm (IntegrationAlgorithms.jl:65-103). */
 int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* desc);
+
+/* The simulation cell and the cell callback of the following runs (NQCBase's PeriodicCell, the `cell`
+ * keyword of Simulation, simulations.jl:12-44; callbacks.jl:4-22).  `vectors` holds the ndofs x ndofs
+ * cell matrix column-major with the lattice vectors as columns (PeriodicCell(vectors)), `periodic`
+ * ndofs flags (NULL: all periodic).  An atom is outside when a fractional coordinate f = vectors^-1 x
+ * is below 0 or above 1 (check_atoms_in_cell); the wrap maps f_j to mod(f_j, 1) on the periodic
+ * dimensions of every atom and sets x = vectors f (apply_cell_boundaries!).  mode NQCD_CELL_NONE
+ * clears the cell (InfiniteCell).  Supported by the one-coordinate FSSH / Ehrenfest (two and n
+ * states), CME / BCME and classical problems (ndofs = 1) and the diatom-surface problem (ndofs = 3,
+ * two atoms); set_problem clears it. */
+int32_t nqcd_mi355x_set_cell(nqcd_handle_t* h, int32_t mode, int32_t ndofs, const double* vectors, const int32_t* periodic);
 
 /* Set model slot `slot` (0..3).  Replaces the NQCModels model held in sim.cache
  * (Simulation cache, simulations.jl:28-30); several slots let one ensemble mix models,
@@ -286,7 +343,7 @@ int32_t nqcd_mi355x_reduce(nqcd_handle_t* h, int32_t output_kind, int32_t reduct
 int32_t nqcd_mi355x_reduce_noalloc(nqcd_handle_t* h, int32_t output_kind, int32_t reduction_kind, double* out, int64_t len);
 
 /* Copy a per-trajectory field to host memory (SortByTrajectoryReduction / EnsembleSaver data,
- * Ensembles.jl:39-63, reductions.jl:208-209).  bytes must equal the field size. */
+ * Ensembles.jl:39-63, reductions.jl:54-55).  bytes must equal the field size. */
 int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes);
 
 /* Statistics of the last nqcd_mi355x_run (device time from HIP events on the handle's stream).
@@ -304,7 +361,8 @@ typedef struct {
 
 /* Autotuner of the persistent work-queue kernels (no counterpart in the reference; the natural
  * caller is the one-step precompile_dynamics pre-run of run_dynamics.jl:100-120).  Runs the
- * loaded problem over [t0, t1] (no saves, no termination) once per candidate launch shape --
+ * loaded problem over [t0, t1] (no saves; with `term` when given, which makes the sample include
+ * the tail of uneven trajectory lengths) once per candidate launch shape --
  * 256, 128 and 64 threads per workgroup, each at the occupancy limit and at 3/4 and 1/2 of it --
  * keeps the fastest (the default unless a candidate is faster by more than 2%) in a process-wide
  * table keyed by kernel family, state count, precision and compute-unit count, and later runs
@@ -312,7 +370,8 @@ typedef struct {
  * of a run do not depend on the shape.  The sample runs overwrite the handle's results; call
  * nqcd_mi355x_run afterwards.  Two-state one-dof (persistent schedule) and analytic
  * diatom-surface problems only; NQCD_ERR_UNSUPPORTED otherwise. */
-int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, uint64_t philox_key, nqcd_tune_result* out);
+int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, const nqcd_term_desc* term, uint64_t philox_key,
+                             nqcd_tune_result* out);
 
 /* ---- MLP surrogate of the diabatic Hamiltonian (no counterpart in the reference) -------------
  * A 6 -> 128 -> 128 -> 3 tanh network for (H11, H22, H12) of a six-coordinate two-state model,
@@ -346,6 +405,12 @@ int32_t nqcd_mi355x_surrogate_get_params(nqcd_handle_t* h, int32_t member, float
  * Im det, 1 (0 when a pivot column is zero). */
 int32_t nqcd_mi355x_test_determinant(nqcd_handle_t* h, const double* re, const double* im, int32_t n, int32_t nmat,
                                      double* out);
+
+/* Test hook (no counterpart in the reference): the BCME broadening integrals n(h) = int A f and
+ * n2(h) = int (e - h) A f over [mu - W/2, mu + W/2] (evaluate_broadening / evaluate_force_broadening,
+ * cme.jl:167-182) by the device routine behind the BCME force, for count values of h. */
+int32_t nqcd_mi355x_test_broadening(nqcd_handle_t* h, const double* hs, int64_t count, double mu, double kT, double Gamma,
+                                    double W, double* n, double* n2);
 
 /* Message for the last failed call on h (thread-local when h is NULL). */
 const char* nqcd_mi355x_last_error(nqcd_handle_t* h);

@@ -22,6 +22,9 @@
 #include "ring_polymer.h"
 #include "iesh.h"
 #include "fssh_nd.h"
+#include "fssh_n.h"
+#include "classical.h"
+#include "cme.h"
 #include "mlp_host.h"
 #include "autotune.h"
 
@@ -42,7 +45,7 @@ namespace {
 
 bool is_two_state_1d(const nqcd_handle* h) {
     return (h->prob.method == NQCD_METHOD_FSSH || h->prob.method == NQCD_METHOD_EHRENFEST) &&
-           h->prob.algorithm == NQCD_ALG_BAB_TSIT5 && h->prob.n_states == 2 && h->D == 1;
+           h->prob.algorithm == NQCD_ALG_BAB_TSIT5 && h->prob.n_states == 2 && h->D == 1 && !ns_is_problem(h);
 }
 
 bool is_spin_boson(const nqcd_handle* h) {
@@ -95,6 +98,7 @@ int32_t alloc_results(nqcd_handle* h) {
     HIPCHK(h, h->save_epot.ensure(sizeof(double) * h->nsave * n));
     HIPCHK(h, launch_zero(h->save_ekin.p, sizeof(double) * h->nsave * n, h->stream));
     HIPCHK(h, launch_zero(h->save_epot.p, sizeof(double) * h->nsave * n, h->stream));
+    HIPCHK(h, h->final_di.ensure(sizeof(double) * 2 * n));
     // constant tables read by the kernels (models per slot, 1/m per flattened dof)
     HIPCHK(h, h->models_dev.ensure(sizeof(ModelSlot) * kMaxModels));
     HIPCHK(h, h->inv_mass_dev.ensure(sizeof(double) * D));
@@ -114,6 +118,7 @@ Fssh2Args make_args(nqcd_handle* h, double t0, int64_t nsteps, const nqcd_term_d
     A.term_kind = term ? term->kind : 0;
     A.term_dof = term ? term->dof : 0;
     A.term_value = term ? term->value : 0.0;
+    A.cell = h->cell;
     A.dt = h->prob.dt;
     A.t0 = t0;
     A.nsteps = nsteps;
@@ -132,6 +137,7 @@ Fssh2Args make_args(nqcd_handle* h, double t0, int64_t nsteps, const nqcd_term_d
     A.save_pop_ad = h->save_ad.as<double>(); A.save_pop_di = h->save_di.as<double>();
     A.save_state = h->save_state.as<int32_t>();
     A.save_ekin = h->save_ekin.as<double>(); A.save_epot = h->save_epot.as<double>();
+    A.final_pop_di = h->final_di.as<double>();
     A.queue = h->queue.as<unsigned long long>();
     return A;
 }
@@ -141,6 +147,14 @@ Fssh2Args make_args(nqcd_handle* h, double t0, int64_t nsteps, const nqcd_term_d
 extern "C" {
 
 int32_t nqcd_mi355x_abi_version(void) { return NQCD_MI355X_ABI_VERSION; }
+
+int32_t nqcd_mi355x_test_broadening(nqcd_handle_t* h, const double* hs, int64_t count, double mu, double kT, double Gamma,
+                                    double W, double* n, double* n2) {
+    if (!h || !hs || !n || !n2 || count < 1) return fail(h, NQCD_ERR_ARG, "null argument or empty list");
+    if (!(kT > 0.0) || !(Gamma > 0.0) || !(W > 0.0)) return fail(h, NQCD_ERR_ARG, "kT, Gamma and W must be positive");
+    HIPCHK(h, hipSetDevice(h->device));
+    return cme_test_broadening(h, hs, count, mu, kT, Gamma, W, n, n2);
+}
 
 const char* nqcd_mi355x_last_error(nqcd_handle_t* h) { return h ? h->err.c_str() : g_tls_error.c_str(); }
 
@@ -181,7 +195,8 @@ void nqcd_mi355x_close(nqcd_handle_t* h) {
                       &h->r0, &h->v0, &h->sig0, &h->state0, &h->model_id, &h->order, &h->models_dev, &h->bath, &h->inv_mass_dev, &h->args_dev,
                       &h->r, &h->v, &h->sig, &h->state, &h->steps, &h->nhops, &h->save_ad, &h->save_di,
                       &h->save_state, &h->queue, &h->plainS, &h->plainSi, &h->tmp, &h->partial, &h->small, &h->save_centroid, &h->sum64,
-                      &h->save_ekin, &h->save_epot};
+                      &h->save_ekin, &h->save_epot, &h->ns_tab_dev, &h->final_di, &h->save_v, &h->susp_f[0],
+                      &h->susp_f[1], &h->susp_tid[0], &h->susp_tid[1], &h->susp_i[0], &h->susp_i[1], &h->susp_ctr};
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(h->ev0);
     (void)hipEventDestroy(h->ev1);
@@ -201,11 +216,17 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     const bool sh_or_eh = d->method == NQCD_METHOD_FSSH || d->method == NQCD_METHOD_EHRENFEST;
     const bool rp = d->method == NQCD_METHOD_RPMD || d->method == NQCD_METHOD_RPSH || d->method == NQCD_METHOD_RP_EHRENFEST;
     const bool ie = d->method == NQCD_METHOD_IESH || d->method == NQCD_METHOD_EHRENFEST_NA;
-    if (!rp && !ie && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states == 2))
+    const bool cme = d->method == NQCD_METHOD_CME || d->method == NQCD_METHOD_BCME;
+    const bool cls = d->method == NQCD_METHOD_CLASSICAL;
+    if (!rp && !ie && !cme && !cls && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states >= 2 &&
+                        d->n_states <= kNsMaxStates))
         return fail(h, NQCD_ERR_UNSUPPORTED,
-                    "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (two states, BABwithTsit5), "
-                    "Simulation{AdiabaticIESH|EhrenfestNA} (VerletwithElectronics) and "
+                    "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (2 to 8 states, BABwithTsit5), "
+                    "Simulation{AdiabaticIESH|EhrenfestNA} (VerletwithElectronics), Simulation{CME|BCME}, "
+                    "Simulation{Classical} (VelocityVerlet) and "
                     "RingPolymerSimulation{Classical|FSSH|Ehrenfest} (BCB / BCBwithTsit5)");
+    if (sh_or_eh && d->n_states > 2 && d->ndofs * d->natoms != 1)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "FSSH / Ehrenfest with more than two states: one nuclear coordinate");
     h->prob = *d;
     h->masses.assign(d->masses, d->masses + d->natoms);
     for (double m : h->masses)
@@ -222,8 +243,60 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
         int32_t rc = ie_check_problem(h);
         if (rc != NQCD_OK) return rc;
     }
+    if (cme) {
+        int32_t rc = cme_check_problem(h);
+        if (rc != NQCD_OK) return rc;
+    }
+    if (cls) {
+        int32_t rc = cls_check_problem(h);
+        if (rc != NQCD_OK) return rc;
+    }
+    h->cell = CellDesc{};
     h->have_problem = true;
     h->have_initial = h->have_results = false;
+    return NQCD_OK;
+}
+
+int32_t nqcd_mi355x_set_cell(nqcd_handle_t* h, int32_t mode, int32_t ndofs, const double* vectors, const int32_t* periodic) {
+    if (!h) return fail(h, NQCD_ERR_ARG, "null handle");
+    if (!h->have_problem) return fail(h, NQCD_ERR_STATE, "set_problem must be called first");
+    if (mode == NQCD_CELL_NONE) { h->cell = CellDesc{}; return NQCD_OK; }
+    if (mode != NQCD_CELL_BOUNDARY && mode != NQCD_CELL_TERMINATE) return fail(h, NQCD_ERR_ARG, "unknown cell mode");
+    if (ndofs != h->prob.ndofs || ndofs < 1 || ndofs > 3)
+        return fail(h, NQCD_ERR_ARG, "the cell has ndofs x ndofs vectors (ndofs of the problem, at most 3)");
+    if (!vectors) return fail(h, NQCD_ERR_ARG, "vectors is null");
+    CellDesc c{};
+    c.mode = mode;
+    c.nd = ndofs;
+    for (int j = 0; j < ndofs; ++j) c.per[j] = periodic ? (periodic[j] != 0) : 1;
+    for (int j = 0; j < ndofs; ++j)
+        for (int k = 0; k < ndofs; ++k) c.a[j + 3 * k] = vectors[j + ndofs * k];
+    // inverse by cofactors (the cell matrix is at most 3 x 3)
+    double det;
+    if (ndofs == 1) {
+        det = c.a[0];
+        if (det != 0.0) c.inv[0] = 1.0 / det;
+    } else if (ndofs == 2) {
+        det = c.a[0] * c.a[4] - c.a[3] * c.a[1];
+        if (det != 0.0) { c.inv[0] = c.a[4] / det; c.inv[3] = -c.a[3] / det; c.inv[1] = -c.a[1] / det; c.inv[4] = c.a[0] / det; }
+    } else {
+        auto A = [&](int i, int j) { return c.a[i + 3 * j]; };
+        const double c00 = A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1), c01 = A(1, 2) * A(2, 0) - A(1, 0) * A(2, 2),
+                     c02 = A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0);
+        det = A(0, 0) * c00 + A(0, 1) * c01 + A(0, 2) * c02;
+        if (det != 0.0) {
+            // inv(i, j) = cofactor(j, i) / det
+            c.inv[0 + 3 * 0] = c00 / det; c.inv[1 + 3 * 0] = c01 / det; c.inv[2 + 3 * 0] = c02 / det;
+            c.inv[0 + 3 * 1] = (A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2)) / det;
+            c.inv[1 + 3 * 1] = (A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0)) / det;
+            c.inv[2 + 3 * 1] = (A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1)) / det;
+            c.inv[0 + 3 * 2] = (A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1)) / det;
+            c.inv[1 + 3 * 2] = (A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2)) / det;
+            c.inv[2 + 3 * 2] = (A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0)) / det;
+        }
+    }
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return fail(h, NQCD_ERR_ARG, "the cell vectors are singular");
+    h->cell = c;
     return NQCD_OK;
 }
 
@@ -240,7 +313,9 @@ int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t kind, cons
     case NQCD_MODEL_TULLY3: need = 3; break;
     case NQCD_MODEL_DW_GAUSSIAN: need = 5; break;
     case NQCD_MODEL_HARMONIC: need = 3; break;
+    case NQCD_MODEL_ANDERSON_HOLSTEIN_1: need = 4; break;
     case NQCD_MODEL_NEWNS_ANDERSON: return ie_set_model(h, slot, params, n);
+    case NQCD_MODEL_MORSE_N: return ns_set_model(h, slot, params, n);
     case NQCD_MODEL_DIATOM_SURFACE:
     case NQCD_MODEL_MLP_SURROGATE:
         if (slot != 0) return fail(h, NQCD_ERR_UNSUPPORTED, "the diatom-surface / surrogate model must use slot 0");
@@ -279,8 +354,16 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
         if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
         return rc;
     }
-    if (!r || !v || ((!sig_re || !sig_im) && h->prob.method != NQCD_METHOD_RPMD))
+    if (!r || !v || ((!sig_re || !sig_im) && h->prob.method != NQCD_METHOD_RPMD && !cme_is_problem(h) &&
+                     !cls_is_problem(h)))
         return fail(h, NQCD_ERR_ARG, "null input array");
+    if (cls_is_problem(h)) {
+        if (!h->model_set[0]) return fail(h, NQCD_ERR_STATE, "model slot 0 is not set");
+        HIPCHK(h, hipSetDevice(h->device));
+        int32_t rc = cls_load_initial(h, ntraj, r, v);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     const int D = h->D, n = h->prob.n_states;
     if (model_id)
         for (int64_t i = 0; i < ntraj; ++i)
@@ -290,6 +373,19 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
     if (state)
         for (int64_t i = 0; i < ntraj; ++i)
             if (state[i] < -1 || state[i] >= n) return fail(h, NQCD_ERR_ARG, "state out of range");
+    if (cme_is_problem(h)) {
+        if (!r || !v) return fail(h, NQCD_ERR_ARG, "null input array");
+        HIPCHK(h, hipSetDevice(h->device));
+        int32_t rc = cme_load_initial(h, ntraj, r, v, state);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
+    if (ns_is_problem(h)) {
+        HIPCHK(h, hipSetDevice(h->device));
+        int32_t rc = ns_load_initial(h, ntraj, r, v, sig_re, sig_im, state, model_id);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     if (is_rp(h)) {
         HIPCHK(h, hipSetDevice(h->device));
         int32_t rc = rp_load_initial(h, ntraj, r, v, sig_re, sig_im, state);
@@ -377,6 +473,21 @@ int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S,
         if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = true; h->have_results = false; }
         return rc;
     }
+    if (cls_is_problem(h)) {
+        int32_t rc = cls_sample_initial(h, S, seed);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
+    if (cme_is_problem(h)) {
+        int32_t rc = cme_sample_initial(h, S, seed);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
+    if (ns_is_problem(h)) {
+        int32_t rc = ns_sample_initial(h, S, seed);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     if (!is_two_state_1d(h) && !nd_is_problem(h))
         return fail(h, NQCD_ERR_UNSUPPORTED, "sampler needs a two-state one-dof or diatom-surface problem");
     if (nd_is_problem(h) && (S->kind != NQCD_SAMPLER_GAUSSIAN || S->state_basis != 0))
@@ -455,9 +566,23 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
     h->save_every = save_every;
     h->nsave = (save_every > 0) ? (int32_t)(nsteps / save_every + 1) : 1;
     HIPCHK(h, hipSetDevice(h->device));
+    if (h->cell.mode != NQCD_CELL_NONE) {
+        const bool one = h->cell.nd == 1 && h->D == 1 &&
+                         (cls_is_problem(h) || cme_is_problem(h) || ns_is_problem(h) ||
+                          (is_two_state_1d(h) && !is_rp(h) && !is_ie(h) && !is_spin_boson(h)));
+        const bool diatom = nd_is_problem(h) && h->cell.nd == 3;
+        if (!one && !diatom)
+            return fail(h, NQCD_ERR_UNSUPPORTED, "cell callbacks: one-coordinate FSSH / Ehrenfest, CME / BCME and classical "
+                                                 "problems, and the diatom-surface problem (3 x 3 cell)");
+    }
     const auto wall0 = std::chrono::steady_clock::now();
     h->stats = nqcd_run_stats{};
-    if (is_rp(h)) {
+    if (cls_is_problem(h)) {
+        if (term && term->kind != NQCD_TERM_NONE && term->dof != 0)
+            return fail(h, NQCD_ERR_ARG, "termination dof out of range");
+        int32_t rc = cls_run(h, t0, nsteps, term);
+        if (rc != NQCD_OK) return rc;
+    } else if (is_rp(h)) {
         int32_t rc = rp_run(h, t0, nsteps, term, key);
         if (rc != NQCD_OK) return rc;
     } else if (is_ie(h)) {
@@ -465,6 +590,18 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
         if (rc != NQCD_OK) return rc;
     } else if (nd_is_problem(h)) {
         int32_t rc = nd_run(h, t0, nsteps, term, key);
+        if (rc != NQCD_OK) return rc;
+    } else if (cme_is_problem(h)) {
+        if (term && term->kind != NQCD_TERM_NONE && term->dof != 0)
+            return fail(h, NQCD_ERR_ARG, "termination dof out of range");
+        int32_t rc = cme_run(h, t0, nsteps, term, key);
+        if (rc != NQCD_OK) return rc;
+    } else if (ns_is_problem(h)) {
+        if (h->prob.schedule != NQCD_SCHED_PERSISTENT)
+            return fail(h, NQCD_ERR_UNSUPPORTED, "n-state FSSH / Ehrenfest run on the persistent schedule");
+        if (term && term->kind != NQCD_TERM_NONE && term->dof != 0)
+            return fail(h, NQCD_ERR_ARG, "termination dof out of range");
+        int32_t rc = ns_run(h, t0, nsteps, term, key);
         if (rc != NQCD_OK) return rc;
     } else if (is_spin_boson(h)) {
         if (!h->sb_initial) return fail(h, NQCD_ERR_STATE, "spin-boson runs need nqcd_mi355x_sample_initial");
@@ -480,10 +617,28 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
             HIPCHK(h, fssh2_persistent_occupancy(h->prob.precision, block, &bpc));
             const int grid = (int)persistent_grid(bpc, sh, h->cu_count, h->ntraj);
             HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-            HIPCHK(h, h->args_dev.ensure(sizeof(Fssh2Args)));
-            HIPCHK(h, fssh2_launch_persistent(A, h->args_dev.as<Fssh2Args>(), h->prob.precision, grid, block, h->stream));
+            HIPCHK(h, h->args_dev.ensure(sizeof(Fssh2Args) * kSuspRounds));
+            const int64_t cap = (int64_t)grid * block;
+            SuspSet sets[2];
+            HIPCHK(h, h->susp_ctr.ensure(sizeof(unsigned long long) * 4));
+            for (int b = 0; b < 2; ++b) {
+                HIPCHK(h, h->susp_f[b].ensure(sizeof(double) * kSuspRows * cap));
+                HIPCHK(h, h->susp_tid[b].ensure(sizeof(int64_t) * cap));
+                HIPCHK(h, h->susp_i[b].ensure(sizeof(int32_t) * 3 * cap));
+                sets[b].f = h->susp_f[b].as<double>();
+                sets[b].tid = h->susp_tid[b].as<int64_t>();
+                sets[b].i = h->susp_i[b].as<int32_t>();
+                sets[b].count = h->susp_ctr.as<unsigned long long>() + 2 * b;
+                sets[b].claim = h->susp_ctr.as<unsigned long long>() + 2 * b + 1;
+                sets[b].cap = cap;
+            }
+            A.prio_span = fssh2_prio_span(cap);
+            HIPCHK(h, fssh2_launch_persistent(A, h->args_dev.as<Fssh2Args>(), h->prob.precision, grid, block, h->stream,
+                                              sets));
             HIPCHK(h, hipEventRecord(h->ev1, h->stream));
-            h->stats.block = block; h->stats.grid = grid; h->stats.launches = 2;
+            int rounds = 1, thr = 0;
+            fssh2_suspend_policy(&rounds, &thr);
+            h->stats.block = block; h->stats.grid = grid; h->stats.launches = 2 * rounds;
         } else {
             if (h->prob.precision != NQCD_PREC_FP64)
                 return fail(h, NQCD_ERR_UNSUPPORTED, "the step-per-launch schedule runs in FP64 only");
@@ -562,6 +717,26 @@ int32_t reduce_plan(nqcd_handle_t* h, int32_t kind, int32_t red, const double* o
         else if (kind == NQCD_OUT_DIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_di.as<double>(); }
         else return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA outputs: ADIABATIC_POPULATION, DIABATIC_POPULATION");
     }
+    if (cme_is_problem(h) || cls_is_problem(h)) {
+        const bool cls = cls_is_problem(h);
+        if (kind == NQCD_OUT_KINETIC_ENERGY) { rows = h->nsave; src = h->save_ekin.as<double>(); }
+        else if (kind == NQCD_OUT_TOTAL_ENERGY && h->prob.method != NQCD_METHOD_BCME) { rows = h->nsave; src = h->save_ekin.as<double>(); }
+        else if (kind == NQCD_OUT_POSITION) { rows = h->nsave; src = h->save_centroid.as<double>(); }
+        else if (kind == NQCD_OUT_VELOCITY && cls) { rows = h->nsave; src = h->save_v.as<double>(); }
+        else return fail(h, NQCD_ERR_UNSUPPORTED, cls ? "classical outputs: KINETIC_ENERGY, TOTAL_ENERGY, POSITION, VELOCITY"
+                                                      : "CME outputs: KINETIC_ENERGY, TOTAL_ENERGY (CME), POSITION");
+        if (len != rows) return fail(h, NQCD_ERR_ARG, "len does not match the output size");
+        P = ReducePlan{};
+        P.rows = rows;
+        P.src = src;
+        P.src2 = kind == NQCD_OUT_TOTAL_ENERGY ? h->save_epot.as<double>() : nullptr;
+        P.scale = (red == NQCD_RED_MEAN) ? 1.0 / (double)nt : 1.0;
+        return NQCD_OK;
+    }
+    if (kind == NQCD_OUT_POSITION || kind == NQCD_OUT_VELOCITY)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "position / velocity outputs: CME / BCME and classical problems");
+    if (ns_is_problem(h) && kind != NQCD_OUT_ADIABATIC_POPULATION && kind != NQCD_OUT_DIABATIC_POPULATION)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "n-state FSSH / Ehrenfest outputs: ADIABATIC_POPULATION, DIABATIC_POPULATION");
     const bool energy = kind == NQCD_OUT_KINETIC_ENERGY || kind == NQCD_OUT_TOTAL_ENERGY;
     if (energy && (is_rp(h) || nd_is_problem(h) || is_spin_boson(h) || !is_two_state_1d(h)))
         return fail(h, NQCD_ERR_UNSUPPORTED, "energy outputs exist for two-state one-dof FSSH / Ehrenfest");
@@ -572,6 +747,10 @@ int32_t reduce_plan(nqcd_handle_t* h, int32_t kind, int32_t red, const double* o
     case NQCD_OUT_DIABATIC_POPULATION: rows = n * h->nsave; src = h->save_di.as<double>(); break;
     case NQCD_OUT_FINAL_ADIABATIC_POPULATION: rows = n; break;
     case NQCD_OUT_SCATTERING_1D: rows = 2 * n; break;
+    case NQCD_OUT_SCATTERING_1D_DIABATIC:
+        if (!is_two_state_1d(h))
+            return fail(h, NQCD_ERR_UNSUPPORTED, "diabatic final populations exist for two-state one-dof FSSH / Ehrenfest");
+        rows = 2 * n; break;
     case NQCD_OUT_POPULATION_CORRELATION: rows = n * n * h->nsave; break;
     case NQCD_OUT_CENTROID_POSITION:
         if (!is_rp(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "centroid output needs a ring-polymer problem");
@@ -595,6 +774,7 @@ int32_t reduce_plan(nqcd_handle_t* h, int32_t kind, int32_t red, const double* o
         P.S.sig = h->sig.as<double>();
         P.S.state = h->state.as<int32_t>();
         P.S.save_pop_di = h->save_di.as<double>();
+        P.S.final_pop_di = h->final_di.as<double>();
     }
     P.scale = (red == NQCD_RED_MEAN) ? 1.0 / (double)nt : 1.0;
     return NQCD_OK;
@@ -656,6 +836,15 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
     const int D = (int)nqcd_rows(h), n = h->prob.n_states == 1 ? 2 : h->prob.n_states, ns = h->nsave;
     HIPCHK(h, hipSetDevice(h->device));
     if (is_ie(h)) return ie_download(h, field, out, bytes);
+    if (ns_is_problem(h)) return ns_download(h, field, out, bytes);
+    if (cme_is_problem(h) && field != NQCD_FIELD_R && field != NQCD_FIELD_V && field != NQCD_FIELD_STATE &&
+        field != NQCD_FIELD_STEPS && field != NQCD_FIELD_NHOPS && field != NQCD_FIELD_SAVE_R &&
+        field != NQCD_FIELD_SAVE_EKIN && field != NQCD_FIELD_SAVE_EPOT && field != NQCD_FIELD_SAVE_STATE)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "CME / BCME fields: R, V, STATE, STEPS, NHOPS, SAVE_R, SAVE_EKIN, SAVE_EPOT, SAVE_STATE");
+    if (cls_is_problem(h) && field != NQCD_FIELD_R && field != NQCD_FIELD_V && field != NQCD_FIELD_STEPS &&
+        field != NQCD_FIELD_SAVE_R && field != NQCD_FIELD_SAVE_V && field != NQCD_FIELD_SAVE_EKIN &&
+        field != NQCD_FIELD_SAVE_EPOT)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "classical fields: R, V, STEPS, SAVE_R, SAVE_V, SAVE_EKIN, SAVE_EPOT");
     auto fetch = [&](const DevBuf& b, size_t sz, std::vector<char>& host) -> hipError_t {
         host.resize(sz);
         return hipMemcpy(host.data(), b.p, sz, hipMemcpyDeviceToHost);
@@ -714,8 +903,19 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
             for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = s[(size_t)sl * nt + i];
         return NQCD_OK;
     }
+    case NQCD_FIELD_SAVE_V: {
+        if (!cls_is_problem(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "velocities per save exist for classical problems");
+        if (bytes != (int64_t)sizeof(double) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(h->save_v, sizeof(double) * ns * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i)
+            for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = s[(size_t)sl * nt + i];
+        return NQCD_OK;
+    }
     case NQCD_FIELD_SAVE_R: {
-        if (!nd_is_problem(h)) return fail(h, NQCD_ERR_UNSUPPORTED, "configurations per save exist for the diatom-surface model");
+        if (!nd_is_problem(h) && !cme_is_problem(h) && !cls_is_problem(h))
+            return fail(h, NQCD_ERR_UNSUPPORTED, "configurations per save exist for the diatom-surface and CME models");
         if (bytes != (int64_t)sizeof(double) * D * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
         HIPCHK(h, fetch(h->save_centroid, sizeof(double) * D * ns * nt, raw));
         const double* s = reinterpret_cast<const double*>(raw.data());
@@ -734,7 +934,8 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
     }
     case NQCD_FIELD_SAVE_EKIN:
     case NQCD_FIELD_SAVE_EPOT: {
-        if (is_rp(h) || nd_is_problem(h) || is_spin_boson(h) || !is_two_state_1d(h))
+        if (!cme_is_problem(h) && !cls_is_problem(h) &&
+            (is_rp(h) || nd_is_problem(h) || is_spin_boson(h) || !is_two_state_1d(h)))
             return fail(h, NQCD_ERR_UNSUPPORTED, "energies per save exist for two-state one-dof FSSH / Ehrenfest");
         if (bytes != (int64_t)sizeof(double) * ns * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
         HIPCHK(h, fetch(field == NQCD_FIELD_SAVE_EKIN ? h->save_ekin : h->save_epot, sizeof(double) * ns * nt, raw));
@@ -742,6 +943,16 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
         double* o = static_cast<double*>(out);
         for (int64_t i = 0; i < nt; ++i)
             for (int sl = 0; sl < ns; ++sl) o[(size_t)i * ns + sl] = s[(size_t)sl * nt + i];
+        return NQCD_OK;
+    }
+    case NQCD_FIELD_FINAL_POP_DI: {
+        if (!is_two_state_1d(h))
+            return fail(h, NQCD_ERR_UNSUPPORTED, "diabatic final populations exist for two-state one-dof FSSH / Ehrenfest");
+        if (bytes != (int64_t)sizeof(double) * 2 * nt) return fail(h, NQCD_ERR_ARG, "bytes mismatch");
+        HIPCHK(h, fetch(h->final_di, sizeof(double) * 2 * nt, raw));
+        const double* s = reinterpret_cast<const double*>(raw.data());
+        double* o = static_cast<double*>(out);
+        for (int64_t i = 0; i < nt; ++i) { o[2 * i] = s[i]; o[2 * i + 1] = s[nt + i]; }
         return NQCD_OK;
     }
     case NQCD_FIELD_SAVE_STATE: {
@@ -797,7 +1008,8 @@ int32_t nqcd_mi355x_last_run_stats(nqcd_handle_t* h, nqcd_run_stats* out) {
     return NQCD_OK;
 }
 
-int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, uint64_t philox_key, nqcd_tune_result* out) {
+int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, const nqcd_term_desc* term, uint64_t philox_key,
+                             nqcd_tune_result* out) {
     if (!h || !out) return fail(h, NQCD_ERR_ARG, "null argument");
     if (!h->have_problem || !h->have_initial) return fail(h, NQCD_ERR_STATE, "autotune needs a problem and initial conditions");
     int32_t family = 0;
@@ -831,7 +1043,7 @@ int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, uint64_t ph
         h->force_wpc = cand[i].wpc;
         double ms = 0.0;
         for (int rep = 0; rep < 2; ++rep) {
-            const int32_t rc = nqcd_mi355x_run(h, t0, t1, 0, nullptr, philox_key);
+            const int32_t rc = nqcd_mi355x_run(h, t0, t1, 0, term, philox_key);
             if (rc != NQCD_OK) {
                 h->force_block = h->force_wpc = 0;
                 return rc;

@@ -7,7 +7,7 @@
 
 namespace nqcd {
 
-enum { OUTK_FINAL_ADIABATIC = 4, OUTK_SCATTERING_1D = 2, OUTK_POP_CORRELATION = 3 };
+enum { OUTK_FINAL_ADIABATIC = 4, OUTK_SCATTERING_1D = 2, OUTK_POP_CORRELATION = 3, OUTK_SCATTERING_1D_DIABATIC = 8 };
 
 hipError_t launch_sample_tully_grid(int64_t ntraj, int32_t nm, double x0, double p_lo, int32_t n_p, double mass,
                                     uint64_t key, double* r0, double* v0, int32_t* model_id, hipStream_t s);
@@ -26,6 +26,7 @@ struct OutputSrc {
     const double* sig = nullptr;  // [4][ntraj]
     const int32_t* state = nullptr;
     const double* save_pop_di = nullptr;  // [nsave][2][ntraj]
+    const double* final_pop_di = nullptr; // [2][ntraj] diabatic populations of the final state
 };
 hipError_t launch_fill_output2(const OutputSrc& S, int64_t ntraj, double* out, hipStream_t s);
 int32_t fill_output2_rows(int32_t kind, int32_t nsave);

@@ -67,18 +67,20 @@ __global__ __launch_bounds__(256) void init_density2(int64_t ntraj, int32_t stat
 // ---------------------------------------------------------------------------
 // Number of rows of an output computed from the final state or the save slots.
 __host__ __device__ inline int32_t output2_rows(int32_t kind, int32_t nsave) {
-    return kind == OUTK_FINAL_ADIABATIC ? 2 : kind == OUTK_SCATTERING_1D ? 4 : 4 * nsave;
+    return kind == OUTK_FINAL_ADIABATIC ? 2 : (kind == OUTK_SCATTERING_1D || kind == OUTK_SCATTERING_1D_DIABATIC) ? 4 : 4 * nsave;
 }
 
 // Row `row` of trajectory i (one expression shared by the materialising and the fused reductions,
 // so both sum the same values in the same order).
 __device__ __forceinline__ double output2_value(const OutputSrc& S, int64_t ntraj, int32_t row, int64_t i) {
-    if (S.kind == OUTK_FINAL_ADIABATIC || S.kind == OUTK_SCATTERING_1D) {
+    if (S.kind == OUTK_FINAL_ADIABATIC || S.kind == OUTK_SCATTERING_1D || S.kind == OUTK_SCATTERING_1D_DIABATIC) {
         double p0, p1;
-        if (S.method == 0) { p0 = (S.state[i] == 0) ? 1.0 : 0.0; p1 = 1.0 - p0; }
+        if (S.kind == OUTK_SCATTERING_1D_DIABATIC) { p0 = S.final_pop_di[i]; p1 = S.final_pop_di[ntraj + i]; }
+        else if (S.method == 0) { p0 = (S.state[i] == 0) ? 1.0 : 0.0; p1 = 1.0 - p0; }
         else { p0 = S.sig[i]; p1 = S.sig[ntraj + i]; }
         if (S.kind == OUTK_FINAL_ADIABATIC) return row == 0 ? p0 : p1;
-        // OutputStateResolvedScattering1D: rows [reflection(2), transmission(2)] (DynamicsOutputs.jl:331-336)
+        // OutputStateResolvedScattering1D (adiabatic or diabatic populations of the final state): rows
+        // [reflection(2), transmission(2)] (DynamicsOutputs.jl:313-338)
         const bool trans = S.r[(int64_t)S.xdof * ntraj + i] > 0.0;
         const double p = (row & 1) ? p1 : p0;
         return (row < 2) ? (trans ? 0.0 : p) : (trans ? p : 0.0);

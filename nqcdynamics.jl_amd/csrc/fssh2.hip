@@ -17,6 +17,10 @@
 //    masked rather than replaced.
 #include "fssh2.h"
 
+#include <cstdio>
+#include <algorithm>
+#include <cstdlib>
+
 namespace nqcd {
 
 // Makes a pointer opaque to the optimiser at this point, so loads through it are
@@ -134,6 +138,37 @@ __device__ __forceinline__ void load_traj(const Fssh2Args* Ap, Lane2<D, R>& L, i
     save_slot(Ap, L, 0);
 }
 
+// Set a running trajectory aside in slot `slot` of S, and take one up again (bit-identical state: FP32
+// nuclei are stored widened and narrowed back exactly; the model registers come from the model id).
+template <int D, typename R>
+__device__ __forceinline__ void suspend_traj(const SuspSet& S, uint64_t slot, const Lane2<D, R>& L) {
+    static_assert(D == 1, "one-dimensional models");
+    const int64_t c = S.cap;
+    double* f = S.f + slot;
+    f[0 * c] = (double)L.r[0]; f[1 * c] = (double)L.v[0]; f[2 * c] = (double)L.k[0];
+    f[3 * c] = L.s.q; f[4 * c] = L.s.x; f[5 * c] = L.s.y; f[6 * c] = L.tr;
+    f[7 * c] = L.U[0]; f[8 * c] = L.U[1]; f[9 * c] = L.U[2]; f[10 * c] = L.U[3];
+    f[11 * c] = L.dEc; f[12 * c] = L.wc; f[13 * c] = L.tc; f[14 * c] = L.t;
+    S.tid[slot] = L.tid;
+    S.i[0 * c + slot] = L.st; S.i[1 * c + slot] = L.nh; S.i[2 * c + slot] = L.step;
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void resume_traj(const Fssh2Args* Ap, const SuspSet& S, uint64_t slot, Lane2<D, R>& L) {
+    static_assert(D == 1, "one-dimensional models");
+    const Fssh2Args& A = *opaque(Ap);
+    const int64_t c = S.cap;
+    const double* f = S.f + slot;
+    L.r[0] = (R)f[0 * c]; L.v[0] = (R)f[1 * c]; L.k[0] = (R)f[2 * c];
+    L.s.q = f[3 * c]; L.s.x = f[4 * c]; L.s.y = f[5 * c]; L.tr = f[6 * c];
+    L.U[0] = f[7 * c]; L.U[1] = f[8 * c]; L.U[2] = f[9 * c]; L.U[3] = f[10 * c];
+    L.dEc = f[11 * c]; L.wc = f[12 * c]; L.tc = f[13 * c]; L.t = f[14 * c];
+    L.tid = S.tid[slot];
+    L.st = S.i[0 * c + slot]; L.nh = S.i[1 * c + slot]; L.step = S.i[2 * c + slot];
+    const int mid = A.model_id ? A.model_id[L.tid] : 0;
+    L.m = model_regs(A.models, mid);
+}
+
 template <int D, typename R>
 __device__ __forceinline__ void finish_traj(const Fssh2Args* Ap, const Lane2<D, R>& L) {
     const Fssh2Args& A = *opaque(Ap);
@@ -150,11 +185,19 @@ __device__ __forceinline__ void finish_traj(const Fssh2Args* Ap, const Lane2<D, 
     A.state[L.tid] = L.st;
     A.steps[L.tid] = L.step;
     A.nhops[L.tid] = L.nh;
+    if (A.final_pop_di) {  // diag(U S U^T) of the final state (fssh.jl:132-142, ehrenfest.jl:75-83)
+        double pa0, pa1;
+        if (A.method == METHOD_FSSH) { pa0 = (L.st == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0; }
+        else { pa0 = 0.5 * (L.tr + L.s.q); pa1 = 0.5 * (L.tr - L.s.q); }
+        A.final_pop_di[L.tid] = L.U[0] * L.U[0] * pa0 + L.U[1] * L.U[1] * pa1 + 2.0 * L.U[0] * L.U[1] * L.s.x;
+        A.final_pop_di[n + L.tid] = L.U[2] * L.U[2] * pa0 + L.U[3] * L.U[3] * pa1 + 2.0 * L.U[2] * L.U[3] * L.s.x;
+    }
 }
 
 template <int D, typename R>
 __device__ __forceinline__ bool is_done(const Fssh2Args& A, const Lane2<D, R>& L) {
     if (L.step >= A.nsteps) return true;
+    if (A.cell.mode == 2 && cell_outside<1, D, R>(A.cell, L.r)) return true;  // TerminateCellCallback
     if (A.term_kind != 0) {
         double x = 0.0;  // select, not a runtime index, so the lane state stays in registers
 #pragma unroll
@@ -192,6 +235,9 @@ __device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& 
     propagate2(P, L.s, L.t, L.tc, A.dt, A.nsub);
     L.dEc = dEn; L.wc = w; L.tc = tn;
     L.t = tn;
+    // A save time's values are those before the step's discrete callbacks (DiffEqBase's
+    // apply_discrete_callback! saves the due saveat points before affect!): the hop comes after the save.
+    if (A.save_every > 0 && ((L.step + 1) % A.save_every) == 0) save_slot(Ap, L, (L.step + 1) / A.save_every);
     if (A.method == METHOD_FSSH) {
         const int s = L.st;
         const double sss = 0.5 * (L.tr + ((s == 0) ? L.s.q : -L.s.q));
@@ -214,8 +260,10 @@ __device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& 
             }
         }
     }
+    // CellBoundaryCallback after the hop callback (the problem's callbacks precede the user's); the
+    // acceleration of the step end is kept, as BABwithTsit5's cache keeps it (bab_electronics.jl:61-74)
+    if (A.cell.mode == 1 && cell_outside<1, D, R>(A.cell, L.r)) cell_wrap<1, D, R>(A.cell, L.r);
     ++L.step;
-    if (A.save_every > 0 && (L.step % A.save_every) == 0) save_slot(Ap, L, L.step / A.save_every);
 }
 
 template <int D, typename R>
@@ -225,31 +273,70 @@ __device__ __forceinline__ void persistent_body(const Fssh2Args* __restrict__ Ap
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     Lane2<D, R> L;
     bool active = false, exhausted = false;
+    int lvl = 0, wave_lvl = 0;   // priority class of the lane's trajectory; the wave's current priority
     for (;;) {
         const uint64_t want = __ballot(!active && !exhausted);
         if (want) {
+            // the first launch claims trajectory ids from the queue, a resuming launch slots of sin
+            const bool resuming = opaque(Ap)->sin.f != nullptr;
+            const unsigned long long limit = resuming ? *opaque(Ap)->sin.count : (unsigned long long)A.ntraj;
             const int cnt = __popcll(want);
             const int leader = __ffsll((unsigned long long)want) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(opaque(Ap)->queue, (unsigned long long)cnt);
+            if (lane == leader)
+                base = atomicAdd(resuming ? opaque(Ap)->sin.claim : opaque(Ap)->queue, (unsigned long long)cnt);
             base = __shfl(base, leader);
             if (!active && !exhausted) {
                 const unsigned long long q = base + (unsigned long long)__popcll(want & below);
-                if (q < (unsigned long long)A.ntraj) {
-                    const int32_t* order = opaque(Ap)->order;
-                    load_traj(Ap, L, order ? (int64_t)order[q] : (int64_t)q);
-                    // the termination predicate is a DiscreteCallback, evaluated after each step and
-                    // not at t0 (callbacks.jl:15-29): a fresh trajectory is done only for zero steps
-                    if (A.nsteps <= 0) finish_traj(Ap, L); else active = true;
+                if (q < limit) {
+                    if (resuming) {
+                        resume_traj(Ap, opaque(Ap)->sin, q, L);
+                        active = true;
+                        lvl = 3;
+                    } else {
+                        const int64_t span = opaque(Ap)->prio_span;
+                        lvl = span > 0 ? 3 - (int)min((unsigned long long)3, q / (unsigned long long)span) : 0;
+                        const int32_t* order = opaque(Ap)->order;
+                        load_traj(Ap, L, order ? (int64_t)order[q] : (int64_t)q);
+                        // the termination predicate is a DiscreteCallback, evaluated after each step and
+                        // not at t0 (callbacks.jl:15-29): a fresh trajectory is done only for zero steps
+                        if (A.nsteps <= 0) finish_traj(Ap, L); else active = true;
+                    }
                 } else {
                     exhausted = true;
                 }
             }
         }
         if (__ballot(active) == 0) break;
+        if (A.prio_span > 0) {   // s_setprio takes an immediate
+            const int w = __ballot(active && lvl >= 3) ? 3 : __ballot(active && lvl >= 2) ? 2
+                        : __ballot(active && lvl >= 1) ? 1 : 0;
+            if (w != wave_lvl) {
+                wave_lvl = w;
+                if (w == 3) __builtin_amdgcn_s_setprio(3);
+                else if (w == 2) __builtin_amdgcn_s_setprio(2);
+                else if (w == 1) __builtin_amdgcn_s_setprio(1);
+                else __builtin_amdgcn_s_setprio(0);
+            }
+        }
         if (active) {
             step_traj(Ap, A, L);
             if (is_done(A, L)) { finish_traj(Ap, L); active = false; }
+        }
+        // Once the queue has drained, a wave with few running lanes sets them aside for the next launch,
+        // which packs them into full waves (instead of issuing for its idle lanes to the end).
+        if (__ballot(exhausted) != 0 && opaque(Ap)->sout.f != nullptr) {
+            const uint64_t act = __ballot(active);
+            if (__popcll(act) <= opaque(Ap)->susp_thr) {
+                if (act) {
+                    const int leader = __ffsll((unsigned long long)act) - 1;
+                    unsigned long long base = 0;
+                    if (lane == leader) base = atomicAdd(opaque(Ap)->sout.count, (unsigned long long)__popcll(act));
+                    base = __shfl(base, leader);
+                    if (active) suspend_traj(opaque(Ap)->sout, base + (unsigned long long)__popcll(act & below), L);
+                }
+                break;
+            }
         }
     }
 }
@@ -354,6 +441,8 @@ __global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, doubl
     P.E0c = S[PL_E0C * n + tid]; P.E1c = S[PL_E1C * n + tid]; P.Wc = S[PL_WC * n + tid]; P.tc = S[PL_TC * n + tid];
     P.E0n = E0; P.E1n = E1; P.Wn = w; P.inv = 1.0 / (tn - P.tc);
     propagate_dense(P, s, t, dt, A.nsub);
+    if (A.save_every > 0 && (step + 1) % A.save_every == 0)  // before the hop, as the persistent kernel
+        plain_save(A, tid, (step + 1) / A.save_every, st, s, U, v, E0, E1);
     if (A.method == METHOD_FSSH) {
         const int m = 1 - st;
         // real(sigma[m,s] / sigma[s,s]) with full complex division
@@ -372,6 +461,12 @@ __global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, doubl
             v = vv[0];
         }
     }
+    bool cell_end = false;
+    if (A.cell.mode != 0) {  // cell callback after the hop callback, as the persistent kernel
+        double rr[1] = {r};
+        cell_end = cell_callback<1, 1, double>(A.cell, rr);
+        r = rr[0];
+    }
     ++step;
     S[PL_R * n + tid] = r; S[PL_V * n + tid] = v; S[PL_K * n + tid] = k;
     for (int J = 0; J < 4; ++J) { S[(PL_SRE + J) * n + tid] = s.re[J]; S[(PL_SIM + J) * n + tid] = s.im[J]; }
@@ -379,8 +474,7 @@ __global__ __launch_bounds__(256) void fssh2_plain_step(const Fssh2Args A, doubl
     for (int J = 0; J < 4; ++J) S[(PL_U + J) * n + tid] = U[J];
     S[PL_T * n + tid] = tn;
     Si[PI_ST * n + tid] = st; Si[PI_NH * n + tid] = nh; Si[PI_STEP * n + tid] = step;
-    if (A.save_every > 0 && step % A.save_every == 0) plain_save(A, tid, step / A.save_every, st, s, U, v, E0, E1);
-    const bool done = (step >= A.nsteps) || term_hit(A.term_kind, r, A.term_value);
+    const bool done = (step >= A.nsteps) || cell_end || term_hit(A.term_kind, r, A.term_value);
     if (done) Si[PI_DONE * n + tid] = 1;
 }
 
@@ -397,23 +491,73 @@ __global__ __launch_bounds__(256) void fssh2_plain_finish(const Fssh2Args A, con
     A.state[tid] = Si[PI_ST * n + tid];
     A.nhops[tid] = Si[PI_NH * n + tid];
     A.steps[tid] = Si[PI_STEP * n + tid];
+    if (A.final_pop_di) {  // diag(U S U^T) of the final state (fssh.jl:132-142, ehrenfest.jl:75-83)
+        double U[4];
+        for (int J = 0; J < 4; ++J) U[J] = S[(PL_U + J) * n + tid];
+        const int st = Si[PI_ST * n + tid];
+        double S00, S11;
+        if (A.method == METHOD_FSSH) { S00 = (st == 0) ? 1.0 : 0.0; S11 = 1.0 - S00; }
+        else { S00 = S[PL_SRE * n + tid]; S11 = S[(PL_SRE + 3) * n + tid]; }
+        const double S01 = S[(PL_SRE + 1) * n + tid];
+        A.final_pop_di[tid] = U[0] * U[0] * S00 + U[1] * U[1] * S11 + 2.0 * U[0] * U[1] * S01;
+        A.final_pop_di[n + tid] = U[2] * U[2] * S00 + U[3] * U[3] * S11 + 2.0 * U[2] * U[3] * S01;
+    }
 }
 
 __global__ void queue_reset(unsigned long long* q) { *q = 0ull; }
 
+// zeroes the counters of a set of set-aside trajectories before a launch fills it
+__global__ void susp_reset(unsigned long long* count, unsigned long long* claim) {
+    if (threadIdx.x == 0) { *count = 0ull; *claim = 0ull; }
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
+int64_t fssh2_prio_span(int64_t lanes) {  // read at every run
+    double f = kPrioDefaultSpan;
+    if (const char* e = std::getenv("NQCD_MI355X_PRIO")) f = std::atof(e);
+    if (!(f > 0.0) || f > 1e6) return 0;
+    return std::max<int64_t>(1, (int64_t)(f * (double)lanes));
+}
+
+void fssh2_suspend_policy(int* rounds, int* threshold) {  // read at every run
+    int r = kSuspDefaultRounds, t = kSuspDefaultThreshold;
+    if (const char* e = std::getenv("NQCD_MI355X_SUSPEND")) {
+        int a = 0, b = 0;
+        if (std::sscanf(e, "%d,%d", &a, &b) == 2 && a >= 1 && a <= kSuspRounds && b >= 0 && b <= 64) { r = a; t = b; }
+    }
+    *rounds = r;
+    *threshold = t;
+}
+
 hipError_t fssh2_launch_persistent(const Fssh2Args& A, Fssh2Args* Adev, int precision, int grid, int block,
-                                   hipStream_t stream) {
-    hipError_t e = hipMemcpyAsync(Adev, &A, sizeof(Fssh2Args), hipMemcpyHostToDevice, stream);
+                                   hipStream_t stream, const SuspSet* sets) {
+    // round r reads sets[(r - 1) % 2] and fills sets[r % 2]; the last round runs every trajectory it
+    // holds to the end, so no trajectory is left set aside
+    int rounds = 1, thr = 0;
+    if (sets) fssh2_suspend_policy(&rounds, &thr);
+    Fssh2Args R[kSuspRounds];
+    for (int r = 0; r < rounds; ++r) {
+        R[r] = A;
+        R[r].sin = (r == 0) ? SuspSet{} : sets[(r - 1) % 2];
+        R[r].sout = (r == rounds - 1) ? SuspSet{} : sets[r % 2];
+        R[r].susp_thr = thr;
+    }
+    hipError_t e = hipMemcpyAsync(Adev, R, sizeof(Fssh2Args) * rounds, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(queue_reset, dim3(1), dim3(1), 0, stream, A.queue);
-    if (precision == 0)
-        hipLaunchKernelGGL(fssh2_persistent_f64, dim3(grid), dim3(block), 0, stream, (const Fssh2Args*)Adev);
-    else
-        hipLaunchKernelGGL(fssh2_persistent_f32, dim3(grid), dim3(block), 0, stream, (const Fssh2Args*)Adev);
-    return hipGetLastError();
+    for (int r = 0; r < rounds; ++r) {
+        if (R[r].sout.f) hipLaunchKernelGGL(susp_reset, dim3(1), dim3(64), 0, stream, R[r].sout.count, R[r].sout.claim);
+        const Fssh2Args* Ar = Adev + r;
+        if (precision == 0)
+            hipLaunchKernelGGL(fssh2_persistent_f64, dim3(grid), dim3(block), 0, stream, Ar);
+        else
+            hipLaunchKernelGGL(fssh2_persistent_f32, dim3(grid), dim3(block), 0, stream, Ar);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t fssh2_persistent_occupancy(int precision, int block, int* blocks_per_cu) {

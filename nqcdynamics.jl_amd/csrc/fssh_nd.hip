@@ -41,6 +41,7 @@ struct FsshNdArgs {
     double* save_pop_ad; double* save_pop_di; int32_t* save_state;
     double* save_r;                   // [slot][dof][traj] or null
     unsigned long long* queue;
+    CellDesc cell;                    // cell callback per atom (3 coordinates, 2 atoms); mode 0: none
 };
 
 namespace {
@@ -256,6 +257,7 @@ __device__ __forceinline__ void nd_finish(const FsshNdArgs* Ap, const LaneN<R>& 
 template <typename R>
 __device__ __forceinline__ bool nd_done(const FsshNdArgs& A, const LaneN<R>& L) {
     if (L.step >= A.nsteps) return true;
+    if (A.cell.mode == 2 && cell_outside<3, 2, R>(A.cell, L.r)) return true;  // TerminateCellCallback
     if (A.term_kind != 0) {
         double x = 0.0;
 #pragma unroll
@@ -306,6 +308,8 @@ __device__ __forceinline__ void nd_finish_step(const FsshNdArgs* Ap, const FsshN
     propagate2(P, L.s, L.t, L.tc, A.dt, A.nsub);
     L.dEc = dEn; L.wc = w; L.tc = tn;
     L.t = tn;
+    // the save precedes the hop callback (a save time holds the state before the step's callbacks)
+    if (A.save_every > 0 && ((L.step + 1) % A.save_every) == 0) nd_save(Ap, L, (L.step + 1) / A.save_every);
     if (A.method == METHOD_FSSH) {
         const int s = L.st;
         const double sss = 0.5 * (L.tr + ((s == 0) ? L.s.q : -L.s.q));
@@ -328,8 +332,9 @@ __device__ __forceinline__ void nd_finish_step(const FsshNdArgs* Ap, const FsshN
             }
         }
     }
+    // CellBoundaryCallback after the hop callback; the step-end acceleration is kept (bab_electronics.jl:61-74)
+    if (A.cell.mode == 1 && cell_outside<3, 2, R>(A.cell, L.r)) cell_wrap<3, 2, R>(A.cell, L.r);
     ++L.step;
-    if (A.save_every > 0 && (L.step % A.save_every) == 0) nd_save(Ap, L, L.step / A.save_every);
 }
 
 template <typename R, bool kMlp>
@@ -482,6 +487,7 @@ int32_t nd_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     A.term_kind = term ? term->kind : 0;
     A.term_dof = term ? term->dof : 0;
     A.term_value = term ? term->value : 0.0;
+    A.cell = h->cell;
     A.dt = h->prob.dt;
     A.t0 = t0;
     A.nsteps = nsteps;

@@ -64,6 +64,11 @@ struct nqcd_handle {
     bool ie_psi_given = false;
     // diatom-surface model parameters
     double nd_params[16]{};
+    // n-state Morse tables per model slot, their device copy, and the diabatic initial state of a
+    // sampled n-state ensemble (-1: the uploaded density)
+    std::vector<double> ns_tab[nqcd::kMaxModels];
+    nqcd::DevBuf ns_tab_dev;
+    int32_t ns_init_diabatic = -1;
     // MLP surrogate: two members, FP32 masters, packed BF16 copies, Adam moments, training data
     bool mlp_ready = false;
     double mlp_threshold = 0.0;
@@ -75,7 +80,12 @@ struct nqcd_handle {
     int32_t nsave = 1, save_every = 0;
     nqcd::DevBuf r0, v0, sig0, state0, model_id, order, models_dev, bath, inv_mass_dev, args_dev;
     nqcd::DevBuf r, v, sig, state, steps, nhops, save_ad, save_di, save_state;
-    nqcd::DevBuf queue, plainS, plainSi, tmp, partial, small, save_centroid, sum64, save_ekin, save_epot;
+    nqcd::DevBuf queue, plainS, plainSi, tmp, partial, small, save_centroid, sum64, save_ekin, save_epot, final_di;
+    nqcd::DevBuf save_v;   // velocities per save (classical dynamics)
+    // trajectories set aside between the launches of a persistent two-state run (two sets used in turn)
+    nqcd::DevBuf susp_f[2], susp_tid[2], susp_i[2], susp_ctr;
+    // simulation cell of the cell callbacks (mode 0: none), set by nqcd_mi355x_set_cell
+    nqcd::CellDesc cell{};
     bool state0_given = false, model_id_given = false, order_given = false;
     nqcd_run_stats stats{};
     int cu_count = 0;

@@ -703,6 +703,10 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
                 sc = eigen(x);
             }
 
+            // the save precedes the hop and decoherence callbacks (a save time holds the state before the
+            // step's callbacks: DiffEqBase saves the due saveat points before affect!)
+            if (A.save_every > 0 && (step + 1) % A.save_every == 0) save((int)((step + 1) / A.save_every));
+
             // ---- IESH hop (iesh.jl:390-407) -------------------------------------------------------------
             if (A.method == 0) {
                 const double rnd = philox_uniform((uint64_t)traj, (uint32_t)step, STREAM_HOP, A.key);
@@ -968,7 +972,6 @@ __global__ __launch_bounds__(64 * NT) __attribute__((amdgpu_waves_per_eu(2))) vo
                     __syncthreads();
                 }
             }
-            if (A.save_every > 0 && (step + 1) % A.save_every == 0) save((int)((step + 1) / A.save_every));
         }
 
         // ---- store --------------------------------------------------------------------------------

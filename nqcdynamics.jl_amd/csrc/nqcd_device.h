@@ -81,6 +81,77 @@ __device__ __forceinline__ bool term_hit(int kind, double x, double value) {
     return (kind == 1) ? fabs(x) > value : (kind == 2) ? x > value : (kind == 3) ? x < value : false;
 }
 
+// Simulation cell of the cell callbacks (DynamicsUtils/callbacks.jl:4-22), NQCBase's PeriodicCell: the
+// lattice vectors are the columns of a (column-major, leading dimension 3), inv = a^-1, per flags the
+// periodic dimensions; nd coordinates per atom.
+struct CellDesc {
+    int32_t mode;    // nqcd_cell_mode: 0 none, 1 CellBoundaryCallback (wrap), 2 TerminateCellCallback
+    int32_t nd;
+    int32_t per[3];
+    int32_t pad;
+    double a[9];
+    double inv[9];
+};
+
+// Julia's mod(f, 1) for a Float64 (rem, then + 1 where the remainder is negative; +0 for a zero remainder)
+__host__ __device__ __forceinline__ double jl_mod1(double f) {
+    const double r = fmod(f, 1.0);
+    if (r == 0.0) return 0.0;
+    return (r > 0.0) ? r : r + 1.0;
+}
+
+// check_atoms_in_cell negated: some atom has a fractional coordinate f = inv x outside [0, 1].  x holds NA
+// atoms of ND coordinates, the coordinate fastest ((ndofs, natoms) column-major); indices are compile-time
+// constants after unrolling, so register arrays stay in registers.
+template <int ND, int NA, typename R>
+__device__ __forceinline__ bool cell_outside(const CellDesc& c, const R* x) {
+    bool out = false;
+#pragma unroll
+    for (int at = 0; at < NA; ++at)
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            double f = 0.0;
+#pragma unroll
+            for (int k = 0; k < ND; ++k) f += c.inv[j + 3 * k] * (double)x[at * ND + k];
+            out = out || (f > 1.0) || (f < 0.0);
+        }
+    return out;
+}
+
+// apply_cell_boundaries!: every atom to f = inv x, f_j -> mod(f_j, 1) on the periodic dimensions, x = a f.
+template <int ND, int NA, typename R>
+__device__ __forceinline__ void cell_wrap(const CellDesc& c, R* x) {
+#pragma unroll
+    for (int at = 0; at < NA; ++at) {
+        double f[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < ND; ++k) s += c.inv[j + 3 * k] * (double)x[at * ND + k];
+            f[j] = c.per[j] ? jl_mod1(s) : s;
+        }
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < ND; ++k) s += c.a[j + 3 * k] * f[k];
+            x[at * ND + j] = (R)s;
+        }
+    }
+}
+
+// The cell callback after a step (after the hop callback): wrap when an atom left the cell
+// (CellBoundaryCallback); returns true when the trajectory ends here (TerminateCellCallback).
+template <int ND, int NA, typename R>
+__device__ __forceinline__ bool cell_callback(const CellDesc& c, R* x) {
+    if (c.mode == 0) return false;
+    if (!cell_outside<ND, NA, R>(c, x)) return false;
+    if (c.mode == 2) return true;
+    cell_wrap<ND, NA, R>(c, x);
+    return false;
+}
+
 __device__ __forceinline__ double fexp(double x) { return exp(x); }
 __device__ __forceinline__ float fexp(float x) { return __expf(x); }
 __device__ __forceinline__ double fabs_(double x) { return fabs(x); }

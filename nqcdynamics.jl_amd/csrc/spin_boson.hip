@@ -160,6 +160,8 @@ __device__ __forceinline__ void sb_body(const SbArgs* __restrict__ Ap) {
         P.dE0 = dEc; P.dE1 = (dEn - dEc) * inv; P.w0 = wc; P.w1 = (w - wc) * inv;
         propagate2(P, s, t, tc, A.dt, A.nsub);
         dEc = dEn; wc = w; tc = tn; t = tn;
+        // the save precedes the hop callback (a save time holds the state before the step's callbacks)
+        if (A.save_every > 0 && ((step + 1) % A.save_every) == 0) sb_save(Ap, tid, (step + 1) / A.save_every, method, st, tr, s, U);
         if (method == 0) {
             const double sss = 0.5 * (tr + ((st == 0) ? s.q : -s.q));
             const double ratio = s.x / sss;
@@ -199,7 +201,6 @@ __device__ __forceinline__ void sb_body(const SbArgs* __restrict__ Ap) {
                 }
             }
         }
-        if (A.save_every > 0 && ((step + 1) % A.save_every) == 0) sb_save(Ap, tid, (step + 1) / A.save_every, method, st, tr, s, U);
     }
     const SbArgs& B = *sb_opaque(Ap);
     const int64_t n = B.ntraj;

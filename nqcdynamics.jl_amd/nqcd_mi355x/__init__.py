@@ -9,15 +9,17 @@ C ABI).  There is no CPU fallback: the library must be built.
 """
 from . import _lib  # noqa: F401
 from .distributions import (Adiabatic, Diabatic, DynamicalDistribution, DynamicsVariables, FermiDiracState, Normal,  # noqa: F401
-                            PositionHarmonicWigner, PureState, VelocityHarmonicWigner)
-from .ensembles import EnsembleMI355X, TerminatingCallback, run_dynamics  # noqa: F401
+                            PositionHarmonicWigner, PureState, VelocityBoltzmann, VelocityHarmonicWigner)
+from .ensembles import (CellBoundaryCallback, EnsembleMI355X, TerminateCellCallback, TerminatingCallback,  # noqa: F401
+                        run_dynamics)
 from .models import (AndersonHolstein, DebyeSpectralDensity, DiatomSurface, DoubleWellGaussian, Harmonic,  # noqa: F401
-                     MiaoSubotnik, MLPSurrogate, OhmicSpectralDensity, SpinBoson, TrapezoidalRule, TullyModelOne,
-                     TullyModelThree, TullyModelTwo, nelectrons, nstates, train_surrogate)
+                     MiaoSubotnik, MLPSurrogate, MorseN, AndersonHolsteinOneMode, OhmicSpectralDensity, SpinBoson,
+                     TrapezoidalRule, TullyModelOne, TullyModelThree, TullyModelTwo, nelectrons, nstates,
+                     train_surrogate)
 from .outputs import (FileReduction, MeanReduction, OutputAdiabaticPopulation, OutputCentroidPosition,  # noqa: F401
-                      OutputDiabaticPopulation,
-                      OutputDiscreteState, OutputFinal, OutputFinalTime, OutputKineticEnergy, OutputTotalEnergy, OutputStateResolvedScattering1D,
-                      OutputSurfaceHops, PopulationCorrelationFunction, SortByOutputReduction,
+                      OutputDiabaticPopulation, OutputDiscreteState, OutputFinal, OutputFinalTime,
+                      OutputKineticEnergy, OutputPosition, OutputStateResolvedScattering1D, OutputSurfaceHops,
+                      OutputTotalEnergy, OutputVelocity, PopulationCorrelationFunction, SortByOutputReduction,
                       SortByTrajectoryReduction, SumReduction)
-from .simulations import (AdiabaticIESH, Atoms, Classical, DecoherenceCorrectionEDC, Ehrenfest,  # noqa: F401
-                          EhrenfestNA, FSSH, RingPolymerSimulation, Simulation, masses)
+from .simulations import (AdiabaticIESH, Atoms, BCME, CME, Classical, DecoherenceCorrectionEDC, Ehrenfest,  # noqa: F401
+                          EhrenfestNA, FSSH, InfiniteCell, PeriodicCell, RingPolymerSimulation, Simulation, masses)

@@ -11,29 +11,34 @@ import numpy as np
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("NQCD_MI355X_LIB", os.path.join(_PKG_ROOT, "libnqcd_mi355x.so"))
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # enum nqcd_status
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_DEVICE, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
 # enum nqcd_method
-METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST = range(7)
+(METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST,
+ METHOD_CME, METHOD_BCME, METHOD_CLASSICAL) = range(10)
 # enum nqcd_algorithm
-ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION = range(5)
+(ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION, ALG_TSIT5,
+ ALG_VELOCITY_VERLET) = range(7)
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = range(3)
 PREC_FP64, PREC_FP32_NUCLEI = range(2)
 SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
 (MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC, MODEL_NEWNS_ANDERSON,
- MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE) = range(9)
+ MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE, MODEL_MORSE_N, MODEL_ANDERSON_HOLSTEIN_1) = range(11)
 MLP_PARAMS = 128 * 6 + 128 + 128 * 128 + 128 + 3 * 128 + 3
 DECOHERENCE_NONE, DECOHERENCE_EDC = range(2)
 TERM_NONE, TERM_ABS_GT, TERM_GT, TERM_LT = range(4)
-SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLYMER_THERMAL = range(4)
+CELL_NONE, CELL_BOUNDARY, CELL_TERMINATE = range(3)
+(SAMPLER_GAUSSIAN, SAMPLER_TULLY_GRID, SAMPLER_HARMONIC_WIGNER, SAMPLER_RING_POLYMER_THERMAL,
+ SAMPLER_RING_POLYMER_PIMC) = range(5)
 (OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION, OUT_FINAL_ADIABATIC_POPULATION,
- OUT_CENTROID_POSITION, OUT_KINETIC_ENERGY, OUT_TOTAL_ENERGY) = range(8)
+ OUT_CENTROID_POSITION, OUT_KINETIC_ENERGY, OUT_TOTAL_ENERGY, OUT_SCATTERING_1D_DIABATIC, OUT_POSITION,
+ OUT_VELOCITY) = range(11)
 RED_MEAN, RED_SUM = range(2)
 (FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
  FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID, FIELD_FALLBACKS, FIELD_SAVE_R, FIELD_SAVE_EKIN,
- FIELD_SAVE_EPOT) = range(15)
+ FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V) = range(17)
 
 EXPORTED_SYMBOLS = (
     "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
@@ -42,6 +47,7 @@ EXPORTED_SYMBOLS = (
     "nqcd_mi355x_surrogate_train", "nqcd_mi355x_surrogate_eval", "nqcd_mi355x_surrogate_gradient",
     "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params", "nqcd_mi355x_test_determinant",
     "nqcd_mi355x_sample_fermi_dirac", "nqcd_mi355x_autotune", "nqcd_mi355x_reduce_noalloc",
+    "nqcd_mi355x_test_broadening", "nqcd_mi355x_set_cell",
 )
 
 
@@ -51,7 +57,8 @@ class ProblemDesc(C.Structure):
                 ("precision", C.c_int32), ("schedule", C.c_int32), ("electronic_substeps", C.c_int32),
                 ("decoherence", C.c_int32), ("masses", C.POINTER(C.c_double)), ("dt", C.c_double),
                 ("temperature", C.c_double), ("edc_C", C.c_double), ("disable_estimate", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("reserved0", C.c_int32), ("bandwidth", C.c_double), ("fermi_level", C.c_double),
+                ("reserved", C.c_int32 * 2)]
 
 
 class TermDesc(C.Structure):
@@ -105,6 +112,7 @@ def lib():
     L.nqcd_mi355x_close.restype = None
     L.nqcd_mi355x_set_problem.argtypes = [H, C.POINTER(ProblemDesc)]
     L.nqcd_mi355x_set_model.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
+    L.nqcd_mi355x_set_cell.argtypes = [H, C.c_int32, C.c_int32, dp, ip]
     L.nqcd_mi355x_load_initial.argtypes = [H, C.c_int64, dp, dp, dp, dp, ip, ip]
     L.nqcd_mi355x_sample_initial.argtypes = [H, C.POINTER(SamplerDesc), C.c_uint64]
     L.nqcd_mi355x_sample_fermi_dirac.argtypes = [H, C.c_double, ip, C.c_int32, C.c_int32, C.c_uint64]
@@ -113,7 +121,7 @@ def lib():
     L.nqcd_mi355x_reduce_noalloc.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_download.argtypes = [H, C.c_int32, C.c_void_p, C.c_int64]
     L.nqcd_mi355x_last_run_stats.argtypes = [H, C.POINTER(RunStats)]
-    L.nqcd_mi355x_autotune.argtypes = [H, C.c_double, C.c_double, C.c_uint64, C.POINTER(TuneResult)]
+    L.nqcd_mi355x_autotune.argtypes = [H, C.c_double, C.c_double, C.POINTER(TermDesc), C.c_uint64, C.POINTER(TuneResult)]
     fp = C.POINTER(C.c_float)
     L.nqcd_mi355x_surrogate_train.argtypes = [H, dp, dp, C.c_int64, C.POINTER(TrainDesc), dp]
     L.nqcd_mi355x_surrogate_eval.argtypes = [H, C.c_int32, dp, C.c_int64, dp, dp]
@@ -121,6 +129,7 @@ def lib():
     L.nqcd_mi355x_surrogate_set_params.argtypes = [H, C.c_int32, fp, fp]
     L.nqcd_mi355x_surrogate_get_params.argtypes = [H, C.c_int32, fp, fp]
     L.nqcd_mi355x_test_determinant.argtypes = [H, dp, dp, C.c_int32, C.c_int32, dp]
+    L.nqcd_mi355x_test_broadening.argtypes = [H, dp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
     L.nqcd_mi355x_last_error.argtypes = [H]
     L.nqcd_mi355x_last_error.restype = C.c_char_p
     for name in EXPORTED_SYMBOLS:
@@ -181,13 +190,14 @@ class Ensemble:
     def set_problem(self, method, n_states, ndofs, natoms, masses, dt, algorithm=ALG_BAB_TSIT5, nbeads=1,
                     n_electrons=0, rescaling=RESCALE_STANDARD, precision=PREC_FP64, schedule=SCHED_PERSISTENT,
                     electronic_substeps=5, temperature=0.0, decoherence=DECOHERENCE_NONE, edc_C=0.1,
-                    estimate_probability=True):
+                    estimate_probability=True, bandwidth=0.0, fermi_level=0.0):
         self._masses = np.ascontiguousarray(np.broadcast_to(np.asarray(masses, dtype=np.float64), (natoms,)))
         d = ProblemDesc()
         d.method, d.algorithm, d.n_states, d.ndofs, d.natoms, d.nbeads = method, algorithm, n_states, ndofs, natoms, nbeads
         d.n_electrons, d.rescaling, d.precision, d.schedule = n_electrons, rescaling, precision, schedule
         d.electronic_substeps, d.masses, d.dt, d.temperature = electronic_substeps, _dp(self._masses), dt, temperature
         d.decoherence, d.edc_C, d.disable_estimate = decoherence, edc_C, 0 if estimate_probability else 1
+        d.bandwidth, d.fermi_level = bandwidth, fermi_level
         self._check(self._L.nqcd_mi355x_set_problem(self._h, C.byref(d)))
         self.method, self.n_electrons = method, n_electrons
         self.n_states, self.D, self.dt = n_states, ndofs * natoms, dt
@@ -198,14 +208,30 @@ class Ensemble:
         p = np.ascontiguousarray(params, dtype=np.float64)
         self._check(self._L.nqcd_mi355x_set_model(self._h, slot, kind, _dp(p), len(p)))
 
+    def set_cell(self, mode, vectors=None, periodic=None):
+        """Simulation cell of the cell callbacks: mode CELL_BOUNDARY (wrap) or CELL_TERMINATE, vectors the
+        ndofs x ndofs cell matrix with the lattice vectors as columns, periodic ndofs flags (None: all);
+        CELL_NONE clears it."""
+        if mode == CELL_NONE:
+            self._check(self._L.nqcd_mi355x_set_cell(self._h, CELL_NONE, 0, None, None))
+            return
+        a = np.asarray(vectors, dtype=np.float64)
+        a = a.reshape(1, 1) if a.ndim == 0 else a
+        nd = a.shape[0]
+        col = np.ascontiguousarray(a.T)   # column-major
+        per = None if periodic is None else np.ascontiguousarray(np.asarray(periodic, dtype=np.int32).ravel())
+        self._check(self._L.nqcd_mi355x_set_cell(self._h, mode, nd, _dp(col), _ip(per)))
+
     def load_initial(self, r, v, sig, state=None, model_id=None):
         r = np.ascontiguousarray(r, dtype=np.float64)
         ntraj = r.shape[0]
         v = np.ascontiguousarray(v, dtype=np.float64)
-        sig = np.asarray(sig)
-        # Julia column-major (n, n, ntraj) == C row-major (ntraj, n, n) transposed per matrix
-        sr = np.ascontiguousarray(np.real(sig).transpose(0, 2, 1), dtype=np.float64)
-        si = np.ascontiguousarray(np.imag(sig).transpose(0, 2, 1), dtype=np.float64)
+        sr = si = None
+        if sig is not None:  # None for problems without a density (RPMD, CME / BCME)
+            sig = np.asarray(sig)
+            # Julia column-major (n, n, ntraj) == C row-major (ntraj, n, n) transposed per matrix
+            sr = np.ascontiguousarray(np.real(sig).transpose(0, 2, 1), dtype=np.float64)
+            si = np.ascontiguousarray(np.imag(sig).transpose(0, 2, 1), dtype=np.float64)
         st = None if state is None else np.ascontiguousarray(state, dtype=np.int32)
         mid = None if model_id is None else np.ascontiguousarray(model_id, dtype=np.int32)
         self._check(self._L.nqcd_mi355x_load_initial(self._h, ntraj, _dp(r), _dp(v), _dp(sr), _dp(si), _ip(st), _ip(mid)))
@@ -245,22 +271,27 @@ class Ensemble:
         self._check(self._L.nqcd_mi355x_sample_fermi_dirac(self._h, float(beta), _ip(av), 0 if av is None else len(av),
                                                            1 if diabatic else 0, C.c_uint64(seed)))
 
+    @staticmethod
+    def _term(term):
+        """TermDesc pointer of (dof, value[, kind]) (kind TERM_ABS_GT by default), or None."""
+        if term is None:
+            return None
+        td = TermDesc()
+        td.kind, td.dof, td.value = (term[2] if len(term) > 2 else TERM_ABS_GT), term[0], term[1]
+        return C.pointer(td)
+
     def run(self, t0, t1, key, save_every=0, term=None):
-        td = None
-        if term is not None:
-            td = TermDesc()
-            td.kind, td.dof, td.value = (term[2] if len(term) > 2 else TERM_ABS_GT), term[0], term[1]
-        self._check(self._L.nqcd_mi355x_run(self._h, t0, t1, save_every, C.byref(td) if td is not None else None,
-                                            C.c_uint64(key)))
+        self._check(self._L.nqcd_mi355x_run(self._h, t0, t1, save_every, self._term(term), C.c_uint64(key)))
         nsteps = int(round((t1 - t0) / self.dt))
         self.nsave = nsteps // save_every + 1 if save_every > 0 else 1
         return self.stats()
 
-    def autotune(self, t0, t1, key):
-        """Time the persistent kernel's candidate launch shapes on [t0, t1] of the loaded problem and keep
-        the fastest for later runs (nqcd_mi355x_autotune); the handle's results are overwritten."""
+    def autotune(self, t0, t1, key, term=None):
+        """Time the persistent kernel's candidate launch shapes on [t0, t1] of the loaded problem (with the
+        termination predicate `term`, as for run) and keep the fastest for later runs
+        (nqcd_mi355x_autotune); the handle's results are overwritten."""
         r = TuneResult()
-        self._check(self._L.nqcd_mi355x_autotune(self._h, t0, t1, C.c_uint64(key), C.byref(r)))
+        self._check(self._L.nqcd_mi355x_autotune(self._h, t0, t1, self._term(term), C.c_uint64(key), C.byref(r)))
         return {k: getattr(r, k) for k, _ in TuneResult._fields_}
 
     def stats(self):
@@ -290,7 +321,10 @@ class Ensemble:
                   FIELD_SAVE_POP_AD: ((nt, nsave, n), np.float64), FIELD_SAVE_POP_DI: ((nt, nsave, n), np.float64),
                   FIELD_SAVE_STATE: ((nt, nsave), np.int32), FIELD_FALLBACKS: ((nt,), np.int32),
                   FIELD_SAVE_R: ((nt, nsave, D), np.float64), FIELD_SAVE_EKIN: ((nt, nsave), np.float64),
-                  FIELD_SAVE_EPOT: ((nt, nsave), np.float64)}
+                  FIELD_SAVE_EPOT: ((nt, nsave), np.float64), FIELD_FINAL_POP_DI: ((nt, n), np.float64),
+                  FIELD_SAVE_V: ((nt, nsave), np.float64)}
+        if getattr(self, "method", None) in (METHOD_CME, METHOD_BCME, METHOD_CLASSICAL):  # one coordinate
+            shapes[FIELD_SAVE_R] = ((nt, nsave), np.float64)
         shape, dtype = shapes[field]
         out = np.zeros(shape, dtype=dtype)
         self._check(self._L.nqcd_mi355x_download(self._h, field, out.ctypes.data_as(C.c_void_p), out.nbytes))
@@ -336,6 +370,13 @@ class Ensemble:
         fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
         self._check(self._L.nqcd_mi355x_surrogate_get_params(self._h, member, fp(p), fp(nv)))
         return p, nv
+
+    def test_broadening(self, hs, mu, kT, Gamma, W):
+        """The device's BCME broadening integrals n(h), n2(h) (test hook)."""
+        hs = np.ascontiguousarray(hs, dtype=np.float64)
+        n, n2 = np.zeros_like(hs), np.zeros_like(hs)
+        self._check(self._L.nqcd_mi355x_test_broadening(self._h, _dp(hs), len(hs), mu, kT, Gamma, W, _dp(n), _dp(n2)))
+        return n, n2
 
     def test_determinant(self, S):
         """Determinants of complex matrices S (nmat, n, n) by the device routine of the IESH

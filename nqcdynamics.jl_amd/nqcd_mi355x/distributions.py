@@ -69,6 +69,12 @@ def VelocityHarmonicWigner(omega, beta, mass):
     return Normal(0.0, np.sqrt(omega / (2 * mass * np.tanh(beta * omega / 2))))
 
 
+def VelocityBoltzmann(temperature, mass):
+    """Maxwell-Boltzmann velocity of one degree of freedom (NQCDistributions' VelocityBoltzmann):
+    Normal(0, sqrt(kT / m))."""
+    return Normal(0.0, np.sqrt(float(temperature) / float(mass)))
+
+
 def _moments(spec, shape):
     """(mean, sd) arrays of `shape` for a fixed value, Normal, or array of Normals/values."""
     if isinstance(spec, Normal):

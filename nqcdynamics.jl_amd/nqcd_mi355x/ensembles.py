@@ -11,9 +11,10 @@ import numpy as np
 
 from . import _lib as L
 from .distributions import Adiabatic, DynamicalDistribution, FermiDiracState, ProductDistribution, PureState
-from .outputs import (MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction, SumReduction,
-                      output_name)
-from .simulations import AdiabaticIESH, Classical, Ehrenfest, EhrenfestNA, FSSH, RingPolymerSimulation, Simulation
+from .outputs import (FileReduction, MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction,
+                      SumReduction, output_name)
+from .simulations import (AdiabaticIESH, BCME, CME, Classical, Ehrenfest, EhrenfestNA, FSSH, PeriodicCell,
+                          RingPolymerSimulation, Simulation)
 
 log = logging.getLogger("nqcd_mi355x")
 
@@ -48,9 +49,38 @@ class TerminatingCallback:
         self.dof, self.value, self.predicate = int(dof), float(value), predicate
 
 
+class CellBoundaryCallback:
+    """CellBoundaryCallback() (DynamicsUtils/callbacks.jl:9-15): after a step in which an atom left the
+    simulation's PeriodicCell, wrap the atoms into the cell (evaluated on the device)."""
+
+
+class TerminateCellCallback:
+    """TerminateCellCallback() (DynamicsUtils/callbacks.jl:17-22): end the trajectory after a step in which
+    an atom left the simulation's PeriodicCell (evaluated on the device)."""
+
+
+def _callbacks(callback):
+    """(termination predicate, cell mode) of a callback or a tuple / list of them (a CallbackSet)."""
+    cbs = [] if callback is None else list(callback) if isinstance(callback, (tuple, list)) else [callback]
+    term, cell = None, L.CELL_NONE
+    for cb in cbs:
+        if isinstance(cb, TerminatingCallback):
+            if term is not None:
+                raise ValueError("EnsembleMI355X evaluates one TerminatingCallback predicate")
+            term = (cb.dof - 1, cb.value, TerminatingCallback.PREDICATES[cb.predicate])
+        elif isinstance(cb, (CellBoundaryCallback, TerminateCellCallback)):
+            if cell != L.CELL_NONE:
+                raise ValueError("EnsembleMI355X evaluates one cell callback")
+            cell = L.CELL_BOUNDARY if isinstance(cb, CellBoundaryCallback) else L.CELL_TERMINATE
+        else:
+            raise ValueError("EnsembleMI355X evaluates TerminatingCallback(|x_dof| > value), CellBoundaryCallback and "
+                             "TerminateCellCallback on the device")
+    return term, cell
+
+
 def _kind(sim):
-    """'sh' (FSSH/Ehrenfest), 'ie' (IESH/EhrenfestNA, also as ring polymers with BCBWavefunction) or
-    'rp' (ring polymers); raises for the rest."""
+    """'sh' (FSSH/Ehrenfest), 'ie' (IESH/EhrenfestNA, also as ring polymers with BCBWavefunction), 'rp'
+    (ring polymers), 'cme' (CME / BCME) or 'cl' (classical dynamics); raises for the rest."""
     m = sim.method
     if isinstance(sim, RingPolymerSimulation):
         if isinstance(m, (Classical, FSSH, Ehrenfest)):
@@ -61,6 +91,10 @@ def _kind(sim):
         return "sh"
     elif isinstance(m, (AdiabaticIESH, EhrenfestNA)):
         return "ie"
+    elif isinstance(m, (CME, BCME)):
+        return "cme"
+    elif isinstance(m, Classical) and sim.model.nstates == 1 and sim.ndofs * sim.natoms == 1:
+        return "cl"
     raise ValueError(f"EnsembleMI355X does not support {type(sim).__name__}{{{type(m).__name__}}}")
 
 
@@ -82,6 +116,17 @@ def _problem(E, sim, dt):
                       decoherence=L.DECOHERENCE_EDC if edc else L.DECOHERENCE_NONE,
                       edc_C=method.decoherence.C if edc else 0.1,
                       estimate_probability=getattr(method, "estimate_probability", True), **common)
+    elif kind == "cl":
+        if sim.precision != "fp64":
+            raise ValueError("classical dynamics run in FP64")
+        E.set_problem(L.METHOD_CLASSICAL, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=L.ALG_VELOCITY_VERLET, temperature=sim.temperature, **common)
+    elif kind == "cme":
+        if sim.precision != "fp64":
+            raise ValueError("CME / BCME run in FP64")
+        E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=L.ALG_TSIT5, temperature=sim.temperature, bandwidth=method.bandwidth,
+                      fermi_level=getattr(sim.model, "fermi_level", 0.0), **common)
     else:
         code = {Classical: L.METHOD_RPMD, FSSH: L.METHOD_RPSH, Ehrenfest: L.METHOD_RP_EHRENFEST}[type(method)]
         E.set_problem(code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
@@ -180,6 +225,42 @@ def _initial_rp(E, sim, dist, trajectories, selection, seed):
     E.load_initial(r, v, sig, state=state)
 
 
+def _initial_cme(E, sim, dist, trajectories, selection, seed):
+    """CME / BCME: a diabatic PureState (DynamicsVariables(sim, v, r, ::PureState{Diabatic}), cme.jl:20-24)
+    and one coordinate, sampled on the device from a Gaussian distribution or uploaded."""
+    el = _electronic(dist)
+    if not el.diabatic:
+        raise ValueError("CME / BCME start from a PureState in the Diabatic basis")
+    if el.state not in (1, 2):
+        raise ValueError(f"electronic state {el.state} out of range 1..2")
+    nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    conf = _configurations(nuc, trajectories, selection)
+    if conf is not None:
+        r, v = conf
+        E.load_initial(r, v, None, state=np.full(r.shape[0], el.state - 1, np.int32))
+    elif isinstance(nuc, DynamicalDistribution):
+        mr, sr, mv, sv = nuc.moments()
+        E.sample_initial(L.SAMPLER_GAUSSIAN, trajectories, [mr[0], sr[0], mv[0], sv[0]], seed,
+                         initial_state=el.state - 1)
+    else:
+        raise ValueError(f"unsupported distribution {type(nuc).__name__}")
+
+
+def _initial_cl(E, sim, dist, trajectories, selection, seed):
+    """Classical dynamics: positions and velocities only (DynamicsVariables(sim, v, r)), sampled on the
+    device from a Gaussian distribution or uploaded."""
+    nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    conf = _configurations(nuc, trajectories, selection)
+    if conf is not None:
+        r, v = conf
+        E.load_initial(r, v, None)
+    elif isinstance(nuc, DynamicalDistribution):
+        mr, sr, mv, sv = nuc.moments()
+        E.sample_initial(L.SAMPLER_GAUSSIAN, trajectories, [mr[0], sr[0], mv[0], sv[0]], seed)
+    else:
+        raise ValueError(f"unsupported distribution {type(nuc).__name__}")
+
+
 def _initial_conditions(E, sim, dist, trajectories, selection, seed):
     """Sample on the device where the distribution is recognised, otherwise upload."""
     kind = _kind(sim)
@@ -187,6 +268,10 @@ def _initial_conditions(E, sim, dist, trajectories, selection, seed):
         return _initial_ie(E, sim, dist, trajectories, selection, seed)
     if kind == "rp":
         return _initial_rp(E, sim, dist, trajectories, selection, seed)
+    if kind == "cme":
+        return _initial_cme(E, sim, dist, trajectories, selection, seed)
+    if kind == "cl":
+        return _initial_cl(E, sim, dist, trajectories, selection, seed)
     el = _electronic(dist)
     nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
     D = sim.ndofs * sim.natoms
@@ -240,6 +325,44 @@ def _results(E, t0, dt, save_every, nsteps):
     return res
 
 
+def save_schedule(saveat, t0, t1, dt):
+    """Save slots for SciML's saveat (which run_dynamics.jl:91-97 forwards): None saves every step; a number
+    d saves t0:d:t1; a sequence must start at t0 and be evenly spaced by a multiple of dt.  Returns
+    (save_every, nsave, times); the device saves slot k at step k * save_every."""
+    nsteps = int(round((t1 - t0) / dt))
+    tol = 1e-9 * max(1.0, abs(dt))
+
+    def multiple(d):
+        k = int(round(d / dt))
+        if k < 1 or abs(k * dt - d) > tol * k:
+            raise ValueError(f"saveat spacing {d} is not a positive multiple of dt = {dt}")
+        return k
+
+    if saveat is None:
+        k, nsave = 1, nsteps + 1
+    elif np.isscalar(saveat):
+        k = multiple(float(saveat))
+        if nsteps % k:
+            raise ValueError("t1 - t0 must be a multiple of saveat")
+        nsave = nsteps // k + 1
+    else:
+        ts = np.asarray(saveat, dtype=np.float64).ravel()
+        if ts.size == 0:
+            raise ValueError("empty saveat")
+        if abs(ts[0] - t0) > tol:
+            raise ValueError("saveat must start at tspan[0]")
+        if ts.size == 1:
+            return 0, 1, ts
+        d = np.diff(ts)
+        if np.any(np.abs(d - d[0]) > tol * ts.size):
+            raise ValueError("saveat times must be evenly spaced (non-uniform saveat is not supported)")
+        k = multiple(float(d[0]))
+        if (ts.size - 1) * k > nsteps:
+            raise ValueError("saveat extends past tspan[1]")
+        return k, int(ts.size), ts
+    return k, nsave, t0 + dt * k * np.arange(nsave)
+
+
 def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=None, ensemble_algorithm=None,
                  trajectories=1, savetime=True, precompile_dynamics=False, dt=1.0, saveat=None, callback=None,
                  **kwargs):
@@ -255,21 +378,14 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
     outputs = output if isinstance(output, tuple) else (output,)
     t0, t1 = float(tspan[0]), float(tspan[1])
     nsteps = int(round((t1 - t0) / dt))
-    if saveat is None:
-        save_every = 0
-    else:
-        step = float(saveat) if np.isscalar(saveat) else float(np.asarray(saveat)[1] - np.asarray(saveat)[0])
-        save_every = int(round(step / dt))
-        if save_every < 1 or abs(save_every * dt - step) > 1e-9 * max(1.0, step):
-            raise ValueError("saveat must be a positive multiple of dt")
-    term = None
-    if callback is not None:
-        if not isinstance(callback, TerminatingCallback):
-            raise ValueError("EnsembleMI355X evaluates TerminatingCallback(|x_dof| > value) predicates only")
-        term = (callback.dof - 1, callback.value, TerminatingCallback.PREDICATES[callback.predicate])
+    save_every, nsave_user, times = save_schedule(saveat, t0, t1, dt)
+    term, cell = _callbacks(callback)
     kind = _kind(sim)
-    if kind != "sh" and callback is not None:
-        raise ValueError("termination callbacks are supported for FSSH / Ehrenfest simulations")
+    if kind not in ("sh", "cme", "cl") and callback is not None:
+        raise ValueError("termination and cell callbacks are supported for FSSH / Ehrenfest / CME / BCME / classical "
+                         "simulations")
+    if not isinstance(getattr(sim, "cell", None), PeriodicCell):
+        cell = L.CELL_NONE   # InfiniteCell: every atom is inside, the cell callbacks never fire
     if int(trajectories) < 1:
         raise ValueError("trajectories must be >= 1")
     _fermi_dirac(sim, distribution)
@@ -282,18 +398,18 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
     E._schedule = ensemble_algorithm.schedule
     try:
         _problem(E, sim, dt)
+        if cell != L.CELL_NONE:
+            E.set_cell(cell, sim.cell.vectors, sim.cell.periodicity)
         _initial_conditions(E, sim, distribution, int(trajectories), selection, ensemble_algorithm.seed)
         if getattr(ensemble_algorithm, "autotune", False) and kind == "sh":
             try:
                 ensemble_algorithm.tuning = E.autotune(t0, t0 + dt * min(nsteps, ensemble_algorithm.autotune_steps),
-                                                       ensemble_algorithm.seed)
+                                                       ensemble_algorithm.seed, term=term)
             except L.NQCDError as e:
                 if e.code != L.ERR_UNSUPPORTED:
                     raise
         stats = E.run(t0, t1, ensemble_algorithm.seed, save_every=save_every, term=term)
-        nsave = E.nsave
-        times = t0 + dt * save_every * np.arange(nsave) if save_every else np.array([t0, t1])
-        result = _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nsteps)
+        result = _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nsteps, nsave_user)
     finally:
         E.close()
     if isinstance(reduction, SortByTrajectoryReduction) and trajectories == 1:
@@ -301,7 +417,18 @@ def run_dynamics(sim, tspan, distribution, *, output, selection=None, reduction=
     return result
 
 
-def _per_traj_results(E, sim, t0, dt, save_every, nsteps):
+def _per_traj_results(E, sim, t0, dt, save_every, nsteps, nsave=None):
+    """Per-trajectory dictionaries of the device fields (per-save arrays cut to the first nsave slots)."""
+    out = _per_traj_fields(E, sim, t0, dt, save_every, nsteps)
+    if nsave is not None and nsave < E.nsave:
+        for d in out:
+            for k in ("pop_ad", "pop_di", "save_state", "centroid", "ekin", "epot", "save_r", "save_v"):
+                if k in d:
+                    d[k] = d[k][:nsave]
+    return out
+
+
+def _per_traj_fields(E, sim, t0, dt, save_every, nsteps):
     kind = _kind(sim)
     if kind == "ie":
         res = {"r": E.download(L.FIELD_R), "v": E.download(L.FIELD_V),
@@ -313,6 +440,26 @@ def _per_traj_results(E, sim, t0, dt, save_every, nsteps):
                  "state": res["state"][i], "steps": int(res["steps"][i]), "nhops": int(res["nhops"][i]),
                  "pop_ad": res["pop_ad"][i], "pop_di": res["pop_di"][i], "final_pop_ad": res["pop_ad"][i][-1],
                  "t0": t0, "dt": dt} for i in range(E.ntraj)]
+    if kind == "cl":
+        r, v, stp = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_STEPS)
+        sr, sv = E.download(L.FIELD_SAVE_R), E.download(L.FIELD_SAVE_V)
+        ek, ep = E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
+        shp = sim.size()
+        return [{"r": r[i].reshape(shp, order="F"), "v": v[i].reshape(shp, order="F"), "steps": int(stp[i]),
+                 "save_r": [x.reshape(shp, order="F") for x in sr[i]],
+                 "save_v": [x.reshape(shp, order="F") for x in sv[i]], "ekin": ek[i], "epot": ep[i],
+                 "t0": t0, "dt": dt} for i in range(E.ntraj)]
+    if kind == "cme":
+        r, v, st = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_STATE)
+        nh, stp = E.download(L.FIELD_NHOPS), E.download(L.FIELD_STEPS)
+        sr, ek, ep = E.download(L.FIELD_SAVE_R), E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
+        ss = E.download(L.FIELD_SAVE_STATE)
+        shp = sim.size()
+        return [{"r": r[i].reshape(shp, order="F"), "v": v[i].reshape(shp, order="F"), "state": int(st[i]),
+                 "steps": int(stp[i]), "nhops": int(nh[i]), "save_r": [x.reshape(shp, order="F") for x in sr[i]],
+                 "ekin": ek[i], "epot": ep[i], "save_state": ss[i], "sigma": None,
+                 "final_pop_ad": np.eye(2)[st[i]], "final_pop_di": np.eye(2)[st[i]], "t0": t0, "dt": dt}
+                for i in range(E.ntraj)]
     if kind == "rp":
         r, v = E.download(L.FIELD_R), E.download(L.FIELD_V)
         cen = E.download(L.FIELD_SAVE_CENTROID)
@@ -343,18 +490,26 @@ def _per_traj_results(E, sim, t0, dt, save_every, nsteps):
     else:
         fin = np.real(np.einsum("tii->ti", res["sigma"]))
     res["final_pop_ad"] = fin
+    try:  # diag(U sigma' U^T) of the final state (fssh.jl:132-142, ehrenfest.jl:75-83): two-state one-dof problems
+        fin_di = E.download(L.FIELD_FINAL_POP_DI)
+    except L.NQCDError:
+        fin_di = None
     out = []
     for i in range(E.ntraj):
         out.append({"r": res["r"][i].reshape(sim.size(), order="F"), "v": res["v"][i].reshape(sim.size(), order="F"),
                     "sigma": res["sigma"][i], "state": int(res["state"][i]), "steps": int(res["steps"][i]),
                     "nhops": int(res["nhops"][i]), "pop_ad": res["pop_ad"][i], "pop_di": res["pop_di"][i],
                     "save_state": res["save_state"][i], "final_pop_ad": fin[i], "t0": t0, "dt": dt})
+        if fin_di is not None:
+            out[-1]["final_pop_di"] = fin_di[i]
         if "ekin" in res:
             out[-1].update({"ekin": res["ekin"][i], "epot": res["epot"][i]})
     return out
 
 
-def _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nsteps):
+def _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nsteps, nsave):
+    """The reduction's result over the device results; per-save outputs keep the first nsave slots (a
+    saveat sequence that stops before t1)."""
     n = sim.model.nstates
     if isinstance(reduction, (MeanReduction, SumReduction)):
         red = L.RED_MEAN if isinstance(reduction, MeanReduction) else L.RED_SUM
@@ -365,29 +520,34 @@ def _reduce(E, sim, outputs, reduction, times, savetime, t0, dt, save_every, nst
             if kind is not None:
                 if kind in (L.OUT_ADIABATIC_POPULATION, L.OUT_DIABATIC_POPULATION):
                     v = E.reduce(kind, red, n * E.nsave).reshape(E.nsave, n)
-                    result[output_name(f)] = list(v)
-                elif kind == L.OUT_SCATTERING_1D:
+                    result[output_name(f)] = list(v[:nsave])
+                elif kind in (L.OUT_SCATTERING_1D, L.OUT_SCATTERING_1D_DIABATIC):
                     v = E.reduce(kind, red, 2 * n)
                     result[output_name(f)] = {"reflection": v[:n], "transmission": v[n:]}
+                elif kind in (L.OUT_POSITION, L.OUT_VELOCITY):
+                    v = E.reduce(kind, red, E.nsave * E.D)
+                    result[output_name(f)] = [x.reshape(sim.size(), order="F") for x in v.reshape(E.nsave, E.D)[:nsave]]
                 elif kind in (L.OUT_CENTROID_POSITION, L.OUT_KINETIC_ENERGY, L.OUT_TOTAL_ENERGY):
-                    result[output_name(f)] = list(E.reduce(kind, red, E.nsave))
+                    result[output_name(f)] = list(E.reduce(kind, red, E.nsave)[:nsave])
                 elif kind == L.OUT_POPULATION_CORRELATION:
                     v = E.reduce(kind, red, n * n * E.nsave).reshape(E.nsave, n, n)
-                    result[output_name(f)] = [m.T for m in v]  # column-major n x n per save
+                    result[output_name(f)] = [m.T for m in v[:nsave]]  # column-major n x n per save
                 continue
             if host is None:
-                host = _per_traj_results(E, sim, t0, dt, save_every, nsteps)
+                host = _per_traj_results(E, sim, t0, dt, save_every, nsteps, nsave)
             vals = [f.per_trajectory(h) for h in host]
             tot = _tree_sum(vals)
             result[output_name(f)] = _scale(tot, 1.0 / len(vals)) if red == L.RED_MEAN else tot
         return result
-    host = _per_traj_results(E, sim, t0, dt, save_every, nsteps)
+    host = _per_traj_results(E, sim, t0, dt, save_every, nsteps, nsave)
     per = []
     for h in host:
         d = {"Time": times} if savetime else {}
         for f in outputs:
             d[output_name(f)] = f.per_trajectory(h) if isinstance(f, Output) else f(h, None)
         per.append(d)
+    if isinstance(reduction, FileReduction):  # reductions.jl:72-91, one batch of every trajectory
+        return reduction([], per, range(1, len(per) + 1))[0]
     if isinstance(reduction, SortByOutputReduction):
         keys = per[0].keys() if per else []
         return {k: [d[k] for d in per] for k in keys}

@@ -171,6 +171,52 @@ def nelectrons(model):
     return model.nelectrons
 
 
+class MorseN(Model):
+    """n-state Morse model of one coordinate (Coker and Xiao, J. Chem. Phys. 102, 496 (1995); NQCModels'
+    ThreeStateMorse is n = 3): V_ii = D_i (1 - e^{-beta_i (x - r_i)})^2 + c_i, V_ij = A_ij e^{-a_ij (x - r_ij)^2}.
+    `states` is a list of (D, beta, r, c) per state and `couplings` a dict {(i, j): (A, a, r)} over i < j
+    (0-based; missing pairs are uncoupled)."""
+    kind = L.MODEL_MORSE_N
+
+    def __init__(self, states, couplings=None):
+        self.states = [tuple(float(x) for x in st) for st in states]
+        self.nstates = len(self.states)
+        if not 2 <= self.nstates <= 8:
+            raise ValueError("MorseN takes 2 to 8 states")
+        self.couplings = {tuple(k): tuple(float(x) for x in v) for k, v in (couplings or {}).items()}
+
+    @classmethod
+    def from_table(cls, n, table):
+        t = np.asarray(table, dtype=np.float64)
+        st = t[:4 * n].reshape(n, 4)
+        pairs = t[4 * n:].reshape(-1, 3)
+        cp, k = {}, 0
+        for i in range(n):
+            for j in range(i + 1, n):
+                cp[(i, j)] = tuple(pairs[k]); k += 1
+        return cls(st, cp)
+
+    def params(self):
+        n = self.nstates
+        out = [n] + [x for st in self.states for x in st]
+        for i in range(n):
+            for j in range(i + 1, n):
+                out += list(self.couplings.get((i, j), (0.0, 1.0, 0.0)))
+        return out
+
+
+class AndersonHolsteinOneMode(Model):
+    """One-mode Anderson-Holstein impurity of the CME / BCME tests (test/Dynamics/cme.jl:27-55): V11 = hw x^2/2,
+    V22 = V11 + Ed + sqrt(2) g x, V12 = sqrt(Gamma / 2 pi); Fermi level of the metal `fermi_level`."""
+    kind = L.MODEL_ANDERSON_HOLSTEIN_1
+
+    def __init__(self, hw, Ed, g, Gamma, fermi_level=0.0):
+        self.hw, self.Ed, self.g, self.Gamma, self.fermi_level = float(hw), float(Ed), float(g), float(Gamma), float(fermi_level)
+
+    def params(self):
+        return [self.hw, self.Ed, np.sqrt(2) * self.g, np.sqrt(self.Gamma / (2 * np.pi))]
+
+
 class DiatomSurface(Model):
     """Diatomic molecule above a surface, six coordinates (ndofs = 3, natoms = 2), two diabats:
     bond b = |R1 - R2|, height Z = mu z1 + (1 - mu) z2,

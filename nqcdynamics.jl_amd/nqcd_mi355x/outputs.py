@@ -5,6 +5,8 @@ Each output turns the device results of one run into the per-trajectory value th
 reference's `f(sol, i)` returns.  Mean/Sum reductions of the outputs that have a
 device reduction run on the GPU in a fixed order; the others reduce on the host.
 """
+import os
+
 import numpy as np
 
 from . import _lib as L
@@ -45,8 +47,13 @@ OutputCentroidPosition = _FnOutput("OutputCentroidPosition", lambda r: list(r["c
 # FSSH / Ehrenfest (potential: fssh.jl:150-154, ehrenfest.jl:85-95).
 OutputKineticEnergy = _FnOutput("OutputKineticEnergy", lambda r: list(r["ekin"]), L.OUT_KINETIC_ENERGY)
 OutputTotalEnergy = _FnOutput("OutputTotalEnergy", lambda r: list(r["ekin"] + r["epot"]), L.OUT_TOTAL_ENERGY)
+# OutputPosition / OutputVelocity (DynamicsOutputs.jl:62-66, 35-39): per save (CME / BCME, classical).
+OutputPosition = _FnOutput("OutputPosition", lambda r: list(r["save_r"]), L.OUT_POSITION)
+OutputVelocity = _FnOutput("OutputVelocity", lambda r: list(r["save_v"]), L.OUT_VELOCITY)
 # OutputFinal (DynamicsOutputs.jl:222): final dynamics variables.
-OutputFinal = _FnOutput("OutputFinal", lambda r: {"v": r["v"], "r": r["r"], "sigma": r["sigma"], "state": r["state"] + 1})
+OutputFinal = _FnOutput("OutputFinal", lambda r: {"v": r["v"], "r": r["r"], **(
+    {"sigma": r["sigma"], "state": r["state"] + 1} if r.get("sigma") is not None else
+    {"state": r["state"] + 1} if "state" in r else {})})
 
 
 class OutputStateResolvedScattering1D(Output):
@@ -57,10 +64,12 @@ class OutputStateResolvedScattering1D(Output):
         if kind not in ("adiabatic", "diabatic"):
             raise ValueError(f"{kind} not recognised. Only `:diabatic` or `:adiabatic` accepted.")
         self.sim, self.kind = sim, kind
-        self.device_reduction = L.OUT_SCATTERING_1D if kind == "adiabatic" else None
+        self.device_reduction = L.OUT_SCATTERING_1D if kind == "adiabatic" else L.OUT_SCATTERING_1D_DIABATIC
 
     def per_trajectory(self, res):
-        pops = res["final_pop_ad"] if self.kind == "adiabatic" else res["pop_di_final"]
+        if self.kind == "diabatic" and "final_pop_di" not in res:
+            raise ValueError("diabatic final populations exist for two-state one-dof FSSH / Ehrenfest")
+        pops = res["final_pop_ad"] if self.kind == "adiabatic" else res["final_pop_di"]
         x = res["r"].reshape(-1)[0]
         z = np.zeros_like(pops)
         return {"reflection": z if x > 0 else pops, "transmission": pops if x > 0 else z}
@@ -84,24 +93,38 @@ class PopulationCorrelationFunction(Output):
 
 # --- reductions -----------------------------------------------------------------------------
 class SortByTrajectoryReduction:
-    """One dictionary per trajectory (reductions.jl:208-209)."""
+    """One dictionary per trajectory (reductions.jl:54-55)."""
 
 
 class SortByOutputReduction:
-    """One list per output name (reductions.jl:272-297)."""
+    """One list per output name (reductions.jl:117-143)."""
 
 
 class MeanReduction:
-    """Mean over trajectories (reductions.jl:190-206)."""
+    """Mean over trajectories (reductions.jl:33-52)."""
 
 
 class SumReduction:
-    """Sum over trajectories (reductions.jl:162-185)."""
+    """Sum over trajectories (reductions.jl:5-31)."""
 
 
 class FileReduction:
+    """Writes each trajectory's output dictionary to an HDF5 file (reductions.jl:57-91): group
+    `trajectory_<id>` (1-based trajectory ids), one dataset per key, in HDF5.jl's layout (see h5.py).
+    A filename without a .h5 / .hdf5 extension gets .h5 in place of its extension."""
+
     def __init__(self, filename):
-        raise NotImplementedError("FileReduction (HDF5) is not provided by the MI355X backend")
+        root, ext = os.path.splitext(str(filename))
+        self.filename = str(filename) if ext in (".h5", ".hdf5") else root + ".h5"
+
+    def __call__(self, u, batch, ids):
+        from . import h5
+        with h5.File(self.filename) as f:
+            for i, traj in zip(ids, batch):
+                with f.create_group(f"trajectory_{i}") as g:
+                    for k, value in traj.items():
+                        g.write(str(k), h5.file_order(value))
+        return f"Output written to {self.filename}.", False
 
 
 def output_name(f):

@@ -8,7 +8,7 @@ import numpy as np
 
 from . import _lib as L
 
-_ELEMENT_MASS = {"H": 1837.4715941070515, "D": 3671.4829, "He": 7296.2993}
+_ELEMENT_MASS = {"H": 1837.4715941070515, "D": 3671.4829, "He": 7296.2993, "C": 12.011 * 1822.888486}
 
 
 class Atoms:
@@ -53,9 +53,30 @@ class Ehrenfest(Method):
 
 
 class Classical(Method):
-    """Classical molecular dynamics (classical.jl:22); ring polymers integrate it with BCB."""
-    method_code = None
+    """Classical molecular dynamics (classical.jl:22): VelocityVerlet for a Simulation (classical.jl:86),
+    BCB for ring polymers."""
+    method_code = L.METHOD_CLASSICAL
+    algorithm = L.ALG_VELOCITY_VERLET
     rescaling_code = L.RESCALE_STANDARD
+
+
+class InfiniteCell:
+    """No simulation cell (NQCBase's InfiniteCell): every atom is always inside, the cell callbacks never fire."""
+
+
+class PeriodicCell:
+    """NQCBase's PeriodicCell(vectors; periodicity): the lattice vectors are the columns of `vectors`
+    (ndofs x ndofs); `periodicity` flags the periodic dimensions (all by default)."""
+
+    def __init__(self, vectors, periodicity=None):
+        a = np.asarray(vectors, dtype=np.float64)
+        self.vectors = a.reshape(1, 1) if a.ndim == 0 else a
+        n = self.vectors.shape[0]
+        if self.vectors.shape != (n, n) or not 1 <= n <= 3:
+            raise ValueError("PeriodicCell vectors must be a square matrix of size 1 to 3")
+        self.periodicity = np.ones(n, bool) if periodicity is None else np.asarray(periodicity, bool)
+        if self.periodicity.shape != (n,):
+            raise ValueError("periodicity needs one flag per dimension")
 
 
 class DecoherenceCorrectionEDC:
@@ -91,9 +112,34 @@ class EhrenfestNA(Method):
     rescaling_code = L.RESCALE_STANDARD
 
 
+class CME(Method):
+    """Classical master equation surface hopping for Anderson-Holstein models (cme.jl:63-113); two states,
+    integrated by Tsit5 at the fixed step dt (IntegrationAlgorithms.jl:68)."""
+    method_code = L.METHOD_CME
+    algorithm = L.ALG_TSIT5
+    rescaling_code = L.RESCALE_OFF
+    bandwidth = 0.0
+
+
+class BCME(Method):
+    """Broadened classical master equation (cme.jl:115-182): CME with the Lorentzian-broadened Fermi force
+    over `bandwidth` around the Fermi level."""
+    method_code = L.METHOD_BCME
+    algorithm = L.ALG_TSIT5
+    rescaling_code = L.RESCALE_OFF
+
+    def __init__(self, bandwidth):
+        self.bandwidth = float(bandwidth)
+        if not self.bandwidth > 0:
+            raise ValueError("bandwidth must be positive")
+
+
 class AbstractSimulation:
-    def __init__(self, method, atoms, model, temperature=0.0, precision="fp64", **method_kwargs):
+    def __init__(self, method, atoms, model, temperature=0.0, precision="fp64", cell=None, **method_kwargs):
         self.method = method(**method_kwargs) if isinstance(method, type) else method
+        self.cell = InfiniteCell() if cell is None else cell
+        if not isinstance(self.cell, (InfiniteCell, PeriodicCell)):
+            raise ValueError("cell must be a PeriodicCell or an InfiniteCell")
         self.atoms = atoms
         self.model = model
         self.temperature = float(temperature)

@@ -101,6 +101,8 @@ enum {
     MODEL_DW_GAUSSIAN = 4,
     MODEL_HARMONIC = 5,
     MODEL_DIATOM_SURFACE = 7,
+    MODEL_MORSE_N = 9,
+    MODEL_ANDERSON_HOLSTEIN_1 = 10,
 };
 
 struct OModelDesc {
@@ -111,6 +113,7 @@ struct OModelDesc {
     double p[16];
     const double* omega; // nbath
     const double* cbath; // nbath
+    const double* ext;   // MODEL_MORSE_N: [D, beta, r0, c] per state, then [A, a, r] per pair i < j (row-major)
 };
 
 static void model_eval(const OModelDesc& m, const double* r, double* V, double* dV, double* V0, double* dV0) {
@@ -217,6 +220,38 @@ static void model_eval(const OModelDesc& m, const double* r, double* V, double* 
             dV[(size_t)j * 4 + 3] = db22 * ub[j] + dZ22 * uZ[j];
             dV[(size_t)j * 4 + 1] = dV[(size_t)j * 4 + 2] = dZ12 * uZ[j];
         }
+        break;
+    }
+    case MODEL_MORSE_N: {
+        // n-state Morse model of Coker and Xiao (J. Chem. Phys. 102, 496 (1995); NQCModels'
+        // ThreeStateMorse for n = 3), one coordinate:
+        //   V_ii = D_i (1 - e^{-beta_i (x - r_i)})^2 + c_i,   V_ij = A_ij e^{-a_ij (x - r_ij)^2}
+        const double x = r[0];
+        const double* q = m.ext;
+        for (int i = 0; i < n; ++i) {
+            const double Dm = q[4 * i], be = q[4 * i + 1], r0 = q[4 * i + 2], c = q[4 * i + 3];
+            const double e = std::exp(-be * (x - r0));
+            V[i * n + i] = Dm * (1.0 - e) * (1.0 - e) + c;
+            dV[i * n + i] = 2.0 * Dm * be * e * (1.0 - e);
+        }
+        int pidx = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j, ++pidx) {
+                const double* c = q + 4 * n + 3 * pidx;
+                const double v = c[0] * std::exp(-c[1] * (x - c[2]) * (x - c[2]));
+                V[i * n + j] = V[j * n + i] = v;
+                dV[i * n + j] = dV[j * n + i] = -2.0 * c[1] * (x - c[2]) * v;
+            }
+        break;
+    }
+    case MODEL_ANDERSON_HOLSTEIN_1: {
+        // One-mode Anderson-Holstein impurity (the CME / BCME model of test/Dynamics/cme.jl:27-55, Dou,
+        // Nitzan, Subotnik, J. Chem. Phys. 142, 084110 (2015)): V11 = k x^2/2, V22 = V11 + Ed + c x,
+        // V12 = const.  params: k, Ed, c, V12.
+        const double k = m.p[0], Ed = m.p[1], c = m.p[2], v12 = m.p[3];
+        const double x = r[0];
+        V[0] = 0.5 * k * x * x; V[3] = V[0] + Ed + c * x; V[1] = V[2] = v12;
+        dV[0] = k * x; dV[3] = k * x + c; dV[1] = dV[2] = 0.0;
         break;
     }
     default: break;
@@ -515,6 +550,64 @@ OR_API int32_t oracle_rescale_velocity(int32_t n, int32_t D, int32_t rescaling, 
 // ---------------------------------------------------------------------------
 enum { METHOD_FSSH = 0, METHOD_EHRENFEST = 1 };
 
+// ---------------------------------------------------------------------------
+// Simulation cell of the cell callbacks (DynamicsUtils/callbacks.jl:4-22).  The cell type and its two
+// functions live in NQCBase (Project.toml: NQCBase = "1", not in the reference tree); restated from its
+// published definition: PeriodicCell(vectors) keeps the lattice vectors as the columns of `vectors`, its
+// inverse and per-dimension periodicity; check_atoms_in_cell is true when every fractional coordinate
+// inverse * r of every atom lies in [0, 1]; apply_cell_boundaries! maps each atom to inverse * r, takes
+// Julia's mod(., 1) of the periodic components and multiplies back by `vectors`.  outside_cell is the
+// negated check; CellBoundaryCallback applies the wrap and TerminateCellCallback terminates when an atom
+// is outside after a step.
+// ---------------------------------------------------------------------------
+struct OCell {
+    int32_t mode;      // 0 none, 1 CellBoundaryCallback, 2 TerminateCellCallback
+    int32_t nd;        // dimensions per atom
+    int32_t per[3];
+    int32_t pad;
+    double a[9];       // column-major, leading dimension 3
+    double inv[9];
+};
+
+// Julia's mod(x, 1) for a Float64: r = rem(x, 1); r == 0 -> +0; r + 1 where r < 0
+static double jl_mod1(double x) {
+    const double r = std::fmod(x, 1.0);
+    if (r == 0.0) return 0.0;
+    return (r > 0.0) ? r : r + 1.0;
+}
+
+static bool ocell_outside(const OCell& c, const double* x, int D) {
+    for (int at = 0; at + c.nd <= D; at += c.nd)
+        for (int j = 0; j < c.nd; ++j) {
+            double f = 0.0;
+            for (int k = 0; k < c.nd; ++k) f += c.inv[j + 3 * k] * x[at + k];
+            if (f > 1.0 || f < 0.0) return true;
+        }
+    return false;
+}
+
+static void ocell_wrap(const OCell& c, double* x, int D) {
+    for (int at = 0; at + c.nd <= D; at += c.nd) {
+        double f[3];
+        for (int j = 0; j < c.nd; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < c.nd; ++k) s += c.inv[j + 3 * k] * x[at + k];
+            f[j] = c.per[j] ? jl_mod1(s) : s;
+        }
+        for (int j = 0; j < c.nd; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < c.nd; ++k) s += c.a[j + 3 * k] * f[k];
+            x[at + j] = s;
+        }
+    }
+}
+
+OR_API int32_t oracle_cell_apply(const OCell* c, int32_t D, double* x) {
+    const bool out = ocell_outside(*c, x, D);
+    if (out && c->mode == 1) ocell_wrap(*c, x, D);
+    return out ? 1 : 0;
+}
+
 struct ORunDesc {
     int32_t method;
     int32_t nstates;
@@ -532,6 +625,7 @@ struct ORunDesc {
     int32_t term_dof;     // flattened dof index of the termination predicate
     int32_t term_kind;    // 0: |x| > term_x when term_x > 0; 2: x > term_x; 3: x < term_x
     int32_t pad;
+    OCell cell;           // cell callback after the hop callback (mode 0: none)
 };
 
 // TerminatingCallback(func) (DynamicsUtils/callbacks.jl:29) for the predicates on one coordinate.
@@ -657,7 +751,7 @@ static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, 
         // TerminatingCallback is a DiscreteCallback: its condition is evaluated after each step and
         // not at t0 (DynamicsUtils/callbacks.jl:15-29), so a trajectory that starts past the
         // boundary takes one step
-        if (step > 0 && term_hit(R, r[R.term_dof])) break;
+        if (step > 0 && (term_hit(R, r[R.term_dof]) || (R.cell.mode == 2 && ocell_outside(R.cell, r.data(), D)))) break;
         std::vector<cd> sigprev = sig;
         for (int I = 0; I < D; ++I) vtmp[I] = v[I] + (R.dt / 2) * k[I];   // step_B!
         for (int I = 0; I < D; ++I) r[I] = r[I] + R.dt * vtmp[I];         // step_A!
@@ -668,6 +762,11 @@ static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, 
         update_parameters(P, c.E.data(), c.d.data(), v.data(), D, t + R.dt);
         propagate_density(P, sig.data(), t, R.dt, R.nsub, ws);
         t = t + R.dt;
+        // A save time's values are those before the step's discrete callbacks: DiffEqBase's
+        // apply_discrete_callback! saves the saveat points due before it calls affect!, and
+        // handle_callbacks! saves after the callbacks only when none fired, so either way the slot holds
+        // the state before the hop (a hop at a save time shows in the next slot).
+        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save((int)((step + 1) / R.save_every));
         if (R.method == METHOD_FSSH) {  // HoppingCallback after the step (surface_hopping.jl:2-16)
             fssh_probability(n, D, v.data(), sig.data(), s, c.d.data(), R.dt, prob.data());
             const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
@@ -677,8 +776,10 @@ static void run_one(const ORunDesc& R, const OModelDesc& m, const double* mass, 
             }
             // Q4: k is not recomputed after an accepted hop.
         }
+        // CellBoundaryCallback after the hop callback (the problem's callbacks precede the user's); the
+        // BABwithTsit5 cache keeps the step-end acceleration (bab_electronics.jl:61-74)
+        if (R.cell.mode == 1 && ocell_outside(R.cell, r.data(), D)) ocell_wrap(R.cell, r.data(), D);
         ++step;
-        if (R.save_every > 0 && step % R.save_every == 0) save((int)(step / R.save_every));
     }
     for (int I = 0; I < D; ++I) { O.r[(size_t)i * D + I] = r[I]; O.v[(size_t)i * D + I] = v[I]; }
     for (int J = 0; J < nn; ++J) { O.sig_re[(size_t)i * nn + J] = sig[J].real(); O.sig_im[(size_t)i * nn + J] = sig[J].imag(); }
@@ -914,6 +1015,8 @@ static void rp_run_one(const ORPDesc& R, const OModelDesc& m, double mass, int64
             propagate_density(Pe, sig.data(), t, R.dt, R.nsub, ws);
         }
         t = t + R.dt;
+        // the save precedes the hop callback (the rule of run_one)
+        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save((int)((step + 1) / R.save_every));
         if (R.method == RP_FSSH) {
             const double vc = centroid_of(v);
             fssh_probability(n, 1, &vc, sig.data(), s, cen.d.data(), R.dt, prob.data());
@@ -943,7 +1046,6 @@ static void rp_run_one(const ORPDesc& R, const OModelDesc& m, double mass, int64
                 if (ok) { s = ns; ++nh; }
             }
         }
-        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save((int)((step + 1) / R.save_every));
     }
     for (int b = 0; b < P; ++b) { O.r[(size_t)i * P + b] = r[b]; O.v[(size_t)i * P + b] = v[b]; }
     if (R.method != RP_CLASSICAL) {
@@ -1098,6 +1200,234 @@ OR_API int32_t oracle_pimc(const OPimcDesc* D, const OModelDesc* m, int64_t ntra
         pimc_one(*D, *m, (uint64_t)i, H.data(), x + (size_t)i * P, v + (size_t)i * P, accepted + i);
     });
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Classical master equation surface hopping (cme.jl:26-182): CME (Dou, Nitzan, Subotnik 2015) and
+// BCME (broadened, Dou and Subotnik 2016) on a two-state diabatic model of one coordinate.
+//   motion!: dr = v, dv = a(r, state) (cme.jl:7-18), integrated by Tsit5 (select_algorithm,
+//     IntegrationAlgorithms.jl:68).  The reference's solve adapts the step to abstol / reltol; here
+//     the step is fixed at dt (OrdinaryDiffEq's Tsit5 tableau, no controller).
+//   CME force: -dV_ss/dx / m (cme.jl:96-105); BCME adds f dh - dh n(h) - (dGamma/Gamma) n2(h)
+//     (cme.jl:137-160) with n, n2 the Lorentzian-broadened Fermi integrals over the bandwidth
+//     (cme.jl:167-182; QuadGK's adaptive Gauss-Kronrod, restated here with a 7/15-point pair and a
+//     tighter tolerance).
+//   HoppingCallback after each step (surface_hopping.jl:2-16): Gamma = 2 pi V12^2, f = fermi(V22 - V11,
+//     mu, beta), P = Gamma f dt from state 1 and Gamma (1 - f) dt from state 2 (cme.jl:26-41, dt the
+//     proposed step, here dt); one uniform, hop when u < P (cme.jl:43-53); no rescaling (cme.jl:55-57).
+// ---------------------------------------------------------------------------
+struct OCmeDesc {
+    int32_t method;        // 0 CME, 1 BCME
+    int32_t save_every;
+    int32_t nsave;
+    int32_t pad;
+    double dt, t0, mass, kT, mu, bandwidth;
+    int64_t nsteps;
+    uint64_t key;
+    OCell cell;
+};
+
+static inline double fermi_fn(double e, double mu, double beta) { return 1.0 / (1.0 + std::exp(beta * (e - mu))); }
+
+// Adaptive Gauss-Kronrod (7/15) on [a, b] to |err| <= max(atol, rtol |I|), bisecting the worst panel.
+template <class F>
+static double quad_gk(F f, double a, double b, double rtol) {
+    static const double xk[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                                 0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                                 0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                                 0.207784955007898467600689403773245, 0.0};
+    static const double wk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                                 0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                                 0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                                 0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+    static const double wg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                                 0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+    struct Seg { double a, b, I, E; };
+    auto eval = [&](double lo, double hi) {
+        const double c = 0.5 * (lo + hi), h = 0.5 * (hi - lo);
+        double fk = 0.0, fg = 0.0;
+        const double f0 = f(c);
+        fk = wk[7] * f0; fg = wg[3] * f0;
+        for (int j = 0; j < 7; ++j) {
+            const double y = f(c - h * xk[j]) + f(c + h * xk[j]);
+            fk += wk[j] * y;
+            if (j % 2 == 1) fg += wg[j / 2] * y;
+        }
+        return Seg{lo, hi, fk * h, std::fabs((fk - fg) * h)};
+    };
+    std::vector<Seg> segs{eval(a, b)};
+    for (int it = 0; it < 4000; ++it) {
+        double I = 0.0, E = 0.0;
+        size_t worst = 0;
+        for (size_t k = 0; k < segs.size(); ++k) {
+            I += segs[k].I; E += segs[k].E;
+            if (segs[k].E > segs[worst].E) worst = k;
+        }
+        if (E <= rtol * std::fabs(I) || E < 1e-300) return I;
+        const Seg w = segs[worst];
+        const double mid = 0.5 * (w.a + w.b);
+        segs[worst] = eval(w.a, mid);
+        segs.push_back(eval(mid, w.b));
+    }
+    double I = 0.0;
+    for (const Seg& g : segs) I += g.I;
+    return I;
+}
+
+// evaluate_broadening / evaluate_force_broadening (cme.jl:167-182)
+static double cme_broadening(double h, double mu, double beta, double Gam, double W, int force) {
+    auto A = [&](double e) { return 1.0 / M_PI * (Gam / 2.0) / ((e - h) * (e - h) + (Gam / 2.0) * (Gam / 2.0)); };
+    auto k = [&](double e) { return (force ? (e - h) : 1.0) * A(e) * fermi_fn(e, mu, beta); };
+    // the Lorentzian peak and the Fermi edge are panel boundaries, so the adaptive rule sees both
+    std::vector<double> pts{mu - W / 2, mu + W / 2};
+    for (double p : {h, mu}) if (p > mu - W / 2 && p < mu + W / 2) pts.push_back(p);
+    std::sort(pts.begin(), pts.end());
+    double I = 0.0;
+    for (size_t j = 0; j + 1 < pts.size(); ++j) I += quad_gk(k, pts[j], pts[j + 1], 1e-13);
+    return I;
+}
+
+static void cme_accel(const OCmeDesc& D, const OModelDesc& m, double x, int s, double* a, double* Vout) {
+    double V[4], dV[4], V0, dV0;
+    model_eval(m, &x, V, dV, &V0, &dV0);
+    double f = -dV[s * 3];
+    if (D.method == 1) {
+        const double Gam = 2.0 * M_PI * V[1] * V[1];
+        const double beta = 1.0 / D.kT;
+        const double h = V[3] - V[0];
+        const double n = cme_broadening(h, D.mu, beta, Gam, D.bandwidth, 0);
+        const double n2 = cme_broadening(h, D.mu, beta, Gam, D.bandwidth, 1);
+        const double fe = fermi_fn(h, D.mu, beta);
+        const double dh = dV[3] - dV[0];
+        const double dGam = 4.0 * M_PI * V[1] * dV[1];
+        f += fe * dh - dh * n - dGam / Gam * n2;
+    }
+    *a = f / D.mass;
+    if (Vout) { Vout[0] = V[0]; Vout[1] = V[1]; Vout[2] = V[3]; }
+}
+
+// One trajectory; x, v in/out; saves the position, kinetic energy, potential V_ss and state per slot.
+static void cme_one(const OCmeDesc& D, const OModelDesc& m, uint64_t gid, double& x, double& v, int& s, int64_t& steps,
+                    int32_t& nh, double* sx, double* sek, double* sep, int32_t* sst) {
+    using namespace tsit5;
+    auto save = [&](int slot) {
+        if (slot >= D.nsave) return;
+        double a, Vv[3];
+        cme_accel(D, m, x, s, &a, Vv);
+        sx[slot] = x; sek[slot] = 0.5 * D.mass * v * v; sep[slot] = s == 0 ? Vv[0] : Vv[2]; sst[slot] = s;
+    };
+    save(0);
+    nh = 0;
+    for (int64_t step = 0; step < D.nsteps; ++step) {
+        const double h = D.dt;
+        double k1x, k1v, k2x, k2v, k3x, k3v, k4x, k4v, k5x, k5v, k6x, k6v, a;
+        k1x = v; cme_accel(D, m, x, s, &a, nullptr); k1v = a;
+        double tx = x + h * (a21 * k1x), tv = v + h * (a21 * k1v);
+        k2x = tv; cme_accel(D, m, tx, s, &a, nullptr); k2v = a;
+        tx = x + h * (a31 * k1x + a32 * k2x); tv = v + h * (a31 * k1v + a32 * k2v);
+        k3x = tv; cme_accel(D, m, tx, s, &a, nullptr); k3v = a;
+        tx = x + h * (a41 * k1x + a42 * k2x + a43 * k3x); tv = v + h * (a41 * k1v + a42 * k2v + a43 * k3v);
+        k4x = tv; cme_accel(D, m, tx, s, &a, nullptr); k4v = a;
+        tx = x + h * (a51 * k1x + a52 * k2x + a53 * k3x + a54 * k4x);
+        tv = v + h * (a51 * k1v + a52 * k2v + a53 * k3v + a54 * k4v);
+        k5x = tv; cme_accel(D, m, tx, s, &a, nullptr); k5v = a;
+        tx = x + h * (a61 * k1x + a62 * k2x + a63 * k3x + a64 * k4x + a65 * k5x);
+        tv = v + h * (a61 * k1v + a62 * k2v + a63 * k3v + a64 * k4v + a65 * k5v);
+        k6x = tv; cme_accel(D, m, tx, s, &a, nullptr); k6v = a;
+        x = x + h * (a71 * k1x + a72 * k2x + a73 * k3x + a74 * k4x + a75 * k5x + a76 * k6x);
+        v = v + h * (a71 * k1v + a72 * k2v + a73 * k3v + a74 * k4v + a75 * k5v + a76 * k6v);
+        steps = step + 1;
+        // the save precedes the hop callback (the rule of run_one)
+        if (D.save_every > 0 && steps % D.save_every == 0) save((int)(steps / D.save_every));
+        // HoppingCallback (surface_hopping.jl:2-16) with the CME probability (cme.jl:26-53)
+        double Vv[3], aa;
+        cme_accel(D, m, x, s, &aa, Vv);
+        const double Gam = 2.0 * M_PI * Vv[1] * Vv[1];
+        const double fe = fermi_fn(Vv[2] - Vv[0], D.mu, 1.0 / D.kT);
+        const double P = (s == 0) ? Gam * fe * D.dt : Gam * (1.0 - fe) * D.dt;
+        const double u = uniform01(gid, (uint32_t)step, STREAM_HOP, D.key);
+        if (u < P) { s = 1 - s; ++nh; }
+        if (D.cell.mode != 0 && ocell_outside(D.cell, &x, 1)) {   // cell callback after the hop callback
+            if (D.cell.mode == 2) break;
+            ocell_wrap(D.cell, &x, 1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Classical dynamics, Simulation{Classical} (classical.jl:40-86) on a single-state model of one
+// coordinate: OrdinaryDiffEq's VelocityVerlet (classical.jl:86) on the DynamicalODEProblem(acceleration!,
+// velocity!, v, r) (classical.jl:80-84).  VelocityVerlet's perform_step! (OrdinaryDiffEq, a dependency
+// absent from the reference tree) evaluates a = f1(v, x) at the step start, then x' = x + dt v +
+// dt^2 (a / 2), a' = f1(v, x'), v' = v + dt (a / 2 + a' / 2); the start value equals the previous end
+// value unless a callback moved x.  Saves precede the callbacks; the cell callback and the termination
+// predicate follow the step.
+// ---------------------------------------------------------------------------
+struct OClsDesc {
+    int32_t save_every, nsave, term_kind, pad;
+    double term_x, dt, mass;
+    int64_t nsteps;
+    OCell cell;
+};
+
+OR_API int32_t oracle_classical_run(const OClsDesc* D, const OModelDesc* m, int64_t ntraj, double* x, double* v,
+                                    int64_t* steps, double* sx, double* sv, double* sek, double* sep, int32_t nthreads) {
+    const int ns = D->nsave;
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        double xi = x[i], vi = v[i];
+        auto accel = [&](double r, double& V) {
+            double Vm[1], dV[1], V0, dV0[1];
+            model_eval(*m, &r, Vm, dV, &V0, dV0);
+            V = Vm[0] + V0;
+            return -(dV[0] + dV0[0]) / D->mass;
+        };
+        double V;
+        double a = accel(xi, V);
+        auto save = [&](int slot) {
+            if (slot >= ns) return;
+            sx[(size_t)i * ns + slot] = xi; sv[(size_t)i * ns + slot] = vi;
+            sek[(size_t)i * ns + slot] = 0.5 * D->mass * vi * vi; sep[(size_t)i * ns + slot] = V;
+        };
+        save(0);
+        int64_t step = 0;
+        while (step < D->nsteps) {
+            const double xn = xi + D->dt * vi + (D->dt * D->dt) * (0.5 * a);
+            const double an = accel(xn, V);
+            vi = vi + D->dt * (0.5 * a + 0.5 * an);
+            xi = xn;
+            a = an;
+            if (D->save_every > 0 && (step + 1) % D->save_every == 0) save((int)((step + 1) / D->save_every));
+            bool end = false;
+            if (D->cell.mode != 0 && ocell_outside(D->cell, &xi, 1)) {
+                if (D->cell.mode == 2) end = true;
+                else { ocell_wrap(D->cell, &xi, 1); a = accel(xi, V); }
+            }
+            ++step;
+            if (end) break;
+            if (D->term_kind == 1 && std::fabs(xi) > D->term_x) break;
+            if (D->term_kind == 2 && xi > D->term_x) break;
+            if (D->term_kind == 3 && xi < D->term_x) break;
+        }
+        x[i] = xi; v[i] = vi; steps[i] = step;
+    });
+    return 0;
+}
+
+OR_API int32_t oracle_cme_run(const OCmeDesc* D, const OModelDesc* m, int64_t ntraj, double* x, double* v,
+                              int32_t* state, int32_t* nhops, double* sx, double* sek, double* sep, int32_t* sst,
+                              int32_t nthreads) {
+    parallel_for(ntraj, nthreads, [&](int64_t i) {
+        int s = state[i];
+        int64_t steps = 0;
+        cme_one(*D, *m, (uint64_t)i, x[i], v[i], s, steps, nhops[i], sx + (size_t)i * D->nsave, sek + (size_t)i * D->nsave,
+                sep + (size_t)i * D->nsave, sst + (size_t)i * D->nsave);
+        state[i] = s;
+    });
+    return 0;
+}
+
+OR_API double oracle_cme_broadening(double h, double mu, double beta, double Gam, double W, int32_t force) {
+    return cme_broadening(h, mu, beta, Gam, W, force);
 }
 
 // ---------------------------------------------------------------------------
@@ -1379,6 +1709,8 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
         }
         psi = tmp;
         t += R.dt;
+        // the save precedes the hop and decoherence callbacks (the rule of run_one)
+        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save_pop((int)((step + 1) / R.save_every));
         if (R.method == 0) {
             const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
             // hopping probabilities (iesh.jl:231-283), selection (iesh.jl:318-335)
@@ -1430,7 +1762,6 @@ static void ie_run_one(const OIEDesc& R, const ONADesc& md, int64_t i, uint64_t 
                 }
             }
         }
-        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save_pop((int)((step + 1) / R.save_every));
     }
     r_out[i] = x; v_out[i] = v;
     for (int k = 0; k < N * Ne; ++k) { psi_re[(size_t)i * N * Ne + k] = psi[k].real(); psi_im[(size_t)i * N * Ne + k] = psi[k].imag(); }
@@ -1563,6 +1894,8 @@ static void ie_rp_run_one(const OIEDesc& R, const ONADesc& md, int P, double tem
         }
         psi = tmp;
         eig(mean(r), E, U, G, d, dU0c);                                                       // centroid at the step end
+        // the save precedes the hop callback (the rule of run_one)
+        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save_pop((int)((step + 1) / R.save_every));
         if (R.method == 0) {
             const double rnd = uniform01(gid, (uint32_t)step, STREAM_HOP, R.key);
             const double vc = mean(v);
@@ -1602,7 +1935,6 @@ static void ie_rp_run_one(const OIEDesc& R, const ONADesc& md, int P, double tem
                 }
             }
         }
-        if (R.save_every > 0 && (step + 1) % R.save_every == 0) save_pop((int)((step + 1) / R.save_every));
     }
     for (int b = 0; b < P; ++b) { r_out[(size_t)i * P + b] = r[b]; v_out[(size_t)i * P + b] = v[b]; }
     for (int kk = 0; kk < N * Ne; ++kk) { psi_re[(size_t)i * N * Ne + kk] = psi[kk].real(); psi_im[(size_t)i * N * Ne + kk] = psi[kk].imag(); }

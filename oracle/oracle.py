@@ -14,20 +14,56 @@ LIB_PATH = os.path.join(_HERE, "build", "libnqcd_oracle.so")
 
 MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC = range(6)
 MODEL_DIATOM_SURFACE = 7
+MODEL_MORSE_N = 9
+MODEL_ANDERSON_HOLSTEIN_1 = 10
 METHOD_FSSH, METHOD_EHRENFEST = 0, 1
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = 0, 1, 2
 
 
 class OModelDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("nstates", C.c_int32), ("ndof", C.c_int32), ("nbath", C.c_int32),
-                ("p", C.c_double * 16), ("omega", C.POINTER(C.c_double)), ("cbath", C.POINTER(C.c_double))]
+                ("p", C.c_double * 16), ("omega", C.POINTER(C.c_double)), ("cbath", C.POINTER(C.c_double)),
+                ("ext", C.POINTER(C.c_double))]
+
+
+class OCell(C.Structure):
+    """Simulation cell of the cell callbacks: mode 0 none, 1 CellBoundaryCallback, 2 TerminateCellCallback."""
+    _fields_ = [("mode", C.c_int32), ("nd", C.c_int32), ("per", C.c_int32 * 3), ("pad", C.c_int32),
+                ("a", C.c_double * 9), ("inv", C.c_double * 9)]
+
+
+def make_cell(mode, vectors, periodic=None):
+    """OCell of PeriodicCell(vectors; periodicity): lattice vectors as the columns of `vectors`."""
+    c = OCell()
+    if mode == 0:
+        return c
+    a = np.atleast_2d(np.asarray(vectors, dtype=np.float64))
+    nd = a.shape[0]
+    inv = np.linalg.inv(a)
+    c.mode, c.nd = mode, nd
+    for j in range(nd):
+        c.per[j] = 1 if periodic is None else int(bool(periodic[j]))
+        for k in range(nd):
+            c.a[j + 3 * k], c.inv[j + 3 * k] = a[j, k], inv[j, k]
+    return c
+
+
+def cell_apply(cell, x):
+    """The cell callback on one configuration (atoms of nd coordinates, coordinate fastest): returns
+    (outside, x after the callback)."""
+    x = np.array(x, dtype=np.float64).ravel()
+    f = lib().oracle_cell_apply
+    f.restype = C.c_int32
+    out = f(C.byref(cell), C.c_int32(len(x)), _p(x))
+    return bool(out), x
 
 
 class ORunDesc(C.Structure):
     _fields_ = [("method", C.c_int32), ("nstates", C.c_int32), ("ndof", C.c_int32), ("rescaling", C.c_int32),
                 ("dt", C.c_double), ("t0", C.c_double), ("nsteps", C.c_int64), ("term_x", C.c_double),
                 ("save_every", C.c_int32), ("nsave", C.c_int32), ("key", C.c_uint64), ("traj_offset", C.c_int64),
-                ("nsub", C.c_int32), ("term_dof", C.c_int32), ("term_kind", C.c_int32), ("pad", C.c_int32)]
+                ("nsub", C.c_int32), ("term_dof", C.c_int32), ("term_kind", C.c_int32), ("pad", C.c_int32),
+                ("cell", OCell)]
 
 
 def build():
@@ -51,15 +87,23 @@ def _p(a, t=C.c_double):
 
 
 def make_model(kind, nstates, ndof, params, omega=None, cbath=None):
+    """Model descriptor; MODEL_MORSE_N takes its whole parameter table as params (any length)."""
     m = OModelDesc()
     m.kind, m.nstates, m.ndof = kind, nstates, ndof
+    keep = []
+    if kind == MODEL_MORSE_N:
+        ext = np.ascontiguousarray(params, dtype=np.float64)
+        if ext.size != 4 * nstates + 3 * nstates * (nstates - 1) // 2:
+            raise ValueError("MORSE_N table: [D, beta, r0, c] per state, [A, a, r] per pair")
+        keep.append(ext)
+        m.ext = _p(ext)
+        params = []
     for i, x in enumerate(params):
         m.p[i] = float(x)
-    keep = []
     if omega is not None:
         om = np.ascontiguousarray(omega, dtype=np.float64)
         cb = np.ascontiguousarray(cbath, dtype=np.float64)
-        keep = [om, cb]
+        keep += [om, cb]
         m.nbath = len(om)
         m.omega, m.cbath = _p(om), _p(cb)
     m._keep = keep  # keep the arrays alive with the descriptor
@@ -180,9 +224,9 @@ def diabatic_to_adiabatic(m, r0, diabatic_state):
 
 def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None, dt=1.0, nsteps=100, term_x=0.0,
                  save_every=0, nsave=1, key=0, traj_offset=0, rescaling=RESCALE_STANDARD, nthreads=None, t0=0.0, nsub=5,
-                 traj_ids=None, term_dof=0, term_kind=0, energies=False):
+                 traj_ids=None, term_dof=0, term_kind=0, energies=False, cell=None):
     """Run an ensemble on the CPU; returns a dict of per-trajectory results (with energies=True also
-    the kinetic and potential energy of every save slot, ekin / epot (ntraj, nsave))."""
+    the kinetic and potential energy of every save slot, ekin / epot (ntraj, nsave)); cell: an OCell."""
     if not isinstance(models, (list, tuple)):
         models = [models]
     ntraj, D = r0.shape
@@ -193,6 +237,8 @@ def run_ensemble(method, models, mass, r0, v0, sig0, state0=None, model_id=None,
     R.save_every, R.nsave, R.key, R.traj_offset, R.nsub = save_every, nsave, key, traj_offset, nsub
     R.term_dof = term_dof
     R.term_kind = term_kind
+    if cell is not None:
+        R.cell = cell
     marr = (OModelDesc * len(models))(*models)
     mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, dtype=np.float64), (D,)))
     sr0 = np.ascontiguousarray(np.real(sig0), dtype=np.float64).reshape(ntraj, n, n)
@@ -300,6 +346,76 @@ def pimc(model, x0, mass, temperature, delta, passes, segment=1.0, width=0, key=
     if rc != 0:
         raise ValueError("pimc: segment_length must be in 1..nbeads")
     return x, v, acc
+
+
+class OCmeDesc(C.Structure):
+    _fields_ = [("method", C.c_int32), ("save_every", C.c_int32), ("nsave", C.c_int32), ("pad", C.c_int32),
+                ("dt", C.c_double), ("t0", C.c_double), ("mass", C.c_double), ("kT", C.c_double), ("mu", C.c_double),
+                ("bandwidth", C.c_double), ("nsteps", C.c_int64), ("key", C.c_uint64), ("cell", OCell)]
+
+
+class OClsDesc(C.Structure):
+    _fields_ = [("save_every", C.c_int32), ("nsave", C.c_int32), ("term_kind", C.c_int32), ("pad", C.c_int32),
+                ("term_x", C.c_double), ("dt", C.c_double), ("mass", C.c_double), ("nsteps", C.c_int64),
+                ("cell", OCell)]
+
+
+def classical_run(model, mass, r0, v0, dt=1.0, nsteps=100, save_every=0, term_kind=0, term_x=0.0, cell=None,
+                  nthreads=None):
+    """Simulation{Classical} trajectories by VelocityVerlet on a single-state model of one coordinate:
+    final x, v, steps and per-save position, velocity, kinetic and potential energy."""
+    ntraj = len(r0)
+    D = OClsDesc()
+    nsave = nsteps // save_every + 1 if save_every else 1
+    D.save_every, D.nsave, D.term_kind, D.term_x, D.dt, D.mass, D.nsteps = (save_every, nsave, term_kind, term_x, dt,
+                                                                           mass, nsteps)
+    if cell is not None:
+        D.cell = cell
+    out = dict(x=np.ascontiguousarray(r0, dtype=np.float64).ravel().copy(),
+               v=np.ascontiguousarray(v0, dtype=np.float64).ravel().copy(), steps=np.zeros(ntraj, np.int64),
+               save_x=np.zeros((ntraj, nsave)), save_v=np.zeros((ntraj, nsave)), ekin=np.zeros((ntraj, nsave)),
+               epot=np.zeros((ntraj, nsave)))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_classical_run
+    f.restype = C.c_int32
+    f(C.byref(D), C.byref(model), C.c_int64(ntraj), _p(out["x"]), _p(out["v"]), _p(out["steps"], C.c_int64),
+      _p(out["save_x"]), _p(out["save_v"]), _p(out["ekin"]), _p(out["epot"]), C.c_int32(nthreads))
+    return out
+
+
+def cme_run(method, model, mass, kT, r0, v0, state0, dt=1.0, nsteps=100, mu=0.0, bandwidth=1.0, save_every=0, key=0,
+            nthreads=None, cell=None):
+    """CME (method 0) / BCME (1) trajectories (cme.jl); r0, v0 (ntraj,), state0 0-based diabatic states.  Returns
+    final x, v, state, hops and per-save position, kinetic energy, potential V_ss and state."""
+    ntraj = len(r0)
+    D = OCmeDesc()
+    nsave = nsteps // save_every + 1 if save_every else 1
+    D.method, D.save_every, D.nsave, D.dt, D.t0, D.mass, D.kT, D.mu, D.bandwidth = (method, save_every, nsave, dt, 0.0,
+                                                                                     mass, kT, mu, bandwidth)
+    D.nsteps, D.key = nsteps, key
+    if cell is not None:
+        D.cell = cell
+    out = dict(x=np.ascontiguousarray(r0, dtype=np.float64).ravel().copy(),
+               v=np.ascontiguousarray(v0, dtype=np.float64).ravel().copy(),
+               state=np.ascontiguousarray(state0, dtype=np.int32).copy(), nhops=np.zeros(ntraj, np.int32),
+               save_x=np.zeros((ntraj, nsave)), ekin=np.zeros((ntraj, nsave)), epot=np.zeros((ntraj, nsave)),
+               save_state=np.zeros((ntraj, nsave), np.int32))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_cme_run
+    f.restype = C.c_int32
+    f(C.byref(D), C.byref(model), C.c_int64(ntraj), _p(out["x"]), _p(out["v"]), _p(out["state"], C.c_int32),
+      _p(out["nhops"], C.c_int32), _p(out["save_x"]), _p(out["ekin"]), _p(out["epot"]), _p(out["save_state"], C.c_int32),
+      C.c_int32(nthreads))
+    return out
+
+
+def cme_broadening(h, mu, beta, Gam, W, force=False):
+    f = lib().oracle_cme_broadening
+    f.restype = C.c_double
+    f.argtypes = [C.c_double] * 5 + [C.c_int32]
+    return f(h, mu, beta, Gam, W, int(force))
 
 
 # --- IESH / EhrenfestNA on the Newns-Anderson model ------------------------------------------

@@ -43,4 +43,4 @@ def test_autotune_entry_point_rejects_null_handle():
     L = _lib.lib()
     r = _lib.TuneResult()
     assert ctypes.sizeof(_lib.TuneResult) == 32
-    assert L.nqcd_mi355x_autotune(None, 0.0, 1.0, ctypes.c_uint64(1), ctypes.byref(r)) == _lib.ERR_ARG
+    assert L.nqcd_mi355x_autotune(None, 0.0, 1.0, None, ctypes.c_uint64(1), ctypes.byref(r)) == _lib.ERR_ARG

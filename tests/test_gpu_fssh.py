@@ -317,3 +317,39 @@ def test_trajectories_starting_past_the_boundary_take_one_step(oracle, key, sche
     assert np.array_equal(g["save_state"], c["save_state"])
     assert np.allclose(g["r"], c["r"], rtol=1e-9, atol=1e-12)
     assert np.allclose(g["v"], c["v"], rtol=1e-9, atol=1e-14)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_set_aside_policies_leave_results_unchanged(oracle, key, monkeypatch, precision):
+    """Once the queue has drained, waves with few running lanes set them aside and the next launch takes them
+    up packed (NQCD_MI355X_SUSPEND = "rounds,threshold").  The state is carried bit for bit and the random
+    numbers are keyed by trajectory id and step, so a single launch, the default policy, and every drained
+    wave setting its lanes aside at every step give identical trajectories, saves and lengths; and the FP64
+    path matches the oracle under the default policy."""
+    L, E = _ens()
+    ntraj = 20000
+    r0, v0, sig0, st0 = _window_inputs(oracle, key, ntraj, 0)
+    runs = {}
+    for pol in ("1,0", None, "3,63", "8,64"):
+        if pol is None:
+            monkeypatch.delenv("NQCD_MI355X_SUSPEND", raising=False)
+        else:
+            monkeypatch.setenv("NQCD_MI355X_SUSPEND", pol)
+        print(f"set-aside policy {pol}", flush=True)
+        runs[pol] = _run_gpu(L, E, 0, 0, r0, v0, sig0, st0, 1500, key, precision=precision, save_every=50,
+                             term=(0, 10.0))
+    monkeypatch.delenv("NQCD_MI355X_SUSPEND", raising=False)
+    E.close()
+    ref = runs["1,0"]
+    assert ref["stats"]["launches"] == 2 and runs["8,64"]["stats"]["launches"] == 16
+    assert len(np.unique(ref["steps"])) > 50 and ref["nhops"].sum() > 0   # uneven lengths: a real tail
+    for pol, g in runs.items():
+        for k in ("r", "v", "sig", "state", "steps", "nhops", "save_state", "pop_ad", "pop_di"):
+            assert np.array_equal(g[k], ref[k]), (pol, k)
+    if precision == 0:
+        sel = np.arange(0, ntraj, 40)
+        m = oracle.make_model(0, 2, 1, TULLY[0])
+        c = oracle.run_ensemble(0, m, MASS, r0[sel], v0[sel], sig0[sel], state0=st0[sel], dt=1.0, nsteps=1500, key=key,
+                                term_x=10.0, traj_ids=sel)
+        assert np.array_equal(runs[None]["steps"][sel], c["steps"]) and np.array_equal(runs[None]["state"][sel], c["state"])
+        assert np.allclose(runs[None]["r"][sel], c["r"], rtol=1e-9, atol=1e-12)

@@ -66,9 +66,11 @@ def test_iesh_fp64_parity(oracle, key, M, Ne, ntraj, nsteps):
     if M == 40:
         assert c["nhops"].sum() >= 3
     _compare(g, c)
-    # one-hot adiabatic populations at the last save slot follow the final occupations
+    # one-hot adiabatic populations at the last save slot: a save holds the state before the step's hop
+    # callback, so the slot of the last step shows the occupations after nsteps - 1 steps
+    prev = _cpu(oracle, 0, M, Ne, r0, v0, occ0, 2.0, nsteps - 1, key)
     last = np.zeros((ntraj, M + 1))
-    np.put_along_axis(last, c["occ"], 1.0, axis=1)
+    np.put_along_axis(last, prev["occ"], 1.0, axis=1)
     assert np.array_equal(g["ad"][:, -1, :], last)
     assert np.allclose(g["di_mean"], c["pop_di"].mean(axis=0), rtol=1e-9, atol=1e-12)
 

@@ -130,3 +130,86 @@ def test_rpmd_fp32_centroid_statistics(oracle, key):
         out.append(E.reduce(L.OUT_CENTROID_POSITION, L.RED_MEAN, E.nsave))
         E.close()
     assert np.abs(out[0] - out[1]).max() < 1e-3
+
+
+def _hartley(P):
+    j = np.arange(P)
+    a = 2 * np.pi * ((j[:, None] * j[None, :]) % P) / P
+    return (np.cos(a) + np.sin(a)) / np.sqrt(P)
+
+
+def test_harmonic_thermal_sampler_moments(key):
+    """The harmonic-approximation sampler (include/nqcd_mi355x.h, SAMPLER_RING_POLYMER_THERMAL): normal mode k
+    of the beads has mean sqrt(P) x0 delta_k0 and variance 1/(beta_P m (w_k^2 + w_well^2)), bead velocities
+    variance 1/(beta_P m), beta_P = beta / P; 5 sigma of the moment estimates."""
+    from nqcd_mi355x import _lib as L
+    P, ntraj, x0, ww = 8, 50000, -3.87, np.sqrt(0.1 / MASS)
+    r, v = _thermal_beads(L, P, ntraj, key, x0=x0)
+    y = r @ _hartley(P).T
+    bP = BETA / P
+    wk = 2 * (P / BETA) * np.sin(np.arange(P) * np.pi / P)
+    var = 1 / (bP * MASS * (wk ** 2 + ww ** 2))
+    mean = np.r_[np.sqrt(P) * x0, np.zeros(P - 1)]
+    assert np.all(np.abs(y.mean(axis=0) - mean) < 5 * np.sqrt(var / ntraj))
+    assert np.all(np.abs(y.var(axis=0) / var - 1) < 5 * np.sqrt(2.0 / ntraj))
+    assert abs(v.var() * bP * MASS - 1) < 5 * np.sqrt(2.0 / v.size) and abs(v.mean()) < 5 * np.sqrt(1 / (bP * MASS) / v.size)
+
+
+def _pimc_device(L, P, ntraj, key, model, params, mass, T, nstates=1):
+    E = L.Ensemble(0)
+    E.set_problem(L.METHOD_RPMD, 1, 1, 1, [mass], 1.0, algorithm=L.ALG_BCB, nbeads=P, temperature=T)
+    E.set_model(0, model[0], model[1])
+    E.sample_initial(L.SAMPLER_RING_POLYMER_PIMC, ntraj, params, key)
+    E.run(0.0, 0.0, key)
+    r, v = E.download(L.FIELD_R), E.download(L.FIELD_V)
+    E.close()
+    return r, v
+
+
+@pytest.mark.parametrize("P,width,segment", [(1, 0, 1.0), (4, 0, 1.0), (8, 1, 0.5), (64, 0, 1.0)])
+def test_pimc_parity(oracle, key, P, width, segment):
+    """Device path-integral Monte Carlo chains (MetropolisHastings.jl:170-258, the reference's second test of
+    the last proposal included) against the oracle's chains on the same Philox draws: final beads to 1e-9
+    and the velocities, on the double well of the ring-polymer workload."""
+    from nqcd_mi355x import _lib as L
+    ntraj, passes, x0, delta = 300, 150, -3.87, 2.0
+    print(f"PIMC parity P={P} width={width} segment={segment}", flush=True)
+    r, v = _pimc_device(L, P, ntraj, key, (L.MODEL_DW_GAUSSIAN, DW1), [x0, delta, passes, segment, width], MASS,
+                        1 / BETA)
+    m = oracle.make_model(oracle.MODEL_DW_GAUSSIAN, 1, 1, DW1)
+    xc, vc, acc = oracle.pimc(m, np.full((ntraj, P), x0), MASS, 1 / BETA, delta, passes, segment=segment, width=width,
+                              key=key, reassess=1)
+    assert acc.min() > 0
+    assert np.allclose(r, xc, rtol=1e-9, atol=1e-9)
+    assert np.allclose(v, vc, rtol=1e-12, atol=1e-15)
+
+
+def test_pimc_path_integral_law_on_device(key):
+    """Without the second test and with the free ring polymer's mode widths (params width = 1, reassess = 0)
+    the chain samples the discretised path-integral distribution: on a harmonic well every normal mode has
+    variance 1/(beta_P m (w^2 + w_k^2)) and the centroid mean 0 (5 sigma), P = 8, 20000 chains."""
+    from nqcd_mi355x import _lib as L
+    P, ntraj, mass, w, T = 8, 20000, 1.3, 0.9, 1.0 / 8
+    r, v = _pimc_device(L, P, ntraj, key, (L.MODEL_HARMONIC, [mass, w, 0.0]), [0.0, 1.5, 300, 1.0, 1, 0], mass, T)
+    y = r @ _hartley(P).T
+    wk = 2 * P * T * np.sin(np.arange(P) * np.pi / P)
+    var = 1 / (mass * (w ** 2 + wk ** 2) / (P * T))
+    z = (y.var(axis=0) - var) / (var * np.sqrt(2.0 / ntraj))
+    assert np.all(np.abs(z) < 5), z
+    assert np.all(np.abs(y.mean(axis=0)) < 5 * np.sqrt(var / ntraj))
+    assert abs(v.var() / (P * T / mass) - 1) < 5 * np.sqrt(2.0 / v.size)
+
+
+def test_rpsh_fp32_beads_follow_fp64(key):
+    """FP32 beads with the FP64 centroid electronics on their own lanes: the final active states of 4096
+    thermal ring polymers (P = 8) agree with the FP64 kernel's on at least 97% of them."""
+    from nqcd_mi355x import _lib as L
+    P, ntraj, nsteps = 8, 4096, 1500
+    states = []
+    for prec in (0, 1):
+        E = _ens(L, L.METHOD_RPSH, P, 2, precision=prec)
+        E.sample_initial(L.SAMPLER_RING_POLYMER_PIMC, ntraj, [-3.87, 2.0, 100, 1.0, 1, 0], key, initial_state=0)
+        E.run(0.0, float(nsteps), key)
+        states.append(E.download(L.FIELD_STATE))
+        E.close()
+    assert np.mean(states[0] == states[1]) > 0.97

@@ -139,3 +139,47 @@ def test_single_trajectory_returns_its_dictionary(oracle, key):
                             state0=np.zeros(1, np.int32), dt=1.0, nsteps=2000, key=key, term_x=10.0)
     assert np.allclose(out["OutputFinal"]["r"].ravel(), c["r"][0], rtol=1e-9)
     assert out["OutputFinal"]["state"] == c["state"][0] + 1
+
+
+@pytest.mark.parametrize("method_cls", [N.FSSH, N.Ehrenfest])
+def test_diabatic_scattering_at_the_final_state(oracle, key, method_cls):
+    """OutputStateResolvedScattering1D(sim, :diabatic) (DynamicsOutputs.jl:313-338) takes the diabatic
+    populations of each trajectory's final state, also for trajectories that terminated between save slots:
+    the device reduction and the per-trajectory path against the oracle's diabatic population at the step
+    each trajectory ended on (saved at every step)."""
+    sim = N.Simulation[method_cls](N.Atoms(MASS), N.TullyModelOne(*TULLY[0]))
+    out = N.OutputStateResolvedScattering1D(sim, "diabatic")
+    _, res_sum, _ = _case(oracle, key, method_cls, 0, (out,), N.SumReduction())
+    _, res_traj, _ = _case(oracle, key, method_cls, 0, (out,), N.SortByTrajectoryReduction())
+    rng = np.random.default_rng(3)
+    p0 = rng.uniform(8.0, 30.0, size=NTRAJ)
+    r0 = np.full((NTRAJ, 1), -8.0); v0 = (p0 / MASS)[:, None]
+    sig0 = np.zeros((NTRAJ, 2, 2), complex); sig0[:, 0, 0] = 1.0
+    m = oracle.make_model(0, 2, 1, TULLY[0])
+    c = oracle.run_ensemble(0 if method_cls is N.FSSH else 1, m, MASS, r0, v0, sig0, state0=np.zeros(NTRAJ, np.int32),
+                            dt=1.0, nsteps=NSTEPS, key=key, save_every=1, nsave=NSTEPS + 1, term_x=10.0)
+    fin = c["pop_di"][np.arange(NTRAJ), c["steps"]]
+    right = c["r"][:, 0] > 0
+    assert np.allclose(res_sum["OutputStateResolvedScattering1D"]["transmission"], fin[right].sum(axis=0), rtol=1e-9)
+    assert np.allclose(res_sum["OutputStateResolvedScattering1D"]["reflection"], fin[~right].sum(axis=0), rtol=1e-9)
+    for i, d in enumerate(res_traj):
+        sc = d["OutputStateResolvedScattering1D"]
+        assert np.allclose(sc["transmission" if right[i] else "reflection"], fin[i], rtol=1e-9, atol=1e-12)
+    assert np.isclose(sum(np.sum(d["OutputStateResolvedScattering1D"]["transmission"]) +
+                          np.sum(d["OutputStateResolvedScattering1D"]["reflection"]) for d in res_traj), NTRAJ)
+
+
+def test_saveat_every_step_and_sequences(oracle, key):
+    """saveat forms (run_dynamics.jl:91-97 forwards them to SciML's solve): none saves every step, a
+    uniform sequence from t0 keeps its own times and slots, and the per-save outputs equal the oracle's."""
+    sim = N.Simulation[N.FSSH](N.Atoms(MASS), N.TullyModelOne(*TULLY[0]))
+    dist = N.DynamicalDistribution(20.0 / MASS, -8.0, (1, 1)) * N.PureState(1, N.Adiabatic())
+    nsteps = 400
+    full = N.run_dynamics(sim, (0.0, float(nsteps)), dist, trajectories=3, dt=1.0, output=N.OutputAdiabaticPopulation,
+                          reduction=N.MeanReduction(), ensemble_algorithm=N.EnsembleMI355X(seed=key))
+    assert len(full["Time"]) == nsteps + 1 and len(full["OutputAdiabaticPopulation"]) == nsteps + 1
+    seq = N.run_dynamics(sim, (0.0, float(nsteps)), dist, trajectories=3, dt=1.0, saveat=np.arange(0.0, 151.0, 10.0),
+                         output=N.OutputAdiabaticPopulation, reduction=N.MeanReduction(),
+                         ensemble_algorithm=N.EnsembleMI355X(seed=key))
+    assert np.allclose(seq["Time"], np.arange(0.0, 151.0, 10.0))
+    assert np.allclose(np.array(seq["OutputAdiabaticPopulation"]), np.array(full["OutputAdiabaticPopulation"])[::10][:16])

@@ -124,3 +124,20 @@ def test_terminating_callback_predicates():
     assert (L.TERM_ABS_GT, L.TERM_GT, L.TERM_LT) == (1, 2, 3)
     with pytest.raises(ValueError, match="predicate"):
         N.TerminatingCallback(1, 10.0, predicate="ge")
+
+
+def test_save_schedule_forms():
+    """saveat as SciML takes it: None -> every step; a number -> t0:saveat:t1; an evenly spaced sequence from
+    t0 -> its own times; anything else raises."""
+    from nqcd_mi355x.ensembles import save_schedule
+    k, n, t = save_schedule(None, 0.0, 10.0, 0.5)
+    assert (k, n) == (1, 21) and np.allclose(t, np.arange(21) * 0.5)
+    k, n, t = save_schedule(2.0, 0.0, 10.0, 0.5)
+    assert (k, n) == (4, 6) and np.allclose(t, [0, 2, 4, 6, 8, 10])
+    k, n, t = save_schedule(np.arange(0.0, 6.1, 1.5), 0.0, 10.0, 0.5)
+    assert (k, n) == (3, 5) and np.allclose(t, [0, 1.5, 3, 4.5, 6])
+    k, n, t = save_schedule([0.0], 0.0, 10.0, 0.5)
+    assert (k, n) == (0, 1)
+    for bad in (0.3, 3.0, [0.0, 1.0, 3.0], [1.0, 2.0], np.arange(0.0, 12.0, 1.0)):
+        with pytest.raises(ValueError):
+            save_schedule(bad, 0.0, 10.0, 0.5)

@@ -49,6 +49,27 @@ def test_model_derivative_finite_difference(oracle, name, kind, params):
         assert np.allclose(dV, (Vp - Vm) / (2 * h), atol=1e-8, rtol=1e-6), (name, x)
 
 
+@pytest.mark.parametrize("n", [3, 4, 8])
+def test_morse_n_model_finite_difference_and_closed_form(oracle, n):
+    """n-state Morse model (Coker and Xiao form): dV by central differences, and the closed form at the
+    Morse minimum x = r_i (V_ii = c_i) and far from every crossing (couplings vanish)."""
+    import nstate_models as NS
+    tab = NS.three_state() if n == 3 else NS.morse_table(n)
+    m = oracle.make_model(oracle.MODEL_MORSE_N, n, 1, tab)
+    for x in np.linspace(2.2, 9.0, 31):
+        h = 1e-6
+        Vp, Vm = oracle.model_eval(m, [x + h])[0], oracle.model_eval(m, [x - h])[0]
+        V, dV = oracle.model_eval(m, [x])[:2]
+        assert np.allclose(V, V.T) and np.allclose(dV[0], dV[0].T)
+        assert np.allclose(dV[0], (Vp - Vm) / (2 * h), atol=1e-8, rtol=1e-6), x
+    st = np.array(tab[:4 * n]).reshape(n, 4)
+    for i in range(n):
+        assert np.isclose(oracle.model_eval(m, [st[i, 2]])[0][i, i], st[i, 3], atol=1e-15)
+    far = oracle.model_eval(m, [200.0])[0]  # every Gaussian coupling underflows to zero out here
+    assert np.allclose(far - np.diag(np.diag(far)), 0.0, atol=1e-300)
+    assert np.allclose(np.diag(far), st[:, 0] * (1 - np.exp(-st[:, 1] * (200.0 - st[:, 2]))) ** 2 + st[:, 3], rtol=1e-13)
+
+
 # --- Spin-boson model (docs/src/NQCModels/systembathmodels.md:21-27) at eps != 0 ----------------
 # V(R) = [[eps + c.R, Delta], [Delta, -eps - c.R]] + (1/2) sum_j w_j^2 R_j^2 (mass 1), so
 # V(0) = [[eps, Delta], [Delta, -eps]] and the adiabatic energies are
