@@ -5,6 +5,7 @@
 // symmetric eigen-decomposition with eigenvector tracking, the nonadiabatic
 // coupling, the Tsit5 density-matrix propagation and the fewest-switches hop.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -192,11 +193,12 @@ __device__ __forceinline__ ModelRegs model_regs(const ModelSlot& s) {
     return m;
 }
 
-template <typename R>
-__device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R& v22, R& v12,
+// One model kind, selected at compile time (the per-kind bodies of model1d_eval, for callers that
+// evaluate many points under one dispatch).
+template <int K, typename R>
+__device__ __forceinline__ void model1d_kind(const ModelRegs& m, R x, R& v11, R& v22, R& v12,
                                              R& d11, R& d22, R& d12) {
-    switch (m.kind) {
-    case MODEL_TULLY1: {
+    if constexpr (K == MODEL_TULLY1) {
         const R A = (R)m.p[0], B = (R)m.p[1], C = (R)m.p[2], D = (R)m.p[3];
         const R e = fexp(-B * fabs_(x));
         const R sgn = (x > (R)0) ? (R)1 : ((x < (R)0) ? (R)-1 : (R)0);
@@ -206,9 +208,7 @@ __device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R&
         d22 = -d11;
         v12 = C * fexp(-D * x * x);
         d12 = (R)-2 * D * x * v12;
-        break;
-    }
-    case MODEL_TULLY2: {
+    } else if constexpr (K == MODEL_TULLY2) {
         const R A = (R)m.p[0], B = (R)m.p[1], C = (R)m.p[2], D = (R)m.p[3], E0 = (R)m.p[4];
         const R g = A * fexp(-B * x * x);
         v11 = (R)0; d11 = (R)0;
@@ -216,16 +216,12 @@ __device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R&
         d22 = (R)2 * B * x * g;
         v12 = C * fexp(-D * x * x);
         d12 = (R)-2 * D * x * v12;
-        break;
-    }
-    case MODEL_TULLY3: {
+    } else if constexpr (K == MODEL_TULLY3) {
         const R A = (R)m.p[0], B = (R)m.p[1], C = (R)m.p[2];
         v11 = A; v22 = -A; d11 = (R)0; d22 = (R)0;
         if (x < (R)0) { const R e = fexp(C * x); v12 = B * e; d12 = B * C * e; }
         else { const R e = fexp(-C * x); v12 = B * ((R)2 - e); d12 = B * C * e; }
-        break;
-    }
-    case MODEL_DW_GAUSSIAN: {
+    } else if constexpr (K == MODEL_DW_GAUSSIAN) {
         const R a = (R)m.p[0], b = (R)m.p[1], off = (R)m.p[2], C = (R)m.p[3], D = (R)m.p[4];
         const R x2 = x * x;
         v11 = a * x2 * x2 - b * x2;
@@ -233,19 +229,34 @@ __device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R&
         v22 = v11 + off; d22 = d11;
         v12 = C * fexp(-D * x2);
         d12 = (R)-2 * D * x * v12;
-        break;
-    }
-    case MODEL_HARMONIC: {  // single state: V = m w^2 (x - x0)^2 / 2
+    } else if constexpr (K == MODEL_HARMONIC) {  // single state: V = m w^2 (x - x0)^2 / 2
         const R mm = (R)m.p[0], w = (R)m.p[1], x0 = (R)m.p[2];
         const R dx = x - x0;
         v11 = (R)0.5 * mm * w * w * dx * dx; d11 = mm * w * w * dx;
         v22 = v11; d22 = d11; v12 = (R)0; d12 = (R)0;
-        break;
-    }
-    default:
+    } else {
         v11 = v22 = v12 = d11 = d22 = d12 = (R)0;
-        break;
     }
+}
+
+// Calls f(std::integral_constant<int, K>{}) for the model kind K of m (one dispatch for a whole
+// block of evaluations); unknown kinds take K = -1, whose model1d_kind is zero.
+template <class F>
+__device__ __forceinline__ void model1d_dispatch(const ModelRegs& m, F&& f) {
+    switch (m.kind) {
+    case MODEL_TULLY1: f(std::integral_constant<int, MODEL_TULLY1>{}); break;
+    case MODEL_TULLY2: f(std::integral_constant<int, MODEL_TULLY2>{}); break;
+    case MODEL_TULLY3: f(std::integral_constant<int, MODEL_TULLY3>{}); break;
+    case MODEL_DW_GAUSSIAN: f(std::integral_constant<int, MODEL_DW_GAUSSIAN>{}); break;
+    case MODEL_HARMONIC: f(std::integral_constant<int, MODEL_HARMONIC>{}); break;
+    default: f(std::integral_constant<int, -1>{}); break;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R& v22, R& v12,
+                                             R& d11, R& d22, R& d12) {
+    model1d_dispatch(m, [&](auto k) { model1d_kind<decltype(k)::value, R>(m, x, v11, v22, v12, d11, d22, d12); });
 }
 
 // ---------------------------------------------------------------------------

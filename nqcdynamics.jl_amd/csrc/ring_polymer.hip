@@ -22,17 +22,21 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 namespace nqcd {
 
-// Workgroup size per bead count and precision: 64 ring polymers (one full wavefront of
-// electronics lanes) up to 512 lanes in FP32 (two waves per SIMD, so up to 256 VGPRs without
-// spilling: the kernel needs 150-250); at most 256 lanes in FP64, whose Hartley rows take twice the
-// registers.
-template <int P, typename R>
+// Workgroup size per bead count, precision and kernel family.  RPMD (E = false): 64 ring polymers up
+// to 512 lanes in FP32 for up to 16 beads, else 256 (FP64 Hartley rows take twice the registers, and
+// 512-lane workgroups at 32 and 64 beads spill).  With the centroid
+// electronics (E = true) at most 256 lanes: the electronics lanes sit in one wavefront of the
+// workgroup while the others wait for them, so several smaller workgroups per compute unit keep its
+// SIMDs busy (their electronics wavefronts land on different SIMDs).
+template <int P, typename R, bool E>
 struct RpShape {
-    static constexpr int kMax = sizeof(R) == 4 ? 512 : 256;
+    static constexpr int kMax = (E || P > 16 || sizeof(R) != 4) ? 256 : 512;
     static constexpr int block = (64 * P < kMax) ? 64 * P : kMax;
     static constexpr int nr = block / P;   // ring polymers per workgroup
 };
@@ -78,6 +82,7 @@ __device__ __forceinline__ R hartley(R x, const R H[P], R* lds, int gbase) {
 #pragma unroll
     for (int k = 0; k < P; ++k) acc += H[k] * lds[gbase + k];
     __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);   // keeps the compiler from overlapping transforms (register pressure)
     return acc;
 }
 
@@ -90,33 +95,170 @@ __device__ __forceinline__ R group_value(R x, R* lds, int gbase, int lane_src) {
     return y;
 }
 
-// Bead force (per lane): RPMD uses the single-state potential; RPSH the active adiabatic
-// surface; RP-Ehrenfest the mean field with the centroid density (ehrenfest_rpmd.jl:23-43).
+// Gradient of the surface a bead moves on: the single-state potential (RPMD, sh = false) or the
+// active adiabatic surface of two states (RPSH), without the eigenvectors: E = m -+ rad with
+// m = (V11 + V22) / 2, h = (V11 - V22) / 2, rad = sqrt(h^2 + V12^2), so
+// dE/dx = dm -+ (h dh + V12 dV12) / rad, in the bead precision.
 template <typename R>
-__device__ __forceinline__ double bead_force(const RpArgs& A, const ModelRegs& m, R x, double Ub[4], bool track,
-                                             int st, double a, double b, double sx) {
+__device__ __forceinline__ R surface_gradient(const ModelRegs& m, R x, bool sh, int st) {
     R v11, v22, v12, d11, d22, d12;
     model1d_eval<R>(m, x, v11, v22, v12, d11, d22, d12);
-    if (A.method == 0) return (double)d11;
-    if (A.method == 1) {
-        // the active surface's gradient (U^T dV U)_ss without the eigenvectors: E = m -+ rad with
-        // m = (V11 + V22) / 2, h = (V11 - V22) / 2, rad = sqrt(h^2 + V12^2), so
-        // dE/dx = dm -+ (h dh + V12 dV12) / rad, in the bead precision
-        const R h = (R)0.5 * (v11 - v22), dh = (R)0.5 * (d11 - d22), dm = (R)0.5 * (d11 + d22);
-        const R rad = sqrt(h * h + v12 * v12);
-        const R g = (rad > (R)0) ? (h * dh + v12 * d12) / rad : (R)0;
-        return (double)((st == 0) ? dm - g : dm + g);
-    }
+    if (!sh) return d11;
+    const R h = (R)0.5 * (v11 - v22), dh = (R)0.5 * (d11 - d22), dm = (R)0.5 * (d11 + d22);
+    const R rad = sqrt(h * h + v12 * v12);
+    const R g = (rad > (R)0) ? (h * dh + v12 * d12) / rad : (R)0;
+    return (st == 0) ? dm - g : dm + g;
+}
+
+// Bead force (per lane): RPMD uses the single-state potential; RPSH the active adiabatic
+// surface; RP-Ehrenfest the mean field with the centroid density (ehrenfest_rpmd.jl:23-43).
+template <typename R, bool E>
+__device__ __forceinline__ double bead_force(const RpArgs& A, const ModelRegs& m, R x, double Ub[4], bool track,
+                                             int st, double a, double b, double sx) {
+    if (!E || A.method != 2) return (double)surface_gradient<R>(m, x, E && A.method == 1, st);
+    R v11, v22, v12, d11, d22, d12;
+    model1d_eval<R>(m, x, v11, v22, v12, d11, d22, d12);
     double E0, E1, G00, G11, G01;
     eigen2((double)v11, (double)v12, (double)v22, E0, E1, Ub, track);
     rotate_derivative(Ub, (double)d11, (double)d22, (double)d12, G00, G11, G01);
-    if (A.method == 1) return (st == 0) ? G00 : G11;
     return G00 * a + G11 * b + 2.0 * G01 * sx;
 }
 
-template <int P, typename R>
+// Centroid electronics of one ring polymer (bcb_electronics.jl:83-96): the density carried as its
+// trace tr and (q, x, y), the active state and hop count, the tracked eigenvectors at the centroid,
+// the interpolation buffer's last end (dE and w at time tc), and the step end's coupling and gap.
+struct RpElec {
+    double tr;
+    Sig2 s;
+    int est, nh;
+    double Uc[4];
+    double dEc, wc, tc;
+    double d01, gap;
+};
+
+// Initial density and active state of ring polymer id from the centroid at rc (fssh.jl:54 draws a
+// state when none is given); returns the density's diagonal and real coherence as given.
+template <typename R>
+__device__ __forceinline__ void rp_elec_init(const RpArgs* __restrict__ Ap, const ModelRegs& m, R rc, int64_t id,
+                                             RpElec& E, double& ea, double& eb, double& ex) {
+    const RpArgs& B = *rp_opaque(Ap);
+    double ey = 0.0;
+    ex = 0.0;
+    if (B.sig0) {
+        ea = B.sig0[id]; eb = B.sig0[B.ntraj + id]; ex = B.sig0[2 * B.ntraj + id]; ey = B.sig0[3 * B.ntraj + id];
+    } else {
+        ea = (B.initial_state == 0) ? 1.0 : 0.0; eb = 1.0 - ea;
+    }
+    E.est = B.state0 ? B.state0[id] : -1;
+    R v11, v22, v12, d11, d22, d12;
+    model1d_eval<R>(m, rc, v11, v22, v12, d11, d22, d12);
+    double E0c, E1c;
+    eigen2((double)v11, (double)v12, (double)v22, E0c, E1c, E.Uc, false);
+    if (B.method == 1 && E.est < 0) {
+        const double u = philox_uniform((uint64_t)id, 0u, STREAM_STATE, B.key) * (ea + eb);
+        E.est = (u < ea) ? 0 : 1;
+    }
+    E.tr = ea + eb;
+    E.s.q = ea - eb; E.s.x = ex; E.s.y = ey;
+}
+
+// The density over [t, tn] under the centroid at the step end (rc, vc): the interpolation buffer's
+// new end is dE = E0 - E1 and w = d01 vc at the centroid (the tracked eigenvectors, bcb_electronics.jl:88-92).
+template <typename R>
+__device__ __forceinline__ void rp_elec_propagate(const RpArgs& A, const ModelRegs& m, R rc, double vc, double t,
+                                                  double tn, RpElec& E) {
+    R v11, v22, v12, d11, d22, d12;
+    model1d_eval<R>(m, rc, v11, v22, v12, d11, d22, d12);
+    double E0c, E1c, G00c, G11c, G01c;
+    eigen2((double)v11, (double)v12, (double)v22, E0c, E1c, E.Uc, true);
+    rotate_derivative(E.Uc, (double)d11, (double)d22, (double)d12, G00c, G11c, G01c);
+    E.gap = E1c - E0c;
+    E.d01 = G01c / E.gap;
+    const double w = E.d01 * vc;
+    const double dEn = E0c - E1c;
+    const double inv = 1.0 / (tn - E.tc);
+    Interp2 Pi;
+    Pi.dE0 = E.dEc; Pi.dE1 = (dEn - E.dEc) * inv; Pi.w0 = E.wc; Pi.w1 = (w - E.wc) * inv;
+    propagate2(Pi, E.s, t, E.tc, A.dt, A.nsub);
+    E.dEc = dEn; E.wc = w; E.tc = tn;
+}
+
+// The centroid hop at the end of step `step` (rpsh.jl:30-50): returns the centroid mode's momentum
+// change (sqrt(P) times the centroid velocity change), zero unless a hop is accepted or reflected.
+__device__ __forceinline__ double rp_elec_hop(const RpArgs& A, double vc, double sqP, int64_t id, int step,
+                                              RpElec& E) {
+    double dp = 0.0;
+    const double sss = 0.5 * (E.tr + ((E.est == 0) ? E.s.q : -E.s.q));
+    const double ratio = E.s.x / sss;
+    const double dsm = (E.est == 0) ? E.d01 : -E.d01;
+    double pr = 2.0 * vc * ratio * dsm * A.dt;
+    if (pr > 0.0) {
+        pr = fmin(pr, 1.0);
+        const double rnd = philox_uniform((uint64_t)id, (uint32_t)step, STREAM_HOP, A.key);
+        if (pr > rnd) {
+            const double d = -dsm;  // coupling[new, old]
+            const double dE = (E.est == 0) ? E.gap : -E.gap;
+            bool ok = true;
+            if (A.rescaling != 2) {
+                const double aq = d * d / A.mass / 2.0;
+                const double bq = d * vc;
+                const double disc = bq * bq - 4.0 * aq * dE;
+                if (disc < 0.0) {
+                    ok = false;
+                    if (A.rescaling == 1) {  // all beads: only the centroid mode moves
+                        const double dn = d / fabs(d);
+                        dp = -(sqP * 2.0 * (vc * dn) * dn);
+                    }
+                } else {
+                    const double root = sqrt(disc);
+                    const double gam = (bq < 0.0) ? (bq + root) / (2.0 * aq) : (bq - root) / (2.0 * aq);
+                    dp = -(sqP * gam * d / A.mass);
+                }
+            }
+            if (ok) { E.est = 1 - E.est; ++E.nh; }
+        }
+    }
+    return dp;
+}
+
+// Electronic save slot of ring polymer id: adiabatic populations (the active state's indicator for
+// RPSH, the density's diagonal for RP-Ehrenfest), diabatic populations through the centroid's
+// eigenvectors, and the active state.
+__device__ __forceinline__ void rp_save_elec(const RpArgs& B, int slot, int64_t id, const RpElec& E) {
+    const int64_t n = B.ntraj;
+    double pa0, pa1;
+    if (B.method == 1) { pa0 = (E.est == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0; }
+    else { pa0 = 0.5 * (E.tr + E.s.q); pa1 = 0.5 * (E.tr - E.s.q); }
+    const double* U = E.Uc;
+    B.save_pop_ad[((int64_t)slot * 2) * n + id] = pa0;
+    B.save_pop_ad[((int64_t)slot * 2 + 1) * n + id] = pa1;
+    B.save_pop_di[((int64_t)slot * 2) * n + id] = U[0] * U[0] * pa0 + U[1] * U[1] * pa1 + 2.0 * U[0] * U[1] * E.s.x;
+    B.save_pop_di[((int64_t)slot * 2 + 1) * n + id] = U[2] * U[2] * pa0 + U[3] * U[3] * pa1 + 2.0 * U[2] * U[3] * E.s.x;
+    B.save_state[(int64_t)slot * n + id] = E.est;
+}
+
+// Final density, state, step and hop counts of ring polymer id.
+__device__ __forceinline__ void rp_store_elec(const RpArgs& B, int64_t id, const RpElec& E) {
+    const int64_t n = B.ntraj;
+    B.sig[id] = 0.5 * (E.tr + E.s.q);
+    B.sig[n + id] = 0.5 * (E.tr - E.s.q);
+    B.sig[2 * n + id] = E.s.x;
+    B.sig[3 * n + id] = E.s.y;
+    B.state[id] = E.est;
+    B.steps[id] = (int32_t)B.nsteps;
+    B.nhops[id] = E.nh;
+}
+
+// Final state of an RPMD ring polymer (one electronic state).
+__device__ __forceinline__ void rp_store_single(const RpArgs& B, int64_t id) {
+    const int64_t n = B.ntraj;
+    B.sig[id] = 1.0; B.sig[n + id] = 0.0; B.sig[2 * n + id] = 0.0; B.sig[3 * n + id] = 0.0;
+    B.state[id] = 0; B.steps[id] = (int32_t)B.nsteps; B.nhops[id] = 0;
+}
+
+template <int P, typename R, bool E>
 __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
-    constexpr int NB = RpShape<P, R>::block, NR = RpShape<P, R>::nr;
+    constexpr int NB = RpShape<P, R, E>::block, NR = RpShape<P, R, E>::nr;
     __shared__ R lds[NB];
     // hand-over between the mode-0 lanes and the electronics lanes, one entry per ring polymer
     __shared__ R x_rc[NR];                          // centroid position (bead precision)
@@ -161,18 +303,22 @@ __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
     }
     R q = hartley<P, R>(rb, H, lds, gbase);
     R p = hartley<P, R>(vb, H, lds, gbase);
-    const bool elec = A.method != 0;
-    // electronics lane: ring polymer e = threadIdx.x of the workgroup
-    const int e = threadIdx.x;
-    const bool eln = elec && e < NR;
+    // the electronics kernel also serves RPMD above 16 beads (electronics off at run time): the
+    // compiler allocates the RPMD-only kernel at those sizes badly (256 VGPRs and spills)
+    const bool elec = E && A.method != 0;
+    // electronics lanes: ring polymer e on lane e of wavefront blockIdx.x % (NB / 64) (rotating the
+    // wavefront, and with it the SIMD, from one workgroup to the next)
+    const int e = (int)threadIdx.x - 64 * (int)(blockIdx.x % (unsigned)(NB / 64));
+    const bool eln = elec && e >= 0 && e < NR;
     const int64_t erp = (int64_t)blockIdx.x * NR + e;
     const bool evalid = eln && erp < n;
     const int64_t eid = evalid ? erp : 0;
-    double tr = 1.0;
-    Sig2 s = {1.0, 0.0, 0.0};
-    int est = 0, nh = 0;
-    double Uc[4] = {1.0, 0.0, 0.0, 1.0}, Ub[4] = {1.0, 0.0, 0.0, 1.0};
-    double dEc = 0.0, wc = 0.0, tc = 0.0, t = A.t0;
+    RpElec El;
+    El.tr = 1.0; El.s = {1.0, 0.0, 0.0}; El.est = 0; El.nh = 0;
+    El.Uc[0] = 1.0; El.Uc[1] = 0.0; El.Uc[2] = 0.0; El.Uc[3] = 1.0;
+    El.dEc = 0.0; El.wc = 0.0; El.tc = 0.0; El.d01 = 0.0; El.gap = 0.0;
+    double Ub[4] = {1.0, 0.0, 0.0, 1.0};
+    double t = A.t0;
     // active state / density of the step start as the bead lanes see them
     int st = 0;
     double a = 1.0, b = 0.0, sx = 0.0;
@@ -180,34 +326,15 @@ __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
         if (l == 0) x_rc[j] = q / (R)sqP;
         __syncthreads();
         if (eln) {
-            double ea = 1.0, eb = 0.0, ex = 0.0, ey = 0.0;
-            if (evalid) {
-                const RpArgs& B = *rp_opaque(Ap);
-                if (B.sig0) {
-                    ea = B.sig0[eid]; eb = B.sig0[n + eid]; ex = B.sig0[2 * n + eid]; ey = B.sig0[3 * n + eid];
-                } else {
-                    ea = (B.initial_state == 0) ? 1.0 : 0.0; eb = 1.0 - ea;
-                }
-                est = B.state0 ? B.state0[eid] : -1;
-                R v11, v22, v12, d11, d22, d12;
-                model1d_eval<R>(m, x_rc[e], v11, v22, v12, d11, d22, d12);
-                double E0c, E1c, G00c, G11c, G01c;
-                eigen2((double)v11, (double)v12, (double)v22, E0c, E1c, Uc, false);
-                rotate_derivative(Uc, (double)d11, (double)d22, (double)d12, G00c, G11c, G01c);
-                if (A.method == 1 && est < 0) {  // fssh.jl:54
-                    const double u = philox_uniform((uint64_t)eid, 0u, STREAM_STATE, A.key) * (ea + eb);
-                    est = (u < ea) ? 0 : 1;
-                }
-            }
-            tr = ea + eb;
-            s.q = ea - eb; s.x = ex; s.y = ey;
-            x_st[e] = est; x_a[e] = ea; x_b[e] = eb; x_sx[e] = ex; x_dp[e] = 0.0;
+            double ea = 1.0, eb = 0.0, ex = 0.0;
+            if (evalid) rp_elec_init<R>(Ap, m, x_rc[e], eid, El, ea, eb, ex);
+            x_st[e] = El.est; x_a[e] = ea; x_b[e] = eb; x_sx[e] = ex; x_dp[e] = 0.0;
         }
         __syncthreads();
         st = x_st[j]; a = x_a[j]; b = x_b[j]; sx = x_sx[j];
     }
     // initial accelerations in modes
-    R acc = (R)(-bead_force<R>(A, m, rb, Ub, false, st, a, b, sx) / A.mass);
+    R acc = (R)(-bead_force<R, E>(A, m, rb, Ub, false, st, a, b, sx) / A.mass);
     R an = hartley<P, R>(acc, H, lds, gbase);
     // centroid slots come from the mode-0 lanes, electronic slots from the electronics lanes
     auto save_centroid = [&](int slot) {
@@ -218,18 +345,12 @@ __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
     auto save_elec = [&](int slot) {
         const RpArgs& B = *rp_opaque(Ap);
         if (slot >= B.nsave || !evalid) return;
-        double pa0, pa1;
-        if (B.method == 1) { pa0 = (est == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0; }
-        else { pa0 = 0.5 * (tr + s.q); pa1 = 0.5 * (tr - s.q); }
-        B.save_pop_ad[((int64_t)slot * 2) * n + eid] = pa0;
-        B.save_pop_ad[((int64_t)slot * 2 + 1) * n + eid] = pa1;
-        B.save_pop_di[((int64_t)slot * 2) * n + eid] = Uc[0] * Uc[0] * pa0 + Uc[1] * Uc[1] * pa1 + 2.0 * Uc[0] * Uc[1] * s.x;
-        B.save_pop_di[((int64_t)slot * 2 + 1) * n + eid] = Uc[2] * Uc[2] * pa0 + Uc[3] * Uc[3] * pa1 + 2.0 * Uc[2] * Uc[3] * s.x;
-        B.save_state[(int64_t)slot * n + eid] = est;
+        rp_save_elec(B, slot, eid, El);
     };
     save_centroid(0);
     save_elec(0);
     const R hdt = (R)(0.5 * A.dt);
+#pragma unroll 1
     for (int step = 0; step < A.nsteps; ++step) {
         if (elec) {
             if (step > 0) {  // hand-over from the previous step's electronics
@@ -251,7 +372,7 @@ __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
             q = qn; p = pn;
         }
         rb = hartley<P, R>(q, H, lds, gbase);                    // bead positions
-        acc = (R)(-bead_force<R>(A, m, rb, Ub, true, st, a, b, sx) / A.mass);
+        acc = (R)(-bead_force<R, E>(A, m, rb, Ub, true, st, a, b, sx) / A.mass);
         an = hartley<P, R>(acc, H, lds, gbase);
         p = p + hdt * an;                                        // step_B!
         const double tn = t + A.dt;
@@ -261,60 +382,15 @@ __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
             if (eln) {
                 double dp = 0.0;
                 if (evalid) {
-                    const R rc = x_rc[e];
                     const double vc = x_vc[e];
-                    R v11, v22, v12, d11, d22, d12;
-                    model1d_eval<R>(m, rc, v11, v22, v12, d11, d22, d12);
-                    double E0c, E1c, G00c, G11c, G01c;
-                    eigen2((double)v11, (double)v12, (double)v22, E0c, E1c, Uc, true);
-                    rotate_derivative(Uc, (double)d11, (double)d22, (double)d12, G00c, G11c, G01c);
-                    const double gap = E1c - E0c;
-                    const double d01 = G01c / gap;
-                    const double w = d01 * vc;
-                    const double dEn = E0c - E1c;
-                    const double inv = 1.0 / (tn - tc);
-                    Interp2 Pi;
-                    Pi.dE0 = dEc; Pi.dE1 = (dEn - dEc) * inv; Pi.w0 = wc; Pi.w1 = (w - wc) * inv;
-                    propagate2(Pi, s, t, tc, A.dt, A.nsub);
-                    dEc = dEn; wc = w; tc = tn;
+                    rp_elec_propagate<R>(A, m, x_rc[e], vc, t, tn, El);
                     // the save precedes the hop callback (a save time holds the state before the step's
                     // callbacks; the hop's velocity change reaches the beads at the next step)
                     if (A.save_every > 0 && ((step + 1) % A.save_every) == 0) save_elec((step + 1) / A.save_every);
-                    if (A.method == 1) {
-                        const double sss = 0.5 * (tr + ((est == 0) ? s.q : -s.q));
-                        const double ratio = s.x / sss;
-                        const double dsm = (est == 0) ? d01 : -d01;
-                        double pr = 2.0 * vc * ratio * dsm * A.dt;
-                        if (pr > 0.0) {
-                            pr = fmin(pr, 1.0);
-                            const double rnd = philox_uniform((uint64_t)eid, (uint32_t)step, STREAM_HOP, A.key);
-                            if (pr > rnd) {
-                                const double d = -dsm;  // coupling[new, old]
-                                const double dE = (est == 0) ? gap : -gap;
-                                bool ok = true;
-                                if (A.rescaling != 2) {
-                                    const double aq = d * d / A.mass / 2.0;
-                                    const double bq = d * vc;
-                                    const double disc = bq * bq - 4.0 * aq * dE;
-                                    if (disc < 0.0) {
-                                        ok = false;
-                                        if (A.rescaling == 1) {  // all beads: only the centroid mode moves
-                                            const double dn = d / fabs(d);
-                                            dp = -(sqP * 2.0 * (vc * dn) * dn);
-                                        }
-                                    } else {
-                                        const double root = sqrt(disc);
-                                        const double gam = (bq < 0.0) ? (bq + root) / (2.0 * aq) : (bq - root) / (2.0 * aq);
-                                        dp = -(sqP * gam * d / A.mass);
-                                    }
-                                }
-                                if (ok) { est = 1 - est; ++nh; }
-                            }
-                        }
-                    }
+                    if (A.method == 1) dp = rp_elec_hop(A, vc, sqP, eid, step, El);
                 }
-                x_st[e] = est;
-                x_a[e] = 0.5 * (tr + s.q); x_b[e] = 0.5 * (tr - s.q); x_sx[e] = s.x;
+                x_st[e] = El.est;
+                x_a[e] = 0.5 * (El.tr + El.s.q); x_b[e] = 0.5 * (El.tr - El.s.q); x_sx[e] = El.s.x;
                 x_dp[e] = dp;
             }
         }
@@ -332,29 +408,186 @@ __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
     if (valid) {
         B.r[(int64_t)l * n + id] = (double)rb;
         B.v[(int64_t)l * n + id] = (double)vb;
-        if (!elec && l == 0) {  // RPMD: no electronic state
-            B.sig[id] = 1.0; B.sig[n + id] = 0.0; B.sig[2 * n + id] = 0.0; B.sig[3 * n + id] = 0.0;
-            B.state[id] = 0; B.steps[id] = (int32_t)B.nsteps; B.nhops[id] = 0;
-        }
+        if (!elec && l == 0) rp_store_single(B, id);  // RPMD: no electronic state
     }
-    if (evalid) {
-        B.sig[eid] = 0.5 * (tr + s.q);
-        B.sig[n + eid] = 0.5 * (tr - s.q);
-        B.sig[2 * n + eid] = s.x;
-        B.sig[3 * n + eid] = s.y;
-        B.state[eid] = est;
-        B.steps[eid] = (int32_t)B.nsteps;
-        B.nhops[eid] = nh;
+    if (evalid) rp_store_elec(B, eid, El);
+}
+
+// ---------------------------------------------------------------------------
+// One ring polymer per lane (RPMD and RPSH; up to 16 beads in FP32, 8 in FP64).  The lane keeps all
+// P normal modes in registers and applies the Hartley transforms as P x P products with the P
+// coefficients cas(2 pi m / P) / sqrt(P) (H[j][k] depends on jk mod P only), so nothing passes
+// through LDS, no workgroup barrier is needed, and every lane runs its own ring polymer's centroid
+// electronics: in the bead-per-lane layout P - 1 of every P lanes wait while the FP64 electronics run.
+// ---------------------------------------------------------------------------
+template <int P, typename R>
+__device__ __forceinline__ void hartley_regs(const R c[P], const R x[P], R y[P]) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        R acc = (R)0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc += c[(j * k) % P] * x[k];
+        y[j] = acc;
     }
 }
 
-#define NQCD_RP_KERNELS(P)                                                                                  \
-    __global__ __launch_bounds__((RpShape<P, float>::block)) void rp_kernel_f32_##P(const RpArgs* __restrict__ Ap) { \
-        rp_body<P, float>(Ap);                                                                              \
-    }                                                                                                       \
-    __global__ __launch_bounds__((RpShape<P, double>::block)) void rp_kernel_f64_##P(const RpArgs* __restrict__ Ap) { \
-        rp_body<P, double>(Ap);                                                                             \
+constexpr int kRpLaneBlock = 64;   // one wavefront per workgroup: small ensembles spread over every CU
+
+// Accelerations of P beads from the gradients of the surface each moves on (surface_gradient), under
+// one model dispatch so the beads form straight-line code.  FP32 beads take 1/rad from the hardware
+// reciprocal square root and multiply by the FP32 inverse mass; FP64 beads keep the rounded square
+// root and the divisions.
+template <int P, typename R, bool SH>
+__device__ __forceinline__ void bead_accelerations(const ModelRegs& m, const R x[P], int st, double mass, R acc[P]) {
+    model1d_dispatch(m, [&](auto kind) {
+        constexpr int K = decltype(kind)::value;
+        const R inv_m = (R)(1.0 / mass);
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            R v11, v22, v12, d11, d22, d12;
+            model1d_kind<K, R>(m, x[l], v11, v22, v12, d11, d22, d12);
+            R g = d11;
+            if constexpr (SH) {
+                const R h = (R)0.5 * (v11 - v22), dh = (R)0.5 * (d11 - d22), dm = (R)0.5 * (d11 + d22);
+                const R r2 = h * h + v12 * v12;
+                const R num = h * dh + v12 * d12;
+                R c;
+                if constexpr (sizeof(R) == 4) {
+                    c = (r2 > 0.0f) ? num * __builtin_amdgcn_rsqf(r2) : 0.0f;
+                } else {
+                    const R rad = sqrt(r2);
+                    c = (rad > (R)0) ? num / rad : (R)0;
+                }
+                g = (st == 0) ? dm - c : dm + c;
+            }
+            if constexpr (sizeof(R) == 4) acc[l] = -g * inv_m;
+            else acc[l] = -g / mass;
+        }
+    });
+}
+
+template <int P, typename R, bool SH>
+__device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
+    const RpArgs& A = *Ap;
+    const int64_t n = A.ntraj;
+    const int64_t id = (int64_t)blockIdx.x * kRpLaneBlock + threadIdx.x;
+    if (id >= n) return;   // no barriers below
+    constexpr bool sh = SH;
+    R c[P], C11[P], C12[P], C21[P];
+    {
+        const double s = 1.0 / sqrt((double)P);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            double sn, cs;
+            sincospi(2.0 * (double)k / (double)P, &sn, &cs);
+            c[k] = (R)((cs + sn) * s);
+            // Cayley map of mode k (ring_polymer.jl:71-82, half = false)
+            const double w = 2.0 * A.wn * sinpi((double)k / (double)P);
+            const double u = A.dt * A.dt * w * w / 4.0;
+            const double si = 1.0 / (1.0 + u);
+            C11[k] = (R)((1.0 - u) * si); C12[k] = (R)(A.dt * si); C21[k] = (R)(-A.dt * w * w * si);
+        }
     }
+    const double sqP = sqrt((double)P);
+    const ModelRegs m = model_regs(A.model);
+    R q[P], p[P], an[P];
+    RpElec El;
+    El.tr = 1.0; El.s = {1.0, 0.0, 0.0}; El.est = 0; El.nh = 0;
+    El.Uc[0] = 1.0; El.Uc[1] = 0.0; El.Uc[2] = 0.0; El.Uc[3] = 1.0;
+    El.dEc = 0.0; El.wc = 0.0; El.tc = 0.0; El.d01 = 0.0; El.gap = 0.0;
+    {
+        const RpArgs& B = *rp_opaque(Ap);
+        R rb[P], vb[P];
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            rb[l] = (R)B.r0[(int64_t)l * n + id];
+            vb[l] = (R)B.v0[(int64_t)l * n + id];
+        }
+        hartley_regs<P, R>(c, rb, q);
+        hartley_regs<P, R>(c, vb, p);
+        if (sh) {
+            double ea, eb, ex;
+            rp_elec_init<R>(Ap, m, q[0] / (R)sqP, id, El, ea, eb, ex);
+        }
+        R acc[P];
+        bead_accelerations<P, R, SH>(m, rb, El.est, A.mass, acc);
+        hartley_regs<P, R>(c, acc, an);
+    }
+    auto save = [&](int slot) {   // centroid and electronics of save slot `slot`
+        const RpArgs& B = *rp_opaque(Ap);
+        if (slot >= B.nsave) return;
+        B.save_centroid[(int64_t)slot * n + id] = (double)q[0] / sqP;
+        if (sh) rp_save_elec(B, slot, id, El);
+    };
+    save(0);
+    const R hdt = (R)(0.5 * A.dt);
+    double t = A.t0;
+#pragma unroll 1
+    for (int step = 0; step < A.nsteps; ++step) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) {                            // step_B!, step_C! (mode k)
+            const R pk = p[k] + hdt * an[k];
+            const R qn = C11[k] * q[k] + C12[k] * pk;
+            p[k] = C21[k] * q[k] + C11[k] * pk;
+            q[k] = qn;
+        }
+        {
+            R rb[P], acc[P];
+            hartley_regs<P, R>(c, q, rb);                        // bead positions
+            bead_accelerations<P, R, SH>(m, rb, El.est, A.mass, acc);
+            hartley_regs<P, R>(c, acc, an);
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * an[k];  // step_B!
+        const double tn = t + A.dt;
+        const bool save_now = A.save_every > 0 && ((step + 1) % A.save_every) == 0;
+        double dp = 0.0;
+        if (sh) {
+            const double vc = (double)p[0] / sqP;
+            rp_elec_propagate<R>(A, m, q[0] / (R)sqP, vc, t, tn, El);
+            // the save precedes the hop callback (a save time holds the state before the step's callbacks)
+            if (save_now) save((step + 1) / A.save_every);
+            dp = rp_elec_hop(A, vc, sqP, id, step, El);
+        } else if (save_now) {
+            save((step + 1) / A.save_every);
+        }
+        if (dp != 0.0) p[0] = (R)((double)p[0] + dp);
+        t = tn;
+    }
+    const RpArgs& B = *rp_opaque(Ap);
+    {
+        R rb[P], vb[P];
+        hartley_regs<P, R>(c, q, rb);
+        hartley_regs<P, R>(c, p, vb);
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            B.r[(int64_t)l * n + id] = (double)rb[l];
+            B.v[(int64_t)l * n + id] = (double)vb[l];
+        }
+    }
+    if (sh) rp_store_elec(B, id, El);
+    else rp_store_single(B, id);
+}
+
+#define NQCD_RPMD_KERNELS(P)                                                                                \
+    __global__ __launch_bounds__((RpShape<P, float, false>::block)) void rp_kernel_f32_##P(const RpArgs* __restrict__ Ap) { \
+        rp_body<P, float, false>(Ap);                                                                       \
+    }                                                                                                       \
+    __global__ __launch_bounds__((RpShape<P, double, false>::block)) void rp_kernel_f64_##P(const RpArgs* __restrict__ Ap) { \
+        rp_body<P, double, false>(Ap);                                                                      \
+    }
+#define NQCD_RP_KERNELS(P)                                                                                  \
+    __global__ __launch_bounds__((RpShape<P, float, true>::block)) void rp_elec_f32_##P(const RpArgs* __restrict__ Ap) { \
+        rp_body<P, float, true>(Ap);                                                                        \
+    }                                                                                                       \
+    __global__ __launch_bounds__((RpShape<P, double, true>::block)) void rp_elec_f64_##P(const RpArgs* __restrict__ Ap) { \
+        rp_body<P, double, true>(Ap);                                                                       \
+    }
+NQCD_RPMD_KERNELS(1)
+NQCD_RPMD_KERNELS(2)
+NQCD_RPMD_KERNELS(4)
+NQCD_RPMD_KERNELS(8)
+NQCD_RPMD_KERNELS(16)
 NQCD_RP_KERNELS(1)
 NQCD_RP_KERNELS(2)
 NQCD_RP_KERNELS(4)
@@ -363,6 +596,26 @@ NQCD_RP_KERNELS(16)
 NQCD_RP_KERNELS(32)
 NQCD_RP_KERNELS(64)
 #undef NQCD_RP_KERNELS
+#undef NQCD_RPMD_KERNELS
+
+// rp_lane_*: RPMD; rp_lanesh_*: RPSH
+#define NQCD_RP_LANE_KERNELS(P, R, SUFFIX)                                                                  \
+    __global__ __launch_bounds__(kRpLaneBlock) void rp_lane_##SUFFIX(const RpArgs* __restrict__ Ap) {      \
+        rp_lane_body<P, R, false>(Ap);                                                                      \
+    }                                                                                                       \
+    __global__ __launch_bounds__(kRpLaneBlock) void rp_lanesh_##SUFFIX(const RpArgs* __restrict__ Ap) {    \
+        rp_lane_body<P, R, true>(Ap);                                                                       \
+    }
+NQCD_RP_LANE_KERNELS(1, float, f32_1)
+NQCD_RP_LANE_KERNELS(2, float, f32_2)
+NQCD_RP_LANE_KERNELS(4, float, f32_4)
+NQCD_RP_LANE_KERNELS(8, float, f32_8)
+NQCD_RP_LANE_KERNELS(16, float, f32_16)
+NQCD_RP_LANE_KERNELS(1, double, f64_1)
+NQCD_RP_LANE_KERNELS(2, double, f64_2)
+NQCD_RP_LANE_KERNELS(4, double, f64_4)
+NQCD_RP_LANE_KERNELS(8, double, f64_8)
+#undef NQCD_RP_LANE_KERNELS
 
 // Thermal ring polymers in the harmonic approximation of a well at x0 with frequency w_well:
 // mode k ~ N(sqrt(P) x0 delta_k0, 1/(beta_P m (w_k^2 + w_well^2))), velocity ~ N(0, 1/(beta_P m)),
@@ -510,12 +763,17 @@ typedef void (*RpKernel)(const RpArgs*);
 
 static bool rp_supported(int P) { return P == 1 || P == 2 || P == 4 || P == 8 || P == 16 || P == 32 || P == 64; }
 
-// Workgroup size and ring polymers per workgroup of the (P, precision) kernel.
-static void rp_shape(int P, int precision, int& block, int& nr) {
+// Workgroup size and ring polymers per workgroup of the (P, precision, electronics) kernel.
+static void rp_shape(int P, int precision, bool elec, int& block, int& nr) {
 #define NQCD_CASE(PP)                                                                                         \
     case PP:                                                                                                  \
-        block = precision == 0 ? (RpShape<PP, double>::block) : (RpShape<PP, float>::block);                  \
-        nr = precision == 0 ? (RpShape<PP, double>::nr) : (RpShape<PP, float>::nr);                           \
+        if (elec) {                                                                                           \
+            block = precision == 0 ? (RpShape<PP, double, true>::block) : (RpShape<PP, float, true>::block);  \
+            nr = precision == 0 ? (RpShape<PP, double, true>::nr) : (RpShape<PP, float, true>::nr);           \
+        } else {                                                                                              \
+            block = precision == 0 ? (RpShape<PP, double, false>::block) : (RpShape<PP, float, false>::block); \
+            nr = precision == 0 ? (RpShape<PP, double, false>::nr) : (RpShape<PP, float, false>::nr);         \
+        }                                                                                                     \
         return;
     switch (P) {
         NQCD_CASE(1) NQCD_CASE(2) NQCD_CASE(4) NQCD_CASE(8) NQCD_CASE(16) NQCD_CASE(32) NQCD_CASE(64)
@@ -524,18 +782,77 @@ static void rp_shape(int P, int precision, int& block, int& nr) {
 #undef NQCD_CASE
 }
 
-static hipError_t rp_launch(int P, int precision, int grid, hipStream_t s, const RpArgs* Adev) {
-#define NQCD_CASE(PP)                                                                                         \
+// elec selects the electronics kernel; the RPMD-only kernels exist up to 16 beads.
+static bool rp_needs_elec(int method, int P) { return method != NQCD_METHOD_RPMD || P > 16; }
+
+// The one-ring-polymer-per-lane kernels exist for RPMD and RPSH up to 16 beads in FP32 and 8 in FP64.
+static bool rp_lane_exists(int method, int P, int precision) {
+    if (method != NQCD_METHOD_RPMD && method != NQCD_METHOD_RPSH) return false;
+    return precision == 0 ? P <= 8 : P <= 16;
+}
+
+// Layout of a run: one ring polymer per lane where that kernel exists (measured at 1e5 ring polymers
+// of 8 beads, FP32: RPSH 3.4x and RPMD 1.3x the bead-per-lane rate; at 16 beads 4.2x and 2.1x), P
+// lanes per ring polymer otherwise.  NQCD_MI355X_RP_LAYOUT=lane or bead overrides the choice where
+// both kernels exist.
+static bool rp_use_lane(int method, int P, int precision) {
+    if (!rp_lane_exists(method, P, precision)) return false;
+    if (const char* env = getenv("NQCD_MI355X_RP_LAYOUT")) {
+        if (!strcmp(env, "bead")) return false;
+    }
+    return true;
+}
+
+static hipError_t rp_launch_lane(int P, int precision, bool sh, int grid, hipStream_t s, const RpArgs* Adev) {
+    const dim3 g(grid), b(kRpLaneBlock);
+#define NQCD_LANE_CASE(PP, SUFFIX)                                                                            \
+    case PP:                                                                                                  \
+        if (sh) hipLaunchKernelGGL(rp_lanesh_##SUFFIX, g, b, 0, s, Adev);                                     \
+        else hipLaunchKernelGGL(rp_lane_##SUFFIX, g, b, 0, s, Adev);                                          \
+        break;
+    if (precision == 0) {
+        switch (P) {
+            NQCD_LANE_CASE(1, f64_1) NQCD_LANE_CASE(2, f64_2) NQCD_LANE_CASE(4, f64_4) NQCD_LANE_CASE(8, f64_8)
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (P) {
+            NQCD_LANE_CASE(1, f32_1) NQCD_LANE_CASE(2, f32_2) NQCD_LANE_CASE(4, f32_4) NQCD_LANE_CASE(8, f32_8)
+            NQCD_LANE_CASE(16, f32_16)
+        default: return hipErrorInvalidValue;
+        }
+    }
+#undef NQCD_LANE_CASE
+    return hipGetLastError();
+}
+
+static hipError_t rp_launch(int P, int precision, bool elec, int grid, hipStream_t s, const RpArgs* Adev) {
+#define NQCD_ELEC_CASE(PP)                                                                                    \
     case PP:                                                                                                  \
         if (precision == 0)                                                                                   \
-            hipLaunchKernelGGL(rp_kernel_f64_##PP, dim3(grid), dim3((RpShape<PP, double>::block)), 0, s, Adev); \
-        else hipLaunchKernelGGL(rp_kernel_f32_##PP, dim3(grid), dim3((RpShape<PP, float>::block)), 0, s, Adev); \
+            hipLaunchKernelGGL(rp_elec_f64_##PP, dim3(grid), dim3((RpShape<PP, double, true>::block)), 0, s, Adev); \
+        else hipLaunchKernelGGL(rp_elec_f32_##PP, dim3(grid), dim3((RpShape<PP, float, true>::block)), 0, s, Adev); \
         break;
-    switch (P) {
-        NQCD_CASE(1) NQCD_CASE(2) NQCD_CASE(4) NQCD_CASE(8) NQCD_CASE(16) NQCD_CASE(32) NQCD_CASE(64)
-    default: return hipErrorInvalidValue;
+#define NQCD_RPMD_CASE(PP)                                                                                    \
+    case PP:                                                                                                  \
+        if (precision == 0)                                                                                   \
+            hipLaunchKernelGGL(rp_kernel_f64_##PP, dim3(grid), dim3((RpShape<PP, double, false>::block)), 0, s, Adev); \
+        else hipLaunchKernelGGL(rp_kernel_f32_##PP, dim3(grid), dim3((RpShape<PP, float, false>::block)), 0, s, Adev); \
+        break;
+    if (elec) {
+        switch (P) {
+            NQCD_ELEC_CASE(1) NQCD_ELEC_CASE(2) NQCD_ELEC_CASE(4) NQCD_ELEC_CASE(8) NQCD_ELEC_CASE(16)
+            NQCD_ELEC_CASE(32) NQCD_ELEC_CASE(64)
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (P) {
+            NQCD_RPMD_CASE(1) NQCD_RPMD_CASE(2) NQCD_RPMD_CASE(4) NQCD_RPMD_CASE(8) NQCD_RPMD_CASE(16)
+        default: return hipErrorInvalidValue;
+        }
     }
-#undef NQCD_CASE
+#undef NQCD_ELEC_CASE
+#undef NQCD_RPMD_CASE
     return hipGetLastError();
 }
 
@@ -687,11 +1004,16 @@ int32_t rp_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     HIPCHK(h, h->args_dev.ensure(sizeof(RpArgs)));
     HIPCHK(h, hipMemcpyAsync(h->args_dev.p, &A, sizeof(RpArgs), hipMemcpyHostToDevice, h->stream));
     int block = 0, nr = 0;
-    rp_shape(P, h->prob.precision, block, nr);
+    const bool lane = rp_use_lane(h->prob.method, P, h->prob.precision);
+    const bool elec = rp_needs_elec(h->prob.method, P);
+    if (lane) { block = kRpLaneBlock; nr = kRpLaneBlock; }
+    else rp_shape(P, h->prob.precision, elec, block, nr);
     if (nr < 1) return fail(h, NQCD_ERR_UNSUPPORTED, "nbeads");
     const int grid = (int)((n + nr - 1) / nr);
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    HIPCHK(h, rp_launch(P, h->prob.precision, grid, h->stream, h->args_dev.as<const RpArgs>()));
+    if (lane) HIPCHK(h, rp_launch_lane(P, h->prob.precision, h->prob.method == NQCD_METHOD_RPSH, grid, h->stream,
+                                       h->args_dev.as<const RpArgs>()));
+    else HIPCHK(h, rp_launch(P, h->prob.precision, elec, grid, h->stream, h->args_dev.as<const RpArgs>()));
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     h->stats.block = block;
     h->stats.grid = grid;

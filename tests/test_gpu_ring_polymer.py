@@ -30,9 +30,18 @@ def _thermal_beads(L, P, ntraj, key, x0=-3.87, nstates=1):
     return r, v
 
 
+def _layout(monkeypatch, layout, P, exists):
+    """Selects the ring-polymer layout: one ring polymer per lane or P lanes per ring polymer."""
+    if layout == "lane" and not exists:
+        pytest.skip(f"no one-ring-polymer-per-lane kernel at P = {P}")
+    monkeypatch.setenv("NQCD_MI355X_RP_LAYOUT", layout)
+
+
+@pytest.mark.parametrize("layout", ["bead", "lane"])
 @pytest.mark.parametrize("P", [1, 2, 4, 8, 16, 32, 64])
-def test_rpmd_fp64_parity(oracle, key, P):
+def test_rpmd_fp64_parity(oracle, key, P, layout, monkeypatch):
     from nqcd_mi355x import _lib as L
+    _layout(monkeypatch, layout, P, P <= 8)
     ntraj, nsteps = 256, 500
     r0, v0 = _thermal_beads(L, P, ntraj, key)
     assert np.isfinite(r0).all() and abs(r0.mean() + 3.87) < 0.2
@@ -49,10 +58,12 @@ def test_rpmd_fp64_parity(oracle, key, P):
     assert np.allclose(g_c, c["centroid"], rtol=1e-9, atol=1e-10)
 
 
+@pytest.mark.parametrize("layout", ["bead", "lane"])
 @pytest.mark.parametrize("P", [1, 4, 8, 32, 64])
 @pytest.mark.parametrize("method", ["rpsh", "rpeh"])
-def test_rp_electronic_fp64_parity(oracle, key, P, method):
+def test_rp_electronic_fp64_parity(oracle, key, P, method, layout, monkeypatch):
     from nqcd_mi355x import _lib as L
+    _layout(monkeypatch, layout, P, method == "rpsh" and P <= 8)
     ntraj, nsteps = 256, 600
     rng = np.random.default_rng(P)
     # ring polymers crossing the coupling region at x ~ 0 with a spread of velocities
@@ -198,6 +209,27 @@ def test_pimc_path_integral_law_on_device(key):
     assert np.all(np.abs(z) < 5), z
     assert np.all(np.abs(y.mean(axis=0)) < 5 * np.sqrt(var / ntraj))
     assert abs(v.var() / (P * T / mass) - 1) < 5 * np.sqrt(2.0 / v.size)
+
+
+@pytest.mark.parametrize("P", [4, 8, 16])
+def test_rp_layouts_agree_fp32(key, P, monkeypatch):
+    """The two FP32 layouts (one ring polymer per lane; P lanes per ring polymer with the electronics on
+    their own lanes) integrate the same ring polymers: RPSH over 1500 steps from PIMC beads ends in the same
+    active state for at least 97% of 4096 ring polymers, and RPMD centroids agree to FP32 accuracy."""
+    from nqcd_mi355x import _lib as L
+    ntraj, nsteps = 4096, 1500
+    out = {}
+    for layout in ("bead", "lane"):
+        monkeypatch.setenv("NQCD_MI355X_RP_LAYOUT", layout)
+        for method, ns in ((L.METHOD_RPSH, 2), (L.METHOD_RPMD, 1)):
+            E = _ens(L, method, P, ns, precision=1)
+            E.sample_initial(L.SAMPLER_RING_POLYMER_PIMC, ntraj, [-3.87, 2.0, 100, 1.0, 1, 0], key, initial_state=0)
+            E.run(0.0, float(nsteps if ns == 2 else 200), key, save_every=50)
+            out[layout, ns] = (E.download(L.FIELD_STATE), E.download(L.FIELD_SAVE_CENTROID))
+            E.close()
+    assert np.mean(out["bead", 2][0] == out["lane", 2][0]) > 0.97
+    cb, cl = out["bead", 1][1], out["lane", 1][1]
+    assert np.allclose(cb, cl, rtol=0, atol=1e-4 * np.abs(cb).max()), np.abs(cb - cl).max()
 
 
 def test_rpsh_fp32_beads_follow_fp64(key):
