@@ -314,7 +314,8 @@ TRAIN_BATCH = 2048
 def surrogate_extra(L):
     """Learned form on the six-coordinate diatom-surface model: train the two-member BF16 surrogate
     on configurations the analytic FSSH visits, then FSSH on the surrogate vs on the analytic model
-    (persistent kernel) and the final-population error."""
+    (persistent kernel) and vs the plain port of the analytic model (FP64, one launch per step, state in
+    HBM) on the same 1e6 trajectories, and the final-population errors."""
     import nqcd_mi355x as NQ
     analytic = NQ.DiatomSurface()
     mass = [12859.0, 12859.0]
@@ -328,9 +329,9 @@ def surrogate_extra(L):
         v0 = 2e-4 * rng.normal(size=(ntraj, 6)); v0[:, 2] -= 3e-3; v0[:, 5] -= 3e-3
         return r0, v0
 
-    def run(model, r0, v0, save_every=0):
+    def run(model, r0, v0, save_every=0, precision=L.PREC_FP32_NUCLEI, schedule=L.SCHED_PERSISTENT):
         E = L.Ensemble(0)
-        E.set_problem(L.METHOD_FSSH, 2, 3, 2, mass, 2.0, precision=L.PREC_FP32_NUCLEI)
+        E.set_problem(L.METHOD_FSSH, 2, 3, 2, mass, 2.0, precision=precision, schedule=schedule)
         E.set_model(0, model.kind, model.params())
         if model.kind == L.MODEL_MLP_SURROGATE:
             for k, p in enumerate(model.members):
@@ -358,12 +359,15 @@ def surrogate_extra(L):
     r1, v1 = inputs(1000000, 43)
     ref = run(analytic, r1, v1)
     sur = run(model, r1, v1)
+    plain = run(analytic, r1, v1, precision=L.PREC_FP64, schedule=L.SCHED_STEP_PER_LAUNCH)
     n = 2000  # the first trajectories for the oracle check of the CPU-baseline leg
     sample = {"r0": r1[:n].copy(), "v0": v1[:n].copy(), "params": analytic.params(), "mass": np.array(mass),
               "nsteps": 5000, "term": 8.0, "surrogate_state": sur["state"][:n], "analytic_state": ref["state"][:n]}
     rate = lambda o: o["stats"]["traj_steps"] / (o["stats"]["kernel_ms"] / 1e3)
-    p_ref, p_sur = float(ref["final"][1]), float(sur["final"][1])
-    _progress(f"surrogate FSSH {rate(sur):.3e} / analytic {rate(ref):.3e} trajectory-steps/s")
+    p_ref, p_sur, p_plain = float(ref["final"][1]), float(sur["final"][1]), float(plain["final"][1])
+    plain_rate = plain["stats"]["traj_steps"] / (plain["stats"]["wall_ms"] / 1e3)
+    _progress(f"surrogate FSSH {rate(sur):.3e} / analytic {rate(ref):.3e} / plain port {plain_rate:.3e} "
+              f"trajectory-steps/s")
     # BF16 MFMA work of one trajectory-step: two members x (forward 128x128 and 16(3)x128 products +
     # three reverse-mode 128x128 products for the input gradient)
     flop = 2 * (4 * 2 * 128 * 128 + 2 * 16 * 128)
@@ -372,8 +376,11 @@ def surrogate_extra(L):
             "final_loss": [float(hist[0, -100:].mean()), float(hist[1, -100:].mean())],
             "surrogate_traj_steps_per_s": rate(sur), "analytic_traj_steps_per_s": rate(ref),
             "surrogate_kernel_ms": sur["stats"]["kernel_ms"], "analytic_kernel_ms": ref["stats"]["kernel_ms"],
-            "final_excited_population": {"analytic": p_ref, "surrogate": p_sur},
-            "pop_error_vs_analytic": abs(p_sur - p_ref),
+            "plain_traj_steps_per_s": plain_rate, "plain_wall_ms": plain["stats"]["wall_ms"],
+            "plain_launches": plain["stats"]["launches"],
+            "learned_vs_plain": rate(sur) / plain_rate, "batched_vs_plain": rate(ref) / plain_rate,
+            "final_excited_population": {"analytic": p_ref, "surrogate": p_sur, "plain_fp64": p_plain},
+            "pop_error_vs_analytic": abs(p_sur - p_ref), "pop_error_vs_plain_fp64": abs(p_sur - p_plain),
             "pop_sigma": float(np.sqrt(p_ref * (1 - p_ref) / len(r1))),
             "fallback_fraction": sur["fallbacks"] / sur["stats"]["traj_steps"],
             "bf16_flop_per_traj_step": flop,
@@ -404,6 +411,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-port comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the IESH and surrogate measurements")
+    ap.add_argument("--only", choices=["iesh", "spin_boson", "ring_polymer", "learned"],
+                    help="run one of the other workloads alone and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -421,6 +430,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from nqcd_mi355x import _lib as L
+    if args.only:
+        fn = {"iesh": iesh_extra, "spin_boson": spin_boson_extra, "ring_polymer": ring_polymer_extra,
+              "learned": surrogate_extra}[args.only]
+        print(json.dumps({args.only: fn(L)[0]}))
+        return
 
     # --- batched: persistent work-queue kernel, FP32 nuclei / FP64 electronics -------------------
     E = _ensemble(L, L.PREC_FP32_NUCLEI, L.SCHED_PERSISTENT)

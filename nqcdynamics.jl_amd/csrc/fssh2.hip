@@ -43,10 +43,50 @@ struct Lane2 {
     double U[4];
     double dEc, wc, tc; // electronic buffer at the start of the step
     double t;
-    ModelRegs m;
+    ModelLane<R> m;     // the lane's model, its parameters in the nuclear precision
     int64_t tid;
     int32_t st, nh, step;
 };
+
+// The argument fields the step reads, copied once into scalar registers for the whole persistent
+// loop.  Read per step through the argument pointer, each is a vector-memory round trip with a full
+// wait (the loop's stores keep the compiler from using scalar loads there), on the critical path of
+// the longest trajectories once their waves run alone.
+template <int D>
+struct Fssh2Hot {
+    int32_t method, rescaling, nsub, save_every;
+    int32_t cell_mode, term_kind, term_dof, pad;
+    int64_t nsteps;
+    double dt, term_value;
+    uint64_t key;
+    double inv_mass[D];
+};
+
+__device__ __forceinline__ int32_t uniform_sgpr(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uniform_sgpr(uint64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int64_t uniform_sgpr(int64_t x) { return (int64_t)uniform_sgpr((uint64_t)x); }
+__device__ __forceinline__ double uniform_sgpr(double x) {
+    return __longlong_as_double((long long)uniform_sgpr((uint64_t)__double_as_longlong(x)));
+}
+
+template <int D>
+__device__ __forceinline__ Fssh2Hot<D> fssh2_hot(const Fssh2Args& A) {
+    Fssh2Hot<D> H;
+    H.method = uniform_sgpr(A.method); H.rescaling = uniform_sgpr(A.rescaling);
+    H.nsub = uniform_sgpr(A.nsub); H.save_every = uniform_sgpr(A.save_every);
+    H.cell_mode = uniform_sgpr(A.cell.mode); H.term_kind = uniform_sgpr(A.term_kind);
+    H.term_dof = uniform_sgpr(A.term_dof); H.pad = 0;
+    H.nsteps = uniform_sgpr(A.nsteps);
+    H.dt = uniform_sgpr(A.dt); H.term_value = uniform_sgpr(A.term_value);
+    H.key = uniform_sgpr(A.key);
+#pragma unroll
+    for (int I = 0; I < D; ++I) H.inv_mass[I] = uniform_sgpr(A.inv_mass[I]);
+    return H;
+}
 
 template <int D, typename R>
 __device__ __forceinline__ void eval_cache(Lane2<D, R>& L, bool track, double& E0, double& E1, double G00[D],
@@ -59,15 +99,15 @@ __device__ __forceinline__ void eval_cache(Lane2<D, R>& L, bool track, double& E
 }
 
 template <int D, typename R>
-__device__ __forceinline__ void set_accel(const Fssh2Args& A, Lane2<D, R>& L, const double G00[D],
+__device__ __forceinline__ void set_accel(const Fssh2Hot<D>& H, Lane2<D, R>& L, const double G00[D],
                                           const double G11[D], const double G01[D]) {
     const double a = 0.5 * (L.tr + L.s.q), b = 0.5 * (L.tr - L.s.q);
 #pragma unroll
     for (int I = 0; I < D; ++I) {
         double f;
-        if (A.method == METHOD_FSSH) f = (L.st == 0) ? G00[I] : G11[I];
+        if (H.method == METHOD_FSSH) f = (L.st == 0) ? G00[I] : G11[I];
         else f = G00[I] * a + G11[I] * b + 2.0 * G01[I] * L.s.x;
-        L.k[I] = (R)(-f * A.inv_mass[I]);
+        L.k[I] = (R)(-f * H.inv_mass[I]);
     }
 }
 
@@ -106,7 +146,7 @@ __device__ __forceinline__ void save_slot(const Fssh2Args* Ap, const Lane2<D, R>
 }
 
 template <int D, typename R>
-__device__ __forceinline__ void load_traj(const Fssh2Args* Ap, Lane2<D, R>& L, int64_t tid) {
+__device__ __forceinline__ void load_traj(const Fssh2Args* Ap, const Fssh2Hot<D>& H, Lane2<D, R>& L, int64_t tid) {
     const Fssh2Args& A = *opaque(Ap);
     const int64_t n = A.ntraj;
     L.tid = tid;
@@ -121,7 +161,7 @@ __device__ __forceinline__ void load_traj(const Fssh2Args* Ap, Lane2<D, R>& L, i
     L.s.x = A.sig0[2 * n + tid];
     L.s.y = A.sig0[3 * n + tid];
     const int mid = A.model_id ? A.model_id[tid] : 0;
-    L.m = model_regs(A.models, mid);
+    L.m = model_lane<R>(model_regs(A.models, mid));
     int st = A.state0 ? A.state0[tid] : -1;
     if (A.method == METHOD_FSSH && st < 0) {  // fssh.jl:54: sample(Weights(diag(sigma)))
         const double u = philox_uniform((uint64_t)tid, 0u, STREAM_STATE, A.key) * (a + b);
@@ -130,7 +170,7 @@ __device__ __forceinline__ void load_traj(const Fssh2Args* Ap, Lane2<D, R>& L, i
     L.st = st;
     double E0, E1, G00[D], G11[D], G01[D];
     eval_cache(L, false, E0, E1, G00, G11, G01);  // initialize! (bab_electronics.jl:48-59)
-    set_accel(A, L, G00, G11, G01);
+    set_accel(H, L, G00, G11, G01);
     L.dEc = 0.0; L.wc = 0.0; L.tc = 0.0;  // zero buffers stamped t = 0 (electronic_dynamics.jl:118-127)
     L.t = A.t0;
     L.nh = 0;
@@ -166,7 +206,7 @@ __device__ __forceinline__ void resume_traj(const Fssh2Args* Ap, const SuspSet& 
     L.tid = S.tid[slot];
     L.st = S.i[0 * c + slot]; L.nh = S.i[1 * c + slot]; L.step = S.i[2 * c + slot];
     const int mid = A.model_id ? A.model_id[L.tid] : 0;
-    L.m = model_regs(A.models, mid);
+    L.m = model_lane<R>(model_regs(A.models, mid));
 }
 
 template <int D, typename R>
@@ -195,22 +235,23 @@ __device__ __forceinline__ void finish_traj(const Fssh2Args* Ap, const Lane2<D, 
 }
 
 template <int D, typename R>
-__device__ __forceinline__ bool is_done(const Fssh2Args& A, const Lane2<D, R>& L) {
-    if (L.step >= A.nsteps) return true;
-    if (A.cell.mode == 2 && cell_outside<1, D, R>(A.cell, L.r)) return true;  // TerminateCellCallback
-    if (A.term_kind != 0) {
+__device__ __forceinline__ bool is_done(const Fssh2Args& A, const Fssh2Hot<D>& H, const Lane2<D, R>& L) {
+    if (L.step >= H.nsteps) return true;
+    if (H.cell_mode == 2 && cell_outside<1, D, R>(A.cell, L.r)) return true;  // TerminateCellCallback
+    if (H.term_kind != 0) {
         double x = 0.0;  // select, not a runtime index, so the lane state stays in registers
 #pragma unroll
-        for (int I = 0; I < D; ++I) x = (I == A.term_dof) ? (double)L.r[I] : x;
-        if (term_hit(A.term_kind, x, A.term_value)) return true;
+        for (int I = 0; I < D; ++I) x = (I == H.term_dof) ? (double)L.r[I] : x;
+        if (term_hit(H.term_kind, x, H.term_value)) return true;
     }
     return false;
 }
 
 // One nuclear step + hopping callback.
 template <int D, typename R>
-__device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& A, Lane2<D, R>& L) {
-    const R hdt = (R)(0.5 * A.dt), dtR = (R)A.dt;
+__device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& A, const Fssh2Hot<D>& H,
+                                          Lane2<D, R>& L) {
+    const R hdt = (R)(0.5 * H.dt), dtR = (R)H.dt;
     R vt[D];
 #pragma unroll
     for (int I = 0; I < D; ++I) vt[I] = L.v[I] + hdt * L.k[I];           // step_B!
@@ -218,7 +259,7 @@ __device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& 
     for (int I = 0; I < D; ++I) L.r[I] = L.r[I] + dtR * vt[I];           // step_A!
     double E0, E1, G00[D], G11[D], G01[D];
     eval_cache(L, true, E0, E1, G00, G11, G01);                          // update_cache!
-    set_accel(A, L, G00, G11, G01);                                      // sigma of the step start
+    set_accel(H, L, G00, G11, G01);                                      // sigma of the step start
 #pragma unroll
     for (int I = 0; I < D; ++I) L.v[I] = vt[I] + hdt * L.k[I];           // step_B!
     const double gap = E1 - E0;
@@ -227,18 +268,18 @@ __device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& 
     double w = 0.0;
 #pragma unroll
     for (int I = 0; I < D; ++I) { d01[I] = G01[I] * inv_gap; w += d01[I] * (double)L.v[I]; }
-    const double tn = L.t + A.dt;
+    const double tn = L.t + H.dt;
     const double dEn = E0 - E1;
     const double inv = 1.0 / (tn - L.tc);
     Interp2 P;
     P.dE0 = L.dEc; P.dE1 = (dEn - L.dEc) * inv; P.w0 = L.wc; P.w1 = (w - L.wc) * inv;
-    propagate2(P, L.s, L.t, L.tc, A.dt, A.nsub);
+    propagate2(P, L.s, L.t, L.tc, H.dt, H.nsub);
     L.dEc = dEn; L.wc = w; L.tc = tn;
     L.t = tn;
     // A save time's values are those before the step's discrete callbacks (DiffEqBase's
     // apply_discrete_callback! saves the due saveat points before affect!): the hop comes after the save.
-    if (A.save_every > 0 && ((L.step + 1) % A.save_every) == 0) save_slot(Ap, L, (L.step + 1) / A.save_every);
-    if (A.method == METHOD_FSSH) {
+    if (H.save_every > 0 && ((L.step + 1) % H.save_every) == 0) save_slot(Ap, L, (L.step + 1) / H.save_every);
+    if (H.method == METHOD_FSSH) {
         const int s = L.st;
         const double sss = 0.5 * (L.tr + ((s == 0) ? L.s.q : -L.s.q));
         const double ratio = L.s.x / sss;  // Re(sigma_ms / sigma_ss)
@@ -246,29 +287,30 @@ __device__ __forceinline__ void step_traj(const Fssh2Args* Ap, const Fssh2Args& 
 #pragma unroll
         for (int I = 0; I < D; ++I) {
             const double dsm = (s == 0) ? d01[I] : -d01[I];
-            p += 2.0 * (double)L.v[I] * ratio * dsm * A.dt;
+            p += 2.0 * (double)L.v[I] * ratio * dsm * H.dt;
         }
         if (p > 0.0) {  // a uniform in [0,1) can only fall below a positive probability
             p = fmin(p, 1.0);
-            const double rnd = philox_uniform((uint64_t)L.tid, (uint32_t)L.step, STREAM_HOP, A.key);
+            const double rnd = philox_uniform((uint64_t)L.tid, (uint32_t)L.step, STREAM_HOP, H.key);
             if (p > rnd) {
                 double dms[D];
 #pragma unroll
                 for (int I = 0; I < D; ++I) dms[I] = (s == 0) ? -d01[I] : d01[I];  // coupling[new, old]
                 const double dE = (s == 0) ? gap : -gap;
-                if (rescale_velocity<D, R>(A.rescaling, L.v, A.inv_mass, dms, dE)) { L.st = 1 - s; ++L.nh; }
+                if (rescale_velocity<D, R>(H.rescaling, L.v, H.inv_mass, dms, dE)) { L.st = 1 - s; ++L.nh; }
             }
         }
     }
     // CellBoundaryCallback after the hop callback (the problem's callbacks precede the user's); the
     // acceleration of the step end is kept, as BABwithTsit5's cache keeps it (bab_electronics.jl:61-74)
-    if (A.cell.mode == 1 && cell_outside<1, D, R>(A.cell, L.r)) cell_wrap<1, D, R>(A.cell, L.r);
+    if (H.cell_mode == 1 && cell_outside<1, D, R>(A.cell, L.r)) cell_wrap<1, D, R>(A.cell, L.r);
     ++L.step;
 }
 
 template <int D, typename R>
 __device__ __forceinline__ void persistent_body(const Fssh2Args* __restrict__ Ap) {
     const Fssh2Args& A = *Ap;
+    const Fssh2Hot<D> H = fssh2_hot<D>(A);
     const int lane = __lane_id();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     Lane2<D, R> L;
@@ -297,10 +339,10 @@ __device__ __forceinline__ void persistent_body(const Fssh2Args* __restrict__ Ap
                         const int64_t span = opaque(Ap)->prio_span;
                         lvl = span > 0 ? 3 - (int)min((unsigned long long)3, q / (unsigned long long)span) : 0;
                         const int32_t* order = opaque(Ap)->order;
-                        load_traj(Ap, L, order ? (int64_t)order[q] : (int64_t)q);
+                        load_traj(Ap, H, L, order ? (int64_t)order[q] : (int64_t)q);
                         // the termination predicate is a DiscreteCallback, evaluated after each step and
                         // not at t0 (callbacks.jl:15-29): a fresh trajectory is done only for zero steps
-                        if (A.nsteps <= 0) finish_traj(Ap, L); else active = true;
+                        if (H.nsteps <= 0) finish_traj(Ap, L); else active = true;
                     }
                 } else {
                     exhausted = true;
@@ -320,8 +362,8 @@ __device__ __forceinline__ void persistent_body(const Fssh2Args* __restrict__ Ap
             }
         }
         if (active) {
-            step_traj(Ap, A, L);
-            if (is_done(A, L)) { finish_traj(Ap, L); active = false; }
+            step_traj(Ap, A, H, L);
+            if (is_done(A, H, L)) { finish_traj(Ap, L); active = false; }
         }
         // Once the queue has drained, a wave with few running lanes sets them aside for the next launch,
         // which packs them into full waves (instead of issuing for its idle lanes to the end).

@@ -427,6 +427,162 @@ __global__ __launch_bounds__(kMlpBlock) void fssh_nd_mlp_f32(const FsshNdArgs* _
 }
 __global__ void nd_queue_reset(unsigned long long* q) { *q = 0ull; }
 
+// ---------------------------------------------------------------------------
+// Plain schedule for the diatom-surface model: the straightforward port, one launch per nuclear step,
+// the state round-trips through HBM every step, the electronic step uses the dense complex 2x2
+// formulation (propagate_dense), FP64 throughout, terminated trajectories are masked.  Analytic
+// model only.  State rows (double): r[6], v[6], k[6], sigma re[4], im[4], E0c, E1c, Wc, tc, U[4], t;
+// int rows: state, nhops, step, done.
+// ---------------------------------------------------------------------------
+enum { NP_R = 0, NP_V = 6, NP_K = 12, NP_SRE = 18, NP_SIM = 22, NP_E0C = 26, NP_E1C = 27, NP_WC = 28, NP_TC = 29,
+       NP_U = 30, NP_T = 34, NP_NROWS = 35 };
+enum { NPI_ST = 0, NPI_NH = 1, NPI_STEP = 2, NPI_DONE = 3, NPI_NROWS = 4 };
+
+__device__ __forceinline__ void nd_plain_save(const FsshNdArgs& A, int64_t tid, int slot, int st, const C2& s,
+                                              const double U[4], const double r[kNdDof]) {
+    if (slot >= A.nsave) return;
+    const int64_t n = A.ntraj;
+    double pa0, pa1;
+    if (A.method == METHOD_FSSH) { pa0 = (st == 0) ? 1.0 : 0.0; pa1 = 1.0 - pa0; }
+    else { pa0 = s.re[0]; pa1 = s.re[3]; }
+    const double S01 = s.re[1];
+    A.save_pop_ad[((int64_t)slot * 2 + 0) * n + tid] = pa0;
+    A.save_pop_ad[((int64_t)slot * 2 + 1) * n + tid] = pa1;
+    A.save_pop_di[((int64_t)slot * 2 + 0) * n + tid] = U[0] * U[0] * pa0 + U[1] * U[1] * pa1 + 2.0 * U[0] * U[1] * S01;
+    A.save_pop_di[((int64_t)slot * 2 + 1) * n + tid] = U[2] * U[2] * pa0 + U[3] * U[3] * pa1 + 2.0 * U[2] * U[3] * S01;
+    A.save_state[(int64_t)slot * n + tid] = st;
+    if (A.save_r) {
+        for (int I = 0; I < kNdDof; ++I) A.save_r[((int64_t)slot * kNdDof + I) * n + tid] = r[I];
+    }
+}
+
+// Eigen-decomposition at r, U^T dV U per coordinate and the acceleration on the active surface or
+// the mean field of the dense density.
+__device__ __forceinline__ void nd_plain_forces(const FsshNdArgs& A, const double r[kNdDof], int st, const C2& s,
+                                                bool track, double U[4], double& E0, double& E1, double k[kNdDof],
+                                                double G01[kNdDof]) {
+    double v11, v22, v12, d11[kNdDof], d22[kNdDof], d12[kNdDof];
+    diatom_eval<double>(A.p, r, v11, v22, v12, d11, d22, d12);
+    eigen2(v11, v12, v22, E0, E1, U, track);
+    for (int I = 0; I < kNdDof; ++I) {
+        double G00, G11;
+        rotate_derivative(U, d11[I], d22[I], d12[I], G00, G11, G01[I]);
+        const double f = (A.method == METHOD_FSSH) ? ((st == 0) ? G00 : G11)
+                                                   : (G00 * s.re[0] + G11 * s.re[3] + 2.0 * G01[I] * s.re[1]);
+        k[I] = -f * A.inv_mass[I];
+    }
+}
+
+__global__ __launch_bounds__(256) void nd_plain_init(const FsshNdArgs A, double* S, int32_t* Si) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= A.ntraj) return;
+    const int64_t n = A.ntraj;
+    double r[kNdDof], v[kNdDof], k[kNdDof], G01[kNdDof], U[4] = {1.0, 0.0, 0.0, 1.0}, E0, E1;
+    for (int I = 0; I < kNdDof; ++I) { r[I] = A.r0[(int64_t)I * n + tid]; v[I] = A.v0[(int64_t)I * n + tid]; }
+    C2 s;
+    s.re[0] = A.sig0[tid]; s.re[3] = A.sig0[n + tid]; s.re[1] = s.re[2] = A.sig0[2 * n + tid];
+    s.im[0] = s.im[3] = 0.0; s.im[1] = A.sig0[3 * n + tid]; s.im[2] = -A.sig0[3 * n + tid];
+    int st = A.state0 ? A.state0[tid] : -1;
+    if (A.method == METHOD_FSSH && st < 0) {  // fssh.jl:54
+        const double tot = s.re[0] + s.re[3];
+        st = (philox_uniform((uint64_t)tid, 0u, STREAM_STATE, A.key) * tot < s.re[0]) ? 0 : 1;
+    }
+    nd_plain_forces(A, r, st, s, false, U, E0, E1, k, G01);
+    for (int I = 0; I < kNdDof; ++I) {
+        S[(NP_R + I) * n + tid] = r[I]; S[(NP_V + I) * n + tid] = v[I]; S[(NP_K + I) * n + tid] = k[I];
+    }
+    for (int J = 0; J < 4; ++J) { S[(NP_SRE + J) * n + tid] = s.re[J]; S[(NP_SIM + J) * n + tid] = s.im[J]; }
+    S[NP_E0C * n + tid] = 0.0; S[NP_E1C * n + tid] = 0.0; S[NP_WC * n + tid] = 0.0; S[NP_TC * n + tid] = 0.0;
+    for (int J = 0; J < 4; ++J) S[(NP_U + J) * n + tid] = U[J];
+    S[NP_T * n + tid] = A.t0;
+    Si[NPI_ST * n + tid] = st; Si[NPI_NH * n + tid] = 0; Si[NPI_STEP * n + tid] = 0;
+    Si[NPI_DONE * n + tid] = A.nsteps <= 0 ? 1 : 0;  // the predicates are first evaluated after step 1
+    nd_plain_save(A, tid, 0, st, s, U, r);
+}
+
+__global__ __launch_bounds__(256) void nd_plain_step(const FsshNdArgs A, double* S, int32_t* Si) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= A.ntraj) return;
+    const int64_t n = A.ntraj;
+    if (Si[NPI_DONE * n + tid]) return;
+    double r[kNdDof], v[kNdDof], k[kNdDof], G01[kNdDof], U[4], E0, E1;
+    for (int I = 0; I < kNdDof; ++I) {
+        r[I] = S[(NP_R + I) * n + tid]; v[I] = S[(NP_V + I) * n + tid]; k[I] = S[(NP_K + I) * n + tid];
+    }
+    C2 s;
+    for (int J = 0; J < 4; ++J) { s.re[J] = S[(NP_SRE + J) * n + tid]; s.im[J] = S[(NP_SIM + J) * n + tid]; }
+    for (int J = 0; J < 4; ++J) U[J] = S[(NP_U + J) * n + tid];
+    const double t = S[NP_T * n + tid];
+    int st = Si[NPI_ST * n + tid], nh = Si[NPI_NH * n + tid], step = Si[NPI_STEP * n + tid];
+    const double dt = A.dt;
+    double vt[kNdDof];
+    for (int I = 0; I < kNdDof; ++I) { vt[I] = v[I] + 0.5 * dt * k[I]; r[I] = r[I] + dt * vt[I]; }  // step_B!, step_A!
+    nd_plain_forces(A, r, st, s, true, U, E0, E1, k, G01);
+    for (int I = 0; I < kNdDof; ++I) v[I] = vt[I] + 0.5 * dt * k[I];                            // step_B!
+    const double inv_gap = 1.0 / (E1 - E0);
+    double d01[kNdDof], w = 0.0;
+    for (int I = 0; I < kNdDof; ++I) { d01[I] = G01[I] * inv_gap; w += d01[I] * v[I]; }
+    const double tn = t + dt;
+    InterpDense P;
+    P.E0c = S[NP_E0C * n + tid]; P.E1c = S[NP_E1C * n + tid]; P.Wc = S[NP_WC * n + tid]; P.tc = S[NP_TC * n + tid];
+    P.E0n = E0; P.E1n = E1; P.Wn = w; P.inv = 1.0 / (tn - P.tc);
+    propagate_dense(P, s, t, dt, A.nsub);
+    if (A.save_every > 0 && (step + 1) % A.save_every == 0)  // before the hop, as the persistent kernel
+        nd_plain_save(A, tid, (step + 1) / A.save_every, st, s, U, r);
+    if (A.method == METHOD_FSSH) {
+        const int m = 1 - st;
+        // real(sigma[m,s] / sigma[s,s]) with full complex division (fssh.jl:96-121)
+        const double nr = s.re[m * 2 + st], ni = s.im[m * 2 + st];
+        const double dr = s.re[st * 3], di = s.im[st * 3];
+        const double ratio = (nr * dr + ni * di) / (dr * dr + di * di);
+        double p = 0.0;
+        for (int I = 0; I < kNdDof; ++I) {
+            const double dsm = (st == 0) ? d01[I] : -d01[I];
+            p += 2.0 * v[I] * ratio * dsm * dt;
+        }
+        p = fmin(fmax(p, 0.0), 1.0);
+        const double rnd = philox_uniform((uint64_t)tid, (uint32_t)step, STREAM_HOP, A.key);
+        if (p > rnd) {
+            double dms[kNdDof];
+            for (int I = 0; I < kNdDof; ++I) dms[I] = (st == 0) ? -d01[I] : d01[I];
+            const double dE = (st == 0) ? (E1 - E0) : (E0 - E1);
+            if (rescale_velocity<kNdDof, double>(A.rescaling, v, A.inv_mass, dms, dE)) { st = m; ++nh; }
+        }
+    }
+    // CellBoundaryCallback after the hop callback, the step-end acceleration kept
+    if (A.cell.mode == 1 && cell_outside<3, 2, double>(A.cell, r)) cell_wrap<3, 2, double>(A.cell, r);
+    ++step;
+    for (int I = 0; I < kNdDof; ++I) {
+        S[(NP_R + I) * n + tid] = r[I]; S[(NP_V + I) * n + tid] = v[I]; S[(NP_K + I) * n + tid] = k[I];
+    }
+    for (int J = 0; J < 4; ++J) { S[(NP_SRE + J) * n + tid] = s.re[J]; S[(NP_SIM + J) * n + tid] = s.im[J]; }
+    S[NP_E0C * n + tid] = E0; S[NP_E1C * n + tid] = E1; S[NP_WC * n + tid] = w; S[NP_TC * n + tid] = tn;
+    for (int J = 0; J < 4; ++J) S[(NP_U + J) * n + tid] = U[J];
+    S[NP_T * n + tid] = tn;
+    Si[NPI_ST * n + tid] = st; Si[NPI_NH * n + tid] = nh; Si[NPI_STEP * n + tid] = step;
+    bool done = step >= A.nsteps;
+    if (A.cell.mode == 2 && cell_outside<3, 2, double>(A.cell, r)) done = true;  // TerminateCellCallback
+    if (A.term_kind != 0 && term_hit(A.term_kind, r[A.term_dof], A.term_value)) done = true;
+    if (done) Si[NPI_DONE * n + tid] = 1;
+}
+
+__global__ __launch_bounds__(256) void nd_plain_finish(const FsshNdArgs A, const double* S, const int32_t* Si) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= A.ntraj) return;
+    const int64_t n = A.ntraj;
+    for (int I = 0; I < kNdDof; ++I) {
+        A.r[(int64_t)I * n + tid] = S[(NP_R + I) * n + tid];
+        A.v[(int64_t)I * n + tid] = S[(NP_V + I) * n + tid];
+    }
+    A.sig[tid] = S[NP_SRE * n + tid];
+    A.sig[n + tid] = S[(NP_SRE + 3) * n + tid];
+    A.sig[2 * n + tid] = S[(NP_SRE + 1) * n + tid];
+    A.sig[3 * n + tid] = S[(NP_SIM + 1) * n + tid];
+    A.state[tid] = Si[NPI_ST * n + tid];
+    A.nhops[tid] = Si[NPI_NH * n + tid];
+    A.steps[tid] = Si[NPI_STEP * n + tid];
+}
+
 }  // namespace
 
 hipError_t nd_persistent_occupancy(int precision, int block, int* blocks_per_cu) {
@@ -513,6 +669,47 @@ int32_t nd_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
         A.nrm = h->mlp_norm;
         A.threshold = (float)h->mlp_threshold;
         A.fallbacks = h->fallbacks.as<int32_t>();
+    }
+    if (h->prob.schedule == NQCD_SCHED_STEP_PER_LAUNCH) {
+        if (mlp) return fail(h, NQCD_ERR_UNSUPPORTED, "the step-per-launch schedule runs the analytic model");
+        if (h->prob.precision != NQCD_PREC_FP64)
+            return fail(h, NQCD_ERR_UNSUPPORTED, "the step-per-launch schedule runs in FP64 only");
+        HIPCHK(h, h->plainS.ensure(sizeof(double) * NP_NROWS * n));
+        HIPCHK(h, h->plainSi.ensure(sizeof(int32_t) * NPI_NROWS * n));
+        double* S = h->plainS.as<double>();
+        int32_t* Si = h->plainSi.as<int32_t>();
+        const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+        HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+        hipLaunchKernelGGL(nd_plain_init, grid, block, 0, h->stream, A, S, Si);
+        HIPCHK(h, hipGetLastError());
+        // identical step launches captured once into a graph and replayed
+        const int64_t chunk = 64;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (nsteps >= chunk) {
+            HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            for (int64_t j = 0; j < chunk; ++j) hipLaunchKernelGGL(nd_plain_step, grid, block, 0, h->stream, A, S, Si);
+            HIPCHK(h, hipStreamEndCapture(h->stream, &graph));
+            HIPCHK(h, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        int64_t done = 0, launches = 1;
+        for (; done + chunk <= nsteps && exec; done += chunk) {
+            HIPCHK(h, hipGraphLaunch(exec, h->stream));
+            launches += chunk;
+        }
+        for (; done < nsteps; ++done, ++launches) {
+            hipLaunchKernelGGL(nd_plain_step, grid, block, 0, h->stream, A, S, Si);
+            HIPCHK(h, hipGetLastError());
+        }
+        hipLaunchKernelGGL(nd_plain_finish, grid, block, 0, h->stream, A, (const double*)S, (const int32_t*)Si);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        h->stats.block = 256;
+        h->stats.grid = (int)grid.x;
+        h->stats.launches = launches + 1;
+        return NQCD_OK;
     }
     const bool f64 = h->prob.precision == NQCD_PREC_FP64;
     const void* kern = mlp ? (const void*)fssh_nd_mlp_f32 : (f64 ? (const void*)fssh_nd_f64 : (const void*)fssh_nd_f32);

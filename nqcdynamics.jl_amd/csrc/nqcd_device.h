@@ -195,8 +195,8 @@ __device__ __forceinline__ ModelRegs model_regs(const ModelSlot& s) {
 
 // One model kind, selected at compile time (the per-kind bodies of model1d_eval, for callers that
 // evaluate many points under one dispatch).
-template <int K, typename R>
-__device__ __forceinline__ void model1d_kind(const ModelRegs& m, R x, R& v11, R& v22, R& v12,
+template <int K, typename R, class M>
+__device__ __forceinline__ void model1d_kind(const M& m, R x, R& v11, R& v22, R& v12,
                                              R& d11, R& d22, R& d12) {
     if constexpr (K == MODEL_TULLY1) {
         const R A = (R)m.p[0], B = (R)m.p[1], C = (R)m.p[2], D = (R)m.p[3];
@@ -241,8 +241,8 @@ __device__ __forceinline__ void model1d_kind(const ModelRegs& m, R x, R& v11, R&
 
 // Calls f(std::integral_constant<int, K>{}) for the model kind K of m (one dispatch for a whole
 // block of evaluations); unknown kinds take K = -1, whose model1d_kind is zero.
-template <class F>
-__device__ __forceinline__ void model1d_dispatch(const ModelRegs& m, F&& f) {
+template <class M, class F>
+__device__ __forceinline__ void model1d_dispatch(const M& m, F&& f) {
     switch (m.kind) {
     case MODEL_TULLY1: f(std::integral_constant<int, MODEL_TULLY1>{}); break;
     case MODEL_TULLY2: f(std::integral_constant<int, MODEL_TULLY2>{}); break;
@@ -253,10 +253,28 @@ __device__ __forceinline__ void model1d_dispatch(const ModelRegs& m, F&& f) {
     }
 }
 
-template <typename R>
-__device__ __forceinline__ void model1d_eval(const ModelRegs& m, R x, R& v11, R& v22, R& v12,
+// m: a ModelRegs, or a ModelLane (below) whose parameters are already in the working precision
+template <typename R, class M>
+__device__ __forceinline__ void model1d_eval(const M& m, R x, R& v11, R& v22, R& v12,
                                              R& d11, R& d22, R& d12) {
     model1d_dispatch(m, [&](auto k) { model1d_kind<decltype(k)::value, R>(m, x, v11, v22, v12, d11, d22, d12); });
+}
+
+// A model's kind and parameters in the working precision (converted once, where a kernel keeps them
+// per lane).
+template <typename R>
+struct ModelLane {
+    int32_t kind;
+    R p[5];
+};
+
+template <typename R>
+__device__ __forceinline__ ModelLane<R> model_lane(const ModelRegs& m) {
+    ModelLane<R> L;
+    L.kind = m.kind;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) L.p[i] = (R)m.p[i];
+    return L;
 }
 
 // ---------------------------------------------------------------------------

@@ -160,8 +160,9 @@ def test_cme_terminate_cell(oracle, key):
     assert np.allclose(g["R"][:, 0], c["x"], rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("schedule", [0, 1])
 @pytest.mark.parametrize("mode", [1, 2])
-def test_diatom_cell_callbacks_match_oracle(oracle, key, mode):
+def test_diatom_cell_callbacks_match_oracle(oracle, key, mode, schedule):
     """Two atoms of three coordinates in a hexagonal cell (docs/src/initialconditions/ebk.md:47), periodic in
     x and y; the molecule starts near a cell edge so atoms cross it."""
     from nqcd_mi355x import _lib as L
@@ -170,9 +171,9 @@ def test_diatom_cell_callbacks_match_oracle(oracle, key, mode):
     r0[:, 0] += 0.3; r0[:, 3] += 0.3
     r0[:, 1] += 2.5; r0[:, 4] += 2.5
     nsteps = 1200
-    print(f"diatom cell callbacks: mode {mode}", flush=True)
+    print(f"diatom cell callbacks: mode {mode} schedule {schedule}", flush=True)
     E = L.Ensemble(0)
-    E.set_problem(L.METHOD_FSSH, 2, 3, 2, [MASS, MASS], 2.0)
+    E.set_problem(L.METHOD_FSSH, 2, 3, 2, [MASS, MASS], 2.0, schedule=schedule)
     E.set_model(0, L.MODEL_DIATOM_SURFACE, BASE)
     E.set_cell(mode, HEX, [1, 1, 0])
     sig = np.zeros((len(r0), 2, 2), complex); sig[:, 0, 0] = 1

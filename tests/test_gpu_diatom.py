@@ -1,6 +1,7 @@
 """GPU two-state FSSH / Ehrenfest on the six-coordinate diatomic-on-surface model against the
-FP64 oracle (identical hop sequences, R/P/sigma within 1e-9 over the window), and the FP32-nuclei
-path's final populations against the FP64 path."""
+FP64 oracle (identical hop sequences, R/P/sigma within 1e-9 over the window) on both schedules (the
+persistent kernel and the step-per-launch plain port), and the FP32-nuclei path's final populations
+against the FP64 path."""
 import numpy as np
 import pytest
 
@@ -21,10 +22,10 @@ def _inputs(ntraj, seed, Z=6.0, vz=3e-3):
     return r0, v0
 
 
-def _gpu(L, method, r0, v0, nsteps, key, precision=0, save_every=50, term=(2, 8.0)):
+def _gpu(L, method, r0, v0, nsteps, key, precision=0, save_every=50, term=(2, 8.0), schedule=0):
     ntraj = len(r0)
     E = L.Ensemble(0)
-    E.set_problem(method, 2, 3, 2, [MASS, MASS], 2.0, precision=precision)
+    E.set_problem(method, 2, 3, 2, [MASS, MASS], 2.0, precision=precision, schedule=schedule)
     E.set_model(0, L.MODEL_DIATOM_SURFACE, BASE)
     sig = np.zeros((ntraj, 2, 2), complex); sig[:, 0, 0] = 1
     E.load_initial(r0, v0, sig, state=np.zeros(ntraj, np.int32))
@@ -47,12 +48,14 @@ def _cpu(oracle, method, r0, v0, nsteps, key, save_every=50):
                                term_x=8.0, term_dof=2)
 
 
+@pytest.mark.parametrize("schedule", [0, 1])
 @pytest.mark.parametrize("method", [0, 1])
-def test_diatom_fp64_parity(oracle, key, method):
+def test_diatom_fp64_parity(oracle, key, method, schedule):
     from nqcd_mi355x import _lib as L
     r0, v0 = _inputs(256, 31)
     nsteps = 1500
-    g = _gpu(L, method, r0, v0, nsteps, key)
+    print(f"diatom parity: method {method} schedule {schedule}", flush=True)
+    g = _gpu(L, method, r0, v0, nsteps, key, schedule=schedule)
     c = _cpu(oracle, method, r0, v0, nsteps, key)
     assert np.array_equal(g["steps"], c["steps"])
     if method == 0:
@@ -95,3 +98,12 @@ def test_diatom_autotune_keeps_trajectories(key):
     assert tuned["stats"]["block"] == res["block"]
     for k in ("r", "v", "state", "steps", "nhops", "save_state"):
         assert np.array_equal(tuned[k], ref[k]), k
+
+
+def test_diatom_plain_schedule_rejects_fp32_and_surrogate(key):
+    """The step-per-launch port runs the analytic model in FP64 only: FP32 nuclei are refused with a
+    clear error before any launch."""
+    from nqcd_mi355x import _lib as L
+    r0, v0 = _inputs(64, 35)
+    with pytest.raises(L.NQCDError, match="FP64"):
+        _gpu(L, 0, r0, v0, 10, key, precision=1, schedule=1)
