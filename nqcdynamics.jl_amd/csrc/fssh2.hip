@@ -62,17 +62,6 @@ struct Fssh2Hot {
     double inv_mass[D];
 };
 
-__device__ __forceinline__ int32_t uniform_sgpr(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ uint64_t uniform_sgpr(uint64_t x) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ int64_t uniform_sgpr(int64_t x) { return (int64_t)uniform_sgpr((uint64_t)x); }
-__device__ __forceinline__ double uniform_sgpr(double x) {
-    return __longlong_as_double((long long)uniform_sgpr((uint64_t)__double_as_longlong(x)));
-}
-
 template <int D>
 __device__ __forceinline__ Fssh2Hot<D> fssh2_hot(const Fssh2Args& A) {
     Fssh2Hot<D> H;

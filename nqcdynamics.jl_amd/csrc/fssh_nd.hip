@@ -52,6 +52,47 @@ __device__ __forceinline__ const T* nd_opaque(const T* p) {
     return p;
 }
 
+// The launch arguments the step reads, in scalar registers for the whole persistent loop (read per step
+// through the argument pointer, each is a vector-memory round trip with a full wait).
+struct NdHot {
+    int32_t method, rescaling, nsub, save_every;
+    int32_t cell_mode, term_kind, term_dof, pad;
+    int64_t nsteps;
+    double dt, term_value;
+    uint64_t key;
+    double inv_mass[kNdDof];
+};
+
+__device__ __forceinline__ NdHot nd_hot(const FsshNdArgs& A) {
+    NdHot H;
+    H.method = uniform_sgpr(A.method); H.rescaling = uniform_sgpr(A.rescaling);
+    H.nsub = uniform_sgpr(A.nsub); H.save_every = uniform_sgpr(A.save_every);
+    H.cell_mode = uniform_sgpr(A.cell.mode); H.term_kind = uniform_sgpr(A.term_kind);
+    H.term_dof = uniform_sgpr(A.term_dof); H.pad = 0;
+    H.nsteps = uniform_sgpr(A.nsteps);
+    H.dt = uniform_sgpr(A.dt); H.term_value = uniform_sgpr(A.term_value);
+    H.key = uniform_sgpr(A.key);
+#pragma unroll
+    for (int I = 0; I < kNdDof; ++I) H.inv_mass[I] = uniform_sgpr(A.inv_mass[I]);
+    return H;
+}
+
+// The model parameters through a constant-address-space pointer in scalar registers, made opaque where
+// it is used so the sixteen parameters are scalar loads at the model evaluation (they do not change
+// during a launch) instead of values held across the loop.
+using NdParams = __attribute__((address_space(4))) const double*;
+
+__device__ __forceinline__ NdParams nd_params(const FsshNdArgs* Ap) {
+    const uint64_t a = (uint64_t)(&Ap->p[0]);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a), hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return (NdParams)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ NdParams nd_params_here(NdParams p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 template <typename R>
 __device__ __forceinline__ void morse(R D, R a, R q0, R x, R& f, R& df) {
     const R e = fexp(-a * (x - q0));
@@ -59,9 +100,10 @@ __device__ __forceinline__ void morse(R D, R a, R q0, R x, R& f, R& df) {
     df = (R)2 * D * a * e * ((R)1 - e);
 }
 
-// Diabatic matrix and its gradient with respect to the six coordinates.
-template <typename R>
-__device__ __forceinline__ void diatom_eval(const double* P, const R r[kNdDof], R& v11, R& v22, R& v12,
+// Diabatic matrix and its gradient with respect to the six coordinates (P: the 16 parameters, through a
+// generic or a constant-address-space pointer).
+template <typename R, class PT>
+__device__ __forceinline__ void diatom_eval(PT P, const R r[kNdDof], R& v11, R& v22, R& v12,
                                             R d11[kNdDof], R d22[kNdDof], R d12[kNdDof]) {
     const R Dn = (R)P[0], an = (R)P[1], bn = (R)P[2], Aw = (R)P[3], bw = (R)P[4], Di = (R)P[5], ai = (R)P[6];
     const R bi = (R)P[7], Ds = (R)P[8], as = (R)P[9], Zs = (R)P[10], Delta = (R)P[11], C = (R)P[12], g = (R)P[13];
@@ -255,21 +297,21 @@ __device__ __forceinline__ void nd_finish(const FsshNdArgs* Ap, const LaneN<R>& 
 }
 
 template <typename R>
-__device__ __forceinline__ bool nd_done(const FsshNdArgs& A, const LaneN<R>& L) {
-    if (L.step >= A.nsteps) return true;
-    if (A.cell.mode == 2 && cell_outside<3, 2, R>(A.cell, L.r)) return true;  // TerminateCellCallback
-    if (A.term_kind != 0) {
+__device__ __forceinline__ bool nd_done(const FsshNdArgs& A, const NdHot& H, const LaneN<R>& L) {
+    if (L.step >= H.nsteps) return true;
+    if (H.cell_mode == 2 && cell_outside<3, 2, R>(A.cell, L.r)) return true;  // TerminateCellCallback
+    if (H.term_kind != 0) {
         double x = 0.0;
 #pragma unroll
-        for (int I = 0; I < kNdDof; ++I) x = (I == A.term_dof) ? (double)L.r[I] : x;
-        if (term_hit(A.term_kind, x, A.term_value)) return true;
+        for (int I = 0; I < kNdDof; ++I) x = (I == H.term_dof) ? (double)L.r[I] : x;
+        if (term_hit(H.term_kind, x, H.term_value)) return true;
     }
     return false;
 }
 
 // Eigen-decomposition, U^T dV U and the acceleration on the active surface or mean field.
 template <typename R>
-__device__ __forceinline__ void nd_adiabatic(const FsshNdArgs& A, LaneN<R>& L, bool track, R v11, R v22, R v12,
+__device__ __forceinline__ void nd_adiabatic(const NdHot& A, LaneN<R>& L, bool track, R v11, R v22, R v12,
                                              const R d11[kNdDof], const R d22[kNdDof], const R d12[kNdDof],
                                              double& E0, double& E1, double G01[kNdDof]) {
     eigen2((double)v11, (double)v12, (double)v22, E0, E1, L.U, track);
@@ -287,8 +329,9 @@ __device__ __forceinline__ void nd_adiabatic(const FsshNdArgs& A, LaneN<R>& L, b
 
 // Rest of the step after the model at the drifted coordinates: kick, electronics, hop.
 template <typename R>
-__device__ __forceinline__ void nd_finish_step(const FsshNdArgs* Ap, const FsshNdArgs& A, LaneN<R>& L, R v11, R v22,
-                                               R v12, const R d11[kNdDof], const R d22[kNdDof], const R d12[kNdDof]) {
+__device__ __forceinline__ void nd_finish_step(const FsshNdArgs* Ap, const FsshNdArgs& Af, const NdHot& A, LaneN<R>& L,
+                                               R v11, R v22, R v12, const R d11[kNdDof], const R d22[kNdDof],
+                                               const R d12[kNdDof]) {
     const R hdt = (R)(0.5 * A.dt);
     double E0, E1, G01[kNdDof];
     nd_adiabatic(A, L, true, v11, v22, v12, d11, d22, d12, E0, E1, G01);  // sigma of the step start
@@ -333,13 +376,15 @@ __device__ __forceinline__ void nd_finish_step(const FsshNdArgs* Ap, const FsshN
         }
     }
     // CellBoundaryCallback after the hop callback; the step-end acceleration is kept (bab_electronics.jl:61-74)
-    if (A.cell.mode == 1 && cell_outside<3, 2, R>(A.cell, L.r)) cell_wrap<3, 2, R>(A.cell, L.r);
+    if (A.cell_mode == 1 && cell_outside<3, 2, R>(Af.cell, L.r)) cell_wrap<3, 2, R>(Af.cell, L.r);
     ++L.step;
 }
 
 template <typename R, bool kMlp>
 __device__ __forceinline__ void nd_body(const FsshNdArgs* __restrict__ Ap, const unsigned char* lds) {
     const FsshNdArgs& A = *Ap;
+    const NdHot H = nd_hot(A);
+    const NdParams P0 = nd_params(Ap);
     const int lane = __lane_id();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     LaneN<R> L;
@@ -371,7 +416,7 @@ __device__ __forceinline__ void nd_body(const FsshNdArgs* __restrict__ Ap, const
         if (__ballot(active) == 0) break;
         // (2) half kick + drift (step_B!, step_A!)
         if (active && !fresh) {
-            const R hdt = (R)(0.5 * A.dt), dtR = (R)A.dt;
+            const R hdt = (R)(0.5 * H.dt), dtR = (R)H.dt;
 #pragma unroll
             for (int I = 0; I < kNdDof; ++I) {
                 L.vt[I] = L.v[I] + hdt * L.k[I];
@@ -384,17 +429,17 @@ __device__ __forceinline__ void nd_body(const FsshNdArgs* __restrict__ Ap, const
             bool far;
             surrogate_eval(A, lds, L.r, active, v11, v22, v12, d11, d22, d12, far);
             if (active && far) {  // members disagree: the analytic model for this lane and step
-                diatom_eval<R>(nd_opaque(Ap)->p, L.r, v11, v22, v12, d11, d22, d12);
+                diatom_eval<R>(nd_params_here(P0), L.r, v11, v22, v12, d11, d22, d12);
                 ++L.nfb;
             }
         } else {
-            if (active) diatom_eval<R>(nd_opaque(Ap)->p, L.r, v11, v22, v12, d11, d22, d12);
+            if (active) diatom_eval<R>(nd_params_here(P0), L.r, v11, v22, v12, d11, d22, d12);
         }
         // (4) initialise refilled lanes (bab_electronics.jl:48-59), or finish the step
         if (active) {
             if (fresh) {
                 double E0, E1, G01[kNdDof];
-                nd_adiabatic(A, L, false, v11, v22, v12, d11, d22, d12, E0, E1, G01);
+                nd_adiabatic(H, L, false, v11, v22, v12, d11, d22, d12, E0, E1, G01);
                 L.dEc = 0.0; L.wc = 0.0; L.tc = 0.0;  // zero buffers stamped t = 0
                 L.t = A.t0;
                 L.nh = 0;
@@ -403,10 +448,10 @@ __device__ __forceinline__ void nd_body(const FsshNdArgs* __restrict__ Ap, const
                 fresh = false;
                 // the termination predicate is a DiscreteCallback, evaluated after each step and not
                 // at t0 (callbacks.jl:15-29): a fresh trajectory is done only for zero steps
-                if (A.nsteps <= 0) { nd_finish(Ap, L); active = false; }
+                if (H.nsteps <= 0) { nd_finish(Ap, L); active = false; }
             } else {
-                nd_finish_step(Ap, A, L, v11, v22, v12, d11, d22, d12);
-                if (nd_done(A, L)) { nd_finish(Ap, L); active = false; }
+                nd_finish_step(Ap, A, H, L, v11, v22, v12, d11, d22, d12);
+                if (nd_done(A, H, L)) { nd_finish(Ap, L); active = false; }
             }
         }
     }

@@ -72,6 +72,19 @@ __device__ __forceinline__ void philox_normal2(uint64_t traj, uint32_t step, uin
     z1 = rad * s;
 }
 
+// A wavefront-uniform value (a launch argument read through a pointer) moved to scalar registers,
+// so a loop that uses it needs no memory access for it.
+__device__ __forceinline__ int32_t uniform_sgpr(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uniform_sgpr(uint64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int64_t uniform_sgpr(int64_t x) { return (int64_t)uniform_sgpr((uint64_t)x); }
+__device__ __forceinline__ double uniform_sgpr(double x) {
+    return __longlong_as_double((long long)uniform_sgpr((uint64_t)__double_as_longlong(x)));
+}
+
 // ---------------------------------------------------------------------------
 // Transcendentals by precision: FP64 path uses ocml's correctly-rounded-ish
 // double routines; the FP32 path uses the hardware exp.

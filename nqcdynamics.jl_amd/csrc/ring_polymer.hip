@@ -164,8 +164,9 @@ __device__ __forceinline__ void rp_elec_init(const RpArgs* __restrict__ Ap, cons
 
 // The density over [t, tn] under the centroid at the step end (rc, vc): the interpolation buffer's
 // new end is dE = E0 - E1 and w = d01 vc at the centroid (the tracked eigenvectors, bcb_electronics.jl:88-92).
-template <typename R>
-__device__ __forceinline__ void rp_elec_propagate(const RpArgs& A, const ModelRegs& m, R rc, double vc, double t,
+// A: RpArgs, or RpHot (its fields in scalar registers)
+template <typename R, class AT>
+__device__ __forceinline__ void rp_elec_propagate(const AT& A, const ModelRegs& m, R rc, double vc, double t,
                                                   double tn, RpElec& E) {
     R v11, v22, v12, d11, d22, d12;
     model1d_eval<R>(m, rc, v11, v22, v12, d11, d22, d12);
@@ -185,7 +186,8 @@ __device__ __forceinline__ void rp_elec_propagate(const RpArgs& A, const ModelRe
 
 // The centroid hop at the end of step `step` (rpsh.jl:30-50): returns the centroid mode's momentum
 // change (sqrt(P) times the centroid velocity change), zero unless a hop is accepted or reflected.
-__device__ __forceinline__ double rp_elec_hop(const RpArgs& A, double vc, double sqP, int64_t id, int step,
+template <class AT>
+__device__ __forceinline__ double rp_elec_hop(const AT& A, double vc, double sqP, int64_t id, int step,
                                               RpElec& E) {
     double dp = 0.0;
     const double sss = 0.5 * (E.tr + ((E.est == 0) ? E.s.q : -E.s.q));
@@ -219,6 +221,20 @@ __device__ __forceinline__ double rp_elec_hop(const RpArgs& A, double vc, double
         }
     }
     return dp;
+}
+
+// The launch arguments the centroid electronics read every step, in scalar registers.
+struct RpHot {
+    double dt, mass;
+    uint64_t key;
+    int32_t nsub, rescaling;
+};
+
+__device__ __forceinline__ RpHot rp_hot(const RpArgs& A) {
+    RpHot H;
+    H.dt = uniform_sgpr(A.dt); H.mass = uniform_sgpr(A.mass); H.key = uniform_sgpr(A.key);
+    H.nsub = uniform_sgpr(A.nsub); H.rescaling = uniform_sgpr(A.rescaling);
+    return H;
 }
 
 // Electronic save slot of ring polymer id: adiabatic populations (the active state's indicator for
@@ -520,10 +536,14 @@ __device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
         if (sh) rp_save_elec(B, slot, id, El);
     };
     save(0);
-    const R hdt = (R)(0.5 * A.dt);
+    // the step loop's launch arguments in scalar registers (no memory access per step for them)
+    const int64_t nsteps = uniform_sgpr(A.nsteps);
+    const int32_t save_every = uniform_sgpr(A.save_every);
+    const RpHot Hs = rp_hot(A);
+    const R hdt = (R)(0.5 * Hs.dt);
     double t = A.t0;
 #pragma unroll 1
-    for (int step = 0; step < A.nsteps; ++step) {
+    for (int step = 0; step < nsteps; ++step) {
 #pragma unroll
         for (int k = 0; k < P; ++k) {                            // step_B!, step_C! (mode k)
             const R pk = p[k] + hdt * an[k];
@@ -539,17 +559,17 @@ __device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
         }
 #pragma unroll
         for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * an[k];  // step_B!
-        const double tn = t + A.dt;
-        const bool save_now = A.save_every > 0 && ((step + 1) % A.save_every) == 0;
+        const double tn = t + Hs.dt;
+        const bool save_now = save_every > 0 && ((step + 1) % save_every) == 0;
         double dp = 0.0;
         if (sh) {
             const double vc = (double)p[0] / sqP;
-            rp_elec_propagate<R>(A, m, q[0] / (R)sqP, vc, t, tn, El);
+            rp_elec_propagate<R>(Hs, m, q[0] / (R)sqP, vc, t, tn, El);
             // the save precedes the hop callback (a save time holds the state before the step's callbacks)
-            if (save_now) save((step + 1) / A.save_every);
-            dp = rp_elec_hop(A, vc, sqP, id, step, El);
+            if (save_now) save((step + 1) / save_every);
+            dp = rp_elec_hop(Hs, vc, sqP, id, step, El);
         } else if (save_now) {
-            save((step + 1) / A.save_every);
+            save((step + 1) / save_every);
         }
         if (dp != 0.0) p[0] = (R)((double)p[0] + dp);
         t = tn;
