@@ -363,7 +363,7 @@ typedef struct {
  * caller is the one-step precompile_dynamics pre-run of run_dynamics.jl:100-120).  Runs the
  * loaded problem over [t0, t1] (no saves; with `term` when given, which makes the sample include
  * the tail of uneven trajectory lengths) once per candidate launch shape --
- * 256, 128 and 64 threads per workgroup, each at the occupancy limit and at 3/4 and 1/2 of it --
+ * 256, 128 and 64 threads per workgroup, each at the occupancy limit and at 3/4, 1/2 and 1/4 of it --
  * keeps the fastest (the default unless a candidate is faster by more than 2%) in a process-wide
  * table keyed by kernel family, state count, precision and compute-unit count, and later runs
  * of that key use it.  With NQCD_MI355X_TUNE_CACHE set the table persists in that file.  Results

@@ -1026,7 +1026,7 @@ int32_t nqcd_mi355x_autotune(nqcd_handle_t* h, double t0, double t1, const nqcd_
         HIPCHK(h, family == TUNE_FSSH2 ? fssh2_persistent_occupancy(h->prob.precision, block, &occ)
                                         : nd_persistent_occupancy(h->prob.precision, block, &occ));
         if (occ < 1) continue;
-        for (int w : {occ, occ * 3 / 4, occ / 2}) {
+        for (int w : {occ, occ * 3 / 4, occ / 2, occ / 4}) {
             if (w < 1) continue;
             const TuneShape s{block, w == occ ? 0 : w};
             bool dup = false;
