@@ -65,6 +65,9 @@ struct RpArgs {
     double* save_pop_ad;    // [nsave][2][ntraj]
     double* save_pop_di;
     int32_t* save_state;
+    // Cayley coefficients c11[P], c12[P], c21[P] in the bead precision, written on the device before
+    // the one-ring-polymer-per-lane kernels of 32 and more beads (rp_cayley_fill)
+    double cay[3 * 64];
 };
 
 template <class T>
@@ -430,11 +433,12 @@ __device__ __forceinline__ void rp_body(const RpArgs* __restrict__ Ap) {
 }
 
 // ---------------------------------------------------------------------------
-// One ring polymer per lane (RPMD and RPSH; up to 16 beads in FP32, 8 in FP64).  The lane keeps all
-// P normal modes in registers and applies the Hartley transforms as P x P products with the P
-// coefficients cas(2 pi m / P) / sqrt(P) (H[j][k] depends on jk mod P only), so nothing passes
-// through LDS, no workgroup barrier is needed, and every lane runs its own ring polymer's centroid
-// electronics: in the bead-per-lane layout P - 1 of every P lanes wait while the FP64 electronics run.
+// One ring polymer per lane (RPMD and RPSH; up to 64 beads in FP32, 32 in FP64).  The lane keeps all
+// P normal modes in registers and applies the Hartley transform (orthogonal and its own inverse) in
+// registers, so nothing passes through LDS in the step, no workgroup barrier is needed, and every
+// lane runs its own ring polymer's centroid electronics: in the bead-per-lane layout P - 1 of every P
+// lanes wait while the FP64 electronics run.  From 8 beads up the transform is the radix-2 fast
+// Hartley transform (P log2 P butterflies instead of the P^2 products of the matrix form).
 // ---------------------------------------------------------------------------
 template <int P, typename R>
 __device__ __forceinline__ void hartley_regs(const R c[P], const R x[P], R y[P]) {
@@ -447,14 +451,82 @@ __device__ __forceinline__ void hartley_regs(const R c[P], const R x[P], R y[P])
     }
 }
 
+constexpr int bit_reverse(int i, int P) {
+    int r = 0;
+    for (int b = 1; b < P; b <<= 1) { r = (r << 1) | (i & 1); i >>= 1; }
+    return r;
+}
+
+// x <- H x with H[j][k] = cas(2 pi jk / P) / sqrt(P), P a power of two: radix-2 decimation in time
+// (Bracewell).  With E and O the transforms of the even and odd halves (length h), the length-2h
+// transform is H_k = E_k + cos(pi k/h) O_k + sin(pi k/h) O_{h-k} and H_{k+h} = E_k - (the same),
+// indices mod h; modes k and h - k are combined together so the butterfly runs in place.  The input
+// permutation and the twiddles are compile-time constants.
+template <int P, typename R>
+__device__ __forceinline__ void fht_regs(R x[P]) {
+    R t[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) t[i] = x[bit_reverse(i, P)];
+#pragma unroll
+    for (int h = 1; h < P; h <<= 1) {
+#pragma unroll
+        for (int b = 0; b < P; b += 2 * h) {
+            {
+                const R e = t[b], o = t[b + h];
+                t[b] = e + o; t[b + h] = e - o;
+            }
+            if (h >= 2) {   // k = h/2: cos = 0, sin = 1, O_{h-k} = O_k
+                const int k = h / 2;
+                const R e = t[b + k], o = t[b + h + k];
+                t[b + k] = e + o; t[b + h + k] = e - o;
+            }
+#pragma unroll
+            for (int k = 1; k < h / 2; ++k) {
+                const int kk = h - k;
+                double sn, cs;
+                sincospi((double)k / (double)h, &sn, &cs);
+                const R c = (R)cs, s = (R)sn;
+                const R ek = t[b + k], ekk = t[b + kk], ok = t[b + h + k], okk = t[b + h + kk];
+                const R t1 = c * ok + s * okk, t2 = s * ok - c * okk;
+                t[b + k] = ek + t1; t[b + k + h] = ek - t1;
+                t[b + kk] = ekk + t2; t[b + kk + h] = ekk - t2;
+            }
+        }
+    }
+    const R sc = (R)(1.0 / sqrt((double)P));
+#pragma unroll
+    for (int i = 0; i < P; ++i) x[i] = t[i] * sc;
+}
+
+// x <- H x in place: the fast transform from 8 beads, the matrix form below that.
+template <int P, typename R>
+__device__ __forceinline__ void hartley_inplace(R x[P]) {
+    if constexpr (P >= 8) {
+        fht_regs<P, R>(x);
+    } else {
+        R c[P];
+        const double s = 1.0 / sqrt((double)P);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            double sn, cs;
+            sincospi(2.0 * (double)k / (double)P, &sn, &cs);
+            c[k] = (R)((cs + sn) * s);
+        }
+        R y[P];
+        hartley_regs<P, R>(c, x, y);
+#pragma unroll
+        for (int k = 0; k < P; ++k) x[k] = y[k];
+    }
+}
+
 constexpr int kRpLaneBlock = 64;   // one wavefront per workgroup: small ensembles spread over every CU
 
 // Accelerations of P beads from the gradients of the surface each moves on (surface_gradient), under
-// one model dispatch so the beads form straight-line code.  FP32 beads take 1/rad from the hardware
-// reciprocal square root and multiply by the FP32 inverse mass; FP64 beads keep the rounded square
-// root and the divisions.
+// one model dispatch so the beads form straight-line code; in place (x: bead positions in,
+// accelerations out).  FP32 beads take 1/rad from the hardware reciprocal square root and multiply by
+// the FP32 inverse mass; FP64 beads keep the rounded square root and the divisions.
 template <int P, typename R, bool SH>
-__device__ __forceinline__ void bead_accelerations(const ModelRegs& m, const R x[P], int st, double mass, R acc[P]) {
+__device__ __forceinline__ void bead_accelerations(const ModelRegs& m, R x[P], int st, double mass) {
     model1d_dispatch(m, [&](auto kind) {
         constexpr int K = decltype(kind)::value;
         const R inv_m = (R)(1.0 / mass);
@@ -476,58 +548,91 @@ __device__ __forceinline__ void bead_accelerations(const ModelRegs& m, const R x
                 }
                 g = (st == 0) ? dm - c : dm + c;
             }
-            if constexpr (sizeof(R) == 4) acc[l] = -g * inv_m;
-            else acc[l] = -g / mass;
+            if constexpr (sizeof(R) == 4) x[l] = -g * inv_m;
+            else x[l] = -g / mass;
         }
     });
 }
 
+// Cayley map of mode k (ring_polymer.jl:71-82, half = false) in the bead precision.
+template <typename R>
+__device__ __forceinline__ void cayley_mode(const RpArgs& A, int k, int P, R& c11, R& c12, R& c21) {
+    const double w = 2.0 * A.wn * sinpi((double)k / (double)P);
+    const double u = A.dt * A.dt * w * w / 4.0;
+    const double si = 1.0 / (1.0 + u);
+    c11 = (R)((1.0 - u) * si); c12 = (R)(A.dt * si); c21 = (R)(-A.dt * w * w * si);
+}
+
+// The Cayley table of the argument block (the first 3 P entries in the bead precision).
+template <int P, typename R>
+__global__ void rp_cayley_fill(RpArgs* A) {
+    const int k = threadIdx.x;
+    if (k < P) {
+        R* t = reinterpret_cast<R*>(A->cay);
+        cayley_mode<R>(*A, k, P, t[k], t[P + k], t[2 * P + k]);
+    }
+}
+
+template <typename R>
+using RpConstTab = __attribute__((address_space(4))) const R*;
+
+template <typename R>
+__device__ __forceinline__ RpConstTab<R> rp_cay_table(const RpArgs* Ap) {
+    const uint64_t a = (uint64_t)reinterpret_cast<const R*>(Ap->cay);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a), hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return (RpConstTab<R>)(((uint64_t)hi << 32) | lo);
+}
+
+// The table pointer made opaque where it is used, so its entries are scalar-loaded there each step
+// (operands straight from SGPRs) instead of being hoisted into registers for the whole loop.
+template <typename R>
+__device__ __forceinline__ RpConstTab<R> rp_cay_here(RpConstTab<R> p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// Whether the lane kernel of P beads reads its Cayley coefficients from the argument block's table.
+constexpr bool rp_lane_table(int P, bool fp64) { return P >= 32 || (fp64 && P >= 16); }
+
 template <int P, typename R, bool SH>
 __device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
+    // From 32 beads (16 in FP64) the 3 P Cayley coefficients are scalar loads from the argument
+    // block's table (written by rp_cayley_fill) instead of registers, which hold the 2 P modes and the
+    // P-bead work array.
+    constexpr bool kTab = rp_lane_table(P, sizeof(R) == 8);
     const RpArgs& A = *Ap;
     const int64_t n = A.ntraj;
     const int64_t id = (int64_t)blockIdx.x * kRpLaneBlock + threadIdx.x;
     if (id >= n) return;   // no barriers below
     constexpr bool sh = SH;
-    R c[P], C11[P], C12[P], C21[P];
-    {
-        const double s = 1.0 / sqrt((double)P);
+    R C11[kTab ? 1 : P], C12[kTab ? 1 : P], C21[kTab ? 1 : P];
+    if constexpr (!kTab) {
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            double sn, cs;
-            sincospi(2.0 * (double)k / (double)P, &sn, &cs);
-            c[k] = (R)((cs + sn) * s);
-            // Cayley map of mode k (ring_polymer.jl:71-82, half = false)
-            const double w = 2.0 * A.wn * sinpi((double)k / (double)P);
-            const double u = A.dt * A.dt * w * w / 4.0;
-            const double si = 1.0 / (1.0 + u);
-            C11[k] = (R)((1.0 - u) * si); C12[k] = (R)(A.dt * si); C21[k] = (R)(-A.dt * w * w * si);
-        }
+        for (int k = 0; k < P; ++k) cayley_mode<R>(A, k, P, C11[k], C12[k], C21[k]);
     }
     const double sqP = sqrt((double)P);
     const ModelRegs m = model_regs(A.model);
-    R q[P], p[P], an[P];
+    R q[P], p[P], w[P];   // modes, and the work array (bead positions -> accelerations -> their modes)
     RpElec El;
     El.tr = 1.0; El.s = {1.0, 0.0, 0.0}; El.est = 0; El.nh = 0;
     El.Uc[0] = 1.0; El.Uc[1] = 0.0; El.Uc[2] = 0.0; El.Uc[3] = 1.0;
     El.dEc = 0.0; El.wc = 0.0; El.tc = 0.0; El.d01 = 0.0; El.gap = 0.0;
     {
         const RpArgs& B = *rp_opaque(Ap);
-        R rb[P], vb[P];
 #pragma unroll
         for (int l = 0; l < P; ++l) {
-            rb[l] = (R)B.r0[(int64_t)l * n + id];
-            vb[l] = (R)B.v0[(int64_t)l * n + id];
+            q[l] = (R)B.r0[(int64_t)l * n + id];
+            p[l] = (R)B.v0[(int64_t)l * n + id];
+            w[l] = q[l];
         }
-        hartley_regs<P, R>(c, rb, q);
-        hartley_regs<P, R>(c, vb, p);
+        hartley_inplace<P, R>(q);
+        hartley_inplace<P, R>(p);
         if (sh) {
             double ea, eb, ex;
             rp_elec_init<R>(Ap, m, q[0] / (R)sqP, id, El, ea, eb, ex);
         }
-        R acc[P];
-        bead_accelerations<P, R, SH>(m, rb, El.est, A.mass, acc);
-        hartley_regs<P, R>(c, acc, an);
+        bead_accelerations<P, R, SH>(m, w, El.est, A.mass);   // at the initial beads
+        hartley_inplace<P, R>(w);
     }
     auto save = [&](int slot) {   // centroid and electronics of save slot `slot`
         const RpArgs& B = *rp_opaque(Ap);
@@ -542,23 +647,32 @@ __device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
     const RpHot Hs = rp_hot(A);
     const R hdt = (R)(0.5 * Hs.dt);
     double t = A.t0;
+    const RpConstTab<R> cay0 = kTab ? rp_cay_table<R>(Ap) : nullptr;
+    // The accelerations of a step serve both its closing half kick and the next step's opening one,
+    // applied together at the step's end (each rounded as before) so they are not held across steps.
+    if (nsteps > 0) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * w[k];  // step_B! of the first step
+    }
 #pragma unroll 1
     for (int step = 0; step < nsteps; ++step) {
+        const RpConstTab<R> cay = kTab ? rp_cay_here<R>(cay0) : cay0;
 #pragma unroll
-        for (int k = 0; k < P; ++k) {                            // step_B!, step_C! (mode k)
-            const R pk = p[k] + hdt * an[k];
-            const R qn = C11[k] * q[k] + C12[k] * pk;
-            p[k] = C21[k] * q[k] + C11[k] * pk;
+        for (int k = 0; k < P; ++k) {                            // step_C! (mode k)
+            R c11, c12, c21;
+            if constexpr (kTab) { c11 = cay[k]; c12 = cay[P + k]; c21 = cay[2 * P + k]; }
+            else { c11 = C11[k]; c12 = C12[k]; c21 = C21[k]; }
+            const R qn = c11 * q[k] + c12 * p[k];
+            p[k] = c21 * q[k] + c11 * p[k];
             q[k] = qn;
         }
-        {
-            R rb[P], acc[P];
-            hartley_regs<P, R>(c, q, rb);                        // bead positions
-            bead_accelerations<P, R, SH>(m, rb, El.est, A.mass, acc);
-            hartley_regs<P, R>(c, acc, an);
-        }
 #pragma unroll
-        for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * an[k];  // step_B!
+        for (int k = 0; k < P; ++k) w[k] = q[k];
+        hartley_inplace<P, R>(w);                                // bead positions
+        bead_accelerations<P, R, SH>(m, w, El.est, A.mass);
+        hartley_inplace<P, R>(w);                                // accelerations in modes
+#pragma unroll
+        for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * w[k];   // step_B!
         const double tn = t + Hs.dt;
         const bool save_now = save_every > 0 && ((step + 1) % save_every) == 0;
         double dp = 0.0;
@@ -573,17 +687,18 @@ __device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
         }
         if (dp != 0.0) p[0] = (R)((double)p[0] + dp);
         t = tn;
+        if (step + 1 < nsteps) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * w[k];  // step_B! of the next step
+        }
     }
     const RpArgs& B = *rp_opaque(Ap);
-    {
-        R rb[P], vb[P];
-        hartley_regs<P, R>(c, q, rb);
-        hartley_regs<P, R>(c, p, vb);
+    hartley_inplace<P, R>(q);
+    hartley_inplace<P, R>(p);
 #pragma unroll
-        for (int l = 0; l < P; ++l) {
-            B.r[(int64_t)l * n + id] = (double)rb[l];
-            B.v[(int64_t)l * n + id] = (double)vb[l];
-        }
+    for (int l = 0; l < P; ++l) {
+        B.r[(int64_t)l * n + id] = (double)q[l];
+        B.v[(int64_t)l * n + id] = (double)p[l];
     }
     if (sh) rp_store_elec(B, id, El);
     else rp_store_single(B, id);
@@ -631,10 +746,14 @@ NQCD_RP_LANE_KERNELS(2, float, f32_2)
 NQCD_RP_LANE_KERNELS(4, float, f32_4)
 NQCD_RP_LANE_KERNELS(8, float, f32_8)
 NQCD_RP_LANE_KERNELS(16, float, f32_16)
+NQCD_RP_LANE_KERNELS(32, float, f32_32)
+NQCD_RP_LANE_KERNELS(64, float, f32_64)
 NQCD_RP_LANE_KERNELS(1, double, f64_1)
 NQCD_RP_LANE_KERNELS(2, double, f64_2)
 NQCD_RP_LANE_KERNELS(4, double, f64_4)
 NQCD_RP_LANE_KERNELS(8, double, f64_8)
+NQCD_RP_LANE_KERNELS(16, double, f64_16)
+NQCD_RP_LANE_KERNELS(32, double, f64_32)
 #undef NQCD_RP_LANE_KERNELS
 
 // Thermal ring polymers in the harmonic approximation of a well at x0 with frequency w_well:
@@ -805,15 +924,28 @@ static void rp_shape(int P, int precision, bool elec, int& block, int& nr) {
 // elec selects the electronics kernel; the RPMD-only kernels exist up to 16 beads.
 static bool rp_needs_elec(int method, int P) { return method != NQCD_METHOD_RPMD || P > 16; }
 
-// The one-ring-polymer-per-lane kernels exist for RPMD and RPSH up to 16 beads in FP32 and 8 in FP64.
+// The one-ring-polymer-per-lane kernels: RPMD up to 64 beads in FP32 and 32 in FP64, RPSH up to 32 and
+// 16 (beyond those the lane's registers spill).
 static bool rp_lane_exists(int method, int P, int precision) {
     if (method != NQCD_METHOD_RPMD && method != NQCD_METHOD_RPSH) return false;
-    return precision == 0 ? P <= 8 : P <= 16;
+    const int pmax = (precision == 0 ? 32 : 64) / (method == NQCD_METHOD_RPSH ? 2 : 1);
+    return P <= pmax;
 }
 
-// Layout of a run: one ring polymer per lane where that kernel exists (measured at 1e5 ring polymers
-// of 8 beads, FP32: RPSH 3.4x and RPMD 1.3x the bead-per-lane rate; at 16 beads 4.2x and 2.1x), P
-// lanes per ring polymer otherwise.  NQCD_MI355X_RP_LAYOUT=lane or bead overrides the choice where
+static hipError_t rp_fill_cayley(int P, int precision, hipStream_t s, RpArgs* Adev) {
+    if (!rp_lane_table(P, precision == 0)) return hipSuccess;
+    switch (P * (precision == 0 ? -1 : 1)) {
+    case 32: hipLaunchKernelGGL((rp_cayley_fill<32, float>), dim3(1), dim3(64), 0, s, Adev); break;
+    case 64: hipLaunchKernelGGL((rp_cayley_fill<64, float>), dim3(1), dim3(64), 0, s, Adev); break;
+    case -16: hipLaunchKernelGGL((rp_cayley_fill<16, double>), dim3(1), dim3(64), 0, s, Adev); break;
+    case -32: hipLaunchKernelGGL((rp_cayley_fill<32, double>), dim3(1), dim3(64), 0, s, Adev); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Layout of a run: one ring polymer per lane where that kernel exists, P lanes per ring polymer
+// otherwise.  NQCD_MI355X_RP_LAYOUT=lane or bead overrides the choice where
 // both kernels exist.
 static bool rp_use_lane(int method, int P, int precision) {
     if (!rp_lane_exists(method, P, precision)) return false;
@@ -833,12 +965,13 @@ static hipError_t rp_launch_lane(int P, int precision, bool sh, int grid, hipStr
     if (precision == 0) {
         switch (P) {
             NQCD_LANE_CASE(1, f64_1) NQCD_LANE_CASE(2, f64_2) NQCD_LANE_CASE(4, f64_4) NQCD_LANE_CASE(8, f64_8)
+            NQCD_LANE_CASE(16, f64_16) NQCD_LANE_CASE(32, f64_32)
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (P) {
             NQCD_LANE_CASE(1, f32_1) NQCD_LANE_CASE(2, f32_2) NQCD_LANE_CASE(4, f32_4) NQCD_LANE_CASE(8, f32_8)
-            NQCD_LANE_CASE(16, f32_16)
+            NQCD_LANE_CASE(16, f32_16) NQCD_LANE_CASE(32, f32_32) NQCD_LANE_CASE(64, f32_64)
         default: return hipErrorInvalidValue;
         }
     }
@@ -1031,6 +1164,7 @@ int32_t rp_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_desc
     if (nr < 1) return fail(h, NQCD_ERR_UNSUPPORTED, "nbeads");
     const int grid = (int)((n + nr - 1) / nr);
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    if (lane) HIPCHK(h, rp_fill_cayley(P, h->prob.precision, h->stream, h->args_dev.as<RpArgs>()));
     if (lane) HIPCHK(h, rp_launch_lane(P, h->prob.precision, h->prob.method == NQCD_METHOD_RPSH, grid, h->stream,
                                        h->args_dev.as<const RpArgs>()));
     else HIPCHK(h, rp_launch(P, h->prob.precision, elec, grid, h->stream, h->args_dev.as<const RpArgs>()));

@@ -41,7 +41,7 @@ def _layout(monkeypatch, layout, P, exists):
 @pytest.mark.parametrize("P", [1, 2, 4, 8, 16, 32, 64])
 def test_rpmd_fp64_parity(oracle, key, P, layout, monkeypatch):
     from nqcd_mi355x import _lib as L
-    _layout(monkeypatch, layout, P, P <= 8)
+    _layout(monkeypatch, layout, P, P <= 32)
     ntraj, nsteps = 256, 500
     r0, v0 = _thermal_beads(L, P, ntraj, key)
     assert np.isfinite(r0).all() and abs(r0.mean() + 3.87) < 0.2
@@ -59,11 +59,11 @@ def test_rpmd_fp64_parity(oracle, key, P, layout, monkeypatch):
 
 
 @pytest.mark.parametrize("layout", ["bead", "lane"])
-@pytest.mark.parametrize("P", [1, 4, 8, 32, 64])
+@pytest.mark.parametrize("P", [1, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("method", ["rpsh", "rpeh"])
 def test_rp_electronic_fp64_parity(oracle, key, P, method, layout, monkeypatch):
     from nqcd_mi355x import _lib as L
-    _layout(monkeypatch, layout, P, method == "rpsh" and P <= 8)
+    _layout(monkeypatch, layout, P, method == "rpsh" and P <= 16)
     ntraj, nsteps = 256, 600
     rng = np.random.default_rng(P)
     # ring polymers crossing the coupling region at x ~ 0 with a spread of velocities
@@ -211,7 +211,7 @@ def test_pimc_path_integral_law_on_device(key):
     assert abs(v.var() / (P * T / mass) - 1) < 5 * np.sqrt(2.0 / v.size)
 
 
-@pytest.mark.parametrize("P", [4, 8, 16])
+@pytest.mark.parametrize("P", [4, 8, 16, 32])
 def test_rp_layouts_agree_fp32(key, P, monkeypatch):
     """The two FP32 layouts (one ring polymer per lane; P lanes per ring polymer with the electronics on
     their own lanes) integrate the same ring polymers: RPSH over 1500 steps from PIMC beads ends in the same
