@@ -673,6 +673,14 @@ __device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
         hartley_inplace<P, R>(w);                                // accelerations in modes
 #pragma unroll
         for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * w[k];   // step_B!
+        // the next step's opening kick of modes 1.. now (the hop changes only the centroid mode), so
+        // the accelerations are dead during the electronics
+        const R w0 = w[0];
+        const bool more = step + 1 < nsteps;
+        if (more) {
+#pragma unroll
+            for (int k = 1; k < P; ++k) p[k] = p[k] + hdt * w[k];
+        }
         const double tn = t + Hs.dt;
         const bool save_now = save_every > 0 && ((step + 1) % save_every) == 0;
         double dp = 0.0;
@@ -687,10 +695,7 @@ __device__ __forceinline__ void rp_lane_body(const RpArgs* __restrict__ Ap) {
         }
         if (dp != 0.0) p[0] = (R)((double)p[0] + dp);
         t = tn;
-        if (step + 1 < nsteps) {
-#pragma unroll
-            for (int k = 0; k < P; ++k) p[k] = p[k] + hdt * w[k];  // step_B! of the next step
-        }
+        if (more) p[0] = p[0] + hdt * w0;                        // the next step's opening kick, centroid mode
     }
     const RpArgs& B = *rp_opaque(Ap);
     hartley_inplace<P, R>(q);
@@ -734,26 +739,29 @@ NQCD_RP_KERNELS(64)
 #undef NQCD_RPMD_KERNELS
 
 // rp_lane_*: RPMD; rp_lanesh_*: RPSH
-#define NQCD_RP_LANE_KERNELS(P, R, SUFFIX)                                                                  \
+// WSH: the RPSH kernel's minimum waves per SIMD (2 where that fits without spilling much: 1e5 ring
+// polymers are 1.5 wavefronts per SIMD, so a one-wave kernel runs the rest as a second round).
+#define NQCD_RP_LANE_KERNELS(P, R, SUFFIX, WSH)                                                             \
     __global__ __launch_bounds__(kRpLaneBlock) void rp_lane_##SUFFIX(const RpArgs* __restrict__ Ap) {      \
         rp_lane_body<P, R, false>(Ap);                                                                      \
     }                                                                                                       \
-    __global__ __launch_bounds__(kRpLaneBlock) void rp_lanesh_##SUFFIX(const RpArgs* __restrict__ Ap) {    \
+    __global__ __launch_bounds__(kRpLaneBlock) __attribute__((amdgpu_waves_per_eu(WSH)))                    \
+    void rp_lanesh_##SUFFIX(const RpArgs* __restrict__ Ap) {                                               \
         rp_lane_body<P, R, true>(Ap);                                                                       \
     }
-NQCD_RP_LANE_KERNELS(1, float, f32_1)
-NQCD_RP_LANE_KERNELS(2, float, f32_2)
-NQCD_RP_LANE_KERNELS(4, float, f32_4)
-NQCD_RP_LANE_KERNELS(8, float, f32_8)
-NQCD_RP_LANE_KERNELS(16, float, f32_16)
-NQCD_RP_LANE_KERNELS(32, float, f32_32)
-NQCD_RP_LANE_KERNELS(64, float, f32_64)
-NQCD_RP_LANE_KERNELS(1, double, f64_1)
-NQCD_RP_LANE_KERNELS(2, double, f64_2)
-NQCD_RP_LANE_KERNELS(4, double, f64_4)
-NQCD_RP_LANE_KERNELS(8, double, f64_8)
-NQCD_RP_LANE_KERNELS(16, double, f64_16)
-NQCD_RP_LANE_KERNELS(32, double, f64_32)
+NQCD_RP_LANE_KERNELS(1, float, f32_1, 1)
+NQCD_RP_LANE_KERNELS(2, float, f32_2, 1)
+NQCD_RP_LANE_KERNELS(4, float, f32_4, 1)
+NQCD_RP_LANE_KERNELS(8, float, f32_8, 1)
+NQCD_RP_LANE_KERNELS(16, float, f32_16, 1)
+NQCD_RP_LANE_KERNELS(32, float, f32_32, 2)
+NQCD_RP_LANE_KERNELS(64, float, f32_64, 1)
+NQCD_RP_LANE_KERNELS(1, double, f64_1, 1)
+NQCD_RP_LANE_KERNELS(2, double, f64_2, 1)
+NQCD_RP_LANE_KERNELS(4, double, f64_4, 1)
+NQCD_RP_LANE_KERNELS(8, double, f64_8, 1)
+NQCD_RP_LANE_KERNELS(16, double, f64_16, 1)
+NQCD_RP_LANE_KERNELS(32, double, f64_32, 1)
 #undef NQCD_RP_LANE_KERNELS
 
 // Thermal ring polymers in the harmonic approximation of a well at x0 with frequency w_well:
