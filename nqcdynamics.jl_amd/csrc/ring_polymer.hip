@@ -932,12 +932,11 @@ static void rp_shape(int P, int precision, bool elec, int& block, int& nr) {
 // elec selects the electronics kernel; the RPMD-only kernels exist up to 16 beads.
 static bool rp_needs_elec(int method, int P) { return method != NQCD_METHOD_RPMD || P > 16; }
 
-// The one-ring-polymer-per-lane kernels: RPMD up to 64 beads in FP32 and 32 in FP64, RPSH up to 32 and
-// 16 (beyond those the lane's registers spill).
+// The one-ring-polymer-per-lane kernels: RPMD and RPSH up to 64 beads in FP32 and 32 in FP64 (the
+// largest hold one wave per SIMD; their few spills are outside the step loop).
 static bool rp_lane_exists(int method, int P, int precision) {
     if (method != NQCD_METHOD_RPMD && method != NQCD_METHOD_RPSH) return false;
-    const int pmax = (precision == 0 ? 32 : 64) / (method == NQCD_METHOD_RPSH ? 2 : 1);
-    return P <= pmax;
+    return P <= (precision == 0 ? 32 : 64);
 }
 
 static hipError_t rp_fill_cayley(int P, int precision, hipStream_t s, RpArgs* Adev) {

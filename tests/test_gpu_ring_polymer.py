@@ -63,7 +63,7 @@ def test_rpmd_fp64_parity(oracle, key, P, layout, monkeypatch):
 @pytest.mark.parametrize("method", ["rpsh", "rpeh"])
 def test_rp_electronic_fp64_parity(oracle, key, P, method, layout, monkeypatch):
     from nqcd_mi355x import _lib as L
-    _layout(monkeypatch, layout, P, method == "rpsh" and P <= 16)
+    _layout(monkeypatch, layout, P, method == "rpsh" and P <= 32)
     ntraj, nsteps = 256, 600
     rng = np.random.default_rng(P)
     # ring polymers crossing the coupling region at x ~ 0 with a spread of velocities
