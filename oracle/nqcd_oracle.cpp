@@ -1431,6 +1431,194 @@ OR_API double oracle_cme_broadening(double h, double mu, double beta, double Gam
 }
 
 // ---------------------------------------------------------------------------
+// Spin mapping (SpinMappingW, spin_mapping.jl:1-131) integrated by MInt (mint.jl:42-181), one
+// coordinate.  With gamma = 2 (sqrt(n + 1) - 1) / n (spin_mapping.jl:4-9) the Hamiltonian is
+//   H = m v^2 / 2 + V0 + sum_i V_ii (X_i^2 + P_i^2 - gamma) / 2 + sum_{i<j} V_ij (X_i X_j + P_i P_j)
+// (classical_potential_energy, spin_mapping.jl:102-118), X the mapping positions, P the momenta.
+// MInt step (mint.jl:42-86): r' = r + dt/2 v; at r' the eigenpairs (w, Z) of V; X, P <- C X - D P,
+// C P + D X with C = Z diag(cos(w dt)) Z^T, D = Z diag(sin(-w dt)) Z^T (mint.jl:88-124); then with
+// W = Z^T dV Z: Gamma_ii = W_ii dt, Gamma_ji = sin((w_i - w_j) dt) W_ji / (w_i - w_j), Xi_ji =
+// (1 - cos((w_i - w_j) dt)) W_ji / (w_i - w_j) (j != i), E = Z Gamma Z^T, F = Z Xi Z^T
+// (mint.jl:132-162), force = (X^T E X + P^T E P) / 2 - X^T F P (mint.jl:164-181),
+// v <- v - force / m - dV0 / m dt + tr(dV) gamma / m dt / 2, and r'' = r' + dt/2 v.  set_xi! writes
+// only the off-diagonal entries of the cache matrix that transform_matrix! then overwrites with
+// Z Xi Z^T, so each step's Xi carries the previous step's transformed diagonal (zero at a
+// trajectory's first step); restated as the reference computes it.
+// ---------------------------------------------------------------------------
+struct OMapDesc {
+    int32_t nstates, save_every, nsave, pad;
+    double dt, mass;
+    int64_t nsteps;
+};
+
+static double mapping_gamma(int n) { return 2.0 / n * (std::sqrt((double)n + 1.0) - 1.0); }
+
+// H at (r, v, X, P): the classical Hamiltonian of the mapping variables.
+static double mapping_energy(const OMapDesc& D, const OModelDesc& m, double r, double v, const double* X, const double* P) {
+    const int n = D.nstates;
+    std::vector<double> V(n * n), dV(n * n);
+    double V0, dV0;
+    model_eval(m, &r, V.data(), dV.data(), &V0, &dV0);
+    const double g = mapping_gamma(n);
+    double e = 0.5 * D.mass * v * v + V0;
+    for (int i = 0; i < n; ++i) {
+        e += V[i * n + i] * (X[i] * X[i] + P[i] * P[i] - g) / 2.0;
+        for (int j = i + 1; j < n; ++j) e += V[j * n + i] * (X[i] * X[j] + P[i] * P[j]);
+    }
+    return e;
+}
+
+// motion! (spin_mapping.jl:56-100): dr = v, dv = -(dV0 + sum_i dV_ii (X_i^2 + P_i^2 - gamma)/2
+// + sum_{i<j} dV_ji (X_i X_j + P_i P_j)) / m, dX = V P, dP = -V X.  u = [r, v, X(n), P(n)].
+static void mapping_motion(const OMapDesc& D, const OModelDesc& m, const double* u, double* du) {
+    const int n = D.nstates;
+    const double r = u[0], v = u[1];
+    const double* X = u + 2;
+    const double* P = u + 2 + n;
+    std::vector<double> V(n * n), dV(n * n);
+    double V0, dV0;
+    model_eval(m, &r, V.data(), dV.data(), &V0, &dV0);
+    const double g = mapping_gamma(n);
+    double f = -dV0;
+    for (int i = 0; i < n; ++i) {
+        f -= dV[i * n + i] * (X[i] * X[i] + P[i] * P[i] - g) / 2.0;
+        for (int j = i + 1; j < n; ++j) f -= dV[j * n + i] * (X[i] * X[j] + P[i] * P[j]);
+    }
+    du[0] = v;
+    du[1] = f / D.mass;
+    for (int i = 0; i < n; ++i) {
+        double sx = 0.0, sp = 0.0;
+        for (int j = 0; j < n; ++j) { sx += V[i * n + j] * P[j]; sp += V[i * n + j] * X[j]; }
+        du[2 + i] = sx;
+        du[2 + n + i] = -sp;
+    }
+}
+
+// M <- Z M Z^T (transform_matrix!, mint.jl:126-130)
+static void mapping_transform(int n, const double* Z, double* M) {
+    std::vector<double> T(n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += M[i * n + k] * Z[j * n + k];
+            T[i * n + j] = s;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += Z[i * n + k] * T[k * n + j];
+            M[i * n + j] = s;
+        }
+}
+
+// One MInt step; Xi holds the cache matrix Xi between steps (its diagonal carries over).
+static void mint_step(const OMapDesc& D, const OModelDesc& m, double& r, double& v, double* X, double* P, double* Xi) {
+    const int n = D.nstates;
+    const double dt = D.dt, g = mapping_gamma(n);
+    const double rt = r + dt / 2.0 * v;
+    std::vector<double> V(n * n), dV(n * n), w(n), Z(n * n), C(n * n, 0.0), Dm(n * n, 0.0), tmp(n), tmp2(n);
+    double V0, dV0;
+    model_eval(m, &rt, V.data(), dV.data(), &V0, &dV0);
+    eigen_sym(n, V.data(), w.data(), Z.data(), nullptr);
+    for (int i = 0; i < n; ++i) { C[i * n + i] = std::cos(w[i] * dt); Dm[i * n + i] = std::sin(-w[i] * dt); }
+    mapping_transform(n, Z.data(), C.data());
+    mapping_transform(n, Z.data(), Dm.data());
+    for (int i = 0; i < n; ++i) {   // tmp = C X - D P, tmp2 = C P + D X
+        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+        for (int j = 0; j < n; ++j) { a += C[i * n + j] * X[j]; b += Dm[i * n + j] * P[j]; c += C[i * n + j] * P[j]; d += Dm[i * n + j] * X[j]; }
+        tmp[i] = a - b;
+        tmp2[i] = c + d;
+    }
+    for (int i = 0; i < n; ++i) { X[i] = tmp[i]; P[i] = tmp2[i]; }
+    // W = Z^T dV Z
+    std::vector<double> W(n * n), G(n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int a = 0; a < n; ++a)
+                for (int b = 0; b < n; ++b) s += Z[a * n + i] * dV[a * n + b] * Z[b * n + j];
+            W[i * n + j] = s;
+        }
+    for (int i = 0; i < n; ++i) {   // set_gamma! / set_xi! (column i, row j)
+        G[i * n + i] = W[i * n + i] * dt;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double dl = w[i] - w[j];
+            G[j * n + i] = std::sin(dl * dt) * W[j * n + i] / dl;
+            Xi[j * n + i] = (1.0 - std::cos(dl * dt)) * W[j * n + i] / dl;
+        }
+    }
+    mapping_transform(n, Z.data(), G.data());
+    mapping_transform(n, Z.data(), Xi);
+    double f = 0.0;   // get_mapping_nuclear_force
+    for (int i = 0; i < n; ++i) {
+        double eq = 0.0, ep = 0.0, fp = 0.0;
+        for (int j = 0; j < n; ++j) { eq += G[i * n + j] * X[j]; ep += G[i * n + j] * P[j]; fp += Xi[i * n + j] * P[j]; }
+        f += X[i] * eq + P[i] * ep - 2.0 * X[i] * fp;
+    }
+    f /= 2.0;
+    double trdV = 0.0;
+    for (int i = 0; i < n; ++i) trdV += dV[i * n + i];
+    v -= f / D.mass;
+    v -= dV0 / D.mass * dt;
+    v += trdV * g / D.mass * dt / 2.0;
+    r = rt + dt / 2.0 * v;
+}
+
+// Initial mapping variables of a pure diabatic state (spin_mapping.jl:18-37): theta_i uniform in
+// [0, 2 pi) from the Philox block (trajectory, i, stream 4); radius sqrt(2 + gamma) for the occupied
+// state, sqrt(gamma) for the others; the mapping positions take the sines and the momenta the cosines
+// (DynamicsVariables sets qmap = P and pmap = X of its cos / sin pair).
+OR_API void oracle_mapping_sample(int32_t n, int64_t ntraj, const int32_t* state, uint64_t key, const int64_t* traj_ids,
+                                  double* X, double* P) {
+    const double g = mapping_gamma(n);
+    for (int64_t t = 0; t < ntraj; ++t) {
+        const uint64_t id = traj_ids ? (uint64_t)traj_ids[t] : (uint64_t)t;
+        for (int i = 0; i < n; ++i) {
+            const double th = 2.0 * M_PI * uniform01(id, (uint32_t)i, 4u, key);
+            const double rad = std::sqrt(i == state[t] ? 2.0 + g : g);
+            X[t * n + i] = rad * std::sin(th);
+            P[t * n + i] = rad * std::cos(th);
+        }
+    }
+}
+
+OR_API double oracle_mapping_energy(const OMapDesc* D, const OModelDesc* m, const double* u) {
+    return mapping_energy(*D, *m, u[0], u[1], u + 2, u + 2 + D->nstates);
+}
+
+OR_API void oracle_mapping_motion(const OMapDesc* D, const OModelDesc* m, const double* u, double* du) {
+    mapping_motion(*D, *m, u, du);
+}
+
+// Trajectories by MInt: x, v (ntraj), X, P ([traj][n]) in place; per save slot the diabatic
+// populations (X_i^2 + P_i^2 - gamma) / 2 (spin_mapping.jl:120-131), the kinetic and the potential
+// energy (H - m v^2 / 2).
+OR_API int32_t oracle_mapping_run(const OMapDesc* D, const OModelDesc* m, int64_t ntraj, double* x, double* v,
+                                  double* X, double* P, double* spop, double* sek, double* sep, int32_t nthreads) {
+    const int n = D->nstates, ns = D->nsave;
+    const double g = mapping_gamma(n);
+    parallel_for(ntraj, nthreads, [&](int64_t t) {
+        double* Xt = X + t * n;
+        double* Pt = P + t * n;
+        std::vector<double> Xi(n * n, 0.0);
+        auto save = [&](int slot) {
+            if (slot >= ns) return;
+            for (int i = 0; i < n; ++i) spop[((size_t)t * ns + slot) * n + i] = (Xt[i] * Xt[i] + Pt[i] * Pt[i] - g) / 2.0;
+            const double ek = 0.5 * D->mass * v[t] * v[t];
+            sek[(size_t)t * ns + slot] = ek;
+            sep[(size_t)t * ns + slot] = mapping_energy(*D, *m, x[t], v[t], Xt, Pt) - ek;
+        };
+        save(0);
+        for (int64_t step = 0; step < D->nsteps; ++step) {
+            mint_step(*D, *m, x[t], v[t], Xt, Pt, Xi.data());
+            if (D->save_every > 0 && (step + 1) % D->save_every == 0) save((int)((step + 1) / D->save_every));
+        }
+    });
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Independent-electron surface hopping (IESH) and EhrenfestNA on a Newns-Anderson
 // (impurity + discretised bath) model.
 //   Model (NQCModels AndersonHolstein with a MiaoSubotnik impurity and a TrapezoidalRule

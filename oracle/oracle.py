@@ -384,6 +384,74 @@ def classical_run(model, mass, r0, v0, dt=1.0, nsteps=100, save_every=0, term_ki
     return out
 
 
+class OMapDesc(C.Structure):
+    _fields_ = [("nstates", C.c_int32), ("save_every", C.c_int32), ("nsave", C.c_int32), ("pad", C.c_int32),
+                ("dt", C.c_double), ("mass", C.c_double), ("nsteps", C.c_int64)]
+
+
+def mapping_gamma(n):
+    """SpinMappingW's gamma = 2 (sqrt(n + 1) - 1) / n (spin_mapping.jl:4-9)."""
+    return 2.0 / n * (np.sqrt(n + 1.0) - 1.0)
+
+
+def _map_desc(n, mass, dt=0.01, nsteps=0, save_every=0):
+    D = OMapDesc()
+    D.nstates, D.save_every, D.dt, D.mass, D.nsteps = n, save_every, dt, mass, nsteps
+    D.nsave = nsteps // save_every + 1 if save_every else 1
+    return D
+
+
+def mapping_sample(n, state, key, traj_ids=None):
+    """Initial mapping variables of pure diabatic states (0-based `state` per trajectory): X, P (ntraj, n)."""
+    st = np.ascontiguousarray(state, dtype=np.int32)
+    ntraj = len(st)
+    X, P = np.zeros((ntraj, n)), np.zeros((ntraj, n))
+    ids = None if traj_ids is None else np.ascontiguousarray(traj_ids, dtype=np.int64)
+    f = lib().oracle_mapping_sample
+    f.restype = None
+    f(C.c_int32(n), C.c_int64(ntraj), _p(st, C.c_int32), C.c_uint64(key), _p(ids, C.c_int64), _p(X), _p(P))
+    return X, P
+
+
+def mapping_energy(model, n, mass, u):
+    """The spin-mapping Hamiltonian at u = [r, v, X(n), P(n)]."""
+    D = _map_desc(n, mass)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    f = lib().oracle_mapping_energy
+    f.restype = C.c_double
+    return f(C.byref(D), C.byref(model), _p(uu))
+
+
+def mapping_motion(model, n, mass, u):
+    """motion! of spin mapping at u = [r, v, X(n), P(n)]: du in the same layout."""
+    D = _map_desc(n, mass)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.zeros_like(uu)
+    f = lib().oracle_mapping_motion
+    f.restype = None
+    f(C.byref(D), C.byref(model), _p(uu), _p(du))
+    return du
+
+
+def mapping_run(model, n, mass, r0, v0, X0, P0, dt=0.01, nsteps=100, save_every=0, nthreads=None):
+    """Spin-mapping trajectories by MInt: final x, v, X, P and per save the diabatic populations (ntraj, nsave,
+    n), kinetic and potential energy."""
+    ntraj = len(r0)
+    D = _map_desc(n, mass, dt, nsteps, save_every)
+    out = dict(x=np.ascontiguousarray(r0, dtype=np.float64).ravel().copy(),
+               v=np.ascontiguousarray(v0, dtype=np.float64).ravel().copy(),
+               X=np.ascontiguousarray(X0, dtype=np.float64).reshape(ntraj, n).copy(),
+               P=np.ascontiguousarray(P0, dtype=np.float64).reshape(ntraj, n).copy(),
+               pop=np.zeros((ntraj, D.nsave, n)), ekin=np.zeros((ntraj, D.nsave)), epot=np.zeros((ntraj, D.nsave)))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_mapping_run
+    f.restype = C.c_int32
+    f(C.byref(D), C.byref(model), C.c_int64(ntraj), _p(out["x"]), _p(out["v"]), _p(out["X"]), _p(out["P"]),
+      _p(out["pop"]), _p(out["ekin"]), _p(out["epot"]), C.c_int32(nthreads))
+    return out
+
+
 def cme_run(method, model, mass, kT, r0, v0, state0, dt=1.0, nsteps=100, mu=0.0, bandwidth=1.0, save_every=0, key=0,
             nthreads=None, cell=None):
     """CME (method 0) / BCME (1) trajectories (cme.jl); r0, v0 (ntraj,), state0 0-based diabatic states.  Returns
