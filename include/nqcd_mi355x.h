@@ -61,7 +61,8 @@ enum nqcd_method {
     NQCD_METHOD_RP_EHRENFEST = 6,
     NQCD_METHOD_CME = 7,          /* classical master equation, cme.jl:63-105 */
     NQCD_METHOD_BCME = 8,         /* broadened CME, cme.jl:115-182 (bandwidth in nqcd_problem_desc) */
-    NQCD_METHOD_CLASSICAL = 9     /* Simulation{Classical}, classical.jl (single-state model, one coordinate) */
+    NQCD_METHOD_CLASSICAL = 9,    /* Simulation{Classical}, classical.jl (single-state model, one coordinate) */
+    NQCD_METHOD_SPIN_MAPPING = 10 /* Simulation{SpinMappingW}, spin_mapping.jl:1-131 (2 to 4 states, one coordinate) */
 };
 
 /* IntegrationAlgorithms (IntegrationAlgorithms.jl:19-56): BABwithTsit5, VerletwithElectronics,
@@ -73,7 +74,8 @@ enum nqcd_algorithm {
     NQCD_ALG_BCB_TSIT5 = 3,
     NQCD_ALG_BCB_WAVEFUNCTION = 4,   /* ring-polymer IESH / EhrenfestNA (bcb_wavefunction.jl) */
     NQCD_ALG_TSIT5 = 5,              /* the nuclear ODE by Tsit5 at a fixed step (CME / BCME, IntegrationAlgorithms.jl:68) */
-    NQCD_ALG_VELOCITY_VERLET = 6     /* OrdinaryDiffEq's VelocityVerlet (Simulation{Classical}, IntegrationAlgorithms.jl:61) */
+    NQCD_ALG_VELOCITY_VERLET = 6,    /* OrdinaryDiffEq's VelocityVerlet (Simulation{Classical}, IntegrationAlgorithms.jl:61) */
+    NQCD_ALG_MINT = 7                /* MInt, the mapping integrator (mint.jl:42-181; select_algorithm, IntegrationAlgorithms.jl:79) */
 };
 
 /* rescaling keyword of Simulation{FSSH} (fssh.jl:41, surface_hopping.jl:219,234-245). */
@@ -245,7 +247,9 @@ enum nqcd_field {
     NQCD_FIELD_SAVE_EPOT = 14,     /* double (nsave, ntraj) classical potential energy per save (same problems) */
     NQCD_FIELD_FINAL_POP_DI = 15,  /* double (n, ntraj) diabatic populations of the final state (fssh.jl:132-142,
                                       ehrenfest.jl:75-83); two-state one-dof FSSH / Ehrenfest */
-    NQCD_FIELD_SAVE_V = 16         /* double (nsave, ntraj) velocity per save (classical dynamics) */
+    NQCD_FIELD_SAVE_V = 16,        /* double (nsave, ntraj) velocity per save (classical dynamics) */
+    NQCD_FIELD_MAP_X = 17,         /* double (n, ntraj) final mapping positions (spin mapping: u.qmap) */
+    NQCD_FIELD_MAP_P = 18          /* double (n, ntraj) final mapping momenta (u.pmap) */
 };
 /* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
  * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD / SAVE_POP_DI (n, nsave, ntraj). */
@@ -299,6 +303,14 @@ int32_t nqcd_mi355x_set_model(nqcd_handle_t* h, int32_t slot, int32_t model_kind
 int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const double* v,
                                  const double* sig_re, const double* sig_im, const int32_t* state,
                                  const int32_t* model_id);
+
+/* Spin mapping: set the mapping positions X and momenta P (n, ntraj) of the loaded trajectories (the
+ * qmap / pmap of DynamicsVariables, spin_mapping.jl:15-33) instead of drawing them in the first step of
+ * nqcd_mi355x_run; X = P = NULL returns to drawing.  Without this call every run draws theta_i = 2 pi u
+ * (u from Philox stream 4 of philox_key at counter (trajectory, i)), radius sqrt(2 + gamma) on the
+ * trajectory's occupied diabatic state and sqrt(gamma) elsewhere, X_i = radius sin(theta_i),
+ * P_i = radius cos(theta_i).  Call after load_initial or sample_initial, with their ntraj. */
+int32_t nqcd_mi355x_load_mapping(nqcd_handle_t* h, int64_t ntraj, const double* X, const double* P);
 
 /* Sample initial conditions on the device from a recognised distribution
  * (RandomSelection, selections.jl:70-99) using Philox stream 0 of `seed`. */

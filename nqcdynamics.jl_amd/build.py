@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libnqcd_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 SOURCES = ["abi.cpp", "fssh2.hip", "ensemble.hip", "spin_boson.hip", "ring_polymer.hip", "iesh.hip", "fssh_nd.hip",
-           "fssh_n.hip", "fssh_n8.hip", "cme.hip", "classical.hip", "mlp.hip", "autotune.cpp"]
+           "fssh_n.hip", "fssh_n8.hip", "cme.hip", "classical.hip", "mapping.hip", "mlp.hip", "autotune.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-I", os.path.join(HERE, "..", "include")]
 

@@ -24,6 +24,7 @@
 #include "fssh_nd.h"
 #include "fssh_n.h"
 #include "classical.h"
+#include "mapping.h"
 #include "cme.h"
 #include "mlp_host.h"
 #include "autotune.h"
@@ -218,12 +219,13 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     const bool ie = d->method == NQCD_METHOD_IESH || d->method == NQCD_METHOD_EHRENFEST_NA;
     const bool cme = d->method == NQCD_METHOD_CME || d->method == NQCD_METHOD_BCME;
     const bool cls = d->method == NQCD_METHOD_CLASSICAL;
-    if (!rp && !ie && !cme && !cls && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states >= 2 &&
+    const bool map = d->method == NQCD_METHOD_SPIN_MAPPING;
+    if (!rp && !ie && !cme && !cls && !map && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states >= 2 &&
                         d->n_states <= kNsMaxStates))
         return fail(h, NQCD_ERR_UNSUPPORTED,
                     "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (2 to 8 states, BABwithTsit5), "
                     "Simulation{AdiabaticIESH|EhrenfestNA} (VerletwithElectronics), Simulation{CME|BCME}, "
-                    "Simulation{Classical} (VelocityVerlet) and "
+                    "Simulation{Classical} (VelocityVerlet), Simulation{SpinMappingW} (MInt) and "
                     "RingPolymerSimulation{Classical|FSSH|Ehrenfest} (BCB / BCBwithTsit5)");
     if (sh_or_eh && d->n_states > 2 && d->ndofs * d->natoms != 1)
         return fail(h, NQCD_ERR_UNSUPPORTED, "FSSH / Ehrenfest with more than two states: one nuclear coordinate");
@@ -251,6 +253,11 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
         int32_t rc = cls_check_problem(h);
         if (rc != NQCD_OK) return rc;
     }
+    if (map) {
+        int32_t rc = map_check_problem(h);
+        if (rc != NQCD_OK) return rc;
+    }
+    h->map_given = false;
     h->cell = CellDesc{};
     h->have_problem = true;
     h->have_initial = h->have_results = false;
@@ -355,7 +362,7 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
         return rc;
     }
     if (!r || !v || ((!sig_re || !sig_im) && h->prob.method != NQCD_METHOD_RPMD && !cme_is_problem(h) &&
-                     !cls_is_problem(h)))
+                     !cls_is_problem(h) && !map_is_problem(h)))
         return fail(h, NQCD_ERR_ARG, "null input array");
     if (cls_is_problem(h)) {
         if (!h->model_set[0]) return fail(h, NQCD_ERR_STATE, "model slot 0 is not set");
@@ -373,6 +380,12 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
     if (state)
         for (int64_t i = 0; i < ntraj; ++i)
             if (state[i] < -1 || state[i] >= n) return fail(h, NQCD_ERR_ARG, "state out of range");
+    if (map_is_problem(h)) {
+        HIPCHK(h, hipSetDevice(h->device));
+        int32_t rc = map_load_initial(h, ntraj, r, v, state, model_id);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
     if (cme_is_problem(h)) {
         if (!r || !v) return fail(h, NQCD_ERR_ARG, "null input array");
         HIPCHK(h, hipSetDevice(h->device));
@@ -436,6 +449,16 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
     return NQCD_OK;
 }
 
+int32_t nqcd_mi355x_load_mapping(nqcd_handle_t* h, int64_t ntraj, const double* X, const double* P) {
+    if (!h) return fail(h, NQCD_ERR_ARG, "null handle");
+    if (!h->have_problem || !map_is_problem(h)) return fail(h, NQCD_ERR_STATE, "set a spin-mapping problem first");
+    if (!h->have_initial) return fail(h, NQCD_ERR_STATE, "load or sample the initial conditions first");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int32_t rc = map_load_variables(h, ntraj, X, P);
+    if (rc == NQCD_OK) h->have_results = false;
+    return rc;
+}
+
 int32_t nqcd_mi355x_sample_fermi_dirac(nqcd_handle_t* h, double beta, const int32_t* available_states,
                                        int32_t n_available, int32_t basis, uint64_t seed) {
     if (!h) return fail(h, NQCD_ERR_ARG, "null argument");
@@ -480,6 +503,11 @@ int32_t nqcd_mi355x_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S,
     }
     if (cme_is_problem(h)) {
         int32_t rc = cme_sample_initial(h, S, seed);
+        if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
+        return rc;
+    }
+    if (map_is_problem(h)) {
+        int32_t rc = map_sample_initial(h, S, seed);
         if (rc == NQCD_OK) { h->have_initial = true; h->sb_initial = false; h->have_results = false; }
         return rc;
     }
@@ -596,6 +624,11 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
             return fail(h, NQCD_ERR_ARG, "termination dof out of range");
         int32_t rc = cme_run(h, t0, nsteps, term, key);
         if (rc != NQCD_OK) return rc;
+    } else if (map_is_problem(h)) {
+        if (term && term->kind != NQCD_TERM_NONE)
+            return fail(h, NQCD_ERR_UNSUPPORTED, "spin mapping runs to t1 (no termination predicate)");
+        int32_t rc = map_run(h, t0, nsteps, key);
+        if (rc != NQCD_OK) return rc;
     } else if (ns_is_problem(h)) {
         if (h->prob.schedule != NQCD_SCHED_PERSISTENT)
             return fail(h, NQCD_ERR_UNSUPPORTED, "n-state FSSH / Ehrenfest run on the persistent schedule");
@@ -678,6 +711,10 @@ int32_t nqcd_mi355x_run(nqcd_handle_t* h, double t0, double t1, int32_t save_eve
     float ms = 0.0f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.kernel_ms = ms;
+    // every run path writes the steps taken per trajectory; a path that did not is a library error, not
+    // something to read
+    if (!h->steps.p || h->steps.bytes < sizeof(int32_t) * (size_t)h->ntraj)
+        return fail(h, NQCD_ERR_DEVICE, "internal: the run did not record the steps per trajectory");
     HIPCHK(h, h->sum64.ensure(sizeof(long long)));
     HIPCHK(h, launch_sum_i32(h->steps.as<int32_t>(), h->ntraj, h->sum64.as<long long>(), h->stream));
     long long tot = 0;
@@ -716,6 +753,18 @@ int32_t reduce_plan(nqcd_handle_t* h, int32_t kind, int32_t red, const double* o
         if (kind == NQCD_OUT_ADIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_ad.as<double>(); }
         else if (kind == NQCD_OUT_DIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_di.as<double>(); }
         else return fail(h, NQCD_ERR_UNSUPPORTED, "IESH / EhrenfestNA outputs: ADIABATIC_POPULATION, DIABATIC_POPULATION");
+    }
+    if (map_is_problem(h)) {
+        if (kind == NQCD_OUT_DIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_di.as<double>(); }
+        else if (kind == NQCD_OUT_KINETIC_ENERGY || kind == NQCD_OUT_TOTAL_ENERGY) { rows = h->nsave; src = h->save_ekin.as<double>(); }
+        else return fail(h, NQCD_ERR_UNSUPPORTED, "spin-mapping outputs: DIABATIC_POPULATION, KINETIC_ENERGY, TOTAL_ENERGY");
+        if (len != rows) return fail(h, NQCD_ERR_ARG, "len does not match the output size");
+        P = ReducePlan{};
+        P.rows = rows;
+        P.src = src;
+        P.src2 = kind == NQCD_OUT_TOTAL_ENERGY ? h->save_epot.as<double>() : nullptr;
+        P.scale = (red == NQCD_RED_MEAN) ? 1.0 / (double)nt : 1.0;
+        return NQCD_OK;
     }
     if (cme_is_problem(h) || cls_is_problem(h)) {
         const bool cls = cls_is_problem(h);
@@ -837,6 +886,7 @@ int32_t nqcd_mi355x_download(nqcd_handle_t* h, int32_t field, void* out, int64_t
     HIPCHK(h, hipSetDevice(h->device));
     if (is_ie(h)) return ie_download(h, field, out, bytes);
     if (ns_is_problem(h)) return ns_download(h, field, out, bytes);
+    if (map_is_problem(h)) return map_download(h, field, out, bytes);
     if (cme_is_problem(h) && field != NQCD_FIELD_R && field != NQCD_FIELD_V && field != NQCD_FIELD_STATE &&
         field != NQCD_FIELD_STEPS && field != NQCD_FIELD_NHOPS && field != NQCD_FIELD_SAVE_R &&
         field != NQCD_FIELD_SAVE_EKIN && field != NQCD_FIELD_SAVE_EPOT && field != NQCD_FIELD_SAVE_STATE)

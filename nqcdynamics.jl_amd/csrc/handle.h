@@ -62,6 +62,8 @@ struct nqcd_handle {
     int32_t rp_state = 0;
     // IESH / EhrenfestNA
     bool ie_psi_given = false;
+    // spin mapping: mapping variables uploaded by nqcd_mi355x_load_mapping (in sig0) rather than drawn
+    bool map_given = false;
     // diatom-surface model parameters
     double nd_params[16]{};
     // n-state Morse tables per model slot, their device copy, and the diabatic initial state of a

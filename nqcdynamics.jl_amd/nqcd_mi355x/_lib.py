@@ -17,10 +17,10 @@ ABI_VERSION = 2
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_DEVICE, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
 # enum nqcd_method
 (METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST,
- METHOD_CME, METHOD_BCME, METHOD_CLASSICAL) = range(10)
+ METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING) = range(11)
 # enum nqcd_algorithm
 (ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION, ALG_TSIT5,
- ALG_VELOCITY_VERLET) = range(7)
+ ALG_VELOCITY_VERLET, ALG_MINT) = range(8)
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = range(3)
 PREC_FP64, PREC_FP32_NUCLEI = range(2)
 SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
@@ -38,7 +38,7 @@ CELL_NONE, CELL_BOUNDARY, CELL_TERMINATE = range(3)
 RED_MEAN, RED_SUM = range(2)
 (FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
  FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID, FIELD_FALLBACKS, FIELD_SAVE_R, FIELD_SAVE_EKIN,
- FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V) = range(17)
+ FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V, FIELD_MAP_X, FIELD_MAP_P) = range(19)
 
 EXPORTED_SYMBOLS = (
     "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
@@ -47,7 +47,7 @@ EXPORTED_SYMBOLS = (
     "nqcd_mi355x_surrogate_train", "nqcd_mi355x_surrogate_eval", "nqcd_mi355x_surrogate_gradient",
     "nqcd_mi355x_surrogate_set_params", "nqcd_mi355x_surrogate_get_params", "nqcd_mi355x_test_determinant",
     "nqcd_mi355x_sample_fermi_dirac", "nqcd_mi355x_autotune", "nqcd_mi355x_reduce_noalloc",
-    "nqcd_mi355x_test_broadening", "nqcd_mi355x_set_cell",
+    "nqcd_mi355x_test_broadening", "nqcd_mi355x_set_cell", "nqcd_mi355x_load_mapping",
 )
 
 
@@ -114,6 +114,7 @@ def lib():
     L.nqcd_mi355x_set_model.argtypes = [H, C.c_int32, C.c_int32, dp, C.c_int64]
     L.nqcd_mi355x_set_cell.argtypes = [H, C.c_int32, C.c_int32, dp, ip]
     L.nqcd_mi355x_load_initial.argtypes = [H, C.c_int64, dp, dp, dp, dp, ip, ip]
+    L.nqcd_mi355x_load_mapping.argtypes = [H, C.c_int64, dp, dp]
     L.nqcd_mi355x_sample_initial.argtypes = [H, C.POINTER(SamplerDesc), C.c_uint64]
     L.nqcd_mi355x_sample_fermi_dirac.argtypes = [H, C.c_double, ip, C.c_int32, C.c_int32, C.c_uint64]
     L.nqcd_mi355x_run.argtypes = [H, C.c_double, C.c_double, C.c_int32, C.POINTER(TermDesc), C.c_uint64]
@@ -255,6 +256,16 @@ class Ensemble:
         self._check(self._L.nqcd_mi355x_load_initial(self._h, ntraj, _dp(r), _dp(v), _dp(pr), _dp(pi), _ip(occ), None))
         self.ntraj = ntraj
 
+    def load_mapping(self, X, P):
+        """Spin mapping: mapping positions X and momenta P (ntraj, n) of the loaded trajectories (None for
+        both: draw them in the run from the run's key)."""
+        if X is None and P is None:
+            self._check(self._L.nqcd_mi355x_load_mapping(self._h, self.ntraj, None, None))
+            return
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        self._check(self._L.nqcd_mi355x_load_mapping(self._h, X.shape[0], _dp(X), _dp(P)))
+
     def sample_initial(self, kind, ntraj, params, seed, initial_state=0, state_basis=0, n_models=1):
         p = np.ascontiguousarray(params, dtype=np.float64)
         s = SamplerDesc()
@@ -322,7 +333,8 @@ class Ensemble:
                   FIELD_SAVE_STATE: ((nt, nsave), np.int32), FIELD_FALLBACKS: ((nt,), np.int32),
                   FIELD_SAVE_R: ((nt, nsave, D), np.float64), FIELD_SAVE_EKIN: ((nt, nsave), np.float64),
                   FIELD_SAVE_EPOT: ((nt, nsave), np.float64), FIELD_FINAL_POP_DI: ((nt, n), np.float64),
-                  FIELD_SAVE_V: ((nt, nsave), np.float64)}
+                  FIELD_SAVE_V: ((nt, nsave), np.float64), FIELD_MAP_X: ((nt, n), np.float64),
+                  FIELD_MAP_P: ((nt, n), np.float64)}
         if getattr(self, "method", None) in (METHOD_CME, METHOD_BCME, METHOD_CLASSICAL):  # one coordinate
             shapes[FIELD_SAVE_R] = ((nt, nsave), np.float64)
         shape, dtype = shapes[field]

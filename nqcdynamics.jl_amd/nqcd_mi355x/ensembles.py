@@ -14,7 +14,7 @@ from .distributions import Adiabatic, DynamicalDistribution, FermiDiracState, Pr
 from .outputs import (FileReduction, MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction,
                       SumReduction, output_name)
 from .simulations import (AdiabaticIESH, BCME, CME, Classical, Ehrenfest, EhrenfestNA, FSSH, PeriodicCell,
-                          RingPolymerSimulation, Simulation)
+                          RingPolymerSimulation, Simulation, SpinMappingW)
 
 log = logging.getLogger("nqcd_mi355x")
 
@@ -80,7 +80,8 @@ def _callbacks(callback):
 
 def _kind(sim):
     """'sh' (FSSH/Ehrenfest), 'ie' (IESH/EhrenfestNA, also as ring polymers with BCBWavefunction), 'rp'
-    (ring polymers), 'cme' (CME / BCME) or 'cl' (classical dynamics); raises for the rest."""
+    (ring polymers), 'cme' (CME / BCME), 'cl' (classical dynamics) or 'map' (spin mapping); raises for the
+    rest."""
     m = sim.method
     if isinstance(sim, RingPolymerSimulation):
         if isinstance(m, (Classical, FSSH, Ehrenfest)):
@@ -95,6 +96,8 @@ def _kind(sim):
         return "cme"
     elif isinstance(m, Classical) and sim.model.nstates == 1 and sim.ndofs * sim.natoms == 1:
         return "cl"
+    elif isinstance(m, SpinMappingW) and sim.ndofs * sim.natoms == 1:
+        return "map"
     raise ValueError(f"EnsembleMI355X does not support {type(sim).__name__}{{{type(m).__name__}}}")
 
 
@@ -121,6 +124,11 @@ def _problem(E, sim, dt):
             raise ValueError("classical dynamics run in FP64")
         E.set_problem(L.METHOD_CLASSICAL, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
                       algorithm=L.ALG_VELOCITY_VERLET, temperature=sim.temperature, **common)
+    elif kind == "map":
+        if sim.precision != "fp64":
+            raise ValueError("spin mapping runs in FP64")
+        E.set_problem(L.METHOD_SPIN_MAPPING, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=L.ALG_MINT, temperature=sim.temperature, **common)
     elif kind == "cme":
         if sim.precision != "fp64":
             raise ValueError("CME / BCME run in FP64")
@@ -261,9 +269,34 @@ def _initial_cl(E, sim, dist, trajectories, selection, seed):
         raise ValueError(f"unsupported distribution {type(nuc).__name__}")
 
 
+def _initial_map(E, sim, dist, trajectories, selection, seed):
+    """Spin mapping: a diabatic PureState (DynamicsVariables(sim, v, r, ::PureState{Diabatic}),
+    spin_mapping.jl:15-33) and one coordinate, sampled on the device from a Gaussian distribution or
+    uploaded; the mapping variables are drawn on the device in the run."""
+    el = _electronic(dist)
+    n = sim.model.nstates
+    if not el.diabatic:
+        raise ValueError("spin mapping starts from a PureState in the Diabatic basis")
+    if el.state < 1 or el.state > n:
+        raise ValueError(f"electronic state {el.state} out of range 1..{n}")
+    nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    conf = _configurations(nuc, trajectories, selection)
+    if conf is not None:
+        r, v = conf
+        E.load_initial(r, v, None, state=np.full(r.shape[0], el.state - 1, np.int32))
+    elif isinstance(nuc, DynamicalDistribution):
+        mr, sr, mv, sv = nuc.moments()
+        E.sample_initial(L.SAMPLER_GAUSSIAN, trajectories, [mr[0], sr[0], mv[0], sv[0]], seed,
+                         initial_state=el.state - 1, state_basis=1)
+    else:
+        raise ValueError(f"unsupported distribution {type(nuc).__name__}")
+
+
 def _initial_conditions(E, sim, dist, trajectories, selection, seed):
     """Sample on the device where the distribution is recognised, otherwise upload."""
     kind = _kind(sim)
+    if kind == "map":
+        return _initial_map(E, sim, dist, trajectories, selection, seed)
     if kind == "ie":
         return _initial_ie(E, sim, dist, trajectories, selection, seed)
     if kind == "rp":
@@ -449,6 +482,13 @@ def _per_traj_fields(E, sim, t0, dt, save_every, nsteps):
                  "save_r": [x.reshape(shp, order="F") for x in sr[i]],
                  "save_v": [x.reshape(shp, order="F") for x in sv[i]], "ekin": ek[i], "epot": ep[i],
                  "t0": t0, "dt": dt} for i in range(E.ntraj)]
+    if kind == "map":
+        r, v, X, P = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_MAP_X), E.download(L.FIELD_MAP_P)
+        pd, ek, ep = E.download(L.FIELD_SAVE_POP_DI), E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
+        shp = sim.size()
+        return [{"r": r[i].reshape(shp, order="F"), "v": v[i].reshape(shp, order="F"), "qmap": X[i], "pmap": P[i],
+                 "pop_di": pd[i], "ekin": ek[i], "epot": ep[i], "steps": nsteps, "t0": t0, "dt": dt}
+                for i in range(E.ntraj)]
     if kind == "cme":
         r, v, st = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_STATE)
         nh, stp = E.download(L.FIELD_NHOPS), E.download(L.FIELD_STEPS)

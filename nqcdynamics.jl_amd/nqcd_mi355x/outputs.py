@@ -53,6 +53,7 @@ OutputVelocity = _FnOutput("OutputVelocity", lambda r: list(r["save_v"]), L.OUT_
 # OutputFinal (DynamicsOutputs.jl:222): final dynamics variables.
 OutputFinal = _FnOutput("OutputFinal", lambda r: {"v": r["v"], "r": r["r"], **(
     {"sigma": r["sigma"], "state": r["state"] + 1} if r.get("sigma") is not None else
+    {"qmap": r["qmap"], "pmap": r["pmap"]} if "qmap" in r else
     {"state": r["state"] + 1} if "state" in r else {})})
 
 

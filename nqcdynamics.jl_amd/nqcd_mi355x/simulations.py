@@ -134,9 +134,33 @@ class BCME(Method):
             raise ValueError("bandwidth must be positive")
 
 
+class SpinMappingW(Method):
+    """Spin-mapping method (spin_mapping.jl:1-13) for n electronic states: gamma = 2 (sqrt(n + 1) - 1) / n,
+    R^2 = 2 sqrt(n + 1); integrated by MInt (IntegrationAlgorithms.jl:79).  Simulation[SpinMappingW](atoms,
+    model) takes n from the model, as Simulation{SpinMappingW} does (spin_mapping.jl:11-13)."""
+    method_code = L.METHOD_SPIN_MAPPING
+    algorithm = L.ALG_MINT
+    rescaling_code = L.RESCALE_STANDARD
+
+    def __init__(self, n_states=None):
+        self.n_states = None
+        if n_states is not None:
+            self.bind(n_states)
+
+    def bind(self, n_states):
+        n = int(n_states)
+        if n < 1:
+            raise ValueError("n_states must be positive")
+        self.n_states = n
+        self.gamma = 2.0 / n * (np.sqrt(n + 1.0) - 1.0)
+        self.R2 = 2.0 * np.sqrt(n + 1.0)
+
+
 class AbstractSimulation:
     def __init__(self, method, atoms, model, temperature=0.0, precision="fp64", cell=None, **method_kwargs):
         self.method = method(**method_kwargs) if isinstance(method, type) else method
+        if isinstance(self.method, SpinMappingW) and self.method.n_states is None:
+            self.method.bind(model.nstates)
         self.cell = InfiniteCell() if cell is None else cell
         if not isinstance(self.cell, (InfiniteCell, PeriodicCell)):
             raise ValueError("cell must be a PeriodicCell or an InfiniteCell")
