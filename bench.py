@@ -107,7 +107,7 @@ def cpu_baseline(L, gpu_states, nthreads, samples):
            "state_agreement_sample": agree}
     checks = {}
     for name, fn in (("spin_boson", _check_spin_boson), ("ring_polymer", _check_ring_polymer), ("iesh", _check_iesh),
-                     ("learned", _check_learned)):
+                     ("learned", _check_learned), ("mapping", _check_mapping)):
         if name in samples:
             t0 = time.perf_counter()
             checks[name] = fn(O, samples[name], nthreads)
@@ -307,6 +307,60 @@ def ring_polymer_extra(L):
     return res, sample
 
 
+# three Morse curves with one wall (D = 0.02, beta = 0.5, r = 4) offset by 0, 0.004 and 0.008, Gaussian-coupled:
+# the mapping Hamiltonian weighs V_ii by (X_i^2 + P_i^2 - gamma) / 2, which can be negative, so walls of different
+# steepness make its potential unbounded below (trajectories then run off); with one wall shape it stays bounded
+MAP_MORSE3 = [0.02, 0.5, 4.0, 0.0, 0.02, 0.5, 4.0, 0.004, 0.02, 0.5, 4.0, 0.008,
+              0.003, 0.5, 4.0, 0.002, 0.5, 4.2, 0.003, 0.5, 4.4]
+
+
+def mapping_extra(L):
+    """Spin mapping (SpinMappingW) integrated by MInt, FP64, 1e6 trajectories for 1000 steps with the
+    diabatic populations and energies saved every 100 steps: Tully's model 1 (two states, mass 2000, x0 ~
+    N(-6, 0.5), p0 ~ N(15, 1), dt = 1, occupied diabat 1) and three coupled Morse curves (MAP_MORSE3, mass
+    20000, x0 ~ N(4.1, 0.1), v0 ~ N(0, 2e-4), dt = 2, occupied diabat 1).  The mapping variables are drawn on the
+    device.  Rates are trajectory-steps/s of kernel time."""
+    ntraj, nsteps, se = 1_000_000, 1000, 100
+    res, sample = {}, {}
+    cases = (("tully1_n2", L.MODEL_TULLY1, [0.01, 1.6, 0.005, 1.0], 2, 2000.0, [-6.0, 0.5, 15.0 / 2000.0, 1.0 / 2000.0], 1.0),
+             ("morse_n3", L.MODEL_MORSE_N, [3] + MAP_MORSE3, 3, 20000.0, [4.1, 0.1, 0.0, 2e-4], 2.0))
+    for name, kind, prm, n, mass, gauss, dt in cases:
+        E = L.Ensemble(0)
+        E.set_problem(L.METHOD_SPIN_MAPPING, n, 1, 1, [mass], dt, algorithm=L.ALG_MINT)
+        E.set_model(0, kind, prm)
+        E.sample_initial(L.SAMPLER_GAUSSIAN, ntraj, gauss, KEY, initial_state=0, state_basis=1)
+        E.run(0.0, 0.0, KEY)
+        m = 256   # first trajectories handed to the oracle check of the CPU-baseline leg
+        sample[name] = {"kind": kind, "prm": prm, "n": n, "mass": mass, "dt": dt, "nsteps": nsteps, "save_every": se,
+                        "r0": E.download(L.FIELD_R)[:m].copy(), "v0": E.download(L.FIELD_V)[:m].copy()}
+        st = E.run(0.0, dt * nsteps, KEY, save_every=se)
+        pop = E.reduce(L.OUT_DIABATIC_POPULATION, L.RED_MEAN, n * E.nsave).reshape(E.nsave, n)
+        sample[name]["pop"] = E.download(L.FIELD_SAVE_POP_DI)[:m].copy()
+        E.close()
+        rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+        res[name] = {"ntraj": ntraj, "nsteps": nsteps, "dt": dt, "nstates": n, "kernel_ms": st["kernel_ms"],
+                     "traj_steps_per_s": rate, "grid": st["grid"], "block": st["block"],
+                     "final_diabatic_population": [float(x) for x in pop[-1]]}
+        _progress(f"spin mapping {name}: {rate:.3e} trajectory-steps/s")
+    return res, sample
+
+
+def _check_mapping(O, S, nthreads):
+    """Spin mapping on the first trajectories of each workload against the FP64 oracle from the same initial
+    conditions and mapping-variable draws: largest per-save diabatic population difference."""
+    out = {}
+    for name, d in S.items():
+        n = d["n"]
+        m = (O.make_model(O.MODEL_TULLY1, 2, 1, d["prm"]) if d["kind"] == 0 else
+             O.make_model(O.MODEL_MORSE_N, n, 1, d["prm"][1:]))
+        k = len(d["r0"])
+        X0, P0 = O.mapping_sample(n, np.zeros(k, np.int32), KEY)
+        c = O.mapping_run(m, n, d["mass"], d["r0"][:, 0], d["v0"][:, 0], X0, P0, dt=d["dt"], nsteps=d["nsteps"],
+                          save_every=d["save_every"], nthreads=nthreads)
+        out[name] = {"trajectories": k, "max_abs_err_population": float(np.abs(c["pop"] - d["pop"]).max())}
+    return out
+
+
 TRAIN_STEPS = 40000
 TRAIN_BATCH = 2048
 
@@ -411,7 +465,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-port comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the IESH and surrogate measurements")
-    ap.add_argument("--only", choices=["iesh", "spin_boson", "ring_polymer", "learned"],
+    ap.add_argument("--only", choices=["iesh", "spin_boson", "ring_polymer", "learned", "mapping"],
                     help="run one of the other workloads alone and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -432,7 +486,7 @@ def main():
     from nqcd_mi355x import _lib as L
     if args.only:
         fn = {"iesh": iesh_extra, "spin_boson": spin_boson_extra, "ring_polymer": ring_polymer_extra,
-              "learned": surrogate_extra}[args.only]
+              "learned": surrogate_extra, "mapping": mapping_extra}[args.only]
         print(json.dumps({args.only: fn(L)[0]}))
         return
 
@@ -525,6 +579,7 @@ def main():
         out["spin_boson"], samples["spin_boson"] = spin_boson_extra(L)
         out["ring_polymer"], samples["ring_polymer"] = ring_polymer_extra(L)
         out["learned"], samples["learned"] = surrogate_extra(L)
+        out["mapping"], samples["mapping"] = mapping_extra(L)
     if not args.no_cpu:
         nthreads = min(CPU_THREADS, os.cpu_count() or 1)
         _progress(f"cpu baseline on {nthreads} threads and on one core")

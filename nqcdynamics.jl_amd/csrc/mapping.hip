@@ -124,18 +124,18 @@ __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
             double vd[NM], vo[NP > 0 ? NP : 1], dd[NM], dof[NP > 0 ? NP : 1];
             ns_model<NM, double>(L, rt, vd, vo, dd, dof);
             double w[NM], Z[NM * NM];
-            ns_eigen<NM>(NM, vd, vo, w, Z, false);
+            if constexpr (NM == 2) eigen2(vd[0], vo[0], vd[1], w[0], w[1], Z, false);
+            else ns_eigen<NM>(NM, vd, vo, w, Z, false);
             // mapping variables in the eigenbasis, rotated by exp(-i w dt)
-            double a[NM], b[NM];
+            double a[NM], b[NM], sk[NM], ck[NM];
 #pragma unroll
             for (int k = 0; k < NM; ++k) {
                 double sa = 0.0, sb = 0.0;
 #pragma unroll
                 for (int i = 0; i < NM; ++i) { sa += Z[i * NM + k] * X[i]; sb += Z[i * NM + k] * P[i]; }
-                double s, c;
-                sincos(w[k] * dt, &s, &c);
-                a[k] = c * sa + s * sb;
-                b[k] = c * sb - s * sa;
+                sincos(w[k] * dt, &sk[k], &ck[k]);
+                a[k] = ck[k] * sa + sk[k] * sb;
+                b[k] = ck[k] * sb - sk[k] * sa;
             }
 #pragma unroll
             for (int i = 0; i < NM; ++i) {
@@ -155,9 +155,9 @@ __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
                 tr += dd[i];
 #pragma unroll
                 for (int j = i + 1; j < NM; ++j) {
+                    // sin and cos of (w_j - w_i) dt from those of the rotation angles
                     const double dl = w[j] - w[i];
-                    double s, c;
-                    sincos(dl * dt, &s, &c);
+                    const double s = sk[j] * ck[i] - ck[j] * sk[i], c = ck[j] * ck[i] + sk[j] * sk[i];
                     const double wij = Wo[pidx<NM>(i, j)] / dl;
                     f += s * wij * (a[i] * a[j] + b[i] * b[j]) - (1.0 - c) * wij * (a[i] * b[j] - a[j] * b[i]);
                 }

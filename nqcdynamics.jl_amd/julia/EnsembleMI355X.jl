@@ -9,6 +9,9 @@
 #   RingPolymerSimulation{Classical}                              BCB
 #   RingPolymerSimulation{FSSH} / RingPolymerSimulation{Ehrenfest} BCBwithTsit5
 #   RingPolymerSimulation{AdiabaticIESH} / {EhrenfestNA}          BCBWavefunction
+#   Simulation{CME} / Simulation{BCME}                            Tsit5 at fixed dt
+#   Simulation{Classical}                                         VelocityVerlet
+#   Simulation{SpinMappingW}                                      MInt
 #
 # Not executed in this repository (no Julia toolchain in the build image); the same C ABI is
 # exercised by the Python mirror (nqcd_mi355x/) and its tests.  Load it next to NQCDynamics:
@@ -26,7 +29,8 @@ module EnsembleMI355XBackend
 using SciMLBase
 using NQCDynamics
 using NQCDynamics: DynamicsMethods, DynamicsUtils, Ensembles, RingPolymers
-using NQCDynamics.DynamicsMethods: SurfaceHoppingMethods, EhrenfestMethods, ClassicalMethods, IntegrationAlgorithms
+using NQCDynamics.DynamicsMethods: SurfaceHoppingMethods, EhrenfestMethods, ClassicalMethods, IntegrationAlgorithms,
+      MappingVariableMethods
 using NQCModels
 using NQCBase: NQCBase
 using NQCDistributions: NQCDistributions
@@ -38,15 +42,15 @@ const libnqcd = get(ENV, "NQCD_MI355X_LIB", joinpath(@__DIR__, "..", "libnqcd_mi
 
 # --- codes of include/nqcd_mi355x.h ------------------------------------------------------------
 const METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST,
-      METHOD_CME, METHOD_BCME, METHOD_CLASSICAL = Int32.(0:9)
+      METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING = Int32.(0:10)
 const ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION, ALG_TSIT5,
-      ALG_VELOCITY_VERLET = Int32.(0:6)
+      ALG_VELOCITY_VERLET, ALG_MINT = Int32.(0:7)
 const MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC,
       MODEL_NEWNS_ANDERSON, MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE, MODEL_MORSE_N, MODEL_ANDERSON_HOLSTEIN_1 = Int32.(0:10)
 const FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS,
       FIELD_SAVE_POP_AD, FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID = Int32.(0:10)
-const FIELD_SAVE_R, FIELD_SAVE_EKIN, FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V =
-      Int32(12), Int32(13), Int32(14), Int32(15), Int32(16)
+const FIELD_SAVE_R, FIELD_SAVE_EKIN, FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V, FIELD_MAP_X, FIELD_MAP_P =
+      Int32(12), Int32(13), Int32(14), Int32(15), Int32(16), Int32(17), Int32(18)
 const OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION,
       OUT_FINAL_ADIABATIC_POPULATION, OUT_CENTROID_POSITION, OUT_KINETIC_ENERGY, OUT_TOTAL_ENERGY,
       OUT_SCATTERING_1D_DIABATIC, OUT_POSITION, OUT_VELOCITY = Int32.(0:10)
@@ -113,6 +117,7 @@ method_code(::EhrenfestMethods.EhrenfestNA) = METHOD_EHRENFEST_NA
 method_code(::SurfaceHoppingMethods.CME) = METHOD_CME
 method_code(::SurfaceHoppingMethods.BCME) = METHOD_BCME
 method_code(::ClassicalMethods.Classical) = METHOD_CLASSICAL      # Simulation{Classical}: VelocityVerlet
+method_code(::MappingVariableMethods.SpinMappingW) = METHOD_SPIN_MAPPING   # Simulation{SpinMappingW}: MInt
 method_code(m) = unsupported(m)
 function method_code(sim::NQCDynamics.RingPolymerSimulation)
     m = sim.method
@@ -155,7 +160,9 @@ function problem_desc(sim, alg, dt, ensemblealg, masses)
     W = method == METHOD_BCME ? Float64(sim.method.bandwidth) : 0.0
     mu = cme ? Float64(NQCModels.fermilevel(sim)) : 0.0
     # Simulation{Classical}: VelocityVerlet (classical.jl:86)
-    algc = method == METHOD_CLASSICAL ? ALG_VELOCITY_VERLET : cme ? ALG_TSIT5 : algorithm_code(alg)
+    # Simulation{SpinMappingW}: MInt (IntegrationAlgorithms.jl:79)
+    algc = method == METHOD_CLASSICAL ? ALG_VELOCITY_VERLET : method == METHOD_SPIN_MAPPING ? ALG_MINT :
+           cme ? ALG_TSIT5 : algorithm_code(alg)
     ProblemDesc(method, algc, NQCModels.nstates(sim), size(sim)[1], size(sim)[2],
         rp ? RingPolymers.nbeads(sim) : 1, ie ? NQCModels.nelectrons(sim) : 0, cme ? Int32(2) : rescaling_code(sim.method),
         ensemblealg.fp32_nuclei ? 1 : 0, ensemblealg.persistent ? 0 : 1, 5, dec, pointer(masses), dt,
@@ -231,7 +238,7 @@ function device_sampler(prob, sim, model_kind)
     el = d.electronic
     el isa NQCDistributions.PureState || return nothing
     mc = method_code(sim)
-    mc in (METHOD_FSSH, METHOD_EHRENFEST, METHOD_CME, METHOD_BCME) || return nothing
+    mc in (METHOD_FSSH, METHOD_EHRENFEST, METHOD_CME, METHOD_BCME, METHOD_SPIN_MAPPING) || return nothing
     nuc = d.nuclear
     n = prod(size(sim))
     mv = gaussian_moments(nuc.velocity, n); mr = gaussian_moments(nuc.position, n)
@@ -240,6 +247,10 @@ function device_sampler(prob, sim, model_kind)
     if mc in (METHOD_CME, METHOD_BCME)     # the diabatic state is the discrete state (cme.jl:20-24)
         basis == 1 || throw(ArgumentError("CME / BCME start from a PureState in the Diabatic basis"))
         return (SAMPLER_GAUSSIAN, Float64[mr[1][1], mr[2][1], mv[1][1], mv[2][1]], Int32(el.state - 1), Int32(0))
+    end
+    if mc == METHOD_SPIN_MAPPING           # mapping variables drawn on the device in the run (spin_mapping.jl:15-33)
+        basis == 1 || throw(ArgumentError("spin mapping starts from a PureState in the Diabatic basis"))
+        return (SAMPLER_GAUSSIAN, Float64[mr[1][1], mr[2][1], mv[1][1], mv[2][1]], Int32(el.state - 1), Int32(1))
     end
     if model_kind == MODEL_SPIN_BOSON      # params: mu_r(nb), sd_r(nb), mu_v(nb), sd_v(nb)
         return (SAMPLER_HARMONIC_WIGNER, vcat(mr[1], mr[2], mv[1], mv[2]), Int32(el.state - 1), basis)
@@ -267,6 +278,18 @@ function load_initial!(h, prob, sim, trajectories, seed, model_kind)
         check(h, ccall((:nqcd_mi355x_load_initial, libnqcd), Int32,
             (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
             h, trajectories, r, v, C_NULL, C_NULL, C_NULL, C_NULL))
+        return
+    end
+    if method_code(sim) == METHOD_SPIN_MAPPING   # (v, r, qmap, pmap) of DynamicsVariables (spin_mapping.jl:15-33)
+        n = NQCModels.nstates(sim)
+        X = reduce(hcat, [vec(MappingVariableMethods.get_mapping_positions(u)) for u in us])
+        P = reduce(hcat, [vec(MappingVariableMethods.get_mapping_momenta(u)) for u in us])
+        st = zeros(Int32, trajectories)   # the occupied state only seeds device draws; the variables are given
+        check(h, ccall((:nqcd_mi355x_load_initial, libnqcd), Int32,
+            (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
+            h, trajectories, r, v, C_NULL, C_NULL, st, C_NULL))
+        check(h, ccall((:nqcd_mi355x_load_mapping, libnqcd), Int32, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}),
+            h, trajectories, X, P))
         return
     end
     if method_code(sim) in (METHOD_CME, METHOD_BCME)   # SurfaceHoppingVariables(r, v, state) only (cme.jl:20-24)
@@ -452,10 +475,12 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         ie = method_code(sim) in (METHOD_IESH, METHOD_EHRENFEST_NA)
         cme = method_code(sim) in (METHOD_CME, METHOD_BCME)
         cls = method_code(sim) == METHOD_CLASSICAL
-        two_state = !ie && !cme && !cls && !(sim isa NQCDynamics.RingPolymerSimulation) && n == 2
+        spinmap = method_code(sim) == METHOD_SPIN_MAPPING
+        two_state = !ie && !cme && !cls && !spinmap && !(sim isa NQCDynamics.RingPolymerSimulation) && n == 2
         device_reducible(f) = begin
             nm = output_name(f)
             cls && return nm in (:OutputKineticEnergy, :OutputTotalEnergy, :OutputPosition, :OutputVelocity)
+            spinmap && return nm in (:OutputDiabaticPopulation, :OutputKineticEnergy, :OutputTotalEnergy)
             cme && return nm in (:OutputKineticEnergy, :OutputPosition) ||
                           (nm === :OutputTotalEnergy && method_code(sim) == METHOD_CME)
             nm in (:OutputAdiabaticPopulation, :OutputDiabaticPopulation, :OutputCentroidPosition,
@@ -493,7 +518,7 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         # handed to the problem's own reduction functor (Sort*, FileReduction, ...)
         nst = ie ? d.n_electrons : 1
         wants_energy = any(f -> output_name(f) in (:OutputKineticEnergy, :OutputTotalEnergy), outputs)
-        state = cls ? zeros(Int32, nst, trajectories) : download(h[], FIELD_STATE, Int32, nst, trajectories)
+        state = cls || spinmap ? zeros(Int32, nst, trajectories) : download(h[], FIELD_STATE, Int32, nst, trajectories)
         # adiabatic populations of the final state: one-hot of the active state (FSSH, fssh.jl:144-148),
         # occupied orbitals (IESH, iesh.jl:370-374), diag(sigma) (Ehrenfest, ehrenfest.jl:70-73)
         final_ad = zeros(n, trajectories)
@@ -510,13 +535,13 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         final_di = two_state &&
                    any(f -> output_name(f) === :OutputStateResolvedScattering1D && f.type === :diabatic, outputs) ?
                    download(h[], FIELD_FINAL_POP_DI, Float64, n, trajectories) : zeros(0, 0)
-        elec = d.method != METHOD_RPMD && !cme && !cls  # RPMD, CME / BCME and classical: no density save slots
+        elec = d.method != METHOD_RPMD && !cme && !cls && !spinmap  # RPMD, CME / BCME, classical, spin mapping: no density
         res = DeviceResults(n, nsave, size(sim),
             download(h[], FIELD_R, Float64, prod(size(sim)), trajectories), download(h[], FIELD_V, Float64, prod(size(sim)), trajectories),
             state, download(h[], FIELD_STEPS, Int32, trajectories),
-            cls ? zeros(Int32, trajectories) : download(h[], FIELD_NHOPS, Int32, trajectories),
+            cls || spinmap ? zeros(Int32, trajectories) : download(h[], FIELD_NHOPS, Int32, trajectories),
             elec ? download(h[], FIELD_SAVE_POP_AD, Float64, n, ndev, trajectories)[:, 1:nsave, :] : zeros(n, nsave, trajectories),
-            elec ? download(h[], FIELD_SAVE_POP_DI, Float64, n, ndev, trajectories)[:, 1:nsave, :] : zeros(n, nsave, trajectories),
+            elec || spinmap ? download(h[], FIELD_SAVE_POP_DI, Float64, n, ndev, trajectories)[:, 1:nsave, :] : zeros(n, nsave, trajectories),
             ie || !(elec || cme) ? zeros(Int32, nsave, trajectories) : download(h[], FIELD_SAVE_STATE, Int32, ndev, trajectories)[1:nsave, :],
             wants_energy ? download(h[], FIELD_SAVE_EKIN, Float64, ndev, trajectories)[1:nsave, :] : zeros(0, 0),
             wants_energy ? download(h[], FIELD_SAVE_EPOT, Float64, ndev, trajectories)[1:nsave, :] : zeros(0, 0),
