@@ -62,7 +62,9 @@ enum nqcd_method {
     NQCD_METHOD_CME = 7,          /* classical master equation, cme.jl:63-105 */
     NQCD_METHOD_BCME = 8,         /* broadened CME, cme.jl:115-182 (bandwidth in nqcd_problem_desc) */
     NQCD_METHOD_CLASSICAL = 9,    /* Simulation{Classical}, classical.jl (single-state model, one coordinate) */
-    NQCD_METHOD_SPIN_MAPPING = 10 /* Simulation{SpinMappingW}, spin_mapping.jl:1-131 (2 to 4 states, one coordinate) */
+    NQCD_METHOD_SPIN_MAPPING = 10,/* Simulation{SpinMappingW}, spin_mapping.jl:1-131 (2 to 4 states, one coordinate) */
+    NQCD_METHOD_ECMM = 11         /* Simulation{eCMM}, cmm.jl:1-120 (2 to 4 states, one coordinate; gamma in
+                                     nqcd_problem_desc.mapping_gamma) */
 };
 
 /* IntegrationAlgorithms (IntegrationAlgorithms.jl:19-56): BABwithTsit5, VerletwithElectronics,
@@ -148,7 +150,8 @@ typedef struct {
     int32_t reserved0;
     double bandwidth;            /* BCME: bandwidth of the broadening integrals (cme.jl:130-133) */
     double fermi_level;          /* CME / BCME: Fermi level mu of the metal (NQCModels.fermilevel) */
-    int32_t reserved[2];
+    double mapping_gamma;        /* eCMM: the gamma keyword of Simulation{eCMM}(atoms, model; gamma) (cmm.jl:22-24);
+                                    ignored otherwise (spin mapping's gamma follows from n_states) */
 } nqcd_problem_desc;
 
 /* TerminatingCallback (DynamicsUtils/callbacks.jl:15-29) with a predicate the device evaluates after
@@ -304,12 +307,14 @@ int32_t nqcd_mi355x_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* 
                                  const double* sig_re, const double* sig_im, const int32_t* state,
                                  const int32_t* model_id);
 
-/* Spin mapping: set the mapping positions X and momenta P (n, ntraj) of the loaded trajectories (the
- * qmap / pmap of DynamicsVariables, spin_mapping.jl:15-33) instead of drawing them in the first step of
- * nqcd_mi355x_run; X = P = NULL returns to drawing.  Without this call every run draws theta_i = 2 pi u
- * (u from Philox stream 4 of philox_key at counter (trajectory, i)), radius sqrt(2 + gamma) on the
- * trajectory's occupied diabatic state and sqrt(gamma) elsewhere, X_i = radius sin(theta_i),
- * P_i = radius cos(theta_i).  Call after load_initial or sample_initial, with their ntraj. */
+/* Spin mapping / eCMM: set the mapping positions X and momenta P (n, ntraj) of the loaded trajectories
+ * (the qmap / pmap of DynamicsVariables, spin_mapping.jl:15-33, cmm.jl:26-34) instead of drawing them in
+ * the first step of nqcd_mi355x_run; X = P = NULL returns to drawing.  Without this call every run draws
+ * them from Philox stream 4 of philox_key: spin mapping theta_i = 2 pi u (counter (trajectory, i)),
+ * radius sqrt(2 + gamma) on the trajectory's occupied diabatic state and sqrt(gamma) elsewhere,
+ * X_i = radius sin(theta_i), P_i = radius cos(theta_i); eCMM a point uniform on the 2n-sphere of radius
+ * sqrt(2 + 2 n gamma) (normals (z_2k, z_2k+1) at counter (trajectory, k), normalised; X the first n
+ * coordinates, P the last n).  Call after load_initial or sample_initial, with their ntraj. */
 int32_t nqcd_mi355x_load_mapping(nqcd_handle_t* h, int64_t ntraj, const double* X, const double* P);
 
 /* Sample initial conditions on the device from a recognised distribution

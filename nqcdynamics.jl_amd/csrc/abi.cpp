@@ -219,13 +219,13 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     const bool ie = d->method == NQCD_METHOD_IESH || d->method == NQCD_METHOD_EHRENFEST_NA;
     const bool cme = d->method == NQCD_METHOD_CME || d->method == NQCD_METHOD_BCME;
     const bool cls = d->method == NQCD_METHOD_CLASSICAL;
-    const bool map = d->method == NQCD_METHOD_SPIN_MAPPING;
+    const bool map = d->method == NQCD_METHOD_SPIN_MAPPING || d->method == NQCD_METHOD_ECMM;
     if (!rp && !ie && !cme && !cls && !map && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states >= 2 &&
                         d->n_states <= kNsMaxStates))
         return fail(h, NQCD_ERR_UNSUPPORTED,
                     "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (2 to 8 states, BABwithTsit5), "
                     "Simulation{AdiabaticIESH|EhrenfestNA} (VerletwithElectronics), Simulation{CME|BCME}, "
-                    "Simulation{Classical} (VelocityVerlet), Simulation{SpinMappingW} (MInt) and "
+                    "Simulation{Classical} (VelocityVerlet), Simulation{SpinMappingW|eCMM} (MInt) and "
                     "RingPolymerSimulation{Classical|FSSH|Ehrenfest} (BCB / BCBwithTsit5)");
     if (sh_or_eh && d->n_states > 2 && d->ndofs * d->natoms != 1)
         return fail(h, NQCD_ERR_UNSUPPORTED, "FSSH / Ehrenfest with more than two states: one nuclear coordinate");

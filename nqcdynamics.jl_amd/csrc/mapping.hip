@@ -24,13 +24,23 @@
 // the others, X_i = radius sin(theta_i), P_i = radius cos(theta_i) (the reference stores the sines as
 // the mapping positions); or uploaded by nqcd_mi355x_load_mapping.
 //
-// Device layout: X, P [state][traj]; diabatic populations (X_i^2 + P_i^2 - gamma) / 2
-// (spin_mapping.jl:120-131) [slot][state][traj]; kinetic and potential energy [slot][traj].
+// eCMM (Simulation{eCMM}, cmm.jl:1-120) runs the same kernel: its Hamiltonian
+// (q^T V q + p^T V p - 2 gamma tr V) / 2 is the form above with 2 gamma in place of spin mapping's gamma on
+// the diagonal; MInt's trace term is tr(dV) gamma dt / 2m for either method (mint.jl:85), which for eCMM
+// is half of its Hamiltonian's and is kept as the reference computes it; its initial variables are a point
+// uniform on the 2n-sphere of radius sqrt(2 + 2 n gamma) (2n Philox normals, pair k from stream 4 at
+// counter (trajectory, k), normalised; the first n coordinates are the positions); its diabatic
+// population is the inverse mapping kernel (1 + n) / (1 + n gamma)^2 (X^2 + P^2) / 2 - (1 - gamma) /
+// (1 + n gamma) (cmm.jl:96-112).
+//
+// Device layout: X, P [state][traj]; diabatic populations (spin mapping (X_i^2 + P_i^2 - gamma) / 2,
+// spin_mapping.jl:120-131) [slot][state][traj]; kinetic and potential energy [slot][traj].
 #include "mapping.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -45,7 +55,13 @@ struct MapArgs {
     int64_t ntraj, nsteps;
     int32_t save_every, nsave;
     int32_t slot_tab[kMaxModels];   // offset of a MORSE_N slot's table in mtab, -1 for closed-form models
-    double dt, inv_mass, mass, gamma;
+    int32_t ecmm, pad;              // 1: eCMM's initial variables (sphere) instead of spin mapping's
+    double dt, inv_mass, mass;
+    double gamma;                   // constant of the diagonal terms V_ii (X_i^2 + P_i^2 - gamma) / 2 of H
+    double gamma_force;             // gamma of MInt's trace term tr(dV) gamma dt / 2m
+    double pop_a, pop_b;            // population estimator pop_a (X_i^2 + P_i^2) / 2 - pop_b
+    double radius;                  // eCMM: radius of the initial sphere
+    double occ_radius, free_radius; // spin mapping: initial radii of the occupied and the other states
     uint64_t key;
     const ModelSlot* models;
     const double* mtab;
@@ -86,7 +102,7 @@ template <int NM>
 __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
     constexpr int NP = NM * (NM - 1) / 2;
     const int64_t nt = A.ntraj;
-    const double dt = A.dt, hdt = 0.5 * A.dt, g = A.gamma, im = A.inv_mass;
+    const double dt = A.dt, hdt = 0.5 * A.dt, g = A.gamma, gf = A.gamma_force, im = A.inv_mass;
     for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < nt; tid += (int64_t)gridDim.x * blockDim.x) {
         const int mid = A.model_id ? A.model_id[tid] : 0;
         LaneN<NM, double> L;
@@ -98,12 +114,22 @@ __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
         if (A.X0) {
 #pragma unroll
             for (int i = 0; i < NM; ++i) { X[i] = A.X0[(int64_t)i * nt + tid]; P[i] = A.P0[(int64_t)i * nt + tid]; }
+        } else if (A.ecmm) {
+            double z[2 * NM], nrm = 0.0;
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                philox_normal2((uint64_t)tid, (uint32_t)k, kMapStream, A.key, z[2 * k], z[2 * k + 1]);
+                nrm += z[2 * k] * z[2 * k] + z[2 * k + 1] * z[2 * k + 1];
+            }
+            const double sc = A.radius / sqrt(nrm);
+#pragma unroll
+            for (int i = 0; i < NM; ++i) { X[i] = z[i] * sc; P[i] = z[NM + i] * sc; }
         } else {
             const int st = A.state0[tid];
 #pragma unroll
             for (int i = 0; i < NM; ++i) {
                 const double th = 6.283185307179586 * philox_uniform((uint64_t)tid, (uint32_t)i, kMapStream, A.key);
-                const double rad = sqrt(i == st ? 2.0 + g : g);
+                const double rad = i == st ? A.occ_radius : A.free_radius;
                 double s, c;
                 sincos(th, &s, &c);
                 X[i] = rad * s;
@@ -113,7 +139,7 @@ __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
         auto save = [&](int slot) {
 #pragma unroll
             for (int i = 0; i < NM; ++i)
-                A.save_pop[((int64_t)slot * NM + i) * nt + tid] = (X[i] * X[i] + P[i] * P[i] - g) * 0.5;
+                A.save_pop[((int64_t)slot * NM + i) * nt + tid] = A.pop_a * ((X[i] * X[i] + P[i] * P[i]) * 0.5) - A.pop_b;
             A.save_ekin[(int64_t)slot * nt + tid] = 0.5 * A.mass * v * v;
             A.save_epot[(int64_t)slot * nt + tid] = map_potential<NM>(L, x, X, P, g);
         };
@@ -162,7 +188,7 @@ __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
                     f += s * wij * (a[i] * a[j] + b[i] * b[j]) - (1.0 - c) * wij * (a[i] * b[j] - a[j] * b[i]);
                 }
             }
-            v = v - f * im + tr * g * dt * 0.5 * im;
+            v = v - f * im + tr * gf * dt * 0.5 * im;
             x = rt + hdt * v;
             if (A.save_every > 0 && step + 1 == until) {
                 if (slot < A.nsave) save(slot);
@@ -185,7 +211,9 @@ template __global__ void mapping_kernel<4>(const MapArgs);
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
-bool map_is_problem(const nqcd_handle_t* h) { return h->prob.method == NQCD_METHOD_SPIN_MAPPING; }
+bool map_is_problem(const nqcd_handle_t* h) {
+    return h->prob.method == NQCD_METHOD_SPIN_MAPPING || h->prob.method == NQCD_METHOD_ECMM;
+}
 
 static double map_gamma(int n) { return 2.0 / n * (std::sqrt((double)n + 1.0) - 1.0); }
 
@@ -195,10 +223,13 @@ static int map_slot_states(const nqcd_handle_t* h, int s) {
 
 int32_t map_check_problem(nqcd_handle_t* h) {
     const nqcd_problem_desc& d = h->prob;
-    if (d.algorithm != NQCD_ALG_MINT) return fail(h, NQCD_ERR_UNSUPPORTED, "Simulation{SpinMappingW} integrates with MInt");
+    if (d.algorithm != NQCD_ALG_MINT)
+        return fail(h, NQCD_ERR_UNSUPPORTED, "Simulation{SpinMappingW | eCMM} integrates with MInt");
     if (d.n_states < 2 || d.n_states > kMapMaxStates || d.ndofs * d.natoms != 1 || d.nbeads != 1)
-        return fail(h, NQCD_ERR_UNSUPPORTED, "spin mapping: 2 to 4 states, one nuclear coordinate");
-    if (d.precision != NQCD_PREC_FP64) return fail(h, NQCD_ERR_UNSUPPORTED, "spin mapping runs in FP64");
+        return fail(h, NQCD_ERR_UNSUPPORTED, "spin mapping / eCMM: 2 to 4 states, one nuclear coordinate");
+    if (d.precision != NQCD_PREC_FP64) return fail(h, NQCD_ERR_UNSUPPORTED, "spin mapping / eCMM run in FP64");
+    if (d.method == NQCD_METHOD_ECMM && (!std::isfinite(d.mapping_gamma) || 2.0 + 2.0 * d.n_states * d.mapping_gamma <= 0.0))
+        return fail(h, NQCD_ERR_ARG, "eCMM: mapping_gamma must be finite with 2 + 2 n gamma > 0");
     return NQCD_OK;
 }
 
@@ -218,6 +249,11 @@ static int32_t map_check_models(nqcd_handle_t* h) {
 int32_t map_load_initial(nqcd_handle_t* h, int64_t ntraj, const double* r, const double* v, const int32_t* state,
                          const int32_t* model_id) {
     if (int32_t rc = map_check_models(h)) return rc;
+    std::vector<int32_t> zeros;
+    if (!state && h->prob.method == NQCD_METHOD_ECMM) {   // eCMM's draw does not depend on a state (cmm.jl:26)
+        zeros.assign(ntraj, 0);
+        state = zeros.data();
+    }
     if (!state) return fail(h, NQCD_ERR_ARG, "spin mapping needs the occupied diabatic state of each trajectory");
     h->ntraj = ntraj;
     HIPCHK(h, h->r0.ensure(sizeof(double) * ntraj));
@@ -240,7 +276,7 @@ int32_t map_sample_initial(nqcd_handle_t* h, const nqcd_sampler_desc* S, uint64_
     if (int32_t rc = map_check_models(h)) return rc;
     if (S->kind != NQCD_SAMPLER_GAUSSIAN || S->nparams != 4 || !S->params)
         return fail(h, NQCD_ERR_UNSUPPORTED, "spin mapping samples with GAUSSIAN (params mu_r, sd_r, mu_v, sd_v)");
-    if (S->state_basis != 1)
+    if (S->state_basis != 1 && h->prob.method == NQCD_METHOD_SPIN_MAPPING)
         return fail(h, NQCD_ERR_UNSUPPORTED, "spin mapping starts from a diabatic pure state (spin_mapping.jl:15)");
     if (!h->model_set[0]) return fail(h, NQCD_ERR_STATE, "model slot 0 is not set");
     if (S->initial_state < 0 || S->initial_state >= h->prob.n_states) return fail(h, NQCD_ERR_ARG, "initial_state out of range");
@@ -316,7 +352,16 @@ int32_t map_run(nqcd_handle_t* h, double t0, int64_t nsteps, uint64_t key) {
     A.dt = h->prob.dt;
     A.mass = h->masses[0];
     A.inv_mass = h->inv_mass[0];
-    A.gamma = map_gamma(n);
+    const bool ecmm = h->prob.method == NQCD_METHOD_ECMM;
+    const double ge = h->prob.mapping_gamma, gs = map_gamma(n);
+    A.ecmm = ecmm ? 1 : 0;
+    A.gamma = ecmm ? 2.0 * ge : gs;
+    A.gamma_force = ecmm ? ge : gs;
+    A.pop_a = ecmm ? (1.0 + n) / ((1.0 + n * ge) * (1.0 + n * ge)) : 1.0;
+    A.pop_b = ecmm ? (1.0 - ge) / (1.0 + n * ge) : 0.5 * gs;
+    A.radius = std::sqrt(2.0 + 2.0 * n * ge);
+    A.occ_radius = std::sqrt(2.0 + gs);
+    A.free_radius = std::sqrt(gs);
     A.key = key;
     A.models = h->models_dev.as<ModelSlot>();
     A.mtab = h->ns_tab_dev.as<double>();

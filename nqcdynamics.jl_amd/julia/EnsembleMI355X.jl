@@ -42,7 +42,7 @@ const libnqcd = get(ENV, "NQCD_MI355X_LIB", joinpath(@__DIR__, "..", "libnqcd_mi
 
 # --- codes of include/nqcd_mi355x.h ------------------------------------------------------------
 const METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST,
-      METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING = Int32.(0:10)
+      METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING, METHOD_ECMM = Int32.(0:11)
 const ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION, ALG_TSIT5,
       ALG_VELOCITY_VERLET, ALG_MINT = Int32.(0:7)
 const MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC,
@@ -89,7 +89,7 @@ struct ProblemDesc
     method::Int32; algorithm::Int32; n_states::Int32; ndofs::Int32; natoms::Int32; nbeads::Int32
     n_electrons::Int32; rescaling::Int32; precision::Int32; schedule::Int32; electronic_substeps::Int32
     decoherence::Int32; masses::Ptr{Float64}; dt::Float64; temperature::Float64; edc_C::Float64
-    disable_estimate::Int32; reserved0::Int32; bandwidth::Float64; fermi_level::Float64; reserved::NTuple{2,Int32}
+    disable_estimate::Int32; reserved0::Int32; bandwidth::Float64; fermi_level::Float64; mapping_gamma::Float64
 end
 struct TermDesc
     kind::Int32; dof::Int32; value::Float64
@@ -118,6 +118,7 @@ method_code(::SurfaceHoppingMethods.CME) = METHOD_CME
 method_code(::SurfaceHoppingMethods.BCME) = METHOD_BCME
 method_code(::ClassicalMethods.Classical) = METHOD_CLASSICAL      # Simulation{Classical}: VelocityVerlet
 method_code(::MappingVariableMethods.SpinMappingW) = METHOD_SPIN_MAPPING   # Simulation{SpinMappingW}: MInt
+method_code(::MappingVariableMethods.eCMM) = METHOD_ECMM                  # Simulation{eCMM}: MInt
 method_code(m) = unsupported(m)
 function method_code(sim::NQCDynamics.RingPolymerSimulation)
     m = sim.method
@@ -160,13 +161,14 @@ function problem_desc(sim, alg, dt, ensemblealg, masses)
     W = method == METHOD_BCME ? Float64(sim.method.bandwidth) : 0.0
     mu = cme ? Float64(NQCModels.fermilevel(sim)) : 0.0
     # Simulation{Classical}: VelocityVerlet (classical.jl:86)
-    # Simulation{SpinMappingW}: MInt (IntegrationAlgorithms.jl:79)
-    algc = method == METHOD_CLASSICAL ? ALG_VELOCITY_VERLET : method == METHOD_SPIN_MAPPING ? ALG_MINT :
-           cme ? ALG_TSIT5 : algorithm_code(alg)
+    # Simulation{SpinMappingW | eCMM}: MInt (IntegrationAlgorithms.jl:78-79)
+    mapping = method in (METHOD_SPIN_MAPPING, METHOD_ECMM)
+    algc = method == METHOD_CLASSICAL ? ALG_VELOCITY_VERLET : mapping ? ALG_MINT : cme ? ALG_TSIT5 : algorithm_code(alg)
+    γmap = method == METHOD_ECMM ? Float64(sim.method.γ) : 0.0
     ProblemDesc(method, algc, NQCModels.nstates(sim), size(sim)[1], size(sim)[2],
         rp ? RingPolymers.nbeads(sim) : 1, ie ? NQCModels.nelectrons(sim) : 0, cme ? Int32(2) : rescaling_code(sim.method),
         ensemblealg.fp32_nuclei ? 1 : 0, ensemblealg.persistent ? 0 : 1, 5, dec, pointer(masses), dt,
-        Float64(NQCDynamics.get_temperature(sim)), C, est, Int32(0), W, mu, (Int32(0), Int32(0)))
+        Float64(NQCDynamics.get_temperature(sim)), C, est, Int32(0), W, mu, γmap)
 end
 
 # --- initial conditions from the Selection closure (selections.jl:38-42,70-73) --------------
@@ -238,7 +240,7 @@ function device_sampler(prob, sim, model_kind)
     el = d.electronic
     el isa NQCDistributions.PureState || return nothing
     mc = method_code(sim)
-    mc in (METHOD_FSSH, METHOD_EHRENFEST, METHOD_CME, METHOD_BCME, METHOD_SPIN_MAPPING) || return nothing
+    mc in (METHOD_FSSH, METHOD_EHRENFEST, METHOD_CME, METHOD_BCME, METHOD_SPIN_MAPPING, METHOD_ECMM) || return nothing
     nuc = d.nuclear
     n = prod(size(sim))
     mv = gaussian_moments(nuc.velocity, n); mr = gaussian_moments(nuc.position, n)
@@ -251,6 +253,9 @@ function device_sampler(prob, sim, model_kind)
     if mc == METHOD_SPIN_MAPPING           # mapping variables drawn on the device in the run (spin_mapping.jl:15-33)
         basis == 1 || throw(ArgumentError("spin mapping starts from a PureState in the Diabatic basis"))
         return (SAMPLER_GAUSSIAN, Float64[mr[1][1], mr[2][1], mv[1][1], mv[2][1]], Int32(el.state - 1), Int32(1))
+    end
+    if mc == METHOD_ECMM                   # any electronic distribution: the sphere draw ignores it (cmm.jl:26)
+        return (SAMPLER_GAUSSIAN, Float64[mr[1][1], mr[2][1], mv[1][1], mv[2][1]], Int32(0), Int32(1))
     end
     if model_kind == MODEL_SPIN_BOSON      # params: mu_r(nb), sd_r(nb), mu_v(nb), sd_v(nb)
         return (SAMPLER_HARMONIC_WIGNER, vcat(mr[1], mr[2], mv[1], mv[2]), Int32(el.state - 1), basis)
@@ -280,7 +285,7 @@ function load_initial!(h, prob, sim, trajectories, seed, model_kind)
             h, trajectories, r, v, C_NULL, C_NULL, C_NULL, C_NULL))
         return
     end
-    if method_code(sim) == METHOD_SPIN_MAPPING   # (v, r, qmap, pmap) of DynamicsVariables (spin_mapping.jl:15-33)
+    if method_code(sim) in (METHOD_SPIN_MAPPING, METHOD_ECMM)   # (v, r, qmap, pmap): spin_mapping.jl:15-33, cmm.jl:26-34
         n = NQCModels.nstates(sim)
         X = reduce(hcat, [vec(MappingVariableMethods.get_mapping_positions(u)) for u in us])
         P = reduce(hcat, [vec(MappingVariableMethods.get_mapping_momenta(u)) for u in us])
@@ -475,7 +480,7 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         ie = method_code(sim) in (METHOD_IESH, METHOD_EHRENFEST_NA)
         cme = method_code(sim) in (METHOD_CME, METHOD_BCME)
         cls = method_code(sim) == METHOD_CLASSICAL
-        spinmap = method_code(sim) == METHOD_SPIN_MAPPING
+        spinmap = method_code(sim) in (METHOD_SPIN_MAPPING, METHOD_ECMM)
         two_state = !ie && !cme && !cls && !spinmap && !(sim isa NQCDynamics.RingPolymerSimulation) && n == 2
         device_reducible(f) = begin
             nm = output_name(f)

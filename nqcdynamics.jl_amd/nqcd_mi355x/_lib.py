@@ -17,7 +17,7 @@ ABI_VERSION = 2
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_DEVICE, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
 # enum nqcd_method
 (METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST,
- METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING) = range(11)
+ METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING, METHOD_ECMM) = range(12)
 # enum nqcd_algorithm
 (ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION, ALG_TSIT5,
  ALG_VELOCITY_VERLET, ALG_MINT) = range(8)
@@ -58,7 +58,7 @@ class ProblemDesc(C.Structure):
                 ("decoherence", C.c_int32), ("masses", C.POINTER(C.c_double)), ("dt", C.c_double),
                 ("temperature", C.c_double), ("edc_C", C.c_double), ("disable_estimate", C.c_int32),
                 ("reserved0", C.c_int32), ("bandwidth", C.c_double), ("fermi_level", C.c_double),
-                ("reserved", C.c_int32 * 2)]
+                ("mapping_gamma", C.c_double)]
 
 
 class TermDesc(C.Structure):
@@ -191,14 +191,14 @@ class Ensemble:
     def set_problem(self, method, n_states, ndofs, natoms, masses, dt, algorithm=ALG_BAB_TSIT5, nbeads=1,
                     n_electrons=0, rescaling=RESCALE_STANDARD, precision=PREC_FP64, schedule=SCHED_PERSISTENT,
                     electronic_substeps=5, temperature=0.0, decoherence=DECOHERENCE_NONE, edc_C=0.1,
-                    estimate_probability=True, bandwidth=0.0, fermi_level=0.0):
+                    estimate_probability=True, bandwidth=0.0, fermi_level=0.0, mapping_gamma=0.0):
         self._masses = np.ascontiguousarray(np.broadcast_to(np.asarray(masses, dtype=np.float64), (natoms,)))
         d = ProblemDesc()
         d.method, d.algorithm, d.n_states, d.ndofs, d.natoms, d.nbeads = method, algorithm, n_states, ndofs, natoms, nbeads
         d.n_electrons, d.rescaling, d.precision, d.schedule = n_electrons, rescaling, precision, schedule
         d.electronic_substeps, d.masses, d.dt, d.temperature = electronic_substeps, _dp(self._masses), dt, temperature
         d.decoherence, d.edc_C, d.disable_estimate = decoherence, edc_C, 0 if estimate_probability else 1
-        d.bandwidth, d.fermi_level = bandwidth, fermi_level
+        d.bandwidth, d.fermi_level, d.mapping_gamma = bandwidth, fermi_level, mapping_gamma
         self._check(self._L.nqcd_mi355x_set_problem(self._h, C.byref(d)))
         self.method, self.n_electrons = method, n_electrons
         self.n_states, self.D, self.dt = n_states, ndofs * natoms, dt

@@ -14,7 +14,7 @@ from .distributions import Adiabatic, DynamicalDistribution, FermiDiracState, Pr
 from .outputs import (FileReduction, MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction,
                       SumReduction, output_name)
 from .simulations import (AdiabaticIESH, BCME, CME, Classical, Ehrenfest, EhrenfestNA, FSSH, PeriodicCell,
-                          RingPolymerSimulation, Simulation, SpinMappingW)
+                          RingPolymerSimulation, Simulation, SpinMappingW, eCMM)
 
 log = logging.getLogger("nqcd_mi355x")
 
@@ -96,7 +96,7 @@ def _kind(sim):
         return "cme"
     elif isinstance(m, Classical) and sim.model.nstates == 1 and sim.ndofs * sim.natoms == 1:
         return "cl"
-    elif isinstance(m, SpinMappingW) and sim.ndofs * sim.natoms == 1:
+    elif isinstance(m, (SpinMappingW, eCMM)) and sim.ndofs * sim.natoms == 1:
         return "map"
     raise ValueError(f"EnsembleMI355X does not support {type(sim).__name__}{{{type(m).__name__}}}")
 
@@ -126,9 +126,10 @@ def _problem(E, sim, dt):
                       algorithm=L.ALG_VELOCITY_VERLET, temperature=sim.temperature, **common)
     elif kind == "map":
         if sim.precision != "fp64":
-            raise ValueError("spin mapping runs in FP64")
-        E.set_problem(L.METHOD_SPIN_MAPPING, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
-                      algorithm=L.ALG_MINT, temperature=sim.temperature, **common)
+            raise ValueError("spin mapping / eCMM run in FP64")
+        E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=L.ALG_MINT, temperature=sim.temperature,
+                      mapping_gamma=getattr(method, "gamma", 0.0) if isinstance(method, eCMM) else 0.0, **common)
     elif kind == "cme":
         if sim.precision != "fp64":
             raise ValueError("CME / BCME run in FP64")
@@ -272,9 +273,12 @@ def _initial_cl(E, sim, dist, trajectories, selection, seed):
 def _initial_map(E, sim, dist, trajectories, selection, seed):
     """Spin mapping: a diabatic PureState (DynamicsVariables(sim, v, r, ::PureState{Diabatic}),
     spin_mapping.jl:15-33) and one coordinate, sampled on the device from a Gaussian distribution or
-    uploaded; the mapping variables are drawn on the device in the run."""
+    uploaded; the mapping variables are drawn on the device in the run.  eCMM (cmm.jl:26-34) takes any
+    electronic distribution and draws its variables on a sphere."""
     el = _electronic(dist)
     n = sim.model.nstates
+    if isinstance(sim.method, eCMM):   # any electronic distribution: eCMM's draw ignores it (cmm.jl:26)
+        el = PureState(1)
     if not el.diabatic:
         raise ValueError("spin mapping starts from a PureState in the Diabatic basis")
     if el.state < 1 or el.state > n:

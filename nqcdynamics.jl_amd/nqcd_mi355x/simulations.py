@@ -156,6 +156,17 @@ class SpinMappingW(Method):
         self.R2 = 2.0 * np.sqrt(n + 1.0)
 
 
+class eCMM(Method):
+    """Extended classical mapping model (cmm.jl:1-24): Simulation[eCMM](atoms, model, gamma=0.0) mirrors
+    Simulation{eCMM}(atoms, model; γ=0); integrated by MInt (IntegrationAlgorithms.jl:78)."""
+    method_code = L.METHOD_ECMM
+    algorithm = L.ALG_MINT
+    rescaling_code = L.RESCALE_STANDARD
+
+    def __init__(self, gamma=0.0):
+        self.gamma = float(gamma)
+
+
 class AbstractSimulation:
     def __init__(self, method, atoms, model, temperature=0.0, precision="fp64", cell=None, **method_kwargs):
         self.method = method(**method_kwargs) if isinstance(method, type) else method

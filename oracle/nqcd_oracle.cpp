@@ -1444,14 +1444,37 @@ OR_API double oracle_cme_broadening(double h, double mu, double beta, double Gam
 // only the off-diagonal entries of the cache matrix that transform_matrix! then overwrites with
 // Z Xi Z^T, so each step's Xi carries the previous step's transformed diagonal (zero at a
 // trajectory's first step); restated as the reference computes it.
+//
+// eCMM (method 1, cmm.jl:1-120) has H = m v^2 / 2 + (q^T V q + p^T V p - 2 gamma tr V) / 2
+// (classical_potential_energy, cmm.jl:84-94) with its own gamma (keyword, default 0) and is integrated
+// by the same MInt step, whose force term adds tr(dV) gamma / m dt / 2 for either method (mint.jl:85):
+// for eCMM that is half of what its Hamiltonian's -gamma tr V gives, which matters only where
+// tr dV != 0; restated as the reference computes it.  Populations: spin mapping (X^2 + P^2 - gamma)/2;
+// eCMM the inverse mapping kernel (1 + F) / (1 + F gamma)^2 (X^2 + P^2)/2 - (1 - gamma) / (1 + F gamma)
+// (diabatic_population, cmm.jl:96-112).
 // ---------------------------------------------------------------------------
 struct OMapDesc {
-    int32_t nstates, save_every, nsave, pad;
+    int32_t nstates, save_every, nsave, method;   // method: 0 SpinMappingW, 1 eCMM
     double dt, mass;
     int64_t nsteps;
+    double gamma;                                 // eCMM's gamma (unused for spin mapping)
 };
 
 static double mapping_gamma(int n) { return 2.0 / n * (std::sqrt((double)n + 1.0) - 1.0); }
+
+// gamma of the method: spin mapping's from n, eCMM's as given
+static double map_gamma(const OMapDesc& D) { return D.method == 1 ? D.gamma : mapping_gamma(D.nstates); }
+// the constant each diagonal V_ii carries in H: V_ii (X^2 + P^2 - c) / 2
+static double map_diag_const(const OMapDesc& D) { return D.method == 1 ? 2.0 * D.gamma : mapping_gamma(D.nstates); }
+
+static double map_population(const OMapDesc& D, double x, double p) {
+    if (D.method == 1) {   // inverse_mapping_kernel, cmm.jl:99-104
+        const int F = D.nstates;
+        const double g = D.gamma;
+        return (1.0 + F) / ((1.0 + F * g) * (1.0 + F * g)) * (x * x + p * p) / 2.0 - (1.0 - g) / (1.0 + F * g);
+    }
+    return (x * x + p * p - mapping_gamma(D.nstates)) / 2.0;
+}
 
 // H at (r, v, X, P): the classical Hamiltonian of the mapping variables.
 static double mapping_energy(const OMapDesc& D, const OModelDesc& m, double r, double v, const double* X, const double* P) {
@@ -1459,7 +1482,7 @@ static double mapping_energy(const OMapDesc& D, const OModelDesc& m, double r, d
     std::vector<double> V(n * n), dV(n * n);
     double V0, dV0;
     model_eval(m, &r, V.data(), dV.data(), &V0, &dV0);
-    const double g = mapping_gamma(n);
+    const double g = map_diag_const(D);
     double e = 0.5 * D.mass * v * v + V0;
     for (int i = 0; i < n; ++i) {
         e += V[i * n + i] * (X[i] * X[i] + P[i] * P[i] - g) / 2.0;
@@ -1478,7 +1501,7 @@ static void mapping_motion(const OMapDesc& D, const OModelDesc& m, const double*
     std::vector<double> V(n * n), dV(n * n);
     double V0, dV0;
     model_eval(m, &r, V.data(), dV.data(), &V0, &dV0);
-    const double g = mapping_gamma(n);
+    const double g = map_diag_const(D);   // eCMM: -(q^T D q + p^T D p - 2 gamma tr D) / 2, cmm.jl:55-70
     double f = -dV0;
     for (int i = 0; i < n; ++i) {
         f -= dV[i * n + i] * (X[i] * X[i] + P[i] * P[i] - g) / 2.0;
@@ -1514,7 +1537,7 @@ static void mapping_transform(int n, const double* Z, double* M) {
 // One MInt step; Xi holds the cache matrix Xi between steps (its diagonal carries over).
 static void mint_step(const OMapDesc& D, const OModelDesc& m, double& r, double& v, double* X, double* P, double* Xi) {
     const int n = D.nstates;
-    const double dt = D.dt, g = mapping_gamma(n);
+    const double dt = D.dt, g = map_gamma(D);
     const double rt = r + dt / 2.0 * v;
     std::vector<double> V(n * n), dV(n * n), w(n), Z(n * n), C(n * n, 0.0), Dm(n * n, 0.0), tmp(n), tmp2(n);
     double V0, dV0;
@@ -1583,6 +1606,26 @@ OR_API void oracle_mapping_sample(int32_t n, int64_t ntraj, const int32_t* state
     }
 }
 
+// eCMM's initial mapping variables (cmm.jl:26-43): a point uniform on the sphere of radius sqrt(2 + 2 F gamma)
+// in 2F dimensions (2F normals normalised: the pair (z_2k, z_2k+1) from the Philox block (trajectory, k,
+// stream 4)); the first F coordinates are the positions q, the last F the momenta p.  (The reference
+// splits at div(F, 2) + 1, which is F only for F = 2; for other F its q and p lengths differ from F.)
+OR_API void oracle_ecmm_sample(int32_t n, int64_t ntraj, double gamma, uint64_t key, const int64_t* traj_ids, double* X,
+                               double* P) {
+    const double R = std::sqrt(2.0 + 2.0 * n * gamma);
+    std::vector<double> z(2 * n);
+    for (int64_t t = 0; t < ntraj; ++t) {
+        const uint64_t id = traj_ids ? (uint64_t)traj_ids[t] : (uint64_t)t;
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) {
+            normal2(id, (uint32_t)k, 4u, key, &z[2 * k], &z[2 * k + 1]);
+            s += z[2 * k] * z[2 * k] + z[2 * k + 1] * z[2 * k + 1];
+        }
+        const double sc = R / std::sqrt(s);
+        for (int i = 0; i < n; ++i) { X[t * n + i] = z[i] * sc; P[t * n + i] = z[n + i] * sc; }
+    }
+}
+
 OR_API double oracle_mapping_energy(const OMapDesc* D, const OModelDesc* m, const double* u) {
     return mapping_energy(*D, *m, u[0], u[1], u + 2, u + 2 + D->nstates);
 }
@@ -1592,19 +1635,18 @@ OR_API void oracle_mapping_motion(const OMapDesc* D, const OModelDesc* m, const 
 }
 
 // Trajectories by MInt: x, v (ntraj), X, P ([traj][n]) in place; per save slot the diabatic
-// populations (X_i^2 + P_i^2 - gamma) / 2 (spin_mapping.jl:120-131), the kinetic and the potential
-// energy (H - m v^2 / 2).
+// populations (spin mapping: (X_i^2 + P_i^2 - gamma) / 2, spin_mapping.jl:120-131; eCMM: the inverse
+// mapping kernel, cmm.jl:99-112), the kinetic and the potential energy (H - m v^2 / 2).
 OR_API int32_t oracle_mapping_run(const OMapDesc* D, const OModelDesc* m, int64_t ntraj, double* x, double* v,
                                   double* X, double* P, double* spop, double* sek, double* sep, int32_t nthreads) {
     const int n = D->nstates, ns = D->nsave;
-    const double g = mapping_gamma(n);
     parallel_for(ntraj, nthreads, [&](int64_t t) {
         double* Xt = X + t * n;
         double* Pt = P + t * n;
         std::vector<double> Xi(n * n, 0.0);
         auto save = [&](int slot) {
             if (slot >= ns) return;
-            for (int i = 0; i < n; ++i) spop[((size_t)t * ns + slot) * n + i] = (Xt[i] * Xt[i] + Pt[i] * Pt[i] - g) / 2.0;
+            for (int i = 0; i < n; ++i) spop[((size_t)t * ns + slot) * n + i] = map_population(*D, Xt[i], Pt[i]);
             const double ek = 0.5 * D->mass * v[t] * v[t];
             sek[(size_t)t * ns + slot] = ek;
             sep[(size_t)t * ns + slot] = mapping_energy(*D, *m, x[t], v[t], Xt, Pt) - ek;

@@ -385,8 +385,11 @@ def classical_run(model, mass, r0, v0, dt=1.0, nsteps=100, save_every=0, term_ki
 
 
 class OMapDesc(C.Structure):
-    _fields_ = [("nstates", C.c_int32), ("save_every", C.c_int32), ("nsave", C.c_int32), ("pad", C.c_int32),
-                ("dt", C.c_double), ("mass", C.c_double), ("nsteps", C.c_int64)]
+    _fields_ = [("nstates", C.c_int32), ("save_every", C.c_int32), ("nsave", C.c_int32), ("method", C.c_int32),
+                ("dt", C.c_double), ("mass", C.c_double), ("nsteps", C.c_int64), ("gamma", C.c_double)]
+
+
+MAP_SPIN, MAP_ECMM = 0, 1
 
 
 def mapping_gamma(n):
@@ -394,9 +397,10 @@ def mapping_gamma(n):
     return 2.0 / n * (np.sqrt(n + 1.0) - 1.0)
 
 
-def _map_desc(n, mass, dt=0.01, nsteps=0, save_every=0):
+def _map_desc(n, mass, dt=0.01, nsteps=0, save_every=0, method=MAP_SPIN, gamma=0.0):
     D = OMapDesc()
     D.nstates, D.save_every, D.dt, D.mass, D.nsteps = n, save_every, dt, mass, nsteps
+    D.method, D.gamma = method, gamma
     D.nsave = nsteps // save_every + 1 if save_every else 1
     return D
 
@@ -413,18 +417,29 @@ def mapping_sample(n, state, key, traj_ids=None):
     return X, P
 
 
-def mapping_energy(model, n, mass, u):
-    """The spin-mapping Hamiltonian at u = [r, v, X(n), P(n)]."""
-    D = _map_desc(n, mass)
+def ecmm_sample(n, ntraj, gamma, key, traj_ids=None):
+    """eCMM's initial mapping variables: a uniform point on the 2n-sphere of radius sqrt(2 + 2 n gamma),
+    positions X the first n coordinates, momenta P the last n: X, P (ntraj, n)."""
+    X, P = np.zeros((ntraj, n)), np.zeros((ntraj, n))
+    ids = None if traj_ids is None else np.ascontiguousarray(traj_ids, dtype=np.int64)
+    f = lib().oracle_ecmm_sample
+    f.restype = None
+    f(C.c_int32(n), C.c_int64(ntraj), C.c_double(gamma), C.c_uint64(key), _p(ids, C.c_int64), _p(X), _p(P))
+    return X, P
+
+
+def mapping_energy(model, n, mass, u, method=MAP_SPIN, gamma=0.0):
+    """The mapping Hamiltonian (spin mapping, or eCMM with its gamma) at u = [r, v, X(n), P(n)]."""
+    D = _map_desc(n, mass, method=method, gamma=gamma)
     uu = np.ascontiguousarray(u, dtype=np.float64)
     f = lib().oracle_mapping_energy
     f.restype = C.c_double
     return f(C.byref(D), C.byref(model), _p(uu))
 
 
-def mapping_motion(model, n, mass, u):
-    """motion! of spin mapping at u = [r, v, X(n), P(n)]: du in the same layout."""
-    D = _map_desc(n, mass)
+def mapping_motion(model, n, mass, u, method=MAP_SPIN, gamma=0.0):
+    """motion! of spin mapping (or eCMM) at u = [r, v, X(n), P(n)]: du in the same layout."""
+    D = _map_desc(n, mass, method=method, gamma=gamma)
     uu = np.ascontiguousarray(u, dtype=np.float64)
     du = np.zeros_like(uu)
     f = lib().oracle_mapping_motion
@@ -433,11 +448,12 @@ def mapping_motion(model, n, mass, u):
     return du
 
 
-def mapping_run(model, n, mass, r0, v0, X0, P0, dt=0.01, nsteps=100, save_every=0, nthreads=None):
-    """Spin-mapping trajectories by MInt: final x, v, X, P and per save the diabatic populations (ntraj, nsave,
-    n), kinetic and potential energy."""
+def mapping_run(model, n, mass, r0, v0, X0, P0, dt=0.01, nsteps=100, save_every=0, nthreads=None, method=MAP_SPIN,
+                gamma=0.0):
+    """Spin-mapping (or eCMM) trajectories by MInt: final x, v, X, P and per save the diabatic populations
+    (ntraj, nsave, n), kinetic and potential energy."""
     ntraj = len(r0)
-    D = _map_desc(n, mass, dt, nsteps, save_every)
+    D = _map_desc(n, mass, dt, nsteps, save_every, method, gamma)
     out = dict(x=np.ascontiguousarray(r0, dtype=np.float64).ravel().copy(),
                v=np.ascontiguousarray(v0, dtype=np.float64).ravel().copy(),
                X=np.ascontiguousarray(X0, dtype=np.float64).reshape(ntraj, n).copy(),

@@ -27,9 +27,12 @@ def _case(L, O, name):
     return L.MODEL_MORSE_N, [n] + tab, O.make_model(O.MODEL_MORSE_N, n, 1, tab), n, 20000.0, 4.1, 0.1, 0.0, 2e-4, 2.0
 
 
-def _device(L, kind, params, n, mass, dt, r0, v0, state, nsteps, save_every, key, X=None, P=None):
+def _device(L, kind, params, n, mass, dt, r0, v0, state, nsteps, save_every, key, X=None, P=None, ecmm_gamma=None):
     E = L.Ensemble(0)
-    E.set_problem(L.METHOD_SPIN_MAPPING, n, 1, 1, [mass], dt, algorithm=L.ALG_MINT)
+    if ecmm_gamma is None:
+        E.set_problem(L.METHOD_SPIN_MAPPING, n, 1, 1, [mass], dt, algorithm=L.ALG_MINT)
+    else:
+        E.set_problem(L.METHOD_ECMM, n, 1, 1, [mass], dt, algorithm=L.ALG_MINT, mapping_gamma=ecmm_gamma)
     E.set_model(0, kind, params)
     E.load_initial(r0.reshape(-1, 1), v0.reshape(-1, 1), None, state=state)
     if X is not None:
@@ -158,3 +161,77 @@ def test_mapping_run_dynamics(oracle, key):
     assert np.allclose(mean["OutputTotalEnergy"], (c["ekin"] + c["epot"]).mean(axis=0), rtol=1e-10, atol=1e-14)
     assert np.allclose(np.array([p["OutputFinal"]["qmap"] for p in per]), c["X"], rtol=0, atol=1e-9)
     assert np.allclose(np.array([p["OutputFinal"]["r"][0, 0] for p in per]), c["x"], rtol=1e-9, atol=1e-9)
+
+
+def test_mapping_run_dynamics_sampled(oracle, key):
+    """run_dynamics with a Gaussian nuclear distribution: positions and velocities sampled on the device (the
+    Philox Gaussian stream the oracle reproduces), mapping variables drawn in the run; the mean diabatic
+    populations per save against the oracle from the same draws, and the per-save total population 1."""
+    import nqcd_mi355x as N
+    ntraj, dt, tf = 2048, 1.0, 800.0
+    sim = N.Simulation[N.SpinMappingW](N.Atoms(2000.0), N.TullyModelOne())
+    dist = N.DynamicalDistribution(N.Normal(12.0 / 2000.0, 1.0 / 2000.0), N.Normal(-3.0, 0.5), (1, 1)) * \
+        N.PureState(1, N.Diabatic())
+    alg = N.EnsembleMI355X()
+    print("spin mapping run_dynamics, sampled", flush=True)
+    res = N.run_dynamics(sim, (0.0, tf), dist, output=N.OutputDiabaticPopulation, trajectories=ntraj, dt=dt,
+                         saveat=50.0, reduction=N.MeanReduction(), ensemble_algorithm=alg)
+    pop = np.array(res["OutputDiabaticPopulation"])
+    assert np.allclose(pop[0], [1.0, 0.0], atol=1e-14)
+    assert np.allclose(pop.sum(axis=1), 1.0, atol=1e-12)
+    r0, v0 = oracle.sample_gaussian(ntraj, 1, -3.0, 0.5, 12.0 / 2000.0, 1.0 / 2000.0, alg.seed)
+    X0, P0 = oracle.mapping_sample(2, np.zeros(ntraj, np.int32), alg.seed)
+    om = oracle.make_model(oracle.MODEL_TULLY1, 2, 1, TULLY1)
+    c = oracle.mapping_run(om, 2, 2000.0, r0[:, 0], v0[:, 0], X0, P0, dt=dt, nsteps=int(tf / dt), save_every=50)
+    assert np.allclose(pop, c["pop"].mean(axis=0), rtol=0, atol=1e-10)
+    assert abs(pop[-1, 1]) > 0.05
+
+
+@pytest.mark.parametrize("name,gamma", [("tully1", 0.5), ("double_well", 0.3), ("morse3", 0.1), ("morse4", 0.0)])
+def test_ecmm_trajectories_match_oracle(oracle, key, name, gamma):
+    """eCMM (cmm.jl) by MInt: the device's sphere draws equal the oracle's, and trajectories, populations (the
+    inverse mapping kernel) and energies agree within 1e-9 over 400 steps; on the double well (tr dV != 0) this
+    includes MInt's trace term at gamma / 2 of eCMM's (mint.jl:85), reproduced on both sides."""
+    from nqcd_mi355x import _lib as L
+    kind, params, om, n, mass, xm, xs, vm, vs, dt = _case(L, oracle, name)
+    ntraj, nsteps, se = 256, 400, 40
+    rng = np.random.default_rng(9)
+    r0 = xm + xs * rng.standard_normal(ntraj)
+    v0 = vm + vs * rng.standard_normal(ntraj)
+    print(f"eCMM parity: model={name} n={n} gamma={gamma}", flush=True)
+    dev = _device(L, kind, params, n, mass, dt, r0, v0, None, nsteps, se, key, ecmm_gamma=gamma)
+    X0, P0 = oracle.ecmm_sample(n, ntraj, gamma, key)
+    c = oracle.mapping_run(om, n, mass, r0, v0, X0, P0, dt=dt, nsteps=nsteps, save_every=se, method=oracle.MAP_ECMM,
+                           gamma=gamma)
+    R2 = 2 + 2 * n * gamma
+    assert np.isfinite(c["pop"]).all()
+    assert np.allclose(dev["R"][:, 0], c["x"], rtol=1e-9, atol=1e-9)
+    assert np.allclose(dev["V"][:, 0], c["v"], rtol=1e-9, atol=1e-11)
+    assert np.allclose(dev["MAP_X"], c["X"], rtol=0, atol=1e-9)
+    assert np.allclose(dev["MAP_P"], c["P"], rtol=0, atol=1e-9)
+    assert np.allclose((dev["MAP_X"] ** 2 + dev["MAP_P"] ** 2).sum(1), R2, rtol=1e-12)
+    assert np.allclose(dev["SAVE_POP_DI"], c["pop"], rtol=0, atol=1e-9)
+    assert np.allclose(dev["SAVE_EKIN"], c["ekin"], rtol=1e-9, atol=1e-13)
+    assert np.allclose(dev["SAVE_EPOT"], c["epot"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(dev["red_pop"], dev["SAVE_POP_DI"].mean(axis=0), rtol=1e-12, atol=1e-15)
+
+
+def test_ecmm_run_dynamics(oracle, key):
+    """Simulation[eCMM](atoms, model, gamma=0.5) through run_dynamics with a Gaussian distribution and any
+    electronic state: the mean diabatic populations against the oracle from the same draws."""
+    import nqcd_mi355x as N
+    ntraj, dt, tf, gamma = 1024, 1.0, 600.0, 0.5
+    sim = N.Simulation[N.eCMM](N.Atoms(2000.0), N.TullyModelOne(), gamma=gamma)
+    dist = N.DynamicalDistribution(N.Normal(12.0 / 2000.0, 1.0 / 2000.0), N.Normal(-3.0, 0.5), (1, 1)) * \
+        N.PureState(1, N.Diabatic())
+    alg = N.EnsembleMI355X()
+    print("eCMM run_dynamics", flush=True)
+    res = N.run_dynamics(sim, (0.0, tf), dist, output=(N.OutputDiabaticPopulation, N.OutputTotalEnergy),
+                         trajectories=ntraj, dt=dt, saveat=100.0, reduction=N.MeanReduction(), ensemble_algorithm=alg)
+    r0, v0 = oracle.sample_gaussian(ntraj, 1, -3.0, 0.5, 12.0 / 2000.0, 1.0 / 2000.0, alg.seed)
+    X0, P0 = oracle.ecmm_sample(2, ntraj, gamma, alg.seed)
+    om = oracle.make_model(oracle.MODEL_TULLY1, 2, 1, TULLY1)
+    c = oracle.mapping_run(om, 2, 2000.0, r0[:, 0], v0[:, 0], X0, P0, dt=dt, nsteps=int(tf / dt), save_every=100,
+                           method=oracle.MAP_ECMM, gamma=gamma)
+    assert np.allclose(np.array(res["OutputDiabaticPopulation"]), c["pop"].mean(axis=0), rtol=0, atol=1e-10)
+    assert np.allclose(res["OutputTotalEnergy"], (c["ekin"] + c["epot"]).mean(axis=0), rtol=1e-10, atol=1e-14)
