@@ -1661,6 +1661,250 @@ OR_API int32_t oracle_mapping_run(const OMapDesc* D, const OModelDesc* m, int64_
 }
 
 // ---------------------------------------------------------------------------
+// Ring-polymer mapping dynamics by RingPolymerMInt (ringpolymer_mint.jl:28-131): NRPMD (nrpmd.jl:1-139,
+// method 0) and ring-polymer eCMM (rpecmm.jl:1-103, method 1), one coordinate per bead, F states.
+// Per bead b the Hamiltonian holds (p_b^T V~ p_b + q_b^T V~ q_b) / 2 + Vbar with Vbar = tr V / F and
+// V~ = V - Vbar I (classical_potential_energy, nrpmd.jl:123-139, rpecmm.jl:62-78), plus the kinetic
+// energy and the springs m w_n^2 / 2 sum_b (r_b - r_b+1)^2 (ring_polymer.jl:89-107).  A step: the free
+// ring polymer's half step in normal modes with sqrt(cay(dt A)) (ring_polymer.jl:71-82, half = true),
+// which for mode frequency w is [[1, dt/2], [-w^2 dt/2, 1]] / sqrt(1 + dt^2 w^2 / 4); at the new beads the
+// mapping variables rotated by the traceless eigenvalues w_k - Vbar (propagate_mapping_variables!,
+// ringpolymer_mint.jl:88-101); the force from Gamma and Xi of W~ = Z^T (dV - Dbar I) Z with Dbar = tr dV / F
+// (Xi built fresh with a zero diagonal, get_gamma / get_xi :111-129) and v_b -= force / m + Dbar / m dt
+// (:65-80); then the second half step.  The normal modes are the real DFT basis (any orthonormal basis of
+// the spring eigenspaces gives the same bead-space step).
+// ---------------------------------------------------------------------------
+struct ORpMapDesc {
+    int32_t nstates, nbeads, method, save_every;   // method: 0 NRPMD, 1 ring-polymer eCMM
+    int32_t nsave, pad;
+    double dt, mass, temperature, gamma;
+    int64_t nsteps;
+};
+
+// Potential part (without springs) of bead b: (p^T V~ p + q^T V~ q) / 2 + Vbar.
+static double rpmap_bead_potential(int F, const OModelDesc& m, double r, const double* q, const double* p) {
+    std::vector<double> V(F * F), dV(F * F);
+    double V0, dV0;
+    model_eval(m, &r, V.data(), dV.data(), &V0, &dV0);
+    double tr = 0.0;
+    for (int i = 0; i < F; ++i) tr += V[i * F + i];
+    const double Vb = tr / F;
+    double e = Vb + V0;
+    for (int i = 0; i < F; ++i)
+        for (int j = 0; j < F; ++j) {
+            const double vt = V[i * F + j] - (i == j ? Vb : 0.0);
+            e += 0.5 * vt * (p[i] * p[j] + q[i] * q[j]);
+        }
+    return e;
+}
+
+static double rpmap_spring(const ORpMapDesc& D, const double* r) {
+    const int P = D.nbeads;
+    const double wn = P * D.temperature;
+    double e = 0.0;
+    if (P > 1)
+        for (int b = 0; b < P; ++b) { const double d = r[b] - r[(b + 1) % P]; e += d * d; }
+    return e * D.mass * wn * wn / 2.0;
+}
+
+// classical_hamiltonian (DynamicsUtils.jl:74-79) at u = [r(P), v(P), q(P*F), p(P*F)] (bead-major).
+static double rpmap_energy(const ORpMapDesc& D, const OModelDesc& m, const double* u) {
+    const int P = D.nbeads, F = D.nstates;
+    const double* r = u; const double* v = u + P; const double* q = u + 2 * P; const double* p = u + 2 * P + P * F;
+    double e = rpmap_spring(D, r);
+    for (int b = 0; b < P; ++b) e += 0.5 * D.mass * v[b] * v[b] + rpmap_bead_potential(F, m, r[b], q + b * F, p + b * F);
+    return e;
+}
+
+// motion! (nrpmd.jl:67-121, rpecmm.jl:36-60): dr = v; dv = -((q^T D~ q + p^T D~ p) / 2 + Dbar) / m plus the
+// springs (apply_interbead_coupling!); dq = V~ p, dp = -V~ q per bead.
+static void rpmap_motion(const ORpMapDesc& D, const OModelDesc& m, const double* u, double* du) {
+    const int P = D.nbeads, F = D.nstates;
+    const double wn = P * D.temperature;
+    const double* r = u; const double* v = u + P; const double* q = u + 2 * P; const double* p = u + 2 * P + P * F;
+    std::vector<double> V(F * F), dV(F * F);
+    for (int b = 0; b < P; ++b) {
+        double V0, dV0;
+        model_eval(m, &r[b], V.data(), dV.data(), &V0, &dV0);
+        double trV = 0.0, trD = 0.0;
+        for (int i = 0; i < F; ++i) { trV += V[i * F + i]; trD += dV[i * F + i]; }
+        const double Vb = trV / F, Db = trD / F;
+        const double* qb = q + b * F; const double* pb = p + b * F;
+        double f = 0.0;
+        for (int i = 0; i < F; ++i)
+            for (int j = 0; j < F; ++j) f += (dV[i * F + j] - (i == j ? Db : 0.0)) * (qb[i] * qb[j] + pb[i] * pb[j]);
+        double a = -(0.5 * f + Db + dV0) / D.mass;
+        if (P > 1) a -= wn * wn * (2.0 * r[b] - r[(b + 1) % P] - r[(b + P - 1) % P]);
+        du[b] = v[b];
+        du[P + b] = a;
+        for (int i = 0; i < F; ++i) {
+            double sx = 0.0, sp = 0.0;
+            for (int j = 0; j < F; ++j) {
+                const double vt = V[i * F + j] - (i == j ? Vb : 0.0);
+                sx += vt * pb[j]; sp += vt * qb[j];
+            }
+            du[2 * P + b * F + i] = sx;
+            du[2 * P + P * F + b * F + i] = -sp;
+        }
+    }
+}
+
+static void rpmap_half_cayley(double w, double dt, double Ch[4]) {
+    const double s = 1.0 / std::sqrt(1.0 + dt * dt * w * w / 4.0);
+    Ch[0] = s; Ch[1] = 0.5 * dt * s; Ch[2] = -0.5 * dt * w * w * s; Ch[3] = s;
+}
+
+// One RingPolymerMInt step on one ring polymer (r, v: P beads; q, p: P*F bead-major).
+static void rpmap_step(const ORpMapDesc& D, const OModelDesc& m, const std::vector<double>& U,
+                       const std::vector<double>& Ch, double* r, double* v, double* q, double* p) {
+    const int P = D.nbeads, F = D.nstates;
+    const double dt = D.dt;
+    std::vector<double> rn(P), vn(P);
+    auto half = [&]() {
+        for (int k = 0; k < P; ++k) {
+            double x = 0.0, y = 0.0;
+            for (int b = 0; b < P; ++b) { x += U[b * P + k] * r[b]; y += U[b * P + k] * v[b]; }
+            rn[k] = Ch[4 * k] * x + Ch[4 * k + 1] * y;
+            vn[k] = Ch[4 * k + 2] * x + Ch[4 * k + 3] * y;
+        }
+        for (int b = 0; b < P; ++b) {
+            double x = 0.0, y = 0.0;
+            for (int k = 0; k < P; ++k) { x += U[b * P + k] * rn[k]; y += U[b * P + k] * vn[k]; }
+            r[b] = x; v[b] = y;
+        }
+    };
+    half();
+    std::vector<double> V(F * F), dV(F * F), w(F), Z(F * F), C(F * F), Dm(F * F), W(F * F), G(F * F), X(F * F), tq(F), tp(F);
+    for (int b = 0; b < P; ++b) {
+        double V0, dV0;
+        model_eval(m, &r[b], V.data(), dV.data(), &V0, &dV0);
+        eigen_sym(F, V.data(), w.data(), Z.data(), nullptr);
+        double trV = 0.0, trD = 0.0;
+        for (int i = 0; i < F; ++i) { trV += V[i * F + i]; trD += dV[i * F + i]; }
+        const double Vb = trV / F, Db = trD / F;
+        double* qb = q + b * F; double* pb = p + b * F;
+        // C = Z cos((w - Vbar) dt) Z^T, D = Z sin(-(w - Vbar) dt) Z^T (get_C_propagator / get_D_propagator)
+        std::fill(C.begin(), C.end(), 0.0); std::fill(Dm.begin(), Dm.end(), 0.0);
+        for (int i = 0; i < F; ++i) { C[i * F + i] = std::cos((w[i] - Vb) * dt); Dm[i * F + i] = std::sin(-(w[i] - Vb) * dt); }
+        mapping_transform(F, Z.data(), C.data());
+        mapping_transform(F, Z.data(), Dm.data());
+        for (int i = 0; i < F; ++i) {
+            double a = 0.0, bb = 0.0, c = 0.0, d = 0.0;
+            for (int j = 0; j < F; ++j) { a += C[i * F + j] * qb[j]; bb += Dm[i * F + j] * pb[j]; c += C[i * F + j] * pb[j]; d += Dm[i * F + j] * qb[j]; }
+            tq[i] = a - bb; tp[i] = c + d;
+        }
+        for (int i = 0; i < F; ++i) { qb[i] = tq[i]; pb[i] = tp[i]; }
+        // traceless adiabatic derivative W~ = Z^T (dV - Dbar I) Z; Gamma, Xi (element [j, i] from the pair (i, j))
+        for (int i = 0; i < F; ++i)
+            for (int j = 0; j < F; ++j) {
+                double sacc = 0.0;
+                for (int a = 0; a < F; ++a)
+                    for (int c = 0; c < F; ++c) sacc += Z[a * F + i] * (dV[a * F + c] - (a == c ? Db : 0.0)) * Z[c * F + j];
+                W[i * F + j] = sacc;
+            }
+        for (int i = 0; i < F; ++i)
+            for (int j = 0; j < F; ++j) {
+                const double li = w[i] - Vb, lj = w[j] - Vb;
+                if (i == j) { G[j * F + i] = W[i * F + j] * dt; X[j * F + i] = 0.0; }
+                else {
+                    G[j * F + i] = std::sin((li - lj) * dt) * W[i * F + j] / (li - lj);
+                    X[j * F + i] = (1.0 - std::cos((li - lj) * dt)) * W[i * F + j] / (li - lj);
+                }
+            }
+        mapping_transform(F, Z.data(), G.data());
+        mapping_transform(F, Z.data(), X.data());
+        double f = 0.0;
+        for (int i = 0; i < F; ++i) {
+            double eq = 0.0, ep = 0.0, fp = 0.0;
+            for (int j = 0; j < F; ++j) { eq += G[i * F + j] * qb[j]; ep += G[i * F + j] * pb[j]; fp += X[i * F + j] * pb[j]; }
+            f += 0.5 * (qb[i] * eq + pb[i] * ep) - qb[i] * fp;
+        }
+        v[b] -= f / D.mass;
+        v[b] -= Db / D.mass * dt;
+    }
+    half();
+}
+
+// Initial mapping variables of ring polymers.  NRPMD (nrpmd.jl:49-65): theta uniform per (state, bead) from
+// the Philox block (trajectory, b F + i, stream 4); q = R cos theta, p = R sin theta with R = sqrt(2 + 2 gamma)
+// on the occupied state and sqrt(2 gamma) elsewhere.  Ring-polymer eCMM (rpecmm.jl:11-25): a point uniform on
+// the sphere of radius sqrt(2 + 2 F gamma N) in 2 F N dimensions (normals z_2k, z_2k+1 from the block
+// (trajectory, k, stream 4), normalised); q the first F N coordinates and p the rest, bead-major.
+OR_API void oracle_rpmap_sample(const ORpMapDesc* D, int64_t ntraj, const int32_t* state, uint64_t key,
+                                const int64_t* traj_ids, double* q, double* p) {
+    const int P = D->nbeads, F = D->nstates, FN = F * P;
+    std::vector<double> z(2 * FN);
+    for (int64_t t = 0; t < ntraj; ++t) {
+        const uint64_t id = traj_ids ? (uint64_t)traj_ids[t] : (uint64_t)t;
+        if (D->method == 0) {
+            for (int b = 0; b < P; ++b)
+                for (int i = 0; i < F; ++i) {
+                    const double th = 2.0 * M_PI * uniform01(id, (uint32_t)(b * F + i), 4u, key);
+                    const double R = std::sqrt(i == state[t] ? 2.0 + 2.0 * D->gamma : 2.0 * D->gamma);
+                    q[(t * P + b) * F + i] = R * std::cos(th);
+                    p[(t * P + b) * F + i] = R * std::sin(th);
+                }
+        } else {
+            double s = 0.0;
+            for (int k = 0; k < FN; ++k) {
+                normal2(id, (uint32_t)k, 4u, key, &z[2 * k], &z[2 * k + 1]);
+                s += z[2 * k] * z[2 * k] + z[2 * k + 1] * z[2 * k + 1];
+            }
+            const double sc = std::sqrt(2.0 + 2.0 * F * D->gamma * P) / std::sqrt(s);
+            for (int j = 0; j < FN; ++j) { q[t * FN + j] = z[j] * sc; p[t * FN + j] = z[FN + j] * sc; }
+        }
+    }
+}
+
+OR_API double oracle_rpmap_energy(const ORpMapDesc* D, const OModelDesc* m, const double* u) { return rpmap_energy(*D, *m, u); }
+OR_API void oracle_rpmap_motion(const ORpMapDesc* D, const OModelDesc* m, const double* u, double* du) { rpmap_motion(*D, *m, u, du); }
+
+// Ring polymers by RingPolymerMInt: r, v (ntraj, P), q, p (ntraj, P, F) in place; per save the diabatic
+// populations (NRPMD: the bead mean of (q^2 + p^2)/2 - gamma, nrpmd.jl:110-121; eCMM: the bead sum of the
+// inverse mapping kernel with N beads, rpecmm.jl:80-94), the kinetic energy, the potential energy with the
+// springs, and the centroid.
+OR_API int32_t oracle_rpmap_run(const ORpMapDesc* D, const OModelDesc* m, int64_t ntraj, double* r, double* v, double* q,
+                                double* p, double* spop, double* sek, double* sep, double* scen, int32_t nthreads) {
+    const int P = D->nbeads, F = D->nstates, ns = D->nsave;
+    const double wn = P * D->temperature;
+    std::vector<double> U(P * P), Ch(4 * P);
+    normal_mode_matrix(P, U.data());
+    for (int k = 0; k < P; ++k) rpmap_half_cayley(matsubara(P, wn, k), D->dt, &Ch[4 * k]);
+    const double g = D->gamma, NF = (double)P * F;
+    parallel_for(ntraj, nthreads, [&](int64_t t) {
+        double* rt = r + t * P; double* vt = v + t * P;
+        double* qt = q + t * P * F; double* pt = p + t * P * F;
+        auto save = [&](int slot) {
+            if (slot >= ns) return;
+            double ek = 0.0, ep = rpmap_spring(*D, rt), cen = 0.0;
+            for (int b = 0; b < P; ++b) {
+                ek += 0.5 * D->mass * vt[b] * vt[b];
+                ep += rpmap_bead_potential(F, *m, rt[b], qt + b * F, pt + b * F);
+                cen += rt[b];
+            }
+            for (int i = 0; i < F; ++i) {
+                double pop = 0.0;
+                for (int b = 0; b < P; ++b) {
+                    const double x = qt[b * F + i], y = pt[b * F + i];
+                    if (D->method == 0) pop += (x * x + y * y) / 2.0 - g;
+                    else pop += (1.0 + NF) / ((1.0 + NF * g) * (1.0 + NF * g)) * (x * x + y * y) / 2.0 - (1.0 - g) / (1.0 + NF * g);
+                }
+                spop[((size_t)t * ns + slot) * F + i] = D->method == 0 ? pop / P : pop;
+            }
+            sek[(size_t)t * ns + slot] = ek;
+            sep[(size_t)t * ns + slot] = ep;
+            scen[(size_t)t * ns + slot] = cen / P;
+        };
+        save(0);
+        for (int64_t step = 0; step < D->nsteps; ++step) {
+            rpmap_step(*D, *m, U, Ch, rt, vt, qt, pt);
+            if (D->save_every > 0 && (step + 1) % D->save_every == 0) save((int)((step + 1) / D->save_every));
+        }
+    });
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Independent-electron surface hopping (IESH) and EhrenfestNA on a Newns-Anderson
 // (impurity + discretised bath) model.
 //   Model (NQCModels AndersonHolstein with a MiaoSubotnik impurity and a TrapezoidalRule

@@ -468,6 +468,76 @@ def mapping_run(model, n, mass, r0, v0, X0, P0, dt=0.01, nsteps=100, save_every=
     return out
 
 
+class ORpMapDesc(C.Structure):
+    _fields_ = [("nstates", C.c_int32), ("nbeads", C.c_int32), ("method", C.c_int32), ("save_every", C.c_int32),
+                ("nsave", C.c_int32), ("pad", C.c_int32), ("dt", C.c_double), ("mass", C.c_double),
+                ("temperature", C.c_double), ("gamma", C.c_double), ("nsteps", C.c_int64)]
+
+
+RPMAP_NRPMD, RPMAP_ECMM = 0, 1
+
+
+def _rpmap_desc(method, n, P, mass, temperature, gamma, dt=0.01, nsteps=0, save_every=0):
+    D = ORpMapDesc()
+    D.nstates, D.nbeads, D.method, D.save_every = n, P, method, save_every
+    D.nsave = nsteps // save_every + 1 if save_every else 1
+    D.dt, D.mass, D.temperature, D.gamma, D.nsteps = dt, mass, temperature, gamma, nsteps
+    return D
+
+
+def rpmap_sample(method, n, P, gamma, ntraj, key, state=None, traj_ids=None):
+    """Initial mapping variables of ring polymers (NRPMD: occupied diabatic `state` per trajectory; eCMM: a point
+    on the 2nP-sphere): q, p (ntraj, P, n)."""
+    D = _rpmap_desc(method, n, P, 1.0, 1.0, gamma)
+    q, p = np.zeros((ntraj, P, n)), np.zeros((ntraj, P, n))
+    st = np.zeros(ntraj, np.int32) if state is None else np.ascontiguousarray(state, dtype=np.int32)
+    ids = None if traj_ids is None else np.ascontiguousarray(traj_ids, dtype=np.int64)
+    f = lib().oracle_rpmap_sample
+    f.restype = None
+    f(C.byref(D), C.c_int64(ntraj), _p(st, C.c_int32), C.c_uint64(key), _p(ids, C.c_int64), _p(q), _p(p))
+    return q, p
+
+
+def rpmap_energy(method, model, n, P, mass, temperature, gamma, u):
+    """classical_hamiltonian of a mapping ring polymer at u = [r(P), v(P), q(P n), p(P n)] (bead-major)."""
+    D = _rpmap_desc(method, n, P, mass, temperature, gamma)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    f = lib().oracle_rpmap_energy
+    f.restype = C.c_double
+    return f(C.byref(D), C.byref(model), _p(uu))
+
+
+def rpmap_motion(method, model, n, P, mass, temperature, gamma, u):
+    D = _rpmap_desc(method, n, P, mass, temperature, gamma)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.zeros_like(uu)
+    f = lib().oracle_rpmap_motion
+    f.restype = None
+    f(C.byref(D), C.byref(model), _p(uu), _p(du))
+    return du
+
+
+def rpmap_run(method, model, n, P, mass, temperature, gamma, r0, v0, q0, p0, dt=0.01, nsteps=100, save_every=0,
+              nthreads=None):
+    """Ring polymers by RingPolymerMInt: final r, v (ntraj, P), q, p (ntraj, P, n) and per save the diabatic
+    populations (ntraj, nsave, n), kinetic energy, potential energy with springs and centroid (ntraj, nsave)."""
+    r0 = np.ascontiguousarray(r0, dtype=np.float64).reshape(-1, P)
+    ntraj = r0.shape[0]
+    D = _rpmap_desc(method, n, P, mass, temperature, gamma, dt, nsteps, save_every)
+    out = dict(r=r0.copy(), v=np.ascontiguousarray(v0, dtype=np.float64).reshape(ntraj, P).copy(),
+               q=np.ascontiguousarray(q0, dtype=np.float64).reshape(ntraj, P, n).copy(),
+               p=np.ascontiguousarray(p0, dtype=np.float64).reshape(ntraj, P, n).copy(),
+               pop=np.zeros((ntraj, D.nsave, n)), ekin=np.zeros((ntraj, D.nsave)), epot=np.zeros((ntraj, D.nsave)),
+               centroid=np.zeros((ntraj, D.nsave)))
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    f = lib().oracle_rpmap_run
+    f.restype = C.c_int32
+    f(C.byref(D), C.byref(model), C.c_int64(ntraj), _p(out["r"]), _p(out["v"]), _p(out["q"]), _p(out["p"]),
+      _p(out["pop"]), _p(out["ekin"]), _p(out["epot"]), _p(out["centroid"]), C.c_int32(nthreads))
+    return out
+
+
 def cme_run(method, model, mass, kT, r0, v0, state0, dt=1.0, nsteps=100, mu=0.0, bandwidth=1.0, save_every=0, key=0,
             nthreads=None, cell=None):
     """CME (method 0) / BCME (1) trajectories (cme.jl); r0, v0 (ntraj,), state0 0-based diabatic states.  Returns
