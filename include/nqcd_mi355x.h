@@ -63,8 +63,11 @@ enum nqcd_method {
     NQCD_METHOD_BCME = 8,         /* broadened CME, cme.jl:115-182 (bandwidth in nqcd_problem_desc) */
     NQCD_METHOD_CLASSICAL = 9,    /* Simulation{Classical}, classical.jl (single-state model, one coordinate) */
     NQCD_METHOD_SPIN_MAPPING = 10,/* Simulation{SpinMappingW}, spin_mapping.jl:1-131 (2 to 4 states, one coordinate) */
-    NQCD_METHOD_ECMM = 11         /* Simulation{eCMM}, cmm.jl:1-120 (2 to 4 states, one coordinate; gamma in
-                                     nqcd_problem_desc.mapping_gamma) */
+    NQCD_METHOD_ECMM = 11,        /* Simulation{eCMM}, cmm.jl:1-120 (2 to 4 states, one coordinate; gamma in
+                                     nqcd_problem_desc.mapping_gamma); with RP_MINT RingPolymerSimulation{eCMM},
+                                     rpecmm.jl:1-103 */
+    NQCD_METHOD_NRPMD = 12        /* RingPolymerSimulation{NRPMD}, nrpmd.jl:1-139 (gamma in mapping_gamma; the
+                                     reference's default is 0.5) */
 };
 
 /* IntegrationAlgorithms (IntegrationAlgorithms.jl:19-56): BABwithTsit5, VerletwithElectronics,
@@ -77,7 +80,9 @@ enum nqcd_algorithm {
     NQCD_ALG_BCB_WAVEFUNCTION = 4,   /* ring-polymer IESH / EhrenfestNA (bcb_wavefunction.jl) */
     NQCD_ALG_TSIT5 = 5,              /* the nuclear ODE by Tsit5 at a fixed step (CME / BCME, IntegrationAlgorithms.jl:68) */
     NQCD_ALG_VELOCITY_VERLET = 6,    /* OrdinaryDiffEq's VelocityVerlet (Simulation{Classical}, IntegrationAlgorithms.jl:61) */
-    NQCD_ALG_MINT = 7                /* MInt, the mapping integrator (mint.jl:42-181; select_algorithm, IntegrationAlgorithms.jl:79) */
+    NQCD_ALG_MINT = 7,               /* MInt, the mapping integrator (mint.jl:42-181; select_algorithm, IntegrationAlgorithms.jl:79) */
+    NQCD_ALG_RP_MINT = 8             /* RingPolymerMInt (ringpolymer_mint.jl:28-131; IntegrationAlgorithms.jl:102-103):
+                                        NRPMD and ring-polymer eCMM, 1 to 64 beads */
 };
 
 /* rescaling keyword of Simulation{FSSH} (fssh.jl:41, surface_hopping.jl:219,234-245). */
@@ -150,8 +155,9 @@ typedef struct {
     int32_t reserved0;
     double bandwidth;            /* BCME: bandwidth of the broadening integrals (cme.jl:130-133) */
     double fermi_level;          /* CME / BCME: Fermi level mu of the metal (NQCModels.fermilevel) */
-    double mapping_gamma;        /* eCMM: the gamma keyword of Simulation{eCMM}(atoms, model; gamma) (cmm.jl:22-24);
-                                    ignored otherwise (spin mapping's gamma follows from n_states) */
+    double mapping_gamma;        /* eCMM / NRPMD: the gamma keyword of Simulation{eCMM}(atoms, model; gamma)
+                                    (cmm.jl:22-24) or RingPolymerSimulation{NRPMD | eCMM}(...; gamma) (nrpmd.jl:42,
+                                    rpecmm.jl:7); ignored otherwise (spin mapping's gamma follows from n_states) */
 } nqcd_problem_desc;
 
 /* TerminatingCallback (DynamicsUtils/callbacks.jl:15-29) with a predicate the device evaluates after
@@ -251,8 +257,8 @@ enum nqcd_field {
     NQCD_FIELD_FINAL_POP_DI = 15,  /* double (n, ntraj) diabatic populations of the final state (fssh.jl:132-142,
                                       ehrenfest.jl:75-83); two-state one-dof FSSH / Ehrenfest */
     NQCD_FIELD_SAVE_V = 16,        /* double (nsave, ntraj) velocity per save (classical dynamics) */
-    NQCD_FIELD_MAP_X = 17,         /* double (n, ntraj) final mapping positions (spin mapping: u.qmap) */
-    NQCD_FIELD_MAP_P = 18          /* double (n, ntraj) final mapping momenta (u.pmap) */
+    NQCD_FIELD_MAP_X = 17,         /* double (n[, nbeads], ntraj) final mapping positions (u.qmap) */
+    NQCD_FIELD_MAP_P = 18          /* double (n[, nbeads], ntraj) final mapping momenta (u.pmap) */
 };
 /* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
  * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD / SAVE_POP_DI (n, nsave, ntraj). */

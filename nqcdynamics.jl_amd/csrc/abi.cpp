@@ -219,13 +219,13 @@ int32_t nqcd_mi355x_set_problem(nqcd_handle_t* h, const nqcd_problem_desc* d) {
     const bool ie = d->method == NQCD_METHOD_IESH || d->method == NQCD_METHOD_EHRENFEST_NA;
     const bool cme = d->method == NQCD_METHOD_CME || d->method == NQCD_METHOD_BCME;
     const bool cls = d->method == NQCD_METHOD_CLASSICAL;
-    const bool map = d->method == NQCD_METHOD_SPIN_MAPPING || d->method == NQCD_METHOD_ECMM;
+    const bool map = d->method == NQCD_METHOD_SPIN_MAPPING || d->method == NQCD_METHOD_ECMM || d->method == NQCD_METHOD_NRPMD;
     if (!rp && !ie && !cme && !cls && !map && !(sh_or_eh && d->algorithm == NQCD_ALG_BAB_TSIT5 && d->nbeads == 1 && d->n_states >= 2 &&
                         d->n_states <= kNsMaxStates))
         return fail(h, NQCD_ERR_UNSUPPORTED,
                     "EnsembleMI355X supports Simulation{FSSH|Ehrenfest} (2 to 8 states, BABwithTsit5), "
                     "Simulation{AdiabaticIESH|EhrenfestNA} (VerletwithElectronics), Simulation{CME|BCME}, "
-                    "Simulation{Classical} (VelocityVerlet), Simulation{SpinMappingW|eCMM} (MInt) and "
+                    "Simulation{Classical} (VelocityVerlet), Simulation{SpinMappingW|eCMM} (MInt), RingPolymerSimulation{NRPMD|eCMM} (RingPolymerMInt) and "
                     "RingPolymerSimulation{Classical|FSSH|Ehrenfest} (BCB / BCBwithTsit5)");
     if (sh_or_eh && d->n_states > 2 && d->ndofs * d->natoms != 1)
         return fail(h, NQCD_ERR_UNSUPPORTED, "FSSH / Ehrenfest with more than two states: one nuclear coordinate");
@@ -757,7 +757,11 @@ int32_t reduce_plan(nqcd_handle_t* h, int32_t kind, int32_t red, const double* o
     if (map_is_problem(h)) {
         if (kind == NQCD_OUT_DIABATIC_POPULATION) { rows = n * h->nsave; src = h->save_di.as<double>(); }
         else if (kind == NQCD_OUT_KINETIC_ENERGY || kind == NQCD_OUT_TOTAL_ENERGY) { rows = h->nsave; src = h->save_ekin.as<double>(); }
-        else return fail(h, NQCD_ERR_UNSUPPORTED, "spin-mapping outputs: DIABATIC_POPULATION, KINETIC_ENERGY, TOTAL_ENERGY");
+        else if (kind == NQCD_OUT_CENTROID_POSITION && h->prob.algorithm == NQCD_ALG_RP_MINT) {
+            rows = h->nsave; src = h->save_centroid.as<double>();
+        }
+        else return fail(h, NQCD_ERR_UNSUPPORTED, "mapping outputs: DIABATIC_POPULATION, KINETIC_ENERGY, TOTAL_ENERGY "
+                                                  "(CENTROID_POSITION for ring polymers)");
         if (len != rows) return fail(h, NQCD_ERR_ARG, "len does not match the output size");
         P = ReducePlan{};
         P.rows = rows;

@@ -22,5 +22,5 @@ from .outputs import (FileReduction, MeanReduction, OutputAdiabaticPopulation, O
                       OutputTotalEnergy, OutputVelocity, PopulationCorrelationFunction, SortByOutputReduction,
                       SortByTrajectoryReduction, SumReduction)
 from .simulations import (AdiabaticIESH, Atoms, BCME, CME, Classical, DecoherenceCorrectionEDC, Ehrenfest,  # noqa: F401
-                          EhrenfestNA, FSSH, InfiniteCell, PeriodicCell, RingPolymerSimulation, Simulation, SpinMappingW, eCMM,
+                          EhrenfestNA, FSSH, InfiniteCell, PeriodicCell, RingPolymerSimulation, Simulation, SpinMappingW, eCMM, NRPMD,
                           masses)

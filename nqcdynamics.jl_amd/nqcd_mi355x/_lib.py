@@ -17,10 +17,10 @@ ABI_VERSION = 2
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_DEVICE, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
 # enum nqcd_method
 (METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST,
- METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING, METHOD_ECMM) = range(12)
+ METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING, METHOD_ECMM, METHOD_NRPMD) = range(13)
 # enum nqcd_algorithm
 (ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION, ALG_TSIT5,
- ALG_VELOCITY_VERLET, ALG_MINT) = range(8)
+ ALG_VELOCITY_VERLET, ALG_MINT, ALG_RP_MINT) = range(9)
 RESCALE_STANDARD, RESCALE_VINVERSION, RESCALE_OFF = range(3)
 PREC_FP64, PREC_FP32_NUCLEI = range(2)
 SCHED_PERSISTENT, SCHED_STEP_PER_LAUNCH = range(2)
@@ -203,7 +203,9 @@ class Ensemble:
         self.method, self.n_electrons = method, n_electrons
         self.n_states, self.D, self.dt = n_states, ndofs * natoms, dt
         self.nbeads = nbeads
-        self.rows = ndofs * natoms * (nbeads if method in (METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST) else 1)
+        self.rows = ndofs * natoms * (nbeads if method in (METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST) or
+                                      algorithm == ALG_RP_MINT else 1)
+        self.map_beads = nbeads if algorithm == ALG_RP_MINT else 0   # 0: no bead axis
 
     def set_model(self, slot, kind, params):
         p = np.ascontiguousarray(params, dtype=np.float64)
@@ -257,8 +259,8 @@ class Ensemble:
         self.ntraj = ntraj
 
     def load_mapping(self, X, P):
-        """Spin mapping: mapping positions X and momenta P (ntraj, n) of the loaded trajectories (None for
-        both: draw them in the run from the run's key)."""
+        """Spin mapping / eCMM: mapping positions X and momenta P (ntraj, n), or (ntraj, nbeads, n) for ring
+        polymers, of the loaded trajectories (None for both: draw them in the run from the run's key)."""
         if X is None and P is None:
             self._check(self._L.nqcd_mi355x_load_mapping(self._h, self.ntraj, None, None))
             return
@@ -333,8 +335,9 @@ class Ensemble:
                   FIELD_SAVE_STATE: ((nt, nsave), np.int32), FIELD_FALLBACKS: ((nt,), np.int32),
                   FIELD_SAVE_R: ((nt, nsave, D), np.float64), FIELD_SAVE_EKIN: ((nt, nsave), np.float64),
                   FIELD_SAVE_EPOT: ((nt, nsave), np.float64), FIELD_FINAL_POP_DI: ((nt, n), np.float64),
-                  FIELD_SAVE_V: ((nt, nsave), np.float64), FIELD_MAP_X: ((nt, n), np.float64),
-                  FIELD_MAP_P: ((nt, n), np.float64)}
+                  FIELD_SAVE_V: ((nt, nsave), np.float64)}
+        mb = getattr(self, "map_beads", 0)
+        shapes[FIELD_MAP_X] = shapes[FIELD_MAP_P] = (((nt, n) if mb == 0 else (nt, mb, n)), np.float64)
         if getattr(self, "method", None) in (METHOD_CME, METHOD_BCME, METHOD_CLASSICAL):  # one coordinate
             shapes[FIELD_SAVE_R] = ((nt, nsave), np.float64)
         shape, dtype = shapes[field]

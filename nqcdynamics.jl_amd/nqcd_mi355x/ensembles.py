@@ -14,7 +14,7 @@ from .distributions import Adiabatic, DynamicalDistribution, FermiDiracState, Pr
 from .outputs import (FileReduction, MeanReduction, Output, SortByOutputReduction, SortByTrajectoryReduction,
                       SumReduction, output_name)
 from .simulations import (AdiabaticIESH, BCME, CME, Classical, Ehrenfest, EhrenfestNA, FSSH, PeriodicCell,
-                          RingPolymerSimulation, Simulation, SpinMappingW, eCMM)
+                          RingPolymerSimulation, Simulation, SpinMappingW, eCMM, NRPMD)
 
 log = logging.getLogger("nqcd_mi355x")
 
@@ -86,6 +86,8 @@ def _kind(sim):
     if isinstance(sim, RingPolymerSimulation):
         if isinstance(m, (Classical, FSSH, Ehrenfest)):
             return "rp"
+        if isinstance(m, (NRPMD, eCMM)) and sim.ndofs * sim.natoms == 1:
+            return "rpmap"
         if isinstance(m, (AdiabaticIESH, EhrenfestNA)):
             return "ie"
     elif isinstance(m, (FSSH, Ehrenfest)):
@@ -124,6 +126,12 @@ def _problem(E, sim, dt):
             raise ValueError("classical dynamics run in FP64")
         E.set_problem(L.METHOD_CLASSICAL, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
                       algorithm=L.ALG_VELOCITY_VERLET, temperature=sim.temperature, **common)
+    elif kind == "rpmap":
+        if sim.precision != "fp64":
+            raise ValueError("NRPMD / ring-polymer eCMM run in FP64")
+        E.set_problem(method.method_code, sim.model.nstates, sim.ndofs, sim.natoms, sim.atoms.masses, dt,
+                      algorithm=L.ALG_RP_MINT, nbeads=sim.nbeads, temperature=sim.temperature,
+                      mapping_gamma=method.gamma, **common)
     elif kind == "map":
         if sim.precision != "fp64":
             raise ValueError("spin mapping / eCMM run in FP64")
@@ -296,9 +304,30 @@ def _initial_map(E, sim, dist, trajectories, selection, seed):
         raise ValueError(f"unsupported distribution {type(nuc).__name__}")
 
 
+def _initial_rpmap(E, sim, dist, trajectories, selection, seed):
+    """NRPMD (a diabatic PureState, nrpmd.jl:49-65) / ring-polymer eCMM (any electronic distribution,
+    rpecmm.jl:11-25): bead configurations uploaded; the mapping variables are drawn on the device in the run."""
+    el = _electronic(dist)
+    n = sim.model.nstates
+    if isinstance(sim.method, eCMM):
+        el = PureState(1)
+    if not el.diabatic:
+        raise ValueError("NRPMD starts from a PureState in the Diabatic basis")
+    if el.state < 1 or el.state > n:
+        raise ValueError(f"electronic state {el.state} out of range 1..{n}")
+    nuc = dist.nuclear if isinstance(dist, ProductDistribution) else dist
+    conf = _configurations(nuc, trajectories, selection)
+    if conf is None:
+        raise ValueError("NRPMD / ring-polymer eCMM take bead configurations (DynamicsVariables or a list)")
+    r, v = conf
+    E.load_initial(r, v, None, state=np.full(r.shape[0], el.state - 1, np.int32))
+
+
 def _initial_conditions(E, sim, dist, trajectories, selection, seed):
     """Sample on the device where the distribution is recognised, otherwise upload."""
     kind = _kind(sim)
+    if kind == "rpmap":
+        return _initial_rpmap(E, sim, dist, trajectories, selection, seed)
     if kind == "map":
         return _initial_map(E, sim, dist, trajectories, selection, seed)
     if kind == "ie":
@@ -486,6 +515,14 @@ def _per_traj_fields(E, sim, t0, dt, save_every, nsteps):
                  "save_r": [x.reshape(shp, order="F") for x in sr[i]],
                  "save_v": [x.reshape(shp, order="F") for x in sv[i]], "ekin": ek[i], "epot": ep[i],
                  "t0": t0, "dt": dt} for i in range(E.ntraj)]
+    if kind == "rpmap":   # qmap / pmap (n, nbeads) per trajectory, as the reference's ComponentVector holds them
+        r, v, X, P = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_MAP_X), E.download(L.FIELD_MAP_P)
+        pd, ek, ep = E.download(L.FIELD_SAVE_POP_DI), E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
+        cen = E.download(L.FIELD_SAVE_CENTROID)
+        shp = sim.size()
+        return [{"r": r[i].reshape(shp, order="F"), "v": v[i].reshape(shp, order="F"), "qmap": X[i].T.copy(),
+                 "pmap": P[i].T.copy(), "pop_di": pd[i], "ekin": ek[i], "epot": ep[i], "centroid": cen[i],
+                 "steps": nsteps, "t0": t0, "dt": dt} for i in range(E.ntraj)]
     if kind == "map":
         r, v, X, P = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_MAP_X), E.download(L.FIELD_MAP_P)
         pd, ek, ep = E.download(L.FIELD_SAVE_POP_DI), E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)

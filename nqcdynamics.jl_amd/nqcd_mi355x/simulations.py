@@ -167,6 +167,18 @@ class eCMM(Method):
         self.gamma = float(gamma)
 
 
+class NRPMD(Method):
+    """Nonadiabatic ring-polymer molecular dynamics (nrpmd.jl:1-47): RingPolymerSimulation[NRPMD](atoms, model,
+    n_beads, gamma=0.5, temperature=T) mirrors RingPolymerSimulation{NRPMD}(atoms, model, n_beads; γ=0.5);
+    integrated by RingPolymerMInt (IntegrationAlgorithms.jl:103)."""
+    method_code = L.METHOD_NRPMD
+    algorithm = L.ALG_RP_MINT
+    rescaling_code = L.RESCALE_STANDARD
+
+    def __init__(self, gamma=0.5):
+        self.gamma = float(gamma)
+
+
 class AbstractSimulation:
     def __init__(self, method, atoms, model, temperature=0.0, precision="fp64", cell=None, **method_kwargs):
         self.method = method(**method_kwargs) if isinstance(method, type) else method
