@@ -11,7 +11,8 @@
 #   RingPolymerSimulation{AdiabaticIESH} / {EhrenfestNA}          BCBWavefunction
 #   Simulation{CME} / Simulation{BCME}                            Tsit5 at fixed dt
 #   Simulation{Classical}                                         VelocityVerlet
-#   Simulation{SpinMappingW}                                      MInt
+#   Simulation{SpinMappingW} / Simulation{eCMM}                   MInt
+#   RingPolymerSimulation{NRPMD} / RingPolymerSimulation{eCMM}    RingPolymerMInt
 #
 # Not executed in this repository (no Julia toolchain in the build image); the same C ABI is
 # exercised by the Python mirror (nqcd_mi355x/) and its tests.  Load it next to NQCDynamics:
@@ -42,9 +43,9 @@ const libnqcd = get(ENV, "NQCD_MI355X_LIB", joinpath(@__DIR__, "..", "libnqcd_mi
 
 # --- codes of include/nqcd_mi355x.h ------------------------------------------------------------
 const METHOD_FSSH, METHOD_EHRENFEST, METHOD_IESH, METHOD_EHRENFEST_NA, METHOD_RPMD, METHOD_RPSH, METHOD_RP_EHRENFEST,
-      METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING, METHOD_ECMM = Int32.(0:11)
+      METHOD_CME, METHOD_BCME, METHOD_CLASSICAL, METHOD_SPIN_MAPPING, METHOD_ECMM, METHOD_NRPMD = Int32.(0:12)
 const ALG_BAB_TSIT5, ALG_VERLET_ELECTRONICS, ALG_BCB, ALG_BCB_TSIT5, ALG_BCB_WAVEFUNCTION, ALG_TSIT5,
-      ALG_VELOCITY_VERLET, ALG_MINT = Int32.(0:7)
+      ALG_VELOCITY_VERLET, ALG_MINT, ALG_RP_MINT = Int32.(0:8)
 const MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSSIAN, MODEL_HARMONIC,
       MODEL_NEWNS_ANDERSON, MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE, MODEL_MORSE_N, MODEL_ANDERSON_HOLSTEIN_1 = Int32.(0:10)
 const FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS,
@@ -127,6 +128,8 @@ function method_code(sim::NQCDynamics.RingPolymerSimulation)
     m isa EhrenfestMethods.Ehrenfest && return METHOD_RP_EHRENFEST
     m isa SurfaceHoppingMethods.AdiabaticIESH && return METHOD_IESH          # with BCBWavefunction
     m isa EhrenfestMethods.EhrenfestNA && return METHOD_EHRENFEST_NA
+    m isa MappingVariableMethods.NRPMD && return METHOD_NRPMD                 # with RingPolymerMInt
+    m isa MappingVariableMethods.eCMM && return METHOD_ECMM                   # with RingPolymerMInt
     unsupported(m)
 end
 
@@ -161,10 +164,12 @@ function problem_desc(sim, alg, dt, ensemblealg, masses)
     W = method == METHOD_BCME ? Float64(sim.method.bandwidth) : 0.0
     mu = cme ? Float64(NQCModels.fermilevel(sim)) : 0.0
     # Simulation{Classical}: VelocityVerlet (classical.jl:86)
-    # Simulation{SpinMappingW | eCMM}: MInt (IntegrationAlgorithms.jl:78-79)
-    mapping = method in (METHOD_SPIN_MAPPING, METHOD_ECMM)
-    algc = method == METHOD_CLASSICAL ? ALG_VELOCITY_VERLET : mapping ? ALG_MINT : cme ? ALG_TSIT5 : algorithm_code(alg)
-    γmap = method == METHOD_ECMM ? Float64(sim.method.γ) : 0.0
+    # Simulation{SpinMappingW | eCMM}: MInt (IntegrationAlgorithms.jl:78-79); RingPolymerSimulation{NRPMD | eCMM}:
+    # RingPolymerMInt (:102-103)
+    mapping = method in (METHOD_SPIN_MAPPING, METHOD_ECMM, METHOD_NRPMD)
+    algc = method == METHOD_CLASSICAL ? ALG_VELOCITY_VERLET : mapping ? (rp ? ALG_RP_MINT : ALG_MINT) :
+           cme ? ALG_TSIT5 : algorithm_code(alg)
+    γmap = method in (METHOD_ECMM, METHOD_NRPMD) ? Float64(sim.method.γ) : 0.0
     ProblemDesc(method, algc, NQCModels.nstates(sim), size(sim)[1], size(sim)[2],
         rp ? RingPolymers.nbeads(sim) : 1, ie ? NQCModels.nelectrons(sim) : 0, cme ? Int32(2) : rescaling_code(sim.method),
         ensemblealg.fp32_nuclei ? 1 : 0, ensemblealg.persistent ? 0 : 1, 5, dec, pointer(masses), dt,
@@ -241,6 +246,7 @@ function device_sampler(prob, sim, model_kind)
     el isa NQCDistributions.PureState || return nothing
     mc = method_code(sim)
     mc in (METHOD_FSSH, METHOD_EHRENFEST, METHOD_CME, METHOD_BCME, METHOD_SPIN_MAPPING, METHOD_ECMM) || return nothing
+    sim isa NQCDynamics.RingPolymerSimulation && return nothing   # ring polymers upload their bead configurations
     nuc = d.nuclear
     n = prod(size(sim))
     mv = gaussian_moments(nuc.velocity, n); mr = gaussian_moments(nuc.position, n)
@@ -285,7 +291,9 @@ function load_initial!(h, prob, sim, trajectories, seed, model_kind)
             h, trajectories, r, v, C_NULL, C_NULL, C_NULL, C_NULL))
         return
     end
-    if method_code(sim) in (METHOD_SPIN_MAPPING, METHOD_ECMM)   # (v, r, qmap, pmap): spin_mapping.jl:15-33, cmm.jl:26-34
+    # (v, r, qmap, pmap): spin_mapping.jl:15-33, cmm.jl:26-34; ring polymers (F, nbeads) qmap / pmap, nrpmd.jl:49-65,
+    # rpecmm.jl:11-25
+    if method_code(sim) in (METHOD_SPIN_MAPPING, METHOD_ECMM, METHOD_NRPMD)
         n = NQCModels.nstates(sim)
         X = reduce(hcat, [vec(MappingVariableMethods.get_mapping_positions(u)) for u in us])
         P = reduce(hcat, [vec(MappingVariableMethods.get_mapping_momenta(u)) for u in us])
@@ -480,12 +488,13 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
         ie = method_code(sim) in (METHOD_IESH, METHOD_EHRENFEST_NA)
         cme = method_code(sim) in (METHOD_CME, METHOD_BCME)
         cls = method_code(sim) == METHOD_CLASSICAL
-        spinmap = method_code(sim) in (METHOD_SPIN_MAPPING, METHOD_ECMM)
+        spinmap = method_code(sim) in (METHOD_SPIN_MAPPING, METHOD_ECMM, METHOD_NRPMD)
         two_state = !ie && !cme && !cls && !spinmap && !(sim isa NQCDynamics.RingPolymerSimulation) && n == 2
         device_reducible(f) = begin
             nm = output_name(f)
             cls && return nm in (:OutputKineticEnergy, :OutputTotalEnergy, :OutputPosition, :OutputVelocity)
-            spinmap && return nm in (:OutputDiabaticPopulation, :OutputKineticEnergy, :OutputTotalEnergy)
+            spinmap && return nm in (:OutputDiabaticPopulation, :OutputKineticEnergy, :OutputTotalEnergy) ||
+                              (nm === :OutputCentroidPosition && sim isa NQCDynamics.RingPolymerSimulation)
             cme && return nm in (:OutputKineticEnergy, :OutputPosition) ||
                           (nm === :OutputTotalEnergy && method_code(sim) == METHOD_CME)
             nm in (:OutputAdiabaticPopulation, :OutputDiabaticPopulation, :OutputCentroidPosition,
