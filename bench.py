@@ -342,6 +342,29 @@ def mapping_extra(L):
                      "traj_steps_per_s": rate, "grid": st["grid"], "block": st["block"],
                      "final_diabatic_population": [float(x) for x in pop[-1]]}
         _progress(f"spin mapping {name}: {rate:.3e} trajectory-steps/s")
+    # NRPMD by RingPolymerMInt: 1e5 ring polymers of 8 beads on Tully's model 1 (gamma 0.5, T = 1e-3, beads
+    # ~ N(-5, 0.05), p0 ~ N(15, 1) shared by the beads, dt = 1), occupied diabat 1
+    nrp, P = 100_000, 8
+    rng = np.random.default_rng(12)
+    r0 = -5.0 + 0.05 * rng.standard_normal((nrp, P))
+    v0 = np.repeat((15.0 + rng.standard_normal((nrp, 1))) / 2000.0, P, axis=1)
+    E = L.Ensemble(0)
+    E.set_problem(L.METHOD_NRPMD, 2, 1, 1, [2000.0], 1.0, algorithm=L.ALG_RP_MINT, nbeads=P, temperature=1e-3,
+                  mapping_gamma=0.5)
+    E.set_model(0, L.MODEL_TULLY1, [0.01, 1.6, 0.005, 1.0])
+    E.load_initial(r0, v0, None, state=np.zeros(nrp, np.int32))
+    st = E.run(0.0, float(nsteps), KEY, save_every=se)
+    pop = E.reduce(L.OUT_DIABATIC_POPULATION, L.RED_MEAN, 2 * E.nsave).reshape(E.nsave, 2)
+    m = 128
+    sample["nrpmd_P8"] = {"kind": 0, "prm": [0.01, 1.6, 0.005, 1.0], "n": 2, "P": P, "mass": 2000.0, "dt": 1.0,
+                          "nsteps": nsteps, "save_every": se, "r0": r0[:m].copy(), "v0": v0[:m].copy(),
+                          "pop": E.download(L.FIELD_SAVE_POP_DI)[:m].copy()}
+    E.close()
+    rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+    res["nrpmd_P8"] = {"ring_polymers": nrp, "nbeads": P, "nsteps": nsteps, "dt": 1.0, "kernel_ms": st["kernel_ms"],
+                       "ring_polymer_steps_per_s": rate, "bead_steps_per_s": rate * P, "grid": st["grid"],
+                       "block": st["block"], "final_diabatic_population": [float(x) for x in pop[-1]]}
+    _progress(f"NRPMD P=8: {rate:.3e} ring-polymer steps/s")
     return res, sample
 
 
@@ -351,6 +374,14 @@ def _check_mapping(O, S, nthreads):
     out = {}
     for name, d in S.items():
         n = d["n"]
+        if "P" in d:   # NRPMD ring polymers
+            k = len(d["r0"])
+            m = O.make_model(O.MODEL_TULLY1, 2, 1, d["prm"])
+            q0, p0 = O.rpmap_sample(O.RPMAP_NRPMD, n, d["P"], 0.5, k, KEY, state=np.zeros(k, np.int32))
+            c = O.rpmap_run(O.RPMAP_NRPMD, m, n, d["P"], d["mass"], 1e-3, 0.5, d["r0"], d["v0"], q0, p0, dt=d["dt"],
+                            nsteps=d["nsteps"], save_every=d["save_every"], nthreads=nthreads)
+            out[name] = {"ring_polymers": k, "max_abs_err_population": float(np.abs(c["pop"] - d["pop"]).max())}
+            continue
         m = (O.make_model(O.MODEL_TULLY1, 2, 1, d["prm"]) if d["kind"] == 0 else
              O.make_model(O.MODEL_MORSE_N, n, 1, d["prm"][1:]))
         k = len(d["r0"])
