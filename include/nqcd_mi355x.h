@@ -258,7 +258,10 @@ enum nqcd_field {
                                       ehrenfest.jl:75-83); two-state one-dof FSSH / Ehrenfest */
     NQCD_FIELD_SAVE_V = 16,        /* double (nsave, ntraj) velocity per save (classical dynamics) */
     NQCD_FIELD_MAP_X = 17,         /* double (n[, nbeads], ntraj) final mapping positions (u.qmap) */
-    NQCD_FIELD_MAP_P = 18          /* double (n[, nbeads], ntraj) final mapping momenta (u.pmap) */
+    NQCD_FIELD_MAP_P = 18,         /* double (n[, nbeads], ntraj) final mapping momenta (u.pmap) */
+    NQCD_FIELD_INIT_POP_DI = 19    /* double (n, ntraj) mapping methods: the initial estimator of the population
+                                      correlation at t0 (initial_diabatic_population, Estimators.jl:121, cmm.jl:106-110,
+                                      rpecmm.jl:95-103) */
 };
 /* IESH / EhrenfestNA shapes: SIGMA_RE / SIGMA_IM hold the wavefunction psi (n, n_electrons, ntraj),
  * STATE the occupied adiabatic states (n_electrons, ntraj), SAVE_POP_AD / SAVE_POP_DI (n, nsave, ntraj). */

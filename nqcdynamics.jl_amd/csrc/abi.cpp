@@ -760,12 +760,22 @@ int32_t reduce_plan(nqcd_handle_t* h, int32_t kind, int32_t red, const double* o
         else if (kind == NQCD_OUT_CENTROID_POSITION && h->prob.algorithm == NQCD_ALG_RP_MINT) {
             rows = h->nsave; src = h->save_centroid.as<double>();
         }
-        else return fail(h, NQCD_ERR_UNSUPPORTED, "mapping outputs: DIABATIC_POPULATION, KINETIC_ENERGY, TOTAL_ENERGY "
-                                                  "(CENTROID_POSITION for ring polymers)");
+        else if (kind == NQCD_OUT_POPULATION_CORRELATION) rows = n * n * h->nsave;
+        else return fail(h, NQCD_ERR_UNSUPPORTED, "mapping outputs: DIABATIC_POPULATION, POPULATION_CORRELATION, "
+                                                  "KINETIC_ENERGY, TOTAL_ENERGY (CENTROID_POSITION for ring polymers)");
         if (len != rows) return fail(h, NQCD_ERR_ARG, "len does not match the output size");
         P = ReducePlan{};
         P.rows = rows;
         P.src = src;
+        if (kind == NQCD_OUT_POPULATION_CORRELATION) {   // normalisation F for eCMM (cmm.jl:114-119), 1 otherwise
+            P.fill = true;
+            P.S.kind = kind;
+            P.S.nsave = h->nsave;
+            P.S.nst = n;
+            P.S.save_pop_di = h->save_di.as<double>();
+            P.S.init_pop = h->final_di.as<double>();
+            P.S.norm = h->prob.method == NQCD_METHOD_ECMM ? (double)n : 1.0;
+        }
         P.src2 = kind == NQCD_OUT_TOTAL_ENERGY ? h->save_epot.as<double>() : nullptr;
         P.scale = (red == NQCD_RED_MEAN) ? 1.0 / (double)nt : 1.0;
         return NQCD_OK;

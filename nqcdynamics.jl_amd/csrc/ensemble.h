@@ -25,8 +25,13 @@ struct OutputSrc {
     const double* r = nullptr;    // [D][ntraj]
     const double* sig = nullptr;  // [4][ntraj]
     const int32_t* state = nullptr;
-    const double* save_pop_di = nullptr;  // [nsave][2][ntraj]
+    const double* save_pop_di = nullptr;  // [nsave][nst][ntraj]
     const double* final_pop_di = nullptr; // [2][ntraj] diabatic populations of the final state
+    // population correlation: states, the initial estimator [nst][ntraj] (null: save slot 0) and the
+    // normalisation (TimeCorrelationFunctions.jl:26-40)
+    int32_t nst = 2, pad = 0;
+    const double* init_pop = nullptr;
+    double norm = 1.0;
 };
 hipError_t launch_fill_output2(const OutputSrc& S, int64_t ntraj, double* out, hipStream_t s);
 int32_t fill_output2_rows(int32_t kind, int32_t nsave);

@@ -66,8 +66,9 @@ __global__ __launch_bounds__(256) void init_density2(int64_t ntraj, int32_t stat
 // Per-trajectory output rows for the two-state kernels.
 // ---------------------------------------------------------------------------
 // Number of rows of an output computed from the final state or the save slots.
-__host__ __device__ inline int32_t output2_rows(int32_t kind, int32_t nsave) {
-    return kind == OUTK_FINAL_ADIABATIC ? 2 : (kind == OUTK_SCATTERING_1D || kind == OUTK_SCATTERING_1D_DIABATIC) ? 4 : 4 * nsave;
+__host__ __device__ inline int32_t output2_rows(int32_t kind, int32_t nsave, int32_t nst = 2) {
+    return kind == OUTK_FINAL_ADIABATIC ? 2 : (kind == OUTK_SCATTERING_1D || kind == OUTK_SCATTERING_1D_DIABATIC) ? 4
+                                                                                                     : nst * nst * nsave;
 }
 
 // Row `row` of trajectory i (one expression shared by the materialising and the fused reductions,
@@ -85,16 +86,18 @@ __device__ __forceinline__ double output2_value(const OutputSrc& S, int64_t ntra
         const double p = (row & 1) ? p1 : p0;
         return (row < 2) ? (trans ? 0.0 : p) : (trans ? p : 0.0);
     }
-    // OUTK_POP_CORRELATION: row = 4 slot + (a + 2 b) holds P_a(0) P_b(t_slot) (TimeCorrelationFunctions.jl:86-88),
-    // column-major 2x2 per slot
-    const int32_t slot = row >> 2, a = row & 1, b = (row >> 1) & 1;
-    return S.save_pop_di[(int64_t)a * ntraj + i] * S.save_pop_di[((int64_t)slot * 2 + b) * ntraj + i];
+    // OUTK_POP_CORRELATION: row = n^2 slot + (a + n b) holds norm K_a(0) P_b(t_slot), column-major n x n per slot
+    // (TimeCorrelationFunctions.jl:26-40,86-88), K the initial estimator (the slot-0 population unless given)
+    const int32_t n = S.nst, nn = n * n;
+    const int32_t slot = row / nn, ab = row - slot * nn, a = ab % n, b = ab / n;
+    const double* K0 = S.init_pop ? S.init_pop : S.save_pop_di;
+    return S.norm * K0[(int64_t)a * ntraj + i] * S.save_pop_di[((int64_t)slot * n + b) * ntraj + i];
 }
 
 __global__ __launch_bounds__(256) void fill_output2(OutputSrc S, int64_t ntraj, double* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntraj) return;
-    const int32_t rows = output2_rows(S.kind, S.nsave);
+    const int32_t rows = output2_rows(S.kind, S.nsave, S.nst);
     for (int32_t row = 0; row < rows; ++row) out[(int64_t)row * ntraj + i] = output2_value(S, ntraj, row, i);
 }
 

@@ -79,6 +79,8 @@ struct MapArgs {
     double* save_pop;        // [slot][state][traj]
     double* save_ekin;       // [slot][traj]
     double* save_epot;
+    double* init_pop;        // [state][traj] initial estimator (X^2 + P^2) / 2 - k0_b of the population correlation
+    double k0_b;
 };
 
 // Potential part of H at coordinate x for mapping variables X, P.
@@ -193,6 +195,8 @@ __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
                 P[i] = rad * c;
             }
         }
+#pragma unroll
+        for (int i = 0; i < NM; ++i) A.init_pop[(int64_t)i * nt + tid] = (X[i] * X[i] + P[i] * P[i]) * 0.5 - A.k0_b;
         auto save = [&](int slot) {
 #pragma unroll
             for (int i = 0; i < NM; ++i)
@@ -264,6 +268,8 @@ struct RpMapArgs {
     double* save_ekin;                // kinetic energy
     double* save_epot;                // potential energy with the springs
     double* save_centroid;
+    double* init_pop;                 // [state][ring polymer] k0_scale sum_b [(X^2 + P^2) / 2 - k0_b] at t0
+    double k0_b, k0_scale;
 };
 
 // sum over the lanes g P .. g P + P - 1 of val, returned to every lane of the group (LDS buf of 64 doubles)
@@ -364,6 +370,11 @@ __global__ __launch_bounds__(64) void rpmap_kernel(const RpMapArgs A) {
                 X[i] = rad * cs;
                 Pm[i] = rad * sn;
             }
+        }
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            const double k0 = A.k0_scale * rpmap_group_sum(buf, l, g, P, (X[i] * X[i] + Pm[i] * Pm[i]) * 0.5 - A.k0_b);
+            if (on && b == 0) A.init_pop[(int64_t)i * nrp + rp] = k0;
         }
         auto save = [&](int slot) {
             const double rb = rpmap_transform(buf, U, l, g, P, b, q, false);
@@ -614,6 +625,11 @@ static int32_t map_run_rp(nqcd_handle_t* h, int64_t nsteps, uint64_t key, const 
     A.save_ekin = h->save_ekin.as<double>();
     A.save_epot = h->save_epot.as<double>();
     A.save_centroid = h->save_centroid.as<double>();
+    // initial estimator of the population correlation: NRPMD's population itself (Estimators.jl:121), eCMM's
+    // bead-summed mapping kernel (rpecmm.jl:95-103)
+    A.init_pop = h->final_di.as<double>();
+    A.k0_b = g;
+    A.k0_scale = ecmm ? 1.0 : 1.0 / P;
     const int block = 64, G = 64 / P;
     const size_t lds = sizeof(double) * ((size_t)P * P + 64);
     int bpc = 0;
@@ -649,6 +665,7 @@ int32_t map_run(nqcd_handle_t* h, double t0, int64_t nsteps, uint64_t key) {
     HIPCHK(h, h->v.ensure(sizeof(double) * B * nt));
     HIPCHK(h, h->sig.ensure(sizeof(double) * 2 * n * B * nt));
     HIPCHK(h, h->steps.ensure(sizeof(int32_t) * nt));
+    HIPCHK(h, h->final_di.ensure(sizeof(double) * n * nt));
     if (map_rp(h)) {
         HIPCHK(h, h->save_centroid.ensure(sizeof(double) * ns * nt));
         HIPCHK(h, launch_zero(h->save_centroid.p, sizeof(double) * ns * nt, h->stream));
@@ -708,6 +725,10 @@ int32_t map_run(nqcd_handle_t* h, double t0, int64_t nsteps, uint64_t key) {
     A.save_pop = h->save_di.as<double>();
     A.save_ekin = h->save_ekin.as<double>();
     A.save_epot = h->save_epot.as<double>();
+    // initial estimator of the population correlation: spin mapping's population (Estimators.jl:121), eCMM's
+    // mapping kernel (X^2 + P^2) / 2 - gamma (cmm.jl:106-110)
+    A.init_pop = h->final_di.as<double>();
+    A.k0_b = ecmm ? ge : 0.5 * gs;
     const int block = 256;
     int bpc = 0;
     switch (n) {
@@ -775,6 +796,13 @@ int32_t map_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes) 
             for (int i = 0; i < rows; ++i) o[(size_t)t * rows + i] = tmp[(size_t)i * nt + t];
         return NQCD_OK;
     }
+    case NQCD_FIELD_INIT_POP_DI: {
+        if (int32_t rc = need(8 * (int64_t)n * nt)) return rc;
+        HIPCHK(h, fetch(h->final_di.p, (size_t)n * nt));
+        for (int64_t t = 0; t < nt; ++t)
+            for (int i = 0; i < n; ++i) o[(size_t)t * n + i] = tmp[(size_t)i * nt + t];
+        return NQCD_OK;
+    }
     case NQCD_FIELD_SAVE_POP_DI: {
         if (int32_t rc = need(8 * (int64_t)n * ns * nt)) return rc;
         HIPCHK(h, fetch(h->save_di.p, (size_t)n * ns * nt));
@@ -792,7 +820,8 @@ int32_t map_download(nqcd_handle_t* h, int32_t field, void* out, int64_t bytes) 
         return NQCD_OK;
     }
     default:
-        return fail(h, NQCD_ERR_UNSUPPORTED, "mapping fields: R, V, STEPS, MAP_X, MAP_P, SAVE_POP_DI, SAVE_EKIN, SAVE_EPOT (and SAVE_CENTROID for ring polymers)");
+        return fail(h, NQCD_ERR_UNSUPPORTED, "mapping fields: R, V, STEPS, MAP_X, MAP_P, INIT_POP_DI, SAVE_POP_DI, SAVE_EKIN, SAVE_EPOT (and SAVE_CENTROID for "
+                                             "ring polymers)");
     }
 }
 

@@ -38,7 +38,7 @@ CELL_NONE, CELL_BOUNDARY, CELL_TERMINATE = range(3)
 RED_MEAN, RED_SUM = range(2)
 (FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS, FIELD_SAVE_POP_AD,
  FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID, FIELD_FALLBACKS, FIELD_SAVE_R, FIELD_SAVE_EKIN,
- FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V, FIELD_MAP_X, FIELD_MAP_P) = range(19)
+ FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V, FIELD_MAP_X, FIELD_MAP_P, FIELD_INIT_POP_DI) = range(20)
 
 EXPORTED_SYMBOLS = (
     "nqcd_mi355x_abi_version", "nqcd_mi355x_open", "nqcd_mi355x_close", "nqcd_mi355x_set_problem",
@@ -336,6 +336,7 @@ class Ensemble:
                   FIELD_SAVE_R: ((nt, nsave, D), np.float64), FIELD_SAVE_EKIN: ((nt, nsave), np.float64),
                   FIELD_SAVE_EPOT: ((nt, nsave), np.float64), FIELD_FINAL_POP_DI: ((nt, n), np.float64),
                   FIELD_SAVE_V: ((nt, nsave), np.float64)}
+        shapes[FIELD_INIT_POP_DI] = ((nt, n), np.float64)
         mb = getattr(self, "map_beads", 0)
         shapes[FIELD_MAP_X] = shapes[FIELD_MAP_P] = (((nt, n) if mb == 0 else (nt, mb, n)), np.float64)
         if getattr(self, "method", None) in (METHOD_CME, METHOD_BCME, METHOD_CLASSICAL):  # one coordinate

@@ -518,18 +518,21 @@ def _per_traj_fields(E, sim, t0, dt, save_every, nsteps):
     if kind == "rpmap":   # qmap / pmap (n, nbeads) per trajectory, as the reference's ComponentVector holds them
         r, v, X, P = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_MAP_X), E.download(L.FIELD_MAP_P)
         pd, ek, ep = E.download(L.FIELD_SAVE_POP_DI), E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
-        cen = E.download(L.FIELD_SAVE_CENTROID)
+        cen, k0 = E.download(L.FIELD_SAVE_CENTROID), E.download(L.FIELD_INIT_POP_DI)
+        norm = float(sim.model.nstates) if isinstance(sim.method, eCMM) else 1.0
         shp = sim.size()
         return [{"r": r[i].reshape(shp, order="F"), "v": v[i].reshape(shp, order="F"), "qmap": X[i].T.copy(),
                  "pmap": P[i].T.copy(), "pop_di": pd[i], "ekin": ek[i], "epot": ep[i], "centroid": cen[i],
-                 "steps": nsteps, "t0": t0, "dt": dt} for i in range(E.ntraj)]
+                 "pop_init": k0[i], "corr_norm": norm, "steps": nsteps, "t0": t0, "dt": dt} for i in range(E.ntraj)]
     if kind == "map":
         r, v, X, P = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_MAP_X), E.download(L.FIELD_MAP_P)
         pd, ek, ep = E.download(L.FIELD_SAVE_POP_DI), E.download(L.FIELD_SAVE_EKIN), E.download(L.FIELD_SAVE_EPOT)
+        k0 = E.download(L.FIELD_INIT_POP_DI)
+        norm = float(sim.model.nstates) if isinstance(sim.method, eCMM) else 1.0
         shp = sim.size()
         return [{"r": r[i].reshape(shp, order="F"), "v": v[i].reshape(shp, order="F"), "qmap": X[i], "pmap": P[i],
-                 "pop_di": pd[i], "ekin": ek[i], "epot": ep[i], "steps": nsteps, "t0": t0, "dt": dt}
-                for i in range(E.ntraj)]
+                 "pop_di": pd[i], "ekin": ek[i], "epot": ep[i], "pop_init": k0[i], "corr_norm": norm,
+                 "steps": nsteps, "t0": t0, "dt": dt} for i in range(E.ntraj)]
     if kind == "cme":
         r, v, st = E.download(L.FIELD_R), E.download(L.FIELD_V), E.download(L.FIELD_STATE)
         nh, stp = E.download(L.FIELD_NHOPS), E.download(L.FIELD_STEPS)

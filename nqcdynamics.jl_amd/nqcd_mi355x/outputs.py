@@ -77,7 +77,8 @@ class OutputStateResolvedScattering1D(Output):
 
 
 class PopulationCorrelationFunction(Output):
-    """P(0) P(t)^T per save point (TimeCorrelationFunctions.jl:26-40,61-88)."""
+    """norm K(0) P(t)^T per save point (TimeCorrelationFunctions.jl:26-40,61-88): K the initial estimator (the
+    population itself but for eCMM's mapping kernel), norm 1 (F for eCMM, cmm.jl:114-119)."""
     name = "PopulationCorrelationFunction"
 
     def __init__(self, sim, statetype=None):
@@ -89,7 +90,9 @@ class PopulationCorrelationFunction(Output):
 
     def per_trajectory(self, res):
         P = res["pop_di"] if self.diabatic else res["pop_ad"]
-        return [np.outer(P[0], p) for p in P]
+        K0 = res.get("pop_init", P[0]) if self.diabatic else P[0]
+        norm = res.get("corr_norm", 1.0)
+        return [norm * np.outer(K0, p) for p in P]
 
 
 # --- reductions -----------------------------------------------------------------------------

@@ -1637,8 +1637,12 @@ OR_API void oracle_mapping_motion(const OMapDesc* D, const OModelDesc* m, const 
 // Trajectories by MInt: x, v (ntraj), X, P ([traj][n]) in place; per save slot the diabatic
 // populations (spin mapping: (X_i^2 + P_i^2 - gamma) / 2, spin_mapping.jl:120-131; eCMM: the inverse
 // mapping kernel, cmm.jl:99-112), the kinetic and the potential energy (H - m v^2 / 2).
+// sinit (ntraj, n): the initial estimator of the population correlation at t0 (initial_diabatic_population:
+// the population itself for spin mapping, Estimators.jl:121; eCMM's mapping kernel (X^2 + P^2)/2 - gamma,
+// cmm.jl:106-110).
 OR_API int32_t oracle_mapping_run(const OMapDesc* D, const OModelDesc* m, int64_t ntraj, double* x, double* v,
-                                  double* X, double* P, double* spop, double* sek, double* sep, int32_t nthreads) {
+                                  double* X, double* P, double* spop, double* sek, double* sep, double* sinit,
+                                  int32_t nthreads) {
     const int n = D->nstates, ns = D->nsave;
     parallel_for(ntraj, nthreads, [&](int64_t t) {
         double* Xt = X + t * n;
@@ -1651,6 +1655,9 @@ OR_API int32_t oracle_mapping_run(const OMapDesc* D, const OModelDesc* m, int64_
             sek[(size_t)t * ns + slot] = ek;
             sep[(size_t)t * ns + slot] = mapping_energy(*D, *m, x[t], v[t], Xt, Pt) - ek;
         };
+        for (int i = 0; i < n; ++i)
+            sinit[(size_t)t * n + i] = D->method == 1 ? (Xt[i] * Xt[i] + Pt[i] * Pt[i]) / 2.0 - D->gamma
+                                                      : map_population(*D, Xt[i], Pt[i]);
         save(0);
         for (int64_t step = 0; step < D->nsteps; ++step) {
             mint_step(*D, *m, x[t], v[t], Xt, Pt, Xi.data());
@@ -1863,8 +1870,11 @@ OR_API void oracle_rpmap_motion(const ORpMapDesc* D, const OModelDesc* m, const 
 // populations (NRPMD: the bead mean of (q^2 + p^2)/2 - gamma, nrpmd.jl:110-121; eCMM: the bead sum of the
 // inverse mapping kernel with N beads, rpecmm.jl:80-94), the kinetic energy, the potential energy with the
 // springs, and the centroid.
+// sinit (ntraj, F): the initial estimator at t0 (NRPMD: its population, Estimators.jl:121; eCMM: the bead sum of
+// the mapping kernel (q^2 + p^2)/2 - gamma, rpecmm.jl:95-103).
 OR_API int32_t oracle_rpmap_run(const ORpMapDesc* D, const OModelDesc* m, int64_t ntraj, double* r, double* v, double* q,
-                                double* p, double* spop, double* sek, double* sep, double* scen, int32_t nthreads) {
+                                double* p, double* spop, double* sek, double* sep, double* scen, double* sinit,
+                                int32_t nthreads) {
     const int P = D->nbeads, F = D->nstates, ns = D->nsave;
     const double wn = P * D->temperature;
     std::vector<double> U(P * P), Ch(4 * P);
@@ -1895,6 +1905,11 @@ OR_API int32_t oracle_rpmap_run(const ORpMapDesc* D, const OModelDesc* m, int64_
             sep[(size_t)t * ns + slot] = ep;
             scen[(size_t)t * ns + slot] = cen / P;
         };
+        for (int i = 0; i < F; ++i) {
+            double k = 0.0;
+            for (int b = 0; b < P; ++b) k += (qt[b * F + i] * qt[b * F + i] + pt[b * F + i] * pt[b * F + i]) / 2.0 - g;
+            sinit[(size_t)t * F + i] = D->method == 0 ? k / P : k;
+        }
         save(0);
         for (int64_t step = 0; step < D->nsteps; ++step) {
             rpmap_step(*D, *m, U, Ch, rt, vt, qt, pt);

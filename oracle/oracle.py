@@ -458,13 +458,14 @@ def mapping_run(model, n, mass, r0, v0, X0, P0, dt=0.01, nsteps=100, save_every=
                v=np.ascontiguousarray(v0, dtype=np.float64).ravel().copy(),
                X=np.ascontiguousarray(X0, dtype=np.float64).reshape(ntraj, n).copy(),
                P=np.ascontiguousarray(P0, dtype=np.float64).reshape(ntraj, n).copy(),
-               pop=np.zeros((ntraj, D.nsave, n)), ekin=np.zeros((ntraj, D.nsave)), epot=np.zeros((ntraj, D.nsave)))
+               pop=np.zeros((ntraj, D.nsave, n)), ekin=np.zeros((ntraj, D.nsave)), epot=np.zeros((ntraj, D.nsave)),
+               init=np.zeros((ntraj, n)))
     if nthreads is None:
         nthreads = min(16, os.cpu_count() or 1)
     f = lib().oracle_mapping_run
     f.restype = C.c_int32
     f(C.byref(D), C.byref(model), C.c_int64(ntraj), _p(out["x"]), _p(out["v"]), _p(out["X"]), _p(out["P"]),
-      _p(out["pop"]), _p(out["ekin"]), _p(out["epot"]), C.c_int32(nthreads))
+      _p(out["pop"]), _p(out["ekin"]), _p(out["epot"]), _p(out["init"]), C.c_int32(nthreads))
     return out
 
 
@@ -528,13 +529,13 @@ def rpmap_run(method, model, n, P, mass, temperature, gamma, r0, v0, q0, p0, dt=
                q=np.ascontiguousarray(q0, dtype=np.float64).reshape(ntraj, P, n).copy(),
                p=np.ascontiguousarray(p0, dtype=np.float64).reshape(ntraj, P, n).copy(),
                pop=np.zeros((ntraj, D.nsave, n)), ekin=np.zeros((ntraj, D.nsave)), epot=np.zeros((ntraj, D.nsave)),
-               centroid=np.zeros((ntraj, D.nsave)))
+               centroid=np.zeros((ntraj, D.nsave)), init=np.zeros((ntraj, n)))
     if nthreads is None:
         nthreads = min(16, os.cpu_count() or 1)
     f = lib().oracle_rpmap_run
     f.restype = C.c_int32
     f(C.byref(D), C.byref(model), C.c_int64(ntraj), _p(out["r"]), _p(out["v"]), _p(out["q"]), _p(out["p"]),
-      _p(out["pop"]), _p(out["ekin"]), _p(out["epot"]), _p(out["centroid"]), C.c_int32(nthreads))
+      _p(out["pop"]), _p(out["ekin"]), _p(out["epot"]), _p(out["centroid"]), _p(out["init"]), C.c_int32(nthreads))
     return out
 
 

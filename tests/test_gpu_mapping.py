@@ -44,6 +44,9 @@ def _device(L, kind, params, n, mass, dt, r0, v0, state, nsteps, save_every, key
     g["red_ekin"] = E.reduce(L.OUT_KINETIC_ENERGY, L.RED_SUM, E.nsave)
     g["red_etot"] = E.reduce(L.OUT_TOTAL_ENERGY, L.RED_MEAN, E.nsave)
     g["noalloc_pop"] = E.reduce_noalloc(L.OUT_DIABATIC_POPULATION, L.RED_MEAN, n * E.nsave).reshape(E.nsave, n)
+    # column-major n x n per slot: element [slot, b, a] = norm K_a(0) P_b(t_slot)
+    g["red_corr"] = E.reduce(L.OUT_POPULATION_CORRELATION, L.RED_MEAN, n * n * E.nsave).reshape(E.nsave, n, n)
+    g["noalloc_corr"] = E.reduce_noalloc(L.OUT_POPULATION_CORRELATION, L.RED_MEAN, n * n * E.nsave).reshape(E.nsave, n, n)
     g["stats"] = E.stats()
     E.close()
     return g
@@ -76,6 +79,10 @@ def test_mapping_trajectories_match_oracle(oracle, key, name):
     assert np.array_equal(dev["noalloc_pop"], dev["red_pop"])
     assert np.allclose(dev["red_ekin"], dev["SAVE_EKIN"].sum(axis=0), rtol=1e-12, atol=1e-15)
     assert np.allclose(dev["red_etot"], (dev["SAVE_EKIN"] + dev["SAVE_EPOT"]).mean(axis=0), rtol=1e-12, atol=1e-15)
+    # PopulationCorrelationFunction(sim, Diabatic()) (TimeCorrelationFunctions.jl:26-40,61-88): K(0) P(t)^T
+    want = np.einsum("ta,tsb->sba", c["init"], c["pop"]) / ntraj
+    assert np.allclose(dev["red_corr"], want, rtol=0, atol=1e-9)
+    assert np.array_equal(dev["noalloc_corr"], dev["red_corr"])
 
 
 def test_mapping_uploaded_variables(oracle, key):
@@ -214,6 +221,9 @@ def test_ecmm_trajectories_match_oracle(oracle, key, name, gamma):
     assert np.allclose(dev["SAVE_EKIN"], c["ekin"], rtol=1e-9, atol=1e-13)
     assert np.allclose(dev["SAVE_EPOT"], c["epot"], rtol=1e-9, atol=1e-12)
     assert np.allclose(dev["red_pop"], dev["SAVE_POP_DI"].mean(axis=0), rtol=1e-12, atol=1e-15)
+    # eCMM's correlation: normalisation F, initial estimator the mapping kernel (cmm.jl:106-119)
+    want = n * np.einsum("ta,tsb->sba", c["init"], c["pop"]) / ntraj
+    assert np.allclose(dev["red_corr"], want, rtol=0, atol=1e-9)
 
 
 def test_ecmm_run_dynamics(oracle, key):
@@ -235,3 +245,32 @@ def test_ecmm_run_dynamics(oracle, key):
                            method=oracle.MAP_ECMM, gamma=gamma)
     assert np.allclose(np.array(res["OutputDiabaticPopulation"]), c["pop"].mean(axis=0), rtol=0, atol=1e-10)
     assert np.allclose(res["OutputTotalEnergy"], (c["ekin"] + c["epot"]).mean(axis=0), rtol=1e-10, atol=1e-14)
+
+
+def test_mapping_population_correlation_paths(oracle, key):
+    """PopulationCorrelationFunction(sim, Diabatic()) through run_dynamics for eCMM: the device MeanReduction equals
+    the mean of the per-trajectory values (SortByTrajectoryReduction, built from the downloaded initial estimator
+    and saves) and, at t0, F times the mean of K(0) P(0)^T."""
+    import nqcd_mi355x as N
+    ntraj, dt, tf = 512, 1.0, 300.0
+    sim = N.Simulation[N.eCMM](N.Atoms(2000.0), N.TullyModelOne(), gamma=0.3)
+    dist = N.DynamicalDistribution(N.Normal(12.0 / 2000.0, 1.0 / 2000.0), N.Normal(-2.0, 0.5), (1, 1)) * \
+        N.PureState(1, N.Diabatic())
+    corr = N.PopulationCorrelationFunction(sim, N.Diabatic())
+    alg = N.EnsembleMI355X()
+    print("eCMM population correlation", flush=True)
+    mean = N.run_dynamics(sim, (0.0, tf), dist, output=corr, trajectories=ntraj, dt=dt, saveat=50.0,
+                          reduction=N.MeanReduction(), ensemble_algorithm=alg)
+    per = N.run_dynamics(sim, (0.0, tf), dist, output=corr, trajectories=ntraj, dt=dt, saveat=50.0,
+                         reduction=N.SortByTrajectoryReduction(), ensemble_algorithm=alg)
+    host = np.mean([np.array(p["PopulationCorrelationFunction"]) for p in per], axis=0)
+    dev = np.array(mean["PopulationCorrelationFunction"])
+    assert dev.shape == host.shape == (7, 2, 2)
+    assert np.allclose(dev, host, rtol=1e-12, atol=1e-14)
+    r0, v0 = oracle.sample_gaussian(ntraj, 1, -2.0, 0.5, 12.0 / 2000.0, 1.0 / 2000.0, alg.seed)
+    X0, P0 = oracle.ecmm_sample(2, ntraj, 0.3, alg.seed)
+    om = oracle.make_model(oracle.MODEL_TULLY1, 2, 1, TULLY1)
+    c = oracle.mapping_run(om, 2, 2000.0, r0[:, 0], v0[:, 0], X0, P0, dt=dt, nsteps=int(tf / dt), save_every=50,
+                           method=oracle.MAP_ECMM, gamma=0.3)
+    want = 2 * np.einsum("ta,tsb->sab", c["init"], c["pop"]) / ntraj
+    assert np.allclose(dev, want, rtol=0, atol=1e-9)

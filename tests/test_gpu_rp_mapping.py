@@ -50,6 +50,8 @@ def _device(L, case, r0, v0, state, nsteps, se, key, X=None, P_=None):
     out["red_pop"] = E.reduce(L.OUT_DIABATIC_POPULATION, L.RED_MEAN, n * E.nsave).reshape(E.nsave, n)
     out["red_etot"] = E.reduce(L.OUT_TOTAL_ENERGY, L.RED_SUM, E.nsave)
     out["red_cen"] = E.reduce(L.OUT_CENTROID_POSITION, L.RED_MEAN, E.nsave)
+    out["red_corr"] = E.reduce(L.OUT_POPULATION_CORRELATION, L.RED_MEAN, n * n * E.nsave).reshape(E.nsave, n, n)
+    out["norm"] = n if method == "ECMM" else 1
     E.close()
     return out
 
@@ -75,6 +77,8 @@ def _check(dev, c, nsteps):
     assert np.allclose(dev["red_pop"], dev["SAVE_POP_DI"].mean(axis=0), rtol=1e-12, atol=1e-15)
     assert np.allclose(dev["red_etot"], (dev["SAVE_EKIN"] + dev["SAVE_EPOT"]).sum(axis=0), rtol=1e-12, atol=1e-13)
     assert np.allclose(dev["red_cen"], dev["SAVE_CENTROID"].mean(axis=0), rtol=1e-12, atol=1e-14)
+    want = dev["norm"] * np.einsum("ta,tsb->sba", c["init"], c["pop"]) / len(c["init"])
+    assert np.allclose(dev["red_corr"], want, rtol=0, atol=1e-9)
 
 
 @pytest.mark.parametrize("name", list(CASES))

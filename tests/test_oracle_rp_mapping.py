@@ -147,3 +147,24 @@ def test_rp_mapping_initial_laws(oracle, key):
                            qe, pe, dt=1.0, nsteps=0)
     pop = out["pop"][:, 0]
     assert np.abs(pop.mean(0) - 1 / n).max() < 5 * pop.std(0).max() / np.sqrt(ntraj)
+
+
+def test_mapping_initial_estimators(oracle, key):
+    """The initial estimators of the population correlation: NRPMD's is its population (Estimators.jl:121), so for
+    PureState(1) the correlation at t0 is [[1, 0], [0, 0]] exactly (test/Dynamics/nrpmd.jl "Population
+    correlation", which allows 0.1); eCMM's mapping kernel (X^2 + P^2)/2 - gamma summed over the states (and beads)
+    is 1 for every sample (cmm.jl:106-110, rpecmm.jl:95-103: the sphere radius)."""
+    n, P, ntraj = 2, 4, 500
+    m = oracle.make_model(oracle.MODEL_TULLY1, 2, 1, [0.01, 1.6, 0.005, 1.0])
+    r0, v0 = np.full((ntraj, P), -8.0), np.zeros((ntraj, P))
+    q, p = oracle.rpmap_sample(oracle.RPMAP_NRPMD, n, P, 0.5, ntraj, key, state=np.zeros(ntraj, np.int32))
+    c = oracle.rpmap_run(oracle.RPMAP_NRPMD, m, n, P, 2000.0, 1e-3, 0.5, r0, v0, q, p, dt=1.0, nsteps=0)
+    corr = np.einsum("ta,tb->ab", c["init"], c["pop"][:, 0]) / ntraj
+    assert np.allclose(corr, [[1, 0], [0, 0]], atol=1e-13)
+    for g in (0.0, 0.3):
+        qe, pe = oracle.rpmap_sample(oracle.RPMAP_ECMM, n, P, g, ntraj, key)
+        ce = oracle.rpmap_run(oracle.RPMAP_ECMM, m, n, P, 2000.0, 1e-3, g, r0, v0, qe, pe, dt=1.0, nsteps=0)
+        assert np.allclose(ce["init"].sum(1), 1.0, rtol=1e-13)
+        X, Pm = oracle.ecmm_sample(n, ntraj, g, key)
+        cm = oracle.mapping_run(m, n, 2000.0, r0[:, 0], v0[:, 0], X, Pm, dt=1.0, nsteps=0, method=oracle.MAP_ECMM, gamma=g)
+        assert np.allclose(cm["init"].sum(1), 1.0, rtol=1e-13)
