@@ -50,8 +50,8 @@ const MODEL_TULLY1, MODEL_TULLY2, MODEL_TULLY3, MODEL_SPIN_BOSON, MODEL_DW_GAUSS
       MODEL_NEWNS_ANDERSON, MODEL_DIATOM_SURFACE, MODEL_MLP_SURROGATE, MODEL_MORSE_N, MODEL_ANDERSON_HOLSTEIN_1 = Int32.(0:10)
 const FIELD_R, FIELD_V, FIELD_SIGMA_RE, FIELD_SIGMA_IM, FIELD_STATE, FIELD_STEPS, FIELD_NHOPS,
       FIELD_SAVE_POP_AD, FIELD_SAVE_POP_DI, FIELD_SAVE_STATE, FIELD_SAVE_CENTROID = Int32.(0:10)
-const FIELD_SAVE_R, FIELD_SAVE_EKIN, FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V, FIELD_MAP_X, FIELD_MAP_P =
-      Int32(12), Int32(13), Int32(14), Int32(15), Int32(16), Int32(17), Int32(18)
+const FIELD_SAVE_R, FIELD_SAVE_EKIN, FIELD_SAVE_EPOT, FIELD_FINAL_POP_DI, FIELD_SAVE_V, FIELD_MAP_X, FIELD_MAP_P,
+      FIELD_INIT_POP_DI = Int32(12), Int32(13), Int32(14), Int32(15), Int32(16), Int32(17), Int32(18), Int32(19)
 const OUT_ADIABATIC_POPULATION, OUT_DIABATIC_POPULATION, OUT_SCATTERING_1D, OUT_POPULATION_CORRELATION,
       OUT_FINAL_ADIABATIC_POPULATION, OUT_CENTROID_POSITION, OUT_KINETIC_ENERGY, OUT_TOTAL_ENERGY,
       OUT_SCATTERING_1D_DIABATIC, OUT_POSITION, OUT_VELOCITY = Int32.(0:10)
@@ -344,6 +344,8 @@ struct DeviceResults
     final_di::Matrix{Float64}                       # (n, ntraj) diabatic populations of the final state, or empty
     save_r::Array{Float64,3}                        # (ndofs * natoms, nsave, ntraj) for CME / BCME / classical, or empty
     save_v::Array{Float64,3}                        # (ndofs * natoms, nsave, ntraj) for classical dynamics, or empty
+    init_pop::Matrix{Float64}                       # (n, ntraj) mapping methods' initial estimator, or empty
+    corr_norm::Float64                              # normalisation of the population correlation (n for eCMM)
 end
 
 statetype_is_adiabatic(f) = hasproperty(f, :statetype) && f.statetype isa NQCDistributions.Adiabatic
@@ -367,7 +369,8 @@ function per_trajectory(f, d::DeviceResults, i, sim, t0, dt)
     end
     if name === :PopulationCorrelationFunction            # TimeCorrelationFunctions.jl:26-40,61-88
         P = statetype_is_adiabatic(f) ? d.pop_ad : d.pop_di
-        return [P[:, 1, i] * P[:, s, i]' for s in 1:d.nsave]
+        K0 = isempty(d.init_pop) || statetype_is_adiabatic(f) ? P[:, 1, i] : d.init_pop[:, i]
+        return [d.corr_norm .* (K0 * P[:, s, i]') for s in 1:d.nsave]
     end
     if name === :OutputStateResolvedScattering1D            # DynamicsOutputs.jl:313-338, at the final state
         if f.type === :adiabatic
@@ -494,6 +497,7 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
             nm = output_name(f)
             cls && return nm in (:OutputKineticEnergy, :OutputTotalEnergy, :OutputPosition, :OutputVelocity)
             spinmap && return nm in (:OutputDiabaticPopulation, :OutputKineticEnergy, :OutputTotalEnergy) ||
+                              (nm === :PopulationCorrelationFunction && !statetype_is_adiabatic(f)) ||
                               (nm === :OutputCentroidPosition && sim isa NQCDynamics.RingPolymerSimulation)
             cme && return nm in (:OutputKineticEnergy, :OutputPosition) ||
                           (nm === :OutputTotalEnergy && method_code(sim) == METHOD_CME)
@@ -563,7 +567,9 @@ function SciMLBase.__solve(prob::SciMLBase.EnsembleProblem, alg, ensemblealg::En
             (cme || cls) && any(f -> output_name(f) === :OutputPosition, outputs) ?
                 download(h[], FIELD_SAVE_R, Float64, prod(size(sim)), ndev, trajectories)[:, 1:nsave, :] : zeros(0, 0, 0),
             cls && any(f -> output_name(f) === :OutputVelocity, outputs) ?
-                download(h[], FIELD_SAVE_V, Float64, prod(size(sim)), ndev, trajectories)[:, 1:nsave, :] : zeros(0, 0, 0))
+                download(h[], FIELD_SAVE_V, Float64, prod(size(sim)), ndev, trajectories)[:, 1:nsave, :] : zeros(0, 0, 0),
+            spinmap ? download(h[], FIELD_INIT_POP_DI, Float64, n, trajectories) : zeros(0, 0),
+            d.method == METHOD_ECMM ? Float64(n) : 1.0)
         batch = map(1:trajectories) do i
             out = Ensembles.Dictionary{Symbol,Any}()
             savetime && insert!(out, :Time, times)
