@@ -82,7 +82,7 @@ __device__ __forceinline__ void eval_cache(Lane2<D, R>& L, bool track, double& E
                                            double G11[D], double G01[D]) {
     static_assert(D == 1, "one-dimensional models");
     R v11, v22, v12, d11, d22, d12;
-    model1d_eval<R>(L.m, L.r[0], v11, v22, v12, d11, d22, d12);
+    model1d_eval_lane<R>(L.m, L.r[0], v11, v22, v12, d11, d22, d12);
     eigen2((double)v11, (double)v12, (double)v22, E0, E1, L.U, track);
     rotate_derivative(L.U, (double)d11, (double)d22, (double)d12, G00[0], G11[0], G01[0]);
 }
@@ -123,7 +123,7 @@ __device__ __forceinline__ void save_slot(const Fssh2Args* Ap, const Lane2<D, R>
     if (A.save_state) A.save_state[(int64_t)slot * n + L.tid] = L.st;
     if (A.save_ekin) {  // classical_kinetic_energy and classical_potential_energy at the slot
         R v11, v22, v12, d11, d22, d12;
-        model1d_eval<R>(L.m, L.r[0], v11, v22, v12, d11, d22, d12);
+        model1d_eval_lane<R>(L.m, L.r[0], v11, v22, v12, d11, d22, d12);
         double E0, E1, Ut[4] = {1.0, 0.0, 0.0, 1.0};
         eigen2((double)v11, (double)v12, (double)v22, E0, E1, Ut, false);
         double ek = 0.0;

@@ -281,6 +281,38 @@ struct ModelLane {
     R p[5];
 };
 
+// Tully's models 1-3 without a branch on the kind: the two exponentials they use are shared,
+// e1 = exp(-B|x|) (model 1), exp(-B x^2) (model 2), exp(-C|x|) (model 3, which is exp(C x) for x < 0 and
+// exp(-C x) above), e2 = exp(-D x^2) (models 1 and 2), and each output is selected per lane.  Every value
+// is formed with the same operations in the same order as model1d_kind's, so the results are bitwise
+// those of the per-kind bodies; a wavefront whose lanes mix the three models then evaluates two
+// exponentials instead of the five of its three branches.  Other kinds go through model1d_eval.
+template <typename R>
+__device__ __forceinline__ void model1d_eval_lane(const ModelLane<R>& m, R x, R& v11, R& v22, R& v12, R& d11, R& d22,
+                                                  R& d12) {
+    if (m.kind > MODEL_TULLY3) {
+        model1d_eval<R>(m, x, v11, v22, v12, d11, d22, d12);
+        return;
+    }
+    const bool t1 = m.kind == MODEL_TULLY1, t2 = m.kind == MODEL_TULLY2;
+    const R A = m.p[0], B = m.p[1], C = m.p[2], D = m.p[3], E0 = m.p[4];
+    // model 3 keeps its coupling amplitude in p[1] and its rate in p[2]
+    const R k1 = t1 || t2 ? B : C;
+    const R a1 = t2 ? -k1 * x * x : -k1 * fabs_(x);
+    const R e1 = fexp(a1);
+    const R e2 = fexp(-D * x * x);
+    const R sgn = (x > (R)0) ? (R)1 : ((x < (R)0) ? (R)-1 : (R)0);
+    const R g = A * e1;
+    const R v11_1 = A * ((R)1 - e1) * sgn, d11_1 = A * B * e1;
+    const R v12_12 = C * e2;
+    v11 = t1 ? v11_1 : (t2 ? (R)0 : A);
+    d11 = t1 ? d11_1 : (R)0;
+    v22 = t1 ? -v11_1 : (t2 ? -g + E0 : -A);
+    d22 = t1 ? -d11_1 : (t2 ? (R)2 * B * x * g : (R)0);
+    v12 = (t1 || t2) ? v12_12 : (x < (R)0 ? B * e1 : B * ((R)2 - e1));
+    d12 = (t1 || t2) ? (R)-2 * D * x * v12_12 : B * C * e1;
+}
+
 template <typename R>
 __device__ __forceinline__ ModelLane<R> model_lane(const ModelRegs& m) {
     ModelLane<R> L;
