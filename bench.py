@@ -107,7 +107,8 @@ def cpu_baseline(L, gpu_states, nthreads, samples):
            "state_agreement_sample": agree}
     checks = {}
     for name, fn in (("spin_boson", _check_spin_boson), ("ring_polymer", _check_ring_polymer), ("iesh", _check_iesh),
-                     ("learned", _check_learned), ("mapping", _check_mapping)):
+                     ("learned", _check_learned), ("mapping", _check_mapping), ("nstate", _check_nstate),
+                     ("cme", _check_cme)):
         if name in samples:
             t0 = time.perf_counter()
             checks[name] = fn(O, samples[name], nthreads)
@@ -392,6 +393,113 @@ def _check_mapping(O, S, nthreads):
     return out
 
 
+# three-state Morse chain (the n-state Morse form, ThreeStateMorse-like values): [D, beta, r, c] per state,
+# then [A, a, r] per pair (1,2), (1,3), (2,3)
+MORSE3_FSSH = [0.02, 0.4, 4.0, 0.02, 0.02, 0.65, 4.5, 0.0, 0.003, 0.65, 6.0, 0.02,
+               0.005, 32.0, 3.40, 0.0, 32.0, 4.8, 0.005, 32.0, 4.97]
+
+
+def nstate_extra(L):
+    """FSSH with three states (BABwithTsit5 on the three-state Morse chain MORSE3_FSSH, mass 20000, x0 ~ N(3, 0.15),
+    kinetic energy U(0.02, 0.04), start on adiabatic state 1, 2000 steps of dt = 1, 1e5 trajectories) in FP32 nuclei
+    and FP64, and one persistent launch mixing 5e4 two-state Tully-1 trajectories (x0 ~ N(-5, 0.5), p0 ~ N(60, 2)) with
+    5e4 three-state Morse ones (one mass, 20000, for both).  Rates are trajectory-steps/s of kernel time."""
+    ntraj, nsteps, mass = 100_000, 2000, 20000.0
+    rng = np.random.default_rng(21)
+    r0 = rng.normal(3.0, 0.15, size=(ntraj, 1))
+    v0 = np.sqrt(2 * rng.uniform(0.02, 0.04, size=(ntraj, 1)) / mass)
+    sig0 = np.zeros((ntraj, 3, 3), complex); sig0[:, 0, 0] = 1.0
+    st0 = np.zeros(ntraj, np.int32)
+    res, sample = {}, {}
+    for name, prec in (("fssh_n3_fp32", L.PREC_FP32_NUCLEI), ("fssh_n3_fp64", L.PREC_FP64)):
+        E = L.Ensemble(0)
+        E.set_problem(L.METHOD_FSSH, 3, 1, 1, [mass], 1.0, precision=prec)
+        E.set_model(0, L.MODEL_MORSE_N, [3] + MORSE3_FSSH)
+        E.load_initial(r0, v0, sig0, state=st0)
+        st = E.run(0.0, float(nsteps), KEY, save_every=100)
+        pop = E.reduce(L.OUT_ADIABATIC_POPULATION, L.RED_MEAN, 3 * E.nsave).reshape(E.nsave, 3)
+        if prec == L.PREC_FP64:
+            m = 64
+            sample = {"mass": mass, "nsteps": nsteps, "r0": r0[:m].copy(), "v0": v0[:m].copy(),
+                      "state": E.download(L.FIELD_STATE)[:m].copy(), "r": E.download(L.FIELD_R)[:m].copy()}
+        E.close()
+        rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+        res[name] = {"ntraj": ntraj, "nsteps": nsteps, "kernel_ms": st["kernel_ms"], "traj_steps_per_s": rate,
+                     "grid": st["grid"], "block": st["block"], "final_adiabatic_population": [float(x) for x in pop[-1]]}
+        _progress(f"n-state {name}: {rate:.3e} trajectory-steps/s")
+    # one launch mixing two- and three-state trajectories (model slot 0: Tully 1, slot 1: the Morse chain)
+    half = ntraj // 2
+    mid = np.concatenate([np.zeros(half, np.int32), np.ones(ntraj - half, np.int32)])
+    rm = np.where(mid[:, None] == 0, rng.normal(-5.0, 0.5, size=(ntraj, 1)), r0)
+    vm = np.where(mid[:, None] == 0, rng.normal(60.0, 2.0, size=(ntraj, 1)) / mass, v0)
+    E = L.Ensemble(0)
+    E.set_problem(L.METHOD_FSSH, 3, 1, 1, [mass], 1.0, precision=L.PREC_FP32_NUCLEI)
+    E.set_model(0, L.MODEL_TULLY1, [0.01, 1.6, 0.005, 1.0])
+    E.set_model(1, L.MODEL_MORSE_N, [3] + MORSE3_FSSH)
+    E.load_initial(rm, vm, sig0, state=st0, model_id=mid)
+    st = E.run(0.0, float(nsteps), KEY, term=(0, 10.0, L.TERM_ABS_GT))
+    E.close()
+    rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+    res["mixed_n2_n3_fp32"] = {"ntraj": ntraj, "nsteps_max": nsteps, "traj_steps": st["traj_steps"],
+                               "kernel_ms": st["kernel_ms"], "traj_steps_per_s": rate}
+    _progress(f"n-state mixed n2/n3: {rate:.3e} trajectory-steps/s")
+    return res, sample
+
+
+def _check_nstate(O, S, nthreads):
+    """Three-state FSSH in FP64 on its first trajectories against the oracle's general-n path: final active
+    states and positions."""
+    k = len(S["r0"])
+    m = O.make_model(O.MODEL_MORSE_N, 3, 1, MORSE3_FSSH)
+    sig0 = np.zeros((k, 3, 3), complex); sig0[:, 0, 0] = 1.0
+    c = O.run_ensemble(0, m, S["mass"], S["r0"], S["v0"], sig0, state0=np.zeros(k, np.int32), dt=1.0,
+                       nsteps=S["nsteps"], key=KEY, nthreads=nthreads)
+    return {"trajectories": k, "state_agreement": float(np.mean(c["state"] == S["state"])),
+            "max_abs_err_r": float(np.abs(c["r"].ravel() - S["r"].ravel()).max())}
+
+
+def cme_extra(L):
+    """Classical master equation surface hopping on the one-mode Anderson-Holstein model (the reference's phonon
+    relaxation test, test/Dynamics/cme.jl:57-88: hw = Gamma = 0.003, kT = 0.03, g = 0.005), 1e6 trajectories for
+    1000 steps of dt = 5 (CME), and BCME (Gamma = 0.1, kT = 0.01, g = 0.015, bandwidth 100) on 1e5 trajectories for
+    1000 steps of dt = 2; FP64.  Rates are trajectory-steps/s of kernel time."""
+    res, sample = {}, {}
+    cases = (("cme", L.METHOD_CME, 0.003, 0.003, 0.03, 0.005, 1_000_000, 5.0, 0.0),
+             ("bcme", L.METHOD_BCME, 0.003, 0.1, 0.01, 0.015, 100_000, 2.0, 100.0))
+    for name, method, hw, Gam, kT, g, ntraj, dt, W in cases:
+        Ed = 0.0 if method == L.METHOD_CME else g ** 2 / hw
+        prm = [hw, Ed, np.sqrt(2) * g, np.sqrt(Gam / (2 * np.pi))]
+        samp = [0.0, np.sqrt(2 * kT / hw), 0.0, np.sqrt(2 * kT * hw)]
+        E = L.Ensemble(0)
+        E.set_problem(method, 2, 1, 1, [1 / hw], dt, algorithm=L.ALG_TSIT5, rescaling=L.RESCALE_OFF, temperature=kT,
+                      bandwidth=W)
+        E.set_model(0, L.MODEL_ANDERSON_HOLSTEIN_1, prm)
+        E.sample_initial(L.SAMPLER_GAUSSIAN, ntraj, samp, KEY)
+        st = E.run(0.0, 1000 * dt, KEY, save_every=100)
+        ek = E.reduce(L.OUT_KINETIC_ENERGY, L.RED_MEAN, E.nsave)
+        if method == L.METHOD_CME:
+            m = 256
+            sample = {"prm": prm, "samp": samp, "hw": hw, "kT": kT, "dt": dt, "state": E.download(L.FIELD_STATE)[:m].copy(),
+                      "r": E.download(L.FIELD_R)[:m].copy(), "n": m}
+        E.close()
+        rate = st["traj_steps"] / (st["kernel_ms"] / 1e3)
+        res[name] = {"ntraj": ntraj, "nsteps": 1000, "dt": dt, "kernel_ms": st["kernel_ms"], "traj_steps_per_s": rate,
+                     "grid": st["grid"], "block": st["block"], "final_kinetic_energy": float(ek[-1])}
+        _progress(f"{name}: {rate:.3e} trajectory-steps/s")
+    return res, sample
+
+
+def _check_cme(O, S, nthreads):
+    """CME on its first trajectories against the oracle from the same draws: final states and positions."""
+    n = S["n"]
+    r0, v0 = O.sample_gaussian(n, 1, *S["samp"], KEY)
+    m = O.make_model(O.MODEL_ANDERSON_HOLSTEIN_1, 2, 1, S["prm"])
+    c = O.cme_run(0, m, 1 / S["hw"], S["kT"], r0[:, 0], v0[:, 0], np.zeros(n, np.int32), dt=S["dt"], nsteps=1000,
+                  key=KEY, nthreads=nthreads)
+    return {"trajectories": n, "state_agreement": float(np.mean(c["state"] == S["state"])),
+            "max_abs_err_r": float(np.abs(c["x"] - S["r"].ravel()).max())}
+
+
 TRAIN_STEPS = 40000
 TRAIN_BATCH = 2048
 
@@ -496,7 +604,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-port comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the IESH and surrogate measurements")
-    ap.add_argument("--only", choices=["iesh", "spin_boson", "ring_polymer", "learned", "mapping"],
+    ap.add_argument("--only", choices=["iesh", "spin_boson", "ring_polymer", "learned", "mapping", "nstate", "cme"],
                     help="run one of the other workloads alone and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -517,7 +625,8 @@ def main():
     from nqcd_mi355x import _lib as L
     if args.only:
         fn = {"iesh": iesh_extra, "spin_boson": spin_boson_extra, "ring_polymer": ring_polymer_extra,
-              "learned": surrogate_extra, "mapping": mapping_extra}[args.only]
+              "learned": surrogate_extra, "mapping": mapping_extra, "nstate": nstate_extra,
+              "cme": cme_extra}[args.only]
         print(json.dumps({args.only: fn(L)[0]}))
         return
 
@@ -611,6 +720,8 @@ def main():
         out["ring_polymer"], samples["ring_polymer"] = ring_polymer_extra(L)
         out["learned"], samples["learned"] = surrogate_extra(L)
         out["mapping"], samples["mapping"] = mapping_extra(L)
+        out["nstate"], samples["nstate"] = nstate_extra(L)
+        out["cme"], samples["cme"] = cme_extra(L)
     if not args.no_cpu:
         nthreads = min(CPU_THREADS, os.cpu_count() or 1)
         _progress(f"cpu baseline on {nthreads} threads and on one core")
