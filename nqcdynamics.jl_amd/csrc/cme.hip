@@ -12,8 +12,9 @@
 // A the Lorentzian of half width Gamma / 2 at h (cme.jl:167-182).  The reference integrates them with
 // QuadGK; here by parts: n = [F f] - int F f' de, F(e) = atan((e - h)/(Gamma/2))/pi, and
 // n2 = [G f] - int G f' de, G(e) = (Gamma / 4pi) ln((e - h)^2 + (Gamma/2)^2), where f' = -beta f (1 - f)
-// vanishes beyond 40 kT of mu, so the remaining integrals are 8-point Gauss-Legendre panels over that
-// window (panel width min(2 kT, Gamma/2), with panel edges at h and h +- 4 Gamma/2).
+// vanishes beyond 40 kT of mu.  When the band holds that window the remaining integrals have a closed form
+// through the complex digamma function (bcme_broadening); a narrower band takes 8-point Gauss-Legendre panels
+// over the window (panel width min(2 kT, Gamma/2), with panel edges at h and h +- 4 Gamma/2).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -67,6 +68,46 @@ __device__ __forceinline__ double fermi_deriv(double e, double mu, double beta) 
     return beta * q / ((1.0 + q) * (1.0 + q));
 }
 
+// Digamma of z = x + i y for x >= 1/2: psi(z) = psi(z + m) - sum_{k<m} 1/(z + k), with m the fewest shifts
+// that give |z + m| >= 10, then Stirling's series through the B14 term (truncation below 5e-17 at |w| >= 10).
+// The shift sum is accumulated as one fraction, so the shifts cost no division.
+__device__ __forceinline__ void digamma_c(double x, double y, double& re, double& im) {
+    constexpr double R = 10.0;
+    const double y2 = y * y;
+    const int m = (y2 >= R * R) ? 0 : (int)fmax(0.0, ceil(sqrt(R * R - y2) - x));
+    double nr = 0.0, ni = 0.0, dr = 1.0, di = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double wr = x + k;
+        const double tr = nr * wr - ni * y + dr, ti = nr * y + ni * wr + di;
+        const double ur = dr * wr - di * y, ui = dr * y + di * wr;
+        nr = tr; ni = ti; dr = ur; di = ui;
+    }
+    const double dd = 1.0 / (dr * dr + di * di);
+    const double sr = (nr * dr + ni * di) * dd, si = (ni * dr - nr * di) * dd;
+    const double wr = x + m;
+    const double ww = wr * wr + y2;
+    const double iw = 1.0 / ww;
+    const double tr = wr * iw, ti = -y * iw;                 // t = 1 / w
+    const double qr = tr * tr - ti * ti, qi = 2.0 * tr * ti;  // t^2
+    // P = 1/12 - t2 (1/120 - t2 (1/252 - t2 (1/240 - t2 (1/132 - t2 (691/32760 - t2 / 12)))))
+    double pr = 691.0 / 32760.0 - qr / 12.0, pi = -qi / 12.0;
+    const double cs[5] = {1.0 / 132.0, 1.0 / 240.0, 1.0 / 252.0, 1.0 / 120.0, 1.0 / 12.0};
+    for (int j = 0; j < 5; ++j) {
+        const double ar = qr * pr - qi * pi, ai = qr * pi + qi * pr;
+        pr = cs[j] - ar; pi = -ai;
+    }
+    const double er = qr * pr - qi * pi, ei = qr * pi + qi * pr;  // t^2 P
+    re = 0.5 * log(ww) - 0.5 * tr - er - sr;
+    im = atan2(y, wr) - 0.5 * ti - ei - si;
+}
+
+// n(h) = int_a^b A f and n2(h) = int_a^b (e - h) A f, a, b = mu -+ W/2, A the Lorentzian of half-width
+// Gamma / 2 at h.  Integrated by parts: the boundary terms plus int (-f') G with G = atan((e - h) / (Gamma / 2))
+// for n and (Gamma / 4 pi) log((e - h)^2 + Gamma^2 / 4) for n2.  -f' is the logistic density of scale kT about
+// mu; when the band holds +-40 kT of it (the rest weighs below 1e-17), the remaining integral is taken over the
+// whole line in closed form:  int (-f') log(Gamma / 2 + i (e - h)) de = log(2 pi kT) + psi(1/2 + (Gamma / 2 +
+// i (mu - h)) / (2 pi kT)), whose real and imaginary parts give the two integrals.  A narrower band takes
+// eight-point Gauss-Legendre panels.
 __device__ void bcme_broadening(double h, double mu, double beta, double Gam, double W, double& n, double& n2) {
     const double hg = 0.5 * Gam;
     const double c4 = Gam / (4.0 * M_PI);
@@ -74,6 +115,13 @@ __device__ void bcme_broadening(double h, double mu, double beta, double Gam, do
     const double fa = fermi_dev(a, mu, beta), fb = fermi_dev(b, mu, beta);
     n = atan((b - h) / hg) / M_PI * fb - atan((a - h) / hg) / M_PI * fa;
     n2 = c4 * log((b - h) * (b - h) + hg * hg) * fb - c4 * log((a - h) * (a - h) + hg * hg) * fa;
+    if (0.5 * W * beta >= 40.0) {
+        double pr, pi;
+        digamma_c(0.5 + beta * hg / (2.0 * M_PI), beta * (mu - h) / (2.0 * M_PI), pr, pi);
+        n += pi / M_PI;
+        n2 += 2.0 * c4 * (log(2.0 * M_PI / beta) + pr);
+        return;
+    }
     const double lo = fmax(a, mu - 40.0 / beta), hi = fmin(b, mu + 40.0 / beta);
     if (!(hi > lo)) return;
     // 8-point Gauss-Legendre

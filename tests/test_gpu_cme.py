@@ -1,5 +1,5 @@
 """Classical master equation surface hopping on the device (CME / BCME, cme.jl:26-182) against the FP64
-oracle: the broadening integrals of the BCME force (device: integration by parts plus Gauss-Legendre panels;
+oracle: the broadening integrals of the BCME force (device: integration by parts, then a closed form or panels;
 oracle: adaptive Gauss-Kronrod to 1e-13), CME trajectories bit-for-bit in their hop sequences with positions
 and energies to 1e-9, BCME trajectories to the accuracy of the device quadrature, and the reference's own
 phonon-relaxation test (test/Dynamics/cme.jl:57-88) run through run_dynamics against its golden curve."""
@@ -18,10 +18,15 @@ def _ah(hw, Ed, g, Gam):
     return [hw, Ed, np.sqrt(2) * g, np.sqrt(Gam / (2 * np.pi))]
 
 
-@pytest.mark.parametrize("mu,kT,Gam,W", [(0.0, 0.01, 0.1, 100.0), (0.05, 0.03, 0.003, 1.0), (0.0, 0.01, 0.1, 0.5)])
+@pytest.mark.parametrize("mu,kT,Gam,W", [(0.0, 0.01, 0.1, 100.0), (0.05, 0.03, 0.003, 10.0), (0.0, 0.001, 0.5, 5.0),
+                                          (0.1, 0.2, 0.02, 100.0), (0.0, 1e-4, 1e-3, 1.0),
+                                          (0.05, 0.03, 0.003, 1.0), (0.0, 0.01, 0.1, 0.5)])
 def test_broadening_kernel_matches_oracle(oracle, mu, kT, Gam, W):
-    """Eight-point panels no wider than Gamma / 2 around h: the nearest poles of the integrands (h +- i Gamma / 2)
-    bound the panel error near 1e-10 of the integral; the test allows 1e-9."""
+    """A band holding +-40 kT of mu takes the closed form (complex digamma: Stirling's series after shifting the
+    argument to |z| >= 10, z = 1/2 + (Gamma/2 + i (mu - h)) / (2 pi kT), from near 1/2 to |z| in the hundreds
+    over these cases); the last two cases, narrower bands, take eight-point panels no wider than Gamma / 2
+    around h, whose error the nearest poles of the integrands (h +- i Gamma / 2) bound near 1e-10 of the
+    integral.  The test allows 1e-9."""
     from nqcd_mi355x import _lib as L
     hs = np.concatenate([np.linspace(-1.0, 1.0, 81), [mu, mu + 1e-3, mu - 0.3 * kT, 0.5 * W, -0.5 * W]])
     print(f"broadening: mu={mu} kT={kT} Gamma={Gam} W={W}", flush=True)
