@@ -80,9 +80,13 @@ __device__ __forceinline__ void ns_model(const LaneN<NM, R>& L, R x, double vd[N
 
 // Cyclic Jacobi on the leading n x n block of the symmetric matrix (vd, vo): eigenvalues ascending in
 // E, eigenvectors in U (columns), signs by the fresh rule or tracked against Uprev (= U on entry).
+// warm: the sweeps start from the basis U holds on entry (an orthogonal matrix that is the identity outside
+// the leading block, such as the previous step's eigenvectors): A = U^T V U and Q = U, so a basis close to
+// the new one needs one or two sweeps to meet the convergence test instead of four or five.  From the
+// identity the warm start is the cold one bit for bit (products with exact ones and zeros).
 template <int NM>
 __device__ __forceinline__ void ns_eigen(int n, const double vd[NM], const double vo[], double E[NM], double U[NM * NM],
-                                         bool track) {
+                                         bool track, bool warm = false) {
     constexpr int NP = NM * (NM - 1) / 2;
     double A[NM * NM], Q[NM * NM];
 #pragma unroll
@@ -90,8 +94,30 @@ __device__ __forceinline__ void ns_eigen(int n, const double vd[NM], const doubl
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             A[i * NM + j] = (i == j) ? vd[i] : (i < j ? vo[pidx<NM>(i, j)] : vo[pidx<NM>(j, i)]);
-            Q[i * NM + j] = (i == j) ? 1.0 : 0.0;
+            Q[i * NM + j] = warm ? U[i * NM + j] : ((i == j) ? 1.0 : 0.0);
         }
+    if (warm) {
+        double T[NM * NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i)  // T = V U
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) t += A[i * NM + k] * U[k * NM + j];
+                T[i * NM + j] = t;
+            }
+#pragma unroll
+        for (int i = 0; i < NM; ++i)  // A = U^T T, the upper triangle mirrored
+#pragma unroll
+            for (int j = i; j < NM; ++j) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) t += U[k * NM + i] * T[k * NM + j];
+                A[i * NM + j] = t;
+                A[j * NM + i] = t;
+            }
+    }
     (void)NP;
     for (int sweep = 0; sweep < 100; ++sweep) {
         double off = 0.0, tot = 0.0;

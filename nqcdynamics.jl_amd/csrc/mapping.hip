@@ -105,16 +105,17 @@ __device__ __forceinline__ double map_potential(const LaneN<NM, double>& L, doub
 // (a^T Gamma a + b^T Gamma b) / 2 - a^T Xi b of the rotated variables with Gamma, Xi from
 // W = Z^T (dV - dshift I) Z.  TRACELESS (RingPolymerMInt, ringpolymer_mint.jl:65-101): shift = tr V / NM and
 // dshift = tr dV / NM, the traceless eigenvalues and derivative; otherwise both zero.  Returns f and
-// tr = tr dV.
+// tr = tr dV.  Z holds the eigenvectors on return; above two states the Jacobi sweeps start from the Z it holds
+// on entry (the previous step's, or the identity), which the step's results do not depend on beyond rounding.
 template <int NM, bool TRACELESS>
 __device__ __forceinline__ void map_electronic_step(const LaneN<NM, double>& L, double x, double X[NM], double P[NM],
-                                                    double dt, double& f, double& tr) {
+                                                    double dt, double Z[NM * NM], double& f, double& tr) {
     constexpr int NP = NM * (NM - 1) / 2;
     double vd[NM], vo[NP > 0 ? NP : 1], dd[NM], dof[NP > 0 ? NP : 1];
     ns_model<NM, double>(L, x, vd, vo, dd, dof);
-    double w[NM], Z[NM * NM];
+    double w[NM];
     if constexpr (NM == 2) eigen2(vd[0], vo[0], vd[1], w[0], w[1], Z, false);
-    else ns_eigen<NM>(NM, vd, vo, w, Z, false);
+    else ns_eigen<NM>(NM, vd, vo, w, Z, false, true);
     double vbar = 0.0, dbar = 0.0;
     tr = 0.0;
 #pragma unroll
@@ -206,10 +207,13 @@ __global__ __launch_bounds__(256) void mapping_kernel(const MapArgs A) {
         };
         save(0);
         int slot = 1, until = A.save_every;
+        double Z[NM * NM];
+#pragma unroll
+        for (int i = 0; i < NM * NM; ++i) Z[i] = (i % (NM + 1) == 0) ? 1.0 : 0.0;
         for (int64_t step = 0; step < A.nsteps; ++step) {
             const double rt = x + hdt * v;
             double f, tr;
-            map_electronic_step<NM, false>(L, rt, X, P, dt, f, tr);
+            map_electronic_step<NM, false>(L, rt, X, P, dt, Z, f, tr);
             v = v - f * im + tr * gf * dt * 0.5 * im;
             x = rt + hdt * v;
             if (A.save_every > 0 && step + 1 == until) {
@@ -407,11 +411,14 @@ __global__ __launch_bounds__(64) void rpmap_kernel(const RpMapArgs A) {
         };
         save(0);
         int slot = 1, until = A.save_every;
+        double Z[NM * NM];
+#pragma unroll
+        for (int i = 0; i < NM * NM; ++i) Z[i] = (i % (NM + 1) == 0) ? 1.0 : 0.0;
         for (int64_t step = 0; step < A.nsteps; ++step) {
             double q1 = hs * q + c12 * p, p1 = c21 * q + hs * p;
             const double rb = rpmap_transform(buf, U, l, g, P, b, q1, false);
             double f, tr;
-            map_electronic_step<NM, true>(L, rb, X, Pm, dt, f, tr);
+            map_electronic_step<NM, true>(L, rb, X, Pm, dt, Z, f, tr);
             const double dvb = -(f + tr * (1.0 / NM) * dt) * im;
             p1 += rpmap_transform(buf, U, l, g, P, b, dvb, true);
             q = hs * q1 + c12 * p1;
