@@ -160,6 +160,32 @@ __device__ __forceinline__ void ns_eigen(int n, const double vd[NM], const doubl
                 }
             }
     }
+    if (warm) {
+        // one Newton-Schulz step Q <- Q (3 I - Q^T Q) / 2: a basis carried from step to step would otherwise
+        // lose orthogonality by rounding, step after step
+        double G[NM * NM], Qn[NM * NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+#pragma unroll
+            for (int j = i; j < NM; ++j) {
+                double g = 0.0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) g += Q[k * NM + i] * Q[k * NM + j];
+                G[i * NM + j] = ((i == j) ? 1.5 : 0.0) - 0.5 * g;
+                G[j * NM + i] = G[i * NM + j];
+            }
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) t += Q[i * NM + k] * G[k * NM + j];
+                Qn[i * NM + j] = t;
+            }
+#pragma unroll
+        for (int i = 0; i < NM * NM; ++i) Q[i] = Qn[i];
+    }
 #pragma unroll
     for (int i = 0; i < NM; ++i) E[i] = A[i * NM + i];
     // ascending order over the first n (a stable exchange sort on fixed indices)
