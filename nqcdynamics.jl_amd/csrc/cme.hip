@@ -112,16 +112,20 @@ __device__ void bcme_broadening(double h, double mu, double beta, double Gam, do
     const double hg = 0.5 * Gam;
     const double c4 = Gam / (4.0 * M_PI);
     const double a = mu - 0.5 * W, b = mu + 0.5 * W;
-    const double fa = fermi_dev(a, mu, beta), fb = fermi_dev(b, mu, beta);
-    n = atan((b - h) / hg) / M_PI * fb - atan((a - h) / hg) / M_PI * fa;
-    n2 = c4 * log((b - h) * (b - h) + hg * hg) * fb - c4 * log((a - h) * (a - h) + hg * hg) * fa;
     if (0.5 * W * beta >= 40.0) {
+        // f(a) = 1 / (1 + e^{-beta W/2}) rounds to 1 and f(b) < 4.3e-18, so the boundary terms at b (below
+        // 2.2e-18 for n, and Gamma/4pi ln(...) f(b) for n2) are dropped and those at a taken with f(a) = 1
+        n = -atan((a - h) / hg) / M_PI;
+        n2 = -c4 * log((a - h) * (a - h) + hg * hg);
         double pr, pi;
         digamma_c(0.5 + beta * hg / (2.0 * M_PI), beta * (mu - h) / (2.0 * M_PI), pr, pi);
         n += pi / M_PI;
         n2 += 2.0 * c4 * (log(2.0 * M_PI / beta) + pr);
         return;
     }
+    const double fa = fermi_dev(a, mu, beta), fb = fermi_dev(b, mu, beta);
+    n = atan((b - h) / hg) / M_PI * fb - atan((a - h) / hg) / M_PI * fa;
+    n2 = c4 * log((b - h) * (b - h) + hg * hg) * fb - c4 * log((a - h) * (a - h) + hg * hg) * fa;
     const double lo = fmax(a, mu - 40.0 / beta), hi = fmin(b, mu + 40.0 / beta);
     if (!(hi > lo)) return;
     // 8-point Gauss-Legendre
@@ -149,11 +153,13 @@ __device__ void bcme_broadening(double h, double mu, double beta, double Gam, do
     n2 += c4 * sG;
 }
 
+// BCME: the broadened force (A.method 1); a kernel of its own, so the CME kernel carries none of its code
+template <bool BCME>
 __device__ __forceinline__ double cme_accel(const CmeArgs& A, const ModelRegs& m, double x, int s) {
     double v11, v22, v12, d11, d22, d12;
     cme_model(m, x, v11, v22, v12, d11, d22, d12);
     double f = -((s == 0) ? d11 : d22);
-    if (A.method == 1) {
+    if constexpr (BCME) {
         const double Gam = 2.0 * M_PI * v12 * v12;
         const double beta = 1.0 / A.kT;
         const double h = v22 - v11;
@@ -167,6 +173,7 @@ __device__ __forceinline__ double cme_accel(const CmeArgs& A, const ModelRegs& m
     return f / A.mass;
 }
 
+template <bool BCME>
 __global__ __launch_bounds__(256) void cme_kernel(const CmeArgs A) {
     const ModelRegs m = model_regs(A.model);
     for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < A.ntraj;
@@ -189,19 +196,19 @@ __global__ __launch_bounds__(256) void cme_kernel(const CmeArgs A) {
         while (step < A.nsteps) {
             using namespace ts5;
             const double h = A.dt;
-            const double k1x = v, k1v = cme_accel(A, m, x, s);
+            const double k1x = v, k1v = cme_accel<BCME>(A, m, x, s);
             double tx = x + h * (a21 * k1x), tv = v + h * (a21 * k1v);
-            const double k2x = tv, k2v = cme_accel(A, m, tx, s);
+            const double k2x = tv, k2v = cme_accel<BCME>(A, m, tx, s);
             tx = x + h * (a31 * k1x + a32 * k2x); tv = v + h * (a31 * k1v + a32 * k2v);
-            const double k3x = tv, k3v = cme_accel(A, m, tx, s);
+            const double k3x = tv, k3v = cme_accel<BCME>(A, m, tx, s);
             tx = x + h * (a41 * k1x + a42 * k2x + a43 * k3x); tv = v + h * (a41 * k1v + a42 * k2v + a43 * k3v);
-            const double k4x = tv, k4v = cme_accel(A, m, tx, s);
+            const double k4x = tv, k4v = cme_accel<BCME>(A, m, tx, s);
             tx = x + h * (a51 * k1x + a52 * k2x + a53 * k3x + a54 * k4x);
             tv = v + h * (a51 * k1v + a52 * k2v + a53 * k3v + a54 * k4v);
-            const double k5x = tv, k5v = cme_accel(A, m, tx, s);
+            const double k5x = tv, k5v = cme_accel<BCME>(A, m, tx, s);
             tx = x + h * (a61 * k1x + a62 * k2x + a63 * k3x + a64 * k4x + a65 * k5x);
             tv = v + h * (a61 * k1v + a62 * k2v + a63 * k3v + a64 * k4v + a65 * k5v);
-            const double k6x = tv, k6v = cme_accel(A, m, tx, s);
+            const double k6x = tv, k6v = cme_accel<BCME>(A, m, tx, s);
             x = x + h * (a71 * k1x + a72 * k2x + a73 * k3x + a74 * k4x + a75 * k5x + a76 * k6x);
             v = v + h * (a71 * k1v + a72 * k2v + a73 * k3v + a74 * k4v + a75 * k5v + a76 * k6v);
             // the save precedes the hop callback (a save time holds the state before the step's callbacks)
@@ -329,7 +336,8 @@ int32_t cme_run(nqcd_handle_t* h, double t0, int64_t nsteps, const nqcd_term_des
     const int block = 128;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, (int64_t)h->cu_count * 16));
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    hipLaunchKernelGGL(cme_kernel, dim3((unsigned)grid), dim3(block), 0, h->stream, A);
+    if (A.method == 1) hipLaunchKernelGGL(cme_kernel<true>, dim3((unsigned)grid), dim3(block), 0, h->stream, A);
+    else hipLaunchKernelGGL(cme_kernel<false>, dim3((unsigned)grid), dim3(block), 0, h->stream, A);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     h->stats.block = block;
